@@ -1,0 +1,238 @@
+"""galpha2d_amd -- MI355X-native 2D Delaunay triangulation + alpha shapes.
+
+Python mirror of the reference host interface (gpudt/gpudt.h: gpudtComputeDT /
+gpudtReleaseDTOutput, and AlphaShape/Alpha_Shapes_2D.h: Alpha_Shapes_2D::computeAlphaShapes)
+over the C ABI of libgalpha.so (include/gpudt.h).  All compute runs in hand-written HIP
+kernels for gfx950; this module only marshals arrays.  There is no CPU fallback: when the
+shared library or a GPU is missing, every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgalpha.so")
+_lib = None
+
+
+class GalphaError(RuntimeError):
+    pass
+
+
+class _Stats(C.Structure):
+    _fields_ = [("nTris", C.c_int), ("nEdges", C.c_int), ("specSize", C.c_int),
+                ("islandSites", C.c_int), ("pbaIterations", C.c_int),
+                ("reconTris", C.c_int), ("shiftBad", C.c_int), ("deleted", C.c_int),
+                ("missing", C.c_int), ("inserted", C.c_int), ("duplicates", C.c_int),
+                ("insertRounds", C.c_int), ("flipPasses", C.c_int),
+                ("flips", C.c_longlong), ("incircleTests", C.c_longlong),
+                ("incircleExact", C.c_longlong), ("orientExact", C.c_longlong),
+                ("alphaExactCompares", C.c_longlong), ("alphaExactAttach", C.c_longlong),
+                ("alphaClusters", C.c_longlong), ("alphaMaxCluster", C.c_longlong),
+                ("ms_pba", C.c_double), ("ms_recon", C.c_double), ("ms_shift", C.c_double),
+                ("ms_missing", C.c_double), ("ms_boundary", C.c_double), ("ms_flip", C.c_double),
+                ("ms_canon", C.c_double), ("ms_alpha", C.c_double), ("ms_total", C.c_double)]
+
+
+class _Alpha(C.Structure):
+    _fields_ = [("nPoints", C.c_int), ("nEdges", C.c_int), ("nTris", C.c_int),
+                ("specSize", C.c_int), ("spectrum", C.POINTER(C.c_double)),
+                ("tri_ro_rank", C.POINTER(C.c_int)), ("edge_t", C.POINTER(C.c_int)),
+                ("edge_vopp", C.POINTER(C.c_int)), ("edge_ro_rank", C.POINTER(C.c_int)),
+                ("edge_nu_min_rank", C.POINTER(C.c_int)), ("edge_nu_max_rank", C.POINTER(C.c_int)),
+                ("vert_nu_min_rank", C.POINTER(C.c_int)), ("vert_nu_max_rank", C.POINTER(C.c_int)),
+                ("masterSize", C.c_int), ("master_simplex", C.POINTER(C.c_int)),
+                ("master_st", C.POINTER(C.c_int)), ("master_at", C.POINTER(C.c_int)),
+                ("nSpecPointers", C.c_int), ("spec_pointers", C.POINTER(C.c_int))]
+
+
+# every symbol include/gpudt.h declares
+EXPORTS = ["gpudtComputeDT", "gpudtReleaseDTOutput", "gpudtGetLastError", "gpudtComputeAlpha",
+           "gpudtReleaseAlpha", "galphaCreate", "galphaDestroy", "galphaLastError",
+           "galphaComputeHost", "galphaComputeDevice", "galphaDeviceTriangles",
+           "galphaCopyTriangles", "galphaGetStats", "galphaVoronoi", "galphaCopyAlpha",
+           "galphaBatch", "galphaFree", "galphaAutoTexture"]
+
+
+def lib():
+    """Load libgalpha.so (raises GalphaError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GalphaError("libgalpha.so not built: run __graft_entry__.build() or make -C galpha2d_amd/csrc")
+    L = C.CDLL(LIB_PATH)
+    vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+    L.galphaCreate.argtypes = [C.c_int]
+    L.galphaCreate.restype = vp
+    L.galphaDestroy.argtypes = [vp]
+    L.galphaLastError.argtypes = [vp]
+    L.galphaLastError.restype = C.c_char_p
+    L.gpudtGetLastError.restype = C.c_char_p
+    L.galphaComputeHost.argtypes = [vp, dp, C.c_int, C.c_int, ip, ip]
+    L.galphaComputeHost.restype = C.c_int
+    L.galphaComputeDevice.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, vp]
+    L.galphaComputeDevice.restype = C.c_int
+    L.galphaDeviceTriangles.argtypes = [vp, C.POINTER(vp), ip]
+    L.galphaCopyTriangles.argtypes = [vp, ip, C.c_int]
+    L.galphaCopyTriangles.restype = C.c_int
+    L.galphaGetStats.argtypes = [vp, C.POINTER(_Stats)]
+    L.galphaVoronoi.argtypes = [vp, dp, C.c_int, C.c_int, ip, ip, C.c_int]
+    L.galphaVoronoi.restype = C.c_int
+    L.galphaCopyAlpha.argtypes = [vp, C.POINTER(C.POINTER(_Alpha))]
+    L.galphaCopyAlpha.restype = C.c_int
+    L.gpudtReleaseAlpha.argtypes = [C.POINTER(_Alpha)]
+    L.galphaAutoTexture.argtypes = [C.c_int]
+    L.galphaAutoTexture.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def auto_texture(n):
+    return lib().galphaAutoTexture(int(n))
+
+
+def _alpha_to_numpy(A):
+    a = A.contents
+
+    def arr(p, n, dt):
+        if n <= 0:
+            return np.zeros(0, dtype=dt)
+        return np.ctypeslib.as_array(p, shape=(n,)).astype(dt, copy=True)
+    return dict(
+        spectrum=arr(a.spectrum, a.specSize, np.float64),
+        tri_ro_rank=arr(a.tri_ro_rank, a.nTris, np.int32),
+        edge_t=arr(a.edge_t, a.nEdges, np.int32), edge_vopp=arr(a.edge_vopp, a.nEdges, np.int32),
+        edge_ro_rank=arr(a.edge_ro_rank, a.nEdges, np.int32),
+        edge_nu_min=arr(a.edge_nu_min_rank, a.nEdges, np.int32),
+        edge_nu_max=arr(a.edge_nu_max_rank, a.nEdges, np.int32),
+        vert_nu_min=arr(a.vert_nu_min_rank, a.nPoints, np.int32),
+        vert_nu_max=arr(a.vert_nu_max_rank, a.nPoints, np.int32),
+        master_simplex=arr(a.master_simplex, a.masterSize, np.int32),
+        master_st=arr(a.master_st, a.masterSize, np.int32),
+        master_at=arr(a.master_at, a.masterSize, np.int32),
+        spec_pointers=arr(a.spec_pointers, a.nSpecPointers, np.int32))
+
+
+class Context:
+    """One device context (buffers + HIP stream) -- the analogue of the reference's global
+    GPU-DT state (gpudt.cpp:64-77, Cuda/cudaMain.cu:69-92), made reentrant."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        self._h = self._L.galphaCreate(int(device))
+        if not self._h:
+            raise GalphaError("galphaCreate failed: " + self._L.gpudtGetLastError().decode())
+
+    def close(self):
+        if self._h:
+            self._L.galphaDestroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _err(self, what):
+        raise GalphaError(f"{what}: {self._L.galphaLastError(self._h).decode()}")
+
+    def delaunay(self, xy, fbo_size=None):
+        """Canonical DT of xy ([n, 2] float64) as int32 [F, 9] (tri[3], vtx[3], tmp[3])."""
+        xy = np.ascontiguousarray(xy, dtype=np.float64)
+        n = xy.shape[0]
+        T = int(fbo_size or auto_texture(n))
+        cap = C.c_int(0)
+        F = self._L.galphaComputeHost(self._h, _dp(xy), n, T, None, C.byref(cap))
+        if F < 0:
+            self._err("galphaComputeHost")
+        out = np.zeros((max(F, 0), 9), dtype=np.int32)
+        if F > 0:
+            got = self._L.galphaCopyTriangles(self._h, _ip(out), F)
+            if got < 0:
+                self._err("galphaCopyTriangles")
+        return out
+
+    def compute_device(self, d_xy_ptr, n, fbo_size, with_alpha=False, stream=None):
+        """Device-resident run; d_xy_ptr is a device address of n (x, y) doubles."""
+        F = self._L.galphaComputeDevice(self._h, C.c_void_p(d_xy_ptr), int(n), int(fbo_size),
+                                        1 if with_alpha else 0, C.c_void_p(stream or 0))
+        if F < 0:
+            self._err("galphaComputeDevice")
+        return F
+
+    def triangles(self):
+        p, nt = C.c_void_p(), C.c_int()
+        self._L.galphaDeviceTriangles(self._h, C.byref(p), C.byref(nt))
+        out = np.zeros((nt.value, 9), dtype=np.int32)
+        if nt.value:
+            if self._L.galphaCopyTriangles(self._h, _ip(out), nt.value) < 0:
+                self._err("galphaCopyTriangles")
+        return out
+
+    def alpha(self):
+        """Alpha result of the last compute_device(with_alpha=True)."""
+        A = C.POINTER(_Alpha)()
+        if self._L.galphaCopyAlpha(self._h, C.byref(A)) < 0:
+            self._err("galphaCopyAlpha")
+        try:
+            return _alpha_to_numpy(A)
+        finally:
+            self._L.gpudtReleaseAlpha(A)
+
+    def stats(self):
+        s = _Stats()
+        self._L.galphaGetStats(self._h, C.byref(s))
+        return {k: getattr(s, k) for k, _ in _Stats._fields_}
+
+    def voronoi(self, xy, fbo_size):
+        """Discrete Voronoi texture: (labels [T, T] x-major, pix [n + frame], n_frame)."""
+        xy = np.ascontiguousarray(xy, dtype=np.float64)
+        n, T = xy.shape[0], int(fbo_size)
+        labels = np.zeros(T * T, dtype=np.int32)
+        cap = n + 4 * T
+        pix = np.zeros(cap, dtype=np.int32)
+        nf = self._L.galphaVoronoi(self._h, _dp(xy), n, T, _ip(labels), _ip(pix), cap)
+        if nf < 0:
+            self._err("galphaVoronoi")
+        return labels.reshape(T, T), pix[:n + nf].copy(), nf
+
+
+_default = None
+
+
+def default_context():
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+def gpudt_compute_dt(points, fbo_size):
+    """Mirror of gpudtComputeDT (gpudt/gpudt.h:131): points [n, 2] -> triangles [F, 9]."""
+    return default_context().delaunay(points, fbo_size)
+
+
+def compute_alpha_shapes(points, fbo_size):
+    """Mirror of Alpha_Shapes_2D(pInput, pDelaunay).computeAlphaShapes()
+    (AlphaShape/Alpha_Shapes_2D.cu:206-233): DT + alpha spectrum, intervals and master list."""
+    import ctypes
+    ctx = default_context()
+    xy = np.ascontiguousarray(points, dtype=np.float64)
+    import torch  # device memory for the device-resident entry point
+    d = torch.from_numpy(xy).to("cuda")
+    ctx.compute_device(d.data_ptr(), xy.shape[0], fbo_size, with_alpha=True)
+    tris = ctx.triangles()
+    res = ctx.alpha()
+    del ctypes
+    return tris, res

@@ -1,0 +1,697 @@
+// alpha.hip -- alpha-shape spectrum, intervals and master list on gfx950.
+//
+// Reference semantics (paths relative to the reference root):
+//   edge ownership / ids      AlphaShape/kernel.cu:160-217, :252-308  -- triangle t owns the edge
+//                             opposite vtx[i] iff vtx[i+1] < vtx[i+2] or tri[i] == -1; edges are
+//                             numbered by (t, i) through an exclusive scan of per-triangle counts.
+//   rho                       AlphaShape/predicates.h:1297-1357 (birth_time): same FP64 expression
+//                             order (no FMA), so spectrum values match the reference bit for bit.
+//   attachment                AlphaShape/predicates.h:1359-1444, kernel.cu:310-358: attached iff
+//                             the exact |i-j|^2 - |i+j-2k|^2 is >= 0 for the owning triangle's apex,
+//                             else for the opposite apex.
+//   exact order + merging     Alpha_Shapes_2D.cu:308-522, kernel.cu:360-401: the reference sorts by
+//                             FP64 value, runs odd-even exact fix-up rounds on the device with a
+//                             200-double expansion budget and sends overflow to a CPU insertion
+//                             sort.  Here every entry carries a certified interval; a radix sort by
+//                             FP64 value, a prefix-max / suffix-min scan of the interval ends and a
+//                             per-cluster exact insertion sort (integer arithmetic on the device)
+//                             give the exact order with no host pass.  Equal exact values share a
+//                             rank; rank = 1 + number of smaller distinct values.
+//   intervals                 kernel.cu:427-462 (edges), :518-624 (vertices; here by per-edge
+//                             atomic min/max instead of a fan walk -- same result).
+//   master list               kernel.cu:626-690 + Alpha_Shapes_2D.cu:641-727: stable radix sort of
+//                             rank keys; entries with rank < 1 dropped; spec_pointers = [-1] + the
+//                             first index of each rank.
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+#include "dev_common.h"
+#include "kernels.h"
+#include "alpha_host.h"
+
+#define GS(i, n) for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+#define AGRID 2048
+#define U53 1.1102230246251565e-16
+
+GA_INLINE double2 P(const double2 *xy, int v) { return xy[v]; }
+
+// ---- rho in the reference's expression order -------------------------------------------------
+GA_INLINE double rho_tri(double2 pa, double2 pb, double2 pc)
+{
+    double x1 = pa.x, y1 = pa.y, x2 = pb.x, y2 = pb.y, x3 = pc.x, y3 = pc.y;
+    double ax = x1 - x2, ay = y1 - y2, bx = x2 - x3, by = y2 - y3, cx = x3 - x1, cy = y3 - y1;
+    ax = ax * ax; ay = ay * ay; bx = bx * bx; by = by * by; cx = cx * cx; cy = cy * cy;
+    double a = ax + ay, b = bx + by, c = cx + cy;
+    double ar1 = x2 * y3 - x3 * y2 - x1 * y3 + x3 * y1 + x1 * y2 - x2 * y1;
+    double den = 4 * ar1 * ar1;
+    return (a * b * c) / den;
+}
+
+GA_INLINE double rho_edge(double2 pa, double2 pb)
+{
+    double ax = pa.x - pb.x, ay = pa.y - pb.y;
+    ax = ax * ax; ay = ay * ay;
+    return (ax + ay) / 4;
+}
+
+// ---- exact helpers (integer arithmetic over a common unit) ------------------------------------
+#define ANL 40
+typedef BigInt<ANL> BI;
+
+GA_INLINE void bi_sq_dist(BI &r, const BI &ax, const BI &ay, const BI &bx, const BI &by, int *ovf)
+{
+    BI dx, dy, t;
+    bi_sub(dx, ax, bx, ovf);
+    bi_sub(dy, ay, by, ovf);
+    bi_mul(t, dx, dx, ovf);
+    bi_mul(dy, dy, dy, ovf);
+    bi_add(r, t, dy, ovf);
+}
+
+// entries: st 0 = edge (p, q), 1 = triangle (a, b, c).  Returns sign(rho_A - rho_B).
+__device__ __noinline__ int rho_cmp_exact(int sta, const double2 *pa, int stb, const double2 *pb, int *ovf)
+{
+    double v[12];
+    int na = sta ? 3 : 2, nb = stb ? 3 : 2, k = 0;
+    for (int i = 0; i < na; i++) { v[k++] = pa[i].x; v[k++] = pa[i].y; }
+    for (int i = 0; i < nb; i++) { v[k++] = pb[i].x; v[k++] = pb[i].y; }
+    int s = min_ulp_exp(v, k);
+    BI X[6], Y[6];
+    for (int i = 0; i < na; i++) { bi_from_double(X[i], pa[i].x, s, ovf); bi_from_double(Y[i], pa[i].y, s, ovf); }
+    for (int i = 0; i < nb; i++) { bi_from_double(X[3 + i], pb[i].x, s, ovf); bi_from_double(Y[3 + i], pb[i].y, s, ovf); }
+    // value = num / den (den > 0): edge num = |pq|^2, den = 1 ; triangle num = l1 l2 l3, den = ar^2
+    BI num[2], den[2], t1, t2;
+    int stv[2] = {sta, stb};
+    for (int e = 0; e < 2; e++) {
+        int o = e * 3;
+        if (!stv[e]) {
+            bi_sq_dist(num[e], X[o], Y[o], X[o + 1], Y[o + 1], ovf);
+            bi_zero(den[e]);
+            den[e].l[0] = 1; den[e].n = 1;
+        } else {
+            BI l1, l2, l3;
+            bi_sq_dist(l1, X[o], Y[o], X[o + 1], Y[o + 1], ovf);
+            bi_sq_dist(l2, X[o + 1], Y[o + 1], X[o + 2], Y[o + 2], ovf);
+            bi_sq_dist(l3, X[o + 2], Y[o + 2], X[o], Y[o], ovf);
+            bi_mul(t1, l1, l2, ovf);
+            bi_mul(num[e], t1, l3, ovf);
+            // ar = (bx-ax)(cy-ay) - (by-ay)(cx-ax)
+            BI ux, uy, vx, vy;
+            bi_sub(ux, X[o + 1], X[o], ovf); bi_sub(uy, Y[o + 1], Y[o], ovf);
+            bi_sub(vx, X[o + 2], X[o], ovf); bi_sub(vy, Y[o + 2], Y[o], ovf);
+            bi_mul(t1, ux, vy, ovf);
+            bi_mul(t2, uy, vx, ovf);
+            bi_sub(t1, t1, t2, ovf);
+            bi_mul(den[e], t1, t1, ovf);
+        }
+    }
+    bi_mul(t1, num[0], den[1], ovf);
+    bi_mul(t2, num[1], den[0], ovf);
+    bi_sub(t1, t1, t2, ovf);
+    return bi_sign(t1);
+}
+
+// attachment: exact sign of |i-j|^2 - |i+j-2k|^2 (>= 0 attached)
+__device__ __noinline__ int attached_exact(double2 i, double2 j, double2 k, int *ovf)
+{
+    double v[6] = {i.x, i.y, j.x, j.y, k.x, k.y};
+    int s = min_ulp_exp(v, 6) - 1;   // -1: 2k stays an integer multiple of the unit
+    BigInt<16> ix, iy, jx, jy, kx, ky, d1, sx, sy, t;
+    bi_from_double(ix, i.x, s, ovf); bi_from_double(iy, i.y, s, ovf);
+    bi_from_double(jx, j.x, s, ovf); bi_from_double(jy, j.y, s, ovf);
+    bi_from_double(kx, k.x, s, ovf); bi_from_double(ky, k.y, s, ovf);
+    BigInt<16> dx, dy;
+    bi_sub(dx, ix, jx, ovf); bi_sub(dy, iy, jy, ovf);
+    bi_mul(dx, dx, dx, ovf); bi_mul(dy, dy, dy, ovf);
+    bi_add(d1, dx, dy, ovf);
+    bi_add(sx, ix, jx, ovf); bi_sub(sx, sx, kx, ovf); bi_sub(sx, sx, kx, ovf);
+    bi_add(sy, iy, jy, ovf); bi_sub(sy, sy, ky, ovf); bi_sub(sy, sy, ky, ovf);
+    bi_mul(sx, sx, sx, ovf); bi_mul(sy, sy, sy, ovf);
+    bi_add(t, sx, sy, ovf);
+    bi_sub(t, d1, t, ovf);
+    return bi_sign(t) >= 0;
+}
+
+struct AlphaK {
+    const double2 *xy;
+    const int *tris;     // F x 9
+    int n, F;
+    int *cnt, *eoff;     // [F]
+    int *edge_t, *edge_v, *edge_att, *edge_rank, *edge_numin, *edge_numax;  // [E]
+    int *tri_rank;       // [F]
+    unsigned long long *key;   // [S0] rho bits
+    int *id, *id2;       // [S0] entry ids (t or F + e)
+    unsigned long long *key2;
+    double *lo, *hi;     // [F + E] per entry id
+    double *slo, *shi;   // [S0] sorted
+    int *flag;           // [S0] cluster starts
+    int *gflag;          // [S0] group starts (exact)
+    int *rank;           // [S0]
+    unsigned long long *specbits;  // [S0 + 1]
+    int *vnumin, *vnumax;  // [n]
+    int *status;
+    int *counters;       // [16]
+};
+
+#define A_NUNATT 0
+#define A_EXACT_CMP 1
+#define A_EXACT_ATT 2
+#define A_CLUSTERS 3
+#define A_MAXCLUSTER 4
+#define A_BIGCLUSTER 5
+
+GA_INLINE int tv(const AlphaK &a, int t, int i) { return a.tris[(size_t)t * 9 + 3 + i]; }
+GA_INLINE int tn(const AlphaK &a, int t, int i) { return a.tris[(size_t)t * 9 + i]; }
+GA_INLINE int pl1(int i) { return i == 2 ? 0 : i + 1; }
+GA_INLINE int mi1(int i) { return i == 0 ? 2 : i - 1; }
+
+__global__ void k_edge_count(AlphaK a)
+{
+    GS(t, a.F) {
+        int c = 0;
+        for (int i = 0; i < 3; i++)
+            c += (tv(a, t, pl1(i)) < tv(a, t, mi1(i))) || tn(a, t, i) == -1;
+        a.cnt[t] = c;
+    }
+}
+
+__global__ void k_edges(AlphaK a)
+{
+    GS(t, a.F) {
+        int e = a.eoff[t];
+        for (int i = 0; i < 3; i++)
+            if ((tv(a, t, pl1(i)) < tv(a, t, mi1(i))) || tn(a, t, i) == -1) {
+                a.edge_t[e] = t;
+                a.edge_v[e] = i;
+                e++;
+            }
+    }
+}
+
+// relative error bound of the certified rho estimate used for ordering
+__global__ void k_tri_rho(AlphaK a)
+{
+    GS(t, a.F) {
+        double2 A = a.xy[tv(a, t, 0)], B = a.xy[tv(a, t, 1)], Cc = a.xy[tv(a, t, 2)];
+        double v = rho_tri(A, B, Cc);
+        a.key[t] = (unsigned long long)__double_as_longlong(v);
+        a.id[t] = t;
+        // ar with Shewchuk's orient2d bound: |ar - det| <= eb
+        double detleft = (A.x - Cc.x) * (B.y - Cc.y), detright = (A.y - Cc.y) * (B.x - Cc.x);
+        double det = detleft - detright;
+        double eb = GA_CCWERRA * (fabs(detleft) + fabs(detright));
+        double la = (A.x - B.x) * (A.x - B.x) + (A.y - B.y) * (A.y - B.y);
+        double lb = (B.x - Cc.x) * (B.x - Cc.x) + (B.y - Cc.y) * (B.y - Cc.y);
+        double lc = (Cc.x - A.x) * (Cc.x - A.x) + (Cc.y - A.y) * (Cc.y - A.y);
+        double lo, hi;
+        if (eb < 0.25 * fabs(det)) {
+            double r = eb / fabs(det);
+            double est = (la * lb * lc) / (4.0 * det * det);
+            double rel = 32.0 * U53 + 3.0 * r;
+            lo = est * (1.0 - rel);
+            hi = est * (1.0 + rel);
+        } else {
+            lo = 0.0;
+            hi = INFINITY;
+        }
+        a.lo[t] = lo;
+        a.hi[t] = hi;
+    }
+}
+
+__global__ void k_attached(AlphaK a)
+{
+    int E = a.eoff[a.F - 1] + a.cnt[a.F - 1];
+    GS(e, E) {
+        int t = a.edge_t[e], i = a.edge_v[e];
+        double2 p = a.xy[tv(a, t, pl1(i))], q = a.xy[tv(a, t, mi1(i))];
+        int nb = tn(a, t, i);
+        int att = 0;
+        for (int side = 0; side < 2 && !att; side++) {
+            double2 k;
+            if (side == 0) k = a.xy[tv(a, t, i)];
+            else {
+                if (nb < 0) break;
+                k = a.xy[tv(a, nb >> 2, nb & 3)];
+            }
+            double dx = p.x - q.x, dy = p.y - q.y;
+            double d1 = dx * dx + dy * dy;
+            double sx = p.x + q.x - 2 * k.x, sy = p.y + q.y - 2 * k.y;
+            double d2 = sx * sx + sy * sy;
+            double diff = d1 - d2;
+            double px = fabs(p.x + q.x) + 2 * fabs(k.x), py = fabs(p.y + q.y) + 2 * fabs(k.y);
+            double eb = 32.0 * U53 * (d1 + px * px + py * py);
+            if (diff > eb) att = 1;
+            else if (-diff > eb) att = 0;
+            else {
+                atomicAdd(&a.counters[A_EXACT_ATT], 1);
+                int ovf = 0;
+                att = attached_exact(p, q, k, &ovf);
+                if (ovf) atomicOr(a.status, GA_ERR_EXACT_OVERFLOW);
+            }
+        }
+        a.edge_att[e] = att;
+        a.flag[e] = !att;
+        double v = rho_edge(p, q);
+        a.lo[a.F + e] = v * (1.0 - 8.0 * U53);
+        a.hi[a.F + e] = v * (1.0 + 8.0 * U53);
+    }
+}
+
+// unattached edges appended after the triangles (flag scanned into rank[] positions)
+__global__ void k_edge_entries(AlphaK a, int E)
+{
+    GS(e, E) {
+        if (a.edge_att[e]) continue;
+        int k = a.F + a.rank[e];
+        int t = a.edge_t[e], i = a.edge_v[e];
+        double v = rho_edge(a.xy[tv(a, t, pl1(i))], a.xy[tv(a, t, mi1(i))]);
+        a.key[k] = (unsigned long long)__double_as_longlong(v);
+        a.id[k] = a.F + e;
+    }
+}
+
+__global__ void k_gather_iv(AlphaK a, int S0, double *neg_lo_rev)
+{
+    GS(k, S0) {
+        int id = a.id2[k];
+        a.shi[k] = a.hi[id];
+        double l = a.lo[id];
+        a.slo[k] = l;
+        neg_lo_rev[S0 - 1 - k] = -l;
+    }
+}
+
+// cluster starts: boundary before k when max(hi[0..k-1]) < min(lo[k..])
+__global__ void k_cluster_flags(AlphaK a, int S0, const double *pmax, const double *neg_smin_rev)
+{
+    GS(k, S0) {
+        int f = 1;
+        if (k > 0) {
+            double m = pmax[k - 1];
+            double s = -neg_smin_rev[S0 - 1 - k];
+            f = m < s;
+        }
+        a.flag[k] = f;
+    }
+}
+
+GA_INLINE void entry_pts(const AlphaK &a, int id, int *st, double2 *p)
+{
+    if (id < a.F) {
+        *st = 1;
+        p[0] = a.xy[tv(a, id, 0)]; p[1] = a.xy[tv(a, id, 1)]; p[2] = a.xy[tv(a, id, 2)];
+    } else {
+        int e = id - a.F;
+        int t = a.edge_t[e], i = a.edge_v[e];
+        *st = 0;
+        p[0] = a.xy[tv(a, t, pl1(i))]; p[1] = a.xy[tv(a, t, mi1(i))];
+    }
+}
+
+GA_INLINE int cmp_ids(const AlphaK &a, int ia, int ib)
+{
+    double la = a.lo[ia], ha = a.hi[ia], lb = a.lo[ib], hb = a.hi[ib];
+    if (ha < lb) return -1;
+    if (hb < la) return 1;
+    atomicAdd(&a.counters[A_EXACT_CMP], 1);
+    int sa, sb;
+    double2 pa[3], pb[3];
+    entry_pts(a, ia, &sa, pa);
+    entry_pts(a, ib, &sb, pb);
+    int ovf = 0;
+    int r = rho_cmp_exact(sa, pa, sb, pb, &ovf);
+    if (ovf) atomicOr(a.status, GA_ERR_EXACT_OVERFLOW);
+    return r;
+}
+
+#define MAX_SMALL_CLUSTER 4096
+// one thread per cluster start with size >= 2: exact insertion sort, then group flags
+__global__ void k_clusters(AlphaK a, int S0)
+{
+    GS(k, S0) {
+        if (!a.flag[k]) continue;
+        if (k + 1 >= S0 || a.flag[k + 1]) { continue; }
+        int end = k + 1;
+        while (end < S0 && !a.flag[end]) end++;
+        int m = end - k;
+        atomicAdd(&a.counters[A_CLUSTERS], 1);
+        atomicMax(&a.counters[A_MAXCLUSTER], m);
+        if (m > MAX_SMALL_CLUSTER) { atomicOr(a.status, GA_ERR_CAPACITY); atomicAdd(&a.counters[A_BIGCLUSTER], 1); continue; }
+        int *ids = a.id2 + k;
+        for (int i = 1; i < m; i++) {
+            int x = ids[i], j = i - 1;
+            while (j >= 0 && cmp_ids(a, ids[j], x) > 0) { ids[j + 1] = ids[j]; j--; }
+            ids[j + 1] = x;
+        }
+        // group starts inside the cluster (gflag[k] stays 1)
+        for (int i = 1; i < m; i++) a.gflag[k + i] = cmp_ids(a, ids[i - 1], ids[i]) != 0;
+    }
+}
+
+__global__ void k_copy_i32(const int *a, int *b, int n)
+{
+    GS(i, n) b[i] = a[i];
+}
+
+// inclusive = exclusive + flag (in place over the flag array)
+__global__ void k_incl_from_excl(const int *excl, int *flag_incl, int n)
+{
+    GS(i, n) flag_incl[i] = excl[i] + flag_incl[i];
+}
+
+__global__ void k_ranks(AlphaK a, int S0, const int *incl)
+{
+    GS(k, S0) {
+        int r = incl[k];   // 1-based rank
+        int id = a.id2[k];
+        double v;
+        if (id < a.F) { a.tri_rank[id] = r; v = rho_tri(a.xy[tv(a, id, 0)], a.xy[tv(a, id, 1)], a.xy[tv(a, id, 2)]); }
+        else {
+            int e = id - a.F;
+            a.edge_rank[e] = r;
+            int t = a.edge_t[e], i = a.edge_v[e];
+            v = rho_edge(a.xy[tv(a, t, pl1(i))], a.xy[tv(a, t, mi1(i))]);
+        }
+        atomicMin(&a.specbits[r], (unsigned long long)__double_as_longlong(v));
+    }
+}
+
+__global__ void k_edge_intervals(AlphaK a, int E)
+{
+    GS(e, E) {
+        int t = a.edge_t[e], i = a.edge_v[e];
+        if (a.edge_att[e]) a.edge_rank[e] = 0;
+        int r1 = a.tri_rank[t];
+        int nb = tn(a, t, i);
+        int mn, mx;
+        if (nb < 0) { mn = r1; mx = 0; }
+        else {
+            int r2 = a.tri_rank[nb >> 2];
+            mn = r1 <= r2 ? r1 : r2;
+            mx = r1 <= r2 ? r2 : r1;
+        }
+        a.edge_numin[e] = mn;
+        a.edge_numax[e] = mx;
+        int key = a.edge_rank[e] != 0 ? a.edge_rank[e] : mn;
+        int u = tv(a, t, pl1(i)), w = tv(a, t, mi1(i));
+        atomicMin(&a.vnumin[u], key); atomicMin(&a.vnumin[w], key);
+        atomicMax(&a.vnumax[u], mx); atomicMax(&a.vnumax[w], mx);
+    }
+}
+
+__global__ void k_vert_fix(AlphaK a)
+{
+    GS(v, a.n) {
+        if (a.vnumin[v] == 0x7fffffff) { a.vnumin[v] = -1; a.vnumax[v] = -1; }
+    }
+}
+
+// master keys: q < 2n vertices, < 2n + 3E edges, then triangles
+__global__ void k_master_keys(AlphaK a, int E, long long M, unsigned int *mkey, int *mval, int *mflag)
+{
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < M; q += (long long)gridDim.x * blockDim.x) {
+        int r;
+        long long n2 = 2LL * a.n, e3 = 3LL * E;
+        if (q < n2) r = (q & 1) ? a.vnumax[q >> 1] : a.vnumin[q >> 1];
+        else if (q < n2 + e3) {
+            long long w = q - n2;
+            int e = (int)(w / 3), at = (int)(w % 3);
+            r = at == 0 ? a.edge_rank[e] : (at == 1 ? a.edge_numin[e] : a.edge_numax[e]);
+        } else r = a.tri_rank[q - n2 - e3];
+        mflag[q] = r >= 1;
+        mkey[q] = r >= 1 ? (unsigned)r : 0u;
+        mval[q] = (int)q;
+    }
+}
+
+__global__ void k_master_compact(long long M, const unsigned int *mkey, const int *mval, const int *mflag,
+                                 const int *mpos, unsigned int *okey, int *oval)
+{
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < M; q += (long long)gridDim.x * blockDim.x) {
+        if (!mflag[q]) continue;
+        int d = mpos[q];
+        okey[d] = mkey[q];
+        oval[d] = mval[q];
+    }
+}
+
+__global__ void k_master_decode(int n, int E, int Mk, const unsigned int *skey, const int *sval, int *simplex, int *st,
+                                int *at, int *pflag)
+{
+    GS(k, Mk) {
+        long long q = sval[k];
+        long long n2 = 2LL * n, e3 = 3LL * E;
+        if (q < n2) { simplex[k] = (int)(q >> 1); st[k] = 2; at[k] = (int)(q & 1) + 1; }
+        else if (q < n2 + e3) { long long w = q - n2; simplex[k] = (int)(w / 3); st[k] = 0; at[k] = (int)(w % 3); }
+        else { simplex[k] = (int)(q - n2 - e3); st[k] = 1; at[k] = 0; }
+        pflag[k] = (k == 0) || (skey[k] != skey[k - 1]);
+    }
+}
+
+__global__ void k_spec_pointers(int Mk, const int *pflag, const int *ppos, int *sp)
+{
+    GS(k, Mk) {
+        if (pflag[k]) sp[1 + ppos[k]] = k;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) sp[0] = -1;
+}
+
+__global__ void k_spec_values(const unsigned long long *bits, int R, double *spec)
+{
+    GS(r, R + 1) spec[r] = r == 0 ? 0.0 : __longlong_as_double((long long)bits[r]);
+}
+
+// ---- host side ------------------------------------------------------------------------------------
+enum {
+    AB_CNT, AB_EOFF, AB_ET, AB_EV, AB_EATT, AB_ERANK, AB_ENMIN, AB_ENMAX, AB_TRANK, AB_KEY, AB_KEY2, AB_ID,
+    AB_ID2, AB_LO, AB_HI, AB_SLO, AB_SHI, AB_FLAG, AB_RANK, AB_SPECBITS, AB_VMIN, AB_VMAX, AB_STATUS, AB_CNTRS,
+    AB_TMP, AB_SCAN, AB_NEG, AB_PMAX, AB_NSMIN, AB_MKEY, AB_MVAL, AB_MFLAG, AB_MPOS, AB_OKEY, AB_OVAL,
+    AB_SKEY, AB_SVAL, AB_SPEC, AB_SP, AB_MISC
+};
+
+static void ab(AlphaDev &d, int i, size_t bytes)
+{
+    AlphaBuf &b = d.b[i];
+    if (bytes <= b.bytes) return;
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    size_t nb = bytes + bytes / 8 + 256;
+    if (hipMalloc(&b.p, nb) != hipSuccess) throw GalphaError("alpha: device allocation failed");
+    b.bytes = nb;
+}
+template <class T> static T *AP(AlphaDev &d, int i) { return (T *)d.b[i].p; }
+
+void alpha_release(AlphaDev &d)
+{
+    for (auto &b : d.b)
+        if (b.p) { hipFree(b.p); b.p = nullptr; b.bytes = 0; }
+}
+
+static void chk(hipError_t e, const char *w)
+{
+    if (e != hipSuccess) throw GalphaError(std::string("alpha: ") + w + ": " + hipGetErrorString(e));
+}
+
+void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F, hipStream_t st, galpha_stats *stats)
+{
+    d.n = n; d.F = F; d.E = 0; d.S = 0; d.specSize = 1; d.M = 0; d.nsp = 1;
+    ab(d, AB_STATUS, 16);
+    ab(d, AB_CNTRS, 64 * 4);
+    chk(hipMemsetAsync(d.b[AB_STATUS].p, 0, 16, st), "memset");
+    chk(hipMemsetAsync(d.b[AB_CNTRS].p, 0, 256, st), "memset");
+    ab(d, AB_VMIN, 4 * (size_t)(n + 1));
+    ab(d, AB_VMAX, 4 * (size_t)(n + 1));
+    ab(d, AB_SPEC, 8 * 2);
+    if (F <= 0) {
+        double z = 0;
+        chk(hipMemcpyAsync(d.b[AB_SPEC].p, &z, 8, hipMemcpyHostToDevice, st), "copy");
+        chk(hipMemsetAsync(d.b[AB_VMIN].p, 0xff, 4 * (size_t)n + 4, st), "memset");
+        chk(hipMemsetAsync(d.b[AB_VMAX].p, 0xff, 4 * (size_t)n + 4, st), "memset");
+        ab(d, AB_SP, 8);
+        int m1 = -1;
+        chk(hipMemcpyAsync(d.b[AB_SP].p, &m1, 4, hipMemcpyHostToDevice, st), "copy");
+        chk(hipStreamSynchronize(st), "sync");
+        return;
+    }
+    ab(d, AB_CNT, 4 * (size_t)F);
+    ab(d, AB_EOFF, 4 * (size_t)F);
+    ab(d, AB_SCAN, scan_temp_bytes((size_t)3 * F + 2 * (size_t)n + 16) + 4096);
+    AlphaK a{};
+    a.xy = (const double2 *)d_xy; a.tris = d_tris9; a.n = n; a.F = F;
+    a.cnt = AP<int>(d, AB_CNT); a.eoff = AP<int>(d, AB_EOFF);
+    a.status = AP<int>(d, AB_STATUS); a.counters = AP<int>(d, AB_CNTRS);
+    hipLaunchKernelGGL(k_edge_count, dim3(AGRID), dim3(256), 0, st, a);
+    launch_exclusive_scan(a.cnt, a.eoff, F, d.b[AB_SCAN].p, st);
+    int h2[2];
+    chk(hipMemcpyAsync(&h2[0], a.eoff + F - 1, 4, hipMemcpyDeviceToHost, st), "copy");
+    chk(hipMemcpyAsync(&h2[1], a.cnt + F - 1, 4, hipMemcpyDeviceToHost, st), "copy");
+    chk(hipStreamSynchronize(st), "sync");
+    int E = h2[0] + h2[1];
+    d.E = E;
+    size_t NE = (size_t)F + E;
+    ab(d, AB_ET, 4 * (size_t)E); ab(d, AB_EV, 4 * (size_t)E); ab(d, AB_EATT, 4 * (size_t)E);
+    ab(d, AB_ERANK, 4 * (size_t)E); ab(d, AB_ENMIN, 4 * (size_t)E); ab(d, AB_ENMAX, 4 * (size_t)E);
+    ab(d, AB_TRANK, 4 * (size_t)F);
+    ab(d, AB_KEY, 8 * NE); ab(d, AB_KEY2, 8 * NE); ab(d, AB_ID, 4 * NE); ab(d, AB_ID2, 4 * NE);
+    ab(d, AB_LO, 8 * NE); ab(d, AB_HI, 8 * NE); ab(d, AB_SLO, 8 * NE); ab(d, AB_SHI, 8 * NE);
+    ab(d, AB_FLAG, 4 * NE); ab(d, AB_RANK, 4 * NE); ab(d, AB_SPECBITS, 8 * (NE + 2));
+    ab(d, AB_NEG, 8 * NE); ab(d, AB_PMAX, 8 * NE); ab(d, AB_NSMIN, 8 * NE); ab(d, AB_MISC, 4 * NE);
+    a.edge_t = AP<int>(d, AB_ET); a.edge_v = AP<int>(d, AB_EV); a.edge_att = AP<int>(d, AB_EATT);
+    a.edge_rank = AP<int>(d, AB_ERANK); a.edge_numin = AP<int>(d, AB_ENMIN); a.edge_numax = AP<int>(d, AB_ENMAX);
+    a.tri_rank = AP<int>(d, AB_TRANK);
+    a.key = AP<unsigned long long>(d, AB_KEY); a.key2 = AP<unsigned long long>(d, AB_KEY2);
+    a.id = AP<int>(d, AB_ID); a.id2 = AP<int>(d, AB_ID2);
+    a.lo = AP<double>(d, AB_LO); a.hi = AP<double>(d, AB_HI);
+    a.slo = AP<double>(d, AB_SLO); a.shi = AP<double>(d, AB_SHI);
+    a.flag = AP<int>(d, AB_FLAG); a.rank = AP<int>(d, AB_RANK); a.gflag = AP<int>(d, AB_MISC);
+    a.specbits = AP<unsigned long long>(d, AB_SPECBITS);
+    a.vnumin = AP<int>(d, AB_VMIN); a.vnumax = AP<int>(d, AB_VMAX);
+    hipLaunchKernelGGL(k_edges, dim3(AGRID), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_tri_rho, dim3(AGRID), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_attached, dim3(AGRID), dim3(256), 0, st, a);
+    // positions of unattached edges
+    launch_exclusive_scan(a.flag, a.rank, E, d.b[AB_SCAN].p, st);
+    int h3[2];
+    chk(hipMemcpyAsync(&h3[0], a.rank + E - 1, 4, hipMemcpyDeviceToHost, st), "copy");
+    chk(hipMemcpyAsync(&h3[1], a.flag + E - 1, 4, hipMemcpyDeviceToHost, st), "copy");
+    chk(hipStreamSynchronize(st), "sync");
+    int nun = h3[0] + h3[1];
+    int S0 = F + nun;
+    d.S = S0;
+    hipLaunchKernelGGL(k_edge_entries, dim3(AGRID), dim3(256), 0, st, a, E);
+    // radix sort by FP64 bits (positive doubles order like their bit patterns)
+    size_t tb = 0;
+    chk(rocprim::radix_sort_pairs(nullptr, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, 64u, st), "sort");
+    ab(d, AB_TMP, tb + 256);
+    tb = d.b[AB_TMP].bytes;
+    chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, 64u, st), "sort");
+    // certified intervals in sorted order, prefix max of hi, suffix min of lo
+    double *neg = AP<double>(d, AB_NEG), *pmax = AP<double>(d, AB_PMAX), *nsmin = AP<double>(d, AB_NSMIN);
+    hipLaunchKernelGGL(k_gather_iv, dim3(AGRID), dim3(256), 0, st, a, S0, neg);
+    tb = 0;
+    chk(rocprim::inclusive_scan(nullptr, tb, a.shi, pmax, (size_t)S0, rocprim::maximum<double>(), st), "scan");
+    ab(d, AB_TMP, tb + 256);
+    tb = d.b[AB_TMP].bytes;
+    chk(rocprim::inclusive_scan(d.b[AB_TMP].p, tb, a.shi, pmax, (size_t)S0, rocprim::maximum<double>(), st), "scan");
+    tb = d.b[AB_TMP].bytes;
+    chk(rocprim::inclusive_scan(d.b[AB_TMP].p, tb, neg, nsmin, (size_t)S0, rocprim::maximum<double>(), st), "scan");
+    hipLaunchKernelGGL(k_cluster_flags, dim3(AGRID), dim3(256), 0, st, a, S0, (const double *)pmax, (const double *)nsmin);
+    hipLaunchKernelGGL(k_copy_i32, dim3(AGRID), dim3(256), 0, st, (const int *)a.flag, a.gflag, S0);
+    hipLaunchKernelGGL(k_clusters, dim3(AGRID), dim3(256), 0, st, a, S0);
+    // ranks: inclusive scan of group starts
+    launch_exclusive_scan(a.gflag, a.rank, S0, d.b[AB_SCAN].p, st);
+    int h4[2];
+    chk(hipMemcpyAsync(&h4[0], a.rank + S0 - 1, 4, hipMemcpyDeviceToHost, st), "copy");
+    chk(hipMemcpyAsync(&h4[1], a.gflag + S0 - 1, 4, hipMemcpyDeviceToHost, st), "copy");
+    chk(hipStreamSynchronize(st), "sync");
+    int R = h4[0] + h4[1];
+    d.specSize = R + 1;
+    chk(hipMemsetAsync(a.specbits, 0xff, 8 * (size_t)(R + 2), st), "memset");
+    // incl = excl + flag: reuse flag buffer as inclusive ranks
+    int *incl = a.gflag;
+    hipLaunchKernelGGL(k_incl_from_excl, dim3(AGRID), dim3(256), 0, st, (const int *)a.rank, incl, S0);
+    hipLaunchKernelGGL(k_ranks, dim3(AGRID), dim3(256), 0, st, a, S0, (const int *)incl);
+    ab(d, AB_SPEC, 8 * (size_t)(R + 2));
+    hipLaunchKernelGGL(k_spec_values, dim3(AGRID), dim3(256), 0, st, (const unsigned long long *)a.specbits, R,
+                       AP<double>(d, AB_SPEC));
+    // intervals
+    launch_fill_i32(a.vnumin, 0x7fffffff, (size_t)n, st);
+    launch_fill_i32(a.vnumax, -1, (size_t)n, st);
+    hipLaunchKernelGGL(k_edge_intervals, dim3(AGRID), dim3(256), 0, st, a, E);
+    hipLaunchKernelGGL(k_vert_fix, dim3(AGRID), dim3(256), 0, st, a);
+    // master list
+    long long M = 2LL * n + 3LL * E + F;
+    if (M > 0x7fffffffLL) throw GalphaError("alpha: master list too large");
+    ab(d, AB_MKEY, 4 * (size_t)M); ab(d, AB_MVAL, 4 * (size_t)M); ab(d, AB_MFLAG, 4 * (size_t)M);
+    ab(d, AB_MPOS, 4 * (size_t)M);
+    ab(d, AB_SCAN, scan_temp_bytes((size_t)M) + 4096);
+    unsigned *mkey = AP<unsigned>(d, AB_MKEY);
+    int *mval = AP<int>(d, AB_MVAL), *mflag = AP<int>(d, AB_MFLAG), *mpos = AP<int>(d, AB_MPOS);
+    hipLaunchKernelGGL(k_master_keys, dim3(AGRID), dim3(256), 0, st, a, E, M, mkey, mval, mflag);
+    launch_exclusive_scan(mflag, mpos, (size_t)M, d.b[AB_SCAN].p, st);
+    int h5[2];
+    chk(hipMemcpyAsync(&h5[0], mpos + M - 1, 4, hipMemcpyDeviceToHost, st), "copy");
+    chk(hipMemcpyAsync(&h5[1], mflag + M - 1, 4, hipMemcpyDeviceToHost, st), "copy");
+    chk(hipStreamSynchronize(st), "sync");
+    int Mk = h5[0] + h5[1];
+    d.M = Mk;
+    ab(d, AB_OKEY, 4 * (size_t)Mk + 16); ab(d, AB_OVAL, 4 * (size_t)Mk + 16);
+    ab(d, AB_SKEY, 4 * (size_t)Mk + 16); ab(d, AB_SVAL, 4 * (size_t)Mk + 16);
+    unsigned *okey = AP<unsigned>(d, AB_OKEY), *skey = AP<unsigned>(d, AB_SKEY);
+    int *oval = AP<int>(d, AB_OVAL), *sval = AP<int>(d, AB_SVAL);
+    hipLaunchKernelGGL(k_master_compact, dim3(AGRID), dim3(256), 0, st, M, (const unsigned *)mkey, (const int *)mval,
+                       (const int *)mflag, (const int *)mpos, okey, oval);
+    int bits = 1;
+    while ((1LL << bits) <= (long long)R) bits++;
+    tb = 0;
+    chk(rocprim::radix_sort_pairs(nullptr, tb, okey, skey, oval, sval, (unsigned)Mk, 0u, (unsigned)bits, st), "sort");
+    ab(d, AB_TMP, tb + 256);
+    tb = d.b[AB_TMP].bytes;
+    chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, okey, skey, oval, sval, (unsigned)Mk, 0u, (unsigned)bits, st), "sort");
+    // decode into mval/mflag/mpos reused: simplex -> okey (as int), st -> oval, at -> mkey ; pflag -> mflag
+    int *msimp = (int *)okey, *mst = oval, *mat = (int *)mkey;
+    hipLaunchKernelGGL(k_master_decode, dim3(AGRID), dim3(256), 0, st, n, E, Mk, (const unsigned *)skey, (const int *)sval,
+                       msimp, mst, mat, mflag);
+    launch_exclusive_scan(mflag, mpos, (size_t)Mk, d.b[AB_SCAN].p, st);
+    int h6[2] = {0, 0};
+    if (Mk > 0) {
+        chk(hipMemcpyAsync(&h6[0], mpos + Mk - 1, 4, hipMemcpyDeviceToHost, st), "copy");
+        chk(hipMemcpyAsync(&h6[1], mflag + Mk - 1, 4, hipMemcpyDeviceToHost, st), "copy");
+    }
+    chk(hipStreamSynchronize(st), "sync");
+    int np = h6[0] + h6[1];
+    d.nsp = np + 1;
+    ab(d, AB_SP, 4 * (size_t)(np + 2));
+    hipLaunchKernelGGL(k_spec_pointers, dim3(AGRID), dim3(256), 0, st, Mk, (const int *)mflag, (const int *)mpos,
+                       AP<int>(d, AB_SP));
+    int hc[8], hs = 0;
+    chk(hipMemcpyAsync(hc, a.counters, sizeof(hc), hipMemcpyDeviceToHost, st), "copy");
+    chk(hipMemcpyAsync(&hs, a.status, 4, hipMemcpyDeviceToHost, st), "copy");
+    chk(hipStreamSynchronize(st), "sync");
+    if (stats) {
+        stats->nEdges = E;
+        stats->specSize = R + 1;
+        stats->alphaExactCompares = hc[A_EXACT_CMP];
+        stats->alphaExactAttach = hc[A_EXACT_ATT];
+        stats->alphaClusters = hc[A_CLUSTERS];
+        stats->alphaMaxCluster = hc[A_MAXCLUSTER];
+    }
+    if (hs) throw GalphaError("alpha: device status " + std::to_string(hs) +
+                              (hc[A_BIGCLUSTER] ? " (exact cluster larger than supported)" : ""));
+}
+
+PGPUDTALPHA alpha_copy_to_host(AlphaDev &d, hipStream_t st)
+{
+    PGPUDTALPHA A = (PGPUDTALPHA)calloc(1, sizeof(GPUDTALPHA));
+    A->nPoints = d.n; A->nEdges = d.E; A->nTris = d.F; A->specSize = d.specSize;
+    A->masterSize = d.M; A->nSpecPointers = d.nsp;
+    auto cp = [&](void **dst, int idx, size_t bytes) {
+        *dst = malloc(bytes > 0 ? bytes : 1);
+        if (bytes) chk(hipMemcpyAsync(*dst, d.b[idx].p, bytes, hipMemcpyDeviceToHost, st), "copy");
+    };
+    cp((void **)&A->spectrum, AB_SPEC, 8 * (size_t)d.specSize);
+    cp((void **)&A->tri_ro_rank, AB_TRANK, 4 * (size_t)d.F);
+    cp((void **)&A->edge_t, AB_ET, 4 * (size_t)d.E);
+    cp((void **)&A->edge_vopp, AB_EV, 4 * (size_t)d.E);
+    cp((void **)&A->edge_ro_rank, AB_ERANK, 4 * (size_t)d.E);
+    cp((void **)&A->edge_nu_min_rank, AB_ENMIN, 4 * (size_t)d.E);
+    cp((void **)&A->edge_nu_max_rank, AB_ENMAX, 4 * (size_t)d.E);
+    cp((void **)&A->vert_nu_min_rank, AB_VMIN, 4 * (size_t)d.n);
+    cp((void **)&A->vert_nu_max_rank, AB_VMAX, 4 * (size_t)d.n);
+    cp((void **)&A->master_simplex, AB_OKEY, 4 * (size_t)d.M);
+    cp((void **)&A->master_st, AB_OVAL, 4 * (size_t)d.M);
+    cp((void **)&A->master_at, AB_MKEY, 4 * (size_t)d.M);
+    cp((void **)&A->spec_pointers, AB_SP, 4 * (size_t)d.nsp);
+    chk(hipStreamSynchronize(st), "sync");
+    return A;
+}
+
+void alpha_free_host(PGPUDTALPHA A)
+{
+    if (!A) return;
+    free(A->spectrum); free(A->tri_ro_rank); free(A->edge_t); free(A->edge_vopp); free(A->edge_ro_rank);
+    free(A->edge_nu_min_rank); free(A->edge_nu_max_rank); free(A->vert_nu_min_rank); free(A->vert_nu_max_rank);
+    free(A->master_simplex); free(A->master_st); free(A->master_at); free(A->spec_pointers);
+    free(A);
+}

@@ -1,0 +1,109 @@
+// kernels.h -- launchers shared between the HIP translation units and the host orchestrator.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+struct PbaBuffers {
+    int *smap;     // T*T site id per pixel (0x7fffffff = empty)
+    short *g;      // T*T nearest-site row per column
+    int *S;        // T*T band stacks
+    int *cnt;      // m2*T stack sizes
+    int *E;        // T*T compacted envelopes
+    int *ecnt;     // T envelope sizes
+    int *vor;      // T*T nearest site id
+};
+
+// Mesh state of one point set on the device.
+struct MeshDev {
+    int4 *tv;            // triangle vertices (x == -2: dead slot)
+    int4 *tn;            // neighbour encodings (t << 2 | j), -1 hull
+    int4 *remap;         // per slot: new encoding of each old edge after a modification
+    int *stamp;          // per slot: id of the last operation that modified it
+    unsigned long long *claim;  // per slot: vote word
+    int cap;             // slot capacity
+    int *ntri;           // device: slots in use
+    double2 *pos;        // current vertex positions [V]
+    const double2 *xy;   // true positions of the real points [n]
+    int *vt;             // vertex -> an incident triangle [V]
+    int *vstate;         // vertex state [V]
+    int *pix;            // vertex pixel (px << 16 | py) [V]
+    int n, V;            // real points, all vertices (real + frame)
+    double shx, shy, scale;
+    int margin;
+    int *status;         // device error bits
+    int *counters;       // device scratch counters [64]
+    int *modA, *modB;    // modified-triangle lists (capacity cap)
+    int *cand;           // candidate list (capacity 3*cap)
+    int *scratch;        // per-winner scratch arena
+    size_t scratch_cap;
+};
+
+// vertex states
+#define VS_IN_MESH 1       // in the mesh, position final
+#define VS_PENDING 2       // in the mesh at its snapped position
+#define VS_MISSING 3       // to be inserted
+#define VS_DUP 4           // exact duplicate of another point, dropped
+#define VS_REMOVED 5       // frame vertex removed
+
+// counters[] slots
+#define C_FLAG 0
+#define C_CAND 1
+#define C_MODN 2
+#define C_MODN2 3
+#define C_WIN 4
+#define C_SCR 5
+#define C_MISS 6
+#define C_PEND 7
+#define C_EXACT_ORIENT 8
+#define C_EXACT_INCIRCLE 9
+#define C_FLIPS 10
+#define C_INSERTED 11
+#define C_DUPS 12
+#define C_FRAME 13
+#define C_TESTS_INCIRCLE 14
+#define C_TESTS_ORIENT 15
+#define C_RECON_TRIS 16
+#define C_SHIFT_BAD 17
+#define C_DELETED 18
+#define C_ISLAND_SITES 19
+#define C_HSTART 20
+#define C_NCOUNTERS 64
+
+// pba.hip
+void launch_fill_i32(int *a, int v, size_t n, hipStream_t st);
+void launch_splat(const double2 *xy, int n, double shx, double shy, double inv, int margin, int logT,
+                  int *smap, int *pix, const int *fpix, int nf, hipStream_t st);
+void launch_pba(const PbaBuffers &b, int logT, int m2, int m3, hipStream_t st);
+void launch_islands(const int *vor, const int *pix, int logT, int *bad, int *flag, hipStream_t st);
+void launch_remove_bad(int *bad, const int *pix, int n, int logT, int *smap, hipStream_t st);
+
+// scan.hip
+size_t scan_temp_bytes(size_t n);
+void launch_exclusive_scan(const int *in, int *out, size_t n, void *temp, hipStream_t st);
+
+// mesh.hip
+void launch_recon(const int *vor, const int *pix, int logT, MeshDev &m, int *tile_cnt, int *tile_off,
+                  void *scan_tmp, hipStream_t st);
+void launch_adjacency(MeshDev &m, int *deg, int *off, int *fill, int *fan, void *scan_tmp, hipStream_t st);
+void launch_init_positions(MeshDev &m, hipStream_t st);
+void launch_shift_bulk(MeshDev &m, unsigned char *moved, int first, hipStream_t st);
+void launch_shift_commit(MeshDev &m, const unsigned char *moved, hipStream_t st);
+void launch_shift_greedy(MeshDev &m, int round, hipStream_t st);
+void launch_delete_round(MeshDev &m, unsigned int *mx, int round, int op, hipStream_t st);
+void launch_missing_init(MeshDev &m, const int *vor, int logT, int *mlist, int *loc, hipStream_t st);
+void launch_insert_round(MeshDev &m, int *mlist, int nmiss_cap, int *loc, int op, hipStream_t st);
+void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, hipStream_t st);
+void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st);
+void launch_flip_pass(MeshDev &m, const int *list_in, int cnt_in_slot, int *list_out, int cnt_out_slot,
+                      int prev_op, int op, hipStream_t st);
+void launch_frame_round(MeshDev &m, unsigned int *mx, int round, int *list_out, int cnt_out_slot, int op,
+                        hipStream_t st);
+void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, hipStream_t st);
+void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot, hipStream_t st);
+void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, int F, int *out9,
+                       hipStream_t st);
+int sort_pairs_u64(unsigned long long *keys_in, int *vals_in, unsigned long long *keys_out, int *vals_out,
+                   int n, int begin_bit, int end_bit, void *temp, size_t *temp_bytes, hipStream_t st);
+int sort_pairs_u32(unsigned int *keys_in, int *vals_in, unsigned int *keys_out, int *vals_out,
+                   int n, int begin_bit, int end_bit, void *temp, size_t *temp_bytes, hipStream_t st);

@@ -1,0 +1,1071 @@
+// mesh.hip -- triangle-mesh stages of the Delaunay pipeline on gfx950.
+//
+// Reference stages replaced here (paths relative to the reference root):
+//   reconstruction      Cuda/cudaReconstruction.cu:134-574 (row counting + CPU prefix + CPU hull
+//                       fix + CPU vertex fans).  Here: one pass over 2x2 pixel blocks with a
+//                       tile scan, triangles emitted counter-clockwise; the frame sites at the
+//                       texture border make the mesh boundary a convex rectangle, so no hull fix.
+//   adjacency           cpuVertexArray + kernelFindNextTriangles (:348-436): vertex fans built
+//                       on the device as a CSR (degree count, scan, fill), then edge matching.
+//   shifting            Cuda/cudaShifting.cu:117-599: bulk move of every vertex to its true
+//                       position with exact orientation checks, then greedy independent rounds,
+//                       then deletion (ear clipping of the star-shaped hole) of the rest.
+//   missing insertion   Cuda/cudaMissing.cu:125-808: stochastic visibility walk, 1->3 / 2->4
+//                       splits, one insertion per triangle per round by 64-bit vote words.
+//   flipping            Cuda/cudaFlipping.cu:227-685: Lawson flips with an FP64 filter and an
+//                       exact integer fallback on the device (no CPU pass), lifted-perturbation
+//                       tie-break for co-circular quadruples.
+//   boundary removal    Cuda/cudaBoundary.cu:106-608: distance-2 independent frame-vertex
+//                       deletions with a ccw-stack fill, then a hull convexification pass.
+// Links after a modification are repaired by one generic pass: a modified triangle keeps the
+// old external link of each outer edge (stored as -(e+2)); if that neighbour was modified in
+// the same operation the new location is read from its remap[] entry, otherwise the link is
+// kept and the neighbour's back-link is written.
+#include "dev_common.h"
+#include "kernels.h"
+
+#define GRID_STRIDE(i, n) for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+#define FIXED_GRID 2048
+
+GA_INLINE int ic(const int4 &a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+GA_INLINE void sc(int4 &a, int i, int v)
+{
+    if (i == 0) a.x = v;
+    else if (i == 1) a.y = v;
+    else a.z = v;
+}
+GA_INLINE int slot_of(const int4 &t, int v) { return t.x == v ? 0 : (t.y == v ? 1 : 2); }
+GA_INLINE int mark_old(int e) { return e < 0 ? -1 : -(e + 2); }
+
+GA_INLINE double2 snapped(const MeshDev &m, int v)
+{
+    int pp = m.pix[v];
+    int px = pp >> 16, py = pp & 0xffff;
+    return make_double2(m.shx + (double)(px - m.margin) * m.scale, m.shy + (double)(py - m.margin) * m.scale);
+}
+
+__device__ __noinline__ int orient_slow(double2 a, double2 b, double2 c, int *status, int *cnt)
+{
+    atomicAdd(cnt, 1);
+    int ovf = 0;
+    int s = orient_exact(a, b, c, &ovf);
+    if (ovf) atomicOr(status, GA_ERR_EXACT_OVERFLOW);
+    return s;
+}
+
+GA_INLINE int orient_dev(const MeshDev &m, double2 a, double2 b, double2 c)
+{
+    int s = orient_filter(a, b, c);
+    if (s != 0) return s;
+    return orient_slow(a, b, c, m.status, m.counters + C_EXACT_ORIENT);
+}
+
+__device__ __noinline__ int incircle_slow(double2 a, int ia, double2 b, int ib, double2 c, int ic_,
+                                          double2 d, int id, int *status, int *cnt)
+{
+    atomicAdd(cnt, 1);
+    int ovf = 0;
+    int s = incircle_exact_sos(a, ia, b, ib, c, ic_, d, id, &ovf);
+    if (ovf) atomicOr(status, GA_ERR_EXACT_OVERFLOW);
+    return s;
+}
+
+GA_INLINE int incircle_dev(const MeshDev &m, int ia, int ib, int ic_, int id)
+{
+    double2 a = m.pos[ia], b = m.pos[ib], c = m.pos[ic_], d = m.pos[id];
+    int s = incircle_filter(a, b, c, d);
+    if (s != 0) return s;
+    return incircle_slow(a, ia, b, ib, c, ic_, d, id, m.status, m.counters + C_EXACT_INCIRCLE);
+}
+
+// ---- reconstruction --------------------------------------------------------------------
+// Block corners counter-clockwise: a=(x,y) b=(x+1,y) c=(x+1,y+1) d=(x,y+1).
+GA_INLINE int recon_count(int a, int b, int c, int d)
+{
+    if (a == c || b == d) return 0;
+    int neq = (a == b) + (b == c) + (c == d) + (d == a);
+    if (neq >= 2) return 0;
+    return neq == 1 ? 1 : 2;
+}
+
+GA_INLINE long long iorient(int pa, int pb, int pc)
+{
+    long long ax = pa >> 16, ay = pa & 0xffff, bx = pb >> 16, by = pb & 0xffff, cx = pc >> 16, cy = pc & 0xffff;
+    return (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
+}
+
+__global__ __launch_bounds__(256) void k_recon_count(const int *__restrict__ vor, int logT, int nyb,
+                                                     int *__restrict__ tile_cnt)
+{
+    int T = 1 << logT;
+    int tile = blockIdx.x;
+    int x = tile / nyb, y = (tile % nyb) * 256 + threadIdx.x;
+    int c = 0;
+    if (x < T - 1 && y < T - 1) {
+        size_t i0 = ((size_t)x << logT) + y, i1 = i0 + T;
+        c = recon_count(vor[i0], vor[i1], vor[i1 + 1], vor[i0 + 1]);
+    }
+    __shared__ int ws[4];
+    int s = c;
+    for (int d = 32; d > 0; d >>= 1) s += __shfl_down(s, d, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_cnt[tile] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ __launch_bounds__(256) void k_recon_emit(const int *__restrict__ vor, const int *__restrict__ pix,
+                                                    int logT, int nyb, const int *__restrict__ tile_off,
+                                                    int4 *__restrict__ tv, int4 *__restrict__ tn,
+                                                    int *__restrict__ status, int cap)
+{
+    int T = 1 << logT;
+    int tile = blockIdx.x;
+    int x = tile / nyb, y = (tile % nyb) * 256 + threadIdx.x;
+    int c = 0, a = 0, b = 0, cc = 0, d = 0;
+    if (x < T - 1 && y < T - 1) {
+        size_t i0 = ((size_t)x << logT) + y, i1 = i0 + T;
+        a = vor[i0]; b = vor[i1]; cc = vor[i1 + 1]; d = vor[i0 + 1];
+        c = recon_count(a, b, cc, d);
+    }
+    // block exclusive scan of c
+    __shared__ int ws[4];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = c;
+    for (int k = 1; k < 64; k <<= 1) {
+        int t = __shfl_up(inc, k, 64);
+        if (lane >= k) inc += t;
+    }
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    int off = tile_off[tile];
+    for (int i = 0; i < w; i++) off += ws[i];
+    off += inc - c;
+    if (c == 0) return;
+    if (off + c > cap) { atomicOr(status, GA_ERR_CAPACITY); return; }
+    int4 none = make_int4(-1, -1, -1, 0);
+    if (c == 1) {
+        int t0, t1, t2;
+        if (a == b) { t0 = a; t1 = cc; t2 = d; }
+        else if (b == cc) { t0 = a; t1 = b; t2 = d; }
+        else { t0 = a; t1 = b; t2 = cc; }   // c == d or d == a
+        if (iorient(pix[t0], pix[t1], pix[t2]) <= 0) atomicOr(status, GA_ERR_RECON);
+        tv[off] = make_int4(t0, t1, t2, 0);
+        tn[off] = none;
+    } else {
+        int pa = pix[a], pb = pix[b], pc = pix[cc], pd = pix[d];
+        if (iorient(pa, pb, pc) > 0 && iorient(pa, pc, pd) > 0) {
+            tv[off] = make_int4(a, b, cc, 0);
+            tv[off + 1] = make_int4(a, cc, d, 0);
+        } else if (iorient(pb, pc, pd) > 0 && iorient(pb, pd, pa) > 0) {
+            tv[off] = make_int4(b, cc, d, 0);
+            tv[off + 1] = make_int4(b, d, a, 0);
+        } else {
+            atomicOr(status, GA_ERR_RECON);
+            tv[off] = make_int4(a, b, cc, 0);
+            tv[off + 1] = make_int4(a, cc, d, 0);
+        }
+        tn[off] = none;
+        tn[off + 1] = none;
+    }
+}
+
+__global__ void k_set_ntri(const int *tile_cnt, const int *tile_off, int ntiles, int *ntri, int cap)
+{
+    int F = tile_off[ntiles - 1] + tile_cnt[ntiles - 1];
+    *ntri = F > cap ? cap : F;
+}
+
+void launch_recon(const int *vor, const int *pix, int logT, MeshDev &m, int *tile_cnt, int *tile_off,
+                  void *scan_tmp, hipStream_t st)
+{
+    int T = 1 << logT;
+    int nyb = (T - 1 + 255) / 256;
+    int ntiles = (T - 1) * nyb;
+    hipLaunchKernelGGL(k_recon_count, dim3(ntiles), dim3(256), 0, st, vor, logT, nyb, tile_cnt);
+    launch_exclusive_scan(tile_cnt, tile_off, ntiles, scan_tmp, st);
+    hipLaunchKernelGGL(k_recon_emit, dim3(ntiles), dim3(256), 0, st, vor, pix, logT, nyb, tile_off, m.tv, m.tn,
+                       m.status, m.cap);
+    hipLaunchKernelGGL(k_set_ntri, dim3(1), dim3(1), 0, st, tile_cnt, tile_off, ntiles, m.ntri, m.cap);
+}
+
+// ---- adjacency from scratch ----------------------------------------------------------------
+__global__ void k_deg(const int4 *__restrict__ tv, const int *__restrict__ ntri, int *__restrict__ deg)
+{
+    int n = *ntri;
+    GRID_STRIDE(t, n) {
+        int4 v = tv[t];
+        if (v.x == GA_DEAD) continue;
+        atomicAdd(&deg[v.x], 1);
+        atomicAdd(&deg[v.y], 1);
+        atomicAdd(&deg[v.z], 1);
+    }
+}
+
+__global__ void k_fanfill(const int4 *__restrict__ tv, const int *__restrict__ ntri, const int *__restrict__ off,
+                          int *__restrict__ fill, int *__restrict__ fan, int *__restrict__ vt)
+{
+    int n = *ntri;
+    GRID_STRIDE(t, n) {
+        int4 v = tv[t];
+        if (v.x == GA_DEAD) continue;
+        for (int i = 0; i < 3; i++) {
+            int a = ic(v, i);
+            int k = atomicAdd(&fill[a], 1);
+            fan[off[a] + k] = enc(t, i);
+            if (k == 0) vt[a] = t;
+        }
+    }
+}
+
+__global__ void k_match(const int4 *__restrict__ tv, const int *__restrict__ ntri, const int *__restrict__ off,
+                        const int *__restrict__ fan, int4 *__restrict__ tn)
+{
+    int n = *ntri;
+    GRID_STRIDE(t, n) {
+        int4 v = tv[t];
+        if (v.x == GA_DEAD) continue;
+        int4 r = make_int4(-1, -1, -1, 0);
+        for (int i = 0; i < 3; i++) {
+            int a = ic(v, p1(i)), b = ic(v, m1(i));
+            for (int k = off[a]; k < off[a + 1]; k++) {
+                int e = fan[k];
+                int u = enc_t(e), ia = enc_j(e);
+                if (u == t) continue;
+                int4 w = tv[u];
+                if (ic(w, m1(ia)) == b) { sc(r, i, enc(u, p1(ia))); break; }
+            }
+        }
+        tn[t] = r;
+    }
+}
+
+void launch_adjacency(MeshDev &m, int *deg, int *off, int *fill, int *fan, void *scan_tmp, hipStream_t st)
+{
+    hipMemsetAsync(deg, 0, sizeof(int) * (m.V + 1), st);
+    hipMemsetAsync(fill, 0, sizeof(int) * (m.V + 1), st);
+    hipLaunchKernelGGL(k_deg, dim3(FIXED_GRID), dim3(256), 0, st, m.tv, m.ntri, deg);
+    launch_exclusive_scan(deg, off, m.V + 1, scan_tmp, st);
+    hipLaunchKernelGGL(k_fanfill, dim3(FIXED_GRID), dim3(256), 0, st, m.tv, m.ntri, off, fill, fan, m.vt);
+    hipLaunchKernelGGL(k_match, dim3(FIXED_GRID), dim3(256), 0, st, m.tv, m.ntri, off, fan, m.tn);
+}
+
+// ---- positions / states after reconstruction ------------------------------------------------
+// deg (from the adjacency build, still valid) says which vertices are in the mesh.
+__global__ void k_init_positions(MeshDev m, const int *__restrict__ deg)
+{
+    GRID_STRIDE(v, m.V) {
+        int in = deg[v] > 0;
+        if (v >= m.n) {
+            m.vstate[v] = VS_IN_MESH;
+            m.pos[v] = snapped(m, v);
+        } else if (in) {
+            m.vstate[v] = VS_PENDING;
+            m.pos[v] = snapped(m, v);
+        } else {
+            m.vstate[v] = VS_MISSING;
+            m.pos[v] = m.xy[v];
+            m.vt[v] = -1;
+        }
+    }
+}
+
+static int *g_deg_for_init = nullptr;
+void set_deg_for_init(int *deg) { g_deg_for_init = deg; }
+void launch_init_positions(MeshDev &m, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_init_positions, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)g_deg_for_init);
+}
+
+// ---- shifting ----------------------------------------------------------------------------------
+GA_INLINE double2 cand_pos(const MeshDev &m, const unsigned char *moved, int v)
+{
+    if (v < m.n && moved[v]) return m.xy[v];
+    return m.pos[v];
+}
+
+// first == 1: initialise moved[] for pending vertices.  Otherwise: every triangle that is not
+// counter-clockwise with the current choice reverts its moved vertices.
+__global__ void k_shift_init(MeshDev m, unsigned char *moved)
+{
+    GRID_STRIDE(v, m.n) moved[v] = (m.vstate[v] == VS_PENDING) ? 1 : 0;
+}
+
+__global__ void k_shift_bulk(MeshDev m, unsigned char *moved)
+{
+    int n = *m.ntri;
+    GRID_STRIDE(t, n) {
+        int4 v = m.tv[t];
+        if (v.x == GA_DEAD) continue;
+        bool any = (v.x < m.n && moved[v.x]) || (v.y < m.n && moved[v.y]) || (v.z < m.n && moved[v.z]);
+        if (!any) continue;
+        double2 a = cand_pos(m, moved, v.x), b = cand_pos(m, moved, v.y), c = cand_pos(m, moved, v.z);
+        if (orient_dev(m, a, b, c) <= 0) {
+            if (v.x < m.n) moved[v.x] = 0;
+            if (v.y < m.n) moved[v.y] = 0;
+            if (v.z < m.n) moved[v.z] = 0;
+            m.counters[C_FLAG] = 1;
+        }
+    }
+}
+
+void launch_shift_bulk(MeshDev &m, unsigned char *moved, int first, hipStream_t st)
+{
+    if (first) hipLaunchKernelGGL(k_shift_init, dim3(FIXED_GRID), dim3(256), 0, st, m, moved);
+    hipLaunchKernelGGL(k_shift_bulk, dim3(FIXED_GRID), dim3(256), 0, st, m, moved);
+}
+
+__global__ void k_shift_commit(MeshDev m, const unsigned char *moved)
+{
+    GRID_STRIDE(v, m.n) {
+        if (m.vstate[v] == VS_PENDING && moved[v]) {
+            m.pos[v] = m.xy[v];
+            m.vstate[v] = VS_IN_MESH;
+        } else if (m.vstate[v] == VS_PENDING) {
+            atomicAdd(&m.counters[C_PEND], 1);
+        }
+    }
+}
+
+void launch_shift_commit(MeshDev &m, const unsigned char *moved, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_shift_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, moved);
+}
+
+GA_INLINE uint32_t vprio(int v, int round) { return hash32((uint32_t)v * 0x9E3779B1u ^ hash32((uint32_t)round + 0x1234567u)); }
+
+// Greedy round: a pending vertex whose pending neighbours all have lower priority moves to
+// its true position when all its star triangles stay counter-clockwise.  Decisions only;
+// the commit kernel applies them (so every decision sees the same snapshot).
+__global__ void k_shift_greedy(MeshDev m, int round, int *decide)
+{
+    GRID_STRIDE(v, m.n) {
+        decide[v] = 0;
+        if (m.vstate[v] != VS_PENDING) continue;
+        uint32_t p = vprio(v, round);
+        int t0 = m.vt[v];
+        int t = t0, i = slot_of(m.tv[t0], v);
+        bool ok = true, winner = true;
+        double2 q = m.xy[v];
+        int guard = 0;
+        do {
+            int4 tt = m.tv[t];
+            int a = ic(tt, p1(i)), b = ic(tt, m1(i));
+            if (a < m.n && m.vstate[a] == VS_PENDING) {
+                uint32_t pa = vprio(a, round);
+                if (pa > p || (pa == p && a > v)) { winner = false; break; }
+            }
+            if (ok && orient_dev(m, q, m.pos[a], m.pos[b]) <= 0) ok = false;
+            int e = ic(m.tn[t], p1(i));
+            if (e < 0) { atomicOr(m.status, GA_ERR_WALK); winner = false; break; }
+            t = enc_t(e);
+            i = p1(enc_j(e));
+            if (++guard > 100000) { atomicOr(m.status, GA_ERR_WALK); winner = false; break; }
+        } while (t != t0);
+        if (winner && ok) decide[v] = 1;
+    }
+}
+
+__global__ void k_shift_greedy_commit(MeshDev m, const int *decide)
+{
+    GRID_STRIDE(v, m.n) {
+        if (decide[v]) {
+            m.pos[v] = m.xy[v];
+            m.vstate[v] = VS_IN_MESH;
+            atomicAdd(&m.counters[C_FLAG], 1);
+        }
+    }
+}
+
+static int *g_decide = nullptr;
+void set_decide_buffer(int *d) { g_decide = d; }
+void launch_shift_greedy(MeshDev &m, int round, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_shift_greedy, dim3(FIXED_GRID), dim3(256), 0, st, m, round, g_decide);
+    hipLaunchKernelGGL(k_shift_greedy_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)g_decide);
+}
+
+// ---- deletion of vertices that cannot move (star-shaped hole, ear clipping) --------------------
+GA_INLINE unsigned long long prio64(int v, int round)
+{
+    return ((unsigned long long)vprio(v, round) << 32) | (unsigned)v;
+}
+
+// distance-2 independence: every vertex records the largest priority in its closed neighbourhood
+__global__ void k_del_prio(MeshDev m, unsigned long long *mx, int round)
+{
+    GRID_STRIDE(v, m.n) {
+        if (m.vstate[v] != VS_PENDING) continue;
+        unsigned long long p = prio64(v, round);
+        atomicMax(&mx[v], p);
+        int t0 = m.vt[v], t = t0, i = slot_of(m.tv[t0], v), guard = 0;
+        do {
+            int a = ic(m.tv[t], p1(i));
+            atomicMax(&mx[a], p);
+            int e = ic(m.tn[t], p1(i));
+            if (e < 0) { atomicOr(m.status, GA_ERR_WALK); break; }
+            t = enc_t(e);
+            i = p1(enc_j(e));
+            if (++guard > 100000) break;
+        } while (t != t0);
+    }
+}
+
+// link two (triangle, slot) sides of one edge; X < 0 means hull
+// single-word stores: other threads may write other slots of the same neighbour record
+GA_INLINE void set_link(const MeshDev &m, int T, int k, int v) { ((int *)(m.tn + T))[k] = v; }
+GA_INLINE void bond(const MeshDev &m, int T, int k, int X)
+{
+    set_link(m, T, k, X);
+    if (X >= 0) set_link(m, enc_t(X), enc_j(X), enc(T, k));
+}
+
+__global__ void k_del_apply(MeshDev m, const unsigned long long *mx, int round, int op, int *list_out)
+{
+    GRID_STRIDE(v, m.n) {
+        if (m.vstate[v] != VS_PENDING) continue;
+        unsigned long long p = prio64(v, round);
+        if (mx[v] != p) continue;
+        int t0 = m.vt[v], t = t0, i = slot_of(m.tv[t0], v), d = 0, guard = 0;
+        bool win = true;
+        do {
+            int a = ic(m.tv[t], p1(i));
+            if (mx[a] != p) { win = false; break; }
+            int e = ic(m.tn[t], p1(i));
+            if (e < 0) { atomicOr(m.status, GA_ERR_WALK); win = false; break; }
+            t = enc_t(e);
+            i = p1(enc_j(e));
+            d++;
+            if (++guard > 100000) { win = false; break; }
+        } while (t != t0);
+        if (!win) continue;
+        // scratch: poly[d], plink[d], slots[d]
+        int base = atomicAdd(&m.counters[C_SCR], 3 * d);
+        if ((size_t)(base + 3 * d) > m.scratch_cap) { atomicOr(m.status, GA_ERR_SCRATCH); continue; }
+        int *poly = m.scratch + base, *plink = poly + d, *slots = plink + d;
+        t = t0; i = slot_of(m.tv[t0], v);
+        for (int k = 0; k < d; k++) {
+            int4 tt = m.tv[t];
+            poly[k] = ic(tt, p1(i));                 // triangle (v, l_k, l_k+1)
+            plink[k] = ic(m.tn[t], i);               // outer side of edge (l_k, l_k+1)
+            slots[k] = t;
+            int e = ic(m.tn[t], p1(i));
+            t = enc_t(e);
+            i = p1(enc_j(e));
+        }
+        double2 K = m.pos[v];
+        int msz = d, used = 0;
+        while (msz > 3) {
+            int pick = -1;
+            for (int k = 0; k < msz && pick < 0; k++) {
+                int kp = k == 0 ? msz - 1 : k - 1, kn = k == msz - 1 ? 0 : k + 1;
+                double2 A = m.pos[poly[kp]], B = m.pos[poly[k]], C = m.pos[poly[kn]];
+                if (orient_dev(m, A, B, C) <= 0) continue;
+                bool inside = orient_dev(m, A, B, K) > 0 && orient_dev(m, B, C, K) > 0 && orient_dev(m, C, A, K) > 0;
+                if (inside) continue;
+                pick = k;
+            }
+            if (pick < 0) { atomicOr(m.status, GA_ERR_PATCH); break; }
+            int kp = pick == 0 ? msz - 1 : pick - 1, kn = pick == msz - 1 ? 0 : pick + 1;
+            int T = slots[used++];
+            int A = poly[kp], B = poly[pick], C = poly[kn];
+            m.tv[T] = make_int4(A, B, C, 0);
+            m.tn[T] = make_int4(-1, -1, -1, 0);
+            // slot 2 (opp C) = edge (A,B) -> plink[kp]; slot 0 (opp A) = edge (B,C) -> plink[pick]
+            bond(m, T, 2, plink[kp]);
+            bond(m, T, 0, plink[pick]);
+            m.vt[A] = T; m.vt[B] = T; m.vt[C] = T;
+            m.stamp[T] = op;
+            list_out[atomicAdd(&m.counters[C_MODN], 1)] = T;
+            // new polygon edge (A, C): its other side is slot 1 (opp B) of T
+            plink[kp] = enc(T, 1);
+            for (int k = pick; k < msz - 1; k++) { poly[k] = poly[k + 1]; plink[k] = plink[k + 1]; }
+            msz--;
+        }
+        if (msz == 3) {
+            int T = slots[used++];
+            m.tv[T] = make_int4(poly[0], poly[1], poly[2], 0);
+            m.tn[T] = make_int4(-1, -1, -1, 0);
+            bond(m, T, 2, plink[0]);   // (p0,p1)
+            bond(m, T, 0, plink[1]);   // (p1,p2)
+            bond(m, T, 1, plink[2]);   // (p2,p0)
+            m.vt[poly[0]] = T; m.vt[poly[1]] = T; m.vt[poly[2]] = T;
+            m.stamp[T] = op;
+            list_out[atomicAdd(&m.counters[C_MODN], 1)] = T;
+        }
+        for (int k = used; k < d; k++) m.tv[slots[k]] = make_int4(GA_DEAD, GA_DEAD, GA_DEAD, 0);
+        m.vstate[v] = VS_MISSING;
+        m.vt[v] = -1;
+        m.pos[v] = m.xy[v];
+        atomicAdd(&m.counters[C_DELETED], 1);
+    }
+}
+
+static int *g_del_list = nullptr;
+void set_del_list(int *l) { g_del_list = l; }
+void launch_delete_round(MeshDev &m, unsigned int *mxraw, int round, int op, hipStream_t st)
+{
+    unsigned long long *mx = (unsigned long long *)mxraw;
+    hipMemsetAsync(mx, 0, sizeof(unsigned long long) * m.V, st);
+    hipLaunchKernelGGL(k_del_prio, dim3(FIXED_GRID), dim3(256), 0, st, m, mx, round);
+    hipLaunchKernelGGL(k_del_apply, dim3(FIXED_GRID), dim3(256), 0, st, m, (const unsigned long long *)mx, round, op,
+                       g_del_list);
+}
+
+// ---- missing-site insertion ------------------------------------------------------------------
+__global__ void k_missing_init(MeshDev m, const int *__restrict__ vor, int logT, int *mlist, int *loc)
+{
+    GRID_STRIDE(v, m.n) {
+        if (m.vstate[v] != VS_MISSING) continue;
+        int k = atomicAdd(&m.counters[C_MISS], 1);
+        mlist[k] = v;
+        int pp = m.pix[v];
+        int s = vor[((size_t)(pp >> 16) << logT) + (pp & 0xffff)];
+        int t = -1;
+        if (s >= 0 && m.vstate[s] == VS_IN_MESH) t = m.vt[s];
+        if (t < 0) t = m.vt[m.n];
+        loc[k] = t;
+    }
+}
+
+void launch_missing_init(MeshDev &m, const int *vor, int logT, int *mlist, int *loc, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_missing_init, dim3(FIXED_GRID), dim3(256), 0, st, m, vor, logT, mlist, loc);
+}
+
+// located: lt = (type << 2) | edge ; type 1 inside, 2 on edge, 3 duplicate ; -1 not located
+__global__ void k_locate(MeshDev m, const int *mlist, int *loc, int *lt, int op)
+{
+    int nm = m.counters[C_MISS];
+    GRID_STRIDE(k, nm) {
+        int v = mlist[k];
+        lt[k] = -1;
+        if (m.vstate[v] != VS_MISSING) continue;
+        double2 p = m.xy[v];
+        int t = loc[k];
+        if (t < 0 || m.tv[t].x == GA_DEAD) t = m.vt[m.n];
+        uint32_t rng = hash32((uint32_t)v ^ ((uint32_t)op * 0x85ebca6bu));
+        int located = 0, o[3];
+        for (int step = 0; step < 4096; step++) {
+            int4 tt = m.tv[t];
+            rng = rng * 1664525u + 1013904223u;
+            int r = (rng >> 16) % 3;
+            int moved = 0;
+            for (int e = 0; e < 3; e++) {
+                int i = (r + e) % 3;
+                o[i] = orient_dev(m, m.pos[ic(tt, p1(i))], m.pos[ic(tt, m1(i))], p);
+                if (o[i] < 0) {
+                    int nb = ic(m.tn[t], i);
+                    if (nb < 0) { atomicOr(m.status, GA_ERR_WALK); moved = -1; break; }
+                    t = enc_t(nb);
+                    moved = 1;
+                    break;
+                }
+            }
+            if (moved == 0) { located = 1; break; }
+            if (moved < 0) break;
+        }
+        loc[k] = t;
+        if (!located) continue;
+        int z = (o[0] == 0) + (o[1] == 0) + (o[2] == 0);
+        int4 tt = m.tv[t];
+        if (z >= 2) {
+            int kk = o[0] != 0 ? 0 : (o[1] != 0 ? 1 : 2);
+            int w = ic(tt, kk);
+            m.vstate[v] = VS_DUP;
+            m.vt[v] = -1 - w;      // duplicate of w
+            atomicAdd(&m.counters[C_DUPS], 1);
+            continue;
+        }
+        unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)v);
+        if (z == 0) {
+            atomicMax(&m.claim[t], key);
+            lt[k] = (1 << 2);
+        } else {
+            int i = o[0] == 0 ? 0 : (o[1] == 0 ? 1 : 2);
+            int nb = ic(m.tn[t], i);
+            if (nb < 0) { atomicOr(m.status, GA_ERR_WALK); continue; }
+            atomicMax(&m.claim[t], key);
+            atomicMax(&m.claim[enc_t(nb)], key);
+            lt[k] = (2 << 2) | i;
+        }
+    }
+}
+
+__global__ void k_insert(MeshDev m, const int *mlist, const int *loc, const int *lt, int op, int *list_out)
+{
+    int nm = m.counters[C_MISS];
+    GRID_STRIDE(k, nm) {
+        int L = lt[k];
+        if (L < 0) continue;
+        int v = mlist[k];
+        int t = loc[k];
+        unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)v);
+        if (m.claim[t] != key) continue;
+        int type = L >> 2, i = L & 3;
+        int4 tt = m.tv[t], nt = m.tn[t];
+        if (type == 1) {
+            int s = atomicAdd(m.ntri, 2);
+            if (s + 2 > m.cap) { atomicOr(m.status, GA_ERR_CAPACITY); continue; }
+            int s1 = s, s2 = s + 1;
+            int a = tt.x, b = tt.y, c = tt.z;
+            m.tv[t] = make_int4(a, b, v, 0);
+            m.tv[s1] = make_int4(b, c, v, 0);
+            m.tv[s2] = make_int4(c, a, v, 0);
+            m.tn[t] = make_int4(enc(s1, 1), enc(s2, 0), mark_old(nt.z), 0);
+            m.tn[s1] = make_int4(enc(s2, 1), enc(t, 0), mark_old(nt.x), 0);
+            m.tn[s2] = make_int4(enc(t, 1), enc(s1, 0), mark_old(nt.y), 0);
+            m.remap[t] = make_int4(enc(s1, 2), enc(s2, 2), enc(t, 2), 0);
+            m.stamp[t] = op; m.stamp[s1] = op; m.stamp[s2] = op;
+            int q = atomicAdd(&m.counters[C_MODN], 3);
+            list_out[q] = t; list_out[q + 1] = s1; list_out[q + 2] = s2;
+            m.vt[v] = t; m.vt[c] = s1;
+        } else {
+            int nb = ic(nt, i);
+            int u = enc_t(nb), j = enc_j(nb);
+            if (m.claim[u] != key) continue;
+            int s = atomicAdd(m.ntri, 2);
+            if (s + 2 > m.cap) { atomicOr(m.status, GA_ERR_CAPACITY); continue; }
+            int s1 = s, s2 = s + 1;
+            int4 ut = m.tv[u], un = m.tn[u];
+            int c = ic(tt, i), a = ic(tt, p1(i)), b = ic(tt, m1(i));
+            int d = ic(ut, j);
+            int o_bc = ic(nt, p1(i)), o_ca = ic(nt, m1(i));
+            int o_ad = ic(un, p1(j)), o_db = ic(un, m1(j));
+            m.tv[t] = make_int4(c, a, v, 0);
+            m.tv[s1] = make_int4(c, v, b, 0);
+            m.tv[u] = make_int4(d, b, v, 0);
+            m.tv[s2] = make_int4(d, v, a, 0);
+            m.tn[t] = make_int4(enc(s2, 0), enc(s1, 2), mark_old(o_ca), 0);
+            m.tn[s1] = make_int4(enc(u, 0), mark_old(o_bc), enc(t, 1), 0);
+            m.tn[u] = make_int4(enc(s1, 0), enc(s2, 2), mark_old(o_db), 0);
+            m.tn[s2] = make_int4(enc(t, 0), mark_old(o_ad), enc(u, 1), 0);
+            int4 rt = make_int4(0, 0, 0, 0), ru = make_int4(0, 0, 0, 0);
+            sc(rt, i, -1); sc(rt, p1(i), enc(s1, 1)); sc(rt, m1(i), enc(t, 2));
+            sc(ru, j, -1); sc(ru, p1(j), enc(s2, 1)); sc(ru, m1(j), enc(u, 2));
+            m.remap[t] = rt; m.remap[u] = ru;
+            m.stamp[t] = op; m.stamp[u] = op; m.stamp[s1] = op; m.stamp[s2] = op;
+            int q = atomicAdd(&m.counters[C_MODN], 4);
+            list_out[q] = t; list_out[q + 1] = s1; list_out[q + 2] = u; list_out[q + 3] = s2;
+            m.vt[v] = t; m.vt[a] = t; m.vt[c] = t; m.vt[b] = u; m.vt[d] = u;
+        }
+        m.vstate[v] = VS_IN_MESH;
+        m.pos[v] = m.xy[v];
+        atomicAdd(&m.counters[C_INSERTED], 1);
+    }
+}
+
+static int *g_lt = nullptr;
+void set_lt_buffer(int *b) { g_lt = b; }
+void launch_insert_round(MeshDev &m, int *mlist, int nmiss_cap, int *loc, int op, hipStream_t st)
+{
+    (void)nmiss_cap;
+    hipLaunchKernelGGL(k_locate, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)mlist, loc, g_lt, op);
+    hipLaunchKernelGGL(k_insert, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)mlist, (const int *)loc,
+                       (const int *)g_lt, op, m.modA);
+}
+
+// ---- generic link repair -------------------------------------------------------------------
+__global__ void k_relink(MeshDev m, const int *list, int cnt_slot, int op)
+{
+    int n = m.counters[cnt_slot];
+    GRID_STRIDE(q, n) {
+        int t = list[q];
+        int4 r = m.tn[t];
+        bool ch = false;
+        for (int k = 0; k < 3; k++) {
+            int e = ic(r, k);
+            if (e > -2) continue;
+            int old = -e - 2;
+            int B = enc_t(old), jb = enc_j(old);
+            if (m.stamp[B] == op) {
+                sc(r, k, ic(m.remap[B], jb));
+            } else {
+                sc(r, k, old);
+                set_link(m, B, jb, enc(t, k));
+            }
+            ch = true;
+        }
+        if (ch) m.tn[t] = r;
+    }
+}
+
+void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_relink, dim3(FIXED_GRID), dim3(256), 0, st, m, modlist, cnt_slot, op);
+}
+
+// ---- flipping ---------------------------------------------------------------------------------
+__global__ void k_flip_all(MeshDev m, int op, int *list, int cnt_slot)
+{
+    int n = *m.ntri;
+    GRID_STRIDE(t, n) {
+        if (m.tv[t].x == GA_DEAD) continue;
+        m.stamp[t] = op;
+        list[atomicAdd(&m.counters[cnt_slot], 1)] = t;
+    }
+}
+
+void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_flip_all, dim3(FIXED_GRID), dim3(256), 0, st, m, op, list, C_MODN);
+}
+
+__global__ void k_flip_test(MeshDev m, const int *list, int cnt_slot, int prev_op, int op)
+{
+    int n = m.counters[cnt_slot];
+    GRID_STRIDE(q, n) {
+        int t = list[q];
+        int4 tt = m.tv[t];
+        if (tt.x == GA_DEAD) continue;
+        int4 nn = m.tn[t];
+        for (int i = 0; i < 3; i++) {
+            int e = ic(nn, i);
+            if (e < 0) continue;
+            int u = enc_t(e), j = enc_j(e);
+            if (m.stamp[u] == prev_op && u < t) continue;
+            int d = ic(m.tv[u], j);
+            atomicAdd(&m.counters[C_TESTS_INCIRCLE], 1);
+            if (incircle_dev(m, tt.x, tt.y, tt.z, d) > 0) {
+                unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)(t * 3 + i));
+                atomicMax(&m.claim[t], key);
+                atomicMax(&m.claim[u], key);
+                m.cand[atomicAdd(&m.counters[C_CAND], 1)] = t * 3 + i;
+            }
+        }
+    }
+}
+
+__global__ void k_flip_commit(MeshDev m, int op, int *list_out, int cnt_out_slot)
+{
+    int n = m.counters[C_CAND];
+    GRID_STRIDE(q, n) {
+        int c = m.cand[q];
+        int t = c / 3, i = c % 3;
+        unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)c);
+        if (m.claim[t] != key) continue;
+        int4 tt = m.tv[t], nt = m.tn[t];
+        int e = ic(nt, i);
+        int u = enc_t(e), j = enc_j(e);
+        if (m.claim[u] != key) continue;
+        int4 ut = m.tv[u], un = m.tn[u];
+        int cc = ic(tt, i), x = ic(tt, p1(i)), y = ic(tt, m1(i));
+        int d = ic(ut, j);
+        int nt1 = ic(nt, p1(i)), nt2 = ic(nt, m1(i));   // (y,c) , (c,x)
+        int nu1 = ic(un, p1(j)), nu2 = ic(un, m1(j));   // (x,d) , (d,y)
+        m.tv[t] = make_int4(cc, x, d, 0);
+        m.tv[u] = make_int4(d, y, cc, 0);
+        m.tn[t] = make_int4(mark_old(nu1), enc(u, 1), mark_old(nt2), 0);
+        m.tn[u] = make_int4(mark_old(nt1), enc(t, 1), mark_old(nu2), 0);
+        int4 rt = make_int4(0, 0, 0, 0), ru = make_int4(0, 0, 0, 0);
+        sc(rt, i, -1); sc(rt, p1(i), enc(u, 0)); sc(rt, m1(i), enc(t, 2));
+        sc(ru, j, -1); sc(ru, p1(j), enc(t, 0)); sc(ru, m1(j), enc(u, 2));
+        m.remap[t] = rt; m.remap[u] = ru;
+        m.stamp[t] = op; m.stamp[u] = op;
+        int qq = atomicAdd(&m.counters[cnt_out_slot], 2);
+        list_out[qq] = t; list_out[qq + 1] = u;
+        m.vt[x] = t; m.vt[cc] = t; m.vt[y] = u; m.vt[d] = u;
+        atomicAdd(&m.counters[C_FLIPS], 1);
+    }
+}
+
+void launch_flip_pass(MeshDev &m, const int *list_in, int cnt_in_slot, int *list_out, int cnt_out_slot,
+                      int prev_op, int op, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_flip_test, dim3(FIXED_GRID), dim3(256), 0, st, m, list_in, cnt_in_slot, prev_op, op);
+    hipLaunchKernelGGL(k_flip_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, op, list_out, cnt_out_slot);
+    hipLaunchKernelGGL(k_relink, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)list_out, cnt_out_slot, op);
+}
+
+// ---- frame removal ---------------------------------------------------------------------------
+// fan of a hull vertex f: first triangle whose cw side (edge (f, l0)) is a hull edge
+GA_INLINE int fan_first(const MeshDev &m, int f, int *slot)
+{
+    int t = m.vt[f], i = slot_of(m.tv[t], f), guard = 0;
+    while (true) {
+        int e = ic(m.tn[t], m1(i));
+        if (e < 0) break;
+        t = enc_t(e);
+        i = slot_of(m.tv[t], f);
+        if (++guard > 1000000) { atomicOr(m.status, GA_ERR_WALK); break; }
+    }
+    *slot = i;
+    return t;
+}
+
+__global__ void k_frame_prio(MeshDev m, unsigned long long *mx, int round)
+{
+    int nf = m.V - m.n;
+    GRID_STRIDE(k, nf) {
+        int f = m.n + k;
+        if (m.vstate[f] != VS_IN_MESH) continue;
+        unsigned long long p = prio64(f, round);
+        atomicMax(&mx[f], p);
+        int i, t = fan_first(m, f, &i);
+        atomicMax(&mx[ic(m.tv[t], p1(i))], p);
+        for (int guard = 0; guard < 1000000; guard++) {
+            atomicMax(&mx[ic(m.tv[t], m1(i))], p);
+            int e = ic(m.tn[t], p1(i));
+            if (e < 0) break;
+            t = enc_t(e);
+            i = slot_of(m.tv[t], f);
+        }
+    }
+}
+
+__global__ void k_frame_remove(MeshDev m, const unsigned long long *mx, int round, int op, int *list_out,
+                               int cnt_out_slot)
+{
+    int nf = m.V - m.n;
+    GRID_STRIDE(k, nf) {
+        int f = m.n + k;
+        if (m.vstate[f] != VS_IN_MESH) continue;
+        unsigned long long p = prio64(f, round);
+        if (mx[f] != p) continue;
+        int i0, t0 = fan_first(m, f, &i0);
+        int t = t0, i = i0, kf = 0;
+        bool win = mx[ic(m.tv[t], p1(i))] == p;
+        for (int guard = 0; win; guard++) {
+            if (guard > 1000000) { win = false; atomicOr(m.status, GA_ERR_WALK); break; }
+            kf++;
+            if (mx[ic(m.tv[t], m1(i))] != p) { win = false; break; }
+            int e = ic(m.tn[t], p1(i));
+            if (e < 0) break;
+            t = enc_t(e);
+            i = slot_of(m.tv[t], f);
+        }
+        if (!win) continue;
+        // chain l_0..l_kf, outer links o_0..o_{kf-1}, fan slots
+        int need = 4 * (kf + 1);
+        int base = atomicAdd(&m.counters[C_SCR], need);
+        if ((size_t)(base + need) > m.scratch_cap) { atomicOr(m.status, GA_ERR_SCRATCH); continue; }
+        int *chain = m.scratch + base, *olink = chain + (kf + 1), *slots = olink + (kf + 1), *st = slots + (kf + 1);
+        t = t0; i = i0;
+        chain[0] = ic(m.tv[t], p1(i));
+        for (int q = 0; q < kf; q++) {
+            chain[q + 1] = ic(m.tv[t], m1(i));
+            olink[q] = ic(m.tn[t], i);
+            slots[q] = t;
+            int e = ic(m.tn[t], p1(i));
+            if (e >= 0) { t = enc_t(e); i = slot_of(m.tv[t], f); }
+        }
+        // chain vertices keep an incident triangle: the outer one
+        for (int q = 0; q <= kf; q++) {
+            int o = q < kf ? olink[q] : -1;
+            if (o < 0 && q > 0) o = olink[q - 1];
+            if (o >= 0) m.vt[chain[q]] = enc_t(o);
+        }
+        // ccw-stack fill: stack st[], slink = olink reused as stack-edge links
+        int *slink = olink;   // slink[s] = outer side of stack edge (st[s], st[s+1]); olink[q] read before overwrite
+        int top = 0, used = 0;
+        st[0] = chain[0];
+        for (int q = 1; q <= kf; q++) {
+            int c = chain[q];
+            int cur = olink[q - 1];
+            while (top >= 1 && orient_dev(m, m.pos[st[top - 1]], m.pos[st[top]], m.pos[c]) > 0) {
+                int A = st[top - 1], B = st[top];
+                int T = slots[used++];
+                m.tv[T] = make_int4(A, B, c, 0);
+                m.tn[T] = make_int4(-1, -1, -1, 0);
+                bond(m, T, 2, slink[top - 1]);   // edge (A, B)
+                bond(m, T, 0, cur);              // edge (B, c)
+                m.vt[A] = T; m.vt[B] = T; m.vt[c] = T;
+                m.stamp[T] = op;
+                list_out[atomicAdd(&m.counters[cnt_out_slot], 1)] = T;
+                cur = enc(T, 1);                 // edge (c, A) seen from T
+                top--;
+            }
+            slink[top] = cur;
+            st[++top] = c;
+        }
+        // remaining stack edges become hull edges
+        for (int s = 0; s < top; s++) {
+            int X = slink[s];
+            if (X >= 0) set_link(m, enc_t(X), enc_j(X), -1);
+        }
+        for (int q = used; q < kf; q++) m.tv[slots[q]] = make_int4(GA_DEAD, GA_DEAD, GA_DEAD, 0);
+        m.vstate[f] = VS_REMOVED;
+        m.vt[f] = -1;
+        atomicAdd(&m.counters[C_FRAME], -1);
+    }
+}
+
+void launch_frame_round(MeshDev &m, unsigned int *mxraw, int round, int *list_out, int cnt_out_slot, int op,
+                        hipStream_t st)
+{
+    unsigned long long *mx = (unsigned long long *)mxraw;
+    hipMemsetAsync(mx, 0, sizeof(unsigned long long) * m.V, st);
+    hipLaunchKernelGGL(k_frame_prio, dim3(FIXED_GRID), dim3(256), 0, st, m, mx, round);
+    hipLaunchKernelGGL(k_frame_remove, dim3(FIXED_GRID), dim3(256), 0, st, m, (const unsigned long long *)mx, round, op,
+                       list_out, cnt_out_slot);
+}
+
+// ---- hull convexification (one workgroup) --------------------------------------------------------
+#define HULL_MAX 8192
+__global__ __launch_bounds__(256) void k_convexify(MeshDev m, int *hnext, int *hedge, int *list_out, int cnt_out_slot,
+                                                   int op)
+{
+    // hull cycle in the scratch arena: verts[], owner[] (enc of the triangle side of hull edge
+    // (verts[k] -> verts[k+1])), nxt[], prv[]
+    __shared__ int s_flag, s_start, s_h, s_cur, s_block;
+    int *verts = m.scratch, *owner = verts + HULL_MAX, *nxt = owner + HULL_MAX, *prv = nxt + HULL_MAX;
+    if (threadIdx.x == 0) {
+        int start = m.counters[C_HSTART];
+        if (start == 0x7fffffff) start = -1;
+        int h = 0, v = start;
+        if (start >= 0) {
+            do {
+                if (h >= HULL_MAX) { atomicOr(m.status, GA_ERR_SCRATCH); h = -1; break; }
+                verts[h] = v;
+                owner[h] = hedge[v];
+                h++;
+                v = hnext[v];
+            } while (v != start && v >= 0);
+        }
+        if (h > 0) for (int k = 0; k < h; k++) { nxt[k] = (k + 1) % h; prv[k] = (k + h - 1) % h; }
+        s_h = h;
+        s_start = 0;
+    }
+    __syncthreads();
+    int h = s_h;
+    if (h < 3) return;
+    for (int pass = 0; pass < h + 2; pass++) {
+        if (threadIdx.x == 0) s_flag = 0;
+        __syncthreads();
+        int cur = s_start, count = 0;
+        // one sweep around the cycle
+        for (int it = 0; it < 4 * HULL_MAX; it++) {
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                s_cur = cur;
+                s_block = 0;
+            }
+            __syncthreads();
+            int kc = s_cur, kp = prv[kc], kn = nxt[kc];
+            double2 P = m.pos[verts[kp]], C = m.pos[verts[kc]], N = m.pos[verts[kn]];
+            bool reflex = orient_dev(m, P, C, N) < 0;
+            if (reflex) {
+                // blocked if any other live hull vertex lies in the closed notch (P, N, C) minus P, N, C
+                for (int q = threadIdx.x; q < h; q += blockDim.x) {
+                    if (q == kc || q == kp || q == kn || nxt[q] < 0) continue;
+                    double2 Q = m.pos[verts[q]];
+                    if (orient_dev(m, P, N, Q) >= 0 && orient_dev(m, N, C, Q) >= 0 && orient_dev(m, C, P, Q) >= 0)
+                        s_block = 1;
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                if (reflex && !s_block) {
+                    int T = atomicAdd(m.ntri, 1);
+                    if (T >= m.cap) { atomicOr(m.status, GA_ERR_CAPACITY); }
+                    else {
+                        int vp = verts[kp], vc = verts[kc], vn = verts[kn];
+                        m.tv[T] = make_int4(vp, vn, vc, 0);
+                        m.tn[T] = make_int4(-1, -1, -1, 0);
+                        bond(m, T, 0, owner[kc]);   // edge (vn, vc) = reverse of hull edge (vc -> vn)
+                        bond(m, T, 1, owner[kp]);   // edge (vc, vp) = reverse of hull edge (vp -> vc)
+                        m.stamp[T] = op;
+                        list_out[atomicAdd(&m.counters[cnt_out_slot], 1)] = T;
+                        owner[kp] = enc(T, 2);       // new hull edge (vp -> vn)
+                        nxt[kp] = kn; prv[kn] = kp;
+                        nxt[kc] = -1;
+                        if (s_start == kc) s_start = kp;
+                        s_flag = 1;
+                        cur = kp;
+                    }
+                } else {
+                    cur = kn;
+                }
+                s_cur = cur;
+            }
+            __syncthreads();
+            cur = s_cur;
+            if (threadIdx.x == 0) count++;
+            if (cur == s_start && it > 0) break;
+        }
+        __syncthreads();
+        if (!s_flag) break;
+    }
+}
+
+__global__ void k_set_counter(int *c, int v) { *c = v; }
+
+__global__ void k_hull_edges(MeshDev m, int *hnext, int *hedge)
+{
+    int n = *m.ntri;
+    GRID_STRIDE(t, n) {
+        int4 v = m.tv[t];
+        if (v.x == GA_DEAD) continue;
+        int4 nb = m.tn[t];
+        for (int i = 0; i < 3; i++)
+            if (ic(nb, i) < 0) {
+                int a = ic(v, p1(i)), b = ic(v, m1(i));
+                hnext[a] = b;
+                hedge[a] = enc(t, i);
+                atomicMin(&m.counters[C_HSTART], a);
+            }
+    }
+}
+
+static int *g_hnext = nullptr, *g_hedge = nullptr;
+void set_hull_buffers(int *hn, int *he) { g_hnext = hn; g_hedge = he; }
+void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, hipStream_t st)
+{
+    hipMemsetAsync(g_hnext, 0xff, sizeof(int) * m.V, st);
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, m.counters + C_HSTART, 0x7fffffff);
+    hipLaunchKernelGGL(k_hull_edges, dim3(FIXED_GRID), dim3(256), 0, st, m, g_hnext, g_hedge);
+    hipLaunchKernelGGL(k_convexify, dim3(1), dim3(256), 0, st, m, g_hnext, g_hedge, list_out, cnt_out_slot, op);
+}
+
+// ---- canonical output ----------------------------------------------------------------------------
+__global__ void k_canon_keys(MeshDev m, unsigned long long *keys, int *vals, int *rot)
+{
+    int n = *m.ntri;
+    GRID_STRIDE(t, n) {
+        int4 v = m.tv[t];
+        vals[t] = t;
+        if (v.x == GA_DEAD) { keys[t] = ~0ull; rot[t] = 0; continue; }
+        int r = 0;
+        if (v.y < ic(v, r)) r = 1;
+        if (v.z < ic(v, r)) r = 2;
+        rot[t] = r;
+        unsigned long long a = (unsigned)ic(v, r), b = (unsigned)ic(v, (r + 1) % 3), c = (unsigned)ic(v, (r + 2) % 3);
+        keys[t] = (a << 42) | (b << 21) | c;
+        atomicAdd(&m.counters[C_RECON_TRIS], 1);
+    }
+}
+
+void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_canon_keys, dim3(FIXED_GRID), dim3(256), 0, st, m, keys, vals, rot);
+}
+
+__global__ void k_canon_newid(const int *order, int F, int *newid)
+{
+    GRID_STRIDE(r, F) newid[order[r]] = r;
+}
+
+__global__ void k_canon_emit(MeshDev m, const int *order, const int *rot, const int *newid, int F, int *out9)
+{
+    GRID_STRIDE(r, F) {
+        int s = order[r], ro = rot[s];
+        int4 v = m.tv[s], nb = m.tn[s];
+        int *o = out9 + (size_t)r * 9;
+        for (int i = 0; i < 3; i++) {
+            int si = (i + ro) % 3;
+            o[3 + i] = ic(v, si);
+            int e = ic(nb, si);
+            if (e < 0) o[i] = -1;
+            else {
+                int u = enc_t(e), j = enc_j(e);
+                o[i] = enc(newid[u], (j - rot[u] + 3) % 3);
+            }
+            o[6 + i] = 0;
+        }
+    }
+}
+
+void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, int F, int *out9, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_canon_newid, dim3(FIXED_GRID), dim3(256), 0, st, order, F, newid);
+    hipLaunchKernelGGL(k_canon_emit, dim3(FIXED_GRID), dim3(256), 0, st, m, order, rot, (const int *)newid, F, out9);
+}

@@ -1,0 +1,153 @@
+/*
+ * gpudt.h -- C ABI of the MI355X-native 2D Delaunay + alpha-shape library (libgalpha.so).
+ *
+ * Drop-in for the reference host API gpudt/gpudt.h (ashwin/gAlpha2D): the structs below
+ * are byte-compatible with gpudt/gpudt.h:77-128 (REAL = double, gpudtTriangle = 36 bytes,
+ * GPUDTPARAMS field order unchanged) and gpudtComputeDT / gpudtReleaseDTOutput keep the
+ * reference's names, argument meaning and ownership (gpudt/gpudt.h:131-132,
+ * gpudt/gpudt.cpp:771-929).  Differences, all on error paths:
+ *   - failures return NULL (or a negative code) and set a thread-local message readable
+ *     with gpudtGetLastError(); the reference calls exit(-1) (Cuda/common.h:60-93);
+ *   - nConstraints > 0 is rejected (the constrained path is not built);
+ *   - minX/minY/maxX/maxY are recomputed from the points;
+ *   - gpudtReleaseDTOutput frees the struct too (the reference leaks it, gpudt.cpp:926-929).
+ * Output triangles are counter-clockwise, each rotated so vtx[0] is its smallest vertex and
+ * listed in increasing (vtx[0], vtx[1], vtx[2]) order; tri[i] = (t << 2) | j names the
+ * neighbour across the edge opposite vtx[i] (t.vtx[j] is its far vertex), -1 on the hull;
+ * tmp[] is zero.  Exact duplicate points keep their smallest index; the others are left out
+ * (the reference drops duplicates too, Cuda/cudaMissing.cu:285-289).
+ *
+ * The alpha-shape entry points replace the C++ class Alpha_Shapes_2D
+ * (AlphaShape/Alpha_Shapes_2D.h:43-121, computeAlphaShapes Alpha_Shapes_2D.cu:206-233) with
+ * a plain struct of arrays (layouts of AlphaShape/headers.h:48-90).
+ *
+ * The galpha* entry points (contexts, device-resident runs, batches, serving) have no
+ * counterpart in the reference; they are for callers that keep data on the GPU.
+ */
+#ifndef GALPHA_GPUDT_H
+#define GALPHA_GPUDT_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define REAL double
+#define MARKER -32768
+
+typedef struct {
+    REAL x, y;
+} gpudtVertex;
+
+typedef struct gpudtTriangle {
+    int tri[3]; /* neighbour across the edge opposite vtx[i], (t << 2) | j, or -1 */
+    int vtx[3]; /* counter-clockwise vertices (input point indices) */
+    int tmp[3]; /* zero */
+} gpudtTriangle;
+
+typedef struct {
+    REAL minX, minY;
+    REAL maxX, maxY;
+    int nPoints;
+    gpudtVertex *points;
+    int nConstraints;
+    int *constraints;
+    int fboSize; /* texture side: power of two, 64..16384 */
+} GPUDTPARAMS, *PGPUDTPARAMS;
+
+typedef struct {
+    gpudtTriangle *triangles;
+    int nTris;
+} GPUDTOUTPUT, *PGPUDTOUTPUT;
+
+/* Reference API (gpudt/gpudt.h:131-132). */
+PGPUDTOUTPUT gpudtComputeDT(PGPUDTPARAMS pParam);
+void gpudtReleaseDTOutput(PGPUDTOUTPUT pGpudtOutput);
+
+/* Thread-local message of the last failure ("" if none). */
+const char *gpudtGetLastError(void);
+
+/* Alpha shapes of a DT returned by gpudtComputeDT (AlphaShape/headers.h:48-90 layouts).
+ * Ranks index spectrum[]; rank 0 is the attached/void slot (spectrum[0] = 0). */
+typedef struct {
+    int nPoints, nEdges, nTris;
+    int specSize;
+    double *spectrum;        /* [specSize] unique exact values, ascending */
+    int *tri_ro_rank;        /* [nTris] */
+    int *edge_t;             /* [nEdges] owning triangle (triangle t owns the edge opposite vtx[i]
+                                iff vtx[i+1] < vtx[i+2] or tri[i] == -1; edges ordered by (t, i)) */
+    int *edge_vopp;          /* [nEdges] i */
+    int *edge_ro_rank;       /* [nEdges] 0 when attached */
+    int *edge_nu_min_rank;   /* [nEdges] */
+    int *edge_nu_max_rank;   /* [nEdges] 0 on the hull */
+    int *vert_nu_min_rank;   /* [nPoints] -1 for dropped duplicates */
+    int *vert_nu_max_rank;   /* [nPoints] */
+    int masterSize;
+    int *master_simplex;     /* [masterSize] */
+    int *master_st;          /* 0 edge, 1 triangle, 2 vertex (simplex_type) */
+    int *master_at;          /* 0 ro, 1 nu_min, 2 nu_max (alpha_type) */
+    int nSpecPointers;
+    int *spec_pointers;      /* [nSpecPointers] [0] = -1, then first master index of each rank */
+} GPUDTALPHA, *PGPUDTALPHA;
+
+int gpudtComputeAlpha(PGPUDTPARAMS pParam, PGPUDTOUTPUT pDT, PGPUDTALPHA *pAlpha);
+void gpudtReleaseAlpha(PGPUDTALPHA pAlpha);
+
+/* ---- context API (no reference counterpart) ------------------------------------------- */
+typedef struct galpha_ctx galpha_ctx;
+
+/* stats of the last run on a context */
+typedef struct {
+    int nTris, nEdges, specSize;
+    int islandSites, pbaIterations;
+    int reconTris, shiftBad, deleted, missing, inserted, duplicates, insertRounds;
+    int flipPasses;
+    long long flips;
+    long long incircleTests, incircleExact, orientExact;
+    long long alphaExactCompares, alphaExactAttach, alphaClusters, alphaMaxCluster;
+    double ms_pba, ms_recon, ms_shift, ms_missing, ms_boundary, ms_flip, ms_canon, ms_alpha, ms_total;
+} galpha_stats;
+
+galpha_ctx *galphaCreate(int device);
+void galphaDestroy(galpha_ctx *ctx);
+const char *galphaLastError(galpha_ctx *ctx);
+
+/* Host-to-host: DT (+ alpha when alpha_out != NULL).  tris_out9: nTris * 9 ints in the
+ * gpudtTriangle layout; *ntris_out on input is its capacity (in triangles). */
+int galphaComputeHost(galpha_ctx *ctx, const double *xy, int n, int fboSize,
+                      int *tris_out9, int *ntris_out);
+
+/* Device-resident: d_xy is a device pointer to n (x, y) doubles.  The triangulation (and the
+ * alpha spectrum when with_alpha) stays on the device; query with the getters below.
+ * stream is a hipStream_t (NULL = the context's own stream). */
+int galphaComputeDevice(galpha_ctx *ctx, const double *d_xy, int n, int fboSize, int with_alpha,
+                        void *stream);
+/* device pointers to the last result: triangles in the 9-int layout, nTris */
+int galphaDeviceTriangles(galpha_ctx *ctx, const int **d_tris9, int *ntris);
+int galphaCopyTriangles(galpha_ctx *ctx, int *tris_out9, int capacity);
+int galphaGetStats(galpha_ctx *ctx, galpha_stats *stats);
+
+/* Discrete Voronoi texture only (splat + PBA + island certificate), for checking the
+ * diagram: labels[T*T] in x-major order (pixel (x, y) at x*T + y), pix[n + frame] packed
+ * (px << 16 | py); returns the number of frame sites or a negative error. */
+int galphaVoronoi(galpha_ctx *ctx, const double *xy, int n, int fboSize, int *labels, int *pix,
+                  int pix_capacity);
+
+/* Copy the alpha result of the last galphaComputeDevice(with_alpha=1) run to host arrays
+ * allocated by the library (release with gpudtReleaseAlpha). */
+int galphaCopyAlpha(galpha_ctx *ctx, PGPUDTALPHA *pAlpha);
+
+/* ---- batches and serving (no reference counterpart) ----------------------------------- */
+/* Batched: nsets independent point sets, concatenated in xy (offsets[nsets+1] in points),
+ * texture side per set (0 = autotune).  Results: tris9 concatenated, tri_offsets[nsets+1]. */
+int galphaBatch(galpha_ctx *ctx, const double *xy, const int *offsets, int nsets, const int *fboSizes,
+                int with_alpha, int **tris9_out, int **tri_offsets_out);
+void galphaFree(void *p);
+
+/* texture side the autotuner picks for n points */
+int galphaAutoTexture(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
