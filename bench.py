@@ -1,0 +1,135 @@
+#!/usr/bin/env python3
+"""Headline benchmark: 2D Delaunay triangulation + full alpha-shape spectrum on one MI355X.
+
+Workload (the plain method at scale): 1,048,576 i.i.d. U[0,1)^2 FP64 points (seed 2), PBA
+texture 4096 x 4096.  A step is one complete pass of the hot path over that set: discrete
+Voronoi diagram, reconstruction, shifting, missing-site insertion, boundary removal, flipping,
+canonical output, then rho, attachment, exact spectrum ranks, intervals and the master list.
+Inputs are resident in HBM when the timed region starts; outputs stay on the device.
+
+Prints one JSON line.  `roofline` covers the dominant kernel (timed with HIP events on the stream
+it runs on, through the library's kernel probe); `cpu_baseline` is the sequential CPU oracle on
+the same set, one core.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_POINTS = 1048576
+TEXTURE = 4096
+SEED = 2
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(n, seed):
+    """The oracle (sequential C: Shewchuk predicates, Bowyer-Watson DT, exact alpha ranks) on the
+    full workload, one core, one run (about 13 s)."""
+    from oracle import pyoracle as O
+    xy = O.gen_uniform(seed, n)
+    t0 = time.perf_counter()
+    tris, _ = O.delaunay(xy)
+    O.alpha(xy, tris)
+    dt = time.perf_counter() - t0
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        model = next((l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")), "")
+    except Exception:
+        pass
+    return {"value": round(n / dt / 1e6, 4), "unit": "Mpoints/s", "cores": 1, "kind": "port",
+            "sample": f"full workload: {n} uniform points seed {seed}, DT + alpha spectrum, 1 run, "
+                      f"{dt:.2f} s on one core of {model or 'the host'} (nproc={os.cpu_count()})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=N_POINTS)
+    ap.add_argument("--texture", type=int, default=TEXTURE)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--probe", default="k_pba_color", help="kernel timed for the roofline figure")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(independent point sets shard across GPUs with no collective; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import galpha2d_amd as G
+    from galpha2d_amd import datagen
+
+    n, T = args.n, args.texture
+    xy = datagen.uniform(SEED, n)
+    ctx = G.Context(0)
+    d_xy = torch.from_numpy(xy).to("cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    torch.cuda.synchronize()
+
+    def step():
+        return ctx.compute_device(d_xy.data_ptr(), n, T, with_alpha=True, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stage_ms = {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        s = ctx.stats()
+        for k, v in s.items():
+            if k.startswith("ms_"):
+                stage_ms[k] = stage_ms.get(k, 0.0) + v
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    value = n * args.steps / elapsed / 1e6
+    st = ctx.stats()
+
+    # roofline of the dominant kernel: algorithmic bytes per launch / mean launch time
+    probe = ctx.kernel_probe(args.probe, lambda: step())
+    roof = None
+    if probe and probe["launches"] > 0:
+        P = T * T
+        # PBA phase 3 (colouring): reads the compacted envelope once per row and writes the
+        # 4-byte label of every pixel; algorithmic traffic = 8 B per pixel (one 4-byte texel read
+        # and one 4-byte texel write: one of the 3 passes of the 24 B/pixel PBA figure).
+        per_launch_bytes = 8.0 * P
+        avg_ms = probe["total_ms"] / probe["launches"]
+        achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": args.probe, "avg_launch_ms": round(avg_ms, 4),
+                "algorithmic_bytes_per_launch": per_launch_bytes}
+
+    out = {
+        "metric": "Mpoints/s DT+alpha at 1M pts (plain method, device-resident)",
+        "value": round(value, 3), "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{n} uniform FP64 points in [0,1)^2 (seed {SEED}), texture {T}x{T}, "
+                               f"DT + full alpha spectrum", "points": n, "texture": T,
+                   "triangles": st["nTris"], "spectrum": st["specSize"]},
+        "stage_ms": {k: round(v / args.steps, 3) for k, v in stage_ms.items()},
+        "precision": {"incircle_tests": st["incircleTests"], "incircle_exact": st["incircleExact"],
+                      "orient_exact": st["orientExact"], "alpha_exact_compares": st["alphaExactCompares"],
+                      "alpha_exact_attach": st["alphaExactAttach"]},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(n, SEED)
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -52,7 +52,8 @@ EXPORTS = ["gpudtComputeDT", "gpudtReleaseDTOutput", "gpudtGetLastError", "gpudt
            "gpudtReleaseAlpha", "galphaCreate", "galphaDestroy", "galphaLastError",
            "galphaComputeHost", "galphaComputeDevice", "galphaDeviceTriangles",
            "galphaCopyTriangles", "galphaGetStats", "galphaVoronoi", "galphaCopyAlpha",
-           "galphaBatch", "galphaFree", "galphaAutoTexture"]
+           "galphaBatch", "galphaFree", "galphaAutoTexture", "galphaKernelProbe",
+           "galphaKernelProbeResult"]
 
 
 def lib():
@@ -83,6 +84,8 @@ def lib():
     L.galphaCopyAlpha.argtypes = [vp, C.POINTER(C.POINTER(_Alpha))]
     L.galphaCopyAlpha.restype = C.c_int
     L.gpudtReleaseAlpha.argtypes = [C.POINTER(_Alpha)]
+    L.galphaKernelProbe.argtypes = [C.c_char_p]
+    L.galphaKernelProbeResult.argtypes = [ip, C.POINTER(C.c_double)]
     L.galphaAutoTexture.argtypes = [C.c_int]
     L.galphaAutoTexture.restype = C.c_int
     _lib = L
@@ -189,6 +192,20 @@ class Context:
             return _alpha_to_numpy(A)
         finally:
             self._L.gpudtReleaseAlpha(A)
+
+    def kernel_probe(self, kernel_name, fn):
+        """Run fn() with HIP events around every launch of `kernel_name` (on its stream).
+        Returns {"launches": k, "total_ms": t}."""
+        L = self._L
+        L.galphaKernelProbe(kernel_name.encode())
+        try:
+            fn()
+            k, t = C.c_int(), C.c_double()
+            if L.galphaKernelProbeResult(C.byref(k), C.byref(t)) < 0:
+                raise GalphaError("kernel probe failed")
+            return {"launches": k.value, "total_ms": t.value}
+        finally:
+            L.galphaKernelProbe(None)
 
     def stats(self):
         s = _Stats()

@@ -27,6 +27,8 @@
 #include "dev_common.h"
 #include "kernels.h"
 #include "alpha_host.h"
+#include <algorithm>
+#include <vector>
 
 #define GS(i, n) for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 #define AGRID 2048
@@ -150,6 +152,9 @@ struct AlphaK {
     int *vnumin, *vnumax;  // [n]
     int *status;
     int *counters;       // [16]
+    int *bigstart, *bigsize;   // large clusters (start position, size)
+    int bigcap;
+    int nS0;
 };
 
 #define A_NUNATT 0
@@ -324,7 +329,186 @@ GA_INLINE int cmp_ids(const AlphaK &a, int ia, int ib)
     return r;
 }
 
-#define MAX_SMALL_CLUSTER 4096
+#define MAX_SMALL_CLUSTER 64
+
+// ---- large clusters: exact 256-bit keys --------------------------------------------------------
+// For every entry of a large cluster, rho = N / (4 D) * 2^(2s) exactly (big integers).  The key is
+// the normalised truncated quotient q = floor(N 2^k / D) in [2^255, 2^256) with the exponent
+// E = 2s - 2 - k: equal values give equal keys, and truncation is monotone, so different keys are
+// ordered correctly.  Only key-equal neighbours need the exact comparator.
+template <int NL>
+GA_INLINE int bi_bitlen(const BigInt<NL> &b)
+{
+    if (b.n == 0) return 0;
+    return 32 * (b.n - 1) + (32 - __clz(b.l[b.n - 1]));
+}
+template <int NL>
+GA_INLINE int bi_bit(const BigInt<NL> &b, int i)
+{
+    int li = i >> 5;
+    return li < b.n ? (int)((b.l[li] >> (i & 31)) & 1u) : 0;
+}
+// r = 2 r + bit
+template <int NL>
+GA_INLINE void bi_shl1_add(BigInt<NL> &r, int bit, int *ovf)
+{
+    uint32_t carry = (uint32_t)bit;
+    for (int i = 0; i < r.n; i++) {
+        uint32_t v = r.l[i];
+        r.l[i] = (v << 1) | carry;
+        carry = v >> 31;
+    }
+    if (carry) {
+        if (r.n < NL) r.l[r.n++] = carry;
+        else *ovf = 1;
+    }
+    if (r.n == 0 && bit) { r.l[0] = 1; r.n = 1; }
+}
+
+__device__ __noinline__ void rho_key_exact(int st, const double2 *p, unsigned long long *q4, int *E, int *ovf)
+{
+    double v[6];
+    int np = st ? 3 : 2;
+    for (int i = 0; i < np; i++) { v[2 * i] = p[i].x; v[2 * i + 1] = p[i].y; }
+    int s = min_ulp_exp(v, 2 * np);
+    BI X[3], Y[3], num, den, t1, t2;
+    for (int i = 0; i < np; i++) { bi_from_double(X[i], p[i].x, s, ovf); bi_from_double(Y[i], p[i].y, s, ovf); }
+    if (!st) {
+        bi_sq_dist(num, X[0], Y[0], X[1], Y[1], ovf);
+        bi_zero(den);
+        den.l[0] = 1; den.n = 1;
+    } else {
+        BI l1, l2, l3, ux, uy, vx, vy;
+        bi_sq_dist(l1, X[0], Y[0], X[1], Y[1], ovf);
+        bi_sq_dist(l2, X[1], Y[1], X[2], Y[2], ovf);
+        bi_sq_dist(l3, X[2], Y[2], X[0], Y[0], ovf);
+        bi_mul(t1, l1, l2, ovf);
+        bi_mul(num, t1, l3, ovf);
+        bi_sub(ux, X[1], X[0], ovf); bi_sub(uy, Y[1], Y[0], ovf);
+        bi_sub(vx, X[2], X[0], ovf); bi_sub(vy, Y[2], Y[0], ovf);
+        bi_mul(t1, ux, vy, ovf);
+        bi_mul(t2, uy, vx, ovf);
+        bi_sub(t1, t1, t2, ovf);
+        bi_mul(den, t1, t1, ovf);
+    }
+    // binary long division of N * 2^k by D, 257 quotient bits at most
+    int Ln = bi_bitlen(num), Ld = bi_bitlen(den);
+    int k = 256 - Ln + Ld;
+    if (k < 0) { *ovf = 1; k = 0; }
+    unsigned long long q[5] = {0, 0, 0, 0, 0};
+    BI r;
+    bi_zero(r);
+    int total = Ln + k;
+    for (int i = total - 1; i >= 0; i--) {
+        int bit = i >= k ? bi_bit(num, i - k) : 0;
+        bi_shl1_add(r, bit, ovf);
+        for (int w = 4; w > 0; w--) q[w] = (q[w] << 1) | (q[w - 1] >> 63);
+        q[0] <<= 1;
+        if (bi_cmpmag(r, den) >= 0) {
+            bi_submag(r, r, den);
+            r.neg = 0;
+            q[0] |= 1ull;
+        }
+    }
+    // normalise to exactly 256 bits (top bit 255)
+    if (q[4] & 1ull) {
+        for (int w = 0; w < 4; w++) q[w] = (q[w] >> 1) | (q[w + 1] << 63);
+        q[4] >>= 1;
+        k -= 1;
+    }
+    for (int w = 0; w < 4; w++) q4[w] = q[w];
+    *E = 2 * s - 2 - k;
+}
+
+struct BigK {
+    int *cstart, *csize, *coff;     // [nbig]
+    int *eid, *epos, *ej;           // [Mbig]
+    unsigned long long *key[5];     // [Mbig] each: mantissa words 0..3 (low..high), top = cluster | exponent
+    unsigned long long *kt, *kt2;
+    int *vj, *vj2;
+};
+
+__global__ void k_big_elems(AlphaK a, BigK b)
+{
+    int c = blockIdx.x;
+    int start = b.cstart[c], size = b.csize[c], off = b.coff[c];
+    for (int j = threadIdx.x; j < size; j += blockDim.x) {
+        int gpos = start + j;
+        int id = a.id2[gpos];
+        int st;
+        double2 p[3];
+        entry_pts(a, id, &st, p);
+        int ovf = 0, E = 0;
+        unsigned long long q[4];
+        rho_key_exact(st, p, q, &E, &ovf);
+        if (ovf) atomicOr(a.status, GA_ERR_EXACT_OVERFLOW);
+        int k = off + j;
+        b.eid[k] = id;
+        b.epos[k] = gpos;
+        b.ej[k] = k;
+        for (int w = 0; w < 4; w++) b.key[w][k] = q[w];
+        b.key[4][k] = ((unsigned long long)c << 32) | (unsigned)(E + 0x40000000);
+    }
+}
+
+__global__ void k_gather_keys(int M, const int *perm, const unsigned long long *src, unsigned long long *dst)
+{
+    GS(k, M) dst[k] = src[perm[k]];
+}
+
+__global__ void k_big_scatter(AlphaK a, BigK b, int Mbig, const int *sorted_j)
+{
+    GS(k, Mbig) a.id2[b.epos[k]] = b.eid[sorted_j[k]];
+}
+
+// neighbours inside a large cluster after the key sort: different keys are ordered and
+// distinct; equal keys go to the exact comparator (equal values share a rank).  An exact
+// inversion (values closer than 2^-255 relative) marks the pair for the fix-up.
+__global__ void k_big_pairs(AlphaK a, BigK b, int Mbig, const int *sorted_j, int *closef, int *inv)
+{
+    GS(k, Mbig) {
+        int gpos = b.epos[k];
+        int jc = sorted_j[k];
+        int c = (int)(b.key[4][jc] >> 32);
+        closef[gpos] = 0;
+        inv[gpos] = 0;
+        if (gpos == b.cstart[c]) continue;   // cluster start keeps its flag
+        int jp = sorted_j[k - 1];
+        bool same = true;
+        for (int w = 0; w < 5; w++) same = same && b.key[w][jp] == b.key[w][jc];
+        int f = 1;
+        if (same) {
+            closef[gpos] = 1;
+            int r = cmp_ids(a, b.eid[jp], b.eid[jc]);
+            if (r > 0) inv[gpos] = 1;
+            f = r != 0;
+        }
+        a.gflag[gpos] = f;
+    }
+}
+
+// runs of key-equal neighbours containing an exact inversion: exact insertion sort (one thread)
+__global__ void k_big_fix(AlphaK a, BigK b, int Mbig, const int *closef, const int *inv, int *claimed)
+{
+    GS(k, Mbig) {
+        int gpos = b.epos[k];
+        if (!inv[gpos]) continue;
+        int s = gpos - 1;
+        while (closef[s]) s--;            // closef is 0 at every cluster start
+        if (atomicCAS(&claimed[s], 0, 1) != 0) continue;
+        int e = gpos;
+        while (e + 1 < a.nS0 && closef[e + 1]) e++;
+        int *ids = a.id2 + s;
+        int m = e - s + 1;
+        for (int i = 1; i < m; i++) {
+            int x = ids[i], j = i - 1;
+            while (j >= 0 && cmp_ids(a, ids[j], x) > 0) { ids[j + 1] = ids[j]; j--; }
+            ids[j + 1] = x;
+        }
+        for (int i = 1; i < m; i++) a.gflag[s + i] = cmp_ids(a, ids[i - 1], ids[i]) != 0;
+    }
+}
+
 // one thread per cluster start with size >= 2: exact insertion sort, then group flags
 __global__ void k_clusters(AlphaK a, int S0)
 {
@@ -336,7 +520,12 @@ __global__ void k_clusters(AlphaK a, int S0)
         int m = end - k;
         atomicAdd(&a.counters[A_CLUSTERS], 1);
         atomicMax(&a.counters[A_MAXCLUSTER], m);
-        if (m > MAX_SMALL_CLUSTER) { atomicOr(a.status, GA_ERR_CAPACITY); atomicAdd(&a.counters[A_BIGCLUSTER], 1); continue; }
+        if (m > MAX_SMALL_CLUSTER) {
+            int q = atomicAdd(&a.counters[A_BIGCLUSTER], 1);
+            if (q < a.bigcap) { a.bigstart[q] = k; a.bigsize[q] = m; }
+            else atomicOr(a.status, GA_ERR_CAPACITY);
+            continue;
+        }
         int *ids = a.id2 + k;
         for (int i = 1; i < m; i++) {
             int x = ids[i], j = i - 1;
@@ -466,7 +655,7 @@ enum {
     AB_CNT, AB_EOFF, AB_ET, AB_EV, AB_EATT, AB_ERANK, AB_ENMIN, AB_ENMAX, AB_TRANK, AB_KEY, AB_KEY2, AB_ID,
     AB_ID2, AB_LO, AB_HI, AB_SLO, AB_SHI, AB_FLAG, AB_RANK, AB_SPECBITS, AB_VMIN, AB_VMAX, AB_STATUS, AB_CNTRS,
     AB_TMP, AB_SCAN, AB_NEG, AB_PMAX, AB_NSMIN, AB_MKEY, AB_MVAL, AB_MFLAG, AB_MPOS, AB_OKEY, AB_OVAL,
-    AB_SKEY, AB_SVAL, AB_SPEC, AB_SP, AB_MISC
+    AB_SKEY, AB_SVAL, AB_SPEC, AB_SP, AB_MISC, AB_BIG0, AB_BIG1, AB_BIG2, AB_BIG3, AB_BIGL
 };
 
 static void ab(AlphaDev &d, int i, size_t bytes)
@@ -491,6 +680,67 @@ void alpha_release(AlphaDev &d)
 static void chk(hipError_t e, const char *w)
 {
     if (e != hipSuccess) throw GalphaError(std::string("alpha: ") + w + ": " + hipGetErrorString(e));
+}
+
+static void big_clusters(AlphaDev &d, AlphaK &a, int S0, hipStream_t st)
+{
+    int nbig = 0;
+    chk(hipMemcpyAsync(&nbig, a.counters + A_BIGCLUSTER, 4, hipMemcpyDeviceToHost, st), "copy");
+    chk(hipStreamSynchronize(st), "sync");
+    if (nbig == 0) return;
+    if (nbig > a.bigcap) throw GalphaError("alpha: too many large clusters");
+    std::vector<int> cs(nbig), sz(nbig), off(nbig + 1, 0);
+    chk(hipMemcpyAsync(cs.data(), a.bigstart, 4 * (size_t)nbig, hipMemcpyDeviceToHost, st), "copy");
+    chk(hipMemcpyAsync(sz.data(), a.bigsize, 4 * (size_t)nbig, hipMemcpyDeviceToHost, st), "copy");
+    chk(hipStreamSynchronize(st), "sync");
+    // order clusters by position so that cluster ids follow spectrum order
+    std::vector<int> ord(nbig);
+    for (int i = 0; i < nbig; i++) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](int x, int y) { return cs[x] < cs[y]; });
+    std::vector<int> cs2(nbig), sz2(nbig);
+    for (int i = 0; i < nbig; i++) { cs2[i] = cs[ord[i]]; sz2[i] = sz[ord[i]]; off[i + 1] = off[i] + sz2[i]; }
+    int Mb = off[nbig];
+    ab(d, AB_BIG0, 4 * (size_t)nbig * 3 + 64);
+    int *cstart = AP<int>(d, AB_BIG0), *csize = cstart + nbig, *coff = csize + nbig;
+    chk(hipMemcpyAsync(cstart, cs2.data(), 4 * (size_t)nbig, hipMemcpyHostToDevice, st), "copy");
+    chk(hipMemcpyAsync(csize, sz2.data(), 4 * (size_t)nbig, hipMemcpyHostToDevice, st), "copy");
+    chk(hipMemcpyAsync(coff, off.data(), 4 * (size_t)nbig, hipMemcpyHostToDevice, st), "copy");
+    ab(d, AB_BIG1, (size_t)Mb * (4 * 5 + 8 * 7) + 1024);
+    char *p = (char *)d.b[AB_BIG1].p;
+    BigK b{};
+    b.cstart = cstart; b.csize = csize; b.coff = coff;
+    auto take = [&](size_t bytes) { char *r = p; p += (bytes + 255) & ~(size_t)255; return (void *)r; };
+    b.eid = (int *)take(4 * (size_t)Mb);
+    b.epos = (int *)take(4 * (size_t)Mb);
+    b.ej = (int *)take(4 * (size_t)Mb);
+    b.vj = (int *)take(4 * (size_t)Mb);
+    b.vj2 = (int *)take(4 * (size_t)Mb);
+    for (int w = 0; w < 5; w++) b.key[w] = (unsigned long long *)take(8 * (size_t)Mb);
+    b.kt = (unsigned long long *)take(8 * (size_t)Mb);
+    b.kt2 = (unsigned long long *)take(8 * (size_t)Mb);
+    GA_LAUNCH(k_big_elems, dim3(nbig), dim3(256), 0, st, a, b);
+    // LSD radix passes: mantissa word 0 (lowest) .. 3, then cluster | exponent
+    size_t tb = 0;
+    chk(rocprim::radix_sort_pairs(nullptr, tb, b.kt, b.kt2, b.ej, b.vj, (unsigned)Mb, 0u, 64u, st), "sort");
+    ab(d, AB_TMP, tb + 256);
+    int *perm_in = b.ej, *perm_out = b.vj;
+    for (int w = 0; w < 5; w++) {
+        GA_LAUNCH(k_gather_keys, dim3(AGRID), dim3(256), 0, st, Mb, (const int *)perm_in,
+                  (const unsigned long long *)b.key[w], b.kt);
+        size_t t2 = d.b[AB_TMP].bytes;
+        chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, t2, b.kt, b.kt2, perm_in, perm_out, (unsigned)Mb, 0u, 64u, st),
+            "sort");
+        std::swap(perm_in, perm_out);
+        if (w == 0) perm_out = b.vj2;
+    }
+    // perm_in: element indices in (cluster, value) order
+    GA_LAUNCH(k_big_scatter, dim3(AGRID), dim3(256), 0, st, a, b, Mb, (const int *)perm_in);
+    ab(d, AB_BIG3, 12 * (size_t)S0 + 64);
+    int *closef = AP<int>(d, AB_BIG3), *inv = closef + S0, *claimed = inv + S0;
+    chk(hipMemsetAsync(closef, 0, 12 * (size_t)S0, st), "memset");
+    a.nS0 = S0;
+    GA_LAUNCH(k_big_pairs, dim3(AGRID), dim3(256), 0, st, a, b, Mb, (const int *)perm_in, closef, inv);
+    GA_LAUNCH(k_big_fix, dim3(AGRID), dim3(256), 0, st, a, b, Mb, (const int *)closef, (const int *)inv, claimed);
 }
 
 void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F, hipStream_t st, galpha_stats *stats)
@@ -521,7 +771,7 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     a.xy = (const double2 *)d_xy; a.tris = d_tris9; a.n = n; a.F = F;
     a.cnt = AP<int>(d, AB_CNT); a.eoff = AP<int>(d, AB_EOFF);
     a.status = AP<int>(d, AB_STATUS); a.counters = AP<int>(d, AB_CNTRS);
-    hipLaunchKernelGGL(k_edge_count, dim3(AGRID), dim3(256), 0, st, a);
+    GA_LAUNCH(k_edge_count, dim3(AGRID), dim3(256), 0, st, a);
     launch_exclusive_scan(a.cnt, a.eoff, F, d.b[AB_SCAN].p, st);
     int h2[2];
     chk(hipMemcpyAsync(&h2[0], a.eoff + F - 1, 4, hipMemcpyDeviceToHost, st), "copy");
@@ -545,11 +795,13 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     a.lo = AP<double>(d, AB_LO); a.hi = AP<double>(d, AB_HI);
     a.slo = AP<double>(d, AB_SLO); a.shi = AP<double>(d, AB_SHI);
     a.flag = AP<int>(d, AB_FLAG); a.rank = AP<int>(d, AB_RANK); a.gflag = AP<int>(d, AB_MISC);
+    ab(d, AB_BIGL, 8 * 65536);
+    a.bigstart = AP<int>(d, AB_BIGL); a.bigsize = a.bigstart + 65536; a.bigcap = 65536;
     a.specbits = AP<unsigned long long>(d, AB_SPECBITS);
     a.vnumin = AP<int>(d, AB_VMIN); a.vnumax = AP<int>(d, AB_VMAX);
-    hipLaunchKernelGGL(k_edges, dim3(AGRID), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(k_tri_rho, dim3(AGRID), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(k_attached, dim3(AGRID), dim3(256), 0, st, a);
+    GA_LAUNCH(k_edges, dim3(AGRID), dim3(256), 0, st, a);
+    GA_LAUNCH(k_tri_rho, dim3(AGRID), dim3(256), 0, st, a);
+    GA_LAUNCH(k_attached, dim3(AGRID), dim3(256), 0, st, a);
     // positions of unattached edges
     launch_exclusive_scan(a.flag, a.rank, E, d.b[AB_SCAN].p, st);
     int h3[2];
@@ -559,7 +811,7 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     int nun = h3[0] + h3[1];
     int S0 = F + nun;
     d.S = S0;
-    hipLaunchKernelGGL(k_edge_entries, dim3(AGRID), dim3(256), 0, st, a, E);
+    GA_LAUNCH(k_edge_entries, dim3(AGRID), dim3(256), 0, st, a, E);
     // radix sort by FP64 bits (positive doubles order like their bit patterns)
     size_t tb = 0;
     chk(rocprim::radix_sort_pairs(nullptr, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, 64u, st), "sort");
@@ -568,7 +820,7 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, 64u, st), "sort");
     // certified intervals in sorted order, prefix max of hi, suffix min of lo
     double *neg = AP<double>(d, AB_NEG), *pmax = AP<double>(d, AB_PMAX), *nsmin = AP<double>(d, AB_NSMIN);
-    hipLaunchKernelGGL(k_gather_iv, dim3(AGRID), dim3(256), 0, st, a, S0, neg);
+    GA_LAUNCH(k_gather_iv, dim3(AGRID), dim3(256), 0, st, a, S0, neg);
     tb = 0;
     chk(rocprim::inclusive_scan(nullptr, tb, a.shi, pmax, (size_t)S0, rocprim::maximum<double>(), st), "scan");
     ab(d, AB_TMP, tb + 256);
@@ -576,9 +828,10 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     chk(rocprim::inclusive_scan(d.b[AB_TMP].p, tb, a.shi, pmax, (size_t)S0, rocprim::maximum<double>(), st), "scan");
     tb = d.b[AB_TMP].bytes;
     chk(rocprim::inclusive_scan(d.b[AB_TMP].p, tb, neg, nsmin, (size_t)S0, rocprim::maximum<double>(), st), "scan");
-    hipLaunchKernelGGL(k_cluster_flags, dim3(AGRID), dim3(256), 0, st, a, S0, (const double *)pmax, (const double *)nsmin);
-    hipLaunchKernelGGL(k_copy_i32, dim3(AGRID), dim3(256), 0, st, (const int *)a.flag, a.gflag, S0);
-    hipLaunchKernelGGL(k_clusters, dim3(AGRID), dim3(256), 0, st, a, S0);
+    GA_LAUNCH(k_cluster_flags, dim3(AGRID), dim3(256), 0, st, a, S0, (const double *)pmax, (const double *)nsmin);
+    GA_LAUNCH(k_copy_i32, dim3(AGRID), dim3(256), 0, st, (const int *)a.flag, a.gflag, S0);
+    GA_LAUNCH(k_clusters, dim3(AGRID), dim3(256), 0, st, a, S0);
+    big_clusters(d, a, S0, st);
     // ranks: inclusive scan of group starts
     launch_exclusive_scan(a.gflag, a.rank, S0, d.b[AB_SCAN].p, st);
     int h4[2];
@@ -590,16 +843,16 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     chk(hipMemsetAsync(a.specbits, 0xff, 8 * (size_t)(R + 2), st), "memset");
     // incl = excl + flag: reuse flag buffer as inclusive ranks
     int *incl = a.gflag;
-    hipLaunchKernelGGL(k_incl_from_excl, dim3(AGRID), dim3(256), 0, st, (const int *)a.rank, incl, S0);
-    hipLaunchKernelGGL(k_ranks, dim3(AGRID), dim3(256), 0, st, a, S0, (const int *)incl);
+    GA_LAUNCH(k_incl_from_excl, dim3(AGRID), dim3(256), 0, st, (const int *)a.rank, incl, S0);
+    GA_LAUNCH(k_ranks, dim3(AGRID), dim3(256), 0, st, a, S0, (const int *)incl);
     ab(d, AB_SPEC, 8 * (size_t)(R + 2));
-    hipLaunchKernelGGL(k_spec_values, dim3(AGRID), dim3(256), 0, st, (const unsigned long long *)a.specbits, R,
+    GA_LAUNCH(k_spec_values, dim3(AGRID), dim3(256), 0, st, (const unsigned long long *)a.specbits, R,
                        AP<double>(d, AB_SPEC));
     // intervals
     launch_fill_i32(a.vnumin, 0x7fffffff, (size_t)n, st);
     launch_fill_i32(a.vnumax, -1, (size_t)n, st);
-    hipLaunchKernelGGL(k_edge_intervals, dim3(AGRID), dim3(256), 0, st, a, E);
-    hipLaunchKernelGGL(k_vert_fix, dim3(AGRID), dim3(256), 0, st, a);
+    GA_LAUNCH(k_edge_intervals, dim3(AGRID), dim3(256), 0, st, a, E);
+    GA_LAUNCH(k_vert_fix, dim3(AGRID), dim3(256), 0, st, a);
     // master list
     long long M = 2LL * n + 3LL * E + F;
     if (M > 0x7fffffffLL) throw GalphaError("alpha: master list too large");
@@ -608,7 +861,7 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     ab(d, AB_SCAN, scan_temp_bytes((size_t)M) + 4096);
     unsigned *mkey = AP<unsigned>(d, AB_MKEY);
     int *mval = AP<int>(d, AB_MVAL), *mflag = AP<int>(d, AB_MFLAG), *mpos = AP<int>(d, AB_MPOS);
-    hipLaunchKernelGGL(k_master_keys, dim3(AGRID), dim3(256), 0, st, a, E, M, mkey, mval, mflag);
+    GA_LAUNCH(k_master_keys, dim3(AGRID), dim3(256), 0, st, a, E, M, mkey, mval, mflag);
     launch_exclusive_scan(mflag, mpos, (size_t)M, d.b[AB_SCAN].p, st);
     int h5[2];
     chk(hipMemcpyAsync(&h5[0], mpos + M - 1, 4, hipMemcpyDeviceToHost, st), "copy");
@@ -620,7 +873,7 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     ab(d, AB_SKEY, 4 * (size_t)Mk + 16); ab(d, AB_SVAL, 4 * (size_t)Mk + 16);
     unsigned *okey = AP<unsigned>(d, AB_OKEY), *skey = AP<unsigned>(d, AB_SKEY);
     int *oval = AP<int>(d, AB_OVAL), *sval = AP<int>(d, AB_SVAL);
-    hipLaunchKernelGGL(k_master_compact, dim3(AGRID), dim3(256), 0, st, M, (const unsigned *)mkey, (const int *)mval,
+    GA_LAUNCH(k_master_compact, dim3(AGRID), dim3(256), 0, st, M, (const unsigned *)mkey, (const int *)mval,
                        (const int *)mflag, (const int *)mpos, okey, oval);
     int bits = 1;
     while ((1LL << bits) <= (long long)R) bits++;
@@ -631,7 +884,7 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, okey, skey, oval, sval, (unsigned)Mk, 0u, (unsigned)bits, st), "sort");
     // decode into mval/mflag/mpos reused: simplex -> okey (as int), st -> oval, at -> mkey ; pflag -> mflag
     int *msimp = (int *)okey, *mst = oval, *mat = (int *)mkey;
-    hipLaunchKernelGGL(k_master_decode, dim3(AGRID), dim3(256), 0, st, n, E, Mk, (const unsigned *)skey, (const int *)sval,
+    GA_LAUNCH(k_master_decode, dim3(AGRID), dim3(256), 0, st, n, E, Mk, (const unsigned *)skey, (const int *)sval,
                        msimp, mst, mat, mflag);
     launch_exclusive_scan(mflag, mpos, (size_t)Mk, d.b[AB_SCAN].p, st);
     int h6[2] = {0, 0};
@@ -643,7 +896,7 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     int np = h6[0] + h6[1];
     d.nsp = np + 1;
     ab(d, AB_SP, 4 * (size_t)(np + 2));
-    hipLaunchKernelGGL(k_spec_pointers, dim3(AGRID), dim3(256), 0, st, Mk, (const int *)mflag, (const int *)mpos,
+    GA_LAUNCH(k_spec_pointers, dim3(AGRID), dim3(256), 0, st, Mk, (const int *)mflag, (const int *)mpos,
                        AP<int>(d, AB_SP));
     int hc[8], hs = 0;
     chk(hipMemcpyAsync(hc, a.counters, sizeof(hc), hipMemcpyDeviceToHost, st), "copy");

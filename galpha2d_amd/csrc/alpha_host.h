@@ -15,7 +15,7 @@ struct AlphaBuf {
 };
 
 struct AlphaDev {
-    AlphaBuf b[40];
+    AlphaBuf b[48];
     int n = 0, F = 0, E = 0, S = 0, specSize = 0, M = 0, nsp = 0;
 };
 

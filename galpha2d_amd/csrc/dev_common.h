@@ -327,6 +327,29 @@ GA_INLINE int incircle_exact_sos(double2 a, int ia, double2 b, int ib, double2 c
     return incircle_sos_tie(a, ia, b, ib, c, ic, d, id, ovf);
 }
 
+// ---- wave-aggregated counters -------------------------------------------------------------
+// Every active lane calls this at the same site; lanes with pred reserve one slot each.  One
+// atomic per wave instead of one per lane (single-address atomics serialise in L2).
+GA_INLINE int wave_reserve(int *counter, bool pred)
+{
+    unsigned long long active = __ballot(1);
+    unsigned long long m = __ballot(pred);
+    int lane = __lane_id();
+    int leader = __ffsll((long long)active) - 1;
+    int base = 0;
+    if (lane == leader && m) base = atomicAdd(counter, (int)__popcll(m));
+    base = __shfl(base, leader, 64);
+    return base + (int)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// sum of v over a fully active wave (call only where all 64 lanes are active, e.g. after a
+// grid-stride loop), one atomic per wave
+GA_INLINE void wave_count_full(int *counter, int v)
+{
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (__lane_id() == 0 && v) atomicAdd(counter, v);
+}
+
 // ---- misc ----------------------------------------------------------------------------
 GA_INLINE uint32_t hash32(uint32_t x)
 {

@@ -59,8 +59,11 @@ struct DevBuf {
     template <class T> T *as() { return (T *)p; }
 };
 
-static const int MARGIN = 5;
-static const int FRAME_STEP = 8;
+// Frame sites sit on the texture border every frame_step(T) pixels; real points are mapped at
+// least margin(T) pixels inside, with frame_step/2 < margin, so every border pixel is nearer
+// to a frame site than to any real site.
+static int frame_step(int T) { return T >= 2048 ? T / 256 : 8; }
+static int margin_for(int T) { return frame_step(T) / 2 + 1; }
 
 struct galpha_ctx {
     int device = 0;
@@ -71,7 +74,7 @@ struct galpha_ctx {
     // mesh
     DevBuf tv, tn, remap, stamp, claim, modA, modB, cand, pos, vt, vstate, pix, deg, off, fill, fan,
         moved, decide, mx, mlist, loc, lt, scratch, hnext, hedge, keys, keys2, vals, vals2, rot, newid,
-        out9, counters, status, xy, scanTmp, sortTmp, tileCnt, tileOff, dellist;
+        out9, counters, status, xy, scanTmp, sortTmp, tileCnt, tileOff, dellist, dbg;
     AlphaDev alpha;
     int last_n = 0, last_F = 0, last_T = 0;
     bool last_alpha = false;
@@ -109,7 +112,7 @@ static SnapGrid snap_grid(double minx, double miny, double maxx, double maxy, in
     while ((1 << logT) < T) logT++;
     double range = std::max(maxx - minx, maxy - miny);
     if (!(range > 0)) range = std::max(1.0, std::fabs(minx) + std::fabs(miny));
-    double raw = range / (double)(T - 2 * MARGIN - 3);
+    double raw = range / (double)(T - 2 * margin_for(T) - 3);
     for (int B = 53 - logT - 2; B >= 4; B--) {
         int e;
         std::frexp(raw, &e);                 // raw = f * 2^e, 0.5 <= f < 1
@@ -130,10 +133,10 @@ static std::vector<int> frame_pixels(int T)
     std::vector<int> f;
     auto add = [&](int x, int y) { f.push_back((x << 16) | y); };
     // bottom (y = 0) left->right, right (x = T-1) bottom->top, top right->left, left top->bottom
-    for (int x = 0; x < T - 1; x += FRAME_STEP) add(x, 0);
-    for (int y = 0; y < T - 1; y += FRAME_STEP) add(T - 1, y);
-    for (int x = T - 1; x > 0; x -= FRAME_STEP) add(x, T - 1);
-    for (int y = T - 1; y > 0; y -= FRAME_STEP) add(0, y);
+    for (int x = 0; x < T - 1; x += frame_step(T)) add(x, 0);
+    for (int y = 0; y < T - 1; y += frame_step(T)) add(T - 1, y);
+    for (int x = T - 1; x > 0; x -= frame_step(T)) add(x, T - 1);
+    for (int y = T - 1; y > 0; y -= frame_step(T)) add(0, y);
     return f;
 }
 
@@ -185,7 +188,7 @@ static void run_pba(galpha_ctx *c, const double *d_xy, int n, int T, const SnapG
     int logT = log2i(T);
     size_t P = (size_t)T * T;
     launch_fill_i32(c->smap.as<int>(), 0x7fffffff, P, st);
-    launch_splat((const double2 *)d_xy, n, sg.shx, sg.shy, sg.inv, MARGIN, logT, c->smap.as<int>(),
+    launch_splat((const double2 *)d_xy, n, sg.shx, sg.shy, sg.inv, margin_for(T), logT, c->smap.as<int>(),
                  c->pix.as<int>(), c->fpix.as<int>(), nf, st);
     HIPCHK(hipMemsetAsync(c->bad.p, 0, 4 * (size_t)(n + nf + 1), st));
     PbaBuffers b{c->smap.as<int>(), c->g.as<short>(), c->S.as<int>(), c->cnt.as<int>(),
@@ -291,10 +294,12 @@ static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double 
     m.cap = cap; m.ntri = c->counters.as<int>() + 40;   // slot 40 holds the slot count
     m.pos = c->pos.as<double2>(); m.xy = (const double2 *)d_xy; m.vt = c->vt.as<int>();
     m.vstate = c->vstate.as<int>(); m.pix = c->pix.as<int>();
-    m.n = n; m.V = V; m.shx = sg.shx; m.shy = sg.shy; m.scale = sg.scale; m.margin = MARGIN;
+    m.n = n; m.V = V; m.shx = sg.shx; m.shy = sg.shy; m.scale = sg.scale; m.margin = margin_for(T);
     m.status = c->status.as<int>(); m.counters = c->counters.as<int>();
     m.modA = c->modA.as<int>(); m.modB = c->modB.as<int>(); m.cand = c->cand.as<int>();
     m.scratch = c->scratch.as<int>(); m.scratch_cap = scap;
+    c->dbg.ensure(64 * sizeof(double));
+    m.dbg = c->dbg.as<double>();
     set_decide_buffer(c->decide.as<int>());
     set_del_list(c->dellist.as<int>());
     set_lt_buffer(c->lt.as<int>());
@@ -316,8 +321,9 @@ static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double 
     HIPCHK(hipEventRecord(c->ev[2], st));
 
     // ---- shifting
-    ctx_counter_set(c, C_FLAG, 1);
-    for (int it = 0; it < 64; it++) {
+    // revert moved vertices of non-ccw triangles until every triangle is ccw (monotone)
+    for (int it = 0;; it++) {
+        if (it > 100000) throw GalphaError("bulk shifting did not converge");
         ctx_counter_set(c, C_FLAG, 0);
         launch_shift_bulk(m, c->moved.as<unsigned char>(), it == 0, st);
         if (ctx_counter(c, C_FLAG) == 0) break;
@@ -343,7 +349,23 @@ static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double 
         pend -= after - before;
         int stt = 0;
         HIPCHK(hipMemcpy(&stt, c->status.p, sizeof(int), hipMemcpyDeviceToHost));
-        if (stt) throw GalphaError("vertex deletion failed (status " + std::to_string(stt) + ")");
+        if (stt) {
+            int dbg[16];
+            ctx_counters_read(c, dbg, C_DBG, 16);
+            std::string d = " [v=" + std::to_string(dbg[1]) + " poly(" + std::to_string(dbg[2]) + "):";
+            for (int k = 0; k < 12 && k < dbg[2]; k++) d += " " + std::to_string(dbg[3 + k]);
+            double dp[48];
+            HIPCHK(hipMemcpy(dp, c->dbg.p, sizeof(dp), hipMemcpyDeviceToHost));
+            char buf[64];
+            d += " pos:";
+            for (int k = 0; k < 2 + 2 * std::min(dbg[2], 15); k++) {
+                snprintf(buf, sizeof(buf), " %.17g", dp[k]);
+                d += buf;
+            }
+            d += " states:";
+            for (int k = 0; k < std::min(dbg[2], 15); k++) d += " " + std::to_string((int)dp[32 + k]);
+            throw GalphaError("vertex deletion failed (status " + std::to_string(stt) + ")" + d + "]");
+        }
     }
     S.deleted = ctx_counter(c, C_DELETED);
     HIPCHK(hipEventRecord(c->ev[3], st));
@@ -491,7 +513,7 @@ static void release_all(galpha_ctx *c)
                       &c->vstate, &c->pix, &c->deg, &c->off, &c->fill, &c->fan, &c->moved, &c->decide, &c->mx,
                       &c->mlist, &c->loc, &c->lt, &c->scratch, &c->hnext, &c->hedge, &c->keys, &c->keys2,
                       &c->vals, &c->vals2, &c->rot, &c->newid, &c->out9, &c->counters, &c->status, &c->xy,
-                      &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist};
+                      &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg};
     for (DevBuf *b : bufs) b->release();
     alpha_release(c->alpha);
 }
@@ -706,5 +728,45 @@ int gpudtComputeAlpha(PGPUDTPARAMS p, PGPUDTOUTPUT dt, PGPUDTALPHA *pA)
 void gpudtReleaseAlpha(PGPUDTALPHA a) { alpha_free_host(a); }
 
 void galphaFree(void *p) { free(p); }
+
+// Batched entry point: every set is triangulated (and optionally gets its alpha spectrum) on
+// this context; results are concatenated.  tris9_out / tri_offsets_out are malloc'ed (release
+// with galphaFree).
+int galphaBatch(galpha_ctx *c, const double *xy, const int *offsets, int nsets, const int *fboSizes,
+                int with_alpha, int **tris9_out, int **tri_offsets_out)
+{
+    if (!c || !xy || !offsets || nsets < 0 || !tris9_out || !tri_offsets_out) return -1;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        std::vector<int> toff(nsets + 1, 0);
+        std::vector<int> all;
+        for (int s = 0; s < nsets; s++) {
+            int b = offsets[s], e = offsets[s + 1];
+            if (e < b) throw GalphaError("offsets must be non-decreasing");
+            int n = e - b;
+            int T = (fboSizes && fboSizes[s] > 0) ? fboSizes[s] : galphaAutoTexture(n);
+            double bb[4] = {0, 0, 1, 1};
+            if (n > 0) bbox_host(xy + 2 * (size_t)b, n, &bb[0], &bb[1], &bb[2], &bb[3]);
+            c->xy.ensure(16 * (size_t)std::max(n, 1));
+            if (n > 0)
+                HIPCHK(hipMemcpyAsync(c->xy.p, xy + 2 * (size_t)b, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+            int F = run_dt(c, c->xy.as<double>(), n, T, bb);
+            if (with_alpha) alpha_run(c->alpha, c->xy.as<double>(), n, c->out9.as<int>(), F, c->stream, &c->stats);
+            size_t old = all.size();
+            all.resize(old + 9 * (size_t)F);
+            if (F > 0)
+                HIPCHK(hipMemcpyAsync(all.data() + old, c->out9.p, 36 * (size_t)F, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            toff[s + 1] = toff[s] + F;
+        }
+        *tris9_out = (int *)malloc(sizeof(int) * std::max<size_t>(all.size(), 1));
+        *tri_offsets_out = (int *)malloc(sizeof(int) * (nsets + 1));
+        memcpy(*tris9_out, all.data(), sizeof(int) * all.size());
+        memcpy(*tri_offsets_out, toff.data(), sizeof(int) * (nsets + 1));
+        return nsets;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
 
 }  // extern "C"

@@ -4,6 +4,16 @@
 #include <stddef.h>
 #include <stdint.h>
 
+void ga_probe_begin(const char *kernel, hipStream_t st);
+void ga_probe_end(const char *kernel, hipStream_t st);
+// kernel launch with the optional timing probe around it
+#define GA_LAUNCH(K, G, B, SH, ST, ...)                      \
+    do {                                                     \
+        ga_probe_begin(#K, ST);                              \
+        hipLaunchKernelGGL(K, G, B, SH, ST, __VA_ARGS__);    \
+        ga_probe_end(#K, ST);                                \
+    } while (0)
+
 struct PbaBuffers {
     int *smap;     // T*T site id per pixel (0x7fffffff = empty)
     short *g;      // T*T nearest-site row per column
@@ -37,6 +47,7 @@ struct MeshDev {
     int *cand;           // candidate list (capacity 3*cap)
     int *scratch;        // per-winner scratch arena
     size_t scratch_cap;
+    double *dbg;         // failure dump: positions (K + up to 15 polygon vertices) and states
 };
 
 // vertex states
@@ -68,6 +79,7 @@ struct MeshDev {
 #define C_DELETED 18
 #define C_ISLAND_SITES 19
 #define C_HSTART 20
+#define C_DBG 48          // 48..63: first failure dump
 #define C_NCOUNTERS 64
 
 // pba.hip

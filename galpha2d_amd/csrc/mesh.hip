@@ -181,11 +181,11 @@ void launch_recon(const int *vor, const int *pix, int logT, MeshDev &m, int *til
     int T = 1 << logT;
     int nyb = (T - 1 + 255) / 256;
     int ntiles = (T - 1) * nyb;
-    hipLaunchKernelGGL(k_recon_count, dim3(ntiles), dim3(256), 0, st, vor, logT, nyb, tile_cnt);
+    GA_LAUNCH(k_recon_count, dim3(ntiles), dim3(256), 0, st, vor, logT, nyb, tile_cnt);
     launch_exclusive_scan(tile_cnt, tile_off, ntiles, scan_tmp, st);
-    hipLaunchKernelGGL(k_recon_emit, dim3(ntiles), dim3(256), 0, st, vor, pix, logT, nyb, tile_off, m.tv, m.tn,
+    GA_LAUNCH(k_recon_emit, dim3(ntiles), dim3(256), 0, st, vor, pix, logT, nyb, tile_off, m.tv, m.tn,
                        m.status, m.cap);
-    hipLaunchKernelGGL(k_set_ntri, dim3(1), dim3(1), 0, st, tile_cnt, tile_off, ntiles, m.ntri, m.cap);
+    GA_LAUNCH(k_set_ntri, dim3(1), dim3(1), 0, st, tile_cnt, tile_off, ntiles, m.ntri, m.cap);
 }
 
 // ---- adjacency from scratch ----------------------------------------------------------------
@@ -243,10 +243,10 @@ void launch_adjacency(MeshDev &m, int *deg, int *off, int *fill, int *fan, void 
 {
     hipMemsetAsync(deg, 0, sizeof(int) * (m.V + 1), st);
     hipMemsetAsync(fill, 0, sizeof(int) * (m.V + 1), st);
-    hipLaunchKernelGGL(k_deg, dim3(FIXED_GRID), dim3(256), 0, st, m.tv, m.ntri, deg);
+    GA_LAUNCH(k_deg, dim3(FIXED_GRID), dim3(256), 0, st, m.tv, m.ntri, deg);
     launch_exclusive_scan(deg, off, m.V + 1, scan_tmp, st);
-    hipLaunchKernelGGL(k_fanfill, dim3(FIXED_GRID), dim3(256), 0, st, m.tv, m.ntri, off, fill, fan, m.vt);
-    hipLaunchKernelGGL(k_match, dim3(FIXED_GRID), dim3(256), 0, st, m.tv, m.ntri, off, fan, m.tn);
+    GA_LAUNCH(k_fanfill, dim3(FIXED_GRID), dim3(256), 0, st, m.tv, m.ntri, off, fill, fan, m.vt);
+    GA_LAUNCH(k_match, dim3(FIXED_GRID), dim3(256), 0, st, m.tv, m.ntri, off, fan, m.tn);
 }
 
 // ---- positions / states after reconstruction ------------------------------------------------
@@ -273,7 +273,7 @@ static int *g_deg_for_init = nullptr;
 void set_deg_for_init(int *deg) { g_deg_for_init = deg; }
 void launch_init_positions(MeshDev &m, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_init_positions, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)g_deg_for_init);
+    GA_LAUNCH(k_init_positions, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)g_deg_for_init);
 }
 
 // ---- shifting ----------------------------------------------------------------------------------
@@ -310,8 +310,8 @@ __global__ void k_shift_bulk(MeshDev m, unsigned char *moved)
 
 void launch_shift_bulk(MeshDev &m, unsigned char *moved, int first, hipStream_t st)
 {
-    if (first) hipLaunchKernelGGL(k_shift_init, dim3(FIXED_GRID), dim3(256), 0, st, m, moved);
-    hipLaunchKernelGGL(k_shift_bulk, dim3(FIXED_GRID), dim3(256), 0, st, m, moved);
+    if (first) GA_LAUNCH(k_shift_init, dim3(FIXED_GRID), dim3(256), 0, st, m, moved);
+    GA_LAUNCH(k_shift_bulk, dim3(FIXED_GRID), dim3(256), 0, st, m, moved);
 }
 
 __global__ void k_shift_commit(MeshDev m, const unsigned char *moved)
@@ -328,7 +328,7 @@ __global__ void k_shift_commit(MeshDev m, const unsigned char *moved)
 
 void launch_shift_commit(MeshDev &m, const unsigned char *moved, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_shift_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, moved);
+    GA_LAUNCH(k_shift_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, moved);
 }
 
 GA_INLINE uint32_t vprio(int v, int round) { return hash32((uint32_t)v * 0x9E3779B1u ^ hash32((uint32_t)round + 0x1234567u)); }
@@ -380,8 +380,8 @@ static int *g_decide = nullptr;
 void set_decide_buffer(int *d) { g_decide = d; }
 void launch_shift_greedy(MeshDev &m, int round, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_shift_greedy, dim3(FIXED_GRID), dim3(256), 0, st, m, round, g_decide);
-    hipLaunchKernelGGL(k_shift_greedy_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)g_decide);
+    GA_LAUNCH(k_shift_greedy, dim3(FIXED_GRID), dim3(256), 0, st, m, round, g_decide);
+    GA_LAUNCH(k_shift_greedy_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)g_decide);
 }
 
 // ---- deletion of vertices that cannot move (star-shaped hole, ear clipping) --------------------
@@ -464,7 +464,21 @@ __global__ void k_del_apply(MeshDev m, const unsigned long long *mx, int round, 
                 if (inside) continue;
                 pick = k;
             }
-            if (pick < 0) { atomicOr(m.status, GA_ERR_PATCH); break; }
+            if (pick < 0) {
+                atomicOr(m.status, GA_ERR_PATCH);
+                if (atomicCAS(&m.counters[C_DBG], 0, 1) == 0) {
+                    m.counters[C_DBG + 1] = v;
+                    m.counters[C_DBG + 2] = msz;
+                    for (int k = 0; k < msz && k < 12; k++) m.counters[C_DBG + 3 + k] = poly[k];
+                    m.dbg[0] = K.x; m.dbg[1] = K.y;
+                    for (int k = 0; k < msz && k < 15; k++) {
+                        m.dbg[2 + 2 * k] = m.pos[poly[k]].x;
+                        m.dbg[3 + 2 * k] = m.pos[poly[k]].y;
+                        m.dbg[32 + k] = (double)m.vstate[poly[k]];
+                    }
+                }
+                break;
+            }
             int kp = pick == 0 ? msz - 1 : pick - 1, kn = pick == msz - 1 ? 0 : pick + 1;
             int T = slots[used++];
             int A = poly[kp], B = poly[pick], C = poly[kn];
@@ -494,7 +508,7 @@ __global__ void k_del_apply(MeshDev m, const unsigned long long *mx, int round, 
         }
         for (int k = used; k < d; k++) m.tv[slots[k]] = make_int4(GA_DEAD, GA_DEAD, GA_DEAD, 0);
         m.vstate[v] = VS_MISSING;
-        m.vt[v] = -1;
+        m.vt[v] = slots[0];      // anchor for the re-insertion walk
         m.pos[v] = m.xy[v];
         atomicAdd(&m.counters[C_DELETED], 1);
     }
@@ -506,8 +520,8 @@ void launch_delete_round(MeshDev &m, unsigned int *mxraw, int round, int op, hip
 {
     unsigned long long *mx = (unsigned long long *)mxraw;
     hipMemsetAsync(mx, 0, sizeof(unsigned long long) * m.V, st);
-    hipLaunchKernelGGL(k_del_prio, dim3(FIXED_GRID), dim3(256), 0, st, m, mx, round);
-    hipLaunchKernelGGL(k_del_apply, dim3(FIXED_GRID), dim3(256), 0, st, m, (const unsigned long long *)mx, round, op,
+    GA_LAUNCH(k_del_prio, dim3(FIXED_GRID), dim3(256), 0, st, m, mx, round);
+    GA_LAUNCH(k_del_apply, dim3(FIXED_GRID), dim3(256), 0, st, m, (const unsigned long long *)mx, round, op,
                        g_del_list);
 }
 
@@ -515,13 +529,18 @@ void launch_delete_round(MeshDev &m, unsigned int *mxraw, int round, int op, hip
 __global__ void k_missing_init(MeshDev m, const int *__restrict__ vor, int logT, int *mlist, int *loc)
 {
     GRID_STRIDE(v, m.n) {
-        if (m.vstate[v] != VS_MISSING) continue;
-        int k = atomicAdd(&m.counters[C_MISS], 1);
+        bool miss = m.vstate[v] == VS_MISSING;
+        int k = wave_reserve(&m.counters[C_MISS], miss);
+        if (!miss) continue;
         mlist[k] = v;
-        int pp = m.pix[v];
-        int s = vor[((size_t)(pp >> 16) << logT) + (pp & 0xffff)];
-        int t = -1;
-        if (s >= 0 && m.vstate[s] == VS_IN_MESH) t = m.vt[s];
+        // a deleted vertex keeps an anchor triangle near its old position in vt[]
+        int t = m.vt[v];
+        if (t < 0 || m.tv[t].x == GA_DEAD) {
+            int pp = m.pix[v];
+            int s = vor[((size_t)(pp >> 16) << logT) + (pp & 0xffff)];
+            t = -1;
+            if (s >= 0 && m.vstate[s] == VS_IN_MESH) t = m.vt[s];
+        }
         if (t < 0) t = m.vt[m.n];
         loc[k] = t;
     }
@@ -529,7 +548,7 @@ __global__ void k_missing_init(MeshDev m, const int *__restrict__ vor, int logT,
 
 void launch_missing_init(MeshDev &m, const int *vor, int logT, int *mlist, int *loc, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_missing_init, dim3(FIXED_GRID), dim3(256), 0, st, m, vor, logT, mlist, loc);
+    GA_LAUNCH(k_missing_init, dim3(FIXED_GRID), dim3(256), 0, st, m, vor, logT, mlist, loc);
 }
 
 // located: lt = (type << 2) | edge ; type 1 inside, 2 on edge, 3 duplicate ; -1 not located
@@ -600,12 +619,22 @@ __global__ void k_insert(MeshDev m, const int *mlist, const int *loc, const int 
         int v = mlist[k];
         int t = loc[k];
         unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)v);
-        if (m.claim[t] != key) continue;
         int type = L >> 2, i = L & 3;
         int4 tt = m.tv[t], nt = m.tn[t];
+        bool win = m.claim[t] == key;
+        if (win && type == 2) win = m.claim[enc_t(ic(nt, i))] == key;
+        int q0 = wave_reserve(&m.counters[C_MODN], win);
+        int q1 = wave_reserve(&m.counters[C_MODN], win);
+        int q2 = wave_reserve(&m.counters[C_MODN], win);
+        int q3 = wave_reserve(&m.counters[C_MODN], win);
+        if (!win) continue;
         if (type == 1) {
             int s = atomicAdd(m.ntri, 2);
-            if (s + 2 > m.cap) { atomicOr(m.status, GA_ERR_CAPACITY); continue; }
+            if (s + 2 > m.cap) {
+                atomicOr(m.status, GA_ERR_CAPACITY);
+                list_out[q0] = list_out[q1] = list_out[q2] = list_out[q3] = -1;
+                continue;
+            }
             int s1 = s, s2 = s + 1;
             int a = tt.x, b = tt.y, c = tt.z;
             m.tv[t] = make_int4(a, b, v, 0);
@@ -616,15 +645,17 @@ __global__ void k_insert(MeshDev m, const int *mlist, const int *loc, const int 
             m.tn[s2] = make_int4(enc(t, 1), enc(s1, 0), mark_old(nt.y), 0);
             m.remap[t] = make_int4(enc(s1, 2), enc(s2, 2), enc(t, 2), 0);
             m.stamp[t] = op; m.stamp[s1] = op; m.stamp[s2] = op;
-            int q = atomicAdd(&m.counters[C_MODN], 3);
-            list_out[q] = t; list_out[q + 1] = s1; list_out[q + 2] = s2;
+            list_out[q0] = t; list_out[q1] = s1; list_out[q2] = s2; list_out[q3] = -1;
             m.vt[v] = t; m.vt[c] = s1;
         } else {
             int nb = ic(nt, i);
             int u = enc_t(nb), j = enc_j(nb);
-            if (m.claim[u] != key) continue;
             int s = atomicAdd(m.ntri, 2);
-            if (s + 2 > m.cap) { atomicOr(m.status, GA_ERR_CAPACITY); continue; }
+            if (s + 2 > m.cap) {
+                atomicOr(m.status, GA_ERR_CAPACITY);
+                list_out[q0] = list_out[q1] = list_out[q2] = list_out[q3] = -1;
+                continue;
+            }
             int s1 = s, s2 = s + 1;
             int4 ut = m.tv[u], un = m.tn[u];
             int c = ic(tt, i), a = ic(tt, p1(i)), b = ic(tt, m1(i));
@@ -644,8 +675,7 @@ __global__ void k_insert(MeshDev m, const int *mlist, const int *loc, const int 
             sc(ru, j, -1); sc(ru, p1(j), enc(s2, 1)); sc(ru, m1(j), enc(u, 2));
             m.remap[t] = rt; m.remap[u] = ru;
             m.stamp[t] = op; m.stamp[u] = op; m.stamp[s1] = op; m.stamp[s2] = op;
-            int q = atomicAdd(&m.counters[C_MODN], 4);
-            list_out[q] = t; list_out[q + 1] = s1; list_out[q + 2] = u; list_out[q + 3] = s2;
+            list_out[q0] = t; list_out[q1] = s1; list_out[q2] = u; list_out[q3] = s2;
             m.vt[v] = t; m.vt[a] = t; m.vt[c] = t; m.vt[b] = u; m.vt[d] = u;
         }
         m.vstate[v] = VS_IN_MESH;
@@ -659,8 +689,8 @@ void set_lt_buffer(int *b) { g_lt = b; }
 void launch_insert_round(MeshDev &m, int *mlist, int nmiss_cap, int *loc, int op, hipStream_t st)
 {
     (void)nmiss_cap;
-    hipLaunchKernelGGL(k_locate, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)mlist, loc, g_lt, op);
-    hipLaunchKernelGGL(k_insert, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)mlist, (const int *)loc,
+    GA_LAUNCH(k_locate, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)mlist, loc, g_lt, op);
+    GA_LAUNCH(k_insert, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)mlist, (const int *)loc,
                        (const int *)g_lt, op, m.modA);
 }
 
@@ -670,6 +700,7 @@ __global__ void k_relink(MeshDev m, const int *list, int cnt_slot, int op)
     int n = m.counters[cnt_slot];
     GRID_STRIDE(q, n) {
         int t = list[q];
+        if (t < 0) continue;
         int4 r = m.tn[t];
         bool ch = false;
         for (int k = 0; k < 3; k++) {
@@ -691,7 +722,7 @@ __global__ void k_relink(MeshDev m, const int *list, int cnt_slot, int op)
 
 void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_relink, dim3(FIXED_GRID), dim3(256), 0, st, m, modlist, cnt_slot, op);
+    GA_LAUNCH(k_relink, dim3(FIXED_GRID), dim3(256), 0, st, m, modlist, cnt_slot, op);
 }
 
 // ---- flipping ---------------------------------------------------------------------------------
@@ -699,54 +730,70 @@ __global__ void k_flip_all(MeshDev m, int op, int *list, int cnt_slot)
 {
     int n = *m.ntri;
     GRID_STRIDE(t, n) {
-        if (m.tv[t].x == GA_DEAD) continue;
+        bool alive = m.tv[t].x != GA_DEAD;
+        int q = wave_reserve(&m.counters[cnt_slot], alive);
+        if (!alive) continue;
         m.stamp[t] = op;
-        list[atomicAdd(&m.counters[cnt_slot], 1)] = t;
+        list[q] = t;
     }
 }
 
 void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_flip_all, dim3(FIXED_GRID), dim3(256), 0, st, m, op, list, C_MODN);
+    GA_LAUNCH(k_flip_all, dim3(FIXED_GRID), dim3(256), 0, st, m, op, list, C_MODN);
 }
 
 __global__ void k_flip_test(MeshDev m, const int *list, int cnt_slot, int prev_op, int op)
 {
     int n = m.counters[cnt_slot];
+    int tests = 0;
     GRID_STRIDE(q, n) {
         int t = list[q];
+        if (t < 0) continue;
         int4 tt = m.tv[t];
         if (tt.x == GA_DEAD) continue;
         int4 nn = m.tn[t];
         for (int i = 0; i < 3; i++) {
             int e = ic(nn, i);
-            if (e < 0) continue;
-            int u = enc_t(e), j = enc_j(e);
-            if (m.stamp[u] == prev_op && u < t) continue;
-            int d = ic(m.tv[u], j);
-            atomicAdd(&m.counters[C_TESTS_INCIRCLE], 1);
-            if (incircle_dev(m, tt.x, tt.y, tt.z, d) > 0) {
+            bool want = false;
+            int u = 0;
+            if (e >= 0) {
+                u = enc_t(e);
+                int j = enc_j(e);
+                if (!(m.stamp[u] == prev_op && u < t)) {
+                    int d = ic(m.tv[u], j);
+                    tests++;
+                    want = incircle_dev(m, tt.x, tt.y, tt.z, d) > 0;
+                }
+            }
+            int slot = wave_reserve(&m.counters[C_CAND], want);
+            if (want) {
                 unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)(t * 3 + i));
                 atomicMax(&m.claim[t], key);
                 atomicMax(&m.claim[u], key);
-                m.cand[atomicAdd(&m.counters[C_CAND], 1)] = t * 3 + i;
+                m.cand[slot] = t * 3 + i;
             }
         }
     }
+    wave_count_full(&m.counters[C_TESTS_INCIRCLE], tests);
 }
 
 __global__ void k_flip_commit(MeshDev m, int op, int *list_out, int cnt_out_slot)
 {
     int n = m.counters[C_CAND];
+    int flips = 0;
     GRID_STRIDE(q, n) {
         int c = m.cand[q];
         int t = c / 3, i = c % 3;
         unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)c);
-        if (m.claim[t] != key) continue;
+        bool win = m.claim[t] == key;
         int4 tt = m.tv[t], nt = m.tn[t];
         int e = ic(nt, i);
         int u = enc_t(e), j = enc_j(e);
-        if (m.claim[u] != key) continue;
+        win = win && m.claim[u] == key;
+        int q1 = wave_reserve(&m.counters[cnt_out_slot], win);
+        int q2 = wave_reserve(&m.counters[cnt_out_slot], win);
+        if (!win) continue;
         int4 ut = m.tv[u], un = m.tn[u];
         int cc = ic(tt, i), x = ic(tt, p1(i)), y = ic(tt, m1(i));
         int d = ic(ut, j);
@@ -761,19 +808,19 @@ __global__ void k_flip_commit(MeshDev m, int op, int *list_out, int cnt_out_slot
         sc(ru, j, -1); sc(ru, p1(j), enc(t, 0)); sc(ru, m1(j), enc(u, 2));
         m.remap[t] = rt; m.remap[u] = ru;
         m.stamp[t] = op; m.stamp[u] = op;
-        int qq = atomicAdd(&m.counters[cnt_out_slot], 2);
-        list_out[qq] = t; list_out[qq + 1] = u;
+        list_out[q1] = t; list_out[q2] = u;
         m.vt[x] = t; m.vt[cc] = t; m.vt[y] = u; m.vt[d] = u;
-        atomicAdd(&m.counters[C_FLIPS], 1);
+        flips++;
     }
+    wave_count_full(&m.counters[C_FLIPS], flips);
 }
 
 void launch_flip_pass(MeshDev &m, const int *list_in, int cnt_in_slot, int *list_out, int cnt_out_slot,
                       int prev_op, int op, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_flip_test, dim3(FIXED_GRID), dim3(256), 0, st, m, list_in, cnt_in_slot, prev_op, op);
-    hipLaunchKernelGGL(k_flip_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, op, list_out, cnt_out_slot);
-    hipLaunchKernelGGL(k_relink, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)list_out, cnt_out_slot, op);
+    GA_LAUNCH(k_flip_test, dim3(FIXED_GRID), dim3(256), 0, st, m, list_in, cnt_in_slot, prev_op, op);
+    GA_LAUNCH(k_flip_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, op, list_out, cnt_out_slot);
+    GA_LAUNCH(k_relink, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)list_out, cnt_out_slot, op);
 }
 
 // ---- frame removal ---------------------------------------------------------------------------
@@ -894,8 +941,8 @@ void launch_frame_round(MeshDev &m, unsigned int *mxraw, int round, int *list_ou
 {
     unsigned long long *mx = (unsigned long long *)mxraw;
     hipMemsetAsync(mx, 0, sizeof(unsigned long long) * m.V, st);
-    hipLaunchKernelGGL(k_frame_prio, dim3(FIXED_GRID), dim3(256), 0, st, m, mx, round);
-    hipLaunchKernelGGL(k_frame_remove, dim3(FIXED_GRID), dim3(256), 0, st, m, (const unsigned long long *)mx, round, op,
+    GA_LAUNCH(k_frame_prio, dim3(FIXED_GRID), dim3(256), 0, st, m, mx, round);
+    GA_LAUNCH(k_frame_remove, dim3(FIXED_GRID), dim3(256), 0, st, m, (const unsigned long long *)mx, round, op,
                        list_out, cnt_out_slot);
 }
 
@@ -1011,32 +1058,34 @@ void set_hull_buffers(int *hn, int *he) { g_hnext = hn; g_hedge = he; }
 void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, hipStream_t st)
 {
     hipMemsetAsync(g_hnext, 0xff, sizeof(int) * m.V, st);
-    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, st, m.counters + C_HSTART, 0x7fffffff);
-    hipLaunchKernelGGL(k_hull_edges, dim3(FIXED_GRID), dim3(256), 0, st, m, g_hnext, g_hedge);
-    hipLaunchKernelGGL(k_convexify, dim3(1), dim3(256), 0, st, m, g_hnext, g_hedge, list_out, cnt_out_slot, op);
+    GA_LAUNCH(k_set_counter, dim3(1), dim3(1), 0, st, m.counters + C_HSTART, 0x7fffffff);
+    GA_LAUNCH(k_hull_edges, dim3(FIXED_GRID), dim3(256), 0, st, m, g_hnext, g_hedge);
+    GA_LAUNCH(k_convexify, dim3(1), dim3(256), 0, st, m, g_hnext, g_hedge, list_out, cnt_out_slot, op);
 }
 
 // ---- canonical output ----------------------------------------------------------------------------
 __global__ void k_canon_keys(MeshDev m, unsigned long long *keys, int *vals, int *rot)
 {
     int n = *m.ntri;
+    int alive = 0;
     GRID_STRIDE(t, n) {
         int4 v = m.tv[t];
         vals[t] = t;
         if (v.x == GA_DEAD) { keys[t] = ~0ull; rot[t] = 0; continue; }
+        alive++;
         int r = 0;
         if (v.y < ic(v, r)) r = 1;
         if (v.z < ic(v, r)) r = 2;
         rot[t] = r;
         unsigned long long a = (unsigned)ic(v, r), b = (unsigned)ic(v, (r + 1) % 3), c = (unsigned)ic(v, (r + 2) % 3);
         keys[t] = (a << 42) | (b << 21) | c;
-        atomicAdd(&m.counters[C_RECON_TRIS], 1);
     }
+    wave_count_full(&m.counters[C_RECON_TRIS], alive);
 }
 
 void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_canon_keys, dim3(FIXED_GRID), dim3(256), 0, st, m, keys, vals, rot);
+    GA_LAUNCH(k_canon_keys, dim3(FIXED_GRID), dim3(256), 0, st, m, keys, vals, rot);
 }
 
 __global__ void k_canon_newid(const int *order, int F, int *newid)
@@ -1066,6 +1115,6 @@ __global__ void k_canon_emit(MeshDev m, const int *order, const int *rot, const 
 
 void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, int F, int *out9, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_canon_newid, dim3(FIXED_GRID), dim3(256), 0, st, order, F, newid);
-    hipLaunchKernelGGL(k_canon_emit, dim3(FIXED_GRID), dim3(256), 0, st, m, order, rot, (const int *)newid, F, out9);
+    GA_LAUNCH(k_canon_newid, dim3(FIXED_GRID), dim3(256), 0, st, order, F, newid);
+    GA_LAUNCH(k_canon_emit, dim3(FIXED_GRID), dim3(256), 0, st, m, order, rot, (const int *)newid, F, out9);
 }

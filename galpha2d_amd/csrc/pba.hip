@@ -302,37 +302,37 @@ void launch_fill_i32(int *a, int v, size_t n, hipStream_t st)
     int blocks = (int)((n + 255) / 256);
     if (blocks > 65536) blocks = 65536;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_fill_i32, dim3(blocks), dim3(256), 0, st, a, v, n);
+    GA_LAUNCH(k_fill_i32, dim3(blocks), dim3(256), 0, st, a, v, n);
 }
 
 void launch_splat(const double2 *xy, int n, double shx, double shy, double inv, int margin,
                   int logT, int *smap, int *pix, const int *fpix, int nf, hipStream_t st)
 {
     if (n > 0)
-        hipLaunchKernelGGL(k_splat, dim3((n + 255) / 256), dim3(256), 0, st, xy, n, shx, shy, inv,
+        GA_LAUNCH(k_splat, dim3((n + 255) / 256), dim3(256), 0, st, xy, n, shx, shy, inv,
                            margin, logT, smap, pix);
     if (nf > 0)
-        hipLaunchKernelGGL(k_splat_frame, dim3((nf + 255) / 256), dim3(256), 0, st, fpix, nf, n, logT,
+        GA_LAUNCH(k_splat_frame, dim3((nf + 255) / 256), dim3(256), 0, st, fpix, nf, n, logT,
                            smap, pix);
 }
 
 void launch_pba(const PbaBuffers &b, int logT, int m2, int m3, hipStream_t st)
 {
     int T = 1 << logT;
-    hipLaunchKernelGGL(k_pba_cols, dim3(T / 4), dim3(256), 0, st, b.smap, b.g, logT);
-    hipLaunchKernelGGL(k_pba_stack, dim3(T / 64, m2), dim3(64), 0, st, b.g, b.S, b.cnt, logT, T / m2);
-    hipLaunchKernelGGL(k_pba_merge, dim3(T / 64), dim3(64), 0, st, b.S, b.cnt, b.E, b.ecnt, logT, T / m2, m2);
-    hipLaunchKernelGGL(k_pba_color, dim3(T / 64, m3), dim3(64), 0, st, b.E, b.ecnt, b.smap, b.vor, logT, T / m3);
+    GA_LAUNCH(k_pba_cols, dim3(T / 4), dim3(256), 0, st, b.smap, b.g, logT);
+    GA_LAUNCH(k_pba_stack, dim3(T / 64, m2), dim3(64), 0, st, b.g, b.S, b.cnt, logT, T / m2);
+    GA_LAUNCH(k_pba_merge, dim3(T / 64), dim3(64), 0, st, b.S, b.cnt, b.E, b.ecnt, logT, T / m2, m2);
+    GA_LAUNCH(k_pba_color, dim3(T / 64, m3), dim3(64), 0, st, b.E, b.ecnt, b.smap, b.vor, logT, T / m3);
 }
 
 void launch_islands(const int *vor, const int *pix, int logT, int *bad, int *flag, hipStream_t st)
 {
     size_t P = (size_t)1 << (2 * logT);
-    hipLaunchKernelGGL(k_islands, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, vor, pix, logT, bad, flag);
+    GA_LAUNCH(k_islands, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, vor, pix, logT, bad, flag);
 }
 
 void launch_remove_bad(int *bad, const int *pix, int n, int logT, int *smap, hipStream_t st)
 {
     if (n > 0)
-        hipLaunchKernelGGL(k_remove_bad, dim3((n + 255) / 256), dim3(256), 0, st, bad, pix, n, logT, smap);
+        GA_LAUNCH(k_remove_bad, dim3((n + 255) / 256), dim3(256), 0, st, bad, pix, n, logT, smap);
 }

@@ -85,14 +85,14 @@ void launch_exclusive_scan(const int *in, int *out, size_t n, void *temp, hipStr
 {
     if (n == 0) return;
     if (n <= SCAN_TILE) {
-        hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(SCAN_BLOCK), 0, st, in, n, (const int *)nullptr, out);
+        GA_LAUNCH(k_scan_apply, dim3(1), dim3(SCAN_BLOCK), 0, st, in, n, (const int *)nullptr, out);
         return;
     }
     size_t b = nblocks(n);
     int *sums = (int *)temp;
     int *offs = sums + b;
     void *rest = (void *)(((uintptr_t)(offs + b) + 255) & ~(uintptr_t)255);
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)b), dim3(SCAN_BLOCK), 0, st, in, n, sums);
+    GA_LAUNCH(k_scan_reduce, dim3((unsigned)b), dim3(SCAN_BLOCK), 0, st, in, n, sums);
     launch_exclusive_scan(sums, offs, b, rest, st);
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)b), dim3(SCAN_BLOCK), 0, st, in, n, (const int *)offs, out);
+    GA_LAUNCH(k_scan_apply, dim3((unsigned)b), dim3(SCAN_BLOCK), 0, st, in, n, (const int *)offs, out);
 }

@@ -30,7 +30,7 @@ void bbox_device(const double *d_xy, int n, double *out4, void *tmp, hipStream_t
 {
     const int blocks = 64;
     double *part = (double *)tmp;
-    hipLaunchKernelGGL(k_bbox, dim3(blocks), dim3(256), 0, st, (const double2 *)d_xy, n, part);
+    GA_LAUNCH(k_bbox, dim3(blocks), dim3(256), 0, st, (const double2 *)d_xy, n, part);
     double h[4 * blocks];
     hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, st);
     hipStreamSynchronize(st);

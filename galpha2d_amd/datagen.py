@@ -1,0 +1,57 @@
+"""Seeded synthetic point sets for the benchmark configurations (NumPy, host side).
+
+uniform(seed, n): i.i.d. U[0,1)^2 as FP64 via splitmix64 -> (u >> 11) * 2^-53, the same stream
+the test oracle's generator produces.
+lattice_clusters(...): half the points drawn without replacement from a side x side lattice
+scaled to [0,1)^2, half from Gaussian clusters resampled into [0,1)^2.
+"""
+import numpy as np
+
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed, count):
+    with np.errstate(over="ignore"):
+        s = np.uint64(seed) + _GAMMA * np.arange(1, count + 1, dtype=np.uint64)
+        z = (s ^ (s >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed, n):
+    u = splitmix64(seed, 2 * n) >> np.uint64(11)
+    return (u.astype(np.float64) * 2.0 ** -53).reshape(n, 2)
+
+
+def lattice_clusters(n, side=1414, nclusters=64, sigma=0.005, seed=3):
+    rng = np.random.default_rng(seed)
+    n_lat = n // 2
+    n_cl = n - n_lat
+    sel = rng.choice(side * side, n_lat, replace=False)
+    lat = np.stack([(sel // side) / side, (sel % side) / side], axis=1)
+    cent = rng.random((nclusters, 2))
+    pts = cent[rng.integers(0, nclusters, n_cl)] + rng.normal(size=(n_cl, 2)) * sigma
+    bad = ((pts < 0) | (pts >= 1)).any(axis=1)
+    while bad.any():
+        k = int(bad.sum())
+        pts[bad] = cent[rng.integers(0, nclusters, k)] + rng.normal(size=(k, 2)) * sigma
+        bad = ((pts < 0) | (pts >= 1)).any(axis=1)
+    return np.concatenate([lat, pts])
+
+
+def batch_sets(nsets=4096, lo=256, hi=65536, seed=4):
+    """C4: sizes log-uniform in [lo, hi], each uniform or 8-cluster Gaussian (sigma 0.01), 50/50."""
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), nsets))).astype(np.int64)
+    sets = []
+    for i, n in enumerate(sizes):
+        if rng.random() < 0.5:
+            sets.append(uniform(seed * 1000003 + i, int(n)))
+        else:
+            cent = rng.random((8, 2))
+            p = cent[rng.integers(0, 8, n)] + rng.normal(size=(n, 2)) * 0.01
+            p = np.clip(p, 0.0, np.nextafter(1.0, 0.0))
+            sets.append(p)
+    return sets

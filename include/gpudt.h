@@ -144,6 +144,11 @@ int galphaBatch(galpha_ctx *ctx, const double *xy, const int *offsets, int nsets
                 int with_alpha, int **tris9_out, int **tri_offsets_out);
 void galphaFree(void *p);
 
+/* Kernel timing probe: while enabled, HIP events are recorded on the launch stream around every
+ * launch of the named kernel; NULL disables.  Result: number of launches and total ms. */
+int galphaKernelProbe(const char *kernel_name);
+int galphaKernelProbeResult(int *launches, double *total_ms);
+
 /* texture side the autotuner picks for n points */
 int galphaAutoTexture(int n);
 

@@ -118,6 +118,50 @@ static int cmp_val(const void *x, const void *y)
     return a->idx - b->idx;
 }
 
+void ora_rho_dd(int st, const double *a0, const double *a1, const double *a2, double *hi, double *lo);
+
+typedef struct { double hi, lo; sent e; } ddent;
+static int cmp_dd(const void *x, const void *y)
+{
+    const ddent *a = (const ddent *)x, *b = (const ddent *)y;
+    if (a->hi != b->hi) return a->hi < b->hi ? -1 : 1;
+    if (a->lo != b->lo) return a->lo < b->lo ? -1 : 1;
+    return 0;
+}
+
+static void msort(sent *v, sent *tmp, int n);
+
+/* clusters of more than 4 entries: pre-sort by a double-double value computed from the exact
+ * expansions (relative error below 2^-96), then split into groups whose consecutive values lie
+ * within 2^-90 of each other; only inside a group can the order differ from the double-double
+ * order, so each group is merge-sorted with the exact comparator.  grp[i] receives the group
+ * id of entry i (entries of different groups have different exact values). */
+static void sort_large_cluster(sent *v, int n, sent *tmp, int *grp, int *ngrp)
+{
+    ddent *d = (ddent *)malloc(sizeof(ddent) * n);
+    for (int i = 0; i < n; i++) {
+        const double *p0, *p1, *p2;
+        ent_pts(&v[i], &p0, &p1, &p2);
+        ora_rho_dd(v[i].st, p0, p1, p2, &d[i].hi, &d[i].lo);
+        d[i].e = v[i];
+    }
+    qsort(d, n, sizeof(ddent), cmp_dd);
+    for (int i = 0; i < n; i++) v[i] = d[i].e;
+    const double tol = 0x1p-90;
+    int gs = 0;
+    for (int i = 0; i < n; i++) {
+        int end = (i + 1 == n) ||
+                  ((d[i + 1].hi - d[i].hi) + (d[i + 1].lo - d[i].lo) > tol * d[i + 1].hi);
+        if (end) {
+            if (i - gs + 1 >= 2) msort(v + gs, tmp, i - gs + 1);
+            for (int k = gs; k <= i; k++) grp[k] = *ngrp;
+            (*ngrp)++;
+            gs = i + 1;
+        }
+    }
+    free(d);
+}
+
 /* stable merge sort with the exact comparator */
 static void msort(sent *v, sent *tmp, int n)
 {
@@ -125,6 +169,7 @@ static void msort(sent *v, sent *tmp, int n)
     int h = n / 2;
     msort(v, tmp, h);
     msort(v + h, tmp, n - h);
+    if (exact_cmp(&v[h - 1], &v[h]) <= 0) return;   /* halves already in order */
     int i = 0, j = h, k = 0;
     while (i < h && j < n) tmp[k++] = (exact_cmp(&v[j], &v[i]) < 0) ? v[j++] : v[i++];
     while (i < h) tmp[k++] = v[i++];
@@ -202,12 +247,18 @@ int ora_alpha_compute(const double *xy, int n, const ora_tri *t, int nt, ora_alp
     sufmin[ns] = INFINITY;
     for (int k = ns - 1; k >= 0; k--) sufmin[k] = fmin(sufmin[k + 1], S[k].lo);
     sent *tmp = (sent *)malloc(sizeof(sent) * (ns + 1));
+    /* grp[k] >= 0: id of a double-double group (exact compare needed only inside one) */
+    int *grp = (int *)malloc(sizeof(int) * (ns + 1));
+    int ngrp = 0;
+    for (int k = 0; k < ns; k++) grp[k] = -1;
     double pmax = -INFINITY;
     int cs = 0;
     for (int k = 0; k < ns; k++) {
         pmax = fmax(pmax, S[k].hi);
         if (pmax < sufmin[k + 1]) {       /* boundary after k */
-            if (k - cs + 1 >= 2) msort(S + cs, tmp, k - cs + 1);
+            int m = k - cs + 1;
+            if (m > 4) sort_large_cluster(S + cs, m, tmp, grp + cs, &ngrp);
+            else if (m >= 2) msort(S + cs, tmp, m);
             cs = k + 1;
         }
     }
@@ -217,13 +268,14 @@ int ora_alpha_compute(const double *xy, int n, const ora_tri *t, int nt, ora_alp
     A->spectrum[0] = 0.0;
     int rank = 0;
     for (int k = 0; k < ns; k++) {
-        int same = (k > 0) && exact_cmp(&S[k - 1], &S[k]) == 0;
+        int same = (k > 0) && (grp[k] < 0 || grp[k] == grp[k - 1]) && exact_cmp(&S[k - 1], &S[k]) == 0;
         if (!same) { rank++; A->spectrum[rank] = S[k].val; }
         else if (S[k].val < A->spectrum[rank]) A->spectrum[rank] = S[k].val;
         if (S[k].st == 1) A->tri_ro_rank[S[k].idx] = rank;
         else A->edge_ro_rank[S[k].idx] = rank;
     }
     A->specSize = rank + 1;
+    free(grp);
     for (int j = 0; j < ne; j++) if (A->edge_attached[j]) A->edge_ro_rank[j] = 0;
     free(S);
     /* 6. edge intervals */

@@ -420,3 +420,44 @@ double ora_orient_exact_estimate(const double *a, const double *b, const double 
     xe_free(&e);
     return s;
 }
+
+/* double-double approximation (hi + lo) of a nonoverlapping expansion stored in increasing
+ * magnitude: each component is added with an error-free two_sum and the pair renormalised,
+ * so the relative error stays near n * 2^-106. */
+static void xe_to_dd(const xexp *e, double *hi, double *lo)
+{
+    double h = 0.0, l = 0.0;
+    for (int i = 0; i < e->n; i++) {
+        double s, t;
+        two_sum(h, e->v[i], &s, &t);
+        t += l;
+        h = s + t;
+        l = t - (h - s);
+    }
+    *hi = h;
+    *lo = l;
+}
+
+/* rho of an entry (st 0 edge, 1 triangle) as a double-double computed from the exact
+ * numerator and denominator expansions (relative error about 2^-100). */
+void ora_rho_dd(int st, const double *a0, const double *a1, const double *a2, double *hi, double *lo)
+{
+    if (!inited) ora_exactinit();
+    xexp num, den;
+    xe_init(&num); xe_init(&den);
+    double nh, nl, dh = 1.0, dl = 0.0;
+    if (st == 0) sqdist_expansion(a0, a1, &num);
+    else tri_num_den(a0, a1, a2, &num, &den);
+    xe_to_dd(&num, &nh, &nl);
+    if (st != 0) xe_to_dd(&den, &dh, &dl);
+    /* (nh + nl) / (4 (dh + dl)) in double-double */
+    double q1 = nh / dh;
+    double p, pe;
+    two_product(q1, dh, &p, &pe);
+    double r = ((nh - p) - pe + nl - q1 * dl) / dh;
+    double h, l;
+    two_sum(q1, r, &h, &l);
+    *hi = h * 0.25;
+    *lo = l * 0.25;
+    xe_free(&num); xe_free(&den);
+}

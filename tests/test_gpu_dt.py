@@ -1,0 +1,125 @@
+"""GPU Delaunay path (HIP, gfx950) against the oracle: bit-exact canonical triangles and
+adjacency, plus size-independent validity checks at full scale."""
+import numpy as np
+import pytest
+
+from conftest import load_fixture, lattice_clusters
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import galpha2d_amd as G
+    c = G.Context(0)
+    yield c
+    c.close()
+
+
+def _check_same(ctx, oracle, xy, T):
+    tris = ctx.delaunay(xy, T)
+    ref, dup = oracle.delaunay(xy)
+    assert tris.shape == ref.shape, (tris.shape, ref.shape)
+    assert np.array_equal(tris, ref)
+    return tris
+
+
+def test_c1_uniform_1000(ctx, oracle):
+    print("case: C1 1000 uniform seed 1 T=512", flush=True)
+    _check_same(ctx, oracle, oracle.gen_uniform(1, 1000), 512)
+
+
+@pytest.mark.parametrize("name,T", [("bird.txt", 256), ("alpha.txt", 256), ("1000_1.txt", 512),
+                                    ("10000_1.txt", 1024)])
+def test_reference_fixtures(ctx, oracle, name, T):
+    print("case: fixture", name, T, flush=True)
+    _check_same(ctx, oracle, load_fixture(name), T)
+
+
+@pytest.mark.parametrize("n,T", [(3, 64), (4, 64), (50, 64), (5000, 256), (5000, 2048), (100000, 2048)])
+def test_uniform_sizes_and_textures(ctx, oracle, n, T):
+    print("case: uniform", n, T, flush=True)
+    _check_same(ctx, oracle, oracle.gen_uniform(7 + n, n), T)
+
+
+def test_duplicates_and_lattice(ctx, oracle):
+    print("case: lattice with duplicates", flush=True)
+    g = np.array([(i / 60.0, j / 60.0) for i in range(60) for j in range(60)])
+    xy = np.concatenate([g, g[::7], g[:3]])
+    tris = _check_same(ctx, oracle, xy, 512)
+    assert tris[:, 3:6].max() < len(g)      # duplicates keep their smallest index
+
+
+def test_collinear_and_tiny(ctx, oracle):
+    print("case: degenerate small inputs", flush=True)
+    col = np.array([[i * 0.25, 0.5 + i * 0.125] for i in range(30)])
+    assert ctx.delaunay(col, 256).shape[0] == 0
+    assert ctx.delaunay(np.zeros((0, 2)), 256).shape[0] == 0
+    assert ctx.delaunay(np.array([[0.0, 0.0], [1.0, 1.0]]), 256).shape[0] == 0
+    tri = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    _check_same(ctx, oracle, tri, 64)
+
+
+def test_lattice_clusters_small(ctx, oracle):
+    print("case: lattice+clusters 200k T=4096", flush=True)
+    _check_same(ctx, oracle, lattice_clusters(100000, 100000, 500, 16, 0.005, 3), 4096)
+
+
+def test_c2_full_scale(ctx, oracle):
+    """1,048,576 uniform points, texture 4096: bit-exact vs the oracle and valid."""
+    print("case: C2 1M uniform seed 2 T=4096", flush=True)
+    xy = oracle.gen_uniform(2, 1048576)
+    tris = _check_same(ctx, oracle, xy, 4096)
+    rc, msg = oracle.validate(xy, tris, None)
+    assert rc == 0, msg
+
+
+def test_deterministic_repeat(ctx, oracle):
+    print("case: repeat determinism 200k", flush=True)
+    xy = oracle.gen_uniform(9, 200000)
+    a = ctx.delaunay(xy, 2048)
+    b = ctx.delaunay(xy, 2048)
+    assert np.array_equal(a, b)
+
+
+def test_reference_api_entry_point(oracle):
+    """gpudtComputeDT / gpudtReleaseDTOutput through the C ABI with the reference's structs."""
+    print("case: gpudtComputeDT", flush=True)
+    import ctypes as C
+    import galpha2d_amd as G
+    L = G.lib()
+
+    class Vertex(C.Structure):
+        _fields_ = [("x", C.c_double), ("y", C.c_double)]
+
+    class Tri(C.Structure):
+        _fields_ = [("tri", C.c_int * 3), ("vtx", C.c_int * 3), ("tmp", C.c_int * 3)]
+
+    class Params(C.Structure):
+        _fields_ = [("minX", C.c_double), ("minY", C.c_double), ("maxX", C.c_double), ("maxY", C.c_double),
+                    ("nPoints", C.c_int), ("points", C.POINTER(Vertex)), ("nConstraints", C.c_int),
+                    ("constraints", C.POINTER(C.c_int)), ("fboSize", C.c_int)]
+
+    class Output(C.Structure):
+        _fields_ = [("triangles", C.POINTER(Tri)), ("nTris", C.c_int)]
+
+    xy = load_fixture("1000_1.txt")
+    pts = (Vertex * len(xy))(*[Vertex(x, y) for x, y in xy])
+    p = Params(xy[:, 0].min(), xy[:, 1].min(), xy[:, 0].max(), xy[:, 1].max(), len(xy), pts, 0, None, 512)
+    L.gpudtComputeDT.argtypes = [C.POINTER(Params)]
+    L.gpudtComputeDT.restype = C.POINTER(Output)
+    L.gpudtReleaseDTOutput.argtypes = [C.POINTER(Output)]
+    out = L.gpudtComputeDT(C.byref(p))
+    assert out
+    F = out.contents.nTris
+    got = np.ctypeslib.as_array(C.cast(out.contents.triangles, C.POINTER(C.c_int)), shape=(F * 9,)).reshape(F, 9).copy()
+    L.gpudtReleaseDTOutput(out)
+    ref, _ = oracle.delaunay(xy)
+    assert np.array_equal(got, ref)
+    # constrained input is rejected with a message, not a crash
+    p.nConstraints = 1
+    assert not L.gpudtComputeDT(C.byref(p))
+    assert b"constrained" in L.gpudtGetLastError()
+    p.nConstraints = 0
+    p.fboSize = 1000
+    assert not L.gpudtComputeDT(C.byref(p))

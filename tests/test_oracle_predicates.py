@@ -1,0 +1,92 @@
+"""Oracle predicates pinned against exact rationals, the reference's own predicates and
+algebraic invariants (no GPU)."""
+import ctypes as C
+import itertools
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+
+def _cases():
+    with open(os.path.join(GOLDEN, "predicates.json")) as f:
+        return json.load(f)
+
+
+def test_orient_known_answers(oracle):
+    for a, b, c, s in _cases()["orient"]:
+        assert oracle.orient_sign(a, b, c) == s
+
+
+def test_incircle_known_answers(oracle):
+    for a, b, c, d, s in _cases()["incircle"]:
+        assert oracle.incircle_sign(a, b, c, d) == s
+
+
+def test_exact_stage_is_exercised(oracle):
+    oracle.lib().ora_pred_reset_counters()
+    for a, b, c, d, s in _cases()["incircle"]:
+        oracle.incircle_sign(a, b, c, d)
+    calls, exact = oracle.pred_counters()[2:4]
+    assert exact > calls // 2   # the grid is built so the FP64 filter rarely decides
+
+
+def test_antisymmetry_and_cyclic(oracle):
+    rnd = random.Random(7)
+    for a, b, c, d, _ in _cases()["incircle"][:150]:
+        o = oracle.orient_sign(a, b, c)
+        assert oracle.orient_sign(b, c, a) == o and oracle.orient_sign(b, a, c) == -o
+        i = oracle.incircle_sign(a, b, c, d)
+        assert oracle.incircle_sign(b, c, a, d) == i
+        assert oracle.incircle_sign(b, a, c, d) == -i
+        assert oracle.incircle_sign(a, b, d, c) == -i   # row swap of the lifted determinant
+
+
+def test_sos_tiebreak_consistent(oracle):
+    """Exactly co-circular quadruples: the tie-break never returns 0 and is a determinant sign
+    (odd permutations flip it)."""
+    circ = [(5.0, 0.0), (4.0, 3.0), (3.0, 4.0), (0.0, 5.0), (-3.0, 4.0), (-4.0, 3.0),
+            (-5.0, 0.0), (-4.0, -3.0), (-3.0, -4.0), (0.0, -5.0), (3.0, -4.0), (4.0, -3.0)]
+    for q in itertools.combinations(range(12), 4):
+        P = [circ[i] for i in q]
+        assert oracle.incircle_sign(*P) == 0
+        s = oracle.incircle_sos(P[0], q[0], P[1], q[1], P[2], q[2], P[3], q[3])
+        assert s != 0
+        s2 = oracle.incircle_sos(P[1], q[1], P[0], q[0], P[2], q[2], P[3], q[3])
+        assert s2 == -s
+
+
+def _ref_lib():
+    path = os.path.join(ROOT, "oracle", "_ref", "libref_predicates.so")
+    if not os.path.exists(path):
+        pytest.skip("oracle/_ref not built (the reference tree is not present)")
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    L.ref_orient2d.argtypes = [dp, dp, dp]
+    L.ref_orient2d.restype = C.c_double
+    L.ref_incircle.argtypes = [dp, dp, dp, dp]
+    L.ref_incircle.restype = C.c_double
+    return L
+
+
+def _sgn(v):
+    return (v > 0) - (v < 0)
+
+
+def test_oracle_matches_reference_predicates(oracle):
+    """oracle/_ref: the reference's own gpudt/predicates.cpp, compiled from its sources."""
+    L = _ref_lib()
+    P = lambda p: np.ascontiguousarray(p, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
+    cases = _cases()
+    keep = []
+    for a, b, c, s in cases["orient"]:
+        A, B, Cc = (np.array(v, dtype=np.float64) for v in (a, b, c))
+        keep += [A, B, Cc]
+        assert _sgn(L.ref_orient2d(P(A), P(B), P(Cc))) == oracle.orient_sign(a, b, c) == s
+    for a, b, c, d, s in cases["incircle"]:
+        A, B, Cc, D = (np.array(v, dtype=np.float64) for v in (a, b, c, d))
+        assert _sgn(L.ref_incircle(P(A), P(B), P(Cc), P(D))) == oracle.incircle_sign(a, b, c, d) == s
