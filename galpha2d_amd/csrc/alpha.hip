@@ -428,12 +428,20 @@ struct BigK {
     int *vj, *vj2;
 };
 
-__global__ void k_big_elems(AlphaK a, BigK b)
+// one thread per entry of all large clusters (grid-stride); the cluster is found by binary
+// search over the entry offsets coff[0..nbig) (coff[0] = 0, increasing)
+__global__ void k_big_elems(AlphaK a, BigK b, int nbig, int Mb)
 {
-    int c = blockIdx.x;
-    int start = b.cstart[c], size = b.csize[c], off = b.coff[c];
-    for (int j = threadIdx.x; j < size; j += blockDim.x) {
-        int gpos = start + j;
+    GS(k, Mb) {
+        int lo = 0, hi = nbig - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (b.coff[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        int c = lo;
+        int j = k - b.coff[c];
+        int gpos = b.cstart[c] + j;
         int id = a.id2[gpos];
         int st;
         double2 p[3];
@@ -442,7 +450,6 @@ __global__ void k_big_elems(AlphaK a, BigK b)
         unsigned long long q[4];
         rho_key_exact(st, p, q, &E, &ovf);
         if (ovf) atomicOr(a.status, GA_ERR_EXACT_OVERFLOW);
-        int k = off + j;
         b.eid[k] = id;
         b.epos[k] = gpos;
         b.ej[k] = k;
@@ -718,7 +725,7 @@ static void big_clusters(AlphaDev &d, AlphaK &a, int S0, hipStream_t st)
     for (int w = 0; w < 5; w++) b.key[w] = (unsigned long long *)take(8 * (size_t)Mb);
     b.kt = (unsigned long long *)take(8 * (size_t)Mb);
     b.kt2 = (unsigned long long *)take(8 * (size_t)Mb);
-    GA_LAUNCH(k_big_elems, dim3(nbig), dim3(256), 0, st, a, b);
+    GA_LAUNCH(k_big_elems, dim3((Mb + 255) / 256), dim3(256), 0, st, a, b, nbig, Mb);
     // LSD radix passes: mantissa word 0 (lowest) .. 3, then cluster | exponent
     size_t tb = 0;
     chk(rocprim::radix_sort_pairs(nullptr, tb, b.kt, b.kt2, b.ej, b.vj, (unsigned)Mb, 0u, 64u, st), "sort");

@@ -95,6 +95,18 @@ static int ctx_counter(galpha_ctx *c, int slot)
     ctx_counters_read(c, &v, slot, 1);
     return v;
 }
+// stream-ordered read of device memory (the caller's stream may be non-blocking)
+static void ctx_read(galpha_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+static int ctx_status(galpha_ctx *c)
+{
+    int v = 0;
+    ctx_read(c, &v, c->status.p, sizeof(int));
+    return v;
+}
 static void ctx_counter_set(galpha_ctx *c, int slot, int v)
 {
     HIPCHK(hipMemsetD32Async((hipDeviceptr_t)(c->counters.as<int>() + slot), v, 1, c->stream));
@@ -315,7 +327,7 @@ static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double 
     if (ctx_counter(c, 40) + 0 > cap) throw GalphaError("triangle capacity exceeded in reconstruction");
     {
         int stt = 0;
-        HIPCHK(hipMemcpy(&stt, c->status.p, sizeof(int), hipMemcpyDeviceToHost));
+        stt = ctx_status(c);
         if (stt) throw GalphaError("reconstruction produced an invalid triangle (status " + std::to_string(stt) + ")");
     }
     HIPCHK(hipEventRecord(c->ev[2], st));
@@ -348,14 +360,14 @@ static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double 
         int after = ctx_counter(c, C_DELETED);
         pend -= after - before;
         int stt = 0;
-        HIPCHK(hipMemcpy(&stt, c->status.p, sizeof(int), hipMemcpyDeviceToHost));
+        stt = ctx_status(c);
         if (stt) {
             int dbg[16];
             ctx_counters_read(c, dbg, C_DBG, 16);
             std::string d = " [v=" + std::to_string(dbg[1]) + " poly(" + std::to_string(dbg[2]) + "):";
             for (int k = 0; k < 12 && k < dbg[2]; k++) d += " " + std::to_string(dbg[3 + k]);
             double dp[48];
-            HIPCHK(hipMemcpy(dp, c->dbg.p, sizeof(dp), hipMemcpyDeviceToHost));
+            ctx_read(c, dp, c->dbg.p, sizeof(dp));
             char buf[64];
             d += " pos:";
             for (int k = 0; k < 2 + 2 * std::min(dbg[2], 15); k++) {
@@ -410,7 +422,7 @@ static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double 
         if (left <= 0) break;
         {
             int stt = 0;
-            HIPCHK(hipMemcpy(&stt, c->status.p, sizeof(int), hipMemcpyDeviceToHost));
+            stt = ctx_status(c);
             if (stt) throw GalphaError("missing-site insertion failed (status " + std::to_string(stt) + ")");
         }
         if (r > 5000) throw GalphaError("missing-site insertion did not finish");
@@ -442,7 +454,7 @@ static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double 
     HIPCHK(hipEventRecord(c->ev[7], st));
     {
         int stt = 0;
-        HIPCHK(hipMemcpy(&stt, c->status.p, sizeof(int), hipMemcpyDeviceToHost));
+        stt = ctx_status(c);
         if (stt) throw GalphaError("device pipeline error (status " + std::to_string(stt) + ")");
     }
 
@@ -582,7 +594,7 @@ int galphaCopyTriangles(galpha_ctx *c, int *out, int capacity)
     try {
         int F = c->last_F;
         if (capacity < F) throw GalphaError("output capacity too small");
-        if (F > 0) HIPCHK(hipMemcpy(out, c->out9.p, 36 * (size_t)F, hipMemcpyDeviceToHost));
+        if (F > 0) ctx_read(c, out, c->out9.p, 36 * (size_t)F);
         return F;
     } catch (const std::exception &e) {
         return fail(c, e);
@@ -687,7 +699,7 @@ PGPUDTOUTPUT gpudtComputeDT(PGPUDTPARAMS p)
         PGPUDTOUTPUT out = new GPUDTOUTPUT;
         out->nTris = F;
         out->triangles = new gpudtTriangle[F > 0 ? F : 1];
-        if (F > 0) HIPCHK(hipMemcpy(out->triangles, c->out9.p, 36 * (size_t)F, hipMemcpyDeviceToHost));
+        if (F > 0) ctx_read(c, out->triangles, c->out9.p, 36 * (size_t)F);
         return out;
     } catch (const std::exception &e) {
         g_last_error = e.what();

@@ -48,3 +48,28 @@ def oracle():
     from oracle import pyoracle
     pyoracle.lib()
     return pyoracle
+
+
+ALPHA_FIELDS = ("spectrum", "tri_ro_rank", "edge_t", "edge_vopp", "edge_ro_rank", "edge_nu_min",
+                "edge_nu_max", "vert_nu_min", "vert_nu_max", "master_simplex", "master_st",
+                "master_at", "spec_pointers")
+
+
+def digest(tris, alpha=None):
+    """SHA-256 digests of a canonical triangulation (int32 [F, 9]) and of each alpha array
+    (spectrum as little-endian float64, the rest as little-endian int32)."""
+    import hashlib
+    out = {"nTris": int(tris.shape[0]),
+           "tris": hashlib.sha256(np.ascontiguousarray(tris, dtype="<i4").tobytes()).hexdigest()}
+    if alpha is not None:
+        out["specSize"] = int(len(alpha["spectrum"]))
+        for k in ALPHA_FIELDS:
+            dt = "<f8" if k == "spectrum" else "<i4"
+            out[k] = hashlib.sha256(np.ascontiguousarray(alpha[k], dtype=dt).tobytes()).hexdigest()
+    return out
+
+
+def golden_digests():
+    import json
+    with open(os.path.join(GOLDEN, "digests.json")) as f:
+        return json.load(f)

@@ -64,8 +64,8 @@ def test_alpha_100k(oracle):
 
 
 def test_alpha_lattice_clusters(oracle):
-    """C3-style input (lattice half + Gaussian clusters): large clusters of near-equal rho are
-    ordered on the device through double-double keys and the exact comparator."""
+    """Lattice half + Gaussian clusters: large clusters of near-equal rho are ordered on the
+    device by exact 256-bit quotient keys, key-equal neighbours by the exact comparator."""
     from conftest import lattice_clusters
     print("case: alpha lattice+clusters 60k", flush=True)
     xy = lattice_clusters(30000, 30000, 200, 8, 0.005, 3)
