@@ -52,7 +52,7 @@ EXPORTS = ["gpudtComputeDT", "gpudtReleaseDTOutput", "gpudtGetLastError", "gpudt
            "gpudtReleaseAlpha", "galphaCreate", "galphaDestroy", "galphaLastError",
            "galphaComputeHost", "galphaComputeDevice", "galphaDeviceTriangles",
            "galphaCopyTriangles", "galphaGetStats", "galphaVoronoi", "galphaCopyAlpha",
-           "galphaBatch", "galphaFree", "galphaAutoTexture", "galphaKernelProbe",
+           "galphaBatch", "galphaBatchCopyAlpha", "galphaFree", "galphaAutoTexture", "galphaKernelProbe",
            "galphaKernelProbeResult"]
 
 
@@ -88,6 +88,11 @@ def lib():
     L.galphaKernelProbeResult.argtypes = [ip, C.POINTER(C.c_double)]
     L.galphaAutoTexture.argtypes = [C.c_int]
     L.galphaAutoTexture.restype = C.c_int
+    L.galphaBatch.argtypes = [vp, dp, ip, C.c_int, ip, C.c_int, C.POINTER(ip), C.POINTER(ip)]
+    L.galphaBatch.restype = C.c_int
+    L.galphaBatchCopyAlpha.argtypes = [vp, C.c_int, C.POINTER(C.POINTER(_Alpha))]
+    L.galphaBatchCopyAlpha.restype = C.c_int
+    L.galphaFree.argtypes = [vp]
     _lib = L
     return L
 
@@ -165,6 +170,41 @@ class Context:
             if got < 0:
                 self._err("galphaCopyTriangles")
         return out
+
+    def batch(self, sets, fbo_sizes=None, with_alpha=False):
+        """All point sets ([n_i, 2] float64 each) through one batched pass of the pipeline.
+        Returns the canonical triangles of every set (int32 [F_i, 9], set-local indices)."""
+        sets = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 2) for x in sets]
+        ns = len(sets)
+        offs = np.zeros(ns + 1, dtype=np.int32)
+        offs[1:] = np.cumsum([x.shape[0] for x in sets])
+        xy = np.ascontiguousarray(np.concatenate(sets) if ns else np.zeros((0, 2)))
+        Ts = None
+        if fbo_sizes is not None:
+            Ts = np.ascontiguousarray(fbo_sizes, dtype=np.int32)
+        tp, op = C.POINTER(C.c_int)(), C.POINTER(C.c_int)()
+        r = self._L.galphaBatch(self._h, _dp(xy), _ip(offs), ns, _ip(Ts) if Ts is not None else None,
+                                1 if with_alpha else 0, C.byref(tp), C.byref(op))
+        if r < 0:
+            self._err("galphaBatch")
+        try:
+            toff = np.ctypeslib.as_array(op, shape=(ns + 1,)).copy()
+            F = int(toff[-1])
+            tris = np.ctypeslib.as_array(tp, shape=(max(F, 1) * 9,))[: F * 9].copy().reshape(F, 9)
+        finally:
+            self._L.galphaFree(C.cast(tp, C.c_void_p))
+            self._L.galphaFree(C.cast(op, C.c_void_p))
+        return [tris[toff[k]:toff[k + 1]] for k in range(ns)]
+
+    def batch_alpha(self, k):
+        """Alpha result of set k of the last batch(with_alpha=True)."""
+        A = C.POINTER(_Alpha)()
+        if self._L.galphaBatchCopyAlpha(self._h, int(k), C.byref(A)) < 0:
+            self._err("galphaBatchCopyAlpha")
+        try:
+            return _alpha_to_numpy(A)
+        finally:
+            self._L.gpudtReleaseAlpha(A)
 
     def compute_device(self, d_xy_ptr, n, fbo_size, with_alpha=False, stream=None):
         """Device-resident run; d_xy_ptr is a device address of n (x, y) doubles."""

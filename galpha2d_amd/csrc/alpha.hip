@@ -155,6 +155,10 @@ struct AlphaK {
     int *bigstart, *bigsize;   // large clusters (start position, size)
     int bigcap;
     int nS0;
+    // batches: set of every vertex, first spectrum slot and rank base of every set
+    const int *vset;     // [n] (NULL for a single set)
+    int nsets;
+    const int *specstart;  // [nsets] slot of rank 0 of set s in the concatenated spectra
 };
 
 #define A_NUNATT 0
@@ -555,20 +559,70 @@ __global__ void k_incl_from_excl(const int *excl, int *flag_incl, int n)
     GS(i, n) flag_incl[i] = excl[i] + flag_incl[i];
 }
 
+GA_INLINE int entry_tri(const AlphaK &a, int id) { return id < a.F ? id : a.edge_t[id - a.F]; }
+GA_INLINE int tri_set(const AlphaK &a, int t) { return a.vset ? a.vset[tv(a, t, 0)] : 0; }
+
+GA_INLINE double entry_value(const AlphaK &a, int id)
+{
+    if (id < a.F) return rho_tri(a.xy[tv(a, id, 0)], a.xy[tv(a, id, 1)], a.xy[tv(a, id, 2)]);
+    int e = id - a.F;
+    int t = a.edge_t[e], i = a.edge_v[e];
+    return rho_edge(a.xy[tv(a, t, pl1(i))], a.xy[tv(a, t, mi1(i))]);
+}
+
+// single set: rank = incl[k] (1-based); the spectrum value of a rank is the smallest FP64 value
+// of its members
 __global__ void k_ranks(AlphaK a, int S0, const int *incl)
 {
     GS(k, S0) {
-        int r = incl[k];   // 1-based rank
+        int r = incl[k];
         int id = a.id2[k];
-        double v;
-        if (id < a.F) { a.tri_rank[id] = r; v = rho_tri(a.xy[tv(a, id, 0)], a.xy[tv(a, id, 1)], a.xy[tv(a, id, 2)]); }
-        else {
-            int e = id - a.F;
-            a.edge_rank[e] = r;
-            int t = a.edge_t[e], i = a.edge_v[e];
-            v = rho_edge(a.xy[tv(a, t, pl1(i))], a.xy[tv(a, t, mi1(i))]);
-        }
-        atomicMin(&a.specbits[r], (unsigned long long)__double_as_longlong(v));
+        if (id < a.F) a.tri_rank[id] = r;
+        else a.edge_rank[id - a.F] = r;
+        atomicMin(&a.specbits[r], (unsigned long long)__double_as_longlong(entry_value(a, id)));
+    }
+}
+
+// batches: the exact order and the global ranks (equal exact value <=> equal global rank) are
+// computed over all sets at once; a stable sort by set then gives every set its entries in
+// exact order, and local ranks count the distinct global ranks inside the set.
+__global__ void k_entry_set(AlphaK a, int S0, unsigned int *sset, int *pos)
+{
+    GS(k, S0) {
+        sset[k] = (unsigned)tri_set(a, entry_tri(a, a.id2[k]));
+        pos[k] = k;
+    }
+}
+
+__global__ void k_batch_flags(int S0, const unsigned int *sset2, const int *order2, const int *incl, int *flags2)
+{
+    GS(j, S0) {
+        int f = 1;
+        if (j > 0) f = sset2[j] != sset2[j - 1] || incl[order2[j]] != incl[order2[j - 1]];
+        flags2[j] = f;
+    }
+}
+
+// first / last inclusive local count of every set (sets without entries keep -1)
+__global__ void k_set_bounds(int S0, const unsigned int *sset2, const int *incl2, int *first, int *last)
+{
+    GS(j, S0) {
+        int s = (int)sset2[j];
+        if (j == 0 || sset2[j - 1] != sset2[j]) first[s] = incl2[j];
+        if (j == S0 - 1 || sset2[j + 1] != sset2[j]) last[s] = incl2[j];
+    }
+}
+
+__global__ void k_ranks_batch(AlphaK a, int S0, const unsigned int *sset2, const int *order2, const int *incl2,
+                              const int *first)
+{
+    GS(j, S0) {
+        int s = (int)sset2[j];
+        int r = incl2[j] - first[s] + 1;
+        int id = a.id2[order2[j]];
+        if (id < a.F) a.tri_rank[id] = r;
+        else a.edge_rank[id - a.F] = r;
+        atomicMin(&a.specbits[a.specstart[s] + r], (unsigned long long)__double_as_longlong(entry_value(a, id)));
     }
 }
 
@@ -614,8 +668,16 @@ __global__ void k_master_keys(AlphaK a, int E, long long M, unsigned int *mkey, 
             int e = (int)(w / 3), at = (int)(w % 3);
             r = at == 0 ? a.edge_rank[e] : (at == 1 ? a.edge_numin[e] : a.edge_numax[e]);
         } else r = a.tri_rank[q - n2 - e3];
+        int base = 0;
+        if (a.vset && r >= 1) {
+            int set;
+            if (q < n2) set = a.vset[q >> 1];
+            else if (q < n2 + e3) set = tri_set(a, a.edge_t[(q - n2) / 3]);
+            else set = tri_set(a, (int)(q - n2 - e3));
+            base = a.specstart[set];
+        }
         mflag[q] = r >= 1;
-        mkey[q] = r >= 1 ? (unsigned)r : 0u;
+        mkey[q] = r >= 1 ? (unsigned)(base + r) : 0u;
         mval[q] = (int)q;
     }
 }
@@ -652,9 +714,34 @@ __global__ void k_spec_pointers(int Mk, const int *pflag, const int *ppos, int *
     if (blockIdx.x == 0 && threadIdx.x == 0) sp[0] = -1;
 }
 
-__global__ void k_spec_values(const unsigned long long *bits, int R, double *spec)
+// slots without a member (rank 0 of every set) read as 0; ~0 is a NaN pattern, never a value
+__global__ void k_spec_values(const unsigned long long *bits, int nslots, double *spec)
 {
-    GS(r, R + 1) spec[r] = r == 0 ? 0.0 : __longlong_as_double((long long)bits[r]);
+    GS(r, nslots) spec[r] = bits[r] == ~0ull ? 0.0 : __longlong_as_double((long long)bits[r]);
+}
+
+// first master entry of every set: keys of set s lie in (specstart[s], specstart[s + 1])
+__global__ void k_master_set_first(int Mk, const unsigned int *skey, const int *specstart, int nsets, int *mfirst)
+{
+    GS(k, Mk) {
+        int key = (int)skey[k];
+        int lo = 0, hi = nsets - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (specstart[mid] < key) lo = mid;
+            else hi = mid - 1;
+        }
+        atomicMin(&mfirst[lo], k);
+    }
+}
+
+// first edge of every set = edge offset of its first triangle (E when the set has no triangle)
+__global__ void k_edge_set_first(const int *tfirst, int nsets, int F, const int *eoff, int E, int *efirst)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsets) return;
+    int t = tfirst[s];
+    efirst[s] = t < F ? eoff[t] : E;
 }
 
 // ---- host side ------------------------------------------------------------------------------------
@@ -662,7 +749,8 @@ enum {
     AB_CNT, AB_EOFF, AB_ET, AB_EV, AB_EATT, AB_ERANK, AB_ENMIN, AB_ENMAX, AB_TRANK, AB_KEY, AB_KEY2, AB_ID,
     AB_ID2, AB_LO, AB_HI, AB_SLO, AB_SHI, AB_FLAG, AB_RANK, AB_SPECBITS, AB_VMIN, AB_VMAX, AB_STATUS, AB_CNTRS,
     AB_TMP, AB_SCAN, AB_NEG, AB_PMAX, AB_NSMIN, AB_MKEY, AB_MVAL, AB_MFLAG, AB_MPOS, AB_OKEY, AB_OVAL,
-    AB_SKEY, AB_SVAL, AB_SPEC, AB_SP, AB_MISC, AB_BIG0, AB_BIG1, AB_BIG2, AB_BIG3, AB_BIGL
+    AB_SKEY, AB_SVAL, AB_SPEC, AB_SP, AB_MISC, AB_BIG0, AB_BIG1, AB_BIG2, AB_BIG3, AB_BIGL,
+    AB_BS0, AB_BS1, AB_BS2, AB_BS3, AB_BS4, AB_BSET, AB_MSET, AB_COUNT
 };
 
 static void ab(AlphaDev &d, int i, size_t bytes)
@@ -752,7 +840,25 @@ static void big_clusters(AlphaDev &d, AlphaK &a, int S0, hipStream_t st)
 
 void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F, hipStream_t st, galpha_stats *stats)
 {
+    AlphaBatchInfo one;
+    one.nsets = 1;
+    one.poff = {0, n};
+    one.tfirst = {0, F};
+    alpha_run_batch(d, d_xy, n, d_tris9, F, one, st, stats);
+}
+
+void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F, const AlphaBatchInfo &info,
+                     hipStream_t st, galpha_stats *stats)
+{
     d.n = n; d.F = F; d.E = 0; d.S = 0; d.specSize = 1; d.M = 0; d.nsp = 1;
+    const int nsets = info.nsets;
+    d.nsets = nsets;
+    d.poff = info.poff;
+    d.tfirst = info.tfirst;
+    d.efirst.assign(nsets + 1, 0);
+    d.specstart.assign(nsets + 1, 0);
+    d.mfirst.assign(nsets + 1, 0);
+    for (int s = 0; s <= nsets; s++) d.specstart[s] = s;   // empty spectra until ranks exist
     ab(d, AB_STATUS, 16);
     ab(d, AB_CNTRS, 64 * 4);
     chk(hipMemsetAsync(d.b[AB_STATUS].p, 0, 16, st), "memset");
@@ -761,8 +867,9 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     ab(d, AB_VMAX, 4 * (size_t)(n + 1));
     ab(d, AB_SPEC, 8 * 2);
     if (F <= 0) {
-        double z = 0;
-        chk(hipMemcpyAsync(d.b[AB_SPEC].p, &z, 8, hipMemcpyHostToDevice, st), "copy");
+        ab(d, AB_SPEC, 8 * (size_t)(nsets + 1));
+        d.specSize = nsets;
+        chk(hipMemsetAsync(d.b[AB_SPEC].p, 0, 8 * (size_t)nsets, st), "memset");
         chk(hipMemsetAsync(d.b[AB_VMIN].p, 0xff, 4 * (size_t)n + 4, st), "memset");
         chk(hipMemsetAsync(d.b[AB_VMAX].p, 0xff, 4 * (size_t)n + 4, st), "memset");
         ab(d, AB_SP, 8);
@@ -792,7 +899,7 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     ab(d, AB_TRANK, 4 * (size_t)F);
     ab(d, AB_KEY, 8 * NE); ab(d, AB_KEY2, 8 * NE); ab(d, AB_ID, 4 * NE); ab(d, AB_ID2, 4 * NE);
     ab(d, AB_LO, 8 * NE); ab(d, AB_HI, 8 * NE); ab(d, AB_SLO, 8 * NE); ab(d, AB_SHI, 8 * NE);
-    ab(d, AB_FLAG, 4 * NE); ab(d, AB_RANK, 4 * NE); ab(d, AB_SPECBITS, 8 * (NE + 2));
+    ab(d, AB_FLAG, 4 * NE); ab(d, AB_RANK, 4 * NE); ab(d, AB_SPECBITS, 8 * (NE + (size_t)nsets + 2));
     ab(d, AB_NEG, 8 * NE); ab(d, AB_PMAX, 8 * NE); ab(d, AB_NSMIN, 8 * NE); ab(d, AB_MISC, 4 * NE);
     a.edge_t = AP<int>(d, AB_ET); a.edge_v = AP<int>(d, AB_EV); a.edge_att = AP<int>(d, AB_EATT);
     a.edge_rank = AP<int>(d, AB_ERANK); a.edge_numin = AP<int>(d, AB_ENMIN); a.edge_numax = AP<int>(d, AB_ENMAX);
@@ -805,6 +912,9 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     ab(d, AB_BIGL, 8 * 65536);
     a.bigstart = AP<int>(d, AB_BIGL); a.bigsize = a.bigstart + 65536; a.bigcap = 65536;
     a.specbits = AP<unsigned long long>(d, AB_SPECBITS);
+    a.vset = nsets > 1 ? info.vset : nullptr;
+    a.nsets = nsets;
+    a.specstart = nullptr;
     a.vnumin = AP<int>(d, AB_VMIN); a.vnumax = AP<int>(d, AB_VMAX);
     GA_LAUNCH(k_edges, dim3(AGRID), dim3(256), 0, st, a);
     GA_LAUNCH(k_tri_rho, dim3(AGRID), dim3(256), 0, st, a);
@@ -846,14 +956,66 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     chk(hipMemcpyAsync(&h4[1], a.gflag + S0 - 1, 4, hipMemcpyDeviceToHost, st), "copy");
     chk(hipStreamSynchronize(st), "sync");
     int R = h4[0] + h4[1];
-    d.specSize = R + 1;
-    chk(hipMemsetAsync(a.specbits, 0xff, 8 * (size_t)(R + 2), st), "memset");
+    int nslots = R + 1;
+    if (nsets == 1) {
+        d.specSize = R + 1;
+        d.specstart = {0, R + 1};
+        chk(hipMemsetAsync(a.specbits, 0xff, 8 * (size_t)(R + 2), st), "memset");
+    }
     // incl = excl + flag: reuse flag buffer as inclusive ranks
     int *incl = a.gflag;
     GA_LAUNCH(k_incl_from_excl, dim3(AGRID), dim3(256), 0, st, (const int *)a.rank, incl, S0);
-    GA_LAUNCH(k_ranks, dim3(AGRID), dim3(256), 0, st, a, S0, (const int *)incl);
-    ab(d, AB_SPEC, 8 * (size_t)(R + 2));
-    GA_LAUNCH(k_spec_values, dim3(AGRID), dim3(256), 0, st, (const unsigned long long *)a.specbits, R,
+    if (nsets == 1) {
+        GA_LAUNCH(k_ranks, dim3(AGRID), dim3(256), 0, st, a, S0, (const int *)incl);
+    } else {
+        // entries in (set, exact value) order: stable sort of positions by set
+        ab(d, AB_BS0, 4 * (size_t)S0 + 64); ab(d, AB_BS1, 4 * (size_t)S0 + 64);
+        ab(d, AB_BS2, 4 * (size_t)S0 + 64); ab(d, AB_BS3, 4 * (size_t)S0 + 64);
+        ab(d, AB_BS4, 4 * (size_t)S0 + 64);
+        ab(d, AB_BSET, 4 * 3 * (size_t)(nsets + 1));
+        unsigned *sset = AP<unsigned>(d, AB_BS0), *sset2 = AP<unsigned>(d, AB_BS2);
+        int *pos = AP<int>(d, AB_BS1), *order2 = AP<int>(d, AB_BS3), *flags2 = AP<int>(d, AB_BS4);
+        int *first = AP<int>(d, AB_BSET), *last = first + (nsets + 1), *sstart = last + (nsets + 1);
+        GA_LAUNCH(k_entry_set, dim3(AGRID), dim3(256), 0, st, a, S0, sset, pos);
+        unsigned sbits = 1;
+        while ((1LL << sbits) <= (long long)nsets) sbits++;
+        tb = 0;
+        chk(rocprim::radix_sort_pairs(nullptr, tb, sset, sset2, pos, order2, (unsigned)S0, 0u, sbits, st), "sort");
+        ab(d, AB_TMP, tb + 256);
+        tb = d.b[AB_TMP].bytes;
+        chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, sset, sset2, pos, order2, (unsigned)S0, 0u, sbits, st),
+            "sort");
+        GA_LAUNCH(k_batch_flags, dim3(AGRID), dim3(256), 0, st, S0, (const unsigned *)sset2, (const int *)order2,
+                  (const int *)incl, flags2);
+        int *incl2 = pos;   // pos is free after the sort
+        launch_exclusive_scan(flags2, incl2, S0, d.b[AB_SCAN].p, st);
+        GA_LAUNCH(k_incl_from_excl, dim3(AGRID), dim3(256), 0, st, (const int *)incl2, flags2, S0);
+        incl2 = flags2;
+        launch_fill_i32(first, -1, 2 * (size_t)(nsets + 1), st);
+        GA_LAUNCH(k_set_bounds, dim3(AGRID), dim3(256), 0, st, S0, (const unsigned *)sset2, (const int *)incl2,
+                  first, last);
+        std::vector<int> hf(nsets), hl(nsets);
+        chk(hipMemcpyAsync(hf.data(), first, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st), "copy");
+        chk(hipMemcpyAsync(hl.data(), last, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st), "copy");
+        chk(hipStreamSynchronize(st), "sync");
+        long long acc = 0;
+        for (int q = 0; q < nsets; q++) {
+            d.specstart[q] = (int)acc;
+            int Rq = hf[q] >= 0 ? hl[q] - hf[q] + 1 : 0;
+            acc += Rq + 1;
+        }
+        d.specstart[nsets] = (int)acc;
+        if (acc > 0x7fffffffLL) throw GalphaError("alpha: batch spectrum too large");
+        nslots = (int)acc;
+        d.specSize = nslots;
+        chk(hipMemcpyAsync(sstart, d.specstart.data(), 4 * (size_t)nsets, hipMemcpyHostToDevice, st), "copy");
+        a.specstart = sstart;
+        chk(hipMemsetAsync(a.specbits, 0xff, 8 * (size_t)nslots, st), "memset");
+        GA_LAUNCH(k_ranks_batch, dim3(AGRID), dim3(256), 0, st, a, S0, (const unsigned *)sset2, (const int *)order2,
+                  (const int *)incl2, (const int *)first);
+    }
+    ab(d, AB_SPEC, 8 * (size_t)(nslots + 1));
+    GA_LAUNCH(k_spec_values, dim3(AGRID), dim3(256), 0, st, (const unsigned long long *)a.specbits, nslots,
                        AP<double>(d, AB_SPEC));
     // intervals
     launch_fill_i32(a.vnumin, 0x7fffffff, (size_t)n, st);
@@ -883,7 +1045,7 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     GA_LAUNCH(k_master_compact, dim3(AGRID), dim3(256), 0, st, M, (const unsigned *)mkey, (const int *)mval,
                        (const int *)mflag, (const int *)mpos, okey, oval);
     int bits = 1;
-    while ((1LL << bits) <= (long long)R) bits++;
+    while ((1LL << bits) <= (long long)nslots) bits++;
     tb = 0;
     chk(rocprim::radix_sort_pairs(nullptr, tb, okey, skey, oval, sval, (unsigned)Mk, 0u, (unsigned)bits, st), "sort");
     ab(d, AB_TMP, tb + 256);
@@ -905,6 +1067,28 @@ void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F
     ab(d, AB_SP, 4 * (size_t)(np + 2));
     GA_LAUNCH(k_spec_pointers, dim3(AGRID), dim3(256), 0, st, Mk, (const int *)mflag, (const int *)mpos,
                        AP<int>(d, AB_SP));
+    if (nsets > 1) {
+        // first master entry and first edge of every set
+        ab(d, AB_MSET, 4 * 3 * (size_t)(nsets + 1));
+        int *mf = AP<int>(d, AB_MSET), *tf = mf + (nsets + 1), *ef = tf + (nsets + 1);
+        launch_fill_i32(mf, 0x7fffffff, (size_t)nsets, st);
+        chk(hipMemcpyAsync(tf, d.tfirst.data(), 4 * (size_t)nsets, hipMemcpyHostToDevice, st), "copy");
+        GA_LAUNCH(k_master_set_first, dim3(AGRID), dim3(256), 0, st, Mk, (const unsigned *)skey, a.specstart,
+                  nsets, mf);
+        GA_LAUNCH(k_edge_set_first, dim3((nsets + 255) / 256), dim3(256), 0, st, (const int *)tf, nsets, F,
+                  (const int *)a.eoff, E, ef);
+        std::vector<int> hm(nsets), he(nsets);
+        chk(hipMemcpyAsync(hm.data(), mf, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st), "copy");
+        chk(hipMemcpyAsync(he.data(), ef, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st), "copy");
+        chk(hipStreamSynchronize(st), "sync");
+        d.mfirst[nsets] = Mk;
+        for (int q = nsets - 1; q >= 0; q--) d.mfirst[q] = hm[q] != 0x7fffffff ? hm[q] : d.mfirst[q + 1];
+        for (int q = 0; q < nsets; q++) d.efirst[q] = he[q];
+        d.efirst[nsets] = E;
+    } else {
+        d.mfirst = {0, Mk};
+        d.efirst = {0, E};
+    }
     int hc[8], hs = 0;
     chk(hipMemcpyAsync(hc, a.counters, sizeof(hc), hipMemcpyDeviceToHost, st), "copy");
     chk(hipMemcpyAsync(&hs, a.status, 4, hipMemcpyDeviceToHost, st), "copy");
@@ -944,6 +1128,69 @@ PGPUDTALPHA alpha_copy_to_host(AlphaDev &d, hipStream_t st)
     cp((void **)&A->master_at, AB_MKEY, 4 * (size_t)d.M);
     cp((void **)&A->spec_pointers, AB_SP, 4 * (size_t)d.nsp);
     chk(hipStreamSynchronize(st), "sync");
+    return A;
+}
+
+PGPUDTALPHA alpha_copy_set_to_host(AlphaDev &d, int set, hipStream_t st)
+{
+    if (set < 0 || set >= d.nsets) throw GalphaError("alpha: set index out of range");
+    if (d.nsets == 1) return alpha_copy_to_host(d, st);
+    int p0 = d.poff[set], np = d.poff[set + 1] - p0;
+    int t0 = d.tfirst[set], nt = d.tfirst[set + 1] - t0;
+    int e0 = d.efirst[set], ne = d.efirst[set + 1] - e0;
+    int s0 = d.specstart[set], ns = d.specstart[set + 1] - s0;
+    int m0 = d.mfirst[set], nm = d.mfirst[set + 1] - m0;
+    PGPUDTALPHA A = (PGPUDTALPHA)calloc(1, sizeof(GPUDTALPHA));
+    if (!A) throw GalphaError("host allocation failed");
+    A->nPoints = np; A->nEdges = ne; A->nTris = nt; A->specSize = ns;
+    A->masterSize = nm; A->nSpecPointers = ns;
+    auto cp = [&](void **dst, int idx, size_t elem, size_t first, size_t count) {
+        *dst = malloc(count * elem > 0 ? count * elem : 1);
+        if (!*dst) throw GalphaError("host allocation failed");
+        if (count)
+            chk(hipMemcpyAsync(*dst, (char *)d.b[idx].p + first * elem, count * elem, hipMemcpyDeviceToHost, st),
+                "copy");
+    };
+    cp((void **)&A->spectrum, AB_SPEC, 8, s0, ns);
+    cp((void **)&A->tri_ro_rank, AB_TRANK, 4, t0, nt);
+    cp((void **)&A->edge_t, AB_ET, 4, e0, ne);
+    cp((void **)&A->edge_vopp, AB_EV, 4, e0, ne);
+    cp((void **)&A->edge_ro_rank, AB_ERANK, 4, e0, ne);
+    cp((void **)&A->edge_nu_min_rank, AB_ENMIN, 4, e0, ne);
+    cp((void **)&A->edge_nu_max_rank, AB_ENMAX, 4, e0, ne);
+    cp((void **)&A->vert_nu_min_rank, AB_VMIN, 4, p0, np);
+    cp((void **)&A->vert_nu_max_rank, AB_VMAX, 4, p0, np);
+    cp((void **)&A->master_simplex, AB_OKEY, 4, m0, nm);
+    cp((void **)&A->master_st, AB_OVAL, 4, m0, nm);
+    cp((void **)&A->master_at, AB_MKEY, 4, m0, nm);
+    // spec pointers of rank r >= 1 of this set: distinct key number specstart - set + r - 1
+    int R = ns - 1;
+    cp((void **)&A->spec_pointers, AB_SP, 4, R > 0 ? (size_t)(1 + s0 - set) - 1 : 0, R > 0 ? (size_t)R + 1 : 1);
+    chk(hipStreamSynchronize(st), "sync");
+    A->spec_pointers[0] = -1;
+    for (int r = 1; r <= R; r++) A->spec_pointers[r] -= m0;
+    for (int k = 0; k < ne; k++) A->edge_t[k] -= t0;
+    for (int k = 0; k < nm; k++) {
+        int stp = A->master_st[k];
+        A->master_simplex[k] -= stp == 2 ? p0 : (stp == 0 ? e0 : t0);
+    }
+    return A;
+}
+
+PGPUDTALPHA alpha_empty_host(int np)
+{
+    PGPUDTALPHA A = (PGPUDTALPHA)calloc(1, sizeof(GPUDTALPHA));
+    if (!A) throw GalphaError("host allocation failed");
+    A->nPoints = np; A->specSize = 1; A->nSpecPointers = 1;
+    A->spectrum = (double *)calloc(1, sizeof(double));
+    A->spec_pointers = (int *)malloc(sizeof(int));
+    A->spec_pointers[0] = -1;
+    A->vert_nu_min_rank = (int *)malloc(sizeof(int) * (np > 0 ? np : 1));
+    A->vert_nu_max_rank = (int *)malloc(sizeof(int) * (np > 0 ? np : 1));
+    for (int i = 0; i < np; i++) { A->vert_nu_min_rank[i] = -1; A->vert_nu_max_rank[i] = -1; }
+    A->tri_ro_rank = (int *)malloc(1); A->edge_t = (int *)malloc(1); A->edge_vopp = (int *)malloc(1);
+    A->edge_ro_rank = (int *)malloc(1); A->edge_nu_min_rank = (int *)malloc(1); A->edge_nu_max_rank = (int *)malloc(1);
+    A->master_simplex = (int *)malloc(1); A->master_st = (int *)malloc(1); A->master_at = (int *)malloc(1);
     return A;
 }
 

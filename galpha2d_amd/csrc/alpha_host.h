@@ -14,13 +14,32 @@ struct AlphaBuf {
     size_t bytes = 0;
 };
 
+#include <vector>
+
+// sets of a batch as the alpha stage sees them (a single set: nsets = 1, vset = NULL)
+struct AlphaBatchInfo {
+    int nsets = 1;
+    const int *vset = nullptr;     // device [n]: set of every real point
+    std::vector<int> poff;         // [nsets + 1] first real point of every set
+    std::vector<int> tfirst;       // [nsets + 1] first triangle of every set
+};
+
 struct AlphaDev {
-    AlphaBuf b[48];
+    AlphaBuf b[64];
     int n = 0, F = 0, E = 0, S = 0, specSize = 0, M = 0, nsp = 0;
+    // per set (host): first point, triangle, edge, spectrum slot and master entry; [nsets + 1]
+    int nsets = 1;
+    std::vector<int> poff, tfirst, efirst, specstart, mfirst;
 };
 
 void alpha_run(AlphaDev &a, const double *d_xy, int n, const int *d_tris9, int F, hipStream_t st,
                galpha_stats *stats);
+void alpha_run_batch(AlphaDev &a, const double *d_xy, int n, const int *d_tris9, int F, const AlphaBatchInfo &info,
+                     hipStream_t st, galpha_stats *stats);
 PGPUDTALPHA alpha_copy_to_host(AlphaDev &a, hipStream_t st);
+// one set of the last batch, with set-local point, triangle and edge indices
+PGPUDTALPHA alpha_copy_set_to_host(AlphaDev &a, int set, hipStream_t st);
+// alpha of a set without triangles: spectrum [0], every vertex interval -1
+PGPUDTALPHA alpha_empty_host(int npoints);
 void alpha_free_host(PGPUDTALPHA p);
 void alpha_release(AlphaDev &a);

@@ -75,6 +75,13 @@ struct galpha_ctx {
     DevBuf tv, tn, remap, stamp, claim, modA, modB, cand, pos, vt, vstate, pix, deg, off, fill, fan,
         moved, decide, mx, mlist, loc, lt, scratch, hnext, hedge, keys, keys2, vals, vals2, rot, newid,
         out9, counters, status, xy, scanTmp, sortTmp, tileCnt, tileOff, dellist, dbg;
+    // batch plan on the device
+    DevBuf setsD, bmD, vset, hullCnt, hullOff, hullStart, setFirst, offsD, bboxTmp;
+    std::vector<int> set_first;       // [nsets + 1] first output triangle of each planned set
+    std::vector<SetDesc> plan_sets;   // host copy of the last plan
+    int plan_ntot = 0;
+    std::vector<int> batch_sets;      // input set of each planned set of the last batch
+    std::vector<int> batch_npts;      // points of every input set of the last batch
     AlphaDev alpha;
     int last_n = 0, last_F = 0, last_T = 0;
     bool last_alpha = false;
@@ -172,55 +179,145 @@ int galphaAutoTexture(int n)
     return T;
 }
 
-static void ensure_pba(galpha_ctx *c, int T, int n, int nf)
+// ---- batch plan: per-set descriptors, block maps and frame pixels ----------------------------
+struct BatchPlan {
+    std::vector<SetDesc> sets;
+    std::vector<int> bm;              // BM_COUNT x (nsets + 1)
+    int blocks[BM_COUNT] = {0};
+    std::vector<int> fpix;            // frame pixels of all sets, in set order
+    int ntot = 0, V = 0, sumT = 0;
+    size_t P = 0;
+    int nsets() const { return (int)sets.size(); }
+};
+
+// n[s] points (>= 3), texture side T[s], box bb5[5 s .. 5 s + 4] (min x, min y, max x, max y, bad)
+static BatchPlan plan_batch(const int *n, const int *T, const double *bb5, int nsets)
 {
-    size_t P = (size_t)T * T;
+    BatchPlan b;
+    b.sets.resize(nsets);
+    long long poff = 0, nftot = 0, roff = 0;
+    size_t toff = 0;
+    std::vector<int> nfs(nsets);
+    for (int s = 0; s < nsets; s++) {
+        check_texture(T[s]);
+        if (bb5[5 * s + 4] != 0.0) throw GalphaError("non-finite input coordinate");
+        SnapGrid sg = snap_grid(bb5[5 * s], bb5[5 * s + 1], bb5[5 * s + 2], bb5[5 * s + 3], T[s]);
+        std::vector<int> fp = frame_pixels(T[s]);
+        SetDesc &d = b.sets[s];
+        d.toff = toff;
+        d.logT = log2i(T[s]);
+        d.n = n[s];
+        d.poff = (int)poff;
+        d.nf = (int)fp.size();
+        d.roff = (int)roff;
+        d.m2 = d.m3 = std::min(16, T[s] / 64);
+        d.margin = margin_for(T[s]);
+        d.shx = sg.shx; d.shy = sg.shy; d.scale = sg.scale; d.inv = sg.inv;
+        nfs[s] = d.nf;
+        b.fpix.insert(b.fpix.end(), fp.begin(), fp.end());
+        poff += n[s];
+        nftot += d.nf;
+        roff += T[s];
+        toff += (size_t)T[s] * T[s];
+    }
+    if (poff + nftot > 0x7fffffffLL / 4) throw GalphaError("batch too large (vertex ids)");
+    b.ntot = (int)poff;
+    b.V = (int)(poff + nftot);
+    b.sumT = (int)roff;
+    b.P = toff;
+    long long fb = poff;
+    for (int s = 0; s < nsets; s++) { b.sets[s].fbase = (int)fb; fb += nfs[s]; }
+    b.bm.assign((size_t)BM_COUNT * (nsets + 1), 0);
+    long long acc[BM_COUNT] = {0};
+    for (int s = 0; s <= nsets; s++) {
+        for (int k = 0; k < BM_COUNT; k++) {
+            if (acc[k] > 0x7fffffffLL) throw GalphaError("batch too large (launch blocks)");
+            b.bm[(size_t)k * (nsets + 1) + s] = (int)acc[k];
+        }
+        if (s == nsets) break;
+        long long Ts = T[s], m = std::min(16, T[s] / 64);
+        acc[BM_COLS] += Ts / 4;
+        acc[BM_STACK] += (Ts / 64) * m;
+        acc[BM_MERGE] += Ts / 64;
+        acc[BM_COLOR] += (Ts / 64) * m;
+        acc[BM_PIX] += Ts * Ts / 256;
+        acc[BM_RECON] += (Ts - 1) * ((Ts - 1 + 255) / 256);
+    }
+    for (int k = 0; k < BM_COUNT; k++) b.blocks[k] = (int)acc[k];
+    return b;
+}
+
+static void ensure_pba(galpha_ctx *c, const BatchPlan &b)
+{
+    size_t P = b.P;
     c->smap.ensure(4 * P);
     c->g.ensure(2 * P);
     c->S.ensure(4 * P);
-    c->cnt.ensure(4 * (size_t)64 * T);
+    c->cnt.ensure(4 * (size_t)16 * b.sumT);
     c->E.ensure(4 * P);
-    c->ecnt.ensure(4 * (size_t)T);
+    c->ecnt.ensure(4 * (size_t)b.sumT);
     c->vor.ensure(4 * P);
-    c->bad.ensure(4 * (size_t)(n + nf + 1));
-    c->fpix.ensure(4 * (size_t)(nf + 1));
-    c->pix.ensure(4 * (size_t)(n + nf + 1));
+    c->bad.ensure(4 * (size_t)(b.V + 1));
+    c->fpix.ensure(4 * (size_t)(b.V - b.ntot + 1));
+    c->pix.ensure(4 * (size_t)(b.V + 1));
+    c->vset.ensure(4 * (size_t)(b.V + 1));
     c->counters.ensure(sizeof(int) * C_NCOUNTERS);
     c->status.ensure(sizeof(int) * 4);
-    size_t ntiles = (size_t)(T - 1) * ((T - 1 + 255) / 256);
+    size_t ntiles = (size_t)std::max(b.blocks[BM_RECON], 1);
     c->tileCnt.ensure(4 * ntiles);
     c->tileOff.ensure(4 * ntiles);
-    c->scanTmp.ensure(scan_temp_bytes(std::max(ntiles, (size_t)(n + nf + 2))) + 4096);
+    c->scanTmp.ensure(scan_temp_bytes(std::max(ntiles, (size_t)(b.V + 2))) + 4096);
+    c->setsD.ensure(sizeof(SetDesc) * b.sets.size());
+    c->bmD.ensure(sizeof(int) * b.bm.size());
+    int ns = b.nsets();
+    c->hullCnt.ensure(4 * (size_t)ns + 16);
+    c->hullOff.ensure(4 * (size_t)ns + 16);
+    c->hullStart.ensure(4 * (size_t)ns + 16);
+    c->setFirst.ensure(4 * (size_t)ns + 16);
 }
 
-static void run_pba(galpha_ctx *c, const double *d_xy, int n, int T, const SnapGrid &sg, int nf, int *iters,
-                    int *island_sites)
+// upload the plan; returns the device view
+static BatchDev upload_plan(galpha_ctx *c, const BatchPlan &b)
 {
     hipStream_t st = c->stream;
-    int logT = log2i(T);
-    size_t P = (size_t)T * T;
-    launch_fill_i32(c->smap.as<int>(), 0x7fffffff, P, st);
-    launch_splat((const double2 *)d_xy, n, sg.shx, sg.shy, sg.inv, margin_for(T), logT, c->smap.as<int>(),
-                 c->pix.as<int>(), c->fpix.as<int>(), nf, st);
-    HIPCHK(hipMemsetAsync(c->bad.p, 0, 4 * (size_t)(n + nf + 1), st));
+    HIPCHK(hipMemcpyAsync(c->setsD.p, b.sets.data(), sizeof(SetDesc) * b.sets.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->bmD.p, b.bm.data(), sizeof(int) * b.bm.size(), hipMemcpyHostToDevice, st));
+    if (!b.fpix.empty())
+        HIPCHK(hipMemcpyAsync(c->fpix.p, b.fpix.data(), 4 * b.fpix.size(), hipMemcpyHostToDevice, st));
+    BatchDev d{};
+    d.sets = c->setsD.as<SetDesc>();
+    d.nsets = b.nsets();
+    d.bm = c->bmD.as<int>();
+    for (int k = 0; k < BM_COUNT; k++) d.blocks[k] = b.blocks[k];
+    d.vset = c->vset.as<int>();
+    d.ntot = b.ntot;
+    d.V = b.V;
+    d.P = b.P;
+    return d;
+}
+
+static void run_pba(galpha_ctx *c, const double *d_xy, const BatchDev &bd, int *iters, int *island_sites)
+{
+    hipStream_t st = c->stream;
+    launch_fill_i32(c->smap.as<int>(), 0x7fffffff, bd.P, st);
+    launch_vset(bd, c->vset.as<int>(), st);
+    launch_splat((const double2 *)d_xy, bd, c->smap.as<int>(), c->pix.as<int>(), c->fpix.as<int>(), st);
+    HIPCHK(hipMemsetAsync(c->bad.p, 0, 4 * (size_t)(bd.V + 1), st));
     PbaBuffers b{c->smap.as<int>(), c->g.as<short>(), c->S.as<int>(), c->cnt.as<int>(),
                  c->E.as<int>(), c->ecnt.as<int>(), c->vor.as<int>()};
-    int m2 = std::min(16, T / 64), m3 = std::min(16, T / 64);
-    int it = 0, removed = 0;
+    int it = 0;
+    ctx_counter_set(c, C_ISLAND_SITES, 0);
     for (; it < 16; it++) {
-        launch_pba(b, logT, m2, m3, st);
+        launch_pba(b, bd, st);
         ctx_counter_set(c, C_FLAG, 0);
-        launch_islands(c->vor.as<int>(), c->pix.as<int>(), logT, c->bad.as<int>(), c->counters.as<int>() + C_FLAG, st);
+        launch_islands(c->vor.as<int>(), c->pix.as<int>(), bd, c->bad.as<int>(), c->counters.as<int>() + C_FLAG, st);
         if (ctx_counter(c, C_FLAG) == 0) break;
-        std::vector<int> hb(n);
-        HIPCHK(hipMemcpyAsync(hb.data(), c->bad.p, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        for (int i = 0; i < n; i++) removed += hb[i] != 0;
-        launch_remove_bad(c->bad.as<int>(), c->pix.as<int>(), n, logT, c->smap.as<int>(), st);
+        launch_remove_bad(c->bad.as<int>(), c->pix.as<int>(), bd, c->smap.as<int>(),
+                          c->counters.as<int>() + C_ISLAND_SITES, st);
     }
     if (it == 16) throw GalphaError("discrete Voronoi diagram did not settle (island certificate)");
     *iters = it + 1;
-    *island_sites = removed;
+    *island_sites = ctx_counter(c, C_ISLAND_SITES);
 }
 
 static float ev_ms(galpha_ctx *c, int a, int b)
@@ -230,41 +327,42 @@ static float ev_ms(galpha_ctx *c, int a, int b)
     return ms;
 }
 
-static void bbox_host(const double *xy, int n, double *mnx, double *mny, double *mxx, double *mxy)
+// box of host points into bb5 (min x, min y, max x, max y, non-finite flag)
+static void bbox_host(const double *xy, int n, double *bb5)
 {
-    double a = INFINITY, b = INFINITY, cc = -INFINITY, d = -INFINITY;
+    double a = INFINITY, b = INFINITY, cc = -INFINITY, d = -INFINITY, bad = 0.0;
     for (int i = 0; i < n; i++) {
         double x = xy[2 * i], y = xy[2 * i + 1];
-        if (!std::isfinite(x) || !std::isfinite(y)) throw GalphaError("non-finite input coordinate");
+        if (!std::isfinite(x) || !std::isfinite(y)) bad = 1.0;
         a = std::min(a, x); b = std::min(b, y); cc = std::max(cc, x); d = std::max(d, y);
     }
-    *mnx = a; *mny = b; *mxx = cc; *mxy = d;
+    bb5[0] = a; bb5[1] = b; bb5[2] = cc; bb5[3] = d; bb5[4] = bad;
 }
 
-void bbox_device(const double *d_xy, int n, double *out4, void *tmp, hipStream_t st);
+void bbox_device(const double *d_xy, int n, double *out5, void *tmp, hipStream_t st);
+void bbox_sets_device(const double *d_xy, const int *d_offs, int nsets, double *out5, void *tmp, hipStream_t st);
 
-// Delaunay triangulation of d_xy (device).  Leaves canonical triangles in c->out9 (9 ints each).
-static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double *bbox)
+// Delaunay triangulation of every set of the plan (points in d_xy, concatenated in set order).
+// Leaves the canonical triangles of all sets in c->out9 (9 ints each, global vertex ids and
+// global triangle positions, set-major) and the first triangle of each set in c->set_first.
+static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
 {
-    check_texture(T);
     hipStream_t st = c->stream;
     galpha_stats &S = c->stats;
     memset(&S, 0, sizeof(S));
-    c->last_n = n;
-    c->last_T = T;
+    int nsets = plan.nsets();
     c->last_F = 0;
-    if (n < 3) return 0;
-    int logT = log2i(T);
-    SnapGrid sg = snap_grid(bbox[0], bbox[1], bbox[2], bbox[3], T);
-    std::vector<int> fp = frame_pixels(T);
-    int nf = (int)fp.size();
-    int V = n + nf;
-    ensure_pba(c, T, n, nf);
-    HIPCHK(hipMemcpyAsync(c->fpix.p, fp.data(), 4 * (size_t)nf, hipMemcpyHostToDevice, st));
+    c->set_first.assign(nsets + 1, 0);
+    c->plan_sets = plan.sets;
+    c->plan_ntot = plan.ntot;
+    if (nsets == 0) return 0;
+    int V = plan.V, n = plan.ntot;
+    ensure_pba(c, plan);
     HIPCHK(hipMemsetAsync(c->counters.p, 0, sizeof(int) * C_NCOUNTERS, st));
     HIPCHK(hipMemsetAsync(c->status.p, 0, sizeof(int) * 4, st));
+    BatchDev bd = upload_plan(c, plan);
     HIPCHK(hipEventRecord(c->ev[0], st));
-    run_pba(c, d_xy, n, T, sg, nf, &S.pbaIterations, &S.islandSites);
+    run_pba(c, d_xy, bd, &S.pbaIterations, &S.islandSites);
     HIPCHK(hipEventRecord(c->ev[1], st));
 
     // ---- mesh buffers
@@ -306,7 +404,8 @@ static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double 
     m.cap = cap; m.ntri = c->counters.as<int>() + 40;   // slot 40 holds the slot count
     m.pos = c->pos.as<double2>(); m.xy = (const double2 *)d_xy; m.vt = c->vt.as<int>();
     m.vstate = c->vstate.as<int>(); m.pix = c->pix.as<int>();
-    m.n = n; m.V = V; m.shx = sg.shx; m.shy = sg.shy; m.scale = sg.scale; m.margin = margin_for(T);
+    m.n = n; m.V = V;
+    m.sets = bd.sets; m.vset = bd.vset; m.nsets = nsets;
     m.status = c->status.as<int>(); m.counters = c->counters.as<int>();
     m.modA = c->modA.as<int>(); m.modB = c->modB.as<int>(); m.cand = c->cand.as<int>();
     m.scratch = c->scratch.as<int>(); m.scratch_cap = scap;
@@ -318,7 +417,7 @@ static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double 
     set_hull_buffers(c->hnext.as<int>(), c->hedge.as<int>());
 
     // ---- reconstruction + adjacency
-    launch_recon(c->vor.as<int>(), c->pix.as<int>(), logT, m, c->tileCnt.as<int>(), c->tileOff.as<int>(),
+    launch_recon(c->vor.as<int>(), c->pix.as<int>(), bd, m, c->tileCnt.as<int>(), c->tileOff.as<int>(),
                  c->scanTmp.p, st);
     launch_adjacency(m, c->deg.as<int>(), c->off.as<int>(), c->fill.as<int>(), c->fan.as<int>(), c->scanTmp.p, st);
     set_deg_for_init(c->deg.as<int>());
@@ -331,7 +430,6 @@ static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double 
         if (stt) throw GalphaError("reconstruction produced an invalid triangle (status " + std::to_string(stt) + ")");
     }
     HIPCHK(hipEventRecord(c->ev[2], st));
-
     // ---- shifting
     // revert moved vertices of non-ccw triangles until every triangle is ccw (monotone)
     for (int it = 0;; it++) {
@@ -412,7 +510,7 @@ static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double 
 
     // ---- missing sites
     ctx_counter_set(c, C_MISS, 0);
-    launch_missing_init(m, c->vor.as<int>(), logT, c->mlist.as<int>(), c->loc.as<int>(), st);
+    launch_missing_init(m, c->vor.as<int>(), c->mlist.as<int>(), c->loc.as<int>(), st);
     int nmiss = ctx_counter(c, C_MISS);
     S.missing = nmiss;
     for (int r = 0; r < 100000; r++) {
@@ -438,7 +536,7 @@ static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double 
     HIPCHK(hipEventRecord(c->ev[5], st));
 
     // ---- frame removal + hull convexification
-    ctx_counter_set(c, C_FRAME, nf);
+    ctx_counter_set(c, C_FRAME, V - n);
     for (int r = 0;; r++) {
         if (r > 100000) throw GalphaError("frame removal did not finish");
         ctx_counter_set(c, C_SCR, 0);
@@ -448,7 +546,8 @@ static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double 
         if (left <= 0) break;
     }
     ctx_counter_set(c, C_MODN, 0);
-    launch_convexify(m, m.modA, C_MODN, c->op++, st);
+    launch_convexify(m, m.modA, C_MODN, c->op++, c->hullCnt.as<int>(), c->hullOff.as<int>(),
+                     c->hullStart.as<int>(), c->scanTmp.p, st);
     HIPCHK(hipEventRecord(c->ev[6], st));
     flip_all();
     HIPCHK(hipEventRecord(c->ev[7], st));
@@ -467,20 +566,35 @@ static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double 
     c->rot.ensure(4 * (size_t)nslots + 64);
     c->newid.ensure(4 * (size_t)nslots + 64);
     ctx_counter_set(c, C_RECON_TRIS, 0);
-    launch_canon_keys(m, c->keys.as<unsigned long long>(), c->vals.as<int>(), c->rot.as<int>(), st);
+    // one 64-bit key (v0, v1, v2) needs ids below 2^21; otherwise two stable passes
+    int wide = (nsets > 1 || n > (1 << 21)) ? 1 : 0;
+    unsigned long long *K = c->keys.as<unsigned long long>(), *K2 = c->keys2.as<unsigned long long>();
+    int *vals = c->vals.as<int>(), *vals2 = c->vals2.as<int>();
+    launch_canon_keys(m, K, vals, c->rot.as<int>(), 1, wide, st);
     size_t tb = 0;
-    sort_pairs_u64(c->keys.as<unsigned long long>(), c->vals.as<int>(), c->keys2.as<unsigned long long>(),
-                   c->vals2.as<int>(), nslots, 0, 64, nullptr, &tb, st);
+    sort_pairs_u64(K, vals, K2, vals2, nslots, 0, 64, nullptr, &tb, st);
     c->sortTmp.ensure(tb + 256);
     tb = c->sortTmp.bytes;
-    int se = sort_pairs_u64(c->keys.as<unsigned long long>(), c->vals.as<int>(), c->keys2.as<unsigned long long>(),
-                            c->vals2.as<int>(), nslots, 0, 64, c->sortTmp.p, &tb, st);
-    if (se) throw GalphaError("radix sort failed");
+    if (sort_pairs_u64(K, vals, K2, vals2, nslots, 0, 64, c->sortTmp.p, &tb, st)) throw GalphaError("radix sort failed");
+    const int *order = vals2;
+    if (wide) {
+        launch_canon_keys(m, K, vals2, c->rot.as<int>(), 2, wide, st);
+        tb = c->sortTmp.bytes;
+        if (sort_pairs_u64(K, vals2, K2, vals, nslots, 0, 64, c->sortTmp.p, &tb, st))
+            throw GalphaError("radix sort failed");
+        order = vals;
+    }
     int F = ctx_counter(c, C_RECON_TRIS);
     c->out9.ensure(36 * (size_t)F + 64);
-    launch_canon_emit(m, c->vals2.as<int>(), c->rot.as<int>(), c->newid.as<int>(), F, c->out9.as<int>(), st);
+    launch_canon_emit(m, order, c->rot.as<int>(), c->newid.as<int>(), F, c->out9.as<int>(),
+                      c->setFirst.as<int>(), st);
     HIPCHK(hipEventRecord(c->ev[8], st));
+    std::vector<int> sf(nsets);
+    HIPCHK(hipMemcpyAsync(sf.data(), c->setFirst.p, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    // sets without triangles start where the next set starts
+    c->set_first[nsets] = F;
+    for (int s = nsets - 1; s >= 0; s--) c->set_first[s] = sf[s] >= 0 ? sf[s] : c->set_first[s + 1];
     c->last_F = F;
     S.nTris = F;
     int cnts[C_NCOUNTERS];
@@ -496,6 +610,99 @@ static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double 
     S.ms_flip = ev_ms(c, 6, 7);
     S.ms_canon = ev_ms(c, 7, 8);
     S.ms_total = ev_ms(c, 0, 8);
+    return F;
+}
+
+// one set as a batch of one (fewer than 3 points: no triangles)
+static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double *bb5)
+{
+    check_texture(T);
+    c->last_n = n;
+    c->last_T = T;
+    if (n < 3) {
+        memset(&c->stats, 0, sizeof(c->stats));
+        c->last_F = 0;
+        c->set_first.assign(1, 0);
+        c->plan_sets.clear();
+        c->plan_ntot = 0;
+        return 0;
+    }
+    BatchPlan plan = plan_batch(&n, &T, bb5, 1);
+    return run_batch(c, d_xy, plan);
+}
+
+// the last plan as the alpha stage sees it
+static AlphaBatchInfo batch_info(galpha_ctx *c)
+{
+    AlphaBatchInfo info;
+    int ns = (int)c->plan_sets.size();
+    info.nsets = std::max(ns, 1);
+    info.vset = ns > 1 ? c->vset.as<int>() : nullptr;
+    info.poff.resize(info.nsets + 1, 0);
+    info.tfirst.resize(info.nsets + 1, 0);
+    for (int s = 0; s < ns; s++) info.poff[s] = c->plan_sets[s].poff;
+    info.poff[info.nsets] = c->plan_ntot;
+    for (int s = 0; s <= ns; s++) info.tfirst[s] = c->set_first[s];
+    if (ns == 0) info.tfirst[1] = 0;
+    return info;
+}
+
+// Runs the pipeline over nsets host point sets (xy concatenated, offsets[nsets + 1] in points,
+// fboSizes[s] <= 0 or NULL: autotuned side).  Sets with fewer than 3 points are left out of the
+// plan (no triangles); c->batch_sets maps planned sets to input sets.  Returns the total number
+// of triangles.
+static int batch_run(galpha_ctx *c, const double *xy, const int *offsets, int nsets, const int *fboSizes,
+                     int with_alpha)
+{
+    hipStream_t st = c->stream;
+    c->batch_sets.clear();
+    c->batch_npts.assign(nsets, 0);
+    std::vector<int> ns, Ts;
+    long long total = 0;
+    for (int s = 0; s < nsets; s++) {
+        int n = offsets[s + 1] - offsets[s];
+        if (n < 0) throw GalphaError("offsets must be non-decreasing");
+        c->batch_npts[s] = n;
+        if (n < 3) continue;
+        if (!xy) throw GalphaError("null point array");
+        int T = (fboSizes && fboSizes[s] > 0) ? fboSizes[s] : galphaAutoTexture(n);
+        c->batch_sets.push_back(s);
+        ns.push_back(n);
+        Ts.push_back(T);
+        total += n;
+    }
+    if (total > 0x7fffffffLL / 8) throw GalphaError("batch too large");
+    int na = (int)ns.size();
+    c->xy.ensure(16 * (size_t)std::max<long long>(total, 1));
+    std::vector<int> poffs(na + 1, 0);
+    for (int k = 0; k < na; k++) poffs[k + 1] = poffs[k] + ns[k];
+    if (na == nsets && nsets > 0) {
+        HIPCHK(hipMemcpyAsync(c->xy.p, xy + 2 * (size_t)offsets[0], 16 * (size_t)total, hipMemcpyHostToDevice, st));
+    } else {
+        for (int k = 0; k < na; k++)
+            HIPCHK(hipMemcpyAsync(c->xy.as<double>() + 2 * (size_t)poffs[k], xy + 2 * (size_t)offsets[c->batch_sets[k]],
+                                  16 * (size_t)ns[k], hipMemcpyHostToDevice, st));
+    }
+    std::vector<double> bb5((size_t)5 * std::max(na, 1));
+    if (na > 0) {
+        c->offsD.ensure(4 * (size_t)(na + 1));
+        c->bboxTmp.ensure(8 * 5 * (size_t)na);
+        HIPCHK(hipMemcpyAsync(c->offsD.p, poffs.data(), 4 * (size_t)(na + 1), hipMemcpyHostToDevice, st));
+        bbox_sets_device(c->xy.as<double>(), c->offsD.as<int>(), na, bb5.data(), c->bboxTmp.p, st);
+    }
+    BatchPlan plan = plan_batch(ns.data(), Ts.data(), bb5.data(), na);
+    int F = run_batch(c, c->xy.as<double>(), plan);
+    c->last_n = (int)total;
+    c->last_alpha = false;
+    if (with_alpha) {
+        auto t0 = std::chrono::steady_clock::now();
+        alpha_run_batch(c->alpha, c->xy.as<double>(), plan.ntot, c->out9.as<int>(), F, batch_info(c), st,
+                        &c->stats);
+        c->last_alpha = true;
+        HIPCHK(hipStreamSynchronize(st));
+        auto t1 = std::chrono::steady_clock::now();
+        c->stats.ms_alpha = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    }
     return F;
 }
 
@@ -525,7 +732,9 @@ static void release_all(galpha_ctx *c)
                       &c->vstate, &c->pix, &c->deg, &c->off, &c->fill, &c->fan, &c->moved, &c->decide, &c->mx,
                       &c->mlist, &c->loc, &c->lt, &c->scratch, &c->hnext, &c->hedge, &c->keys, &c->keys2,
                       &c->vals, &c->vals2, &c->rot, &c->newid, &c->out9, &c->counters, &c->status, &c->xy,
-                      &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg};
+                      &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
+                      &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
+                      &c->offsD, &c->bboxTmp};
     for (DevBuf *b : bufs) b->release();
     alpha_release(c->alpha);
 }
@@ -558,10 +767,10 @@ int galphaComputeDevice(galpha_ctx *c, const double *d_xy, int n, int fboSize, i
         HIPCHK(hipSetDevice(c->device));
         hipStream_t own = c->stream;
         if (stream) c->stream = (hipStream_t)stream;
-        double bb[4];
+        double bb[5] = {0, 0, 1, 1, 0};
         if (n > 0) {
-            c->scanTmp.ensure(4096);
-            bbox_device(d_xy, n, bb, c->scanTmp.p, c->stream);
+            c->bboxTmp.ensure(8 * 5 * 64);
+            bbox_device(d_xy, n, bb, c->bboxTmp.p, c->stream);
         }
         int F = run_dt(c, d_xy, n, fboSize, bb);
         c->last_alpha = false;
@@ -613,8 +822,8 @@ int galphaComputeHost(galpha_ctx *c, const double *xy, int n, int fboSize, int *
     if (!c) return -1;
     try {
         HIPCHK(hipSetDevice(c->device));
-        double bb[4] = {0, 0, 1, 1};
-        if (n > 0) bbox_host(xy, n, &bb[0], &bb[1], &bb[2], &bb[3]);
+        double bb[5] = {0, 0, 1, 1, 0};
+        if (n > 0) bbox_host(xy, n, bb);
         c->xy.ensure(16 * (size_t)std::max(n, 1));
         if (n > 0) HIPCHK(hipMemcpyAsync(c->xy.p, xy, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
         int F = run_dt(c, c->xy.as<double>(), n, fboSize, bb);
@@ -636,19 +845,19 @@ int galphaVoronoi(galpha_ctx *c, const double *xy, int n, int T, int *labels, in
     try {
         HIPCHK(hipSetDevice(c->device));
         check_texture(T);
-        double bb[4] = {0, 0, 1, 1};
-        if (n > 0) bbox_host(xy, n, &bb[0], &bb[1], &bb[2], &bb[3]);
-        SnapGrid sg = snap_grid(bb[0], bb[1], bb[2], bb[3], T);
-        std::vector<int> fp = frame_pixels(T);
-        int nf = (int)fp.size();
+        if (n < 1) throw GalphaError("no points");
+        double bb[5];
+        bbox_host(xy, n, bb);
+        BatchPlan plan = plan_batch(&n, &T, bb, 1);
+        int nf = plan.V - plan.ntot;
         if (pix_capacity < n + nf) throw GalphaError("pix capacity too small");
-        ensure_pba(c, T, n, nf);
-        c->xy.ensure(16 * (size_t)std::max(n, 1));
+        ensure_pba(c, plan);
+        c->xy.ensure(16 * (size_t)n);
         HIPCHK(hipMemcpyAsync(c->xy.p, xy, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(c->fpix.p, fp.data(), 4 * (size_t)nf, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemsetAsync(c->counters.p, 0, sizeof(int) * C_NCOUNTERS, c->stream));
+        BatchDev bd = upload_plan(c, plan);
         int iters = 0, isl = 0;
-        run_pba(c, c->xy.as<double>(), n, T, sg, nf, &iters, &isl);
+        run_pba(c, c->xy.as<double>(), bd, &iters, &isl);
         c->stats.pbaIterations = iters;
         c->stats.islandSites = isl;
         HIPCHK(hipMemcpyAsync(labels, c->vor.p, 4 * (size_t)T * T, hipMemcpyDeviceToHost, c->stream));
@@ -741,41 +950,62 @@ void gpudtReleaseAlpha(PGPUDTALPHA a) { alpha_free_host(a); }
 
 void galphaFree(void *p) { free(p); }
 
-// Batched entry point: every set is triangulated (and optionally gets its alpha spectrum) on
-// this context; results are concatenated.  tris9_out / tri_offsets_out are malloc'ed (release
-// with galphaFree).
+// Batched entry point: all sets go through one pass of the pipeline on this context (every launch
+// covers every set).  Results are concatenated per input set with set-local vertex and triangle
+// indices.  tris9_out / tri_offsets_out are malloc'ed (release with galphaFree).
 int galphaBatch(galpha_ctx *c, const double *xy, const int *offsets, int nsets, const int *fboSizes,
                 int with_alpha, int **tris9_out, int **tri_offsets_out)
 {
-    if (!c || !xy || !offsets || nsets < 0 || !tris9_out || !tri_offsets_out) return -1;
+    if (!c || !offsets || nsets < 0 || !tris9_out || !tri_offsets_out) return -1;
     try {
         HIPCHK(hipSetDevice(c->device));
+        int F = batch_run(c, xy, offsets, nsets, fboSizes, with_alpha);
+        std::vector<int> all((size_t)9 * F);
+        if (F > 0) ctx_read(c, all.data(), c->out9.p, 36 * (size_t)F);
         std::vector<int> toff(nsets + 1, 0);
-        std::vector<int> all;
-        for (int s = 0; s < nsets; s++) {
-            int b = offsets[s], e = offsets[s + 1];
-            if (e < b) throw GalphaError("offsets must be non-decreasing");
-            int n = e - b;
-            int T = (fboSizes && fboSizes[s] > 0) ? fboSizes[s] : galphaAutoTexture(n);
-            double bb[4] = {0, 0, 1, 1};
-            if (n > 0) bbox_host(xy + 2 * (size_t)b, n, &bb[0], &bb[1], &bb[2], &bb[3]);
-            c->xy.ensure(16 * (size_t)std::max(n, 1));
-            if (n > 0)
-                HIPCHK(hipMemcpyAsync(c->xy.p, xy + 2 * (size_t)b, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
-            int F = run_dt(c, c->xy.as<double>(), n, T, bb);
-            if (with_alpha) alpha_run(c->alpha, c->xy.as<double>(), n, c->out9.as<int>(), F, c->stream, &c->stats);
-            size_t old = all.size();
-            all.resize(old + 9 * (size_t)F);
-            if (F > 0)
-                HIPCHK(hipMemcpyAsync(all.data() + old, c->out9.p, 36 * (size_t)F, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
-            toff[s + 1] = toff[s] + F;
+        int na = (int)c->batch_sets.size();
+        std::vector<int> ntri(nsets, 0);
+        for (int k = 0; k < na; k++) ntri[c->batch_sets[k]] = c->set_first[k + 1] - c->set_first[k];
+        for (int s = 0; s < nsets; s++) toff[s + 1] = toff[s] + ntri[s];
+        // planned sets are in input order, so the concatenation order already matches
+        for (int k = 0; k < na; k++) {
+            int t0 = c->set_first[k], t1 = c->set_first[k + 1];
+            int poff = c->plan_sets[k].poff;
+            for (int t = t0; t < t1; t++) {
+                int *o = all.data() + (size_t)9 * t;
+                for (int i = 0; i < 3; i++) {
+                    o[3 + i] -= poff;
+                    if (o[i] >= 0) o[i] = ((o[i] >> 2) - t0) << 2 | (o[i] & 3);
+                }
+            }
         }
         *tris9_out = (int *)malloc(sizeof(int) * std::max<size_t>(all.size(), 1));
         *tri_offsets_out = (int *)malloc(sizeof(int) * (nsets + 1));
+        if (!*tris9_out || !*tri_offsets_out) throw GalphaError("host allocation failed");
         memcpy(*tris9_out, all.data(), sizeof(int) * all.size());
         memcpy(*tri_offsets_out, toff.data(), sizeof(int) * (nsets + 1));
         return nsets;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+// alpha result of input set `set` of the last galphaBatch(with_alpha = 1) call
+int galphaBatchCopyAlpha(galpha_ctx *c, int set, PGPUDTALPHA *pA)
+{
+    if (!c || !pA) return -1;
+    try {
+        if (!c->last_alpha) throw GalphaError("no alpha result on this context");
+        int k = -1;
+        for (int q = 0; q < (int)c->batch_sets.size(); q++)
+            if (c->batch_sets[q] == set) { k = q; break; }
+        if (k < 0) {
+            // a set with fewer than 3 points: no simplices beyond its vertices
+            *pA = alpha_empty_host(c->batch_npts.size() > (size_t)set ? c->batch_npts[set] : 0);
+            return 0;
+        }
+        *pA = alpha_copy_set_to_host(c->alpha, k, c->stream);
+        return 0;
     } catch (const std::exception &e) {
         return fail(c, e);
     }

@@ -4,6 +4,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#define GA_INLINE_HOST_DEVICE __host__ __device__ __forceinline__
+
 void ga_probe_begin(const char *kernel, hipStream_t st);
 void ga_probe_end(const char *kernel, hipStream_t st);
 // kernel launch with the optional timing probe around it
@@ -13,6 +15,46 @@ void ga_probe_end(const char *kernel, hipStream_t st);
         hipLaunchKernelGGL(K, G, B, SH, ST, __VA_ARGS__);    \
         ga_probe_end(#K, ST);                                \
     } while (0)
+
+// One point set of a batch.  Global vertex ids: the real points of all sets first (set s at
+// [poff, poff + n)), then the frame vertices of all sets (set s at [fbase, fbase + nf)), so that
+// inside a set the index order of the single-set layout is kept (real points before frame
+// vertices, each in their own order).  Textures of all sets are concatenated in one pixel array.
+struct SetDesc {
+    size_t toff;         // first pixel of the set's texture
+    int logT;            // texture side T = 1 << logT
+    int n, poff;         // real points and the global id of the first one
+    int nf, fbase;       // frame vertices and the global id of the first one
+    int roff;            // sum of the texture sides of the previous sets (row / column arrays)
+    int m2, m3;          // PBA bands for phases 2 and 3
+    int margin;
+    double shx, shy, scale, inv;   // snap grid: snapped = sh + (pixel - margin) * scale
+};
+
+// block ranges of one launch over all sets: set s owns blocks [start[s], start[s + 1])
+GA_INLINE_HOST_DEVICE int set_of(const int *start, int nsets, int b)
+{
+    int lo = 0, hi = nsets - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// per-launch block maps (device arrays of nsets + 1 entries each)
+enum { BM_COLS, BM_STACK, BM_MERGE, BM_COLOR, BM_PIX, BM_RECON, BM_COUNT };
+struct BatchDev {
+    const SetDesc *sets;   // [nsets]
+    int nsets;
+    const int *bm;         // [BM_COUNT][nsets + 1]
+    int blocks[BM_COUNT];  // total blocks per map (host copy of bm[k][nsets])
+    const int *vset;       // [V] set of every vertex
+    int ntot, V;           // real points, all vertices
+    size_t P;              // pixels of all textures
+    GA_INLINE_HOST_DEVICE const int *bmap(int k) const { return bm + (size_t)k * (nsets + 1); }
+};
 
 struct PbaBuffers {
     int *smap;     // T*T site id per pixel (0x7fffffff = empty)
@@ -38,9 +80,10 @@ struct MeshDev {
     int *vt;             // vertex -> an incident triangle [V]
     int *vstate;         // vertex state [V]
     int *pix;            // vertex pixel (px << 16 | py) [V]
-    int n, V;            // real points, all vertices (real + frame)
-    double shx, shy, scale;
-    int margin;
+    int n, V;            // real points of all sets, all vertices (real + frame)
+    const SetDesc *sets; // per set (snap grid, frame range, texture)
+    const int *vset;     // [V] set of every vertex
+    int nsets;
     int *status;         // device error bits
     int *counters;       // device scratch counters [64]
     int *modA, *modB;    // modified-triangle lists (capacity cap)
@@ -84,18 +127,18 @@ struct MeshDev {
 
 // pba.hip
 void launch_fill_i32(int *a, int v, size_t n, hipStream_t st);
-void launch_splat(const double2 *xy, int n, double shx, double shy, double inv, int margin, int logT,
-                  int *smap, int *pix, const int *fpix, int nf, hipStream_t st);
-void launch_pba(const PbaBuffers &b, int logT, int m2, int m3, hipStream_t st);
-void launch_islands(const int *vor, const int *pix, int logT, int *bad, int *flag, hipStream_t st);
-void launch_remove_bad(int *bad, const int *pix, int n, int logT, int *smap, hipStream_t st);
+void launch_vset(const BatchDev &b, int *vset, hipStream_t st);
+void launch_splat(const double2 *xy, const BatchDev &b, int *smap, int *pix, const int *fpix, hipStream_t st);
+void launch_pba(const PbaBuffers &p, const BatchDev &b, hipStream_t st);
+void launch_islands(const int *vor, const int *pix, const BatchDev &b, int *bad, int *flag, hipStream_t st);
+void launch_remove_bad(int *bad, const int *pix, const BatchDev &b, int *smap, int *removed, hipStream_t st);
 
 // scan.hip
 size_t scan_temp_bytes(size_t n);
 void launch_exclusive_scan(const int *in, int *out, size_t n, void *temp, hipStream_t st);
 
 // mesh.hip
-void launch_recon(const int *vor, const int *pix, int logT, MeshDev &m, int *tile_cnt, int *tile_off,
+void launch_recon(const int *vor, const int *pix, const BatchDev &b, MeshDev &m, int *tile_cnt, int *tile_off,
                   void *scan_tmp, hipStream_t st);
 void launch_adjacency(MeshDev &m, int *deg, int *off, int *fill, int *fan, void *scan_tmp, hipStream_t st);
 void launch_init_positions(MeshDev &m, hipStream_t st);
@@ -103,7 +146,7 @@ void launch_shift_bulk(MeshDev &m, unsigned char *moved, int first, hipStream_t 
 void launch_shift_commit(MeshDev &m, const unsigned char *moved, hipStream_t st);
 void launch_shift_greedy(MeshDev &m, int round, hipStream_t st);
 void launch_delete_round(MeshDev &m, unsigned int *mx, int round, int op, hipStream_t st);
-void launch_missing_init(MeshDev &m, const int *vor, int logT, int *mlist, int *loc, hipStream_t st);
+void launch_missing_init(MeshDev &m, const int *vor, int *mlist, int *loc, hipStream_t st);
 void launch_insert_round(MeshDev &m, int *mlist, int nmiss_cap, int *loc, int op, hipStream_t st);
 void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, hipStream_t st);
 void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st);
@@ -111,10 +154,13 @@ void launch_flip_pass(MeshDev &m, const int *list_in, int cnt_in_slot, int *list
                       int prev_op, int op, hipStream_t st);
 void launch_frame_round(MeshDev &m, unsigned int *mx, int round, int *list_out, int cnt_out_slot, int op,
                         hipStream_t st);
-void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, hipStream_t st);
-void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot, hipStream_t st);
-void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, int F, int *out9,
+void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, int *hull_cnt, int *hull_off,
+                      int *hull_start, void *scan_tmp, hipStream_t st);
+// canonical order: one 64-bit pass when wide == 0 (single set, ids < 2^21), else two passes
+void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot, int pass, int wide,
                        hipStream_t st);
+void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, int F, int *out9,
+                       int *set_first, hipStream_t st);
 int sort_pairs_u64(unsigned long long *keys_in, int *vals_in, unsigned long long *keys_out, int *vals_out,
                    int n, int begin_bit, int end_bit, void *temp, size_t *temp_bytes, hipStream_t st);
 int sort_pairs_u32(unsigned int *keys_in, int *vals_in, unsigned int *keys_out, int *vals_out,

@@ -39,10 +39,15 @@ GA_INLINE int mark_old(int e) { return e < 0 ? -1 : -(e + 2); }
 
 GA_INLINE double2 snapped(const MeshDev &m, int v)
 {
+    const SetDesc &sd = m.sets[m.vset[v]];
     int pp = m.pix[v];
     int px = pp >> 16, py = pp & 0xffff;
-    return make_double2(m.shx + (double)(px - m.margin) * m.scale, m.shy + (double)(py - m.margin) * m.scale);
+    return make_double2(sd.shx + (double)(px - sd.margin) * sd.scale, sd.shy + (double)(py - sd.margin) * sd.scale);
 }
+
+// a triangle of v's set that is certainly alive while frame vertices exist: one at the set's
+// first frame vertex
+GA_INLINE int set_anchor(const MeshDev &m, int v) { return m.vt[m.sets[m.vset[v]].fbase]; }
 
 __device__ __noinline__ int orient_slow(double2 a, double2 b, double2 c, int *status, int *cnt)
 {
@@ -94,12 +99,20 @@ GA_INLINE long long iorient(int pa, int pb, int pc)
     return (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
 }
 
-__global__ __launch_bounds__(256) void k_recon_count(const int *__restrict__ vor, int logT, int nyb,
+// tiles of 256 consecutive rows of one column of 2x2 blocks; set s owns tiles
+// [bm[s], bm[s + 1]) of the BM_RECON map
+__global__ __launch_bounds__(256) void k_recon_count(const int *__restrict__ vor_all, BatchDev bd,
                                                      int *__restrict__ tile_cnt)
 {
-    int T = 1 << logT;
+    const int *bm = bd.bmap(BM_RECON);
+    int set = set_of(bm, bd.nsets, blockIdx.x);
+    const SetDesc &sd = bd.sets[set];
+    int logT = sd.logT;
+    int T = 1 << logT, nyb = (T - 1 + 255) / 256;
+    const int *vor = vor_all + sd.toff;
     int tile = blockIdx.x;
-    int x = tile / nyb, y = (tile % nyb) * 256 + threadIdx.x;
+    int lt = tile - bm[set];
+    int x = lt / nyb, y = (lt % nyb) * 256 + threadIdx.x;
     int c = 0;
     if (x < T - 1 && y < T - 1) {
         size_t i0 = ((size_t)x << logT) + y, i1 = i0 + T;
@@ -113,14 +126,20 @@ __global__ __launch_bounds__(256) void k_recon_count(const int *__restrict__ vor
     if (threadIdx.x == 0) tile_cnt[tile] = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
-__global__ __launch_bounds__(256) void k_recon_emit(const int *__restrict__ vor, const int *__restrict__ pix,
-                                                    int logT, int nyb, const int *__restrict__ tile_off,
+__global__ __launch_bounds__(256) void k_recon_emit(const int *__restrict__ vor_all, const int *__restrict__ pix,
+                                                    BatchDev bd, const int *__restrict__ tile_off,
                                                     int4 *__restrict__ tv, int4 *__restrict__ tn,
                                                     int *__restrict__ status, int cap)
 {
-    int T = 1 << logT;
+    const int *bm = bd.bmap(BM_RECON);
+    int set = set_of(bm, bd.nsets, blockIdx.x);
+    const SetDesc &sd = bd.sets[set];
+    int logT = sd.logT;
+    int T = 1 << logT, nyb = (T - 1 + 255) / 256;
+    const int *vor = vor_all + sd.toff;
     int tile = blockIdx.x;
-    int x = tile / nyb, y = (tile % nyb) * 256 + threadIdx.x;
+    int lt = tile - bm[set];
+    int x = lt / nyb, y = (lt % nyb) * 256 + threadIdx.x;
     int c = 0, a = 0, b = 0, cc = 0, d = 0;
     if (x < T - 1 && y < T - 1) {
         size_t i0 = ((size_t)x << logT) + y, i1 = i0 + T;
@@ -175,15 +194,13 @@ __global__ void k_set_ntri(const int *tile_cnt, const int *tile_off, int ntiles,
     *ntri = F > cap ? cap : F;
 }
 
-void launch_recon(const int *vor, const int *pix, int logT, MeshDev &m, int *tile_cnt, int *tile_off,
+void launch_recon(const int *vor, const int *pix, const BatchDev &b, MeshDev &m, int *tile_cnt, int *tile_off,
                   void *scan_tmp, hipStream_t st)
 {
-    int T = 1 << logT;
-    int nyb = (T - 1 + 255) / 256;
-    int ntiles = (T - 1) * nyb;
-    GA_LAUNCH(k_recon_count, dim3(ntiles), dim3(256), 0, st, vor, logT, nyb, tile_cnt);
+    int ntiles = b.blocks[BM_RECON];
+    GA_LAUNCH(k_recon_count, dim3(ntiles), dim3(256), 0, st, vor, b, tile_cnt);
     launch_exclusive_scan(tile_cnt, tile_off, ntiles, scan_tmp, st);
-    GA_LAUNCH(k_recon_emit, dim3(ntiles), dim3(256), 0, st, vor, pix, logT, nyb, tile_off, m.tv, m.tn,
+    GA_LAUNCH(k_recon_emit, dim3(ntiles), dim3(256), 0, st, vor, pix, b, tile_off, m.tv, m.tn,
                        m.status, m.cap);
     GA_LAUNCH(k_set_ntri, dim3(1), dim3(1), 0, st, tile_cnt, tile_off, ntiles, m.ntri, m.cap);
 }
@@ -526,7 +543,7 @@ void launch_delete_round(MeshDev &m, unsigned int *mxraw, int round, int op, hip
 }
 
 // ---- missing-site insertion ------------------------------------------------------------------
-__global__ void k_missing_init(MeshDev m, const int *__restrict__ vor, int logT, int *mlist, int *loc)
+__global__ void k_missing_init(MeshDev m, const int *__restrict__ vor, int *mlist, int *loc)
 {
     GRID_STRIDE(v, m.n) {
         bool miss = m.vstate[v] == VS_MISSING;
@@ -536,19 +553,20 @@ __global__ void k_missing_init(MeshDev m, const int *__restrict__ vor, int logT,
         // a deleted vertex keeps an anchor triangle near its old position in vt[]
         int t = m.vt[v];
         if (t < 0 || m.tv[t].x == GA_DEAD) {
+            const SetDesc &sd = m.sets[m.vset[v]];
             int pp = m.pix[v];
-            int s = vor[((size_t)(pp >> 16) << logT) + (pp & 0xffff)];
+            int s = vor[sd.toff + ((size_t)(pp >> 16) << sd.logT) + (pp & 0xffff)];
             t = -1;
             if (s >= 0 && m.vstate[s] == VS_IN_MESH) t = m.vt[s];
         }
-        if (t < 0) t = m.vt[m.n];
+        if (t < 0) t = set_anchor(m, v);
         loc[k] = t;
     }
 }
 
-void launch_missing_init(MeshDev &m, const int *vor, int logT, int *mlist, int *loc, hipStream_t st)
+void launch_missing_init(MeshDev &m, const int *vor, int *mlist, int *loc, hipStream_t st)
 {
-    GA_LAUNCH(k_missing_init, dim3(FIXED_GRID), dim3(256), 0, st, m, vor, logT, mlist, loc);
+    GA_LAUNCH(k_missing_init, dim3(FIXED_GRID), dim3(256), 0, st, m, vor, mlist, loc);
 }
 
 // located: lt = (type << 2) | edge ; type 1 inside, 2 on edge, 3 duplicate ; -1 not located
@@ -561,7 +579,7 @@ __global__ void k_locate(MeshDev m, const int *mlist, int *loc, int *lt, int op)
         if (m.vstate[v] != VS_MISSING) continue;
         double2 p = m.xy[v];
         int t = loc[k];
-        if (t < 0 || m.tv[t].x == GA_DEAD) t = m.vt[m.n];
+        if (t < 0 || m.tv[t].x == GA_DEAD) t = set_anchor(m, v);
         uint32_t rng = hash32((uint32_t)v ^ ((uint32_t)op * 0x85ebca6bu));
         int located = 0, o[3];
         for (int step = 0; step < 4096; step++) {
@@ -946,22 +964,27 @@ void launch_frame_round(MeshDev &m, unsigned int *mxraw, int round, int *list_ou
                        list_out, cnt_out_slot);
 }
 
-// ---- hull convexification (one workgroup) --------------------------------------------------------
-#define HULL_MAX 8192
-__global__ __launch_bounds__(256) void k_convexify(MeshDev m, int *hnext, int *hedge, int *list_out, int cnt_out_slot,
+// ---- hull convexification (one workgroup per set) --------------------------------------------
+// Hull cycle of set `blockIdx.x` in the scratch arena at 4 * hull_off[set]: verts[], owner[] (enc
+// of the triangle side of hull edge (verts[k] -> verts[k+1])), nxt[], prv[]; hull_cnt[set] is the
+// number of hull edges, so the cycle fits exactly.
+__global__ __launch_bounds__(256) void k_convexify(MeshDev m, const int *hnext, const int *hedge,
+                                                   const int *hull_cnt, const int *hull_off,
+                                                   const int *hull_start, int *list_out, int cnt_out_slot,
                                                    int op)
 {
-    // hull cycle in the scratch arena: verts[], owner[] (enc of the triangle side of hull edge
-    // (verts[k] -> verts[k+1])), nxt[], prv[]
     __shared__ int s_flag, s_start, s_h, s_cur, s_block;
-    int *verts = m.scratch, *owner = verts + HULL_MAX, *nxt = owner + HULL_MAX, *prv = nxt + HULL_MAX;
+    int set = blockIdx.x;
+    int hcap = hull_cnt[set];
+    int *verts = m.scratch + 4 * (size_t)hull_off[set], *owner = verts + hcap, *nxt = owner + hcap,
+        *prv = nxt + hcap;
     if (threadIdx.x == 0) {
-        int start = m.counters[C_HSTART];
+        int start = hull_start[set];
         if (start == 0x7fffffff) start = -1;
         int h = 0, v = start;
         if (start >= 0) {
             do {
-                if (h >= HULL_MAX) { atomicOr(m.status, GA_ERR_SCRATCH); h = -1; break; }
+                if (h >= hcap) { atomicOr(m.status, GA_ERR_SCRATCH); h = -1; break; }
                 verts[h] = v;
                 owner[h] = hedge[v];
                 h++;
@@ -978,9 +1001,9 @@ __global__ __launch_bounds__(256) void k_convexify(MeshDev m, int *hnext, int *h
     for (int pass = 0; pass < h + 2; pass++) {
         if (threadIdx.x == 0) s_flag = 0;
         __syncthreads();
-        int cur = s_start, count = 0;
+        int cur = s_start;
         // one sweep around the cycle
-        for (int it = 0; it < 4 * HULL_MAX; it++) {
+        for (int it = 0; it < 4 * h + 4; it++) {
             __syncthreads();
             if (threadIdx.x == 0) {
                 s_cur = cur;
@@ -988,14 +1011,14 @@ __global__ __launch_bounds__(256) void k_convexify(MeshDev m, int *hnext, int *h
             }
             __syncthreads();
             int kc = s_cur, kp = prv[kc], kn = nxt[kc];
-            double2 P = m.pos[verts[kp]], C = m.pos[verts[kc]], N = m.pos[verts[kn]];
-            bool reflex = orient_dev(m, P, C, N) < 0;
+            double2 Pp = m.pos[verts[kp]], C = m.pos[verts[kc]], N = m.pos[verts[kn]];
+            bool reflex = orient_dev(m, Pp, C, N) < 0;
             if (reflex) {
                 // blocked if any other live hull vertex lies in the closed notch (P, N, C) minus P, N, C
                 for (int q = threadIdx.x; q < h; q += blockDim.x) {
                     if (q == kc || q == kp || q == kn || nxt[q] < 0) continue;
                     double2 Q = m.pos[verts[q]];
-                    if (orient_dev(m, P, N, Q) >= 0 && orient_dev(m, N, C, Q) >= 0 && orient_dev(m, C, P, Q) >= 0)
+                    if (orient_dev(m, Pp, N, Q) >= 0 && orient_dev(m, N, C, Q) >= 0 && orient_dev(m, C, Pp, Q) >= 0)
                         s_block = 1;
                 }
             }
@@ -1026,7 +1049,6 @@ __global__ __launch_bounds__(256) void k_convexify(MeshDev m, int *hnext, int *h
             }
             __syncthreads();
             cur = s_cur;
-            if (threadIdx.x == 0) count++;
             if (cur == s_start && it > 0) break;
         }
         __syncthreads();
@@ -1034,9 +1056,7 @@ __global__ __launch_bounds__(256) void k_convexify(MeshDev m, int *hnext, int *h
     }
 }
 
-__global__ void k_set_counter(int *c, int v) { *c = v; }
-
-__global__ void k_hull_edges(MeshDev m, int *hnext, int *hedge)
+__global__ void k_hull_edges(MeshDev m, int *hnext, int *hedge, int *hull_cnt, int *hull_start)
 {
     int n = *m.ntri;
     GRID_STRIDE(t, n) {
@@ -1048,23 +1068,32 @@ __global__ void k_hull_edges(MeshDev m, int *hnext, int *hedge)
                 int a = ic(v, p1(i)), b = ic(v, m1(i));
                 hnext[a] = b;
                 hedge[a] = enc(t, i);
-                atomicMin(&m.counters[C_HSTART], a);
+                int s = m.vset[a];
+                atomicMin(&hull_start[s], a);
+                atomicAdd(&hull_cnt[s], 1);
             }
     }
 }
 
 static int *g_hnext = nullptr, *g_hedge = nullptr;
 void set_hull_buffers(int *hn, int *he) { g_hnext = hn; g_hedge = he; }
-void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, hipStream_t st)
+void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, int *hull_cnt, int *hull_off,
+                      int *hull_start, void *scan_tmp, hipStream_t st)
 {
     hipMemsetAsync(g_hnext, 0xff, sizeof(int) * m.V, st);
-    GA_LAUNCH(k_set_counter, dim3(1), dim3(1), 0, st, m.counters + C_HSTART, 0x7fffffff);
-    GA_LAUNCH(k_hull_edges, dim3(FIXED_GRID), dim3(256), 0, st, m, g_hnext, g_hedge);
-    GA_LAUNCH(k_convexify, dim3(1), dim3(256), 0, st, m, g_hnext, g_hedge, list_out, cnt_out_slot, op);
+    hipMemsetAsync(hull_cnt, 0, sizeof(int) * m.nsets, st);
+    launch_fill_i32(hull_start, 0x7fffffff, (size_t)m.nsets, st);
+    GA_LAUNCH(k_hull_edges, dim3(FIXED_GRID), dim3(256), 0, st, m, g_hnext, g_hedge, hull_cnt, hull_start);
+    launch_exclusive_scan(hull_cnt, hull_off, m.nsets, scan_tmp, st);
+    GA_LAUNCH(k_convexify, dim3(m.nsets), dim3(256), 0, st, m, (const int *)g_hnext, (const int *)g_hedge,
+              (const int *)hull_cnt, (const int *)hull_off, (const int *)hull_start, list_out, cnt_out_slot, op);
 }
 
 // ---- canonical output ----------------------------------------------------------------------------
-__global__ void k_canon_keys(MeshDev m, unsigned long long *keys, int *vals, int *rot)
+// Output order: set, then (v0, v1, v2) with v0 the smallest vertex.  wide == 0 (one set, vertex ids
+// below 2^21): one 64-bit key (v0, v1, v2).  wide == 1: two stable passes, pass 1 on (v1, v2), pass 2
+// on (set, v0) over the order of pass 1 (order1 = vals of pass 1).  Dead slots sort last.
+__global__ void k_canon_keys(MeshDev m, unsigned long long *keys, int *vals, int *rot, int wide)
 {
     int n = *m.ntri;
     int alive = 0;
@@ -1078,14 +1107,28 @@ __global__ void k_canon_keys(MeshDev m, unsigned long long *keys, int *vals, int
         if (v.z < ic(v, r)) r = 2;
         rot[t] = r;
         unsigned long long a = (unsigned)ic(v, r), b = (unsigned)ic(v, (r + 1) % 3), c = (unsigned)ic(v, (r + 2) % 3);
-        keys[t] = (a << 42) | (b << 21) | c;
+        keys[t] = wide ? ((b << 32) | c) : ((a << 42) | (b << 21) | c);
     }
     wave_count_full(&m.counters[C_RECON_TRIS], alive);
 }
 
-void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot, hipStream_t st)
+__global__ void k_canon_keys2(MeshDev m, const int *order1, const int *rot, unsigned long long *keys)
 {
-    GA_LAUNCH(k_canon_keys, dim3(FIXED_GRID), dim3(256), 0, st, m, keys, vals, rot);
+    int n = *m.ntri;
+    GRID_STRIDE(r, n) {
+        int t = order1[r];
+        int4 v = m.tv[t];
+        if (v.x == GA_DEAD) { keys[r] = ~0ull; continue; }
+        unsigned long long a = (unsigned)ic(v, rot[t]);
+        keys[r] = ((unsigned long long)m.vset[a] << 32) | a;
+    }
+}
+
+void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot, int pass, int wide,
+                       hipStream_t st)
+{
+    if (pass == 1) GA_LAUNCH(k_canon_keys, dim3(FIXED_GRID), dim3(256), 0, st, m, keys, vals, rot, wide);
+    else GA_LAUNCH(k_canon_keys2, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)vals, (const int *)rot, keys);
 }
 
 __global__ void k_canon_newid(const int *order, int F, int *newid)
@@ -1093,7 +1136,10 @@ __global__ void k_canon_newid(const int *order, int F, int *newid)
     GRID_STRIDE(r, F) newid[order[r]] = r;
 }
 
-__global__ void k_canon_emit(MeshDev m, const int *order, const int *rot, const int *newid, int F, int *out9)
+// out9 holds global vertex ids and global triangle positions; set_first[s] = first output
+// triangle of set s (left at -1 for a set without triangles)
+__global__ void k_canon_emit(MeshDev m, const int *order, const int *rot, const int *newid, int F, int *out9,
+                             int *set_first)
 {
     GRID_STRIDE(r, F) {
         int s = order[r], ro = rot[s];
@@ -1110,11 +1156,21 @@ __global__ void k_canon_emit(MeshDev m, const int *order, const int *rot, const 
             }
             o[6 + i] = 0;
         }
+        int set = m.vset[ic(v, ro)];
+        if (r == 0) set_first[set] = 0;
+        else {
+            int sp = order[r - 1];
+            int ps = m.vset[ic(m.tv[sp], rot[sp])];
+            if (ps != set) set_first[set] = r;
+        }
     }
 }
 
-void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, int F, int *out9, hipStream_t st)
+void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, int F, int *out9,
+                       int *set_first, hipStream_t st)
 {
+    launch_fill_i32(set_first, -1, (size_t)m.nsets, st);
     GA_LAUNCH(k_canon_newid, dim3(FIXED_GRID), dim3(256), 0, st, order, F, newid);
-    GA_LAUNCH(k_canon_emit, dim3(FIXED_GRID), dim3(256), 0, st, m, order, rot, (const int *)newid, F, out9);
+    GA_LAUNCH(k_canon_emit, dim3(FIXED_GRID), dim3(256), 0, st, m, order, rot, (const int *)newid, F, out9,
+              set_first);
 }

@@ -32,45 +32,77 @@ __global__ void k_fill_i32(int *a, int v, size_t n)
     for (; i < n; i += stride) a[i] = v;
 }
 
-// Map real points to pixels.  pix = (px << 16) | py; the smallest index wins a pixel.
-__global__ void k_splat(const double2 *__restrict__ xy, int n, double shx, double shy,
-                        double inv, int margin, int logT, int *__restrict__ smap,
+// Set of every vertex: real points by their poff range, frame vertices by their fbase range.
+__global__ void k_vset(const SetDesc *__restrict__ sets, int nsets, int ntot, int V, int *__restrict__ vset)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    int lo = 0, hi = nsets - 1;
+    if (v < ntot) {
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (sets[mid].poff <= v) lo = mid;
+            else hi = mid - 1;
+        }
+    } else {
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (sets[mid].fbase <= v) lo = mid;
+            else hi = mid - 1;
+        }
+    }
+    vset[v] = lo;
+}
+
+// Map real points to pixels of their set's texture.  pix = (px << 16) | py; the smallest
+// index wins a pixel.
+__global__ void k_splat(const double2 *__restrict__ xy, BatchDev b, int *__restrict__ smap,
                         int *__restrict__ pix)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= b.ntot) return;
+    const SetDesc &sd = b.sets[b.vset[i]];
     double2 p = xy[i];
-    int px = (int)floor((p.x - shx) * inv) + margin;
-    int py = (int)floor((p.y - shy) * inv) + margin;
-    int T = 1 << logT;
+    int margin = sd.margin;
+    int px = (int)floor((p.x - sd.shx) * sd.inv) + margin;
+    int py = (int)floor((p.y - sd.shy) * sd.inv) + margin;
+    int T = 1 << sd.logT;
     px = px < margin ? margin : (px > T - 1 - margin ? T - 1 - margin : px);
     py = py < margin ? margin : (py > T - 1 - margin ? T - 1 - margin : py);
     pix[i] = (px << 16) | py;
-    atomicMin(&smap[((size_t)px << logT) + py], i);
+    atomicMin(&smap[sd.toff + ((size_t)px << sd.logT) + py], i);
 }
 
-// Frame sites (ids n .. n+nf-1) at their border pixels.
-__global__ void k_splat_frame(const int *__restrict__ fpix, int nf, int n, int logT,
-                              int *__restrict__ smap, int *__restrict__ pix)
+// Frame sites (global ids ntot .. V-1) at their border pixels (fpix: packed local pixels).
+__global__ void k_splat_frame(const int *__restrict__ fpix, BatchDev b, int *__restrict__ smap,
+                              int *__restrict__ pix)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nf) return;
+    int f = b.ntot + k;
+    if (f >= b.V) return;
+    const SetDesc &sd = b.sets[b.vset[f]];
     int pp = fpix[k];
-    pix[n + k] = pp;
-    smap[((size_t)(pp >> 16) << logT) + (pp & 0xffff)] = n + k;
+    pix[f] = pp;
+    smap[sd.toff + ((size_t)(pp >> 16) << sd.logT) + (pp & 0xffff)] = f;
 }
 
 // ---- phase 1: nearest site per column --------------------------------------------------
 // block = 256 (4 waves, one column each); g[x*T + y] = row of the nearest site in column x
 // (ties to the lower row), -1 for an empty column.
-__global__ __launch_bounds__(256) void k_pba_cols(const int *__restrict__ smap,
-                                                  short *__restrict__ g, int logT)
+__global__ __launch_bounds__(256) void k_pba_cols(const int *__restrict__ smap_all,
+                                                  short *__restrict__ g_all, BatchDev bd)
 {
     __shared__ unsigned long long masks[4][256];
     __shared__ short prv[4][256], nxt[4][256];
+    const int *bm = bd.bmap(BM_COLS);
+    int set = set_of(bm, bd.nsets, blockIdx.x);
+    const SetDesc &sd = bd.sets[set];
+    int logT = sd.logT;
+    const int *smap = smap_all + sd.toff;
+    short *g = g_all + sd.toff;
     int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int T = 1 << logT, nch = T >> 6;
-    int x = blockIdx.x * 4 + wave;
+    int x = (blockIdx.x - bm[set]) * 4 + wave;
     const int *col = smap + ((size_t)x << logT);
     for (int c = 0; c < nch; c++) {
         int v = col[(c << 6) + lane];
@@ -140,12 +172,20 @@ GA_INLINE bool dominated(int a, int b, int c)
 
 // grid (T/64, m2), block 64: lanes are rows.  Stack entries of band b live in
 // S[(b*Wb + k) * T + y], k < cnt[b*T + y].
-__global__ __launch_bounds__(64) void k_pba_stack(const short *__restrict__ g, int *__restrict__ S,
-                                                  int *__restrict__ cnt, int logT, int Wb)
+__global__ __launch_bounds__(64) void k_pba_stack(const short *__restrict__ g_all, int *__restrict__ S_all,
+                                                  int *__restrict__ cnt_all, BatchDev bd)
 {
-    int T = 1 << logT;
-    int y = blockIdx.x * 64 + threadIdx.x;
-    int b = blockIdx.y;
+    const int *bm = bd.bmap(BM_STACK);
+    int set = set_of(bm, bd.nsets, blockIdx.x);
+    const SetDesc &sd = bd.sets[set];
+    int logT = sd.logT;
+    int T = 1 << logT, nbx = T >> 6, Wb = T / sd.m2;
+    int lb = blockIdx.x - bm[set];
+    const short *g = g_all + sd.toff;
+    int *S = S_all + sd.toff;
+    int *cnt = cnt_all + (size_t)16 * sd.roff;
+    int y = (lb % nbx) * 64 + threadIdx.x;
+    int b = lb / nbx;
     int x0 = b * Wb;
     int top = 0, e0 = 0, e1 = 0;
     for (int x = x0; x < x0 + Wb; x++) {
@@ -167,15 +207,23 @@ __global__ __launch_bounds__(64) void k_pba_stack(const short *__restrict__ g, i
 
 // ---- phase 2b: merge band stacks per row, compact the envelope -------------------------
 #define PBA_MAXB 64
-__global__ __launch_bounds__(64) void k_pba_merge(const int *__restrict__ S, const int *__restrict__ cnt,
-                                                  int *__restrict__ E, int *__restrict__ ecnt,
-                                                  int logT, int Wb, int m2)
+__global__ __launch_bounds__(64) void k_pba_merge(const int *__restrict__ S_all, const int *__restrict__ cnt_all,
+                                                  int *__restrict__ E_all, int *__restrict__ ecnt_all,
+                                                  BatchDev bd)
 {
     __shared__ short lo_s[PBA_MAXB][64], hi_s[PBA_MAXB][64];
     __shared__ unsigned char bst[PBA_MAXB][64];
-    int T = 1 << logT;
+    const int *bm = bd.bmap(BM_MERGE);
+    int set = set_of(bm, bd.nsets, blockIdx.x);
+    const SetDesc &sd = bd.sets[set];
+    int logT = sd.logT;
+    int T = 1 << logT, m2 = sd.m2, Wb = T / m2;
+    const int *S = S_all + sd.toff;
+    const int *cnt = cnt_all + (size_t)16 * sd.roff;
+    int *E = E_all + sd.toff;
+    int *ecnt = ecnt_all + sd.roff;
     int lane = threadIdx.x;
-    int y = blockIdx.x * 64 + lane;
+    int y = (blockIdx.x - bm[set]) * 64 + lane;
     int nb = 0;
 #define ELEM(B, I) S[((size_t)((B) * Wb + (I)) << logT) + y]
     for (int b = 0; b < m2; b++) {
@@ -227,12 +275,22 @@ GA_INLINE long long fdist(int e, int i)
 }
 
 // grid (T/64, m3), block 64; vor[x*T + y] = site id
-__global__ __launch_bounds__(64) void k_pba_color(const int *__restrict__ E, const int *__restrict__ ecnt,
-                                                  const int *__restrict__ smap, int *__restrict__ vor,
-                                                  int logT, int Wp)
+__global__ __launch_bounds__(64) void k_pba_color(const int *__restrict__ E_all, const int *__restrict__ ecnt_all,
+                                                  const int *__restrict__ smap_all, int *__restrict__ vor_all,
+                                                  BatchDev bd)
 {
-    int y = blockIdx.x * 64 + threadIdx.x;
-    int ps = blockIdx.y * Wp, pe = ps + Wp;
+    const int *bm = bd.bmap(BM_COLOR);
+    int set = set_of(bm, bd.nsets, blockIdx.x);
+    const SetDesc &sd = bd.sets[set];
+    int logT = sd.logT;
+    int T = 1 << logT, nbx = T >> 6, Wp = T / sd.m3;
+    int lb = blockIdx.x - bm[set];
+    const int *E = E_all + sd.toff;
+    const int *ecnt = ecnt_all + sd.roff;
+    const int *smap = smap_all + sd.toff;
+    int *vor = vor_all + sd.toff;
+    int y = (lb % nbx) * 64 + threadIdx.x;
+    int ps = (lb / nbx) * Wp, pe = ps + Wp;
     int n = ecnt[y];
     if (n <= 0) {
         for (int i = ps; i < pe; i++) vor[((size_t)i << logT) + y] = -1;
@@ -262,12 +320,17 @@ __global__ __launch_bounds__(64) void k_pba_color(const int *__restrict__ E, con
 }
 
 // ---- island certificate ------------------------------------------------------------------
-__global__ void k_islands(const int *__restrict__ vor, const int *__restrict__ pix, int logT,
-                          int *__restrict__ bad, int *__restrict__ flag)
+// blocks of 256 pixels (a texture has T*T >= 4096 pixels, a multiple of 256)
+__global__ __launch_bounds__(256) void k_islands(const int *__restrict__ vor_all, const int *__restrict__ pix,
+                                                 BatchDev bd, int *__restrict__ bad, int *__restrict__ flag)
 {
+    const int *bm = bd.bmap(BM_PIX);
+    int set = set_of(bm, bd.nsets, blockIdx.x);
+    const SetDesc &sd = bd.sets[set];
+    int logT = sd.logT;
     int T = 1 << logT;
-    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= ((size_t)T << logT)) return;
+    const int *vor = vor_all + sd.toff;
+    size_t idx = (size_t)(blockIdx.x - bm[set]) * 256 + threadIdx.x;
     int x = (int)(idx >> logT), y = (int)(idx & (T - 1));
     int s = vor[idx];
     if (s < 0) return;
@@ -285,15 +348,18 @@ __global__ void k_islands(const int *__restrict__ vor, const int *__restrict__ p
     }
 }
 
-// Take sites flagged by the certificate out of the site map.
-__global__ void k_remove_bad(int *__restrict__ bad, const int *__restrict__ pix, int n, int logT,
-                             int *__restrict__ smap)
+// Take sites flagged by the certificate out of the site map (counted in *removed).
+__global__ void k_remove_bad(int *__restrict__ bad, const int *__restrict__ pix, BatchDev bd,
+                             int *__restrict__ smap, int *__restrict__ removed)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n || !bad[s]) return;
+    bool r = s < bd.ntot && bad[s];
+    wave_count_full(removed, r ? 1 : 0);
+    if (!r) return;
     bad[s] = 0;
+    const SetDesc &sd = bd.sets[bd.vset[s]];
     int sp = pix[s];
-    smap[((size_t)(sp >> 16) << logT) + (sp & 0xffff)] = PBA_EMPTY;
+    smap[sd.toff + ((size_t)(sp >> 16) << sd.logT) + (sp & 0xffff)] = PBA_EMPTY;
 }
 
 // ---- host-side launchers ---------------------------------------------------------------
@@ -305,34 +371,36 @@ void launch_fill_i32(int *a, int v, size_t n, hipStream_t st)
     GA_LAUNCH(k_fill_i32, dim3(blocks), dim3(256), 0, st, a, v, n);
 }
 
-void launch_splat(const double2 *xy, int n, double shx, double shy, double inv, int margin,
-                  int logT, int *smap, int *pix, const int *fpix, int nf, hipStream_t st)
+void launch_vset(const BatchDev &b, int *vset, hipStream_t st)
 {
-    if (n > 0)
-        GA_LAUNCH(k_splat, dim3((n + 255) / 256), dim3(256), 0, st, xy, n, shx, shy, inv,
-                           margin, logT, smap, pix);
+    if (b.V > 0)
+        GA_LAUNCH(k_vset, dim3((b.V + 255) / 256), dim3(256), 0, st, b.sets, b.nsets, b.ntot, b.V, vset);
+}
+
+void launch_splat(const double2 *xy, const BatchDev &b, int *smap, int *pix, const int *fpix, hipStream_t st)
+{
+    if (b.ntot > 0)
+        GA_LAUNCH(k_splat, dim3((b.ntot + 255) / 256), dim3(256), 0, st, xy, b, smap, pix);
+    int nf = b.V - b.ntot;
     if (nf > 0)
-        GA_LAUNCH(k_splat_frame, dim3((nf + 255) / 256), dim3(256), 0, st, fpix, nf, n, logT,
-                           smap, pix);
+        GA_LAUNCH(k_splat_frame, dim3((nf + 255) / 256), dim3(256), 0, st, fpix, b, smap, pix);
 }
 
-void launch_pba(const PbaBuffers &b, int logT, int m2, int m3, hipStream_t st)
+void launch_pba(const PbaBuffers &p, const BatchDev &b, hipStream_t st)
 {
-    int T = 1 << logT;
-    GA_LAUNCH(k_pba_cols, dim3(T / 4), dim3(256), 0, st, b.smap, b.g, logT);
-    GA_LAUNCH(k_pba_stack, dim3(T / 64, m2), dim3(64), 0, st, b.g, b.S, b.cnt, logT, T / m2);
-    GA_LAUNCH(k_pba_merge, dim3(T / 64), dim3(64), 0, st, b.S, b.cnt, b.E, b.ecnt, logT, T / m2, m2);
-    GA_LAUNCH(k_pba_color, dim3(T / 64, m3), dim3(64), 0, st, b.E, b.ecnt, b.smap, b.vor, logT, T / m3);
+    GA_LAUNCH(k_pba_cols, dim3(b.blocks[BM_COLS]), dim3(256), 0, st, p.smap, p.g, b);
+    GA_LAUNCH(k_pba_stack, dim3(b.blocks[BM_STACK]), dim3(64), 0, st, p.g, p.S, p.cnt, b);
+    GA_LAUNCH(k_pba_merge, dim3(b.blocks[BM_MERGE]), dim3(64), 0, st, p.S, p.cnt, p.E, p.ecnt, b);
+    GA_LAUNCH(k_pba_color, dim3(b.blocks[BM_COLOR]), dim3(64), 0, st, p.E, p.ecnt, p.smap, p.vor, b);
 }
 
-void launch_islands(const int *vor, const int *pix, int logT, int *bad, int *flag, hipStream_t st)
+void launch_islands(const int *vor, const int *pix, const BatchDev &b, int *bad, int *flag, hipStream_t st)
 {
-    size_t P = (size_t)1 << (2 * logT);
-    GA_LAUNCH(k_islands, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, vor, pix, logT, bad, flag);
+    GA_LAUNCH(k_islands, dim3(b.blocks[BM_PIX]), dim3(256), 0, st, vor, pix, b, bad, flag);
 }
 
-void launch_remove_bad(int *bad, const int *pix, int n, int logT, int *smap, hipStream_t st)
+void launch_remove_bad(int *bad, const int *pix, const BatchDev &b, int *smap, int *removed, hipStream_t st)
 {
-    if (n > 0)
-        GA_LAUNCH(k_remove_bad, dim3((n + 255) / 256), dim3(256), 0, st, bad, pix, n, logT, smap);
+    if (b.ntot > 0)
+        GA_LAUNCH(k_remove_bad, dim3((b.ntot + 255) / 256), dim3(256), 0, st, bad, pix, b, smap, removed);
 }

@@ -1,42 +1,83 @@
-// util.hip -- small device utilities: bounding box of the input points.
+// util.hip -- bounding boxes of point sets on the device (min x, min y, max x, max y, and a flag
+// for non-finite coordinates, which the host rejects).
 #include "dev_common.h"
 #include "kernels.h"
 
-__global__ __launch_bounds__(256) void k_bbox(const double2 *__restrict__ xy, int n, double *out4)
+// reduce one block's box; returns it in thread 0
+GA_INLINE void block_box(double &a, double &b, double &c, double &d, double &bad)
 {
-    double a = INFINITY, b = INFINITY, c = -INFINITY, d = -INFINITY;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        double2 p = xy[i];
-        a = fmin(a, p.x); b = fmin(b, p.y); c = fmax(c, p.x); d = fmax(d, p.y);
-    }
     for (int s = 32; s > 0; s >>= 1) {
         a = fmin(a, __shfl_down(a, s, 64)); b = fmin(b, __shfl_down(b, s, 64));
         c = fmax(c, __shfl_down(c, s, 64)); d = fmax(d, __shfl_down(d, s, 64));
+        bad = fmax(bad, __shfl_down(bad, s, 64));
     }
-    __shared__ double sh[4][4];
+    __shared__ double sh[4][5];
     int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { sh[w][0] = a; sh[w][1] = b; sh[w][2] = c; sh[w][3] = d; }
+    if ((threadIdx.x & 63) == 0) { sh[w][0] = a; sh[w][1] = b; sh[w][2] = c; sh[w][3] = d; sh[w][4] = bad; }
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int k = 1; k < 4; k++) {
             a = fmin(a, sh[k][0]); b = fmin(b, sh[k][1]); c = fmax(c, sh[k][2]); d = fmax(d, sh[k][3]);
+            bad = fmax(bad, sh[k][4]);
         }
-        out4[4 * blockIdx.x + 0] = a; out4[4 * blockIdx.x + 1] = b;
-        out4[4 * blockIdx.x + 2] = c; out4[4 * blockIdx.x + 3] = d;
     }
 }
 
-void bbox_device(const double *d_xy, int n, double *out4, void *tmp, hipStream_t st)
+GA_INLINE void box_add(double2 p, double &a, double &b, double &c, double &d, double &bad)
+{
+    if (!isfinite(p.x) || !isfinite(p.y)) bad = 1.0;
+    a = fmin(a, p.x); b = fmin(b, p.y); c = fmax(c, p.x); d = fmax(d, p.y);
+}
+
+// one set, many blocks: partial boxes out5[5 * block]
+__global__ __launch_bounds__(256) void k_bbox(const double2 *__restrict__ xy, int n, double *out5)
+{
+    double a = INFINITY, b = INFINITY, c = -INFINITY, d = -INFINITY, bad = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        box_add(xy[i], a, b, c, d, bad);
+    block_box(a, b, c, d, bad);
+    if (threadIdx.x == 0) {
+        double *o = out5 + 5 * blockIdx.x;
+        o[0] = a; o[1] = b; o[2] = c; o[3] = d; o[4] = bad;
+    }
+}
+
+// many sets, one block each: set s covers points [offs[s], offs[s + 1])
+__global__ __launch_bounds__(256) void k_bbox_sets(const double2 *__restrict__ xy, const int *__restrict__ offs,
+                                                   double *out5)
+{
+    int s = blockIdx.x;
+    double a = INFINITY, b = INFINITY, c = -INFINITY, d = -INFINITY, bad = 0.0;
+    for (int i = offs[s] + threadIdx.x; i < offs[s + 1]; i += blockDim.x) box_add(xy[i], a, b, c, d, bad);
+    block_box(a, b, c, d, bad);
+    if (threadIdx.x == 0) {
+        double *o = out5 + 5 * s;
+        o[0] = a; o[1] = b; o[2] = c; o[3] = d; o[4] = bad;
+    }
+}
+
+// box of one set (tmp: >= 5 * 64 doubles of device memory); out5 on the host
+void bbox_device(const double *d_xy, int n, double *out5, void *tmp, hipStream_t st)
 {
     const int blocks = 64;
     double *part = (double *)tmp;
     GA_LAUNCH(k_bbox, dim3(blocks), dim3(256), 0, st, (const double2 *)d_xy, n, part);
-    double h[4 * blocks];
+    double h[5 * blocks];
     hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, st);
     hipStreamSynchronize(st);
-    double a = INFINITY, b = INFINITY, c = -INFINITY, d = -INFINITY;
+    double a = INFINITY, b = INFINITY, c = -INFINITY, d = -INFINITY, bad = 0.0;
     for (int k = 0; k < blocks; k++) {
-        a = fmin(a, h[4 * k]); b = fmin(b, h[4 * k + 1]); c = fmax(c, h[4 * k + 2]); d = fmax(d, h[4 * k + 3]);
+        a = fmin(a, h[5 * k]); b = fmin(b, h[5 * k + 1]); c = fmax(c, h[5 * k + 2]); d = fmax(d, h[5 * k + 3]);
+        bad = fmax(bad, h[5 * k + 4]);
     }
-    out4[0] = a; out4[1] = b; out4[2] = c; out4[3] = d;
+    out5[0] = a; out5[1] = b; out5[2] = c; out5[3] = d; out5[4] = bad;
+}
+
+// boxes of nsets sets (d_offs: device offsets[nsets + 1]; tmp: >= 5 * nsets doubles); out5 on the host
+void bbox_sets_device(const double *d_xy, const int *d_offs, int nsets, double *out5, void *tmp, hipStream_t st)
+{
+    if (nsets <= 0) return;
+    GA_LAUNCH(k_bbox_sets, dim3(nsets), dim3(256), 0, st, (const double2 *)d_xy, d_offs, (double *)tmp);
+    hipMemcpyAsync(out5, tmp, sizeof(double) * 5 * (size_t)nsets, hipMemcpyDeviceToHost, st);
+    hipStreamSynchronize(st);
 }

@@ -139,9 +139,14 @@ int galphaCopyAlpha(galpha_ctx *ctx, PGPUDTALPHA *pAlpha);
 
 /* ---- batches and serving (no reference counterpart) ----------------------------------- */
 /* Batched: nsets independent point sets, concatenated in xy (offsets[nsets+1] in points),
- * texture side per set (0 = autotune).  Results: tris9 concatenated, tri_offsets[nsets+1]. */
+ * texture side per set (NULL or 0 = autotuned).  All sets go through one pass of the pipeline
+ * (every kernel launch covers every set).  Results: triangles of all sets concatenated with
+ * set-local vertex and triangle indices, tri_offsets[nsets+1]; both malloc'ed (galphaFree).
+ * Each set's result is identical to a single-set run on the same points. */
 int galphaBatch(galpha_ctx *ctx, const double *xy, const int *offsets, int nsets, const int *fboSizes,
                 int with_alpha, int **tris9_out, int **tri_offsets_out);
+/* alpha result of input set `set` of the last galphaBatch(with_alpha = 1) (gpudtReleaseAlpha) */
+int galphaBatchCopyAlpha(galpha_ctx *ctx, int set, PGPUDTALPHA *pAlpha);
 void galphaFree(void *p);
 
 /* Kernel timing probe: while enabled, HIP events are recorded on the launch stream around every

@@ -1,0 +1,86 @@
+"""Batched mode: every set of one batched pass equals the oracle (canonical triangles and every
+alpha array, bit-exact), whatever else is in the batch."""
+import numpy as np
+import pytest
+
+from conftest import ALPHA_FIELDS, lattice_clusters
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_sets(oracle):
+    rng = np.random.default_rng(7)
+    sets = []
+    for k, n in enumerate([3, 7, 50, 300, 1000, 2500, 4000, 700, 120, 64]):
+        sets.append(oracle.gen_uniform(100 + k, n))
+    # Gaussian clusters, a lattice with ties, exact duplicates, collinear points, tiny sets
+    c = rng.random((4, 2))
+    sets.append(np.clip(c[rng.integers(0, 4, 3000)] + rng.normal(size=(3000, 2)) * 0.01, 0, 0.999999))
+    sets.append(np.array([(i / 17.0, j / 17.0) for i in range(17) for j in range(17)]))
+    u = oracle.gen_uniform(55, 400)
+    sets.append(np.concatenate([u, u[:40]]))
+    sets.append(np.array([[i * 0.1, i * 0.05] for i in range(12)]))
+    sets.append(np.zeros((0, 2)))
+    sets.append(np.array([[0.5, 0.5], [0.25, 0.75]]))
+    sets.append(lattice_clusters(1500, 1500, 60, 4, 0.01, 9))
+    return sets
+
+
+def _check_set(oracle, xy, tris, alpha):
+    ref, _ = oracle.delaunay(xy) if xy.shape[0] >= 3 else (np.zeros((0, 9), np.int32), None)
+    assert np.array_equal(tris, ref)
+    if alpha is None:
+        return
+    if ref.shape[0] == 0:
+        assert len(alpha["spectrum"]) == 1 and np.all(alpha["vert_nu_min"] == -1)
+        return
+    ra = oracle.alpha(xy, ref)
+    for k in ALPHA_FIELDS:
+        g, r = np.asarray(alpha[k]), np.asarray(ra[k])
+        assert g.shape == r.shape, k
+        if k == "spectrum":
+            assert np.array_equal(g.view(np.uint64), r.view(np.uint64)), "spectrum differs in ulps"
+        else:
+            assert np.array_equal(g, r), k
+
+
+def test_batch_matches_oracle(oracle):
+    import galpha2d_amd as G
+    print("case: mixed batch of 17 sets, DT + alpha", flush=True)
+    sets = _mixed_sets(oracle)
+    ctx = G.Context(0)
+    tris = ctx.batch(sets, with_alpha=True)
+    assert len(tris) == len(sets)
+    for k, xy in enumerate(sets):
+        _check_set(oracle, xy, tris[k], ctx.batch_alpha(k))
+    ctx.close()
+
+
+def test_batch_explicit_textures_and_reuse(oracle):
+    """Explicit texture sides, then a second (smaller) batch on the same context."""
+    import galpha2d_amd as G
+    print("case: batch with explicit textures, context reuse", flush=True)
+    ctx = G.Context(0)
+    sets = [oracle.gen_uniform(200 + k, n) for k, n in enumerate([2000, 500, 9000, 30])]
+    tris = ctx.batch(sets, fbo_sizes=[512, 256, 1024, 64], with_alpha=True)
+    for k, xy in enumerate(sets):
+        _check_set(oracle, xy, tris[k], ctx.batch_alpha(k))
+    sets2 = sets[1:3]
+    tris2 = ctx.batch(sets2, with_alpha=False)
+    for k, xy in enumerate(sets2):
+        _check_set(oracle, xy, tris2[k], None)
+    ctx.close()
+
+
+def test_batch_equals_plain_loop():
+    """The batched workload generator's first sets: batch result == per-set plain API result."""
+    import galpha2d_amd as G
+    from galpha2d_amd import datagen
+    print("case: 96 generated sets, batch vs plain loop", flush=True)
+    sets = datagen.batch_sets(96, seed=4)
+    ctx = G.Context(0)
+    tris = ctx.batch(sets, with_alpha=True)
+    for k, xy in enumerate(sets):
+        plain = ctx.delaunay(xy)
+        assert np.array_equal(tris[k], plain), k
+    ctx.close()
