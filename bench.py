@@ -47,6 +47,120 @@ def cpu_baseline(n, seed):
                       f"{dt:.2f} s on one core of {model or 'the host'} (nproc={os.cpu_count()})"}
 
 
+def batched_vs_plain(G, torch, nsets, plain_sample, seed=4):
+    """Batched workload: nsets point sets, sizes log-uniform in [256, 65536], each uniform or an
+    8-cluster Gaussian mix (50/50), autotuned texture per set; DT + full alpha for every set.
+    Points are resident in HBM before timing; results stay on the device.  Batched = one
+    galphaBatchDevice pass over all sets; plain = one galphaComputeDevice call per set, timed on
+    the first `plain_sample` sets (sets/s is a rate, so a sample of the same distribution is
+    enough)."""
+    import numpy as np
+    from galpha2d_amd import datagen
+    sets = datagen.batch_sets(nsets, seed=seed)
+    sizes = np.array([x.shape[0] for x in sets], dtype=np.int64)
+    offs = np.zeros(nsets + 1, dtype=np.int32)
+    offs[1:] = np.cumsum(sizes)
+    d_xy = torch.from_numpy(np.ascontiguousarray(np.concatenate(sets))).to("cuda:0")
+    del sets
+    ctx = G.Context(0)
+    stream = torch.cuda.current_stream().cuda_stream
+    torch.cuda.synchronize()
+    base = d_xy.data_ptr()
+    # plain loop (warm up on the first 8 sets)
+    ns = min(plain_sample, nsets)
+    for k in range(min(8, ns)):
+        ctx.compute_device(base + 16 * int(offs[k]), int(sizes[k]), G.auto_texture(int(sizes[k])),
+                           with_alpha=True, stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(ns):
+        ctx.compute_device(base + 16 * int(offs[k]), int(sizes[k]), G.auto_texture(int(sizes[k])),
+                           with_alpha=True, stream=stream)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    # batched: one warm-up pass (allocations), then timed passes
+    ctx.batch_device(base, offs, with_alpha=True, stream=stream)
+    torch.cuda.synchronize()
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        F = ctx.batch_device(base, offs, with_alpha=True, stream=stream)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / reps
+    st = ctx.stats()
+    ctx.close()
+    del d_xy
+    torch.cuda.empty_cache()
+    plain_rate = ns / t_plain
+    batch_rate = nsets / t_batch
+    return {"sets": nsets, "points": int(sizes.sum()), "triangles": int(F),
+            "sets_per_s": round(batch_rate, 1), "ms_per_batch": round(t_batch * 1e3, 1),
+            "plain_sets_per_s": round(plain_rate, 1), "plain_sample_sets": ns,
+            "speedup_vs_plain": round(batch_rate / plain_rate, 2),
+            "batch_stage_ms": {k: round(v, 1) for k, v in st.items() if k.startswith("ms_")},
+            "workload": f"{nsets} sets, sizes log-uniform in [256, 65536], uniform or 8-cluster Gaussian "
+                        f"(seed {seed}), autotuned textures, DT + alpha, device-resident"}
+
+
+def serving(G, qps_list, duration_s, seed=5, slots=4, budget_gb=96, max_group=64):
+    """Serving workload: Poisson arrivals at a fixed rate, sizes log-uniform in [1,000, 2,000,000]
+    i.i.d. uniform points (distinct windows of one seeded pool), DT + alpha per request, results
+    copied to host.  Latency is measured inside the library from submit to completion."""
+    import threading
+    import numpy as np
+    from galpha2d_amd import datagen
+    rng = np.random.default_rng(seed)
+    pool = datagen.uniform(seed, 2_000_000 + 1)
+    srv = G.Server(0, slots=slots, hbm_budget_bytes=budget_gb << 30, max_group=max_group)
+    # warm-up: one large and a few small requests (buffer growth)
+    for n in (2_000_000, 300_000, 3000):
+        t = srv.submit(pool[:n], with_alpha=True)
+        srv.wait(t)
+        srv.release(t)
+    out = []
+    for qps in qps_list:
+        nreq = max(1, int(qps * duration_s))
+        gaps = rng.exponential(1.0 / qps, nreq)
+        sizes = np.floor(np.exp(rng.uniform(np.log(1000), np.log(2_000_000), nreq))).astype(np.int64)
+        starts = rng.integers(0, pool.shape[0] - sizes + 1)
+        tickets = [None] * nreq
+        res = [None] * nreq
+        ready = threading.Semaphore(0)
+
+        def waiter():
+            for i in range(nreq):
+                ready.acquire()
+                F, lat, q, g = srv.wait(tickets[i])
+                srv.release(tickets[i])
+                res[i] = (lat, q, g)
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        t_next = t0
+        for i in range(nreq):
+            t_next += gaps[i]
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = srv.submit(pool[starts[i]:starts[i] + sizes[i]], with_alpha=True)
+            ready.release()
+        th.join()
+        wall = time.perf_counter() - t0
+        lat = np.array([r[0] for r in res])
+        grp = np.array([r[2] for r in res])
+        out.append({"qps_offered": qps, "requests": nreq, "points": int(sizes.sum()),
+                    "sets_per_s": round(nreq / wall, 1), "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                    "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                    "mean_queue_ms": round(float(np.mean([r[1] for r in res])), 2),
+                    "mean_group": round(float(grp.mean()), 2)})
+    st = srv.stats()
+    srv.close()
+    return {"runs": out, "slots": slots, "hbm_budget_gb": budget_gb, "max_group": max_group,
+            "held_gb_end": round(st["held_bytes"] / 2**30, 2),
+            "workload": f"Poisson arrivals, sizes log-uniform in [1000, 2000000] uniform points (seed {seed}), "
+                        f"DT + alpha, results copied to host"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,6 +170,10 @@ def main():
     ap.add_argument("--texture", type=int, default=TEXTURE)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe", default="k_pba_color", help="kernel timed for the roofline figure")
+    ap.add_argument("--batched-sets", type=int, default=4096, help="sets of the batched workload (0: skip)")
+    ap.add_argument("--plain-sample", type=int, default=256, help="sets timed for the plain per-set loop")
+    ap.add_argument("--serve-qps", default="50,150", help="offered request rates of the serving runs ('' skips)")
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device once, so that a counter run sees a plain "
                          "copy of known size beside the pipeline kernels")
@@ -135,9 +253,15 @@ def main():
                       "alpha_exact_attach": st["alphaExactAttach"]},
         "roofline": roof,
     }
+    ctx.close()
+    del d_xy
+    torch.cuda.empty_cache()
+    if args.batched_sets > 0:
+        out["batched"] = batched_vs_plain(G, torch, args.batched_sets, args.plain_sample)
+    if args.serve_qps:
+        out["serving"] = serving(G, [float(q) for q in args.serve_qps.split(",")], args.serve_seconds)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, SEED)
-    ctx.close()
     print(json.dumps(out))
 
 

@@ -52,8 +52,10 @@ EXPORTS = ["gpudtComputeDT", "gpudtReleaseDTOutput", "gpudtGetLastError", "gpudt
            "gpudtReleaseAlpha", "galphaCreate", "galphaDestroy", "galphaLastError",
            "galphaComputeHost", "galphaComputeDevice", "galphaDeviceTriangles",
            "galphaCopyTriangles", "galphaGetStats", "galphaVoronoi", "galphaCopyAlpha",
-           "galphaBatch", "galphaBatchCopyAlpha", "galphaFree", "galphaAutoTexture", "galphaKernelProbe",
-           "galphaKernelProbeResult"]
+           "galphaBatch", "galphaBatchDevice", "galphaBatchTriangles", "galphaBatchCopyAlpha", "galphaFree", "galphaAutoTexture", "galphaKernelProbe",
+           "galphaKernelProbeResult", "galphaServerCreate", "galphaServerDestroy", "galphaServerSubmit",
+           "galphaServerWait", "galphaServerTriangles", "galphaServerAlpha", "galphaServerRelease",
+           "galphaServerStats"]
 
 
 def lib():
@@ -90,9 +92,28 @@ def lib():
     L.galphaAutoTexture.restype = C.c_int
     L.galphaBatch.argtypes = [vp, dp, ip, C.c_int, ip, C.c_int, C.POINTER(ip), C.POINTER(ip)]
     L.galphaBatch.restype = C.c_int
+    L.galphaBatchDevice.argtypes = [vp, vp, ip, C.c_int, ip, C.c_int, vp]
+    L.galphaBatchDevice.restype = C.c_int
+    L.galphaBatchTriangles.argtypes = [vp, C.c_int, ip, C.c_int]
+    L.galphaBatchTriangles.restype = C.c_int
     L.galphaBatchCopyAlpha.argtypes = [vp, C.c_int, C.POINTER(C.POINTER(_Alpha))]
     L.galphaBatchCopyAlpha.restype = C.c_int
     L.galphaFree.argtypes = [vp]
+    L.galphaServerCreate.argtypes = [C.c_int, C.c_int, C.c_size_t, C.c_int]
+    L.galphaServerCreate.restype = vp
+    L.galphaServerDestroy.argtypes = [vp]
+    L.galphaServerSubmit.argtypes = [vp, dp, C.c_int, C.c_int, C.c_int]
+    L.galphaServerSubmit.restype = C.c_longlong
+    L.galphaServerWait.argtypes = [vp, C.c_longlong, dp, dp, ip]
+    L.galphaServerWait.restype = C.c_int
+    L.galphaServerTriangles.argtypes = [vp, C.c_longlong, ip, C.c_int]
+    L.galphaServerTriangles.restype = C.c_int
+    L.galphaServerAlpha.argtypes = [vp, C.c_longlong, C.POINTER(C.POINTER(_Alpha))]
+    L.galphaServerAlpha.restype = C.c_int
+    L.galphaServerRelease.argtypes = [vp, C.c_longlong]
+    L.galphaServerRelease.restype = C.c_int
+    L.galphaServerStats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_size_t)]
+    L.galphaServerStats.restype = C.c_int
     _lib = L
     return L
 
@@ -196,6 +217,28 @@ class Context:
             self._L.galphaFree(C.cast(op, C.c_void_p))
         return [tris[toff[k]:toff[k + 1]] for k in range(ns)]
 
+    def batch_device(self, d_xy_ptr, offsets, fbo_sizes=None, with_alpha=False, stream=None):
+        """Device-resident batch: d_xy_ptr is the device address of all sets' (x, y) doubles,
+        offsets[nsets + 1] in points.  Results stay on the device; returns the triangle total."""
+        offs = np.ascontiguousarray(offsets, dtype=np.int32)
+        Ts = None if fbo_sizes is None else np.ascontiguousarray(fbo_sizes, dtype=np.int32)
+        F = self._L.galphaBatchDevice(self._h, C.c_void_p(d_xy_ptr), _ip(offs), len(offs) - 1,
+                                      _ip(Ts) if Ts is not None else None, 1 if with_alpha else 0,
+                                      C.c_void_p(stream or 0))
+        if F < 0:
+            self._err("galphaBatchDevice")
+        return F
+
+    def batch_triangles(self, k):
+        """Triangles of set k of the last batch (set-local indices)."""
+        F = self._L.galphaBatchTriangles(self._h, int(k), None, 0)
+        if F < 0:
+            self._err("galphaBatchTriangles")
+        out = np.zeros((F, 9), dtype=np.int32)
+        if F > 0 and self._L.galphaBatchTriangles(self._h, int(k), _ip(out), F) < 0:
+            self._err("galphaBatchTriangles")
+        return out
+
     def batch_alpha(self, k):
         """Alpha result of set k of the last batch(with_alpha=True)."""
         A = C.POINTER(_Alpha)()
@@ -266,6 +309,73 @@ class Context:
 
 
 _default = None
+
+
+class Server:
+    """Request loop on one device: `slots` contexts with their own streams, admission by HBM
+    footprint, continuous batching of queued requests (see include/gpudt.h, galphaServer*)."""
+
+    def __init__(self, device=0, slots=4, hbm_budget_bytes=64 << 30, max_group=64):
+        self._L = lib()
+        self._h = self._L.galphaServerCreate(int(device), int(slots), int(hbm_budget_bytes), int(max_group))
+        if not self._h:
+            raise GalphaError("galphaServerCreate failed")
+
+    def submit(self, xy, fbo_size=0, with_alpha=True):
+        """Queues one request; returns its ticket."""
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+        t = self._L.galphaServerSubmit(self._h, _dp(xy), xy.shape[0], int(fbo_size), 1 if with_alpha else 0)
+        if t == -2:
+            raise GalphaError("request exceeds the server's HBM budget")
+        if t < 0:
+            raise GalphaError("galphaServerSubmit failed")
+        return t
+
+    def wait(self, ticket):
+        """Blocks until the ticket is done: (triangles, latency_ms, queue_ms, group)."""
+        lat, q, g = C.c_double(), C.c_double(), C.c_int()
+        F = self._L.galphaServerWait(self._h, C.c_longlong(ticket), C.byref(lat), C.byref(q), C.byref(g))
+        if F < 0:
+            raise GalphaError(f"request {ticket} failed")
+        return F, lat.value, q.value, g.value
+
+    def result(self, ticket, alpha=True):
+        """Triangles (int32 [F, 9]) and, if requested at submit, the alpha dict; forgets the ticket."""
+        F = self._L.galphaServerTriangles(self._h, C.c_longlong(ticket), None, 0)
+        if F < 0:
+            raise GalphaError(f"request {ticket} has no result")
+        tris = np.zeros((F, 9), dtype=np.int32)
+        if F > 0:
+            self._L.galphaServerTriangles(self._h, C.c_longlong(ticket), _ip(tris), F)
+        a = None
+        if alpha:
+            A = C.POINTER(_Alpha)()
+            if self._L.galphaServerAlpha(self._h, C.c_longlong(ticket), C.byref(A)) == 0:
+                try:
+                    a = _alpha_to_numpy(A)
+                finally:
+                    self._L.gpudtReleaseAlpha(A)
+        self._L.galphaServerRelease(self._h, C.c_longlong(ticket))
+        return tris, a
+
+    def release(self, ticket):
+        self._L.galphaServerRelease(self._h, C.c_longlong(ticket))
+
+    def stats(self):
+        p, s, h = C.c_longlong(), C.c_longlong(), C.c_size_t()
+        self._L.galphaServerStats(self._h, C.byref(p), C.byref(s), C.byref(h))
+        return {"passes": p.value, "served": s.value, "held_bytes": h.value}
+
+    def close(self):
+        if self._h:
+            self._L.galphaServerDestroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def default_context():

@@ -82,6 +82,7 @@ struct galpha_ctx {
     int plan_ntot = 0;
     std::vector<int> batch_sets;      // input set of each planned set of the last batch
     std::vector<int> batch_npts;      // points of every input set of the last batch
+    const double *batch_xy = nullptr; // device points of the planned sets of the last batch
     AlphaDev alpha;
     int last_n = 0, last_F = 0, last_T = 0;
     bool last_alpha = false;
@@ -652,7 +653,7 @@ static AlphaBatchInfo batch_info(galpha_ctx *c)
 // plan (no triangles); c->batch_sets maps planned sets to input sets.  Returns the total number
 // of triangles.
 static int batch_run(galpha_ctx *c, const double *xy, const int *offsets, int nsets, const int *fboSizes,
-                     int with_alpha)
+                     int with_alpha, bool device_input = false)
 {
     hipStream_t st = c->stream;
     c->batch_sets.clear();
@@ -676,28 +677,34 @@ static int batch_run(galpha_ctx *c, const double *xy, const int *offsets, int ns
     c->xy.ensure(16 * (size_t)std::max<long long>(total, 1));
     std::vector<int> poffs(na + 1, 0);
     for (int k = 0; k < na; k++) poffs[k + 1] = poffs[k] + ns[k];
-    if (na == nsets && nsets > 0) {
-        HIPCHK(hipMemcpyAsync(c->xy.p, xy + 2 * (size_t)offsets[0], 16 * (size_t)total, hipMemcpyHostToDevice, st));
+    hipMemcpyKind kind = device_input ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const double *dx = nullptr;   // device points of the planned sets, contiguous
+    if (device_input && na == nsets && nsets > 0) {
+        dx = xy + 2 * (size_t)offsets[0];
+    } else if (na == nsets && nsets > 0) {
+        HIPCHK(hipMemcpyAsync(c->xy.p, xy + 2 * (size_t)offsets[0], 16 * (size_t)total, kind, st));
+        dx = c->xy.as<double>();
     } else {
         for (int k = 0; k < na; k++)
             HIPCHK(hipMemcpyAsync(c->xy.as<double>() + 2 * (size_t)poffs[k], xy + 2 * (size_t)offsets[c->batch_sets[k]],
-                                  16 * (size_t)ns[k], hipMemcpyHostToDevice, st));
+                                  16 * (size_t)ns[k], kind, st));
+        dx = c->xy.as<double>();
     }
     std::vector<double> bb5((size_t)5 * std::max(na, 1));
     if (na > 0) {
         c->offsD.ensure(4 * (size_t)(na + 1));
         c->bboxTmp.ensure(8 * 5 * (size_t)na);
         HIPCHK(hipMemcpyAsync(c->offsD.p, poffs.data(), 4 * (size_t)(na + 1), hipMemcpyHostToDevice, st));
-        bbox_sets_device(c->xy.as<double>(), c->offsD.as<int>(), na, bb5.data(), c->bboxTmp.p, st);
+        bbox_sets_device(dx, c->offsD.as<int>(), na, bb5.data(), c->bboxTmp.p, st);
     }
     BatchPlan plan = plan_batch(ns.data(), Ts.data(), bb5.data(), na);
-    int F = run_batch(c, c->xy.as<double>(), plan);
+    int F = run_batch(c, dx, plan);
+    c->batch_xy = dx;
     c->last_n = (int)total;
     c->last_alpha = false;
     if (with_alpha) {
         auto t0 = std::chrono::steady_clock::now();
-        alpha_run_batch(c->alpha, c->xy.as<double>(), plan.ntot, c->out9.as<int>(), F, batch_info(c), st,
-                        &c->stats);
+        alpha_run_batch(c->alpha, dx, plan.ntot, c->out9.as<int>(), F, batch_info(c), st, &c->stats);
         c->last_alpha = true;
         HIPCHK(hipStreamSynchronize(st));
         auto t1 = std::chrono::steady_clock::now();
@@ -738,6 +745,37 @@ static void release_all(galpha_ctx *c)
     for (DevBuf *b : bufs) b->release();
     alpha_release(c->alpha);
 }
+
+}  // extern "C"
+
+// device bytes held by a context (all grown buffers)
+size_t galpha_ctx_device_bytes(galpha_ctx *c)
+{
+    DevBuf *bufs[] = {&c->smap, &c->g, &c->S, &c->cnt, &c->E, &c->ecnt, &c->vor, &c->bad, &c->fpix, &c->tv,
+                      &c->tn, &c->remap, &c->stamp, &c->claim, &c->modA, &c->modB, &c->cand, &c->pos, &c->vt,
+                      &c->vstate, &c->pix, &c->deg, &c->off, &c->fill, &c->fan, &c->moved, &c->decide, &c->mx,
+                      &c->mlist, &c->loc, &c->lt, &c->scratch, &c->hnext, &c->hedge, &c->keys, &c->keys2,
+                      &c->vals, &c->vals2, &c->rot, &c->newid, &c->out9, &c->counters, &c->status, &c->xy,
+                      &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
+                      &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
+                      &c->offsD, &c->bboxTmp};
+    size_t t = 0;
+    for (DevBuf *b : bufs) t += b->bytes;
+    for (auto &b : c->alpha.b) t += b.bytes;
+    return t;
+}
+
+// gives every device buffer of an idle context back (they grow again on the next run)
+void galpha_ctx_trim(galpha_ctx *c)
+{
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    release_all(c);
+    c->last_F = 0;
+    c->last_alpha = false;
+}
+
+extern "C" {
 
 void galphaDestroy(galpha_ctx *c)
 {
@@ -953,6 +991,63 @@ void galphaFree(void *p) { free(p); }
 // Batched entry point: all sets go through one pass of the pipeline on this context (every launch
 // covers every set).  Results are concatenated per input set with set-local vertex and triangle
 // indices.  tris9_out / tri_offsets_out are malloc'ed (release with galphaFree).
+// triangles of planned set k with set-local indices into out (9 ints each)
+static void copy_planned_set(galpha_ctx *c, int k, int *out)
+{
+    int t0 = c->set_first[k], t1 = c->set_first[k + 1];
+    if (t1 <= t0) return;
+    ctx_read(c, out, c->out9.as<int>() + (size_t)9 * t0, 36 * (size_t)(t1 - t0));
+    int poff = c->plan_sets[k].poff;
+    for (int t = 0; t < t1 - t0; t++) {
+        int *o = out + (size_t)9 * t;
+        for (int i = 0; i < 3; i++) {
+            o[3 + i] -= poff;
+            if (o[i] >= 0) o[i] = ((o[i] >> 2) - t0) << 2 | (o[i] & 3);
+        }
+    }
+}
+
+int galphaBatchDevice(galpha_ctx *c, const double *d_xy, const int *offsets, int nsets, const int *fboSizes,
+                      int with_alpha, void *stream)
+{
+    if (!c || !offsets || nsets < 0) return -1;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        hipStream_t own = c->stream;
+        if (stream) c->stream = (hipStream_t)stream;
+        int F;
+        try {
+            F = batch_run(c, d_xy, offsets, nsets, fboSizes, with_alpha, true);
+        } catch (...) {
+            c->stream = own;
+            throw;
+        }
+        c->stream = own;
+        return F;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int galphaBatchTriangles(galpha_ctx *c, int set, int *tris9, int capacity)
+{
+    if (!c) return -1;
+    try {
+        if (set < 0 || set >= (int)c->batch_npts.size()) throw GalphaError("set index out of range");
+        int k = -1;
+        for (int q = 0; q < (int)c->batch_sets.size(); q++)
+            if (c->batch_sets[q] == set) { k = q; break; }
+        if (k < 0) return 0;
+        int F = c->set_first[k + 1] - c->set_first[k];
+        if (!tris9) return F;
+        if (capacity < F) throw GalphaError("output capacity too small");
+        copy_planned_set(c, k, tris9);
+        return F;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
 int galphaBatch(galpha_ctx *c, const double *xy, const int *offsets, int nsets, const int *fboSizes,
                 int with_alpha, int **tris9_out, int **tri_offsets_out)
 {

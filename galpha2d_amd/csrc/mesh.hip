@@ -469,6 +469,11 @@ __global__ void k_del_apply(MeshDev m, const unsigned long long *mx, int round, 
             t = enc_t(e);
             i = p1(enc_j(e));
         }
+        // The hole is star-shaped from K (v's position): K lies left of every polygon edge.
+        // An ear (A, B, C) is cut when it is counter-clockwise and K is left of or on the new
+        // edge A -> C; that keeps K left of or on every edge of the remaining polygon, so the
+        // ear holds no other polygon vertex (they are in angular order around K) and an ear
+        // always exists.  (K strictly inside the ear is excluded by the same test.)
         double2 K = m.pos[v];
         int msz = d, used = 0;
         while (msz > 3) {
@@ -477,8 +482,7 @@ __global__ void k_del_apply(MeshDev m, const unsigned long long *mx, int round, 
                 int kp = k == 0 ? msz - 1 : k - 1, kn = k == msz - 1 ? 0 : k + 1;
                 double2 A = m.pos[poly[kp]], B = m.pos[poly[k]], C = m.pos[poly[kn]];
                 if (orient_dev(m, A, B, C) <= 0) continue;
-                bool inside = orient_dev(m, A, B, K) > 0 && orient_dev(m, B, C, K) > 0 && orient_dev(m, C, A, K) > 0;
-                if (inside) continue;
+                if (orient_dev(m, A, C, K) < 0) continue;
                 pick = k;
             }
             if (pick < 0) {
@@ -513,6 +517,7 @@ __global__ void k_del_apply(MeshDev m, const unsigned long long *mx, int round, 
             msz--;
         }
         if (msz == 3) {
+            if (orient_dev(m, m.pos[poly[0]], m.pos[poly[1]], m.pos[poly[2]]) <= 0) atomicOr(m.status, GA_ERR_PATCH);
             int T = slots[used++];
             m.tv[T] = make_int4(poly[0], poly[1], poly[2], 0);
             m.tn[T] = make_int4(-1, -1, -1, 0);

@@ -145,9 +145,33 @@ int galphaCopyAlpha(galpha_ctx *ctx, PGPUDTALPHA *pAlpha);
  * Each set's result is identical to a single-set run on the same points. */
 int galphaBatch(galpha_ctx *ctx, const double *xy, const int *offsets, int nsets, const int *fboSizes,
                 int with_alpha, int **tris9_out, int **tri_offsets_out);
-/* alpha result of input set `set` of the last galphaBatch(with_alpha = 1) (gpudtReleaseAlpha) */
+/* Device-resident batch: d_xy is a device pointer; results stay on the device.  Returns the
+ * total number of triangles.  Query per set with galphaBatchTriangles / galphaBatchCopyAlpha. */
+int galphaBatchDevice(galpha_ctx *ctx, const double *d_xy, const int *offsets, int nsets, const int *fboSizes,
+                      int with_alpha, void *stream);
+/* triangles of input set `set` of the last batch with set-local indices (NULL tris9: count only) */
+int galphaBatchTriangles(galpha_ctx *ctx, int set, int *tris9, int capacity);
+/* alpha result of input set `set` of the last batch run with alpha (release with gpudtReleaseAlpha) */
 int galphaBatchCopyAlpha(galpha_ctx *ctx, int set, PGPUDTALPHA *pAlpha);
 void galphaFree(void *p);
+
+/* Serving: `slots` worker threads, each with its own context and HIP stream (buffers stay
+ * resident).  Requests queue in arrival order; a free slot admits the longest prefix of the
+ * queue that fits hbm_budget_bytes (bytes held by all slots, its own grown to the admitted
+ * requests' footprint), up to max_group requests, and runs them as one batched pass.  Results
+ * are copied to host memory owned by the ticket.  Every result equals a single-set run. */
+typedef struct galpha_server galpha_server;
+galpha_server *galphaServerCreate(int device, int slots, size_t hbm_budget_bytes, int max_group);
+void galphaServerDestroy(galpha_server *server);
+/* returns a ticket (> 0), -2 when the request alone exceeds the budget, -1 on bad arguments */
+long long galphaServerSubmit(galpha_server *server, const double *xy, int n, int fboSize, int with_alpha);
+/* blocks until the ticket is done; returns its triangle count (< 0 on failure); latency_ms =
+ * submit -> done, queue_ms = submit -> start of its pass, group = requests in that pass */
+int galphaServerWait(galpha_server *server, long long ticket, double *latency_ms, double *queue_ms, int *group);
+int galphaServerTriangles(galpha_server *server, long long ticket, int *tris9, int capacity);
+int galphaServerAlpha(galpha_server *server, long long ticket, PGPUDTALPHA *pAlpha);
+int galphaServerRelease(galpha_server *server, long long ticket);
+int galphaServerStats(galpha_server *server, long long *passes, long long *served, size_t *held_bytes);
 
 /* Kernel timing probe: while enabled, HIP events are recorded on the launch stream around every
  * launch of the named kernel; NULL disables.  Result: number of launches and total ms. */

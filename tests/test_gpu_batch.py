@@ -84,3 +84,20 @@ def test_batch_equals_plain_loop():
         plain = ctx.delaunay(xy)
         assert np.array_equal(tris[k], plain), k
     ctx.close()
+
+
+def test_deletion_hole_through_diagonal(oracle):
+    """A clustered set (set 171 of the batched workload generator, seed 4) whose snapped
+    positions put a deleted vertex exactly on an ear diagonal of its hole; the hole must still be
+    triangulated counter-clockwise (plain and batched paths)."""
+    import os
+    import galpha2d_amd as G
+    from conftest import GOLDEN
+    print("case: clustered 19976-point set, deletion hole with K on a diagonal", flush=True)
+    xy = np.load(os.path.join(GOLDEN, "clustered_19976.npy"))
+    ctx = G.Context(0)
+    ref, _ = oracle.delaunay(xy)
+    assert np.array_equal(ctx.delaunay(xy), ref)
+    tris = ctx.batch([xy, oracle.gen_uniform(3, 500)], with_alpha=False)
+    assert np.array_equal(tris[0], ref)
+    ctx.close()
