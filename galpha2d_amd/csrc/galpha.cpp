@@ -21,11 +21,6 @@
 #include <string>
 #include <vector>
 
-void set_deg_for_init(int *deg);
-void set_decide_buffer(int *d);
-void set_del_list(int *l);
-void set_lt_buffer(int *b);
-void set_hull_buffers(int *hn, int *he);
 
 static thread_local std::string g_last_error;
 
@@ -412,16 +407,17 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     m.scratch = c->scratch.as<int>(); m.scratch_cap = scap;
     c->dbg.ensure(64 * sizeof(double));
     m.dbg = c->dbg.as<double>();
-    set_decide_buffer(c->decide.as<int>());
-    set_del_list(c->dellist.as<int>());
-    set_lt_buffer(c->lt.as<int>());
-    set_hull_buffers(c->hnext.as<int>(), c->hedge.as<int>());
+    m.deg = c->deg.as<int>();
+    m.decide = c->decide.as<int>();
+    m.dellist = c->dellist.as<int>();
+    m.lt = c->lt.as<int>();
+    m.hnext = c->hnext.as<int>();
+    m.hedge = c->hedge.as<int>();
 
     // ---- reconstruction + adjacency
     launch_recon(c->vor.as<int>(), c->pix.as<int>(), bd, m, c->tileCnt.as<int>(), c->tileOff.as<int>(),
                  c->scanTmp.p, st);
     launch_adjacency(m, c->deg.as<int>(), c->off.as<int>(), c->fill.as<int>(), c->fan.as<int>(), c->scanTmp.p, st);
-    set_deg_for_init(c->deg.as<int>());
     launch_init_positions(m, st);
     S.reconTris = ctx_counter(c, 40);
     if (ctx_counter(c, 40) + 0 > cap) throw GalphaError("triangle capacity exceeded in reconstruction");

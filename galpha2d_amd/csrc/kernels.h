@@ -91,6 +91,12 @@ struct MeshDev {
     int *scratch;        // per-winner scratch arena
     size_t scratch_cap;
     double *dbg;         // failure dump: positions (K + up to 15 polygon vertices) and states
+    // per-context work buffers (never process globals: contexts run concurrently)
+    int *deg;            // [V + 1] vertex degrees of the reconstruction
+    int *decide;         // [V] greedy shifting decisions
+    int *dellist;        // [cap] triangles created by a deletion round
+    int *lt;             // [n] located type of every missing site
+    int *hnext, *hedge;  // [V] hull successor and hull edge of every hull vertex
 };
 
 // vertex states

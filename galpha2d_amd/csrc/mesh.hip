@@ -286,11 +286,9 @@ __global__ void k_init_positions(MeshDev m, const int *__restrict__ deg)
     }
 }
 
-static int *g_deg_for_init = nullptr;
-void set_deg_for_init(int *deg) { g_deg_for_init = deg; }
 void launch_init_positions(MeshDev &m, hipStream_t st)
 {
-    GA_LAUNCH(k_init_positions, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)g_deg_for_init);
+    GA_LAUNCH(k_init_positions, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)m.deg);
 }
 
 // ---- shifting ----------------------------------------------------------------------------------
@@ -393,12 +391,10 @@ __global__ void k_shift_greedy_commit(MeshDev m, const int *decide)
     }
 }
 
-static int *g_decide = nullptr;
-void set_decide_buffer(int *d) { g_decide = d; }
 void launch_shift_greedy(MeshDev &m, int round, hipStream_t st)
 {
-    GA_LAUNCH(k_shift_greedy, dim3(FIXED_GRID), dim3(256), 0, st, m, round, g_decide);
-    GA_LAUNCH(k_shift_greedy_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)g_decide);
+    GA_LAUNCH(k_shift_greedy, dim3(FIXED_GRID), dim3(256), 0, st, m, round, m.decide);
+    GA_LAUNCH(k_shift_greedy_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)m.decide);
 }
 
 // ---- deletion of vertices that cannot move (star-shaped hole, ear clipping) --------------------
@@ -536,15 +532,13 @@ __global__ void k_del_apply(MeshDev m, const unsigned long long *mx, int round, 
     }
 }
 
-static int *g_del_list = nullptr;
-void set_del_list(int *l) { g_del_list = l; }
 void launch_delete_round(MeshDev &m, unsigned int *mxraw, int round, int op, hipStream_t st)
 {
     unsigned long long *mx = (unsigned long long *)mxraw;
     hipMemsetAsync(mx, 0, sizeof(unsigned long long) * m.V, st);
     GA_LAUNCH(k_del_prio, dim3(FIXED_GRID), dim3(256), 0, st, m, mx, round);
     GA_LAUNCH(k_del_apply, dim3(FIXED_GRID), dim3(256), 0, st, m, (const unsigned long long *)mx, round, op,
-                       g_del_list);
+                       m.dellist);
 }
 
 // ---- missing-site insertion ------------------------------------------------------------------
@@ -707,14 +701,12 @@ __global__ void k_insert(MeshDev m, const int *mlist, const int *loc, const int 
     }
 }
 
-static int *g_lt = nullptr;
-void set_lt_buffer(int *b) { g_lt = b; }
 void launch_insert_round(MeshDev &m, int *mlist, int nmiss_cap, int *loc, int op, hipStream_t st)
 {
     (void)nmiss_cap;
-    GA_LAUNCH(k_locate, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)mlist, loc, g_lt, op);
+    GA_LAUNCH(k_locate, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)mlist, loc, m.lt, op);
     GA_LAUNCH(k_insert, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)mlist, (const int *)loc,
-                       (const int *)g_lt, op, m.modA);
+                       (const int *)m.lt, op, m.modA);
 }
 
 // ---- generic link repair -------------------------------------------------------------------
@@ -1080,17 +1072,15 @@ __global__ void k_hull_edges(MeshDev m, int *hnext, int *hedge, int *hull_cnt, i
     }
 }
 
-static int *g_hnext = nullptr, *g_hedge = nullptr;
-void set_hull_buffers(int *hn, int *he) { g_hnext = hn; g_hedge = he; }
 void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, int *hull_cnt, int *hull_off,
                       int *hull_start, void *scan_tmp, hipStream_t st)
 {
-    hipMemsetAsync(g_hnext, 0xff, sizeof(int) * m.V, st);
+    hipMemsetAsync(m.hnext, 0xff, sizeof(int) * m.V, st);
     hipMemsetAsync(hull_cnt, 0, sizeof(int) * m.nsets, st);
     launch_fill_i32(hull_start, 0x7fffffff, (size_t)m.nsets, st);
-    GA_LAUNCH(k_hull_edges, dim3(FIXED_GRID), dim3(256), 0, st, m, g_hnext, g_hedge, hull_cnt, hull_start);
+    GA_LAUNCH(k_hull_edges, dim3(FIXED_GRID), dim3(256), 0, st, m, m.hnext, m.hedge, hull_cnt, hull_start);
     launch_exclusive_scan(hull_cnt, hull_off, m.nsets, scan_tmp, st);
-    GA_LAUNCH(k_convexify, dim3(m.nsets), dim3(256), 0, st, m, (const int *)g_hnext, (const int *)g_hedge,
+    GA_LAUNCH(k_convexify, dim3(m.nsets), dim3(256), 0, st, m, (const int *)m.hnext, (const int *)m.hedge,
               (const int *)hull_cnt, (const int *)hull_off, (const int *)hull_start, list_out, cnt_out_slot, op);
 }
 

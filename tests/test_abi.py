@@ -86,3 +86,33 @@ def test_auto_texture_rule():
     assert G.auto_texture(1000) == 256
     assert G.auto_texture(1048576) == 4096
     assert G.auto_texture(2000000) == 8192
+
+
+def test_no_process_global_work_state():
+    """Contexts run concurrently (one per serving slot), so the library may keep no work buffers or
+    pointers in process globals: the only writable globals are the mutex-protected default context
+    and kernel-probe state, the thread-local error text and C++ runtime bookkeeping."""
+    import re
+    import shutil
+    import subprocess
+    import galpha2d_amd as G
+    if shutil.which("nm") is None:
+        pytest.skip("nm not available")
+    out = subprocess.run(["nm", "-C", "--defined-only", G.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    allowed = {"g_default", "g_default_mu", "g_last_error[abi:cxx11]", "(anonymous namespace)::g_ev",
+               "(anonymous namespace)::g_mu", "_ZN12_GLOBAL__N_14g_onE.0", "(anonymous namespace)::g_name[abi:cxx11]",
+               "(anonymous namespace)::g_used", "std::__ioinit", "__tls_guard", "__fini", "__init"}
+    runtime = re.compile(r"rocprim|__hip|^_ZZ|guard variable|typeinfo|vtable|__dso_handle|__TMC_END__|completed\.|"
+                         r"__init_array|__fini_array|__frame_dummy|__do_global_dtors|DW\.ref|^_DYNAMIC|"
+                         r"^_GLOBAL_OFFSET_TABLE_|^__do_(init|fini)\.")
+    bad = []
+    for line in out.splitlines():
+        parts = line.split(None, 2)
+        if len(parts) < 3 or parts[1] not in "bBdD":
+            continue
+        name = parts[2]
+        if re.match(r"k_[a-z0-9_]+\(", name) or runtime.search(name) or name in allowed:
+            continue
+        bad.append(name)
+    assert not bad, f"process-global writable state: {bad}"
