@@ -720,19 +720,21 @@ __global__ void k_spec_values(const unsigned long long *bits, int nslots, double
     GS(r, nslots) spec[r] = bits[r] == ~0ull ? 0.0 : __longlong_as_double((long long)bits[r]);
 }
 
-// first master entry of every set: keys of set s lie in (specstart[s], specstart[s + 1])
+// first master entry of every set: keys are sorted and the keys of set s lie in
+// (specstart[s], specstart[s + 1]), so it is the first position with key > specstart[s]
+// (Mk when the set has no master entry); one binary search per set
 __global__ void k_master_set_first(int Mk, const unsigned int *skey, const int *specstart, int nsets, int *mfirst)
 {
-    GS(k, Mk) {
-        int key = (int)skey[k];
-        int lo = 0, hi = nsets - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (specstart[mid] < key) lo = mid;
-            else hi = mid - 1;
-        }
-        atomicMin(&mfirst[lo], k);
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsets) return;
+    unsigned int key = (unsigned)specstart[s];
+    int lo = 0, hi = Mk;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (skey[mid] <= key) lo = mid + 1;
+        else hi = mid;
     }
+    mfirst[s] = lo;
 }
 
 // first edge of every set = edge offset of its first triangle (E when the set has no triangle)
@@ -1071,10 +1073,9 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
         // first master entry and first edge of every set
         ab(d, AB_MSET, 4 * 3 * (size_t)(nsets + 1));
         int *mf = AP<int>(d, AB_MSET), *tf = mf + (nsets + 1), *ef = tf + (nsets + 1);
-        launch_fill_i32(mf, 0x7fffffff, (size_t)nsets, st);
         chk(hipMemcpyAsync(tf, d.tfirst.data(), 4 * (size_t)nsets, hipMemcpyHostToDevice, st), "copy");
-        GA_LAUNCH(k_master_set_first, dim3(AGRID), dim3(256), 0, st, Mk, (const unsigned *)skey, a.specstart,
-                  nsets, mf);
+        GA_LAUNCH(k_master_set_first, dim3((nsets + 255) / 256), dim3(256), 0, st, Mk, (const unsigned *)skey,
+                  a.specstart, nsets, mf);
         GA_LAUNCH(k_edge_set_first, dim3((nsets + 255) / 256), dim3(256), 0, st, (const int *)tf, nsets, F,
                   (const int *)a.eoff, E, ef);
         std::vector<int> hm(nsets), he(nsets);
@@ -1082,7 +1083,7 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
         chk(hipMemcpyAsync(he.data(), ef, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st), "copy");
         chk(hipStreamSynchronize(st), "sync");
         d.mfirst[nsets] = Mk;
-        for (int q = nsets - 1; q >= 0; q--) d.mfirst[q] = hm[q] != 0x7fffffff ? hm[q] : d.mfirst[q + 1];
+        for (int q = 0; q < nsets; q++) d.mfirst[q] = hm[q];
         for (int q = 0; q < nsets; q++) d.efirst[q] = he[q];
         d.efirst[nsets] = E;
     } else {

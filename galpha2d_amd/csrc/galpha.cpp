@@ -71,7 +71,7 @@ struct galpha_ctx {
         moved, decide, mx, mlist, loc, lt, scratch, hnext, hedge, keys, keys2, vals, vals2, rot, newid,
         out9, counters, status, xy, scanTmp, sortTmp, tileCnt, tileOff, dellist, dbg;
     // batch plan on the device
-    DevBuf setsD, bmD, vset, hullCnt, hullOff, hullStart, setFirst, offsD, bboxTmp;
+    DevBuf setsD, bmD, vset, hullCnt, hullOff, hullStart, setFirst, offsD, bboxTmp, lmark;
     std::vector<int> set_first;       // [nsets + 1] first output triangle of each planned set
     std::vector<SetDesc> plan_sets;   // host copy of the last plan
     int plan_ntot = 0;
@@ -390,9 +390,19 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     c->hnext.ensure(4 * (size_t)V);
     c->hedge.ensure(4 * (size_t)V);
     c->dellist.ensure(4 * (size_t)cap);
+    c->lmark.ensure(4 * (size_t)cap);
     HIPCHK(hipMemsetAsync(c->stamp.p, 0, 4 * (size_t)cap, st));
     HIPCHK(hipMemsetAsync(c->claim.p, 0, 8 * (size_t)cap, st));
+    HIPCHK(hipMemsetAsync(c->lmark.p, 0, 4 * (size_t)cap, st));
     HIPCHK(hipMemsetAsync(c->vt.p, 0xff, 4 * (size_t)V, st));
+    // removal rounds (deletion, frame) tag their priorities with an increasing round number, so
+    // the per-vertex maxima are cleared once per run
+    HIPCHK(hipMemsetAsync(c->mx.p, 0, 8 * (size_t)V, st));
+    int prio_round = 0;
+    auto next_prio_round = [&]() {
+        if (++prio_round >= (1 << 20)) throw GalphaError("too many removal rounds");
+        return prio_round;
+    };
 
     MeshDev m{};
     m.tv = c->tv.as<int4>(); m.tn = c->tn.as<int4>(); m.remap = c->remap.as<int4>();
@@ -413,6 +423,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     m.lt = c->lt.as<int>();
     m.hnext = c->hnext.as<int>();
     m.hedge = c->hedge.as<int>();
+    m.lmark = c->lmark.as<int>();
 
     // ---- reconstruction + adjacency
     launch_recon(c->vor.as<int>(), c->pix.as<int>(), bd, m, c->tileCnt.as<int>(), c->tileOff.as<int>(),
@@ -429,31 +440,35 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     HIPCHK(hipEventRecord(c->ev[2], st));
     // ---- shifting
     // revert moved vertices of non-ccw triangles until every triangle is ccw (monotone)
-    for (int it = 0;; it++) {
+    // Round loops launch several rounds per host read of their counters: a round after
+    // convergence finds nothing to do, so over-running is harmless.
+    for (int it = 0;; it += 4) {
         if (it > 100000) throw GalphaError("bulk shifting did not converge");
-        ctx_counter_set(c, C_FLAG, 0);
-        launch_shift_bulk(m, c->moved.as<unsigned char>(), it == 0, st);
+        launch_reset_counters(c->counters.as<int>(), C_FLAG, -1, -1, -1, st);
+        for (int k = 0; k < 4; k++) launch_shift_bulk(m, c->moved.as<unsigned char>(), it + k == 0, st);
         if (ctx_counter(c, C_FLAG) == 0) break;
     }
     ctx_counter_set(c, C_PEND, 0);
     launch_shift_commit(m, c->moved.as<unsigned char>(), st);
     int pend = ctx_counter(c, C_PEND);
     S.shiftBad = pend;
-    for (int r = 0; r < 48 && pend > 0; r++) {
-        ctx_counter_set(c, C_FLAG, 0);
-        launch_shift_greedy(m, r, st);
+    for (int r = 0; r < 48 && pend > 0; r += 4) {
+        launch_reset_counters(c->counters.as<int>(), C_FLAG, -1, -1, -1, st);
+        for (int k = 0; k < 4; k++) launch_shift_greedy(m, r + k, st);
         int movedn = ctx_counter(c, C_FLAG);
         pend -= movedn;
         if (movedn == 0 && r >= 8) break;
     }
-    for (int r = 0; pend > 0; r++) {
+    int deleted_before = 0;
+    for (int r = 0; pend > 0; r += 2) {
         if (r > 10000) throw GalphaError("vertex deletion did not finish");
-        ctx_counter_set(c, C_SCR, 0);
-        ctx_counter_set(c, C_MODN, 0);
-        int before = ctx_counter(c, C_DELETED);
-        launch_delete_round(m, c->mx.as<unsigned int>(), 1000 + r, c->op++, st);
+        for (int k = 0; k < 2; k++) {
+            launch_reset_counters(c->counters.as<int>(), C_SCR, C_MODN, -1, -1, st);
+            launch_delete_round(m, c->mx.as<unsigned int>(), next_prio_round(), c->op++, st);
+        }
         int after = ctx_counter(c, C_DELETED);
-        pend -= after - before;
+        pend -= after - deleted_before;
+        deleted_before = after;
         int stt = 0;
         stt = ctx_status(c);
         if (stt) {
@@ -478,21 +493,21 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     HIPCHK(hipEventRecord(c->ev[3], st));
 
     // ---- flips to a Delaunay mesh of the current vertices
+    // Lawson passes over the list of triangles to test; a pass with no flip has no candidate
+    // left (the highest candidate always wins), so three passes run per host read.
     auto flip_until_done = [&](int *listA, int cntA, int *listB, int cntB, int prev) {
         int *lin = listA, *lout = listB, sin = cntA, sout = cntB;
-        for (int pass = 0; pass < 100000; pass++) {
-            int opn = c->op++;
-            ctx_counter_set(c, C_CAND, 0);
-            ctx_counter_set(c, sout, 0);
-            ctx_counter_set(c, C_FLIPS, 0);
-            launch_flip_pass(m, lin, sin, lout, sout, prev, opn, st);
-            S.flipPasses++;
-            int fl = ctx_counter(c, C_FLIPS);
-            S.flips += fl;
-            prev = opn;
-            std::swap(lin, lout);
-            std::swap(sin, sout);
-            if (fl == 0) return;
+        for (int pass = 0; pass < 100000; pass += 3) {
+            for (int k = 0; k < 3; k++) {
+                int opn = c->op++;
+                launch_reset_counters(c->counters.as<int>(), C_CAND, sout, C_FLIPS, -1, st);
+                launch_flip_pass(m, lin, sin, lout, sout, prev, opn, st);
+                prev = opn;
+                std::swap(lin, lout);
+                std::swap(sin, sout);
+            }
+            S.flipPasses += 3;
+            if (ctx_counter(c, C_FLIPS) == 0) return;
         }
         throw GalphaError("flipping did not converge");
     };
@@ -534,11 +549,12 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
 
     // ---- frame removal + hull convexification
     ctx_counter_set(c, C_FRAME, V - n);
-    for (int r = 0;; r++) {
+    for (int r = 0;; r += 2) {
         if (r > 100000) throw GalphaError("frame removal did not finish");
-        ctx_counter_set(c, C_SCR, 0);
-        ctx_counter_set(c, C_MODN, 0);
-        launch_frame_round(m, c->mx.as<unsigned int>(), 5000 + r, m.modA, C_MODN, c->op++, st);
+        for (int k = 0; k < 2; k++) {
+            launch_reset_counters(c->counters.as<int>(), C_SCR, C_MODN, -1, -1, st);
+            launch_frame_round(m, c->mx.as<unsigned int>(), next_prio_round(), m.modA, C_MODN, c->op++, st);
+        }
         int left = ctx_counter(c, C_FRAME);
         if (left <= 0) break;
     }
@@ -596,6 +612,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     S.nTris = F;
     int cnts[C_NCOUNTERS];
     ctx_counters_read(c, cnts, 0, C_NCOUNTERS);
+    S.flips = cnts[C_FLIPS_TOTAL];
     S.incircleTests = cnts[C_TESTS_INCIRCLE];
     S.incircleExact = cnts[C_EXACT_INCIRCLE];
     S.orientExact = cnts[C_EXACT_ORIENT];
@@ -737,7 +754,7 @@ static void release_all(galpha_ctx *c)
                       &c->vals, &c->vals2, &c->rot, &c->newid, &c->out9, &c->counters, &c->status, &c->xy,
                       &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
-                      &c->offsD, &c->bboxTmp};
+                      &c->offsD, &c->bboxTmp, &c->lmark};
     for (DevBuf *b : bufs) b->release();
     alpha_release(c->alpha);
 }
@@ -754,7 +771,7 @@ size_t galpha_ctx_device_bytes(galpha_ctx *c)
                       &c->vals, &c->vals2, &c->rot, &c->newid, &c->out9, &c->counters, &c->status, &c->xy,
                       &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
-                      &c->offsD, &c->bboxTmp};
+                      &c->offsD, &c->bboxTmp, &c->lmark};
     size_t t = 0;
     for (DevBuf *b : bufs) t += b->bytes;
     for (auto &b : c->alpha.b) t += b.bytes;

@@ -97,6 +97,7 @@ struct MeshDev {
     int *dellist;        // [cap] triangles created by a deletion round
     int *lt;             // [n] located type of every missing site
     int *hnext, *hedge;  // [V] hull successor and hull edge of every hull vertex
+    int *lmark;          // [cap] operation that last put the slot in a flip list (one entry per pass)
 };
 
 // vertex states
@@ -128,6 +129,7 @@ struct MeshDev {
 #define C_DELETED 18
 #define C_ISLAND_SITES 19
 #define C_HSTART 20
+#define C_FLIPS_TOTAL 21
 #define C_DBG 48          // 48..63: first failure dump
 #define C_NCOUNTERS 64
 
@@ -155,6 +157,7 @@ void launch_delete_round(MeshDev &m, unsigned int *mx, int round, int op, hipStr
 void launch_missing_init(MeshDev &m, const int *vor, int *mlist, int *loc, hipStream_t st);
 void launch_insert_round(MeshDev &m, int *mlist, int nmiss_cap, int *loc, int op, hipStream_t st);
 void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, hipStream_t st);
+void launch_reset_counters(int *counters, int a, int b, int d, int e, hipStream_t st);
 void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st);
 void launch_flip_pass(MeshDev &m, const int *list_in, int cnt_in_slot, int *list_out, int cnt_out_slot,
                       int prev_op, int op, hipStream_t st);

@@ -398,9 +398,12 @@ void launch_shift_greedy(MeshDev &m, int round, hipStream_t st)
 }
 
 // ---- deletion of vertices that cannot move (star-shaped hole, ear clipping) --------------------
+// Priorities of one round of vertex removal: the round number (< 2^20, increasing within a run)
+// in the top bits, so the per-vertex maxima left by earlier rounds always lose and mx[] is
+// cleared once per run; then 12 random bits; the vertex id breaks ties.
 GA_INLINE unsigned long long prio64(int v, int round)
 {
-    return ((unsigned long long)vprio(v, round) << 32) | (unsigned)v;
+    return ((unsigned long long)round << 44) | ((unsigned long long)(vprio(v, round) & 0xfffu) << 32) | (unsigned)v;
 }
 
 // distance-2 independence: every vertex records the largest priority in its closed neighbourhood
@@ -535,7 +538,6 @@ __global__ void k_del_apply(MeshDev m, const unsigned long long *mx, int round, 
 void launch_delete_round(MeshDev &m, unsigned int *mxraw, int round, int op, hipStream_t st)
 {
     unsigned long long *mx = (unsigned long long *)mxraw;
-    hipMemsetAsync(mx, 0, sizeof(unsigned long long) * m.V, st);
     GA_LAUNCH(k_del_prio, dim3(FIXED_GRID), dim3(256), 0, st, m, mx, round);
     GA_LAUNCH(k_del_apply, dim3(FIXED_GRID), dim3(256), 0, st, m, (const unsigned long long *)mx, round, op,
                        m.dellist);
@@ -552,11 +554,30 @@ __global__ void k_missing_init(MeshDev m, const int *__restrict__ vor, int *mlis
         // a deleted vertex keeps an anchor triangle near its old position in vt[]
         int t = m.vt[v];
         if (t < 0 || m.tv[t].x == GA_DEAD) {
+            // the nearest labelled pixel (square rings around v's pixel, radius <= 16) whose
+            // site is in the mesh: its triangles are next to v, so the walk stays short
             const SetDesc &sd = m.sets[m.vset[v]];
+            int T = 1 << sd.logT;
             int pp = m.pix[v];
-            int s = vor[sd.toff + ((size_t)(pp >> 16) << sd.logT) + (pp & 0xffff)];
+            int px = pp >> 16, py = pp & 0xffff;
             t = -1;
-            if (s >= 0 && m.vstate[s] == VS_IN_MESH) t = m.vt[s];
+            for (int r = 0; r <= 16 && t < 0; r++) {
+                for (int q = 0; q < (r == 0 ? 1 : 8 * r) && t < 0; q++) {
+                    int dx, dy;
+                    if (r == 0) { dx = 0; dy = 0; }
+                    else {
+                        int side = q / (2 * r), o = q % (2 * r) - r;
+                        if (side == 0) { dx = o; dy = -r; }
+                        else if (side == 1) { dx = r; dy = o; }
+                        else if (side == 2) { dx = -o; dy = r; }
+                        else { dx = -r; dy = -o; }
+                    }
+                    int x = px + dx, y = py + dy;
+                    if (x < 0 || y < 0 || x >= T || y >= T) continue;
+                    int s = vor[sd.toff + ((size_t)x << sd.logT) + y];
+                    if (s >= 0 && m.vstate[s] == VS_IN_MESH) t = m.vt[s];
+                }
+            }
         }
         if (t < 0) t = set_anchor(m, v);
         loc[k] = t;
@@ -806,8 +827,15 @@ __global__ void k_flip_commit(MeshDev m, int op, int *list_out, int cnt_out_slot
         int e = ic(nt, i);
         int u = enc_t(e), j = enc_j(e);
         win = win && m.claim[u] == key;
-        int q1 = wave_reserve(&m.counters[cnt_out_slot], win);
-        int q2 = wave_reserve(&m.counters[cnt_out_slot], win);
+        // Next pass's list: both triangles of a flip, and the first triangle of a losing
+        // candidate (its edge may stay non-Delaunay when the candidate that beat it lost too);
+        // lmark keeps every slot at most once per list, so no candidate key repeats.
+        bool lt_ = atomicExch(&m.lmark[t], op) != op;
+        bool lu_ = win && atomicExch(&m.lmark[u], op) != op;
+        int q1 = wave_reserve(&m.counters[cnt_out_slot], lt_);
+        int q2 = wave_reserve(&m.counters[cnt_out_slot], lu_);
+        if (lt_) list_out[q1] = t;
+        if (lu_) list_out[q2] = u;
         if (!win) continue;
         int4 ut = m.tv[u], un = m.tn[u];
         int cc = ic(tt, i), x = ic(tt, p1(i)), y = ic(tt, m1(i));
@@ -823,11 +851,11 @@ __global__ void k_flip_commit(MeshDev m, int op, int *list_out, int cnt_out_slot
         sc(ru, j, -1); sc(ru, p1(j), enc(t, 0)); sc(ru, m1(j), enc(u, 2));
         m.remap[t] = rt; m.remap[u] = ru;
         m.stamp[t] = op; m.stamp[u] = op;
-        list_out[q1] = t; list_out[q2] = u;
         m.vt[x] = t; m.vt[cc] = t; m.vt[y] = u; m.vt[d] = u;
         flips++;
     }
     wave_count_full(&m.counters[C_FLIPS], flips);
+    wave_count_full(&m.counters[C_FLIPS_TOTAL], flips);
 }
 
 void launch_flip_pass(MeshDev &m, const int *list_in, int cnt_in_slot, int *list_out, int cnt_out_slot,
@@ -955,7 +983,6 @@ void launch_frame_round(MeshDev &m, unsigned int *mxraw, int round, int *list_ou
                         hipStream_t st)
 {
     unsigned long long *mx = (unsigned long long *)mxraw;
-    hipMemsetAsync(mx, 0, sizeof(unsigned long long) * m.V, st);
     GA_LAUNCH(k_frame_prio, dim3(FIXED_GRID), dim3(256), 0, st, m, mx, round);
     GA_LAUNCH(k_frame_remove, dim3(FIXED_GRID), dim3(256), 0, st, m, (const unsigned long long *)mx, round, op,
                        list_out, cnt_out_slot);
@@ -1051,6 +1078,22 @@ __global__ __launch_bounds__(256) void k_convexify(MeshDev m, const int *hnext, 
         __syncthreads();
         if (!s_flag) break;
     }
+}
+
+// zero up to four counter slots in one launch (negative slot: unused)
+__global__ void k_reset_counters(int *c, int a, int b, int d, int e)
+{
+    if (threadIdx.x == 0) {
+        if (a >= 0) c[a] = 0;
+        if (b >= 0) c[b] = 0;
+        if (d >= 0) c[d] = 0;
+        if (e >= 0) c[e] = 0;
+    }
+}
+
+void launch_reset_counters(int *counters, int a, int b, int d, int e, hipStream_t st)
+{
+    GA_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, st, counters, a, b, d, e);
 }
 
 __global__ void k_hull_edges(MeshDev m, int *hnext, int *hedge, int *hull_cnt, int *hull_start)

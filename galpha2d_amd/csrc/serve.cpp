@@ -65,14 +65,16 @@ struct galpha_server {
     long long passes = 0, served = 0;
 };
 
-// Device bytes of a context that runs n points on a T x T texture, DT + alpha: textures
-// (site map, column field, stacks, envelopes, labels: 18 B per pixel), mesh (about 400 B per
-// vertex: slot arrays of 3V triangles, fans, scratch) and alpha (about 900 B per point), with
-// 25% headroom for the buffer growth policy.
+// Device bytes a context grows to for `npoints` points on textures of `pixels` pixels, DT + alpha,
+// from the buffers each stage ensures (all with the 1/8 growth headroom): textures 20 B per pixel
+// (site map, column field, stacks, envelopes, labels); mesh about 610 B per vertex (3V triangle
+// slots with links, remaps, votes, lists and fans; vertex arrays; canonical keys and output);
+// alpha about 1400 B per point (5n spectrum entries with keys, intervals and scans; 13n master
+// entries; sort temporaries); plus a fixed 16 MiB.
 static size_t group_footprint(long long npoints, long long pixels)
 {
-    double b = 18.0 * (double)pixels + 1300.0 * (double)npoints + (1 << 20);
-    return (size_t)(b * 1.25);
+    double b = 20.5 * (double)pixels + 2100.0 * (double)npoints + (16 << 20);
+    return (size_t)b;
 }
 static size_t footprint(long long n, long long T) { return group_footprint(n, T * T); }
 

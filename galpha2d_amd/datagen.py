@@ -31,18 +31,24 @@ def lattice_clusters(n, side=1414, nclusters=64, sigma=0.005, seed=3):
     n_cl = n - n_lat
     sel = rng.choice(side * side, n_lat, replace=False)
     lat = np.stack([(sel // side) / side, (sel % side) / side], axis=1)
+    return np.concatenate([lat, gaussian_clusters(rng, n_cl, nclusters, sigma)])
+
+
+def gaussian_clusters(rng, n, nclusters, sigma):
+    """n points from nclusters Gaussian clusters (centres uniform), resampled into [0,1)^2."""
     cent = rng.random((nclusters, 2))
-    pts = cent[rng.integers(0, nclusters, n_cl)] + rng.normal(size=(n_cl, 2)) * sigma
+    pts = cent[rng.integers(0, nclusters, n)] + rng.normal(size=(n, 2)) * sigma
     bad = ((pts < 0) | (pts >= 1)).any(axis=1)
     while bad.any():
         k = int(bad.sum())
         pts[bad] = cent[rng.integers(0, nclusters, k)] + rng.normal(size=(k, 2)) * sigma
         bad = ((pts < 0) | (pts >= 1)).any(axis=1)
-    return np.concatenate([lat, pts])
+    return pts
 
 
 def batch_sets(nsets=4096, lo=256, hi=65536, seed=4):
-    """C4: sizes log-uniform in [lo, hi], each uniform or 8-cluster Gaussian (sigma 0.01), 50/50."""
+    """Batched workload: sizes log-uniform in [lo, hi], each set uniform or an 8-cluster Gaussian
+    mix (sigma 0.01, points outside [0,1)^2 resampled), 50/50."""
     rng = np.random.default_rng(seed)
     sizes = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), nsets))).astype(np.int64)
     sets = []
@@ -50,8 +56,5 @@ def batch_sets(nsets=4096, lo=256, hi=65536, seed=4):
         if rng.random() < 0.5:
             sets.append(uniform(seed * 1000003 + i, int(n)))
         else:
-            cent = rng.random((8, 2))
-            p = cent[rng.integers(0, 8, n)] + rng.normal(size=(n, 2)) * 0.01
-            p = np.clip(p, 0.0, np.nextafter(1.0, 0.0))
-            sets.append(p)
+            sets.append(gaussian_clusters(rng, int(n), 8, 0.01))
     return sets

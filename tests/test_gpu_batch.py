@@ -87,7 +87,7 @@ def test_batch_equals_plain_loop():
 
 
 def test_deletion_hole_through_diagonal(oracle):
-    """A clustered set (set 171 of the batched workload generator, seed 4) whose snapped
+    """A clustered set (8 Gaussian clusters, sigma 0.01, clipped to the unit square) whose snapped
     positions put a deleted vertex exactly on an ear diagonal of its hole; the hole must still be
     triangulated counter-clockwise (plain and batched paths)."""
     import os
