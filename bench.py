@@ -27,6 +27,37 @@ SEED = 2
 HBM_PEAK_GBS = 8000.0
 
 
+# Algorithmic (compulsory) bytes of one run of a kernel, from the work counters of that run.
+ALGO_BYTES = {
+    "k_frame_remove": {
+        "model": "12 B per frame vertex examined (state 4 + priority 8) + 88 B per fan triangle of a "
+                 "removed frame vertex (triangle record 32 read, chain position 16 read, fill or dead "
+                 "slot 32 written, anchor and back-link 8 written)",
+        "bytes": lambda st, T: 12.0 * st["frameSeen"] + 88.0 * st["frameChain"]},
+    "k_flip_test": {
+        "model": "152 B per listed triangle (vertices + links 32, three neighbour stamps and far "
+                 "vertices 24, six positions 96)",
+        "bytes": lambda st, T: 152.0 * st["flipListed"]},
+    "k_pba_color": {
+        "model": "8 B per pixel (envelope read 4 + label written 4)",
+        "bytes": lambda st, T: 8.0 * T * T * st["pbaIterations"]},
+}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary (FETCH_SIZE / WRITE_SIZE
+    passes scaled by a 256 MiB copy), or None when it has no row for the kernel."""
+    path = os.path.join(ROOT, "profiles", "c2_hbm_traffic.txt")
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 5 and cells[0] == f"`{kernel}`":
+                return round(float(cells[4]) * 1e6)
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(n, seed):
     """The oracle (sequential C: Shewchuk predicates, Bowyer-Watson DT, exact alpha ranks) on the
     full workload, one core, one run (about 13 s)."""
@@ -169,7 +200,8 @@ def main():
     ap.add_argument("--n", type=int, default=N_POINTS)
     ap.add_argument("--texture", type=int, default=TEXTURE)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--probe", default="k_pba_color", help="kernel timed for the roofline figure")
+    ap.add_argument("--probe", default="k_frame_remove", choices=sorted(ALGO_BYTES),
+                    help="kernel timed for the roofline figure")
     ap.add_argument("--batched-sets", type=int, default=4096, help="sets of the batched workload (0: skip)")
     ap.add_argument("--plain-sample", type=int, default=256, help="sets timed for the plain per-set loop")
     ap.add_argument("--serve-qps", default="50,150", help="offered request rates of the serving runs ('' skips)")
@@ -223,21 +255,22 @@ def main():
     value = n * args.steps / elapsed / 1e6
     st = ctx.stats()
 
-    # roofline of the dominant kernel: algorithmic bytes per launch / mean launch time
+    # roofline of the dominant kernel (largest share of GPU time in the rocprof summary of this
+    # workload): algorithmic bytes of the probed run / total time of its launches in that run
     probe = ctx.kernel_probe(args.probe, lambda: step())
+    pst = ctx.stats()
     roof = None
     if probe and probe["launches"] > 0:
-        P = T * T
-        # PBA phase 3 (colouring): reads the compacted envelope once per row and writes the
-        # 4-byte label of every pixel; algorithmic traffic = 8 B per pixel (one 4-byte texel read
-        # and one 4-byte texel write: one of the 3 passes of the 24 B/pixel PBA figure).
-        per_launch_bytes = 8.0 * P
+        model = ALGO_BYTES[args.probe]
+        total_bytes = model["bytes"](pst, T)
+        per_launch = total_bytes / probe["launches"]
         avg_ms = probe["total_ms"] / probe["launches"]
-        achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": args.probe, "avg_launch_ms": round(avg_ms, 4),
-                "algorithmic_bytes_per_launch": per_launch_bytes}
+        achieved = per_launch / (avg_ms * 1e-3) / 1e9
+        traffic = measured_traffic(args.probe)
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": args.probe, "launches": probe["launches"], "avg_launch_ms": round(avg_ms, 4),
+                "algorithmic_bytes_per_launch": round(per_launch), "algorithmic_model": model["model"]}
 
     out = {
         "metric": "Mpoints/s DT+alpha at 1M pts; batched sets/s and p50/p99 latency vs plain",

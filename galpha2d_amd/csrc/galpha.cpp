@@ -613,6 +613,9 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     int cnts[C_NCOUNTERS];
     ctx_counters_read(c, cnts, 0, C_NCOUNTERS);
     S.flips = cnts[C_FLIPS_TOTAL];
+    S.frameSeen = cnts[C_FRAME_SEEN];
+    S.frameChain = cnts[C_FRAME_CHAIN];
+    S.flipListed = cnts[C_FLIP_LISTED];
     S.incircleTests = cnts[C_TESTS_INCIRCLE];
     S.incircleExact = cnts[C_EXACT_INCIRCLE];
     S.orientExact = cnts[C_EXACT_ORIENT];

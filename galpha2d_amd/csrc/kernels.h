@@ -130,6 +130,9 @@ struct MeshDev {
 #define C_ISLAND_SITES 19
 #define C_HSTART 20
 #define C_FLIPS_TOTAL 21
+#define C_FRAME_SEEN 22       // frame vertices examined by k_frame_remove (all launches)
+#define C_FRAME_CHAIN 23      // sum of the fan lengths of removed frame vertices
+#define C_FLIP_LISTED 24      // triangles listed for k_flip_test (all launches)
 #define C_DBG 48          // 48..63: first failure dump
 #define C_NCOUNTERS 64
 

@@ -782,6 +782,7 @@ void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st)
 __global__ void k_flip_test(MeshDev m, const int *list, int cnt_slot, int prev_op, int op)
 {
     int n = m.counters[cnt_slot];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&m.counters[C_FLIP_LISTED], n);
     int tests = 0;
     GRID_STRIDE(q, n) {
         int t = list[q];
@@ -906,9 +907,13 @@ __global__ void k_frame_remove(MeshDev m, const unsigned long long *mx, int roun
                                int cnt_out_slot)
 {
     int nf = m.V - m.n;
-    GRID_STRIDE(k, nf) {
+    int seen = 0, chainsum = 0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < ((nf + 63) & ~63); k += gridDim.x * blockDim.x) {
+        // (full-wave loop so the work counters can be summed per wave at the end)
+        if (k >= nf) continue;
         int f = m.n + k;
         if (m.vstate[f] != VS_IN_MESH) continue;
+        seen++;
         unsigned long long p = prio64(f, round);
         if (mx[f] != p) continue;
         int i0, t0 = fan_first(m, f, &i0);
@@ -976,7 +981,10 @@ __global__ void k_frame_remove(MeshDev m, const unsigned long long *mx, int roun
         m.vstate[f] = VS_REMOVED;
         m.vt[f] = -1;
         atomicAdd(&m.counters[C_FRAME], -1);
+        chainsum += kf;
     }
+    wave_count_full(&m.counters[C_FRAME_SEEN], seen);
+    wave_count_full(&m.counters[C_FRAME_CHAIN], chainsum);
 }
 
 void launch_frame_round(MeshDev &m, unsigned int *mxraw, int round, int *list_out, int cnt_out_slot, int op,

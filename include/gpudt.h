@@ -106,6 +106,9 @@ typedef struct {
     long long incircleTests, incircleExact, orientExact;
     long long alphaExactCompares, alphaExactAttach, alphaClusters, alphaMaxCluster;
     double ms_pba, ms_recon, ms_shift, ms_missing, ms_boundary, ms_flip, ms_canon, ms_alpha, ms_total;
+    /* work counts of the last run: frame vertices examined by the frame-removal kernel, sum of the
+       fan lengths of removed frame vertices, triangles listed for the flip tests */
+    long long frameSeen, frameChain, flipListed;
 } galpha_stats;
 
 galpha_ctx *galphaCreate(int device);

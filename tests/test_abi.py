@@ -116,3 +116,32 @@ def test_no_process_global_work_state():
             continue
         bad.append(name)
     assert not bad, f"process-global writable state: {bad}"
+
+
+def test_python_mirrors_match_c_layouts():
+    """The ctypes mirrors of galpha_stats and GPUDTALPHA have the C sizes and field offsets."""
+    import ctypes as C
+    import subprocess
+    import tempfile
+    import galpha2d_amd as G
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "gpudt.h"
+int main(void) {
+    printf("%zu %zu %zu %zu\n", sizeof(galpha_stats), offsetof(galpha_stats, flips),
+           offsetof(galpha_stats, ms_pba), offsetof(galpha_stats, frameSeen));
+    printf("%zu %zu %zu\n", sizeof(GPUDTALPHA), offsetof(GPUDTALPHA, spectrum),
+           offsetof(GPUDTALPHA, spec_pointers));
+    return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "p.c")
+        exe = os.path.join(d, "p")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+    S, A = G._Stats, G._Alpha
+    assert out[0] == f"{C.sizeof(S)} {S.flips.offset} {S.ms_pba.offset} {S.frameSeen.offset}"
+    assert out[1] == f"{C.sizeof(A)} {A.spectrum.offset} {A.spec_pointers.offset}"
