@@ -71,7 +71,7 @@ struct galpha_ctx {
         moved, decide, mx, mlist, loc, lt, scratch, hnext, hedge, keys, keys2, vals, vals2, rot, newid,
         out9, counters, status, xy, scanTmp, sortTmp, tileCnt, tileOff, dellist, dbg;
     // batch plan on the device
-    DevBuf setsD, bmD, vset, hullCnt, hullOff, hullStart, setFirst, offsD, bboxTmp, lmark;
+    DevBuf setsD, bmD, vset, hullCnt, hullOff, hullStart, setFirst, offsD, bboxTmp, lmark, ffan, modF;
     std::vector<int> set_first;       // [nsets + 1] first output triangle of each planned set
     std::vector<SetDesc> plan_sets;   // host copy of the last plan
     int plan_ntot = 0;
@@ -391,6 +391,8 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     c->hedge.ensure(4 * (size_t)V);
     c->dellist.ensure(4 * (size_t)cap);
     c->lmark.ensure(4 * (size_t)cap);
+    c->ffan.ensure(8 * (size_t)(V - n) + 16);
+    c->modF.ensure(4 * (size_t)cap);
     HIPCHK(hipMemsetAsync(c->stamp.p, 0, 4 * (size_t)cap, st));
     HIPCHK(hipMemsetAsync(c->claim.p, 0, 8 * (size_t)cap, st));
     HIPCHK(hipMemsetAsync(c->lmark.p, 0, 4 * (size_t)cap, st));
@@ -424,6 +426,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     m.hnext = c->hnext.as<int>();
     m.hedge = c->hedge.as<int>();
     m.lmark = c->lmark.as<int>();
+    m.ffan = c->ffan.as<int>();
 
     // ---- reconstruction + adjacency
     launch_recon(c->vor.as<int>(), c->pix.as<int>(), bd, m, c->tileCnt.as<int>(), c->tileOff.as<int>(),
@@ -549,20 +552,24 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
 
     // ---- frame removal + hull convexification
     ctx_counter_set(c, C_FRAME, V - n);
+    // Triangles created by frame removal and convexification go to one list (each slot once);
+    // the mesh was Delaunay before, so only their edges can need flips afterwards.
+    m.list_op = c->op++;
+    launch_reset_counters(c->counters.as<int>(), C_FRAME_LIST, -1, -1, -1, st);
     for (int r = 0;; r += 2) {
         if (r > 100000) throw GalphaError("frame removal did not finish");
         for (int k = 0; k < 2; k++) {
-            launch_reset_counters(c->counters.as<int>(), C_SCR, C_MODN, -1, -1, st);
-            launch_frame_round(m, c->mx.as<unsigned int>(), next_prio_round(), m.modA, C_MODN, c->op++, st);
+            launch_reset_counters(c->counters.as<int>(), C_SCR, -1, -1, -1, st);
+            launch_frame_round(m, c->mx.as<unsigned int>(), next_prio_round(), c->modF.as<int>(), C_FRAME_LIST,
+                               c->op++, st);
         }
         int left = ctx_counter(c, C_FRAME);
         if (left <= 0) break;
     }
-    ctx_counter_set(c, C_MODN, 0);
-    launch_convexify(m, m.modA, C_MODN, c->op++, c->hullCnt.as<int>(), c->hullOff.as<int>(),
+    launch_convexify(m, c->modF.as<int>(), C_FRAME_LIST, c->op++, c->hullCnt.as<int>(), c->hullOff.as<int>(),
                      c->hullStart.as<int>(), c->scanTmp.p, st);
     HIPCHK(hipEventRecord(c->ev[6], st));
-    flip_all();
+    flip_until_done(c->modF.as<int>(), C_FRAME_LIST, m.modB, C_MODN2, m.list_op);
     HIPCHK(hipEventRecord(c->ev[7], st));
     {
         int stt = 0;
@@ -757,7 +764,7 @@ static void release_all(galpha_ctx *c)
                       &c->vals, &c->vals2, &c->rot, &c->newid, &c->out9, &c->counters, &c->status, &c->xy,
                       &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
-                      &c->offsD, &c->bboxTmp, &c->lmark};
+                      &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF};
     for (DevBuf *b : bufs) b->release();
     alpha_release(c->alpha);
 }
@@ -774,7 +781,7 @@ size_t galpha_ctx_device_bytes(galpha_ctx *c)
                       &c->vals, &c->vals2, &c->rot, &c->newid, &c->out9, &c->counters, &c->status, &c->xy,
                       &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
-                      &c->offsD, &c->bboxTmp, &c->lmark};
+                      &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF};
     size_t t = 0;
     for (DevBuf *b : bufs) t += b->bytes;
     for (auto &b : c->alpha.b) t += b.bytes;

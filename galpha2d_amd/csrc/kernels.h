@@ -98,6 +98,8 @@ struct MeshDev {
     int *lt;             // [n] located type of every missing site
     int *hnext, *hedge;  // [V] hull successor and hull edge of every hull vertex
     int *lmark;          // [cap] operation that last put the slot in a flip list (one entry per pass)
+    int *ffan;           // [2 * (V - n)] per frame vertex: first fan triangle (enc) and fan length
+    int list_op;         // lmark tag of the list that frame removal and convexification fill
 };
 
 // vertex states
@@ -133,6 +135,7 @@ struct MeshDev {
 #define C_FRAME_SEEN 22       // frame vertices examined by k_frame_remove (all launches)
 #define C_FRAME_CHAIN 23      // sum of the fan lengths of removed frame vertices
 #define C_FLIP_LISTED 24      // triangles listed for k_flip_test (all launches)
+#define C_FRAME_LIST 25       // triangles created by frame removal and convexification
 #define C_DBG 48          // 48..63: first failure dump
 #define C_NCOUNTERS 64
 

@@ -762,16 +762,15 @@ void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, hipStre
 }
 
 // ---- flipping ---------------------------------------------------------------------------------
+// every slot in use (dead ones are skipped by the test); no atomics: the list is the identity
 __global__ void k_flip_all(MeshDev m, int op, int *list, int cnt_slot)
 {
     int n = *m.ntri;
     GRID_STRIDE(t, n) {
-        bool alive = m.tv[t].x != GA_DEAD;
-        int q = wave_reserve(&m.counters[cnt_slot], alive);
-        if (!alive) continue;
         m.stamp[t] = op;
-        list[q] = t;
+        list[t] = t;
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) m.counters[cnt_slot] = n;
 }
 
 void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st)
@@ -883,6 +882,8 @@ GA_INLINE int fan_first(const MeshDev &m, int f, int *slot)
     return t;
 }
 
+// Every remaining frame vertex marks its closed neighbourhood with its priority and records its
+// first fan triangle and fan length in ffan[] for k_frame_remove of the same round.
 __global__ void k_frame_prio(MeshDev m, unsigned long long *mx, int round)
 {
     int nf = m.V - m.n;
@@ -892,14 +893,45 @@ __global__ void k_frame_prio(MeshDev m, unsigned long long *mx, int round)
         unsigned long long p = prio64(f, round);
         atomicMax(&mx[f], p);
         int i, t = fan_first(m, f, &i);
+        m.ffan[2 * k] = enc(t, i);
         atomicMax(&mx[ic(m.tv[t], p1(i))], p);
+        int kf = 0;
         for (int guard = 0; guard < 1000000; guard++) {
+            kf++;
             atomicMax(&mx[ic(m.tv[t], m1(i))], p);
             int e = ic(m.tn[t], p1(i));
             if (e < 0) break;
             t = enc_t(e);
             i = slot_of(m.tv[t], f);
         }
+        m.ffan[2 * k + 1] = kf;
+    }
+}
+
+// Read-only: a frame vertex wins when every vertex of its closed neighbourhood holds its
+// priority; ffan[2k + 1] keeps the fan length of a winner and -1 otherwise.
+__global__ void k_frame_decide(MeshDev m, const unsigned long long *mx, int round)
+{
+    int nf = m.V - m.n;
+    GRID_STRIDE(k, nf) {
+        int f = m.n + k;
+        if (m.vstate[f] != VS_IN_MESH) continue;
+        unsigned long long p = prio64(f, round);
+        int kf = m.ffan[2 * k + 1];
+        bool win = mx[f] == p;
+        int t = enc_t(m.ffan[2 * k]), i = enc_j(m.ffan[2 * k]);
+        if (win) win = mx[ic(m.tv[t], p1(i))] == p;
+        for (int q = 0; q < kf && win; q++) {
+            int4 tt = m.tv[t];
+            if (mx[ic(tt, m1(i))] != p) win = false;
+            else if (q + 1 < kf) {
+                int e = ic(m.tn[t], p1(i));
+                if (e < 0) { win = false; atomicOr(m.status, GA_ERR_WALK); break; }
+                t = enc_t(e);
+                i = slot_of(m.tv[t], f);
+            }
+        }
+        if (!win) m.ffan[2 * k + 1] = -1;
     }
 }
 
@@ -914,34 +946,29 @@ __global__ void k_frame_remove(MeshDev m, const unsigned long long *mx, int roun
         int f = m.n + k;
         if (m.vstate[f] != VS_IN_MESH) continue;
         seen++;
-        unsigned long long p = prio64(f, round);
-        if (mx[f] != p) continue;
-        int i0, t0 = fan_first(m, f, &i0);
-        int t = t0, i = i0, kf = 0;
-        bool win = mx[ic(m.tv[t], p1(i))] == p;
-        for (int guard = 0; win; guard++) {
-            if (guard > 1000000) { win = false; atomicOr(m.status, GA_ERR_WALK); break; }
-            kf++;
-            if (mx[ic(m.tv[t], m1(i))] != p) { win = false; break; }
-            int e = ic(m.tn[t], p1(i));
-            if (e < 0) break;
-            t = enc_t(e);
-            i = slot_of(m.tv[t], f);
-        }
-        if (!win) continue;
+        // winner decided by k_frame_decide this round; its fan is untouched by other winners
+        // (distance-2 independence), so the chain is read without races
+        int kf = m.ffan[2 * k + 1];
+        if (kf < 0) continue;
+        int t0 = enc_t(m.ffan[2 * k]), i0 = enc_j(m.ffan[2 * k]);
         // chain l_0..l_kf, outer links o_0..o_{kf-1}, fan slots
         int need = 4 * (kf + 1);
         int base = atomicAdd(&m.counters[C_SCR], need);
         if ((size_t)(base + need) > m.scratch_cap) { atomicOr(m.status, GA_ERR_SCRATCH); continue; }
         int *chain = m.scratch + base, *olink = chain + (kf + 1), *slots = olink + (kf + 1), *st = slots + (kf + 1);
-        t = t0; i = i0;
-        chain[0] = ic(m.tv[t], p1(i));
+        int t = t0, i = i0;
+        chain[0] = ic(m.tv[t0], p1(i0));
         for (int q = 0; q < kf; q++) {
-            chain[q + 1] = ic(m.tv[t], m1(i));
-            olink[q] = ic(m.tn[t], i);
+            int4 tt = m.tv[t], nn = m.tn[t];
+            chain[q + 1] = ic(tt, m1(i));
+            olink[q] = ic(nn, i);
             slots[q] = t;
-            int e = ic(m.tn[t], p1(i));
-            if (e >= 0) { t = enc_t(e); i = slot_of(m.tv[t], f); }
+            int e = ic(nn, p1(i));
+            if (q + 1 < kf) {
+                if (e < 0) { atomicOr(m.status, GA_ERR_WALK); break; }
+                t = enc_t(e);
+                i = slot_of(m.tv[t], f);
+            }
         }
         // chain vertices keep an incident triangle: the outer one
         for (int q = 0; q <= kf; q++) {
@@ -965,7 +992,8 @@ __global__ void k_frame_remove(MeshDev m, const unsigned long long *mx, int roun
                 bond(m, T, 0, cur);              // edge (B, c)
                 m.vt[A] = T; m.vt[B] = T; m.vt[c] = T;
                 m.stamp[T] = op;
-                list_out[atomicAdd(&m.counters[cnt_out_slot], 1)] = T;
+                if (atomicExch(&m.lmark[T], m.list_op) != m.list_op)
+                    list_out[atomicAdd(&m.counters[cnt_out_slot], 1)] = T;
                 cur = enc(T, 1);                 // edge (c, A) seen from T
                 top--;
             }
@@ -992,6 +1020,7 @@ void launch_frame_round(MeshDev &m, unsigned int *mxraw, int round, int *list_ou
 {
     unsigned long long *mx = (unsigned long long *)mxraw;
     GA_LAUNCH(k_frame_prio, dim3(FIXED_GRID), dim3(256), 0, st, m, mx, round);
+    GA_LAUNCH(k_frame_decide, dim3(FIXED_GRID), dim3(256), 0, st, m, (const unsigned long long *)mx, round);
     GA_LAUNCH(k_frame_remove, dim3(FIXED_GRID), dim3(256), 0, st, m, (const unsigned long long *)mx, round, op,
                        list_out, cnt_out_slot);
 }
@@ -1066,7 +1095,8 @@ __global__ __launch_bounds__(256) void k_convexify(MeshDev m, const int *hnext, 
                         bond(m, T, 0, owner[kc]);   // edge (vn, vc) = reverse of hull edge (vc -> vn)
                         bond(m, T, 1, owner[kp]);   // edge (vc, vp) = reverse of hull edge (vp -> vc)
                         m.stamp[T] = op;
-                        list_out[atomicAdd(&m.counters[cnt_out_slot], 1)] = T;
+                        if (atomicExch(&m.lmark[T], m.list_op) != m.list_op)
+                            list_out[atomicAdd(&m.counters[cnt_out_slot], 1)] = T;
                         owner[kp] = enc(T, 2);       // new hull edge (vp -> vn)
                         nxt[kp] = kn; prv[kn] = kp;
                         nxt[kc] = -1;
