@@ -97,15 +97,21 @@ def batched_vs_plain(G, torch, nsets, plain_sample, seed=4):
     stream = torch.cuda.current_stream().cuda_stream
     torch.cuda.synchronize()
     base = d_xy.data_ptr()
+    # texture autotuner on a sample (the first 256 sets), not timed; both modes use its choice
+    t0 = time.perf_counter()
+    sample = [d_xy[int(offs[k]):int(offs[k + 1])].cpu().numpy() for k in range(min(256, nsets))]
+    dens = ctx.autotune(sample, reps=1)
+    t_tune = time.perf_counter() - t0
+    del sample
     # plain loop (warm up on the first 8 sets)
     ns = min(plain_sample, nsets)
     for k in range(min(8, ns)):
-        ctx.compute_device(base + 16 * int(offs[k]), int(sizes[k]), G.auto_texture(int(sizes[k])),
+        ctx.compute_device(base + 16 * int(offs[k]), int(sizes[k]), ctx.texture_for(int(sizes[k])),
                            with_alpha=True, stream=stream)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(ns):
-        ctx.compute_device(base + 16 * int(offs[k]), int(sizes[k]), G.auto_texture(int(sizes[k])),
+        ctx.compute_device(base + 16 * int(offs[k]), int(sizes[k]), ctx.texture_for(int(sizes[k])),
                            with_alpha=True, stream=stream)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
@@ -129,6 +135,7 @@ def batched_vs_plain(G, torch, nsets, plain_sample, seed=4):
             "plain_sets_per_s": round(plain_rate, 1), "plain_sample_sets": ns,
             "speedup_vs_plain": round(batch_rate / plain_rate, 2),
             "batch_stage_ms": {k: round(v, 1) for k, v in st.items() if k.startswith("ms_")},
+            "autotuned_pixels_per_point": [float(x) for x in dens], "autotune_s": round(t_tune, 1),
             "workload": f"{nsets} sets, sizes log-uniform in [256, 65536], uniform or 8-cluster Gaussian "
                         f"(seed {seed}), autotuned textures, DT + alpha, device-resident"}
 

@@ -53,7 +53,7 @@ EXPORTS = ["gpudtComputeDT", "gpudtReleaseDTOutput", "gpudtGetLastError", "gpudt
            "gpudtReleaseAlpha", "galphaCreate", "galphaDestroy", "galphaLastError",
            "galphaComputeHost", "galphaComputeDevice", "galphaDeviceTriangles",
            "galphaCopyTriangles", "galphaGetStats", "galphaVoronoi", "galphaCopyAlpha",
-           "galphaBatch", "galphaBatchDevice", "galphaBatchTriangles", "galphaBatchCopyAlpha", "galphaFree", "galphaAutoTexture", "galphaKernelProbe",
+           "galphaTextureFor", "galphaAutotune", "galphaBatch", "galphaBatchDevice", "galphaBatchTriangles", "galphaBatchCopyAlpha", "galphaFree", "galphaAutoTexture", "galphaKernelProbe",
            "galphaKernelProbeResult", "galphaServerCreate", "galphaServerDestroy", "galphaServerSubmit",
            "galphaServerWait", "galphaServerTriangles", "galphaServerAlpha", "galphaServerRelease",
            "galphaServerStats"]
@@ -91,6 +91,10 @@ def lib():
     L.galphaKernelProbeResult.argtypes = [ip, C.POINTER(C.c_double)]
     L.galphaAutoTexture.argtypes = [C.c_int]
     L.galphaAutoTexture.restype = C.c_int
+    L.galphaTextureFor.argtypes = [vp, C.c_int]
+    L.galphaTextureFor.restype = C.c_int
+    L.galphaAutotune.argtypes = [vp, dp, ip, C.c_int, C.c_int, dp]
+    L.galphaAutotune.restype = C.c_int
     L.galphaBatch.argtypes = [vp, dp, ip, C.c_int, ip, C.c_int, C.POINTER(ip), C.POINTER(ip)]
     L.galphaBatch.restype = C.c_int
     L.galphaBatchDevice.argtypes = [vp, vp, ip, C.c_int, ip, C.c_int, vp]
@@ -217,6 +221,21 @@ class Context:
             self._L.galphaFree(C.cast(tp, C.c_void_p))
             self._L.galphaFree(C.cast(op, C.c_void_p))
         return [tris[toff[k]:toff[k + 1]] for k in range(ns)]
+
+    def texture_for(self, n):
+        return self._L.galphaTextureFor(self._h, int(n))
+
+    def autotune(self, sets, reps=2):
+        """Tunes the pixels-per-point density of each size class on a sample of point sets;
+        returns the 4 chosen densities."""
+        sets = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 2) for x in sets]
+        offs = np.zeros(len(sets) + 1, dtype=np.int32)
+        offs[1:] = np.cumsum([x.shape[0] for x in sets])
+        xy = np.ascontiguousarray(np.concatenate(sets))
+        out = np.zeros(4)
+        if self._L.galphaAutotune(self._h, _dp(xy), _ip(offs), len(sets), int(reps), _dp(out)) < 0:
+            self._err("galphaAutotune")
+        return out
 
     def batch_device(self, d_xy_ptr, offsets, fbo_sizes=None, with_alpha=False, stream=None):
         """Device-resident batch: d_xy_ptr is the device address of all sets' (x, y) doubles,

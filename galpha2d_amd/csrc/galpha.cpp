@@ -57,6 +57,7 @@ struct DevBuf {
 // Frame sites sit on the texture border every frame_step(T) pixels; real points are mapped at
 // least margin(T) pixels inside, with frame_step/2 < margin, so every border pixel is nearer
 // to a frame site than to any real site.
+#define GALPHA_TUNE_CLASSES 4
 static int frame_step(int T) { return T >= 2048 ? T / 256 : 8; }
 static int margin_for(int T) { return frame_step(T) / 2 + 1; }
 
@@ -79,6 +80,7 @@ struct galpha_ctx {
     std::vector<int> batch_npts;      // points of every input set of the last batch
     const double *batch_xy = nullptr; // device points of the planned sets of the last batch
     AlphaDev alpha;
+    double density[GALPHA_TUNE_CLASSES] = {16.0, 16.0, 16.0, 16.0};   // pixels per point by size class
     int last_n = 0, last_F = 0, last_T = 0;
     bool last_alpha = false;
     galpha_stats stats{};
@@ -167,12 +169,23 @@ static void check_texture(int T)
     if (T < 64 || T > 16384 || (T & (T - 1))) throw GalphaError("fboSize must be a power of two in [64, 16384]");
 }
 
-int galphaAutoTexture(int n)
+// smallest power-of-two side in [256, 16384] with at least `density` pixels per point
+static int texture_for_density(int n, double density)
 {
-    // about 16 pixels per point, at least 256
     int T = 256;
-    while (T < 16384 && (double)T * T < 16.0 * (double)n) T <<= 1;
+    while (T < 16384 && (double)T * T < density * (double)n) T <<= 1;
     return T;
+}
+
+int galphaAutoTexture(int n) { return texture_for_density(n, 16.0); }
+
+// size classes of the texture autotuner (upper bounds in points, exclusive)
+static const int kTuneClassMax[GALPHA_TUNE_CLASSES] = {4096, 32768, 262144, 0x7fffffff};
+static int tune_class(int n)
+{
+    int k = 0;
+    while (k + 1 < GALPHA_TUNE_CLASSES && n >= kTuneClassMax[k]) k++;
+    return k;
 }
 
 // ---- batch plan: per-set descriptors, block maps and frame pixels ----------------------------
@@ -689,7 +702,7 @@ static int batch_run(galpha_ctx *c, const double *xy, const int *offsets, int ns
         c->batch_npts[s] = n;
         if (n < 3) continue;
         if (!xy) throw GalphaError("null point array");
-        int T = (fboSizes && fboSizes[s] > 0) ? fboSizes[s] : galphaAutoTexture(n);
+        int T = (fboSizes && fboSizes[s] > 0) ? fboSizes[s] : texture_for_density(n, c->density[tune_class(n)]);
         c->batch_sets.push_back(s);
         ns.push_back(n);
         Ts.push_back(T);
@@ -1103,6 +1116,62 @@ int galphaBatch(galpha_ctx *c, const double *xy, const int *offsets, int nsets, 
         memcpy(*tris9_out, all.data(), sizeof(int) * all.size());
         memcpy(*tri_offsets_out, toff.data(), sizeof(int) * (nsets + 1));
         return nsets;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+// texture side this context picks for n points (autotuned density of n's size class)
+int galphaTextureFor(galpha_ctx *c, int n)
+{
+    if (!c) return galphaAutoTexture(n);
+    return texture_for_density(n, c->density[tune_class(n)]);
+}
+
+// Texture autotuner.  For every size class present in the sample (sets of xy / offsets), the
+// sample's sets of that class run as one batch (DT + alpha) at each candidate density; the fastest
+// density (median of `reps` timed runs after one warm-up) becomes the context's choice for that
+// class.  densities_out (optional) receives the 4 chosen densities.  Results never depend on the
+// texture size, only the time does.
+int galphaAutotune(galpha_ctx *c, const double *xy, const int *offsets, int nsets, int reps, double *densities_out)
+{
+    if (!c || !xy || !offsets || nsets <= 0) return -1;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        static const double cand[] = {6.0, 9.0, 12.0, 16.0, 24.0, 32.0};
+        if (reps < 1) reps = 1;
+        for (int k = 0; k < GALPHA_TUNE_CLASSES; k++) {
+            std::vector<double> pts;
+            std::vector<int> offs(1, 0);
+            for (int s = 0; s < nsets; s++) {
+                int n = offsets[s + 1] - offsets[s];
+                if (n < 3 || tune_class(n) != k) continue;
+                pts.insert(pts.end(), xy + 2 * (size_t)offsets[s], xy + 2 * (size_t)offsets[s + 1]);
+                offs.push_back(offs.back() + n);
+            }
+            int ns = (int)offs.size() - 1;
+            if (ns == 0) continue;
+            double best = -1, best_ms = 0;
+            for (double d : cand) {
+                std::vector<int> Ts(ns);
+                for (int q = 0; q < ns; q++) Ts[q] = texture_for_density(offs[q + 1] - offs[q], d);
+                std::vector<double> ms;
+                for (int r = 0; r <= reps; r++) {
+                    auto t0 = std::chrono::steady_clock::now();
+                    batch_run(c, pts.data(), offs.data(), ns, Ts.data(), 1);
+                    HIPCHK(hipStreamSynchronize(c->stream));
+                    auto t1 = std::chrono::steady_clock::now();
+                    if (r > 0) ms.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
+                }
+                std::sort(ms.begin(), ms.end());
+                double med = ms[ms.size() / 2];
+                if (best < 0 || med < best_ms) { best = d; best_ms = med; }
+            }
+            c->density[k] = best;
+        }
+        if (densities_out)
+            for (int k = 0; k < GALPHA_TUNE_CLASSES; k++) densities_out[k] = c->density[k];
+        return 0;
     } catch (const std::exception &e) {
         return fail(c, e);
     }

@@ -181,8 +181,17 @@ int galphaServerStats(galpha_server *server, long long *passes, long long *serve
 int galphaKernelProbe(const char *kernel_name);
 int galphaKernelProbeResult(int *launches, double *total_ms);
 
-/* texture side the autotuner picks for n points */
+/* default texture side for n points (at least 16 pixels per point, power of two >= 256) */
 int galphaAutoTexture(int n);
+/* texture side a context picks for n points: its autotuned pixels-per-point density for n's size
+ * class (< 4096, < 32768, < 262144, larger points), 16 until tuned; used for every set whose
+ * texture is not given */
+int galphaTextureFor(galpha_ctx *ctx, int n);
+/* Texture autotuner: times the sample sets of each size class as one batch at densities
+ * {6, 9, 12, 16, 24, 32} pixels per point (median of reps runs) and keeps the fastest per class.
+ * densities_out (optional): the 4 chosen densities.  Results do not depend on the texture. */
+int galphaAutotune(galpha_ctx *ctx, const double *xy, const int *offsets, int nsets, int reps,
+                   double *densities_out);
 
 #ifdef __cplusplus
 }

@@ -101,3 +101,20 @@ def test_deletion_hole_through_diagonal(oracle):
     tris = ctx.batch([xy, oracle.gen_uniform(3, 500)], with_alpha=False)
     assert np.array_equal(tris[0], ref)
     ctx.close()
+
+
+def test_autotune_picks_candidates_and_keeps_results(oracle):
+    import galpha2d_amd as G
+    print("case: texture autotuner on a small sample", flush=True)
+    sets = [oracle.gen_uniform(500 + k, n) for k, n in enumerate([700, 1500, 5000, 9000, 40000])]
+    ctx = G.Context(0)
+    dens = ctx.autotune(sets, reps=1)
+    assert all(d in (6.0, 9.0, 12.0, 16.0, 24.0, 32.0) for d in dens)
+    tris = ctx.batch(sets, with_alpha=False)
+    for xy, t in zip(sets, tris):
+        ref, _ = oracle.delaunay(xy)
+        assert np.array_equal(t, ref)
+    for n in (1000, 20000, 100000, 500000):
+        T = ctx.texture_for(n)
+        assert 256 <= T <= 16384 and (T & (T - 1)) == 0
+    ctx.close()
