@@ -33,7 +33,8 @@ class _Stats(C.Structure):
                 ("ms_pba", C.c_double), ("ms_recon", C.c_double), ("ms_shift", C.c_double),
                 ("ms_missing", C.c_double), ("ms_boundary", C.c_double), ("ms_flip", C.c_double),
                 ("ms_canon", C.c_double), ("ms_alpha", C.c_double), ("ms_total", C.c_double),
-                ("frameSeen", C.c_longlong), ("frameChain", C.c_longlong), ("flipListed", C.c_longlong)]
+                ("frameSeen", C.c_longlong), ("frameChain", C.c_longlong), ("flipListed", C.c_longlong),
+                ("frameBulk", C.c_int), ("earRounds", C.c_int)]
 
 
 class _Alpha(C.Structure):

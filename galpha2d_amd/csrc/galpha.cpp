@@ -72,7 +72,7 @@ struct galpha_ctx {
         moved, decide, mx, mlist, loc, lt, scratch, hnext, hedge, keys, keys2, vals, vals2, rot, newid,
         out9, counters, status, xy, scanTmp, sortTmp, tileCnt, tileOff, dellist, dbg;
     // batch plan on the device
-    DevBuf setsD, bmD, vset, hullCnt, hullOff, hullStart, setFirst, offsD, bboxTmp, lmark, ffan, modF;
+    DevBuf setsD, bmD, vset, hullCnt, hullOff, hullStart, setFirst, offsD, bboxTmp, lmark, ffan, modF, badset, bnd;
     std::vector<int> set_first;       // [nsets + 1] first output triangle of each planned set
     std::vector<SetDesc> plan_sets;   // host copy of the last plan
     int plan_ntot = 0;
@@ -351,6 +351,73 @@ static void bbox_host(const double *xy, int n, double *bb5)
 void bbox_device(const double *d_xy, int n, double *out5, void *tmp, hipStream_t st);
 void bbox_sets_device(const double *d_xy, const int *d_offs, int nsets, double *out5, void *tmp, hipStream_t st);
 
+// Bulk frame removal: when the real region of every set is bounded by one simple cycle through all
+// of its meshed points, drop all frame triangles and convexify each cycle by parallel ear rounds.
+// Returns false (mesh untouched) when some set does not qualify.
+static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t st, int *ear_rounds)
+{
+    int V = m.V, n = m.n;
+    int *outdeg = m.decide, *cover = m.deg, *bidx = c->fill.as<int>(), *blist = c->mlist.as<int>();
+    c->badset.ensure(4 * (size_t)nsets + 16);
+    int *badset = c->badset.as<int>(), *hcnt = c->hullCnt.as<int>(), *hstart = c->hullStart.as<int>();
+    HIPCHK(hipMemsetAsync(m.hnext, 0xff, 4 * (size_t)V, st));
+    HIPCHK(hipMemsetAsync(outdeg, 0, 4 * (size_t)V, st));
+    HIPCHK(hipMemsetAsync(cover, 0, 4 * (size_t)V, st));
+    HIPCHK(hipMemsetAsync(bidx, 0xff, 4 * (size_t)V, st));
+    HIPCHK(hipMemsetAsync(hcnt, 0, 4 * (size_t)nsets, st));
+    HIPCHK(hipMemsetAsync(badset, 0, 4 * (size_t)nsets, st));
+    launch_fill_i32(hstart, 0x7fffffff, (size_t)nsets, st);
+    launch_reset_counters(c->counters.as<int>(), C_BND, -1, -1, -1, st);
+    launch_bnd_detect(m, outdeg, cover, hcnt, hstart, badset, blist, bidx, st);
+    std::vector<int> bad(nsets), cnt(nsets);
+    int B = 0;
+    HIPCHK(hipMemcpyAsync(bad.data(), badset, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(cnt.data(), hcnt, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&B, c->counters.as<int>() + C_BND, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    long long tot = 0;
+    int maxh = 0;
+    std::vector<int> off(nsets);
+    for (int s = 0; s < nsets; s++) {
+        // a set needs a cycle of at least 3 vertices; very long cycles make the ear test's
+        // per-candidate scan too long, so those sets keep the round-by-round removal
+        if (bad[s] || cnt[s] < 3 || cnt[s] > 65536) return false;
+        off[s] = (int)tot;
+        tot += cnt[s];
+        maxh = std::max(maxh, cnt[s]);
+    }
+    if (tot != B) return false;
+    (void)n;
+    c->bnd.ensure(4 * 11 * (size_t)B + 64);
+    HIPCHK(hipMemcpyAsync(c->hullOff.p, off.data(), 4 * (size_t)nsets, hipMemcpyHostToDevice, st));
+    int rounds = 1;
+    while ((1 << rounds) < maxh) rounds++;
+    rounds += 1;
+    launch_bnd_rank(m, B, rounds, blist, bidx, hstart, hcnt, c->hullOff.as<int>(), badset, c->bnd.as<int>(), st);
+    int collinear = 0;
+    HIPCHK(hipMemcpyAsync(bad.data(), badset, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&collinear, c->counters.as<int>() + C_BND, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int s = 0; s < nsets; s++)
+        if (bad[s]) return false;
+    // many exactly collinear boundary triples (lattices): ears would mostly be blocked
+    if (collinear > std::max(8, B / 100)) return false;
+    // from here on the mesh changes
+    launch_kill_frame(m, st);
+    int round = 0;
+    for (;; round += 2) {
+        if (round > 200000) throw GalphaError("hull convexification did not finish");
+        launch_ear_round(m, B, c->bnd.as<int>(), hcnt, c->hullOff.as<int>(), round, c->modF.as<int>(),
+                         C_FRAME_LIST, c->op++, st);
+        launch_ear_round(m, B, c->bnd.as<int>(), hcnt, c->hullOff.as<int>(), round + 1, c->modF.as<int>(),
+                         C_FRAME_LIST, c->op++, st);
+        if (ctx_counter(c, C_EAR) == 0) break;
+    }
+    *ear_rounds = round + 2;
+    ctx_counter_set(c, C_FRAME, 0);
+    return true;
+}
+
 // Delaunay triangulation of every set of the plan (points in d_xy, concatenated in set order).
 // Leaves the canonical triangles of all sets in c->out9 (9 ints each, global vertex ids and
 // global triangle positions, set-major) and the first triangle of each set in c->set_first.
@@ -569,6 +636,9 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     // the mesh was Delaunay before, so only their edges can need flips afterwards.
     m.list_op = c->op++;
     launch_reset_counters(c->counters.as<int>(), C_FRAME_LIST, -1, -1, -1, st);
+    if (bulk_frame_removal(c, m, nsets, st, &S.earRounds)) {
+        S.frameBulk = 1;
+    } else {
     for (int r = 0;; r += 2) {
         if (r > 100000) throw GalphaError("frame removal did not finish");
         for (int k = 0; k < 2; k++) {
@@ -581,6 +651,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     }
     launch_convexify(m, c->modF.as<int>(), C_FRAME_LIST, c->op++, c->hullCnt.as<int>(), c->hullOff.as<int>(),
                      c->hullStart.as<int>(), c->scanTmp.p, st);
+    }
     HIPCHK(hipEventRecord(c->ev[6], st));
     flip_until_done(c->modF.as<int>(), C_FRAME_LIST, m.modB, C_MODN2, m.list_op);
     HIPCHK(hipEventRecord(c->ev[7], st));
@@ -777,7 +848,7 @@ static void release_all(galpha_ctx *c)
                       &c->vals, &c->vals2, &c->rot, &c->newid, &c->out9, &c->counters, &c->status, &c->xy,
                       &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
-                      &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF};
+                      &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd};
     for (DevBuf *b : bufs) b->release();
     alpha_release(c->alpha);
 }
@@ -794,7 +865,7 @@ size_t galpha_ctx_device_bytes(galpha_ctx *c)
                       &c->vals, &c->vals2, &c->rot, &c->newid, &c->out9, &c->counters, &c->status, &c->xy,
                       &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
-                      &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF};
+                      &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd};
     size_t t = 0;
     for (DevBuf *b : bufs) t += b->bytes;
     for (auto &b : c->alpha.b) t += b.bytes;

@@ -136,6 +136,8 @@ struct MeshDev {
 #define C_FRAME_CHAIN 23      // sum of the fan lengths of removed frame vertices
 #define C_FLIP_LISTED 24      // triangles listed for k_flip_test (all launches)
 #define C_FRAME_LIST 25       // triangles created by frame removal and convexification
+#define C_BND 26              // boundary vertices of the real region (bulk frame removal)
+#define C_EAR 27              // ear candidates of the current round
 #define C_DBG 48          // 48..63: first failure dump
 #define C_NCOUNTERS 64
 
@@ -169,6 +171,14 @@ void launch_flip_pass(MeshDev &m, const int *list_in, int cnt_in_slot, int *list
                       int prev_op, int op, hipStream_t st);
 void launch_frame_round(MeshDev &m, unsigned int *mx, int round, int *list_out, int cnt_out_slot, int op,
                         hipStream_t st);
+// bulk frame removal (see mesh.hip); bnd = 11 * B ints of work space once B is known
+void launch_bnd_detect(MeshDev &m, int *outdeg, int *cover, int *hull_cnt, int *hull_start, int *badset,
+                       int *blist, int *bidx, hipStream_t st);
+void launch_bnd_rank(MeshDev &m, int B, int rounds, const int *blist, const int *bidx, const int *hull_start,
+                     const int *hull_cnt, const int *hull_off, int *badset, int *bnd, hipStream_t st);
+void launch_kill_frame(MeshDev &m, hipStream_t st);
+void launch_ear_round(MeshDev &m, int B, int *bnd, const int *hull_cnt, const int *hull_off, int round,
+                      int *list_out, int cnt_out_slot, int op, hipStream_t st);
 void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, int *hull_cnt, int *hull_off,
                       int *hull_start, void *scan_tmp, hipStream_t st);
 // canonical order: one 64-bit pass when wide == 0 (single set, ids < 2^21), else two passes

@@ -23,6 +23,7 @@
 // kept and the neighbour's back-link is written.
 #include "dev_common.h"
 #include "kernels.h"
+#include <algorithm>
 
 #define GRID_STRIDE(i, n) for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 #define FIXED_GRID 2048
@@ -1163,6 +1164,283 @@ void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, int *
     launch_exclusive_scan(hull_cnt, hull_off, m.nsets, scan_tmp, st);
     GA_LAUNCH(k_convexify, dim3(m.nsets), dim3(256), 0, st, m, (const int *)m.hnext, (const int *)m.hedge,
               (const int *)hull_cnt, (const int *)hull_off, (const int *)hull_start, list_out, cnt_out_slot, op);
+}
+
+// ---- bulk frame removal ------------------------------------------------------------------------
+// Every triangle of the current mesh whose vertices are all real is a triangle of the final
+// triangulation (its circumcircle holds no point at all).  When the region R of those triangles
+// has, in every set, a boundary that is one simple cycle through every meshed real point, all
+// frame triangles are dropped at once and R is made convex by parallel ear rounds on that cycle.
+// The checks below run before anything is modified; if one fails, the caller removes the frame
+// vertices round by round instead.
+GA_INLINE bool is_frame_tri(const MeshDev &m, int4 v) { return v.x >= m.n || v.y >= m.n || v.z >= m.n; }
+
+// boundary edges of R (a -> b with R on the left), their count and smallest tail per set; a vertex
+// with two outgoing boundary edges (a pinch) or a meshed real point in no R triangle marks its set
+__global__ void k_bnd_edges(MeshDev m, int *outdeg, int *cover, int *hull_cnt, int *hull_start, int *badset)
+{
+    int n = *m.ntri;
+    GRID_STRIDE(t, n) {
+        int4 v = m.tv[t];
+        if (v.x == GA_DEAD || is_frame_tri(m, v)) continue;
+        cover[v.x] = 1; cover[v.y] = 1; cover[v.z] = 1;
+        int4 nb = m.tn[t];
+        for (int i = 0; i < 3; i++) {
+            int e = ic(nb, i);
+            bool bnd = e < 0 || is_frame_tri(m, m.tv[enc_t(e)]);
+            if (!bnd) continue;
+            int a = ic(v, p1(i)), b = ic(v, m1(i));
+            m.hnext[a] = b;
+            m.hedge[a] = enc(t, i);
+            int s = m.vset[a];
+            if (atomicAdd(&outdeg[a], 1) != 0) badset[s] = 1;
+            atomicMin(&hull_start[s], a);
+            atomicAdd(&hull_cnt[s], 1);
+        }
+    }
+}
+
+__global__ void k_bnd_cover(MeshDev m, const int *cover, int *badset, int *blist, int *bidx, int *bcount)
+{
+    GRID_STRIDE(v, m.n) {
+        if (m.vstate[v] == VS_IN_MESH && !cover[v]) badset[m.vset[v]] = 1;
+        bool b = m.hnext[v] >= 0;
+        int k = wave_reserve(bcount, b);
+        if (b) { blist[k] = v; bidx[v] = k; }
+    }
+}
+
+// list ranking along each cycle, cut before its start vertex: d = steps to the cut
+__global__ void k_bnd_rank_init(MeshDev m, int B, const int *blist, const int *bidx, const int *hull_start,
+                                int *succ, int *d, int *badset)
+{
+    GRID_STRIDE(k, B) {
+        int a = blist[k];
+        int nx = m.hnext[a];
+        bool last = nx == hull_start[m.vset[a]];
+        int si = last ? -1 : bidx[nx];
+        if (!last && (si < 0 || si >= B)) {   // successor without an outgoing boundary edge
+            badset[m.vset[a]] = 1;
+            si = -1;
+            last = true;
+        }
+        succ[k] = si;
+        d[k] = last ? 0 : 1;
+    }
+}
+
+__global__ void k_bnd_rank_step(int B, const int *succ, const int *d, int *succ2, int *d2)
+{
+    GRID_STRIDE(k, B) {
+        int s = succ[k];
+        if (s < 0) { succ2[k] = -1; d2[k] = d[k]; }
+        else { succ2[k] = succ[s]; d2[k] = d[k] + d[s]; }
+    }
+}
+
+// a cycle that does not return to its start within the rounds (a second cycle in the set), or two
+// vertices ranked to one position, marks the set; verts[] starts at -1, so a position of a marked
+// set may stay unwritten (verts -1, nxt/prv unset) and later kernels skip it
+__global__ void k_bnd_place(MeshDev m, int B, const int *blist, const int *succ, const int *d, const int *hull_cnt,
+                            const int *hull_off, int *badset, int *verts, int *owner, int *nxt, int *prv,
+                            int *alive, int *pset)
+{
+    GRID_STRIDE(k, B) {
+        int a = blist[k];
+        int s = m.vset[a];
+        int h = hull_cnt[s];
+        if (succ[k] >= 0 || d[k] >= h) { badset[s] = 1; continue; }
+        int pos = h - 1 - d[k];
+        int p = hull_off[s] + pos;
+        if (atomicExch(&verts[p], a) != -1) badset[s] = 1;
+        owner[p] = m.hedge[a];
+        nxt[p] = hull_off[s] + (pos + 1 == h ? 0 : pos + 1);
+        prv[p] = hull_off[s] + (pos == 0 ? h - 1 : pos - 1);
+        alive[p] = 1;
+        pset[p] = s;
+    }
+}
+
+// exactly collinear consecutive boundary triples (a degenerate boundary: most ears would be blocked);
+// positions left unwritten by k_bnd_place (marked sets) are skipped
+__global__ void k_bnd_collinear(MeshDev m, int B, const int *verts, const int *nxt, const int *prv, int *count)
+{
+    int c = 0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < ((B + 63) & ~63); p += gridDim.x * blockDim.x) {
+        if (p >= B) continue;
+        int vc = verts[p];
+        if (vc < 0) continue;
+        int vp = verts[prv[p]], vn = verts[nxt[p]];
+        if (vp < 0 || vn < 0) continue;
+        double2 P = m.pos[vp], C = m.pos[vc], N = m.pos[vn];
+        if (orient_dev(m, P, C, N) == 0) c++;
+    }
+    wave_count_full(count, c);
+}
+
+// drop every frame triangle; R's triangles across a dropped edge get a hull link; frame vertices
+// leave the mesh; boundary vertices take their outgoing boundary triangle as anchor
+__global__ void k_kill_frame(MeshDev m)
+{
+    int n = *m.ntri;
+    GRID_STRIDE(t, n) {
+        int4 v = m.tv[t];
+        if (v.x == GA_DEAD || !is_frame_tri(m, v)) continue;
+        int4 nb = m.tn[t];
+        for (int i = 0; i < 3; i++) {
+            int e = ic(nb, i);
+            if (e < 0) continue;
+            int u = enc_t(e);
+            if (!is_frame_tri(m, m.tv[u])) set_link(m, u, enc_j(e), -1);
+        }
+        m.tv[t] = make_int4(GA_DEAD, GA_DEAD, GA_DEAD, 0);
+    }
+    GRID_STRIDE(k, m.V - m.n) {
+        int f = m.n + k;
+        if (m.vstate[f] == VS_IN_MESH) m.vstate[f] = VS_REMOVED;
+        m.vt[f] = -1;
+    }
+    GRID_STRIDE(v, m.n) {
+        if (m.hnext[v] >= 0) m.vt[v] = enc_t(m.hedge[v]);
+    }
+}
+
+// Ear rounds.  A cycle position p (vertex C between P = prev and N = next) is a candidate when C
+// is reflex (P, C, N clockwise) and no other live vertex of its cycle lies in the closed triangle
+// (P, N, C); one wave tests emptiness.
+__global__ __launch_bounds__(256) void k_ear_cand(MeshDev m, int B, const int *verts, const int *nxt,
+                                                  const int *prv, const int *alive, const int *pset,
+                                                  const int *hull_cnt, const int *hull_off, int *cand,
+                                                  int *ncand)
+{
+    int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    int nw = (gridDim.x * blockDim.x) >> 6;
+    for (int p = wave; p < B; p += nw) {
+        int c = 0;
+        if (alive[p]) {
+            int kp = prv[p], kn = nxt[p];
+            if (kp != kn) {
+                double2 P = m.pos[verts[kp]], C = m.pos[verts[p]], N = m.pos[verts[kn]];
+                if (orient_dev(m, P, C, N) < 0) {
+                    int s = pset[p], off = hull_off[s], h = hull_cnt[s];
+                    double lx = fmin(P.x, fmin(C.x, N.x)), hx = fmax(P.x, fmax(C.x, N.x));
+                    double ly = fmin(P.y, fmin(C.y, N.y)), hy = fmax(P.y, fmax(C.y, N.y));
+                    bool blocked = false;
+                    for (int q = off + lane; q < off + h; q += 64) {
+                        if (q == p || q == kp || q == kn || !alive[q]) continue;
+                        double2 Q = m.pos[verts[q]];
+                        if (Q.x < lx || Q.x > hx || Q.y < ly || Q.y > hy) continue;
+                        if (orient_dev(m, P, N, Q) >= 0 && orient_dev(m, N, C, Q) >= 0 && orient_dev(m, C, P, Q) >= 0)
+                            blocked = true;
+                    }
+                    c = __ballot(blocked) ? 0 : 1;
+                }
+            }
+        }
+        if (lane == 0) {
+            cand[p] = c;
+            if (c) atomicAdd(ncand, 1);
+        }
+    }
+}
+
+GA_INLINE uint32_t ear_prio(int p, int round) { return hash32((uint32_t)p * 0x9E3779B1u ^ (uint32_t)round); }
+GA_INLINE bool ear_beats(int p, int q, int round)
+{
+    uint32_t a = ear_prio(p, round), b = ear_prio(q, round);
+    return a > b || (a == b && p > q);
+}
+
+// winners: candidates above every candidate within cycle distance 2 (so two winners share no vertex)
+__global__ void k_ear_select(int B, const int *nxt, const int *prv, const int *cand, int round, int *win)
+{
+    GRID_STRIDE(p, B) {
+        int w = 0;
+        if (cand[p]) {
+            int a = prv[p], b = nxt[p];
+            int q[4] = {a, b, prv[a], nxt[b]};
+            w = 1;
+            for (int k = 0; k < 4; k++)
+                if (q[k] != p && cand[q[k]] && !ear_beats(p, q[k], round)) w = 0;
+        }
+        win[p] = w;
+    }
+}
+
+__global__ void k_ear_apply(MeshDev m, int B, const int *verts, int *nxt, int *prv, int *alive, int *owner,
+                            const int *win, int *list_out, int cnt_out_slot, int op)
+{
+    GRID_STRIDE(p, B) {
+        if (!win[p]) continue;
+        int kp = prv[p], kn = nxt[p];
+        int T = atomicAdd(m.ntri, 1);
+        if (T >= m.cap) { atomicOr(m.status, GA_ERR_CAPACITY); continue; }
+        int vp = verts[kp], vc = verts[p], vn = verts[kn];
+        m.tv[T] = make_int4(vp, vn, vc, 0);
+        m.tn[T] = make_int4(-1, -1, -1, 0);
+        bond(m, T, 0, owner[p]);    // edge (vn, vc) = reverse of boundary edge (vc -> vn)
+        bond(m, T, 1, owner[kp]);   // edge (vc, vp) = reverse of boundary edge (vp -> vc)
+        m.stamp[T] = op;
+        if (atomicExch(&m.lmark[T], m.list_op) != m.list_op)
+            list_out[atomicAdd(&m.counters[cnt_out_slot], 1)] = T;
+        owner[kp] = enc(T, 2);      // new boundary edge (vp -> vn)
+        nxt[kp] = kn;
+        prv[kn] = kp;
+        alive[p] = 0;
+        m.vt[vp] = T; m.vt[vc] = T; m.vt[vn] = T;
+    }
+}
+
+// bnd layout: succ[2][B], d[2][B], verts, owner, nxt, prv, alive, pset, cand, win
+void launch_bnd_detect(MeshDev &m, int *outdeg, int *cover, int *hull_cnt, int *hull_start, int *badset,
+                       int *blist, int *bidx, hipStream_t st)
+{
+    GA_LAUNCH(k_bnd_edges, dim3(FIXED_GRID), dim3(256), 0, st, m, outdeg, cover, hull_cnt, hull_start, badset);
+    GA_LAUNCH(k_bnd_cover, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)cover, badset, blist, bidx,
+              m.counters + C_BND);
+}
+
+void launch_bnd_rank(MeshDev &m, int B, int rounds, const int *blist, const int *bidx, const int *hull_start,
+                     const int *hull_cnt, const int *hull_off, int *badset, int *bnd, hipStream_t st)
+{
+    int *succ[2] = {bnd, bnd + B}, *d[2] = {bnd + 2 * B, bnd + 3 * B};
+    int *verts = bnd + 4 * B, *owner = bnd + 5 * B, *nxt = bnd + 6 * B, *prv = bnd + 7 * B, *alive = bnd + 8 * B,
+        *pset = bnd + 9 * B;
+    int g = std::min(4096, (B + 255) / 256);
+    GA_LAUNCH(k_bnd_rank_init, dim3(g), dim3(256), 0, st, m, B, blist, bidx, hull_start, succ[0], d[0], badset);
+    int cur = 0;
+    for (int r = 0; r < rounds; r++) {
+        GA_LAUNCH(k_bnd_rank_step, dim3(g), dim3(256), 0, st, B, (const int *)succ[cur], (const int *)d[cur],
+                  succ[1 - cur], d[1 - cur]);
+        cur = 1 - cur;
+    }
+    launch_fill_i32(verts, -1, (size_t)B, st);
+    GA_LAUNCH(k_bnd_place, dim3(g), dim3(256), 0, st, m, B, blist, (const int *)succ[cur], (const int *)d[cur],
+              hull_cnt, hull_off, badset, verts, owner, nxt, prv, alive, pset);
+    launch_reset_counters(m.counters, C_BND, -1, -1, -1, st);
+    GA_LAUNCH(k_bnd_collinear, dim3(g), dim3(256), 0, st, m, B, (const int *)verts, (const int *)nxt,
+              (const int *)prv, m.counters + C_BND);
+}
+
+void launch_kill_frame(MeshDev &m, hipStream_t st)
+{
+    GA_LAUNCH(k_kill_frame, dim3(FIXED_GRID), dim3(256), 0, st, m);
+}
+
+void launch_ear_round(MeshDev &m, int B, int *bnd, const int *hull_cnt, const int *hull_off, int round,
+                      int *list_out, int cnt_out_slot, int op, hipStream_t st)
+{
+    int *verts = bnd + 4 * B, *owner = bnd + 5 * B, *nxt = bnd + 6 * B, *prv = bnd + 7 * B, *alive = bnd + 8 * B,
+        *pset = bnd + 9 * B, *cand = bnd + 10 * B, *win = bnd + 2 * B;   // win reuses d[0] (ranking is done)
+    launch_reset_counters(m.counters, C_EAR, -1, -1, -1, st);
+    int gw = std::min(4096, (B + 3) / 4);   // 4 waves per block, one wave per cycle position
+    GA_LAUNCH(k_ear_cand, dim3(gw), dim3(256), 0, st, m, B, (const int *)verts, (const int *)nxt, (const int *)prv,
+              (const int *)alive, (const int *)pset, hull_cnt, hull_off, cand, m.counters + C_EAR);
+    int g = std::min(4096, (B + 255) / 256);
+    GA_LAUNCH(k_ear_select, dim3(g), dim3(256), 0, st, B, (const int *)nxt, (const int *)prv, (const int *)cand,
+              round, win);
+    GA_LAUNCH(k_ear_apply, dim3(g), dim3(256), 0, st, m, B, (const int *)verts, nxt, prv, alive, owner,
+              (const int *)win, list_out, cnt_out_slot, op);
 }
 
 // ---- canonical output ----------------------------------------------------------------------------

@@ -109,6 +109,8 @@ typedef struct {
     /* work counts of the last run: frame vertices examined by the frame-removal kernel, sum of the
        fan lengths of removed frame vertices, triangles listed for the flip tests */
     long long frameSeen, frameChain, flipListed;
+    /* 1 when the frame was removed in bulk (one boundary cycle per set), 0 round by round */
+    int frameBulk, earRounds;
 } galpha_stats;
 
 galpha_ctx *galphaCreate(int device);

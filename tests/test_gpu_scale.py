@@ -29,7 +29,7 @@ def _check(name, xy, T):
     tris, a, st = _run(xy, T)
     got = digest(tris, a)
     bad = [k for k in want if k != "n" and got.get(k) != want[k]]
-    print("stats", {k: st[k] for k in ("missing", "flips", "incircleExact", "alphaExactCompares",
+    print("stats", {k: st[k] for k in ("missing", "flips", "incircleExact", "alphaExactCompares", "frameBulk", "earRounds", "ms_boundary",
                                        "alphaMaxCluster", "ms_total")}, flush=True)
     assert not bad, "differs from the oracle in: %s" % bad
     return st
