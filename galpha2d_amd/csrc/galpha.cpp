@@ -116,6 +116,18 @@ static void ctx_counter_set(galpha_ctx *c, int slot, int v)
 {
     HIPCHK(hipMemsetD32Async((hipDeviceptr_t)(c->counters.as<int>() + slot), v, 1, c->stream));
 }
+// striped statistics (see kernels.h): fold the stripes on the device, then read
+static void ctx_stats_read(galpha_ctx *c, int *dst64)
+{
+    launch_fold_stats(c->counters.as<int>(), c->stream);
+    ctx_counters_read(c, dst64, C_FOLD, 64);
+}
+static int ctx_stat(galpha_ctx *c, int k)
+{
+    launch_fold_stats(c->counters.as<int>(), c->stream);
+    return ctx_counter(c, C_FOLD + k);
+}
+static const size_t COUNTER_BYTES = sizeof(int) * GA_STRIPE_INTS * (1 + GA_STRIPES);
 
 struct SnapGrid {
     double shx, shy, scale, inv;
@@ -270,7 +282,7 @@ static void ensure_pba(galpha_ctx *c, const BatchPlan &b)
     c->fpix.ensure(4 * (size_t)(b.V - b.ntot + 1));
     c->pix.ensure(4 * (size_t)(b.V + 1));
     c->vset.ensure(4 * (size_t)(b.V + 1));
-    c->counters.ensure(sizeof(int) * C_NCOUNTERS);
+    c->counters.ensure(COUNTER_BYTES);
     c->status.ensure(sizeof(int) * 4);
     size_t ntiles = (size_t)std::max(b.blocks[BM_RECON], 1);
     c->tileCnt.ensure(4 * ntiles);
@@ -414,7 +426,6 @@ static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t
         if (ctx_counter(c, C_EAR) == 0) break;
     }
     *ear_rounds = round + 2;
-    ctx_counter_set(c, C_FRAME, 0);
     return true;
 }
 
@@ -434,7 +445,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     if (nsets == 0) return 0;
     int V = plan.V, n = plan.ntot;
     ensure_pba(c, plan);
-    HIPCHK(hipMemsetAsync(c->counters.p, 0, sizeof(int) * C_NCOUNTERS, st));
+    HIPCHK(hipMemsetAsync(c->counters.p, 0, COUNTER_BYTES, st));
     HIPCHK(hipMemsetAsync(c->status.p, 0, sizeof(int) * 4, st));
     BatchDev bd = upload_plan(c, plan);
     HIPCHK(hipEventRecord(c->ev[0], st));
@@ -495,6 +506,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     m.n = n; m.V = V;
     m.sets = bd.sets; m.vset = bd.vset; m.nsets = nsets;
     m.status = c->status.as<int>(); m.counters = c->counters.as<int>();
+    m.stats = m.counters + GA_STRIPE_INTS;
     m.modA = c->modA.as<int>(); m.modB = c->modB.as<int>(); m.cand = c->cand.as<int>();
     m.scratch = c->scratch.as<int>(); m.scratch_cap = scap;
     c->dbg.ensure(64 * sizeof(double));
@@ -531,14 +543,15 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
         for (int k = 0; k < 4; k++) launch_shift_bulk(m, c->moved.as<unsigned char>(), it + k == 0, st);
         if (ctx_counter(c, C_FLAG) == 0) break;
     }
-    ctx_counter_set(c, C_PEND, 0);
     launch_shift_commit(m, c->moved.as<unsigned char>(), st);
-    int pend = ctx_counter(c, C_PEND);
+    int pend = ctx_stat(c, C_PEND);
     S.shiftBad = pend;
+    int moved_before = 0;
     for (int r = 0; r < 48 && pend > 0; r += 4) {
-        launch_reset_counters(c->counters.as<int>(), C_FLAG, -1, -1, -1, st);
         for (int k = 0; k < 4; k++) launch_shift_greedy(m, r + k, st);
-        int movedn = ctx_counter(c, C_FLAG);
+        int moved_total = ctx_stat(c, C_MOVED);
+        int movedn = moved_total - moved_before;
+        moved_before = moved_total;
         pend -= movedn;
         if (movedn == 0 && r >= 8) break;
     }
@@ -549,7 +562,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
             launch_reset_counters(c->counters.as<int>(), C_SCR, C_MODN, -1, -1, st);
             launch_delete_round(m, c->mx.as<unsigned int>(), next_prio_round(), c->op++, st);
         }
-        int after = ctx_counter(c, C_DELETED);
+        int after = ctx_stat(c, C_DELETED);
         pend -= after - deleted_before;
         deleted_before = after;
         int stt = 0;
@@ -572,7 +585,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
             throw GalphaError("vertex deletion failed (status " + std::to_string(stt) + ")" + d + "]");
         }
     }
-    S.deleted = ctx_counter(c, C_DELETED);
+    S.deleted = ctx_stat(c, C_DELETED);
     HIPCHK(hipEventRecord(c->ev[3], st));
 
     // ---- flips to a Delaunay mesh of the current vertices
@@ -609,9 +622,9 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     int nmiss = ctx_counter(c, C_MISS);
     S.missing = nmiss;
     for (int r = 0; r < 100000; r++) {
-        int cnts[C_NCOUNTERS];
-        ctx_counters_read(c, cnts, 0, C_NCOUNTERS);
-        int left = nmiss - cnts[C_INSERTED] - cnts[C_DUPS];
+        int sts[64];
+        ctx_stats_read(c, sts);
+        int left = nmiss - sts[C_INSERTED] - sts[C_DUPS];
         if (left <= 0) break;
         {
             int stt = 0;
@@ -626,12 +639,15 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
         S.insertRounds++;
         flip_until_done(m.modA, C_MODN, m.modB, C_MODN2, opn);
     }
-    S.inserted = ctx_counter(c, C_INSERTED);
-    S.duplicates = ctx_counter(c, C_DUPS);
+    {
+        int sts[64];
+        ctx_stats_read(c, sts);
+        S.inserted = sts[C_INSERTED];
+        S.duplicates = sts[C_DUPS];
+    }
     HIPCHK(hipEventRecord(c->ev[5], st));
 
     // ---- frame removal + hull convexification
-    ctx_counter_set(c, C_FRAME, V - n);
     // Triangles created by frame removal and convexification go to one list (each slot once);
     // the mesh was Delaunay before, so only their edges can need flips afterwards.
     m.list_op = c->op++;
@@ -646,7 +662,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
             launch_frame_round(m, c->mx.as<unsigned int>(), next_prio_round(), c->modF.as<int>(), C_FRAME_LIST,
                                c->op++, st);
         }
-        int left = ctx_counter(c, C_FRAME);
+        int left = (V - n) - ctx_stat(c, C_FRAME_DONE);
         if (left <= 0) break;
     }
     launch_convexify(m, c->modF.as<int>(), C_FRAME_LIST, c->op++, c->hullCnt.as<int>(), c->hullOff.as<int>(),
@@ -669,7 +685,6 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     c->vals2.ensure(4 * (size_t)nslots + 64);
     c->rot.ensure(4 * (size_t)nslots + 64);
     c->newid.ensure(4 * (size_t)nslots + 64);
-    ctx_counter_set(c, C_RECON_TRIS, 0);
     // one 64-bit key (v0, v1, v2) needs ids below 2^21; otherwise two stable passes
     int wide = (nsets > 1 || n > (1 << 21)) ? 1 : 0;
     unsigned long long *K = c->keys.as<unsigned long long>(), *K2 = c->keys2.as<unsigned long long>();
@@ -688,7 +703,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
             throw GalphaError("radix sort failed");
         order = vals;
     }
-    int F = ctx_counter(c, C_RECON_TRIS);
+    int F = ctx_stat(c, C_RECON_TRIS);
     c->out9.ensure(36 * (size_t)F + 64);
     launch_canon_emit(m, order, c->rot.as<int>(), c->newid.as<int>(), F, c->out9.as<int>(),
                       c->setFirst.as<int>(), st);
@@ -701,15 +716,16 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     for (int s = nsets - 1; s >= 0; s--) c->set_first[s] = sf[s] >= 0 ? sf[s] : c->set_first[s + 1];
     c->last_F = F;
     S.nTris = F;
-    int cnts[C_NCOUNTERS];
+    int cnts[C_NCOUNTERS], sts[64];
     ctx_counters_read(c, cnts, 0, C_NCOUNTERS);
-    S.flips = cnts[C_FLIPS_TOTAL];
-    S.frameSeen = cnts[C_FRAME_SEEN];
-    S.frameChain = cnts[C_FRAME_CHAIN];
+    ctx_stats_read(c, sts);
+    S.flips = sts[C_FLIPS_TOTAL];
+    S.frameSeen = sts[C_FRAME_SEEN];
+    S.frameChain = sts[C_FRAME_CHAIN];
     S.flipListed = cnts[C_FLIP_LISTED];
-    S.incircleTests = cnts[C_TESTS_INCIRCLE];
-    S.incircleExact = cnts[C_EXACT_INCIRCLE];
-    S.orientExact = cnts[C_EXACT_ORIENT];
+    S.incircleTests = sts[C_TESTS_INCIRCLE];
+    S.incircleExact = sts[C_EXACT_INCIRCLE];
+    S.orientExact = sts[C_EXACT_ORIENT];
     S.ms_pba = ev_ms(c, 0, 1);
     S.ms_recon = ev_ms(c, 1, 2);
     S.ms_shift = ev_ms(c, 2, 3) + ev_ms(c, 3, 4);
@@ -999,7 +1015,7 @@ int galphaVoronoi(galpha_ctx *c, const double *xy, int n, int T, int *labels, in
         ensure_pba(c, plan);
         c->xy.ensure(16 * (size_t)n);
         HIPCHK(hipMemcpyAsync(c->xy.p, xy, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipMemsetAsync(c->counters.p, 0, sizeof(int) * C_NCOUNTERS, c->stream));
+        HIPCHK(hipMemsetAsync(c->counters.p, 0, COUNTER_BYTES, c->stream));
         BatchDev bd = upload_plan(c, plan);
         int iters = 0, isl = 0;
         run_pba(c, c->xy.as<double>(), bd, &iters, &isl);

@@ -85,7 +85,8 @@ struct MeshDev {
     const int *vset;     // [V] set of every vertex
     int nsets;
     int *status;         // device error bits
-    int *counters;       // device scratch counters [64]
+    int *counters;       // device scratch counters [64]; [64, 128) folded statistics
+    int *stats;          // striped statistics: counters + GA_STRIPE_INTS, GA_STRIPES stripes
     int *modA, *modB;    // modified-triangle lists (capacity cap)
     int *cand;           // candidate list (capacity 3*cap)
     int *scratch;        // per-winner scratch arena
@@ -138,6 +139,16 @@ struct MeshDev {
 #define C_FRAME_LIST 25       // triangles created by frame removal and convexification
 #define C_BND 26              // boundary vertices of the real region (bulk frame removal)
 #define C_EAR 27              // ear candidates of the current round
+#define C_MOVED 28            // vertices moved by greedy shifting rounds (statistic)
+#define C_FRAME_DONE 29       // frame vertices removed round by round (statistic)
+#ifndef GA_STRIPES
+#define GA_STRIPES 16         // statistic stripes, GA_STRIPE_INTS ints apart (also in dev_common.h)
+#define GA_STRIPE_INTS 1024
+#endif
+#define C_FOLD 64             // counters[C_FOLD + k] = stripe sum of statistic k after a fold
+// Counters C_PEND, C_EXACT_*, C_INSERTED, C_DUPS, C_TESTS_INCIRCLE, C_RECON_TRIS, C_DELETED,
+// C_FLIPS_TOTAL, C_FRAME_SEEN, C_FRAME_CHAIN, C_MOVED and C_FRAME_DONE are striped statistics
+// (MeshDev::stats); the others are single words.
 #define C_DBG 48          // 48..63: first failure dump
 #define C_NCOUNTERS 64
 
@@ -166,6 +177,8 @@ void launch_missing_init(MeshDev &m, const int *vor, int *mlist, int *loc, hipSt
 void launch_insert_round(MeshDev &m, int *mlist, int nmiss_cap, int *loc, int op, hipStream_t st);
 void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, hipStream_t st);
 void launch_reset_counters(int *counters, int a, int b, int d, int e, hipStream_t st);
+// counters[C_FOLD + k] = sum over the stripes of statistic k, for k < 64
+void launch_fold_stats(int *counters, hipStream_t st);
 void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st);
 void launch_flip_pass(MeshDev &m, const int *list_in, int cnt_in_slot, int *list_out, int cnt_out_slot,
                       int prev_op, int op, hipStream_t st);

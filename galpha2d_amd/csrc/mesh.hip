@@ -50,9 +50,16 @@ GA_INLINE double2 snapped(const MeshDev &m, int v)
 // first frame vertex
 GA_INLINE int set_anchor(const MeshDev &m, int v) { return m.vt[m.sets[m.vset[v]].fbase]; }
 
-__device__ __noinline__ int orient_slow(double2 a, double2 b, double2 c, int *status, int *cnt)
+// triangle slots in use; a reservation past the capacity sets GA_ERR_CAPACITY and leaves the
+// counter above it, so loops over the slots stop at the capacity until the host sees the error
+GA_INLINE int ntri_used(const MeshDev &m)
 {
-    atomicAdd(cnt, 1);
+    int n = *m.ntri;
+    return n < m.cap ? n : m.cap;
+}
+
+__device__ __noinline__ int orient_slow(double2 a, double2 b, double2 c, int *status)
+{
     int ovf = 0;
     int s = orient_exact(a, b, c, &ovf);
     if (ovf) atomicOr(status, GA_ERR_EXACT_OVERFLOW);
@@ -63,13 +70,13 @@ GA_INLINE int orient_dev(const MeshDev &m, double2 a, double2 b, double2 c)
 {
     int s = orient_filter(a, b, c);
     if (s != 0) return s;
-    return orient_slow(a, b, c, m.status, m.counters + C_EXACT_ORIENT);
+    stat_inc(m.stats, C_EXACT_ORIENT, true);
+    return orient_slow(a, b, c, m.status);
 }
 
 __device__ __noinline__ int incircle_slow(double2 a, int ia, double2 b, int ib, double2 c, int ic_,
-                                          double2 d, int id, int *status, int *cnt)
+                                          double2 d, int id, int *status)
 {
-    atomicAdd(cnt, 1);
     int ovf = 0;
     int s = incircle_exact_sos(a, ia, b, ib, c, ic_, d, id, &ovf);
     if (ovf) atomicOr(status, GA_ERR_EXACT_OVERFLOW);
@@ -81,7 +88,8 @@ GA_INLINE int incircle_dev(const MeshDev &m, int ia, int ib, int ic_, int id)
     double2 a = m.pos[ia], b = m.pos[ib], c = m.pos[ic_], d = m.pos[id];
     int s = incircle_filter(a, b, c, d);
     if (s != 0) return s;
-    return incircle_slow(a, ia, b, ib, c, ic_, d, id, m.status, m.counters + C_EXACT_INCIRCLE);
+    stat_inc(m.stats, C_EXACT_INCIRCLE, true);
+    return incircle_slow(a, ia, b, ib, c, ic_, d, id, m.status);
 }
 
 // ---- reconstruction --------------------------------------------------------------------
@@ -308,7 +316,7 @@ __global__ void k_shift_init(MeshDev m, unsigned char *moved)
 
 __global__ void k_shift_bulk(MeshDev m, unsigned char *moved)
 {
-    int n = *m.ntri;
+    int n = ntri_used(m);
     GRID_STRIDE(t, n) {
         int4 v = m.tv[t];
         if (v.x == GA_DEAD) continue;
@@ -333,12 +341,13 @@ void launch_shift_bulk(MeshDev &m, unsigned char *moved, int first, hipStream_t 
 __global__ void k_shift_commit(MeshDev m, const unsigned char *moved)
 {
     GRID_STRIDE(v, m.n) {
-        if (m.vstate[v] == VS_PENDING && moved[v]) {
+        bool pend = m.vstate[v] == VS_PENDING;
+        if (pend && moved[v]) {
             m.pos[v] = m.xy[v];
             m.vstate[v] = VS_IN_MESH;
-        } else if (m.vstate[v] == VS_PENDING) {
-            atomicAdd(&m.counters[C_PEND], 1);
+            pend = false;
         }
+        stat_inc(m.stats, C_PEND, pend);
     }
 }
 
@@ -384,11 +393,12 @@ __global__ void k_shift_greedy(MeshDev m, int round, int *decide)
 __global__ void k_shift_greedy_commit(MeshDev m, const int *decide)
 {
     GRID_STRIDE(v, m.n) {
-        if (decide[v]) {
+        bool d = decide[v] != 0;
+        if (d) {
             m.pos[v] = m.xy[v];
             m.vstate[v] = VS_IN_MESH;
-            atomicAdd(&m.counters[C_FLAG], 1);
         }
+        stat_inc(m.stats, C_MOVED, d);
     }
 }
 
@@ -455,9 +465,14 @@ __global__ void k_del_apply(MeshDev m, const unsigned long long *mx, int round, 
             if (++guard > 100000) { win = false; break; }
         } while (t != t0);
         if (!win) continue;
-        // scratch: poly[d], plink[d], slots[d]
-        int base = atomicAdd(&m.counters[C_SCR], 3 * d);
-        if ((size_t)(base + 3 * d) > m.scratch_cap) { atomicOr(m.status, GA_ERR_SCRATCH); continue; }
+        // scratch: poly[d], plink[d], slots[d]; list slots for the d - 2 new triangles
+        int base = wave_reserve_n<28>(&m.counters[C_SCR], 3 * d);
+        int lbase = wave_reserve_n<28>(&m.counters[C_MODN], d >= 2 ? d - 2 : 0);
+        if ((size_t)(base + 3 * d) > m.scratch_cap) {
+            atomicOr(m.status, GA_ERR_SCRATCH);
+            for (int k = 0; k < d - 2; k++) list_out[lbase + k] = -1;
+            continue;
+        }
         int *poly = m.scratch + base, *plink = poly + d, *slots = plink + d;
         t = t0; i = slot_of(m.tv[t0], v);
         for (int k = 0; k < d; k++) {
@@ -510,7 +525,7 @@ __global__ void k_del_apply(MeshDev m, const unsigned long long *mx, int round, 
             bond(m, T, 0, plink[pick]);
             m.vt[A] = T; m.vt[B] = T; m.vt[C] = T;
             m.stamp[T] = op;
-            list_out[atomicAdd(&m.counters[C_MODN], 1)] = T;
+            list_out[lbase + used - 1] = T;
             // new polygon edge (A, C): its other side is slot 1 (opp B) of T
             plink[kp] = enc(T, 1);
             for (int k = pick; k < msz - 1; k++) { poly[k] = poly[k + 1]; plink[k] = plink[k + 1]; }
@@ -526,13 +541,14 @@ __global__ void k_del_apply(MeshDev m, const unsigned long long *mx, int round, 
             bond(m, T, 1, plink[2]);   // (p2,p0)
             m.vt[poly[0]] = T; m.vt[poly[1]] = T; m.vt[poly[2]] = T;
             m.stamp[T] = op;
-            list_out[atomicAdd(&m.counters[C_MODN], 1)] = T;
+            list_out[lbase + used - 1] = T;
         }
+        for (int k = used; k < d - 2; k++) list_out[lbase + k] = -1;   // only after a failure
         for (int k = used; k < d; k++) m.tv[slots[k]] = make_int4(GA_DEAD, GA_DEAD, GA_DEAD, 0);
         m.vstate[v] = VS_MISSING;
         m.vt[v] = slots[0];      // anchor for the re-insertion walk
         m.pos[v] = m.xy[v];
-        atomicAdd(&m.counters[C_DELETED], 1);
+        stat_inc(m.stats, C_DELETED, true);
     }
 }
 
@@ -545,43 +561,58 @@ void launch_delete_round(MeshDev &m, unsigned int *mxraw, int round, int op, hip
 }
 
 // ---- missing-site insertion ------------------------------------------------------------------
-__global__ void k_missing_init(MeshDev m, const int *__restrict__ vor, int *mlist, int *loc)
+// list position k for missing vertex v and its walk start: the anchor a deleted vertex keeps,
+// else a triangle of the nearest labelled pixel
+GA_INLINE void missing_anchor(const MeshDev &m, const int *__restrict__ vor, int v, int k, int *mlist, int *loc)
 {
-    GRID_STRIDE(v, m.n) {
-        bool miss = m.vstate[v] == VS_MISSING;
-        int k = wave_reserve(&m.counters[C_MISS], miss);
-        if (!miss) continue;
-        mlist[k] = v;
-        // a deleted vertex keeps an anchor triangle near its old position in vt[]
-        int t = m.vt[v];
-        if (t < 0 || m.tv[t].x == GA_DEAD) {
-            // the nearest labelled pixel (square rings around v's pixel, radius <= 16) whose
-            // site is in the mesh: its triangles are next to v, so the walk stays short
-            const SetDesc &sd = m.sets[m.vset[v]];
-            int T = 1 << sd.logT;
-            int pp = m.pix[v];
-            int px = pp >> 16, py = pp & 0xffff;
-            t = -1;
-            for (int r = 0; r <= 16 && t < 0; r++) {
-                for (int q = 0; q < (r == 0 ? 1 : 8 * r) && t < 0; q++) {
-                    int dx, dy;
-                    if (r == 0) { dx = 0; dy = 0; }
-                    else {
-                        int side = q / (2 * r), o = q % (2 * r) - r;
-                        if (side == 0) { dx = o; dy = -r; }
-                        else if (side == 1) { dx = r; dy = o; }
-                        else if (side == 2) { dx = -o; dy = r; }
-                        else { dx = -r; dy = -o; }
-                    }
-                    int x = px + dx, y = py + dy;
-                    if (x < 0 || y < 0 || x >= T || y >= T) continue;
-                    int s = vor[sd.toff + ((size_t)x << sd.logT) + y];
-                    if (s >= 0 && m.vstate[s] == VS_IN_MESH) t = m.vt[s];
+    mlist[k] = v;
+    // a deleted vertex keeps an anchor triangle near its old position in vt[]
+    int t = m.vt[v];
+    if (t < 0 || m.tv[t].x == GA_DEAD) {
+        // the nearest labelled pixel (square rings around v's pixel, radius <= 16) whose
+        // site is in the mesh: its triangles are next to v, so the walk stays short
+        const SetDesc &sd = m.sets[m.vset[v]];
+        int T = 1 << sd.logT;
+        int pp = m.pix[v];
+        int px = pp >> 16, py = pp & 0xffff;
+        t = -1;
+        for (int r = 0; r <= 16 && t < 0; r++) {
+            for (int q = 0; q < (r == 0 ? 1 : 8 * r) && t < 0; q++) {
+                int dx, dy;
+                if (r == 0) { dx = 0; dy = 0; }
+                else {
+                    int side = q / (2 * r), o = q % (2 * r) - r;
+                    if (side == 0) { dx = o; dy = -r; }
+                    else if (side == 1) { dx = r; dy = o; }
+                    else if (side == 2) { dx = -o; dy = r; }
+                    else { dx = -r; dy = -o; }
                 }
+                int x = px + dx, y = py + dy;
+                if (x < 0 || y < 0 || x >= T || y >= T) continue;
+                int s = vor[sd.toff + ((size_t)x << sd.logT) + y];
+                if (s >= 0 && m.vstate[s] == VS_IN_MESH) t = m.vt[s];
             }
         }
-        if (t < 0) t = set_anchor(m, v);
-        loc[k] = t;
+    }
+    if (t < 0) t = set_anchor(m, v);
+    loc[k] = t;
+}
+
+__global__ __launch_bounds__(256) void k_missing_init(MeshDev m, const int *__restrict__ vor, int *mlist, int *loc)
+{
+    // block-uniform loop: 1024 vertices per block and step, one list reservation per step
+    for (int b0 = blockIdx.x * 1024; b0 < m.n; b0 += gridDim.x * 1024) {
+        unsigned bits = 0;
+        for (int j = 0; j < 4; j++) {
+            int v = b0 + j * 256 + (int)threadIdx.x;
+            if (v < m.n && m.vstate[v] == VS_MISSING) bits |= 1u << j;
+        }
+        int k0 = block_reserve(&m.counters[C_MISS], __popc(bits));
+        for (int j = 0; j < 4; j++) {
+            if (!((bits >> j) & 1)) continue;
+            int v = b0 + j * 256 + (int)threadIdx.x;
+            missing_anchor(m, vor, v, k0 + __popc(bits & ((1u << j) - 1u)), mlist, loc);
+        }
     }
 }
 
@@ -631,7 +662,7 @@ __global__ void k_locate(MeshDev m, const int *mlist, int *loc, int *lt, int op)
             int w = ic(tt, kk);
             m.vstate[v] = VS_DUP;
             m.vt[v] = -1 - w;      // duplicate of w
-            atomicAdd(&m.counters[C_DUPS], 1);
+            stat_inc(m.stats, C_DUPS, true);
             continue;
         }
         unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)v);
@@ -662,13 +693,12 @@ __global__ void k_insert(MeshDev m, const int *mlist, const int *loc, const int 
         int4 tt = m.tv[t], nt = m.tn[t];
         bool win = m.claim[t] == key;
         if (win && type == 2) win = m.claim[enc_t(ic(nt, i))] == key;
-        int q0 = wave_reserve(&m.counters[C_MODN], win);
-        int q1 = wave_reserve(&m.counters[C_MODN], win);
-        int q2 = wave_reserve(&m.counters[C_MODN], win);
-        int q3 = wave_reserve(&m.counters[C_MODN], win);
+        // four list slots and two new triangle slots per winner, one atomic each per wave
+        int q0 = wave_reserve_mult(&m.counters[C_MODN], win, 4);
+        int q1 = q0 + 1, q2 = q0 + 2, q3 = q0 + 3;
+        int s = wave_reserve_mult(m.ntri, win, 2);
         if (!win) continue;
         if (type == 1) {
-            int s = atomicAdd(m.ntri, 2);
             if (s + 2 > m.cap) {
                 atomicOr(m.status, GA_ERR_CAPACITY);
                 list_out[q0] = list_out[q1] = list_out[q2] = list_out[q3] = -1;
@@ -689,7 +719,6 @@ __global__ void k_insert(MeshDev m, const int *mlist, const int *loc, const int 
         } else {
             int nb = ic(nt, i);
             int u = enc_t(nb), j = enc_j(nb);
-            int s = atomicAdd(m.ntri, 2);
             if (s + 2 > m.cap) {
                 atomicOr(m.status, GA_ERR_CAPACITY);
                 list_out[q0] = list_out[q1] = list_out[q2] = list_out[q3] = -1;
@@ -719,7 +748,7 @@ __global__ void k_insert(MeshDev m, const int *mlist, const int *loc, const int 
         }
         m.vstate[v] = VS_IN_MESH;
         m.pos[v] = m.xy[v];
-        atomicAdd(&m.counters[C_INSERTED], 1);
+        stat_inc(m.stats, C_INSERTED, true);
     }
 }
 
@@ -766,7 +795,7 @@ void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, hipStre
 // every slot in use (dead ones are skipped by the test); no atomics: the list is the identity
 __global__ void k_flip_all(MeshDev m, int op, int *list, int cnt_slot)
 {
-    int n = *m.ntri;
+    int n = ntri_used(m);
     GRID_STRIDE(t, n) {
         m.stamp[t] = op;
         list[t] = t;
@@ -779,40 +808,49 @@ void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st)
     GA_LAUNCH(k_flip_all, dim3(FIXED_GRID), dim3(256), 0, st, m, op, list, C_MODN);
 }
 
-__global__ void k_flip_test(MeshDev m, const int *list, int cnt_slot, int prev_op, int op)
+// Candidates are the listed triangles' edges that fail the incircle test (each edge once: the
+// lower slot of a pair that were both modified in the previous pass skips it).  Block-uniform
+// loop over 1024 list entries per block and step (4 per thread) with one reservation of
+// candidate slots per step.
+__global__ __launch_bounds__(256) void k_flip_test(MeshDev m, const int *list, int cnt_slot, int prev_op, int op)
 {
     int n = m.counters[cnt_slot];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&m.counters[C_FLIP_LISTED], n);
     int tests = 0;
-    GRID_STRIDE(q, n) {
-        int t = list[q];
-        if (t < 0) continue;
-        int4 tt = m.tv[t];
-        if (tt.x == GA_DEAD) continue;
-        int4 nn = m.tn[t];
-        for (int i = 0; i < 3; i++) {
-            int e = ic(nn, i);
-            bool want = false;
-            int u = 0;
-            if (e >= 0) {
-                u = enc_t(e);
-                int j = enc_j(e);
-                if (!(m.stamp[u] == prev_op && u < t)) {
-                    int d = ic(m.tv[u], j);
-                    tests++;
-                    want = incircle_dev(m, tt.x, tt.y, tt.z, d) > 0;
+    for (int b0 = blockIdx.x * 1024; b0 < n; b0 += gridDim.x * 1024) {
+        unsigned bits = 0;   // bit 3j + i: edge i of the j-th triangle of this thread wants a flip
+        int tj[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            int q = b0 + j * 256 + (int)threadIdx.x;
+            int t = q < n ? list[q] : -1;
+            tj[j] = t;
+            if (t < 0) continue;
+            int4 tt = m.tv[t];
+            if (tt.x == GA_DEAD) continue;
+            int4 nn = m.tn[t];
+            for (int i = 0; i < 3; i++) {
+                int e = ic(nn, i);
+                if (e < 0) continue;
+                int u = enc_t(e);
+                if (m.stamp[u] == prev_op && u < t) continue;
+                int d = ic(m.tv[u], enc_j(e));
+                tests++;
+                if (incircle_dev(m, tt.x, tt.y, tt.z, d) > 0) {
+                    unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)(t * 3 + i));
+                    atomicMax(&m.claim[t], key);
+                    atomicMax(&m.claim[u], key);
+                    bits |= 1u << (3 * j + i);
                 }
             }
-            int slot = wave_reserve(&m.counters[C_CAND], want);
-            if (want) {
-                unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)(t * 3 + i));
-                atomicMax(&m.claim[t], key);
-                atomicMax(&m.claim[u], key);
-                m.cand[slot] = t * 3 + i;
-            }
         }
+        int slot = block_reserve(&m.counters[C_CAND], __popc(bits));
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            for (int i = 0; i < 3; i++)
+                if ((bits >> (3 * j + i)) & 1) m.cand[slot++] = tj[j] * 3 + i;
     }
-    wave_count_full(&m.counters[C_TESTS_INCIRCLE], tests);
+    stat_add_full(m.stats, C_TESTS_INCIRCLE, tests);
 }
 
 __global__ void k_flip_commit(MeshDev m, int op, int *list_out, int cnt_out_slot)
@@ -833,8 +871,8 @@ __global__ void k_flip_commit(MeshDev m, int op, int *list_out, int cnt_out_slot
         // lmark keeps every slot at most once per list, so no candidate key repeats.
         bool lt_ = atomicExch(&m.lmark[t], op) != op;
         bool lu_ = win && atomicExch(&m.lmark[u], op) != op;
-        int q1 = wave_reserve(&m.counters[cnt_out_slot], lt_);
-        int q2 = wave_reserve(&m.counters[cnt_out_slot], lu_);
+        int q1, q2;
+        wave_reserve2(&m.counters[cnt_out_slot], lt_, lu_, &q1, &q2);
         if (lt_) list_out[q1] = t;
         if (lu_) list_out[q2] = u;
         if (!win) continue;
@@ -855,8 +893,11 @@ __global__ void k_flip_commit(MeshDev m, int op, int *list_out, int cnt_out_slot
         m.vt[x] = t; m.vt[cc] = t; m.vt[y] = u; m.vt[d] = u;
         flips++;
     }
-    wave_count_full(&m.counters[C_FLIPS], flips);
-    wave_count_full(&m.counters[C_FLIPS_TOTAL], flips);
+    // C_FLIPS only tells the host whether the pass flipped anything: a plain store
+    int wf = flips;
+    for (int d = 32; d > 0; d >>= 1) wf += __shfl_xor(wf, d, 64);
+    if (__lane_id() == 0 && wf) m.counters[C_FLIPS] = 1;
+    stat_add_full(m.stats, C_FLIPS_TOTAL, flips);
 }
 
 void launch_flip_pass(MeshDev &m, const int *list_in, int cnt_in_slot, int *list_out, int cnt_out_slot,
@@ -1009,11 +1050,11 @@ __global__ void k_frame_remove(MeshDev m, const unsigned long long *mx, int roun
         for (int q = used; q < kf; q++) m.tv[slots[q]] = make_int4(GA_DEAD, GA_DEAD, GA_DEAD, 0);
         m.vstate[f] = VS_REMOVED;
         m.vt[f] = -1;
-        atomicAdd(&m.counters[C_FRAME], -1);
+        stat_inc(m.stats, C_FRAME_DONE, true);
         chainsum += kf;
     }
-    wave_count_full(&m.counters[C_FRAME_SEEN], seen);
-    wave_count_full(&m.counters[C_FRAME_CHAIN], chainsum);
+    stat_add_full(m.stats, C_FRAME_SEEN, seen);
+    stat_add_full(m.stats, C_FRAME_CHAIN, chainsum);
 }
 
 void launch_frame_round(MeshDev &m, unsigned int *mxraw, int round, int *list_out, int cnt_out_slot, int op,
@@ -1135,9 +1176,21 @@ void launch_reset_counters(int *counters, int a, int b, int d, int e, hipStream_
     GA_LAUNCH(k_reset_counters, dim3(1), dim3(64), 0, st, counters, a, b, d, e);
 }
 
+__global__ void k_fold_stats(int *c)
+{
+    int k = threadIdx.x, sum = 0;
+    for (int s = 0; s < GA_STRIPES; s++) sum += c[GA_STRIPE_INTS * (1 + s) + k];
+    c[C_FOLD + k] = sum;
+}
+
+void launch_fold_stats(int *counters, hipStream_t st)
+{
+    GA_LAUNCH(k_fold_stats, dim3(1), dim3(64), 0, st, counters);
+}
+
 __global__ void k_hull_edges(MeshDev m, int *hnext, int *hedge, int *hull_cnt, int *hull_start)
 {
-    int n = *m.ntri;
+    int n = ntri_used(m);
     GRID_STRIDE(t, n) {
         int4 v = m.tv[t];
         if (v.x == GA_DEAD) continue;
@@ -1179,7 +1232,7 @@ GA_INLINE bool is_frame_tri(const MeshDev &m, int4 v) { return v.x >= m.n || v.y
 // with two outgoing boundary edges (a pinch) or a meshed real point in no R triangle marks its set
 __global__ void k_bnd_edges(MeshDev m, int *outdeg, int *cover, int *hull_cnt, int *hull_start, int *badset)
 {
-    int n = *m.ntri;
+    int n = ntri_used(m);
     GRID_STRIDE(t, n) {
         int4 v = m.tv[t];
         if (v.x == GA_DEAD || is_frame_tri(m, v)) continue;
@@ -1282,7 +1335,7 @@ __global__ void k_bnd_collinear(MeshDev m, int B, const int *verts, const int *n
 // leave the mesh; boundary vertices take their outgoing boundary triangle as anchor
 __global__ void k_kill_frame(MeshDev m)
 {
-    int n = *m.ntri;
+    int n = ntri_used(m);
     GRID_STRIDE(t, n) {
         int4 v = m.tv[t];
         if (v.x == GA_DEAD || !is_frame_tri(m, v)) continue;
@@ -1449,7 +1502,7 @@ void launch_ear_round(MeshDev &m, int B, int *bnd, const int *hull_cnt, const in
 // on (set, v0) over the order of pass 1 (order1 = vals of pass 1).  Dead slots sort last.
 __global__ void k_canon_keys(MeshDev m, unsigned long long *keys, int *vals, int *rot, int wide)
 {
-    int n = *m.ntri;
+    int n = ntri_used(m);
     int alive = 0;
     GRID_STRIDE(t, n) {
         int4 v = m.tv[t];
@@ -1463,12 +1516,12 @@ __global__ void k_canon_keys(MeshDev m, unsigned long long *keys, int *vals, int
         unsigned long long a = (unsigned)ic(v, r), b = (unsigned)ic(v, (r + 1) % 3), c = (unsigned)ic(v, (r + 2) % 3);
         keys[t] = wide ? ((b << 32) | c) : ((a << 42) | (b << 21) | c);
     }
-    wave_count_full(&m.counters[C_RECON_TRIS], alive);
+    stat_add_full(m.stats, C_RECON_TRIS, alive);
 }
 
 __global__ void k_canon_keys2(MeshDev m, const int *order1, const int *rot, unsigned long long *keys)
 {
-    int n = *m.ntri;
+    int n = ntri_used(m);
     GRID_STRIDE(r, n) {
         int t = order1[r];
         int4 v = m.tv[t];
