@@ -35,9 +35,9 @@ ALGO_BYTES = {
                  "slot 32 written, anchor and back-link 8 written)",
         "bytes": lambda st, T: 12.0 * st["frameSeen"] + 88.0 * st["frameChain"]},
     "k_flip_test": {
-        "model": "152 B per listed triangle (vertices + links 32, three neighbour stamps and far "
-                 "vertices 24, six positions 96)",
-        "bytes": lambda st, T: 152.0 * st["flipListed"]},
+        "model": "156 B per listed triangle (list entry 4, vertices + links 32, three neighbour "
+                 "stamps and far vertices 24, six positions 96)",
+        "bytes": lambda st, T: 156.0 * st["flipListed"]},
     "k_pba_color": {
         "model": "8 B per pixel (envelope read 4 + label written 4)",
         "bytes": lambda st, T: 8.0 * T * T * st["pbaIterations"]},
@@ -207,7 +207,7 @@ def main():
     ap.add_argument("--n", type=int, default=N_POINTS)
     ap.add_argument("--texture", type=int, default=TEXTURE)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--probe", default="k_frame_remove", choices=sorted(ALGO_BYTES),
+    ap.add_argument("--probe", default="k_flip_test", choices=sorted(ALGO_BYTES),
                     help="kernel timed for the roofline figure")
     ap.add_argument("--batched-sets", type=int, default=4096, help="sets of the batched workload (0: skip)")
     ap.add_argument("--plain-sample", type=int, default=256, help="sets timed for the plain per-set loop")

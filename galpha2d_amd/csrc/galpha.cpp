@@ -469,7 +469,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     c->deg.ensure(4 * (size_t)(V + 1));
     c->off.ensure(4 * (size_t)(V + 1));
     c->fill.ensure(4 * (size_t)(V + 1));
-    c->fan.ensure(4 * 3 * (size_t)cap);
+    c->fan.ensure(8 * 3 * (size_t)cap);
     c->moved.ensure((size_t)V);
     c->decide.ensure(4 * (size_t)V);
     c->mx.ensure(8 * (size_t)V);
@@ -523,7 +523,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     // ---- reconstruction + adjacency
     launch_recon(c->vor.as<int>(), c->pix.as<int>(), bd, m, c->tileCnt.as<int>(), c->tileOff.as<int>(),
                  c->scanTmp.p, st);
-    launch_adjacency(m, c->deg.as<int>(), c->off.as<int>(), c->fill.as<int>(), c->fan.as<int>(), c->scanTmp.p, st);
+    launch_adjacency(m, c->deg.as<int>(), c->off.as<int>(), c->fill.as<int>(), c->fan.as<int2>(), c->scanTmp.p, st);
     launch_init_positions(m, st);
     S.reconTris = ctx_counter(c, 40);
     if (ctx_counter(c, 40) + 0 > cap) throw GalphaError("triangle capacity exceeded in reconstruction");
@@ -537,13 +537,27 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     // revert moved vertices of non-ccw triangles until every triangle is ccw (monotone)
     // Round loops launch several rounds per host read of their counters: a round after
     // convergence finds nothing to do, so over-running is harmless.
-    for (int it = 0;; it += 4) {
-        if (it > 100000) throw GalphaError("bulk shifting did not converge");
-        launch_reset_counters(c->counters.as<int>(), C_FLAG, -1, -1, -1, st);
-        for (int k = 0; k < 4; k++) launch_shift_bulk(m, c->moved.as<unsigned char>(), it + k == 0, st);
-        if (ctx_counter(c, C_FLAG) == 0) break;
+    {
+        // worklists in the flip lists (free until the flips), marks in decide[] (free until the
+        // greedy rounds)
+        unsigned char *moved = c->moved.as<unsigned char>();
+        HIPCHK(hipMemsetAsync(m.decide, 0, 4 * (size_t)V, st));
+        int *wl_in = m.modA, *wl_out = m.modB, sin = C_MODN, sout = C_MODN2;
+        launch_shift_bulk(m, moved, m.decide, wl_in, sin, st);
+        for (int it = 0;; it += 4) {
+            if (it > 100000) throw GalphaError("bulk shifting did not converge");
+            for (int k = 0; k < 4; k++) {
+                launch_shift_star(m, moved, m.decide, wl_in, sin, wl_out, sout, st);
+                std::swap(wl_in, wl_out);
+                std::swap(sin, sout);
+            }
+            if (ctx_counter(c, sin) == 0) break;
+        }
     }
     launch_shift_commit(m, c->moved.as<unsigned char>(), st);
+    // the missing-site list is free until the missing stage: it holds the pending vertices
+    m.plist = c->mlist.as<int>();
+    launch_pending_list(m, st);
     int pend = ctx_stat(c, C_PEND);
     S.shiftBad = pend;
     int moved_before = 0;

@@ -87,6 +87,7 @@ struct MeshDev {
     int *status;         // device error bits
     int *counters;       // device scratch counters [64]; [64, 128) folded statistics
     int *stats;          // striped statistics: counters + GA_STRIPE_INTS, GA_STRIPES stripes
+    int *plist;          // vertices still pending after bulk shifting (counters[C_PLIST] entries)
     int *modA, *modB;    // modified-triangle lists (capacity cap)
     int *cand;           // candidate list (capacity 3*cap)
     int *scratch;        // per-winner scratch arena
@@ -141,6 +142,7 @@ struct MeshDev {
 #define C_EAR 27              // ear candidates of the current round
 #define C_MOVED 28            // vertices moved by greedy shifting rounds (statistic)
 #define C_FRAME_DONE 29       // frame vertices removed round by round (statistic)
+#define C_PLIST 30            // entries of the pending-vertex list (MeshDev::plist)
 #ifndef GA_STRIPES
 #define GA_STRIPES 16         // statistic stripes, GA_STRIPE_INTS ints apart (also in dev_common.h)
 #define GA_STRIPE_INTS 1024
@@ -167,9 +169,15 @@ void launch_exclusive_scan(const int *in, int *out, size_t n, void *temp, hipStr
 // mesh.hip
 void launch_recon(const int *vor, const int *pix, const BatchDev &b, MeshDev &m, int *tile_cnt, int *tile_off,
                   void *scan_tmp, hipStream_t st);
-void launch_adjacency(MeshDev &m, int *deg, int *off, int *fill, int *fan, void *scan_tmp, hipStream_t st);
+void launch_adjacency(MeshDev &m, int *deg, int *off, int *fill, int2 *fan, void *scan_tmp, hipStream_t st);
 void launch_init_positions(MeshDev &m, hipStream_t st);
-void launch_shift_bulk(MeshDev &m, unsigned char *moved, int first, hipStream_t st);
+// bulk shifting: one pass over all triangles (reverted vertices listed in wl_out, mark[] zero on
+// entry), then passes over the stars of the vertices the previous pass listed
+void launch_shift_bulk(MeshDev &m, unsigned char *moved, int *mark, int *wl_out, int cnt_out_slot, hipStream_t st);
+void launch_shift_star(MeshDev &m, unsigned char *moved, int *mark, const int *wl_in, int cnt_in_slot, int *wl_out,
+                       int cnt_out_slot, hipStream_t st);
+// lists the vertices still pending into m.plist (count in counters[C_PLIST])
+void launch_pending_list(MeshDev &m, hipStream_t st);
 void launch_shift_commit(MeshDev &m, const unsigned char *moved, hipStream_t st);
 void launch_shift_greedy(MeshDev &m, int round, hipStream_t st);
 void launch_delete_round(MeshDev &m, unsigned int *mx, int round, int op, hipStream_t st);

@@ -227,8 +227,9 @@ __global__ void k_deg(const int4 *__restrict__ tv, const int *__restrict__ ntri,
     }
 }
 
+// fan entry of corner i of triangle t: (enc(t, i), the vertex that follows it counter-clockwise)
 __global__ void k_fanfill(const int4 *__restrict__ tv, const int *__restrict__ ntri, const int *__restrict__ off,
-                          int *__restrict__ fill, int *__restrict__ fan, int *__restrict__ vt)
+                          int *__restrict__ fill, int2 *__restrict__ fan, int *__restrict__ vt)
 {
     int n = *ntri;
     GRID_STRIDE(t, n) {
@@ -237,14 +238,16 @@ __global__ void k_fanfill(const int4 *__restrict__ tv, const int *__restrict__ n
         for (int i = 0; i < 3; i++) {
             int a = ic(v, i);
             int k = atomicAdd(&fill[a], 1);
-            fan[off[a] + k] = enc(t, i);
+            fan[off[a] + k] = make_int2(enc(t, i), ic(v, p1(i)));
             if (k == 0) vt[a] = t;
         }
     }
 }
 
+// Edge i of t runs a -> b (a = v[i + 1], b = v[i + 2]); its twin runs b -> a in the triangle u
+// whose fan entry at b has a as its next vertex, and lies opposite the corner before b in u.
 __global__ void k_match(const int4 *__restrict__ tv, const int *__restrict__ ntri, const int *__restrict__ off,
-                        const int *__restrict__ fan, int4 *__restrict__ tn)
+                        const int2 *__restrict__ fan, int4 *__restrict__ tn)
 {
     int n = *ntri;
     GRID_STRIDE(t, n) {
@@ -253,19 +256,16 @@ __global__ void k_match(const int4 *__restrict__ tv, const int *__restrict__ ntr
         int4 r = make_int4(-1, -1, -1, 0);
         for (int i = 0; i < 3; i++) {
             int a = ic(v, p1(i)), b = ic(v, m1(i));
-            for (int k = off[a]; k < off[a + 1]; k++) {
-                int e = fan[k];
-                int u = enc_t(e), ia = enc_j(e);
-                if (u == t) continue;
-                int4 w = tv[u];
-                if (ic(w, m1(ia)) == b) { sc(r, i, enc(u, p1(ia))); break; }
+            for (int k = off[b], k1 = off[b + 1]; k < k1; k++) {
+                int2 f = fan[k];
+                if (f.y == a) { sc(r, i, enc(enc_t(f.x), m1(enc_j(f.x)))); break; }
             }
         }
         tn[t] = r;
     }
 }
 
-void launch_adjacency(MeshDev &m, int *deg, int *off, int *fill, int *fan, void *scan_tmp, hipStream_t st)
+void launch_adjacency(MeshDev &m, int *deg, int *off, int *fill, int2 *fan, void *scan_tmp, hipStream_t st)
 {
     hipMemsetAsync(deg, 0, sizeof(int) * (m.V + 1), st);
     hipMemsetAsync(fill, 0, sizeof(int) * (m.V + 1), st);
@@ -307,35 +307,83 @@ GA_INLINE double2 cand_pos(const MeshDev &m, const unsigned char *moved, int v)
     return m.pos[v];
 }
 
-// first == 1: initialise moved[] for pending vertices.  Otherwise: every triangle that is not
-// counter-clockwise with the current choice reverts its moved vertices.
+// Bulk shifting: every pending vertex tries its true position at once (moved = 1); a triangle
+// that is not counter-clockwise with the current choice reverts its moved vertices, until no
+// triangle does.  Reverting only ever clears moved[], so the first pass looks at every triangle
+// and later passes only at the stars of the vertices reverted by the pass before (a worklist;
+// mark[] lists each vertex once).
 __global__ void k_shift_init(MeshDev m, unsigned char *moved)
 {
     GRID_STRIDE(v, m.n) moved[v] = (m.vstate[v] == VS_PENDING) ? 1 : 0;
 }
 
-__global__ void k_shift_bulk(MeshDev m, unsigned char *moved)
+// test triangle v (vertex ids) with the current choice; revert and list its moved vertices when
+// it is not counter-clockwise
+GA_INLINE void shift_check(const MeshDev &m, unsigned char *moved, int *mark, int4 v, int *wl_out,
+                           int cnt_out_slot)
+{
+    bool any = (v.x < m.n && moved[v.x]) || (v.y < m.n && moved[v.y]) || (v.z < m.n && moved[v.z]);
+    unsigned got = 0;   // bit k: vertex k of the triangle reverted here and listed
+    if (any) {
+        double2 a = cand_pos(m, moved, v.x), b = cand_pos(m, moved, v.y), cc = cand_pos(m, moved, v.z);
+        if (orient_dev(m, a, b, cc) <= 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                int u = ic(v, k);
+                if (u < m.n && moved[u]) {
+                    moved[u] = 0;
+                    if (atomicExch(&mark[u], 1) == 0) got |= 1u << k;
+                }
+            }
+        }
+    }
+    int q = wave_reserve_n<2>(&m.counters[cnt_out_slot], __popc(got));
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        if ((got >> k) & 1) wl_out[q++] = ic(v, k);
+}
+
+__global__ void k_shift_bulk(MeshDev m, unsigned char *moved, int *mark, int *wl_out, int cnt_out_slot)
 {
     int n = ntri_used(m);
     GRID_STRIDE(t, n) {
         int4 v = m.tv[t];
         if (v.x == GA_DEAD) continue;
-        bool any = (v.x < m.n && moved[v.x]) || (v.y < m.n && moved[v.y]) || (v.z < m.n && moved[v.z]);
-        if (!any) continue;
-        double2 a = cand_pos(m, moved, v.x), b = cand_pos(m, moved, v.y), c = cand_pos(m, moved, v.z);
-        if (orient_dev(m, a, b, c) <= 0) {
-            if (v.x < m.n) moved[v.x] = 0;
-            if (v.y < m.n) moved[v.y] = 0;
-            if (v.z < m.n) moved[v.z] = 0;
-            m.counters[C_FLAG] = 1;
-        }
+        shift_check(m, moved, mark, v, wl_out, cnt_out_slot);
     }
 }
 
-void launch_shift_bulk(MeshDev &m, unsigned char *moved, int first, hipStream_t st)
+__global__ void k_shift_star(MeshDev m, unsigned char *moved, int *mark, const int *wl_in, int cnt_in_slot,
+                             int *wl_out, int cnt_out_slot)
 {
-    if (first) GA_LAUNCH(k_shift_init, dim3(FIXED_GRID), dim3(256), 0, st, m, moved);
-    GA_LAUNCH(k_shift_bulk, dim3(FIXED_GRID), dim3(256), 0, st, m, moved);
+    int nw = m.counters[cnt_in_slot];
+    GRID_STRIDE(k, nw) {
+        int v = wl_in[k];
+        int t0 = m.vt[v], t = t0, i = slot_of(m.tv[t0], v), guard = 0;
+        do {
+            shift_check(m, moved, mark, m.tv[t], wl_out, cnt_out_slot);
+            int e = ic(m.tn[t], p1(i));
+            if (e < 0) { atomicOr(m.status, GA_ERR_WALK); break; }
+            t = enc_t(e);
+            i = p1(enc_j(e));
+            if (++guard > 100000) { atomicOr(m.status, GA_ERR_WALK); break; }
+        } while (t != t0);
+    }
+}
+
+void launch_shift_bulk(MeshDev &m, unsigned char *moved, int *mark, int *wl_out, int cnt_out_slot, hipStream_t st)
+{
+    GA_LAUNCH(k_shift_init, dim3(FIXED_GRID), dim3(256), 0, st, m, moved);
+    launch_reset_counters(m.counters, cnt_out_slot, -1, -1, -1, st);
+    GA_LAUNCH(k_shift_bulk, dim3(FIXED_GRID), dim3(256), 0, st, m, moved, mark, wl_out, cnt_out_slot);
+}
+
+void launch_shift_star(MeshDev &m, unsigned char *moved, int *mark, const int *wl_in, int cnt_in_slot, int *wl_out,
+                       int cnt_out_slot, hipStream_t st)
+{
+    launch_reset_counters(m.counters, cnt_out_slot, -1, -1, -1, st);
+    GA_LAUNCH(k_shift_star, dim3(FIXED_GRID), dim3(256), 0, st, m, moved, mark, wl_in, cnt_in_slot, wl_out,
+              cnt_out_slot);
 }
 
 __global__ void k_shift_commit(MeshDev m, const unsigned char *moved)
@@ -356,6 +404,28 @@ void launch_shift_commit(MeshDev &m, const unsigned char *moved, hipStream_t st)
     GA_LAUNCH(k_shift_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, moved);
 }
 
+// The greedy and deletion rounds only look at vertices that are still pending after bulk
+// shifting (a few percent), listed once; a listed vertex that has left the pending state since is
+// skipped by its state test.
+__global__ __launch_bounds__(256) void k_pending_list(MeshDev m)
+{
+    for (int b0 = blockIdx.x * 1024; b0 < m.n; b0 += gridDim.x * 1024) {
+        unsigned bits = 0;
+        for (int j = 0; j < 4; j++) {
+            int v = b0 + j * 256 + (int)threadIdx.x;
+            if (v < m.n && m.vstate[v] == VS_PENDING) bits |= 1u << j;
+        }
+        int k = block_reserve(&m.counters[C_PLIST], __popc(bits));
+        for (int j = 0; j < 4; j++)
+            if ((bits >> j) & 1) m.plist[k++] = b0 + j * 256 + (int)threadIdx.x;
+    }
+}
+
+void launch_pending_list(MeshDev &m, hipStream_t st)
+{
+    GA_LAUNCH(k_pending_list, dim3(FIXED_GRID), dim3(256), 0, st, m);
+}
+
 GA_INLINE uint32_t vprio(int v, int round) { return hash32((uint32_t)v * 0x9E3779B1u ^ hash32((uint32_t)round + 0x1234567u)); }
 
 // Greedy round: a pending vertex whose pending neighbours all have lower priority moves to
@@ -363,8 +433,10 @@ GA_INLINE uint32_t vprio(int v, int round) { return hash32((uint32_t)v * 0x9E377
 // the commit kernel applies them (so every decision sees the same snapshot).
 __global__ void k_shift_greedy(MeshDev m, int round, int *decide)
 {
-    GRID_STRIDE(v, m.n) {
-        decide[v] = 0;
+    int np = m.counters[C_PLIST];
+    GRID_STRIDE(k, np) {
+        decide[k] = 0;
+        int v = m.plist[k];
         if (m.vstate[v] != VS_PENDING) continue;
         uint32_t p = vprio(v, round);
         int t0 = m.vt[v];
@@ -386,15 +458,17 @@ __global__ void k_shift_greedy(MeshDev m, int round, int *decide)
             i = p1(enc_j(e));
             if (++guard > 100000) { atomicOr(m.status, GA_ERR_WALK); winner = false; break; }
         } while (t != t0);
-        if (winner && ok) decide[v] = 1;
+        if (winner && ok) decide[k] = 1;
     }
 }
 
 __global__ void k_shift_greedy_commit(MeshDev m, const int *decide)
 {
-    GRID_STRIDE(v, m.n) {
-        bool d = decide[v] != 0;
+    int np = m.counters[C_PLIST];
+    GRID_STRIDE(k, np) {
+        bool d = decide[k] != 0;
         if (d) {
+            int v = m.plist[k];
             m.pos[v] = m.xy[v];
             m.vstate[v] = VS_IN_MESH;
         }
@@ -420,7 +494,9 @@ GA_INLINE unsigned long long prio64(int v, int round)
 // distance-2 independence: every vertex records the largest priority in its closed neighbourhood
 __global__ void k_del_prio(MeshDev m, unsigned long long *mx, int round)
 {
-    GRID_STRIDE(v, m.n) {
+    int np = m.counters[C_PLIST];
+    GRID_STRIDE(k, np) {
+        int v = m.plist[k];
         if (m.vstate[v] != VS_PENDING) continue;
         unsigned long long p = prio64(v, round);
         atomicMax(&mx[v], p);
@@ -448,7 +524,9 @@ GA_INLINE void bond(const MeshDev &m, int T, int k, int X)
 
 __global__ void k_del_apply(MeshDev m, const unsigned long long *mx, int round, int op, int *list_out)
 {
-    GRID_STRIDE(v, m.n) {
+    int np = m.counters[C_PLIST];
+    GRID_STRIDE(kk, np) {
+        int v = m.plist[kk];
         if (m.vstate[v] != VS_PENDING) continue;
         unsigned long long p = prio64(v, round);
         if (mx[v] != p) continue;

@@ -67,13 +67,13 @@ struct galpha_server {
 
 // Device bytes a context grows to for `npoints` points on textures of `pixels` pixels, DT + alpha,
 // from the buffers each stage ensures (all with the 1/8 growth headroom): textures 20 B per pixel
-// (site map, column field, stacks, envelopes, labels); mesh about 610 B per vertex (3V triangle
+// (site map, column field, stacks, envelopes, labels); mesh about 650 B per vertex (3V triangle
 // slots with links, remaps, votes, lists and fans; vertex arrays; canonical keys and output);
 // alpha about 1400 B per point (5n spectrum entries with keys, intervals and scans; 13n master
 // entries; sort temporaries); plus a fixed 16 MiB.
 static size_t group_footprint(long long npoints, long long pixels)
 {
-    double b = 20.5 * (double)pixels + 2100.0 * (double)npoints + (16 << 20);
+    double b = 20.5 * (double)pixels + 2150.0 * (double)npoints + (16 << 20);
     return (size_t)b;
 }
 static size_t footprint(long long n, long long T) { return group_footprint(n, T * T); }
