@@ -140,10 +140,10 @@ struct AlphaK {
     int *cnt, *eoff;     // [F]
     int *edge_t, *edge_v, *edge_att, *edge_rank, *edge_numin, *edge_numax;  // [E]
     int *tri_rank;       // [F]
-    unsigned long long *key;   // [S0] rho bits
+    unsigned long long *key;   // [S0] sort keys (sort_key of the value)
     int *id, *id2;       // [S0] entry ids (t or F + e)
     unsigned long long *key2;
-    double *lo, *hi;     // [F + E] per entry id
+    double2 *iv;         // [F + E] per entry id: certified interval (lo, hi) of its value
     double *slo, *shi;   // [S0] sorted
     int *flag;           // [S0] cluster starts
     int *gflag;          // [S0] group starts (exact)
@@ -196,13 +196,25 @@ __global__ void k_edges(AlphaK a)
     }
 }
 
+// Sort keys: the FP64 bits of a value rotated right by 24, so that bits [0, 40) hold its top 40
+// bits (the part the radix sort orders: sign, exponent and 28 mantissa bits) and bits [40, 64)
+// the rest; rotating back gives the value.  (Sorting bits [0, 40) of these keys instead of bits
+// [24, 64) of the plain bits: rocprim's radix sort with a nonzero begin_bit misorders 64-bit keys
+// at some sizes.)
+GA_INLINE unsigned long long sort_key(double v)
+{
+    unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return (u >> 24) | (u << 40);
+}
+GA_INLINE unsigned long long key_bits(unsigned long long k) { return (k << 24) | (k >> 40); }
+
 // relative error bound of the certified rho estimate used for ordering
 __global__ void k_tri_rho(AlphaK a)
 {
     GS(t, a.F) {
         double2 A = a.xy[tv(a, t, 0)], B = a.xy[tv(a, t, 1)], Cc = a.xy[tv(a, t, 2)];
         double v = rho_tri(A, B, Cc);
-        a.key[t] = (unsigned long long)__double_as_longlong(v);
+        a.key[t] = sort_key(v);
         a.id[t] = t;
         // ar with Shewchuk's orient2d bound: |ar - det| <= eb
         double detleft = (A.x - Cc.x) * (B.y - Cc.y), detright = (A.y - Cc.y) * (B.x - Cc.x);
@@ -222,8 +234,7 @@ __global__ void k_tri_rho(AlphaK a)
             lo = 0.0;
             hi = INFINITY;
         }
-        a.lo[t] = lo;
-        a.hi[t] = hi;
+        a.iv[t] = make_double2(lo, hi);
     }
 }
 
@@ -261,8 +272,7 @@ __global__ void k_attached(AlphaK a)
         a.edge_att[e] = att;
         a.flag[e] = !att;
         double v = rho_edge(p, q);
-        a.lo[a.F + e] = v * (1.0 - 8.0 * U53);
-        a.hi[a.F + e] = v * (1.0 + 8.0 * U53);
+        a.iv[a.F + e] = make_double2(v * (1.0 - 8.0 * U53), v * (1.0 + 8.0 * U53));
     }
 }
 
@@ -274,7 +284,7 @@ __global__ void k_edge_entries(AlphaK a, int E)
         int k = a.F + a.rank[e];
         int t = a.edge_t[e], i = a.edge_v[e];
         double v = rho_edge(a.xy[tv(a, t, pl1(i))], a.xy[tv(a, t, mi1(i))]);
-        a.key[k] = (unsigned long long)__double_as_longlong(v);
+        a.key[k] = sort_key(v);
         a.id[k] = a.F + e;
     }
 }
@@ -283,8 +293,9 @@ __global__ void k_gather_iv(AlphaK a, int S0, double *neg_lo_rev)
 {
     GS(k, S0) {
         int id = a.id2[k];
-        a.shi[k] = a.hi[id];
-        double l = a.lo[id];
+        double2 w = a.iv[id];
+        a.shi[k] = w.y;
+        double l = w.x;
         a.slo[k] = l;
         neg_lo_rev[S0 - 1 - k] = -l;
     }
@@ -319,7 +330,8 @@ GA_INLINE void entry_pts(const AlphaK &a, int id, int *st, double2 *p)
 
 GA_INLINE int cmp_ids(const AlphaK &a, int ia, int ib)
 {
-    double la = a.lo[ia], ha = a.hi[ia], lb = a.lo[ib], hb = a.hi[ib];
+    double2 wa = a.iv[ia], wb = a.iv[ib];
+    double la = wa.x, ha = wa.y, lb = wb.x, hb = wb.y;
     if (ha < lb) return -1;
     if (hb < la) return 1;
     atomicAdd(&a.counters[A_EXACT_CMP], 1);
@@ -572,6 +584,16 @@ GA_INLINE double entry_value(const AlphaK &a, int id)
 
 // single set: rank = incl[k] (1-based); the spectrum value of a rank is the smallest FP64 value
 // of its members
+// An entry alone in its certified cluster is never moved by the exact sorts, so key2[k] still
+// holds its FP64 value, and it is alone in its rank (plain store).  Entries of larger clusters
+// recompute their value and take the minimum.
+GA_INLINE void spec_value(const AlphaK &a, int S0, int k, int id, int slot)
+{
+    bool alone = a.flag[k] && (k + 1 == S0 || a.flag[k + 1]);
+    if (alone) a.specbits[slot] = key_bits(a.key2[k]);
+    else atomicMin(&a.specbits[slot], (unsigned long long)__double_as_longlong(entry_value(a, id)));
+}
+
 __global__ void k_ranks(AlphaK a, int S0, const int *incl)
 {
     GS(k, S0) {
@@ -579,7 +601,7 @@ __global__ void k_ranks(AlphaK a, int S0, const int *incl)
         int id = a.id2[k];
         if (id < a.F) a.tri_rank[id] = r;
         else a.edge_rank[id - a.F] = r;
-        atomicMin(&a.specbits[r], (unsigned long long)__double_as_longlong(entry_value(a, id)));
+        spec_value(a, S0, k, id, r);
     }
 }
 
@@ -619,10 +641,11 @@ __global__ void k_ranks_batch(AlphaK a, int S0, const unsigned int *sset2, const
     GS(j, S0) {
         int s = (int)sset2[j];
         int r = incl2[j] - first[s] + 1;
-        int id = a.id2[order2[j]];
+        int k = order2[j];
+        int id = a.id2[k];
         if (id < a.F) a.tri_rank[id] = r;
         else a.edge_rank[id - a.F] = r;
-        atomicMin(&a.specbits[a.specstart[s] + r], (unsigned long long)__double_as_longlong(entry_value(a, id)));
+        spec_value(a, S0, k, id, a.specstart[s] + r);
     }
 }
 
@@ -644,8 +667,13 @@ __global__ void k_edge_intervals(AlphaK a, int E)
         a.edge_numax[e] = mx;
         int key = a.edge_rank[e] != 0 ? a.edge_rank[e] : mn;
         int u = tv(a, t, pl1(i)), w = tv(a, t, mi1(i));
-        atomicMin(&a.vnumin[u], key); atomicMin(&a.vnumin[w], key);
-        atomicMax(&a.vnumax[u], mx); atomicMax(&a.vnumax[w], mx);
+        // vnumin only decreases and vnumax only increases, so a plain read that already shows
+        // a value at or beyond this edge's (stale reads are larger / smaller) makes the atomic
+        // a no-op and it is skipped
+        if (a.vnumin[u] > key) atomicMin(&a.vnumin[u], key);
+        if (a.vnumin[w] > key) atomicMin(&a.vnumin[w], key);
+        if (a.vnumax[u] < mx) atomicMax(&a.vnumax[u], mx);
+        if (a.vnumax[w] < mx) atomicMax(&a.vnumax[w], mx);
     }
 }
 
@@ -900,7 +928,7 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     ab(d, AB_ERANK, 4 * (size_t)E); ab(d, AB_ENMIN, 4 * (size_t)E); ab(d, AB_ENMAX, 4 * (size_t)E);
     ab(d, AB_TRANK, 4 * (size_t)F);
     ab(d, AB_KEY, 8 * NE); ab(d, AB_KEY2, 8 * NE); ab(d, AB_ID, 4 * NE); ab(d, AB_ID2, 4 * NE);
-    ab(d, AB_LO, 8 * NE); ab(d, AB_HI, 8 * NE); ab(d, AB_SLO, 8 * NE); ab(d, AB_SHI, 8 * NE);
+    ab(d, AB_LO, 16 * NE); ab(d, AB_SLO, 8 * NE); ab(d, AB_SHI, 8 * NE);
     ab(d, AB_FLAG, 4 * NE); ab(d, AB_RANK, 4 * NE); ab(d, AB_SPECBITS, 8 * (NE + (size_t)nsets + 2));
     ab(d, AB_NEG, 8 * NE); ab(d, AB_PMAX, 8 * NE); ab(d, AB_NSMIN, 8 * NE); ab(d, AB_MISC, 4 * NE);
     a.edge_t = AP<int>(d, AB_ET); a.edge_v = AP<int>(d, AB_EV); a.edge_att = AP<int>(d, AB_EATT);
@@ -908,7 +936,7 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     a.tri_rank = AP<int>(d, AB_TRANK);
     a.key = AP<unsigned long long>(d, AB_KEY); a.key2 = AP<unsigned long long>(d, AB_KEY2);
     a.id = AP<int>(d, AB_ID); a.id2 = AP<int>(d, AB_ID2);
-    a.lo = AP<double>(d, AB_LO); a.hi = AP<double>(d, AB_HI);
+    a.iv = AP<double2>(d, AB_LO);
     a.slo = AP<double>(d, AB_SLO); a.shi = AP<double>(d, AB_SHI);
     a.flag = AP<int>(d, AB_FLAG); a.rank = AP<int>(d, AB_RANK); a.gflag = AP<int>(d, AB_MISC);
     ab(d, AB_BIGL, 8 * 65536);
@@ -931,12 +959,16 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     int S0 = F + nun;
     d.S = S0;
     GA_LAUNCH(k_edge_entries, dim3(AGRID), dim3(256), 0, st, a, E);
-    // radix sort by FP64 bits (positive doubles order like their bit patterns)
+    // radix sort by the top 40 bits of the FP64 value (nonnegative doubles order like their bit
+    // patterns; 5 passes instead of 8; see sort_key).  The cluster boundaries below only need
+    // "everything before is certainly smaller than everything after", which holds for any order,
+    // so entries that tie on those bits (values within 2^-28 relative) end in one cluster and are
+    // put in exact order there.
     size_t tb = 0;
-    chk(rocprim::radix_sort_pairs(nullptr, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, 64u, st), "sort");
+    chk(rocprim::radix_sort_pairs(nullptr, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, 40u, st), "sort");
     ab(d, AB_TMP, tb + 256);
     tb = d.b[AB_TMP].bytes;
-    chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, 64u, st), "sort");
+    chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, 40u, st), "sort");
     // certified intervals in sorted order, prefix max of hi, suffix min of lo
     double *neg = AP<double>(d, AB_NEG), *pmax = AP<double>(d, AB_PMAX), *nsmin = AP<double>(d, AB_NSMIN);
     GA_LAUNCH(k_gather_iv, dim3(AGRID), dim3(256), 0, st, a, S0, neg);

@@ -1616,18 +1616,22 @@ void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot
     else GA_LAUNCH(k_canon_keys2, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)vals, (const int *)rot, keys);
 }
 
-__global__ void k_canon_newid(const int *order, int F, int *newid)
+// newid[slot] = output position << 2 | rotation of the slot's triangle: one gather per neighbour
+// in k_canon_emit
+__global__ void k_canon_newid(const int *order, const int *rot, int F, int *newid)
 {
-    GRID_STRIDE(r, F) newid[order[r]] = r;
+    GRID_STRIDE(r, F) {
+        int s = order[r];
+        newid[s] = (r << 2) | rot[s];
+    }
 }
 
 // out9 holds global vertex ids and global triangle positions; set_first[s] = first output
 // triangle of set s (left at -1 for a set without triangles)
-__global__ void k_canon_emit(MeshDev m, const int *order, const int *rot, const int *newid, int F, int *out9,
-                             int *set_first)
+__global__ void k_canon_emit(MeshDev m, const int *order, const int *newid, int F, int *out9, int *set_first)
 {
     GRID_STRIDE(r, F) {
-        int s = order[r], ro = rot[s];
+        int s = order[r], ro = newid[s] & 3;
         int4 v = m.tv[s], nb = m.tn[s];
         int *o = out9 + (size_t)r * 9;
         for (int i = 0; i < 3; i++) {
@@ -1636,16 +1640,20 @@ __global__ void k_canon_emit(MeshDev m, const int *order, const int *rot, const 
             int e = ic(nb, si);
             if (e < 0) o[i] = -1;
             else {
-                int u = enc_t(e), j = enc_j(e);
-                o[i] = enc(newid[u], (j - rot[u] + 3) % 3);
+                int nu = newid[enc_t(e)];
+                o[i] = enc(nu >> 2, (enc_j(e) - (nu & 3) + 3) % 3);
             }
             o[6 + i] = 0;
+        }
+        if (m.nsets == 1) {
+            if (r == 0) set_first[0] = 0;
+            continue;
         }
         int set = m.vset[ic(v, ro)];
         if (r == 0) set_first[set] = 0;
         else {
             int sp = order[r - 1];
-            int ps = m.vset[ic(m.tv[sp], rot[sp])];
+            int ps = m.vset[ic(m.tv[sp], newid[sp] & 3)];
             if (ps != set) set_first[set] = r;
         }
     }
@@ -1655,7 +1663,6 @@ void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid,
                        int *set_first, hipStream_t st)
 {
     launch_fill_i32(set_first, -1, (size_t)m.nsets, st);
-    GA_LAUNCH(k_canon_newid, dim3(FIXED_GRID), dim3(256), 0, st, order, F, newid);
-    GA_LAUNCH(k_canon_emit, dim3(FIXED_GRID), dim3(256), 0, st, m, order, rot, (const int *)newid, F, out9,
-              set_first);
+    GA_LAUNCH(k_canon_newid, dim3(FIXED_GRID), dim3(256), 0, st, order, rot, F, newid);
+    GA_LAUNCH(k_canon_emit, dim3(FIXED_GRID), dim3(256), 0, st, m, order, (const int *)newid, F, out9, set_first);
 }
