@@ -86,13 +86,19 @@ struct galpha_ctx {
     galpha_stats stats{};
     hipEvent_t ev[16];
     int op = 1;
-    std::vector<int> host_counters;
+    int *hsnap = nullptr;   // pinned staging for counter reads (SNAP_INTS ints)
 };
 
+static const int SNAP_INTS = 256;
+
+// counter reads go through pinned host memory: one device-to-host copy, one wait
 static void ctx_counters_read(galpha_ctx *c, int *dst, int slot, int n)
 {
-    HIPCHK(hipMemcpyAsync(dst, c->counters.as<int>() + slot, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    if (n > SNAP_INTS) throw GalphaError("counter read too large");
+    HIPCHK(hipMemcpyAsync(c->hsnap, c->counters.as<int>() + slot, sizeof(int) * n, hipMemcpyDeviceToHost,
+                          c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    memcpy(dst, c->hsnap, sizeof(int) * n);
 }
 static int ctx_counter(galpha_ctx *c, int slot)
 {
@@ -106,26 +112,30 @@ static void ctx_read(galpha_ctx *c, void *dst, const void *src, size_t bytes)
     HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
 }
-static int ctx_status(galpha_ctx *c)
-{
-    int v = 0;
-    ctx_read(c, &v, c->status.p, sizeof(int));
-    return v;
-}
+static int ctx_status(galpha_ctx *c) { return ctx_counter(c, C_STATUS); }
 static void ctx_counter_set(galpha_ctx *c, int slot, int v)
 {
     HIPCHK(hipMemsetD32Async((hipDeviceptr_t)(c->counters.as<int>() + slot), v, 1, c->stream));
 }
 // striped statistics (see kernels.h): fold the stripes on the device, then read
-static void ctx_stats_read(galpha_ctx *c, int *dst64)
-{
-    launch_fold_stats(c->counters.as<int>(), c->stream);
-    ctx_counters_read(c, dst64, C_FOLD, 64);
-}
 static int ctx_stat(galpha_ctx *c, int k)
 {
     launch_fold_stats(c->counters.as<int>(), c->stream);
     return ctx_counter(c, C_FOLD + k);
+}
+// all single-word counters, the status word and the folded statistics in one read:
+// snap[k] = counter k, snap[C_FOLD + k] = statistic k
+struct Snap {
+    int v[C_FOLD + 64];
+    int stat(int k) const { return v[C_FOLD + k]; }
+    int status() const { return v[C_STATUS]; }
+};
+static Snap ctx_snapshot(galpha_ctx *c)
+{
+    Snap s;
+    launch_fold_stats(c->counters.as<int>(), c->stream);
+    ctx_counters_read(c, s.v, 0, C_FOLD + 64);
+    return s;
 }
 static const size_t COUNTER_BYTES = sizeof(int) * GA_STRIPE_INTS * (1 + GA_STRIPES);
 
@@ -326,19 +336,22 @@ static void run_pba(galpha_ctx *c, const double *d_xy, const BatchDev &bd, int *
     HIPCHK(hipMemsetAsync(c->bad.p, 0, 4 * (size_t)(bd.V + 1), st));
     PbaBuffers b{c->smap.as<int>(), c->g.as<short>(), c->S.as<int>(), c->cnt.as<int>(),
                  c->E.as<int>(), c->ecnt.as<int>(), c->vor.as<int>()};
-    int it = 0;
+    int it = 0, isl = 0;
     ctx_counter_set(c, C_ISLAND_SITES, 0);
     for (; it < 16; it++) {
         launch_pba(b, bd, st);
         ctx_counter_set(c, C_FLAG, 0);
         launch_islands(c->vor.as<int>(), c->pix.as<int>(), bd, c->bad.as<int>(), c->counters.as<int>() + C_FLAG, st);
-        if (ctx_counter(c, C_FLAG) == 0) break;
+        int cv[C_ISLAND_SITES + 1];   // the flag and the island count in one read
+        ctx_counters_read(c, cv, 0, C_ISLAND_SITES + 1);
+        isl = cv[C_ISLAND_SITES];
+        if (cv[C_FLAG] == 0) break;
         launch_remove_bad(c->bad.as<int>(), c->pix.as<int>(), bd, c->smap.as<int>(),
                           c->counters.as<int>() + C_ISLAND_SITES, st);
     }
     if (it == 16) throw GalphaError("discrete Voronoi diagram did not settle (island certificate)");
     *iters = it + 1;
-    *island_sites = ctx_counter(c, C_ISLAND_SITES);
+    *island_sites = isl;
 }
 
 static float ev_ms(galpha_ctx *c, int a, int b)
@@ -416,16 +429,17 @@ static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t
     if (collinear > std::max(8, B / 100)) return false;
     // from here on the mesh changes
     launch_kill_frame(m, st);
+    // rounds after one without candidates return at once, so four run per read
+    ctx_counter_set(c, C_EAR, -1);
     int round = 0;
-    for (;; round += 2) {
+    for (;; round += 4) {
         if (round > 200000) throw GalphaError("hull convexification did not finish");
-        launch_ear_round(m, B, c->bnd.as<int>(), hcnt, c->hullOff.as<int>(), round, c->modF.as<int>(),
-                         C_FRAME_LIST, c->op++, st);
-        launch_ear_round(m, B, c->bnd.as<int>(), hcnt, c->hullOff.as<int>(), round + 1, c->modF.as<int>(),
-                         C_FRAME_LIST, c->op++, st);
+        for (int k = 0; k < 4; k++)
+            launch_ear_round(m, B, c->bnd.as<int>(), hcnt, c->hullOff.as<int>(), round + k, c->modF.as<int>(),
+                             C_FRAME_LIST, c->op++, st);
         if (ctx_counter(c, C_EAR) == 0) break;
     }
-    *ear_rounds = round + 2;
+    *ear_rounds = round + 4;
     return true;
 }
 
@@ -505,7 +519,8 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     m.vstate = c->vstate.as<int>(); m.pix = c->pix.as<int>();
     m.n = n; m.V = V;
     m.sets = bd.sets; m.vset = bd.vset; m.nsets = nsets;
-    m.status = c->status.as<int>(); m.counters = c->counters.as<int>();
+    m.counters = c->counters.as<int>();
+    m.status = m.counters + C_STATUS;
     m.stats = m.counters + GA_STRIPE_INTS;
     m.modA = c->modA.as<int>(); m.modB = c->modB.as<int>(); m.cand = c->cand.as<int>();
     m.scratch = c->scratch.as<int>(); m.scratch_cap = scap;
@@ -525,12 +540,13 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
                  c->scanTmp.p, st);
     launch_adjacency(m, c->deg.as<int>(), c->off.as<int>(), c->fill.as<int>(), c->fan.as<int2>(), c->scanTmp.p, st);
     launch_init_positions(m, st);
-    S.reconTris = ctx_counter(c, 40);
-    if (ctx_counter(c, 40) + 0 > cap) throw GalphaError("triangle capacity exceeded in reconstruction");
     {
-        int stt = 0;
-        stt = ctx_status(c);
-        if (stt) throw GalphaError("reconstruction produced an invalid triangle (status " + std::to_string(stt) + ")");
+        int cv[C_STATUS + 1];
+        ctx_counters_read(c, cv, 0, C_STATUS + 1);
+        S.reconTris = cv[40];
+        if (cv[40] > cap) throw GalphaError("triangle capacity exceeded in reconstruction");
+        if (cv[C_STATUS])
+            throw GalphaError("reconstruction produced an invalid triangle (status " + std::to_string(cv[C_STATUS]) + ")");
     }
     HIPCHK(hipEventRecord(c->ev[2], st));
     // ---- shifting
@@ -576,11 +592,11 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
             launch_reset_counters(c->counters.as<int>(), C_SCR, C_MODN, -1, -1, st);
             launch_delete_round(m, c->mx.as<unsigned int>(), next_prio_round(), c->op++, st);
         }
-        int after = ctx_stat(c, C_DELETED);
+        Snap sn = ctx_snapshot(c);
+        int after = sn.stat(C_DELETED);
         pend -= after - deleted_before;
         deleted_before = after;
-        int stt = 0;
-        stt = ctx_status(c);
+        int stt = sn.status();
         if (stt) {
             int dbg[16];
             ctx_counters_read(c, dbg, C_DBG, 16);
@@ -599,12 +615,15 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
             throw GalphaError("vertex deletion failed (status " + std::to_string(stt) + ")" + d + "]");
         }
     }
-    S.deleted = ctx_stat(c, C_DELETED);
+    S.deleted = deleted_before;
     HIPCHK(hipEventRecord(c->ev[3], st));
 
     // ---- flips to a Delaunay mesh of the current vertices
     // Lawson passes over the list of triangles to test; a pass with no flip has no candidate
     // left (the highest candidate always wins), so three passes run per host read.
+    // `last` keeps the snapshot of the final check (counters, statistics and status after the
+    // passes), which the callers use instead of another read
+    Snap last;
     auto flip_until_done = [&](int *listA, int cntA, int *listB, int cntB, int prev) {
         int *lin = listA, *lout = listB, sin = cntA, sout = cntB;
         for (int pass = 0; pass < 100000; pass += 3) {
@@ -617,7 +636,8 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
                 std::swap(sin, sout);
             }
             S.flipPasses += 3;
-            if (ctx_counter(c, C_FLIPS) == 0) return;
+            last = ctx_snapshot(c);
+            if (last.v[C_FLIPS] == 0) return;
         }
         throw GalphaError("flipping did not converge");
     };
@@ -633,18 +653,15 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     // ---- missing sites
     ctx_counter_set(c, C_MISS, 0);
     launch_missing_init(m, c->vor.as<int>(), c->mlist.as<int>(), c->loc.as<int>(), st);
-    int nmiss = ctx_counter(c, C_MISS);
+    last = ctx_snapshot(c);
+    int nmiss = last.v[C_MISS];
     S.missing = nmiss;
     for (int r = 0; r < 100000; r++) {
-        int sts[64];
-        ctx_stats_read(c, sts);
-        int left = nmiss - sts[C_INSERTED] - sts[C_DUPS];
+        // `last`: the snapshot after the previous round's flips (or after the list was built)
+        int left = nmiss - last.stat(C_INSERTED) - last.stat(C_DUPS);
         if (left <= 0) break;
-        {
-            int stt = 0;
-            stt = ctx_status(c);
-            if (stt) throw GalphaError("missing-site insertion failed (status " + std::to_string(stt) + ")");
-        }
+        if (last.status())
+            throw GalphaError("missing-site insertion failed (status " + std::to_string(last.status()) + ")");
         if (r > 5000) throw GalphaError("missing-site insertion did not finish");
         int opn = c->op++;
         ctx_counter_set(c, C_MODN, 0);
@@ -653,12 +670,8 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
         S.insertRounds++;
         flip_until_done(m.modA, C_MODN, m.modB, C_MODN2, opn);
     }
-    {
-        int sts[64];
-        ctx_stats_read(c, sts);
-        S.inserted = sts[C_INSERTED];
-        S.duplicates = sts[C_DUPS];
-    }
+    S.inserted = last.stat(C_INSERTED);
+    S.duplicates = last.stat(C_DUPS);
     HIPCHK(hipEventRecord(c->ev[5], st));
 
     // ---- frame removal + hull convexification
@@ -685,14 +698,10 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     HIPCHK(hipEventRecord(c->ev[6], st));
     flip_until_done(c->modF.as<int>(), C_FRAME_LIST, m.modB, C_MODN2, m.list_op);
     HIPCHK(hipEventRecord(c->ev[7], st));
-    {
-        int stt = 0;
-        stt = ctx_status(c);
-        if (stt) throw GalphaError("device pipeline error (status " + std::to_string(stt) + ")");
-    }
+    if (last.status()) throw GalphaError("device pipeline error (status " + std::to_string(last.status()) + ")");
 
     // ---- canonical output
-    int nslots = ctx_counter(c, 40);
+    int nslots = std::min(last.v[40], cap);
     c->keys.ensure(8 * (size_t)nslots + 64);
     c->keys2.ensure(8 * (size_t)nslots + 64);
     c->vals.ensure(4 * (size_t)nslots + 64);
@@ -717,29 +726,31 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
             throw GalphaError("radix sort failed");
         order = vals;
     }
-    int F = ctx_stat(c, C_RECON_TRIS);
-    c->out9.ensure(36 * (size_t)F + 64);
-    launch_canon_emit(m, order, c->rot.as<int>(), c->newid.as<int>(), F, c->out9.as<int>(),
-                      c->setFirst.as<int>(), st);
+    // the live-triangle count stays on the device (folded statistic) until the final read
+    c->out9.ensure(36 * (size_t)nslots + 64);
+    launch_fold_stats(c->counters.as<int>(), st);
+    launch_canon_emit(m, order, c->rot.as<int>(), c->newid.as<int>(),
+                      c->counters.as<int>() + C_FOLD + C_RECON_TRIS, c->out9.as<int>(), c->setFirst.as<int>(), st);
     HIPCHK(hipEventRecord(c->ev[8], st));
     std::vector<int> sf(nsets);
+    Snap fin;
     HIPCHK(hipMemcpyAsync(sf.data(), c->setFirst.p, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(c->hsnap, c->counters.p, sizeof(fin.v), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    memcpy(fin.v, c->hsnap, sizeof(fin.v));
+    int F = fin.stat(C_RECON_TRIS);
     // sets without triangles start where the next set starts
     c->set_first[nsets] = F;
     for (int s = nsets - 1; s >= 0; s--) c->set_first[s] = sf[s] >= 0 ? sf[s] : c->set_first[s + 1];
     c->last_F = F;
     S.nTris = F;
-    int cnts[C_NCOUNTERS], sts[64];
-    ctx_counters_read(c, cnts, 0, C_NCOUNTERS);
-    ctx_stats_read(c, sts);
-    S.flips = sts[C_FLIPS_TOTAL];
-    S.frameSeen = sts[C_FRAME_SEEN];
-    S.frameChain = sts[C_FRAME_CHAIN];
-    S.flipListed = cnts[C_FLIP_LISTED];
-    S.incircleTests = sts[C_TESTS_INCIRCLE];
-    S.incircleExact = sts[C_EXACT_INCIRCLE];
-    S.orientExact = sts[C_EXACT_ORIENT];
+    S.flips = fin.stat(C_FLIPS_TOTAL);
+    S.frameSeen = fin.stat(C_FRAME_SEEN);
+    S.frameChain = fin.stat(C_FRAME_CHAIN);
+    S.flipListed = fin.v[C_FLIP_LISTED];
+    S.incircleTests = fin.stat(C_TESTS_INCIRCLE);
+    S.incircleExact = fin.stat(C_EXACT_INCIRCLE);
+    S.orientExact = fin.stat(C_EXACT_ORIENT);
     S.ms_pba = ev_ms(c, 0, 1);
     S.ms_recon = ev_ms(c, 1, 2);
     S.ms_shift = ev_ms(c, 2, 3) + ev_ms(c, 3, 4);
@@ -862,6 +873,7 @@ galpha_ctx *galphaCreate(int device)
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (int i = 0; i < 16; i++) HIPCHK(hipEventCreate(&c->ev[i]));
+        HIPCHK(hipHostMalloc((void **)&c->hsnap, sizeof(int) * SNAP_INTS, hipHostMallocDefault));
         return c;
     } catch (const std::exception &e) {
         g_last_error = e.what();
@@ -922,6 +934,7 @@ void galphaDestroy(galpha_ctx *c)
     release_all(c);
     for (int i = 0; i < 16; i++) hipEventDestroy(c->ev[i]);
     if (c->stream) hipStreamDestroy(c->stream);
+    if (c->hsnap) hipHostFree(c->hsnap);
     delete c;
 }
 

@@ -143,6 +143,8 @@ struct MeshDev {
 #define C_MOVED 28            // vertices moved by greedy shifting rounds (statistic)
 #define C_FRAME_DONE 29       // frame vertices removed round by round (statistic)
 #define C_PLIST 30            // entries of the pending-vertex list (MeshDev::plist)
+#define C_EAR_PREV 31         // ear candidates of the previous round
+#define C_STATUS 32           // device status word (GA_ERR_* bits); MeshDev::status points here
 #ifndef GA_STRIPES
 #define GA_STRIPES 16         // statistic stripes, GA_STRIPE_INTS ints apart (also in dev_common.h)
 #define GA_STRIPE_INTS 1024
@@ -205,7 +207,7 @@ void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, int *
 // canonical order: one 64-bit pass when wide == 0 (single set, ids < 2^21), else two passes
 void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot, int pass, int wide,
                        hipStream_t st);
-void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, int F, int *out9,
+void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, const int *dF, int *out9,
                        int *set_first, hipStream_t st);
 int sort_pairs_u64(unsigned long long *keys_in, int *vals_in, unsigned long long *keys_out, int *vals_out,
                    int n, int begin_bit, int end_bit, void *temp, size_t *temp_bytes, hipStream_t st);

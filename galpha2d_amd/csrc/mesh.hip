@@ -1438,12 +1438,22 @@ __global__ void k_kill_frame(MeshDev m)
 
 // Ear rounds.  A cycle position p (vertex C between P = prev and N = next) is a candidate when C
 // is reflex (P, C, N clockwise) and no other live vertex of its cycle lies in the closed triangle
-// (P, N, C); one wave tests emptiness.
+// (P, N, C); one wave tests emptiness.  A round after one without candidates has nothing to do
+// (nothing changed), so its kernels return at once and the host checks only every few rounds.
+__global__ void k_ear_round_start(int *c)
+{
+    if (threadIdx.x == 0) {
+        c[C_EAR_PREV] = c[C_EAR];
+        c[C_EAR] = 0;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_ear_cand(MeshDev m, int B, const int *verts, const int *nxt,
                                                   const int *prv, const int *alive, const int *pset,
                                                   const int *hull_cnt, const int *hull_off, int *cand,
                                                   int *ncand)
 {
+    if (m.counters[C_EAR_PREV] == 0) return;
     int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
     int nw = (gridDim.x * blockDim.x) >> 6;
     for (int p = wave; p < B; p += nw) {
@@ -1483,8 +1493,10 @@ GA_INLINE bool ear_beats(int p, int q, int round)
 }
 
 // winners: candidates above every candidate within cycle distance 2 (so two winners share no vertex)
-__global__ void k_ear_select(int B, const int *nxt, const int *prv, const int *cand, int round, int *win)
+__global__ void k_ear_select(int B, const int *nxt, const int *prv, const int *cand, int round, int *win,
+                             const int *counters)
 {
+    if (counters[C_EAR] == 0) return;
     GRID_STRIDE(p, B) {
         int w = 0;
         if (cand[p]) {
@@ -1501,6 +1513,7 @@ __global__ void k_ear_select(int B, const int *nxt, const int *prv, const int *c
 __global__ void k_ear_apply(MeshDev m, int B, const int *verts, int *nxt, int *prv, int *alive, int *owner,
                             const int *win, int *list_out, int cnt_out_slot, int op)
 {
+    if (m.counters[C_EAR] == 0) return;
     GRID_STRIDE(p, B) {
         if (!win[p]) continue;
         int kp = prv[p], kn = nxt[p];
@@ -1563,13 +1576,13 @@ void launch_ear_round(MeshDev &m, int B, int *bnd, const int *hull_cnt, const in
 {
     int *verts = bnd + 4 * B, *owner = bnd + 5 * B, *nxt = bnd + 6 * B, *prv = bnd + 7 * B, *alive = bnd + 8 * B,
         *pset = bnd + 9 * B, *cand = bnd + 10 * B, *win = bnd + 2 * B;   // win reuses d[0] (ranking is done)
-    launch_reset_counters(m.counters, C_EAR, -1, -1, -1, st);
+    GA_LAUNCH(k_ear_round_start, dim3(1), dim3(64), 0, st, m.counters);
     int gw = std::min(4096, (B + 3) / 4);   // 4 waves per block, one wave per cycle position
     GA_LAUNCH(k_ear_cand, dim3(gw), dim3(256), 0, st, m, B, (const int *)verts, (const int *)nxt, (const int *)prv,
               (const int *)alive, (const int *)pset, hull_cnt, hull_off, cand, m.counters + C_EAR);
     int g = std::min(4096, (B + 255) / 256);
     GA_LAUNCH(k_ear_select, dim3(g), dim3(256), 0, st, B, (const int *)nxt, (const int *)prv, (const int *)cand,
-              round, win);
+              round, win, (const int *)m.counters);
     GA_LAUNCH(k_ear_apply, dim3(g), dim3(256), 0, st, m, B, (const int *)verts, nxt, prv, alive, owner,
               (const int *)win, list_out, cnt_out_slot, op);
 }
@@ -1618,8 +1631,9 @@ void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot
 
 // newid[slot] = output position << 2 | rotation of the slot's triangle: one gather per neighbour
 // in k_canon_emit
-__global__ void k_canon_newid(const int *order, const int *rot, int F, int *newid)
+__global__ void k_canon_newid(const int *order, const int *rot, const int *dF, int *newid)
 {
+    int F = *dF;
     GRID_STRIDE(r, F) {
         int s = order[r];
         newid[s] = (r << 2) | rot[s];
@@ -1628,8 +1642,10 @@ __global__ void k_canon_newid(const int *order, const int *rot, int F, int *newi
 
 // out9 holds global vertex ids and global triangle positions; set_first[s] = first output
 // triangle of set s (left at -1 for a set without triangles)
-__global__ void k_canon_emit(MeshDev m, const int *order, const int *newid, int F, int *out9, int *set_first)
+__global__ void k_canon_emit(MeshDev m, const int *order, const int *newid, const int *dF, int *out9,
+                             int *set_first)
 {
+    int F = *dF;
     GRID_STRIDE(r, F) {
         int s = order[r], ro = newid[s] & 3;
         int4 v = m.tv[s], nb = m.tn[s];
@@ -1659,10 +1675,10 @@ __global__ void k_canon_emit(MeshDev m, const int *order, const int *newid, int 
     }
 }
 
-void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, int F, int *out9,
+void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, const int *dF, int *out9,
                        int *set_first, hipStream_t st)
 {
     launch_fill_i32(set_first, -1, (size_t)m.nsets, st);
-    GA_LAUNCH(k_canon_newid, dim3(FIXED_GRID), dim3(256), 0, st, order, rot, F, newid);
-    GA_LAUNCH(k_canon_emit, dim3(FIXED_GRID), dim3(256), 0, st, m, order, (const int *)newid, F, out9, set_first);
+    GA_LAUNCH(k_canon_newid, dim3(FIXED_GRID), dim3(256), 0, st, order, rot, dF, newid);
+    GA_LAUNCH(k_canon_emit, dim3(FIXED_GRID), dim3(256), 0, st, m, order, (const int *)newid, dF, out9, set_first);
 }
