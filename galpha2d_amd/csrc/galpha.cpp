@@ -68,6 +68,7 @@ struct galpha_ctx {
     // PBA
     DevBuf smap, g, S, cnt, E, ecnt, vor, bad, fpix;
     // mesh
+    DevBuf plo;   // phase-2 band range starts
     DevBuf tv, tn, remap, stamp, claim, modA, modB, cand, pos, vt, vstate, pix, deg, off, fill, fan,
         moved, decide, mx, mlist, loc, lt, scratch, hnext, hedge, keys, keys2, vals, vals2, rot, newid,
         out9, counters, status, xy, scanTmp, sortTmp, tileCnt, tileOff, dellist, dbg;
@@ -215,6 +216,7 @@ struct BatchPlan {
     std::vector<SetDesc> sets;
     std::vector<int> bm;              // BM_COUNT x (nsets + 1)
     int blocks[BM_COUNT] = {0};
+    int max_m2 = 1;                   // largest phase-2 band count
     std::vector<int> fpix;            // frame pixels of all sets, in set order
     int ntot = 0, V = 0, sumT = 0;
     size_t P = 0;
@@ -241,7 +243,9 @@ static BatchPlan plan_batch(const int *n, const int *T, const double *bb5, int n
         d.poff = (int)poff;
         d.nf = (int)fp.size();
         d.roff = (int)roff;
-        d.m2 = d.m3 = std::min(16, T[s] / 64);
+        // bands per row: 64 columns each, at most 64 (T / 64 is a power of two)
+        d.m2 = d.m3 = std::max(1, std::min(64, T[s] / 64));
+        b.max_m2 = std::max(b.max_m2, d.m2);
         d.margin = margin_for(T[s]);
         d.shx = sg.shx; d.shy = sg.shy; d.scale = sg.scale; d.inv = sg.inv;
         nfs[s] = d.nf;
@@ -266,10 +270,10 @@ static BatchPlan plan_batch(const int *n, const int *T, const double *bb5, int n
             b.bm[(size_t)k * (nsets + 1) + s] = (int)acc[k];
         }
         if (s == nsets) break;
-        long long Ts = T[s], m = std::min(16, T[s] / 64);
+        long long Ts = T[s], m = std::max(1, std::min(64, T[s] / 64));
         acc[BM_COLS] += Ts / 4;
         acc[BM_STACK] += (Ts / 64) * m;
-        acc[BM_MERGE] += Ts / 64;
+        acc[BM_MERGE] += (Ts / 64) * std::max(1LL, m / 2);
         acc[BM_COLOR] += (Ts / 64) * m;
         acc[BM_PIX] += Ts * Ts / 256;
         acc[BM_RECON] += (Ts - 1) * ((Ts - 1 + 255) / 256);
@@ -284,7 +288,8 @@ static void ensure_pba(galpha_ctx *c, const BatchPlan &b)
     c->smap.ensure(4 * P);
     c->g.ensure(2 * P);
     c->S.ensure(4 * P);
-    c->cnt.ensure(4 * (size_t)16 * b.sumT);
+    c->cnt.ensure(4 * (size_t)64 * b.sumT);
+    c->plo.ensure(4 * (size_t)64 * b.sumT);
     c->E.ensure(4 * P);
     c->ecnt.ensure(4 * (size_t)b.sumT);
     c->vor.ensure(4 * P);
@@ -323,6 +328,7 @@ static BatchDev upload_plan(galpha_ctx *c, const BatchPlan &b)
     d.vset = c->vset.as<int>();
     d.ntot = b.ntot;
     d.V = b.V;
+    d.max_m2 = b.max_m2;
     d.P = b.P;
     return d;
 }
@@ -334,7 +340,7 @@ static void run_pba(galpha_ctx *c, const double *d_xy, const BatchDev &bd, int *
     launch_vset(bd, c->vset.as<int>(), st);
     launch_splat((const double2 *)d_xy, bd, c->smap.as<int>(), c->pix.as<int>(), c->fpix.as<int>(), st);
     HIPCHK(hipMemsetAsync(c->bad.p, 0, 4 * (size_t)(bd.V + 1), st));
-    PbaBuffers b{c->smap.as<int>(), c->g.as<short>(), c->S.as<int>(), c->cnt.as<int>(),
+    PbaBuffers b{c->smap.as<int>(), c->g.as<short>(), c->S.as<int>(), c->cnt.as<int>(), c->plo.as<int>(),
                  c->E.as<int>(), c->ecnt.as<int>(), c->vor.as<int>()};
     int it = 0, isl = 0;
     ctx_counter_set(c, C_ISLAND_SITES, 0);
@@ -890,7 +896,7 @@ static void release_all(galpha_ctx *c)
                       &c->vals, &c->vals2, &c->rot, &c->newid, &c->out9, &c->counters, &c->status, &c->xy,
                       &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
-                      &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd};
+                      &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd, &c->plo};
     for (DevBuf *b : bufs) b->release();
     alpha_release(c->alpha);
 }
@@ -907,7 +913,7 @@ size_t galpha_ctx_device_bytes(galpha_ctx *c)
                       &c->vals, &c->vals2, &c->rot, &c->newid, &c->out9, &c->counters, &c->status, &c->xy,
                       &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
-                      &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd};
+                      &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd, &c->plo};
     size_t t = 0;
     for (DevBuf *b : bufs) t += b->bytes;
     for (auto &b : c->alpha.b) t += b.bytes;

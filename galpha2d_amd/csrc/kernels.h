@@ -52,6 +52,7 @@ struct BatchDev {
     int blocks[BM_COUNT];  // total blocks per map (host copy of bm[k][nsets])
     const int *vset;       // [V] set of every vertex
     int ntot, V;           // real points, all vertices
+    int max_m2;            // largest phase-2 band count of the sets
     size_t P;              // pixels of all textures
     GA_INLINE_HOST_DEVICE const int *bmap(int k) const { return bm + (size_t)k * (nsets + 1); }
 };
@@ -60,7 +61,8 @@ struct PbaBuffers {
     int *smap;     // T*T site id per pixel (0x7fffffff = empty)
     short *g;      // T*T nearest-site row per column
     int *S;        // T*T band stacks
-    int *cnt;      // m2*T stack sizes
+    int *cnt;      // 64*T per set: end (hi) of each band's envelope range
+    int *lo;       // 64*T per set: start (lo) of each band's envelope range
     int *E;        // T*T compacted envelopes
     int *ecnt;     // T envelope sizes
     int *vor;      // T*T nearest site id

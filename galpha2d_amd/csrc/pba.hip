@@ -158,6 +158,7 @@ __global__ __launch_bounds__(256) void k_pba_cols(const int *__restrict__ smap_a
 }
 
 // ---- phase 2: proximate-site stacks per (row, band) ------------------------------------
+#define PBA_MAXB 64   // bands per texture row at most (m2, m3 <= 64)
 GA_INLINE int pk(int x, int h) { return (x << 16) | (h & 0xffff); }
 GA_INLINE int pk_x(int e) { return (int)((unsigned)e >> 16); }
 GA_INLINE int pk_h(int e) { return (int)(short)(e & 0xffff); }
@@ -171,9 +172,11 @@ GA_INLINE bool dominated(int a, int b, int c)
 }
 
 // grid (T/64, m2), block 64: lanes are rows.  Stack entries of band b live in
-// S[(b*Wb + k) * T + y], k < cnt[b*T + y].
+// S[(b*Wb + k) * T + y], k < hi[b*T + y]; lo[b*T + y] = 0 (the merge steps move both ends).
+// The column reads of a band do not depend on the stack, so they are issued 8 at a time ahead
+// of the dominance tests.
 __global__ __launch_bounds__(64) void k_pba_stack(const short *__restrict__ g_all, int *__restrict__ S_all,
-                                                  int *__restrict__ cnt_all, BatchDev bd)
+                                                  int *__restrict__ hi_all, int *__restrict__ lo_all, BatchDev bd)
 {
     const int *bm = bd.bmap(BM_STACK);
     int set = set_of(bm, bd.nsets, blockIdx.x);
@@ -183,88 +186,139 @@ __global__ __launch_bounds__(64) void k_pba_stack(const short *__restrict__ g_al
     int lb = blockIdx.x - bm[set];
     const short *g = g_all + sd.toff;
     int *S = S_all + sd.toff;
-    int *cnt = cnt_all + (size_t)16 * sd.roff;
+    size_t roff = (size_t)PBA_MAXB * sd.roff;
     int y = (lb % nbx) * 64 + threadIdx.x;
     int b = lb / nbx;
     int x0 = b * Wb;
     int top = 0, e0 = 0, e1 = 0;
-    for (int x = x0; x < x0 + Wb; x++) {
-        int gy = g[((size_t)x << logT) + y];
-        if (gy < 0) continue;
-        int c = pk(x, gy - y);
-        while (top >= 2 && dominated(e0, e1, c)) {
-            top--;
-            e1 = e0;
-            if (top >= 2) e0 = S[((size_t)(x0 + top - 2) << logT) + y];
+    for (int xb = x0; xb < x0 + Wb; xb += 8) {
+        short gv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) gv[u] = xb + u < x0 + Wb ? g[((size_t)(xb + u) << logT) + y] : (short)-1;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            int gy = gv[u];
+            if (gy < 0) continue;
+            int c = pk(xb + u, gy - y);
+            while (top >= 2 && dominated(e0, e1, c)) {
+                top--;
+                e1 = e0;
+                if (top >= 2) e0 = S[((size_t)(x0 + top - 2) << logT) + y];
+            }
+            S[((size_t)(x0 + top) << logT) + y] = c;
+            top++;
+            e0 = e1;
+            e1 = c;
         }
-        S[((size_t)(x0 + top) << logT) + y] = c;
-        top++;
-        e0 = e1;
-        e1 = c;
     }
-    cnt[(size_t)b * T + y] = top;
+    hi_all[roff + (size_t)b * T + y] = top;
+    lo_all[roff + (size_t)b * T + y] = 0;
 }
 
-// ---- phase 2b: merge band stacks per row, compact the envelope -------------------------
-#define PBA_MAXB 64
-__global__ __launch_bounds__(64) void k_pba_merge(const int *__restrict__ S_all, const int *__restrict__ cnt_all,
-                                                  int *__restrict__ E_all, int *__restrict__ ecnt_all,
-                                                  BatchDev bd)
+// ---- phase 2b: merge band envelopes pairwise, then compact ------------------------------
+// Step s merges, for every row, the envelope of bands [j*2^s, j*2^s + 2^(s-1)) with that of
+// the next 2^(s-1) bands: the left chain's last element and the right chain's first are
+// popped while the triple at the junction shows one of them dominated (the same exact test
+// as the stacks).  A chain runs through the nonempty [lo, hi) ranges of its bands in order.
+// The lower envelope of a row is unique, so the merge order does not change it.
+// grid (T/64) * (m2/2) blocks of 64 (lanes are rows); blocks whose group does not exist at
+// this step return.
+__global__ __launch_bounds__(64) void k_pba_merge_step(const int *__restrict__ S_all, int *__restrict__ hi_all,
+                                                       int *__restrict__ lo_all, BatchDev bd, int s)
 {
-    __shared__ short lo_s[PBA_MAXB][64], hi_s[PBA_MAXB][64];
-    __shared__ unsigned char bst[PBA_MAXB][64];
     const int *bm = bd.bmap(BM_MERGE);
     int set = set_of(bm, bd.nsets, blockIdx.x);
     const SetDesc &sd = bd.sets[set];
     int logT = sd.logT;
-    int T = 1 << logT, m2 = sd.m2, Wb = T / m2;
+    int T = 1 << logT, nbx = T >> 6, m2 = sd.m2, Wb = T / m2;
+    int gsz = 1 << s, half = gsz >> 1;
+    int lb = blockIdx.x - bm[set];
+    int j = lb / nbx;
+    if ((j + 1) * gsz > m2) return;
     const int *S = S_all + sd.toff;
-    const int *cnt = cnt_all + (size_t)16 * sd.roff;
-    int *E = E_all + sd.toff;
-    int *ecnt = ecnt_all + sd.roff;
-    int lane = threadIdx.x;
-    int y = (blockIdx.x - bm[set]) * 64 + lane;
-    int nb = 0;
+    size_t roff = (size_t)PBA_MAXB * sd.roff;
+    int y = (lb % nbx) * 64 + threadIdx.x;
+    int *HI = hi_all + roff + y, *LO = lo_all + roff + y;   // band b at [b * T]
 #define ELEM(B, I) S[((size_t)((B) * Wb + (I)) << logT) + y]
-    for (int b = 0; b < m2; b++) {
-        int lo = 0, hi = cnt[(size_t)b * T + y];
-        if (hi == 0) continue;
-        while (nb > 0) {
-            int L = bst[nb - 1][lane];
-            int Lhi = hi_s[L][lane], Llo = lo_s[L][lane];
-            int l1 = ELEM(L, Lhi - 1);
-            bool has0 = false;
-            int l0 = 0;
-            if (Lhi - 1 > Llo) { l0 = ELEM(L, Lhi - 2); has0 = true; }
-            else if (nb >= 2) { int L2 = bst[nb - 2][lane]; l0 = ELEM(L2, hi_s[L2][lane] - 1); has0 = true; }
-            int r1 = ELEM(b, lo);
-            if (has0 && dominated(l0, l1, r1)) {
-                hi_s[L][lane] = (short)(Lhi - 1);
-                if (Lhi - 1 == Llo) nb--;
-                continue;
-            }
-            if (lo + 1 < hi) {
-                int r2 = ELEM(b, lo + 1);
-                if (dominated(l1, r1, r2)) { lo++; continue; }
-            }
-            break;
+    int b0 = j * gsz, bmid = b0 + half, b1 = b0 + gsz;
+    // L: last nonempty band of the left half; R: first nonempty band of the right half
+    int L = bmid - 1;
+    while (L >= b0 && HI[(size_t)L * T] <= LO[(size_t)L * T]) L--;
+    int R = bmid;
+    while (R < b1 && HI[(size_t)R * T] <= LO[(size_t)R * T]) R++;
+    if (L < b0 || R >= b1) return;
+    int Llo = LO[(size_t)L * T], Lhi = HI[(size_t)L * T];
+    int Rlo = LO[(size_t)R * T], Rhi = HI[(size_t)R * T];
+    for (;;) {
+        int l1 = ELEM(L, Lhi - 1), r1 = ELEM(R, Rlo);
+        // element before l1 on the left chain
+        bool has0 = false;
+        int l0 = 0;
+        if (Lhi - 1 > Llo) { l0 = ELEM(L, Lhi - 2); has0 = true; }
+        else {
+            int P = L - 1;
+            while (P >= b0 && HI[(size_t)P * T] <= LO[(size_t)P * T]) P--;
+            if (P >= b0) { l0 = ELEM(P, HI[(size_t)P * T] - 1); has0 = true; }
         }
-        bst[nb][lane] = (unsigned char)b;
-        lo_s[b][lane] = (short)lo;
-        hi_s[b][lane] = (short)hi;
-        nb++;
-    }
-    int k = 0;
-    for (int s = 0; s < nb; s++) {
-        int B = bst[s][lane];
-        int lo = lo_s[B][lane], hi = hi_s[B][lane];
-        for (int i = lo; i < hi; i++) {
-            E[((size_t)k << logT) + y] = ELEM(B, i);
-            k++;
+        if (has0 && dominated(l0, l1, r1)) {
+            Lhi--;
+            HI[(size_t)L * T] = Lhi;
+            if (Lhi == Llo) {
+                L--;
+                while (L >= b0 && HI[(size_t)L * T] <= LO[(size_t)L * T]) L--;
+                if (L < b0) break;
+                Llo = LO[(size_t)L * T]; Lhi = HI[(size_t)L * T];
+            }
+            continue;
         }
+        // element after r1 on the right chain
+        bool has2 = false;
+        int r2 = 0;
+        if (Rlo + 1 < Rhi) { r2 = ELEM(R, Rlo + 1); has2 = true; }
+        else {
+            int Q = R + 1;
+            while (Q < b1 && HI[(size_t)Q * T] <= LO[(size_t)Q * T]) Q++;
+            if (Q < b1) { r2 = ELEM(Q, LO[(size_t)Q * T]); has2 = true; }
+        }
+        if (has2 && dominated(l1, r1, r2)) {
+            Rlo++;
+            LO[(size_t)R * T] = Rlo;
+            if (Rlo == Rhi) {
+                R++;
+                while (R < b1 && HI[(size_t)R * T] <= LO[(size_t)R * T]) R++;
+                if (R >= b1) break;
+                Rlo = LO[(size_t)R * T]; Rhi = HI[(size_t)R * T];
+            }
+            continue;
+        }
+        break;
     }
-    ecnt[y] = k;
 #undef ELEM
+}
+
+// grid (T/64) * m2 blocks of 64 (lanes are rows): band b's [lo, hi) goes to the row's compacted
+// envelope after the ranges of bands 0 .. b-1; the last band writes the envelope size
+__global__ __launch_bounds__(64) void k_pba_compact(const int *__restrict__ S_all, const int *__restrict__ hi_all,
+                                                    const int *__restrict__ lo_all, int *__restrict__ E_all,
+                                                    int *__restrict__ ecnt_all, BatchDev bd)
+{
+    const int *bm = bd.bmap(BM_STACK);
+    int set = set_of(bm, bd.nsets, blockIdx.x);
+    const SetDesc &sd = bd.sets[set];
+    int logT = sd.logT;
+    int T = 1 << logT, nbx = T >> 6, m2 = sd.m2, Wb = T / m2;
+    int lb = blockIdx.x - bm[set];
+    int b = lb / nbx;
+    int y = (lb % nbx) * 64 + threadIdx.x;
+    size_t roff = (size_t)PBA_MAXB * sd.roff;
+    const int *HI = hi_all + roff + y, *LO = lo_all + roff + y;
+    int off = 0;
+    for (int q = 0; q < b; q++) off += HI[(size_t)q * T] - LO[(size_t)q * T];
+    int lo = LO[(size_t)b * T], hi = HI[(size_t)b * T];
+    const int *S = S_all + sd.toff;
+    int *E = E_all + sd.toff;
+    for (int i = lo; i < hi; i++) E[((size_t)(off + i - lo) << logT) + y] = S[((size_t)(b * Wb + i) << logT) + y];
+    if (b == m2 - 1) ecnt_all[sd.roff + y] = off + (hi > lo ? hi - lo : 0);
 }
 
 // ---- phase 3: colour ------------------------------------------------------------------
@@ -389,8 +443,11 @@ void launch_splat(const double2 *xy, const BatchDev &b, int *smap, int *pix, con
 void launch_pba(const PbaBuffers &p, const BatchDev &b, hipStream_t st)
 {
     GA_LAUNCH(k_pba_cols, dim3(b.blocks[BM_COLS]), dim3(256), 0, st, p.smap, p.g, b);
-    GA_LAUNCH(k_pba_stack, dim3(b.blocks[BM_STACK]), dim3(64), 0, st, p.g, p.S, p.cnt, b);
-    GA_LAUNCH(k_pba_merge, dim3(b.blocks[BM_MERGE]), dim3(64), 0, st, p.S, p.cnt, p.E, p.ecnt, b);
+    GA_LAUNCH(k_pba_stack, dim3(b.blocks[BM_STACK]), dim3(64), 0, st, p.g, p.S, p.cnt, p.lo, b);
+    for (int s = 1; (1 << s) <= b.max_m2; s++)
+        GA_LAUNCH(k_pba_merge_step, dim3(b.blocks[BM_MERGE]), dim3(64), 0, st, (const int *)p.S, p.cnt, p.lo, b, s);
+    GA_LAUNCH(k_pba_compact, dim3(b.blocks[BM_STACK]), dim3(64), 0, st, (const int *)p.S, (const int *)p.cnt,
+              (const int *)p.lo, p.E, p.ecnt, b);
     GA_LAUNCH(k_pba_color, dim3(b.blocks[BM_COLOR]), dim3(64), 0, st, p.E, p.ecnt, p.smap, p.vor, b);
 }
 
