@@ -547,8 +547,8 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     launch_adjacency(m, c->deg.as<int>(), c->off.as<int>(), c->fill.as<int>(), c->fan.as<int2>(), c->scanTmp.p, st);
     launch_init_positions(m, st);
     {
-        int cv[C_STATUS + 1];
-        ctx_counters_read(c, cv, 0, C_STATUS + 1);
+        int cv[C_NCOUNTERS];   // slot count (40) and status word
+        ctx_counters_read(c, cv, 0, C_NCOUNTERS);
         S.reconTris = cv[40];
         if (cv[40] > cap) throw GalphaError("triangle capacity exceeded in reconstruction");
         if (cv[C_STATUS])
