@@ -69,6 +69,7 @@ struct galpha_ctx {
     DevBuf smap, g, S, cnt, E, ecnt, vor, bad, fpix;
     // mesh
     DevBuf plo;   // phase-2 band range starts
+    DevBuf perm, xys, pcell, ccount, ccursor;   // spatial numbering of the real points
     DevBuf tv, tn, remap, stamp, claim, modA, modB, cand, pos, vt, vstate, pix, deg, off, fill, fan,
         moved, decide, mx, mlist, loc, lt, scratch, hnext, hedge, keys, keys2, vals, vals2, rot, newid,
         out9, counters, status, xy, scanTmp, sortTmp, tileCnt, tileOff, dellist, dbg;
@@ -217,6 +218,7 @@ struct BatchPlan {
     std::vector<int> bm;              // BM_COUNT x (nsets + 1)
     int blocks[BM_COUNT] = {0};
     int max_m2 = 1;                   // largest phase-2 band count
+    int ncells = 0;                   // 16 x 16-pixel cells of all textures
     std::vector<int> fpix;            // frame pixels of all sets, in set order
     int ntot = 0, V = 0, sumT = 0;
     size_t P = 0;
@@ -245,6 +247,8 @@ static BatchPlan plan_batch(const int *n, const int *T, const double *bb5, int n
         d.roff = (int)roff;
         // bands per row: 64 columns each, at most 64 (T / 64 is a power of two)
         d.m2 = d.m3 = std::max(1, std::min(64, T[s] / 64));
+        d.coff = b.ncells;
+        b.ncells += (T[s] / 16) * (T[s] / 16);
         b.max_m2 = std::max(b.max_m2, d.m2);
         d.margin = margin_for(T[s]);
         d.shx = sg.shx; d.shy = sg.shy; d.scale = sg.scale; d.inv = sg.inv;
@@ -333,12 +337,14 @@ static BatchDev upload_plan(galpha_ctx *c, const BatchPlan &b)
     return d;
 }
 
-static void run_pba(galpha_ctx *c, const double *d_xy, const BatchDev &bd, int *iters, int *island_sites)
+// perm: input index of each real point when they were renumbered (NULL: input order)
+static void run_pba(galpha_ctx *c, const double *d_xy, const BatchDev &bd, int *iters, int *island_sites,
+                    const int *perm)
 {
     hipStream_t st = c->stream;
     launch_fill_i32(c->smap.as<int>(), 0x7fffffff, bd.P, st);
     launch_vset(bd, c->vset.as<int>(), st);
-    launch_splat((const double2 *)d_xy, bd, c->smap.as<int>(), c->pix.as<int>(), c->fpix.as<int>(), st);
+    launch_splat((const double2 *)d_xy, bd, perm, c->smap.as<int>(), c->pix.as<int>(), c->fpix.as<int>(), st);
     HIPCHK(hipMemsetAsync(c->bad.p, 0, 4 * (size_t)(bd.V + 1), st));
     PbaBuffers b{c->smap.as<int>(), c->g.as<short>(), c->S.as<int>(), c->cnt.as<int>(), c->plo.as<int>(),
                  c->E.as<int>(), c->ecnt.as<int>(), c->vor.as<int>()};
@@ -449,6 +455,29 @@ static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t
     return true;
 }
 
+// Real points are renumbered in spatial order (set, then 16 x 16-pixel cells in Morton order;
+// any order inside a cell), so that vertex-indexed reads of neighbouring vertices share cache
+// lines.  Returns the coordinates in that order; c->perm maps the new index to the input index,
+// which the rules that depend on point order use (a pixel's site, insertion claims, the
+// symbolic tie-break) and the output reports.
+static const double *renumber_points(galpha_ctx *c, const double *d_xy, const BatchDev &bd, const BatchPlan &plan)
+{
+    hipStream_t st = c->stream;
+    int n = bd.ntot;
+    size_t nc = (size_t)plan.ncells + 1;
+    c->perm.ensure(4 * (size_t)n + 64);
+    c->xys.ensure(16 * (size_t)n + 64);
+    c->pcell.ensure(4 * (size_t)n + 64);
+    c->ccount.ensure(4 * nc + 64);
+    c->ccursor.ensure(4 * nc + 64);
+    c->scanTmp.ensure(scan_temp_bytes(nc) + 4096);
+    launch_vset(bd, c->vset.as<int>(), st);
+    HIPCHK(hipMemsetAsync(c->ccount.p, 0, 4 * nc, st));
+    launch_renumber((const double2 *)d_xy, bd, c->pcell.as<int>(), c->ccount.as<int>(), c->ccursor.as<int>(),
+                    plan.ncells, c->scanTmp.p, c->perm.as<int>(), c->xys.as<double2>(), st);
+    return c->xys.as<double>();
+}
+
 // Delaunay triangulation of every set of the plan (points in d_xy, concatenated in set order).
 // Leaves the canonical triangles of all sets in c->out9 (9 ints each, global vertex ids and
 // global triangle positions, set-major) and the first triangle of each set in c->set_first.
@@ -469,7 +498,8 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     HIPCHK(hipMemsetAsync(c->status.p, 0, sizeof(int) * 4, st));
     BatchDev bd = upload_plan(c, plan);
     HIPCHK(hipEventRecord(c->ev[0], st));
-    run_pba(c, d_xy, bd, &S.pbaIterations, &S.islandSites);
+    d_xy = renumber_points(c, d_xy, bd, plan);
+    run_pba(c, d_xy, bd, &S.pbaIterations, &S.islandSites, c->perm.as<int>());
     HIPCHK(hipEventRecord(c->ev[1], st));
 
     // ---- mesh buffers
@@ -522,6 +552,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     m.stamp = c->stamp.as<int>(); m.claim = c->claim.as<unsigned long long>();
     m.cap = cap; m.ntri = c->counters.as<int>() + 40;   // slot 40 holds the slot count
     m.pos = c->pos.as<double2>(); m.xy = (const double2 *)d_xy; m.vt = c->vt.as<int>();
+    m.perm = c->perm.as<int>();
     m.vstate = c->vstate.as<int>(); m.pix = c->pix.as<int>();
     m.n = n; m.V = V;
     m.sets = bd.sets; m.vset = bd.vset; m.nsets = nsets;
@@ -896,7 +927,8 @@ static void release_all(galpha_ctx *c)
                       &c->vals, &c->vals2, &c->rot, &c->newid, &c->out9, &c->counters, &c->status, &c->xy,
                       &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
-                      &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd, &c->plo};
+                      &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd, &c->plo,
+                      &c->perm, &c->xys, &c->pcell, &c->ccount, &c->ccursor};
     for (DevBuf *b : bufs) b->release();
     alpha_release(c->alpha);
 }
@@ -913,7 +945,8 @@ size_t galpha_ctx_device_bytes(galpha_ctx *c)
                       &c->vals, &c->vals2, &c->rot, &c->newid, &c->out9, &c->counters, &c->status, &c->xy,
                       &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
-                      &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd, &c->plo};
+                      &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd, &c->plo,
+                      &c->perm, &c->xys, &c->pcell, &c->ccount, &c->ccursor};
     size_t t = 0;
     for (DevBuf *b : bufs) t += b->bytes;
     for (auto &b : c->alpha.b) t += b.bytes;
@@ -1051,7 +1084,7 @@ int galphaVoronoi(galpha_ctx *c, const double *xy, int n, int T, int *labels, in
         HIPCHK(hipMemsetAsync(c->counters.p, 0, COUNTER_BYTES, c->stream));
         BatchDev bd = upload_plan(c, plan);
         int iters = 0, isl = 0;
-        run_pba(c, c->xy.as<double>(), bd, &iters, &isl);
+        run_pba(c, c->xy.as<double>(), bd, &iters, &isl, nullptr);
         c->stats.pbaIterations = iters;
         c->stats.islandSites = isl;
         HIPCHK(hipMemcpyAsync(labels, c->vor.p, 4 * (size_t)T * T, hipMemcpyDeviceToHost, c->stream));

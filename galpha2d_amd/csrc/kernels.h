@@ -27,6 +27,7 @@ struct SetDesc {
     int nf, fbase;       // frame vertices and the global id of the first one
     int roff;            // sum of the texture sides of the previous sets (row / column arrays)
     int m2, m3;          // PBA bands for phases 2 and 3
+    int coff;            // first 16 x 16-pixel cell of the set (spatial numbering of the points)
     int margin;
     double shx, shy, scale, inv;   // snap grid: snapped = sh + (pixel - margin) * scale
 };
@@ -90,6 +91,7 @@ struct MeshDev {
     int *counters;       // device scratch counters [64]; [64, 128) folded statistics
     int *stats;          // striped statistics: counters + GA_STRIPE_INTS, GA_STRIPES stripes
     int *plist;          // vertices still pending after bulk shifting (counters[C_PLIST] entries)
+    const int *perm;     // input index of real vertex v (real vertices are numbered in spatial order)
     int *modA, *modB;    // modified-triangle lists (capacity cap)
     int *cand;           // candidate list (capacity 3*cap)
     int *scratch;        // per-winner scratch arena
@@ -161,7 +163,13 @@ struct MeshDev {
 // pba.hip
 void launch_fill_i32(int *a, int v, size_t n, hipStream_t st);
 void launch_vset(const BatchDev &b, int *vset, hipStream_t st);
-void launch_splat(const double2 *xy, const BatchDev &b, int *smap, int *pix, const int *fpix, hipStream_t st);
+// perm: input index of each real point (NULL: points in input order)
+void launch_splat(const double2 *xy, const BatchDev &b, const int *perm, int *smap, int *pix, const int *fpix,
+                  hipStream_t st);
+// spatial numbering of the real points: perm[new] = input index, xys[new] = xy[perm[new]];
+// ccount (ncells + 1 ints) zero on entry
+void launch_renumber(const double2 *xy, const BatchDev &b, int *cell, int *ccount, int *cursor, int ncells,
+                     void *scan_tmp, int *perm, double2 *xys, hipStream_t st);
 void launch_pba(const PbaBuffers &p, const BatchDev &b, hipStream_t st);
 void launch_islands(const int *vor, const int *pix, const BatchDev &b, int *bad, int *flag, hipStream_t st);
 void launch_remove_bad(int *bad, const int *pix, const BatchDev &b, int *smap, int *removed, hipStream_t st);

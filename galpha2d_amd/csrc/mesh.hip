@@ -50,6 +50,10 @@ GA_INLINE double2 snapped(const MeshDev &m, int v)
 // first frame vertex
 GA_INLINE int set_anchor(const MeshDev &m, int v) { return m.vt[m.sets[m.vset[v]].fbase]; }
 
+// input index of vertex v: real vertices are numbered in spatial order (perm maps them back),
+// frame vertices keep their ids (above every real index)
+GA_INLINE int oid(const MeshDev &m, int v) { return v < m.n ? m.perm[v] : v; }
+
 // triangle slots in use; a reservation past the capacity sets GA_ERR_CAPACITY and leaves the
 // counter above it, so loops over the slots stop at the capacity until the host sees the error
 GA_INLINE int ntri_used(const MeshDev &m)
@@ -89,7 +93,8 @@ GA_INLINE int incircle_dev(const MeshDev &m, int ia, int ib, int ic_, int id)
     int s = incircle_filter(a, b, c, d);
     if (s != 0) return s;
     stat_inc(m.stats, C_EXACT_INCIRCLE, true);
-    return incircle_slow(a, ia, b, ib, c, ic_, d, id, m.status);
+    // the symbolic tie-break ranks vertices by input index
+    return incircle_slow(a, oid(m, ia), b, oid(m, ib), c, oid(m, ic_), d, oid(m, id), m.status);
 }
 
 // ---- reconstruction --------------------------------------------------------------------
@@ -743,7 +748,7 @@ __global__ void k_locate(MeshDev m, const int *mlist, int *loc, int *lt, int op)
             stat_inc(m.stats, C_DUPS, true);
             continue;
         }
-        unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)v);
+        unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)oid(m, v));
         if (z == 0) {
             atomicMax(&m.claim[t], key);
             lt[k] = (1 << 2);
@@ -766,7 +771,7 @@ __global__ void k_insert(MeshDev m, const int *mlist, const int *loc, const int 
         if (L < 0) continue;
         int v = mlist[k];
         int t = loc[k];
-        unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)v);
+        unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)oid(m, v));
         int type = L >> 2, i = L & 3;
         int4 tt = m.tv[t], nt = m.tn[t];
         bool win = m.claim[t] == key;
@@ -1600,11 +1605,12 @@ __global__ void k_canon_keys(MeshDev m, unsigned long long *keys, int *vals, int
         vals[t] = t;
         if (v.x == GA_DEAD) { keys[t] = ~0ull; rot[t] = 0; continue; }
         alive++;
+        int4 o = make_int4(oid(m, v.x), oid(m, v.y), oid(m, v.z), 0);   // input indices
         int r = 0;
-        if (v.y < ic(v, r)) r = 1;
-        if (v.z < ic(v, r)) r = 2;
+        if (o.y < ic(o, r)) r = 1;
+        if (o.z < ic(o, r)) r = 2;
         rot[t] = r;
-        unsigned long long a = (unsigned)ic(v, r), b = (unsigned)ic(v, (r + 1) % 3), c = (unsigned)ic(v, (r + 2) % 3);
+        unsigned long long a = (unsigned)ic(o, r), b = (unsigned)ic(o, (r + 1) % 3), c = (unsigned)ic(o, (r + 2) % 3);
         keys[t] = wide ? ((b << 32) | c) : ((a << 42) | (b << 21) | c);
     }
     stat_add_full(m.stats, C_RECON_TRIS, alive);
@@ -1617,8 +1623,8 @@ __global__ void k_canon_keys2(MeshDev m, const int *order1, const int *rot, unsi
         int t = order1[r];
         int4 v = m.tv[t];
         if (v.x == GA_DEAD) { keys[r] = ~0ull; continue; }
-        unsigned long long a = (unsigned)ic(v, rot[t]);
-        keys[r] = ((unsigned long long)m.vset[a] << 32) | a;
+        int a = ic(v, rot[t]);
+        keys[r] = ((unsigned long long)m.vset[a] << 32) | (unsigned)oid(m, a);
     }
 }
 
@@ -1652,7 +1658,7 @@ __global__ void k_canon_emit(MeshDev m, const int *order, const int *newid, cons
         int *o = out9 + (size_t)r * 9;
         for (int i = 0; i < 3; i++) {
             int si = (i + ro) % 3;
-            o[3 + i] = ic(v, si);
+            o[3 + i] = oid(m, ic(v, si));
             int e = ic(nb, si);
             if (e < 0) o[i] = -1;
             else {

@@ -54,23 +54,82 @@ __global__ void k_vset(const SetDesc *__restrict__ sets, int nsets, int ntot, in
     vset[v] = lo;
 }
 
-// Map real points to pixels of their set's texture.  pix = (px << 16) | py; the smallest
-// index wins a pixel.
-__global__ void k_splat(const double2 *__restrict__ xy, BatchDev b, int *__restrict__ smap,
-                        int *__restrict__ pix)
+// pixel of a real point in its set's texture (clamped inside the margin)
+GA_INLINE void point_pixel(const SetDesc &sd, double2 p, int *px_, int *py_)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.ntot) return;
-    const SetDesc &sd = b.sets[b.vset[i]];
-    double2 p = xy[i];
     int margin = sd.margin;
     int px = (int)floor((p.x - sd.shx) * sd.inv) + margin;
     int py = (int)floor((p.y - sd.shy) * sd.inv) + margin;
     int T = 1 << sd.logT;
-    px = px < margin ? margin : (px > T - 1 - margin ? T - 1 - margin : px);
-    py = py < margin ? margin : (py > T - 1 - margin ? T - 1 - margin : py);
+    *px_ = px < margin ? margin : (px > T - 1 - margin ? T - 1 - margin : px);
+    *py_ = py < margin ? margin : (py > T - 1 - margin ? T - 1 - margin : py);
+}
+
+// Spatial order of the real points: a counting sort by 16 x 16-pixel cell, cells of a set in
+// Morton order after the cells of the previous sets (SetDesc::coff).  The order inside a cell is
+// arbitrary; rules that depend on point order (the pixel's site, insertion claims) use the
+// input index.
+GA_INLINE unsigned spread14(unsigned v)
+{
+    v &= 0x3fffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+__global__ void k_cell_count(const double2 *__restrict__ xy, BatchDev b, int *__restrict__ cell,
+                             int *__restrict__ ccount)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.ntot) return;
+    const SetDesc &sd = b.sets[b.vset[i]];
+    int px, py;
+    point_pixel(sd, xy[i], &px, &py);
+    int c = sd.coff + (int)((spread14((unsigned)px >> 4) << 1) | spread14((unsigned)py >> 4));
+    cell[i] = c;
+    atomicAdd(&ccount[c], 1);
+}
+
+// cursor = exclusive prefix of the cell counts; perm[new index] = input index
+__global__ void k_cell_scatter(const int *__restrict__ cell, int n, int *__restrict__ cursor, int *__restrict__ perm)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) perm[atomicAdd(&cursor[cell[i]], 1)] = i;
+}
+
+__global__ void k_gather_xy(const double2 *__restrict__ xy, const int *__restrict__ perm, int n,
+                            double2 *__restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = xy[perm[i]];
+}
+
+// Map real points to pixels of their set's texture.  pix = (px << 16) | py; the point with the
+// smallest input index (perm[i], or i without a renumbering) wins a pixel; k_splat_own then
+// stores the winner's own index.
+__global__ void k_splat(const double2 *__restrict__ xy, BatchDev b, const int *__restrict__ perm,
+                        int *__restrict__ smap, int *__restrict__ pix)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.ntot) return;
+    const SetDesc &sd = b.sets[b.vset[i]];
+    int px, py;
+    point_pixel(sd, xy[i], &px, &py);
     pix[i] = (px << 16) | py;
-    atomicMin(&smap[sd.toff + ((size_t)px << sd.logT) + py], i);
+    atomicMin(&smap[sd.toff + ((size_t)px << sd.logT) + py], perm ? perm[i] : i);
+}
+
+__global__ void k_splat_own(BatchDev b, const int *__restrict__ perm, const int *__restrict__ pix,
+                            int *__restrict__ smap)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.ntot) return;
+    const SetDesc &sd = b.sets[b.vset[i]];
+    int pp = pix[i];
+    int *cellp = &smap[sd.toff + ((size_t)(pp >> 16) << sd.logT) + (pp & 0xffff)];
+    if (*cellp == perm[i]) *cellp = i;    // one winner per pixel: no other thread writes here
 }
 
 // Frame sites (global ids ntot .. V-1) at their border pixels (fpix: packed local pixels).
@@ -431,10 +490,26 @@ void launch_vset(const BatchDev &b, int *vset, hipStream_t st)
         GA_LAUNCH(k_vset, dim3((b.V + 255) / 256), dim3(256), 0, st, b.sets, b.nsets, b.ntot, b.V, vset);
 }
 
-void launch_splat(const double2 *xy, const BatchDev &b, int *smap, int *pix, const int *fpix, hipStream_t st)
+// ccount (ncells + 1 entries) must be zero on entry
+void launch_renumber(const double2 *xy, const BatchDev &b, int *cell, int *ccount, int *cursor, int ncells,
+                     void *scan_tmp, int *perm, double2 *xys, hipStream_t st)
 {
-    if (b.ntot > 0)
-        GA_LAUNCH(k_splat, dim3((b.ntot + 255) / 256), dim3(256), 0, st, xy, b, smap, pix);
+    int n = b.ntot;
+    if (n <= 0) return;
+    GA_LAUNCH(k_cell_count, dim3((n + 255) / 256), dim3(256), 0, st, xy, b, cell, ccount);
+    launch_exclusive_scan(ccount, cursor, (size_t)ncells + 1, scan_tmp, st);
+    GA_LAUNCH(k_cell_scatter, dim3((n + 255) / 256), dim3(256), 0, st, (const int *)cell, n, cursor, perm);
+    GA_LAUNCH(k_gather_xy, dim3((n + 255) / 256), dim3(256), 0, st, xy, (const int *)perm, n, xys);
+}
+
+void launch_splat(const double2 *xy, const BatchDev &b, const int *perm, int *smap, int *pix, const int *fpix,
+                  hipStream_t st)
+{
+    if (b.ntot > 0) {
+        GA_LAUNCH(k_splat, dim3((b.ntot + 255) / 256), dim3(256), 0, st, xy, b, perm, smap, pix);
+        if (perm)
+            GA_LAUNCH(k_splat_own, dim3((b.ntot + 255) / 256), dim3(256), 0, st, b, perm, (const int *)pix, smap);
+    }
     int nf = b.V - b.ntot;
     if (nf > 0)
         GA_LAUNCH(k_splat_frame, dim3((nf + 255) / 256), dim3(256), 0, st, fpix, b, smap, pix);
