@@ -767,7 +767,8 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     c->out9.ensure(36 * (size_t)nslots + 64);
     launch_fold_stats(c->counters.as<int>(), st);
     launch_canon_emit(m, order, c->rot.as<int>(), c->newid.as<int>(),
-                      c->counters.as<int>() + C_FOLD + C_RECON_TRIS, c->out9.as<int>(), c->setFirst.as<int>(), st);
+                      c->counters.as<int>() + C_FOLD + C_RECON_TRIS, c->out9.as<int>(), c->setFirst.as<int>(),
+                      c->keys2.as<int>(), st);   // keys2 is free after the sort: output sets
     HIPCHK(hipEventRecord(c->ev[8], st));
     std::vector<int> sf(nsets);
     Snap fin;
@@ -775,6 +776,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     HIPCHK(hipMemcpyAsync(c->hsnap, c->counters.p, sizeof(fin.v), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     memcpy(fin.v, c->hsnap, sizeof(fin.v));
+    if (fin.status()) throw GalphaError("canonical output failed (status " + std::to_string(fin.status()) + ")");
     int F = fin.stat(C_RECON_TRIS);
     // sets without triangles start where the next set starts
     c->set_first[nsets] = F;

@@ -218,7 +218,7 @@ void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, int *
 void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot, int pass, int wide,
                        hipStream_t st);
 void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, const int *dF, int *out9,
-                       int *set_first, hipStream_t st);
+                       int *set_first, int *rset, hipStream_t st);
 int sort_pairs_u64(unsigned long long *keys_in, int *vals_in, unsigned long long *keys_out, int *vals_out,
                    int n, int begin_bit, int end_bit, void *temp, size_t *temp_bytes, hipStream_t st);
 int sort_pairs_u32(unsigned int *keys_in, int *vals_in, unsigned int *keys_out, int *vals_out,

@@ -1646,15 +1646,17 @@ __global__ void k_canon_newid(const int *order, const int *rot, const int *dF, i
     }
 }
 
-// out9 holds global vertex ids and global triangle positions; set_first[s] = first output
-// triangle of set s (left at -1 for a set without triangles)
-__global__ void k_canon_emit(MeshDev m, const int *order, const int *newid, const int *dF, int *out9,
-                             int *set_first)
+// out9 holds global vertex ids and global triangle positions.  The live slots are walked in slot
+// order (their reads stay spatially coherent) and each writes its output record at its position.
+// Batches: rset[r] = set of output triangle r, for k_canon_set_first.
+__global__ void k_canon_emit(MeshDev m, const int *newid, int *out9, int *rset)
 {
-    int F = *dF;
-    GRID_STRIDE(r, F) {
-        int s = order[r], ro = newid[s] & 3;
-        int4 v = m.tv[s], nb = m.tn[s];
+    int n = ntri_used(m);
+    GRID_STRIDE(s, n) {
+        int4 v = m.tv[s];
+        if (v.x == GA_DEAD) continue;
+        int nr = newid[s], r = nr >> 2, ro = nr & 3;
+        int4 nb = m.tn[s];
         int *o = out9 + (size_t)r * 9;
         for (int i = 0; i < 3; i++) {
             int si = (i + ro) % 3;
@@ -1667,24 +1669,29 @@ __global__ void k_canon_emit(MeshDev m, const int *order, const int *newid, cons
             }
             o[6 + i] = 0;
         }
-        if (m.nsets == 1) {
-            if (r == 0) set_first[0] = 0;
-            continue;
-        }
-        int set = m.vset[ic(v, ro)];
-        if (r == 0) set_first[set] = 0;
-        else {
-            int sp = order[r - 1];
-            int ps = m.vset[ic(m.tv[sp], newid[sp] & 3)];
-            if (ps != set) set_first[set] = r;
-        }
+        if (m.nsets > 1) rset[r] = m.vset[v.x];
+    }
+}
+
+// set_first[s] = first output triangle of set s (left at -1 for a set without triangles)
+__global__ void k_canon_set_first(MeshDev m, const int *rset, const int *dF, int *set_first)
+{
+    int F = *dF;
+    if (m.nsets == 1) {
+        if (blockIdx.x == 0 && threadIdx.x == 0 && F > 0) set_first[0] = 0;
+        return;
+    }
+    GRID_STRIDE(r, F) {
+        int set = rset[r];
+        if (r == 0 || rset[r - 1] != set) set_first[set] = r;
     }
 }
 
 void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, const int *dF, int *out9,
-                       int *set_first, hipStream_t st)
+                       int *set_first, int *rset, hipStream_t st)
 {
     launch_fill_i32(set_first, -1, (size_t)m.nsets, st);
     GA_LAUNCH(k_canon_newid, dim3(FIXED_GRID), dim3(256), 0, st, order, rot, dF, newid);
-    GA_LAUNCH(k_canon_emit, dim3(FIXED_GRID), dim3(256), 0, st, m, order, (const int *)newid, dF, out9, set_first);
+    GA_LAUNCH(k_canon_emit, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)newid, out9, rset);
+    GA_LAUNCH(k_canon_set_first, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)rset, dF, set_first);
 }
