@@ -893,18 +893,22 @@ void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st)
 
 // Candidates are the listed triangles' edges that fail the incircle test (each edge once: the
 // lower slot of a pair that were both modified in the previous pass skips it).  Block-uniform
-// loop over 1024 list entries per block and step (4 per thread) with one reservation of
-// candidate slots per step.
+// loop over 256 * per list entries per block and step with one reservation of candidate slots
+// per step: per = 4 for lists longer than four times the grid (fewer reservations), else 1 (more
+// waves in flight for the latency-bound gathers).
 __global__ __launch_bounds__(256) void k_flip_test(MeshDev m, const int *list, int cnt_slot, int prev_op, int op)
 {
     int n = m.counters[cnt_slot];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&m.counters[C_FLIP_LISTED], n);
     int tests = 0;
-    for (int b0 = blockIdx.x * 1024; b0 < n; b0 += gridDim.x * 1024) {
+    int per = n > 4 * (int)(gridDim.x * 256) ? 4 : 1;
+    for (int b0 = blockIdx.x * 256 * per; b0 < n; b0 += gridDim.x * 256 * per) {
         unsigned bits = 0;   // bit 3j + i: edge i of the j-th triangle of this thread wants a flip
         int tj[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
+            tj[j] = -1;
+            if (j >= per) continue;
             int q = b0 + j * 256 + (int)threadIdx.x;
             int t = q < n ? list[q] : -1;
             tj[j] = t;
