@@ -207,6 +207,13 @@ GA_INLINE unsigned long long sort_key(double v)
     return (u >> 24) | (u << 40);
 }
 GA_INLINE unsigned long long key_bits(unsigned long long k) { return (k << 24) | (k >> 40); }
+// Batches order entries by (set, value): the set above the value's top 40 bits, so certified
+// clusters never span sets (equal values in different sets, e.g. repeated requests, are never
+// compared).  The low value bits are not kept; they are recomputed where needed.
+GA_INLINE unsigned long long batch_key(int set, double v)
+{
+    return ((unsigned long long)set << 40) | ((unsigned long long)__double_as_longlong(v) >> 24);
+}
 
 // relative error bound of the certified rho estimate used for ordering
 __global__ void k_tri_rho(AlphaK a)
@@ -214,7 +221,7 @@ __global__ void k_tri_rho(AlphaK a)
     GS(t, a.F) {
         double2 A = a.xy[tv(a, t, 0)], B = a.xy[tv(a, t, 1)], Cc = a.xy[tv(a, t, 2)];
         double v = rho_tri(A, B, Cc);
-        a.key[t] = sort_key(v);
+        a.key[t] = a.vset ? batch_key(a.vset[tv(a, t, 0)], v) : sort_key(v);
         a.id[t] = t;
         // ar with Shewchuk's orient2d bound: |ar - det| <= eb
         double detleft = (A.x - Cc.x) * (B.y - Cc.y), detright = (A.y - Cc.y) * (B.x - Cc.x);
@@ -284,14 +291,19 @@ __global__ void k_edge_entries(AlphaK a, int E)
         int k = a.F + a.rank[e];
         int t = a.edge_t[e], i = a.edge_v[e];
         double v = rho_edge(a.xy[tv(a, t, pl1(i))], a.xy[tv(a, t, mi1(i))]);
-        a.key[k] = sort_key(v);
+        a.key[k] = a.vset ? batch_key(a.vset[tv(a, t, 0)], v) : sort_key(v);
         a.id[k] = a.F + e;
     }
 }
 
-__global__ void k_gather_iv(AlphaK a, int S0, double *neg_lo_rev)
+__global__ void k_gather_iv(AlphaK a, int S0, double *neg_lo_rev, int *sset, int *sset_rev)
 {
     GS(k, S0) {
+        if (sset) {
+            int s = (int)(a.key2[k] >> 40);
+            sset[k] = s;
+            sset_rev[S0 - 1 - k] = s;
+        }
         int id = a.id2[k];
         double2 w = a.iv[id];
         a.shi[k] = w.y;
@@ -301,12 +313,13 @@ __global__ void k_gather_iv(AlphaK a, int S0, double *neg_lo_rev)
     }
 }
 
-// cluster starts: boundary before k when max(hi[0..k-1]) < min(lo[k..])
-__global__ void k_cluster_flags(AlphaK a, int S0, const double *pmax, const double *neg_smin_rev)
+// cluster starts: boundary before k when max(hi[0..k-1]) < min(lo[k..]); with sets (scans
+// segmented by set), also where the set changes
+__global__ void k_cluster_flags(AlphaK a, int S0, const double *pmax, const double *neg_smin_rev, const int *sset)
 {
     GS(k, S0) {
         int f = 1;
-        if (k > 0) {
+        if (k > 0 && !(sset && sset[k] != sset[k - 1])) {
             double m = pmax[k - 1];
             double s = -neg_smin_rev[S0 - 1 - k];
             f = m < s;
@@ -590,7 +603,8 @@ GA_INLINE double entry_value(const AlphaK &a, int id)
 GA_INLINE void spec_value(const AlphaK &a, int S0, int k, int id, int slot)
 {
     bool alone = a.flag[k] && (k + 1 == S0 || a.flag[k + 1]);
-    if (alone) a.specbits[slot] = key_bits(a.key2[k]);
+    if (alone && !a.vset) a.specbits[slot] = key_bits(a.key2[k]);
+    else if (alone) a.specbits[slot] = (unsigned long long)__double_as_longlong(entry_value(a, id));
     else atomicMin(&a.specbits[slot], (unsigned long long)__double_as_longlong(entry_value(a, id)));
 }
 
@@ -605,15 +619,11 @@ __global__ void k_ranks(AlphaK a, int S0, const int *incl)
     }
 }
 
-// batches: the exact order and the global ranks (equal exact value <=> equal global rank) are
-// computed over all sets at once; a stable sort by set then gives every set its entries in
-// exact order, and local ranks count the distinct global ranks inside the set.
-__global__ void k_entry_set(AlphaK a, int S0, unsigned int *sset, int *pos)
+// batches: entries are in (set, exact value) order, so the global group ranks count up through
+// the sets; local ranks count the groups inside the set.
+__global__ void k_iota(int n, int *a)
 {
-    GS(k, S0) {
-        sset[k] = (unsigned)tri_set(a, entry_tri(a, a.id2[k]));
-        pos[k] = k;
-    }
+    GS(k, n) a[k] = k;
 }
 
 __global__ void k_batch_flags(int S0, const unsigned int *sset2, const int *order2, const int *incl, int *flags2)
@@ -965,21 +975,48 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     // so entries that tie on those bits (values within 2^-28 relative) end in one cluster and are
     // put in exact order there.
     size_t tb = 0;
-    chk(rocprim::radix_sort_pairs(nullptr, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, 40u, st), "sort");
+    unsigned sbits = 0;   // batches: set bits above the 40 value bits
+    if (nsets > 1)
+        while ((1LL << sbits) < (long long)nsets) sbits++;
+    const unsigned kbits = 40u + sbits;
+    chk(rocprim::radix_sort_pairs(nullptr, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, kbits, st), "sort");
     ab(d, AB_TMP, tb + 256);
     tb = d.b[AB_TMP].bytes;
-    chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, 40u, st), "sort");
-    // certified intervals in sorted order, prefix max of hi, suffix min of lo
+    chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, kbits, st), "sort");
+    // certified intervals in sorted order, prefix max of hi, suffix min of lo (per set in batches)
     double *neg = AP<double>(d, AB_NEG), *pmax = AP<double>(d, AB_PMAX), *nsmin = AP<double>(d, AB_NSMIN);
-    GA_LAUNCH(k_gather_iv, dim3(AGRID), dim3(256), 0, st, a, S0, neg);
-    tb = 0;
-    chk(rocprim::inclusive_scan(nullptr, tb, a.shi, pmax, (size_t)S0, rocprim::maximum<double>(), st), "scan");
-    ab(d, AB_TMP, tb + 256);
-    tb = d.b[AB_TMP].bytes;
-    chk(rocprim::inclusive_scan(d.b[AB_TMP].p, tb, a.shi, pmax, (size_t)S0, rocprim::maximum<double>(), st), "scan");
-    tb = d.b[AB_TMP].bytes;
-    chk(rocprim::inclusive_scan(d.b[AB_TMP].p, tb, neg, nsmin, (size_t)S0, rocprim::maximum<double>(), st), "scan");
-    GA_LAUNCH(k_cluster_flags, dim3(AGRID), dim3(256), 0, st, a, S0, (const double *)pmax, (const double *)nsmin);
+    int *sset = nullptr, *sset_rev = nullptr;
+    if (nsets > 1) {
+        ab(d, AB_BS0, 4 * (size_t)S0 + 64);
+        ab(d, AB_BS1, 4 * (size_t)S0 + 64);
+        sset = AP<int>(d, AB_BS0);
+        sset_rev = AP<int>(d, AB_BS1);
+    }
+    GA_LAUNCH(k_gather_iv, dim3(AGRID), dim3(256), 0, st, a, S0, neg, sset, sset_rev);
+    if (nsets > 1) {
+        tb = 0;
+        chk(rocprim::inclusive_scan_by_key(nullptr, tb, sset, a.shi, pmax, (size_t)S0, rocprim::maximum<double>(),
+                                           rocprim::equal_to<int>(), st), "scan");
+        ab(d, AB_TMP, tb + 256);
+        tb = d.b[AB_TMP].bytes;
+        chk(rocprim::inclusive_scan_by_key(d.b[AB_TMP].p, tb, sset, a.shi, pmax, (size_t)S0,
+                                           rocprim::maximum<double>(), rocprim::equal_to<int>(), st), "scan");
+        tb = d.b[AB_TMP].bytes;
+        chk(rocprim::inclusive_scan_by_key(d.b[AB_TMP].p, tb, sset_rev, neg, nsmin, (size_t)S0,
+                                           rocprim::maximum<double>(), rocprim::equal_to<int>(), st), "scan");
+    } else {
+        tb = 0;
+        chk(rocprim::inclusive_scan(nullptr, tb, a.shi, pmax, (size_t)S0, rocprim::maximum<double>(), st), "scan");
+        ab(d, AB_TMP, tb + 256);
+        tb = d.b[AB_TMP].bytes;
+        chk(rocprim::inclusive_scan(d.b[AB_TMP].p, tb, a.shi, pmax, (size_t)S0, rocprim::maximum<double>(), st),
+            "scan");
+        tb = d.b[AB_TMP].bytes;
+        chk(rocprim::inclusive_scan(d.b[AB_TMP].p, tb, neg, nsmin, (size_t)S0, rocprim::maximum<double>(), st),
+            "scan");
+    }
+    GA_LAUNCH(k_cluster_flags, dim3(AGRID), dim3(256), 0, st, a, S0, (const double *)pmax, (const double *)nsmin,
+              (const int *)sset);
     GA_LAUNCH(k_copy_i32, dim3(AGRID), dim3(256), 0, st, (const int *)a.flag, a.gflag, S0);
     GA_LAUNCH(k_clusters, dim3(AGRID), dim3(256), 0, st, a, S0);
     big_clusters(d, a, S0, st);
@@ -1002,23 +1039,15 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     if (nsets == 1) {
         GA_LAUNCH(k_ranks, dim3(AGRID), dim3(256), 0, st, a, S0, (const int *)incl);
     } else {
-        // entries in (set, exact value) order: stable sort of positions by set
-        ab(d, AB_BS0, 4 * (size_t)S0 + 64); ab(d, AB_BS1, 4 * (size_t)S0 + 64);
-        ab(d, AB_BS2, 4 * (size_t)S0 + 64); ab(d, AB_BS3, 4 * (size_t)S0 + 64);
+        // entries are already in (set, exact value) order; sset[j] is the set of position j
+        ab(d, AB_BS3, 4 * (size_t)S0 + 64);
         ab(d, AB_BS4, 4 * (size_t)S0 + 64);
         ab(d, AB_BSET, 4 * 3 * (size_t)(nsets + 1));
-        unsigned *sset = AP<unsigned>(d, AB_BS0), *sset2 = AP<unsigned>(d, AB_BS2);
-        int *pos = AP<int>(d, AB_BS1), *order2 = AP<int>(d, AB_BS3), *flags2 = AP<int>(d, AB_BS4);
+        const unsigned *sset2 = (const unsigned *)sset;
+        int *pos = sset_rev;   // free: the reversed sets were only needed by the scan
+        int *order2 = AP<int>(d, AB_BS3), *flags2 = AP<int>(d, AB_BS4);
         int *first = AP<int>(d, AB_BSET), *last = first + (nsets + 1), *sstart = last + (nsets + 1);
-        GA_LAUNCH(k_entry_set, dim3(AGRID), dim3(256), 0, st, a, S0, sset, pos);
-        unsigned sbits = 1;
-        while ((1LL << sbits) <= (long long)nsets) sbits++;
-        tb = 0;
-        chk(rocprim::radix_sort_pairs(nullptr, tb, sset, sset2, pos, order2, (unsigned)S0, 0u, sbits, st), "sort");
-        ab(d, AB_TMP, tb + 256);
-        tb = d.b[AB_TMP].bytes;
-        chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, sset, sset2, pos, order2, (unsigned)S0, 0u, sbits, st),
-            "sort");
+        GA_LAUNCH(k_iota, dim3(AGRID), dim3(256), 0, st, S0, order2);
         GA_LAUNCH(k_batch_flags, dim3(AGRID), dim3(256), 0, st, S0, (const unsigned *)sset2, (const int *)order2,
                   (const int *)incl, flags2);
         int *incl2 = pos;   // pos is free after the sort
@@ -1138,14 +1167,15 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
                               (hc[A_BIGCLUSTER] ? " (exact cluster larger than supported)" : ""));
 }
 
-PGPUDTALPHA alpha_copy_to_host(AlphaDev &d, hipStream_t st)
+PGPUDTALPHA alpha_copy_to_host(AlphaDev &d, HostStage &stage, hipStream_t st)
 {
     PGPUDTALPHA A = (PGPUDTALPHA)calloc(1, sizeof(GPUDTALPHA));
     A->nPoints = d.n; A->nEdges = d.E; A->nTris = d.F; A->specSize = d.specSize;
     A->masterSize = d.M; A->nSpecPointers = d.nsp;
     auto cp = [&](void **dst, int idx, size_t bytes) {
         *dst = malloc(bytes > 0 ? bytes : 1);
-        if (bytes) chk(hipMemcpyAsync(*dst, d.b[idx].p, bytes, hipMemcpyDeviceToHost, st), "copy");
+        if (!*dst) throw GalphaError("host allocation failed");
+        if (bytes) stage_d2h(stage, *dst, d.b[idx].p, bytes, st);
     };
     cp((void **)&A->spectrum, AB_SPEC, 8 * (size_t)d.specSize);
     cp((void **)&A->tri_ro_rank, AB_TRANK, 4 * (size_t)d.F);
@@ -1160,14 +1190,13 @@ PGPUDTALPHA alpha_copy_to_host(AlphaDev &d, hipStream_t st)
     cp((void **)&A->master_st, AB_OVAL, 4 * (size_t)d.M);
     cp((void **)&A->master_at, AB_MKEY, 4 * (size_t)d.M);
     cp((void **)&A->spec_pointers, AB_SP, 4 * (size_t)d.nsp);
-    chk(hipStreamSynchronize(st), "sync");
     return A;
 }
 
-PGPUDTALPHA alpha_copy_set_to_host(AlphaDev &d, int set, hipStream_t st)
+PGPUDTALPHA alpha_copy_set_to_host(AlphaDev &d, int set, HostStage &stage, hipStream_t st)
 {
     if (set < 0 || set >= d.nsets) throw GalphaError("alpha: set index out of range");
-    if (d.nsets == 1) return alpha_copy_to_host(d, st);
+    if (d.nsets == 1) return alpha_copy_to_host(d, stage, st);
     int p0 = d.poff[set], np = d.poff[set + 1] - p0;
     int t0 = d.tfirst[set], nt = d.tfirst[set + 1] - t0;
     int e0 = d.efirst[set], ne = d.efirst[set + 1] - e0;
@@ -1180,9 +1209,7 @@ PGPUDTALPHA alpha_copy_set_to_host(AlphaDev &d, int set, hipStream_t st)
     auto cp = [&](void **dst, int idx, size_t elem, size_t first, size_t count) {
         *dst = malloc(count * elem > 0 ? count * elem : 1);
         if (!*dst) throw GalphaError("host allocation failed");
-        if (count)
-            chk(hipMemcpyAsync(*dst, (char *)d.b[idx].p + first * elem, count * elem, hipMemcpyDeviceToHost, st),
-                "copy");
+        if (count) stage_d2h(stage, *dst, (char *)d.b[idx].p + first * elem, count * elem, st);
     };
     cp((void **)&A->spectrum, AB_SPEC, 8, s0, ns);
     cp((void **)&A->tri_ro_rank, AB_TRANK, 4, t0, nt);
@@ -1199,7 +1226,6 @@ PGPUDTALPHA alpha_copy_set_to_host(AlphaDev &d, int set, hipStream_t st)
     // spec pointers of rank r >= 1 of this set: distinct key number specstart - set + r - 1
     int R = ns - 1;
     cp((void **)&A->spec_pointers, AB_SP, 4, R > 0 ? (size_t)(1 + s0 - set) - 1 : 0, R > 0 ? (size_t)R + 1 : 1);
-    chk(hipStreamSynchronize(st), "sync");
     A->spec_pointers[0] = -1;
     for (int r = 1; r <= R; r++) A->spec_pointers[r] -= m0;
     for (int k = 0; k < ne; k++) A->edge_t[k] -= t0;

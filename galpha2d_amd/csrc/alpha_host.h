@@ -14,6 +14,21 @@ struct AlphaBuf {
     size_t bytes = 0;
 };
 
+// Pinned host staging for host <-> device copies, owned by a context.  Copies run in chunks
+// through two halves used alternately, so the DMA of one chunk overlaps the host copy of the
+// other.  (Copies from pageable memory go through the runtime's shared staging buffer and
+// serialise the copies of all threads, e.g. of concurrent serving slots.)
+struct HostStage {
+    char *p = nullptr;           // 2 * half bytes of pinned memory, allocated on first use
+    size_t half = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+// dst (host) <- src (device); returns when dst holds the data
+void stage_d2h(HostStage &s, void *dst, const void *src, size_t bytes, hipStream_t st);
+// dst (device) <- src (host); returns when src may be reused (the copies are ordered on st)
+void stage_h2d(HostStage &s, void *dst, const void *src, size_t bytes, hipStream_t st);
+void stage_free(HostStage &s);
+
 #include <vector>
 
 // sets of a batch as the alpha stage sees them (a single set: nsets = 1, vset = NULL)
@@ -36,9 +51,9 @@ void alpha_run(AlphaDev &a, const double *d_xy, int n, const int *d_tris9, int F
                galpha_stats *stats);
 void alpha_run_batch(AlphaDev &a, const double *d_xy, int n, const int *d_tris9, int F, const AlphaBatchInfo &info,
                      hipStream_t st, galpha_stats *stats);
-PGPUDTALPHA alpha_copy_to_host(AlphaDev &a, hipStream_t st);
+PGPUDTALPHA alpha_copy_to_host(AlphaDev &a, HostStage &stage, hipStream_t st);
 // one set of the last batch, with set-local point, triangle and edge indices
-PGPUDTALPHA alpha_copy_set_to_host(AlphaDev &a, int set, hipStream_t st);
+PGPUDTALPHA alpha_copy_set_to_host(AlphaDev &a, int set, HostStage &stage, hipStream_t st);
 // alpha of a set without triangles: spectrum [0], every vertex interval -1
 PGPUDTALPHA alpha_empty_host(int npoints);
 void alpha_free_host(PGPUDTALPHA p);

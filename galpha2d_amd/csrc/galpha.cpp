@@ -89,7 +89,63 @@ struct galpha_ctx {
     hipEvent_t ev[16];
     int op = 1;
     int *hsnap = nullptr;   // pinned staging for counter reads (SNAP_INTS ints)
+    HostStage stage;        // pinned staging for large host <-> device copies
 };
+
+// ---- pinned staging ------------------------------------------------------------------------
+static const size_t STAGE_HALF = (size_t)32 << 20;
+
+static void stage_ensure(HostStage &s)
+{
+    if (s.p) return;
+    HIPCHK(hipHostMalloc((void **)&s.p, 2 * STAGE_HALF, hipHostMallocDefault));
+    s.half = STAGE_HALF;
+    for (int k = 0; k < 2; k++) HIPCHK(hipEventCreateWithFlags(&s.ev[k], hipEventDisableTiming));
+}
+
+void stage_free(HostStage &s)
+{
+    if (s.p) hipHostFree(s.p);
+    for (int k = 0; k < 2; k++)
+        if (s.ev[k]) hipEventDestroy(s.ev[k]);
+    s = HostStage();
+}
+
+void stage_d2h(HostStage &s, void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    if (bytes == 0) return;
+    stage_ensure(s);
+    size_t nch = (bytes + s.half - 1) / s.half;
+    for (size_t i = 0; i <= nch; i++) {
+        if (i < nch) {   // DMA of chunk i into half i % 2 (its previous chunk was copied out below)
+            size_t off = i * s.half, len = std::min(s.half, bytes - off);
+            HIPCHK(hipMemcpyAsync(s.p + (i & 1) * s.half, (const char *)src + off, len, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipEventRecord(s.ev[i & 1], st));
+        }
+        if (i > 0) {     // host copy of chunk i - 1 while chunk i is in flight
+            size_t j = i - 1, off = j * s.half, len = std::min(s.half, bytes - off);
+            HIPCHK(hipEventSynchronize(s.ev[j & 1]));
+            memcpy((char *)dst + off, s.p + (j & 1) * s.half, len);
+        }
+    }
+}
+
+void stage_h2d(HostStage &s, void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    if (bytes == 0) return;
+    stage_ensure(s);
+    size_t nch = (bytes + s.half - 1) / s.half;
+    for (size_t i = 0; i < nch; i++) {
+        size_t off = i * s.half, len = std::min(s.half, bytes - off);
+        if (i >= 2) HIPCHK(hipEventSynchronize(s.ev[i & 1]));   // the DMA of chunk i - 2 left this half
+        memcpy(s.p + (i & 1) * s.half, (const char *)src + off, len);
+        HIPCHK(hipMemcpyAsync((char *)dst + off, s.p + (i & 1) * s.half, len, hipMemcpyHostToDevice, st));
+        HIPCHK(hipEventRecord(s.ev[i & 1], st));
+    }
+    // the staging halves are free again once their last DMAs ended
+    HIPCHK(hipEventSynchronize(s.ev[(nch - 1) & 1]));
+    if (nch >= 2) HIPCHK(hipEventSynchronize(s.ev[nch & 1]));
+}
 
 static const int SNAP_INTS = 256;
 
@@ -111,8 +167,7 @@ static int ctx_counter(galpha_ctx *c, int slot)
 // stream-ordered read of device memory (the caller's stream may be non-blocking)
 static void ctx_read(galpha_ctx *c, void *dst, const void *src, size_t bytes)
 {
-    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    stage_d2h(c->stage, dst, src, bytes, c->stream);
 }
 static int ctx_status(galpha_ctx *c) { return ctx_counter(c, C_STATUS); }
 static void ctx_counter_set(galpha_ctx *c, int slot, int v)
@@ -869,12 +924,18 @@ static int batch_run(galpha_ctx *c, const double *xy, const int *offsets, int ns
     if (device_input && na == nsets && nsets > 0) {
         dx = xy + 2 * (size_t)offsets[0];
     } else if (na == nsets && nsets > 0) {
-        HIPCHK(hipMemcpyAsync(c->xy.p, xy + 2 * (size_t)offsets[0], 16 * (size_t)total, kind, st));
+        if (device_input)
+            HIPCHK(hipMemcpyAsync(c->xy.p, xy + 2 * (size_t)offsets[0], 16 * (size_t)total, kind, st));
+        else
+            stage_h2d(c->stage, c->xy.p, xy + 2 * (size_t)offsets[0], 16 * (size_t)total, st);
         dx = c->xy.as<double>();
     } else {
-        for (int k = 0; k < na; k++)
-            HIPCHK(hipMemcpyAsync(c->xy.as<double>() + 2 * (size_t)poffs[k], xy + 2 * (size_t)offsets[c->batch_sets[k]],
-                                  16 * (size_t)ns[k], kind, st));
+        for (int k = 0; k < na; k++) {
+            double *dk = c->xy.as<double>() + 2 * (size_t)poffs[k];
+            const double *sk = xy + 2 * (size_t)offsets[c->batch_sets[k]];
+            if (device_input) HIPCHK(hipMemcpyAsync(dk, sk, 16 * (size_t)ns[k], kind, st));
+            else stage_h2d(c->stage, dk, sk, 16 * (size_t)ns[k], st);
+        }
         dx = c->xy.as<double>();
     }
     std::vector<double> bb5((size_t)5 * std::max(na, 1));
@@ -976,6 +1037,7 @@ void galphaDestroy(galpha_ctx *c)
     for (int i = 0; i < 16; i++) hipEventDestroy(c->ev[i]);
     if (c->stream) hipStreamDestroy(c->stream);
     if (c->hsnap) hipHostFree(c->hsnap);
+    stage_free(c->stage);
     delete c;
 }
 
@@ -1054,12 +1116,11 @@ int galphaComputeHost(galpha_ctx *c, const double *xy, int n, int fboSize, int *
         double bb[5] = {0, 0, 1, 1, 0};
         if (n > 0) bbox_host(xy, n, bb);
         c->xy.ensure(16 * (size_t)std::max(n, 1));
-        if (n > 0) HIPCHK(hipMemcpyAsync(c->xy.p, xy, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+        if (n > 0) stage_h2d(c->stage, c->xy.p, xy, 16 * (size_t)n, c->stream);
         int F = run_dt(c, c->xy.as<double>(), n, fboSize, bb);
         if (tris_out9) {
             if (*ntris_out < F) throw GalphaError("output capacity too small");
-            if (F > 0) HIPCHK(hipMemcpyAsync(tris_out9, c->out9.p, 36 * (size_t)F, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
+            if (F > 0) stage_d2h(c->stage, tris_out9, c->out9.p, 36 * (size_t)F, c->stream);
         }
         *ntris_out = F;
         return F;
@@ -1082,16 +1143,15 @@ int galphaVoronoi(galpha_ctx *c, const double *xy, int n, int T, int *labels, in
         if (pix_capacity < n + nf) throw GalphaError("pix capacity too small");
         ensure_pba(c, plan);
         c->xy.ensure(16 * (size_t)n);
-        HIPCHK(hipMemcpyAsync(c->xy.p, xy, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+        stage_h2d(c->stage, c->xy.p, xy, 16 * (size_t)n, c->stream);
         HIPCHK(hipMemsetAsync(c->counters.p, 0, COUNTER_BYTES, c->stream));
         BatchDev bd = upload_plan(c, plan);
         int iters = 0, isl = 0;
         run_pba(c, c->xy.as<double>(), bd, &iters, &isl, nullptr);
         c->stats.pbaIterations = iters;
         c->stats.islandSites = isl;
-        HIPCHK(hipMemcpyAsync(labels, c->vor.p, 4 * (size_t)T * T, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(pix, c->pix.p, 4 * (size_t)(n + nf), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        stage_d2h(c->stage, labels, c->vor.p, 4 * (size_t)T * T, c->stream);
+        stage_d2h(c->stage, pix, c->pix.p, 4 * (size_t)(n + nf), c->stream);
         return nf;
     } catch (const std::exception &e) {
         return fail(c, e);
@@ -1103,7 +1163,7 @@ int galphaCopyAlpha(galpha_ctx *c, PGPUDTALPHA *pA)
     if (!c || !pA) return -1;
     try {
         if (!c->last_alpha) throw GalphaError("no alpha result on this context");
-        *pA = alpha_copy_to_host(c->alpha, c->stream);
+        *pA = alpha_copy_to_host(c->alpha, c->stage, c->stream);
         return 0;
     } catch (const std::exception &e) {
         return fail(c, e);
@@ -1162,12 +1222,11 @@ int gpudtComputeAlpha(PGPUDTPARAMS p, PGPUDTOUTPUT dt, PGPUDTALPHA *pA)
         std::lock_guard<std::mutex> lk(g_default_mu);
         int n = p->nPoints;
         c->xy.ensure(16 * (size_t)std::max(n, 1));
-        if (n > 0) HIPCHK(hipMemcpyAsync(c->xy.p, p->points, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+        if (n > 0) stage_h2d(c->stage, c->xy.p, p->points, 16 * (size_t)n, c->stream);
         c->out9.ensure(36 * (size_t)std::max(dt->nTris, 1));
-        if (dt->nTris > 0)
-            HIPCHK(hipMemcpyAsync(c->out9.p, dt->triangles, 36 * (size_t)dt->nTris, hipMemcpyHostToDevice, c->stream));
+        if (dt->nTris > 0) stage_h2d(c->stage, c->out9.p, dt->triangles, 36 * (size_t)dt->nTris, c->stream);
         alpha_run(c->alpha, c->xy.as<double>(), n, c->out9.as<int>(), dt->nTris, c->stream, &c->stats);
-        *pA = alpha_copy_to_host(c->alpha, c->stream);
+        *pA = alpha_copy_to_host(c->alpha, c->stage, c->stream);
         return 0;
     } catch (const std::exception &e) {
         g_last_error = e.what();
@@ -1219,6 +1278,24 @@ int galphaBatchDevice(galpha_ctx *c, const double *d_xy, const int *offsets, int
         return fail(c, e);
     }
 }
+
+}  // extern "C"
+
+// One batched pass over host points that leaves every result on the device, for callers that
+// then copy set by set (galphaBatchTriangles, galphaBatchCopyAlpha): the serving loop.
+int galpha_ctx_batch_host(galpha_ctx *c, const double *xy, const int *offsets, int nsets, const int *fboSizes,
+                          int with_alpha)
+{
+    if (!c || !offsets || nsets < 0) return -1;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        return batch_run(c, xy, offsets, nsets, fboSizes, with_alpha);
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+extern "C" {
 
 int galphaBatchTriangles(galpha_ctx *c, int set, int *tris9, int capacity)
 {
@@ -1346,7 +1423,7 @@ int galphaBatchCopyAlpha(galpha_ctx *c, int set, PGPUDTALPHA *pA)
             *pA = alpha_empty_host(c->batch_npts.size() > (size_t)set ? c->batch_npts[set] : 0);
             return 0;
         }
-        *pA = alpha_copy_set_to_host(c->alpha, k, c->stream);
+        *pA = alpha_copy_set_to_host(c->alpha, k, c->stage, c->stream);
         return 0;
     } catch (const std::exception &e) {
         return fail(c, e);

@@ -30,6 +30,8 @@
 
 size_t galpha_ctx_device_bytes(galpha_ctx *c);
 void galpha_ctx_trim(galpha_ctx *c);
+int galpha_ctx_batch_host(galpha_ctx *c, const double *xy, const int *offsets, int nsets, const int *fboSizes,
+                          int with_alpha);
 
 namespace {
 
@@ -139,23 +141,27 @@ static void worker_main(galpha_server *S, int w)
             std::copy(grp[k]->xy.begin(), grp[k]->xy.end(), xy.begin() + 2 * (size_t)offs[k]);
         int with_alpha = 0;
         for (auto &r : grp) with_alpha |= r->with_alpha;
-        int *tris = nullptr, *toff = nullptr;
-        int rc = galphaBatch(ctx, xy.data(), offs.data(), (int)grp.size(), Ts.data(), with_alpha, &tris, &toff);
+        // results stay on the device; each request's triangles and alpha arrays are then copied
+        // straight into its ticket through the context's pinned staging
+        int rc = galpha_ctx_batch_host(ctx, xy.data(), offs.data(), (int)grp.size(), Ts.data(), with_alpha);
         std::string err = rc < 0 ? std::string(galphaLastError(ctx)) : std::string();
         for (size_t k = 0; k < grp.size(); k++) {
             auto &r = grp[k];
             r->started = t0;
             r->group = (int)grp.size();
             if (rc < 0) { r->status = -1; r->err = err; continue; }
-            r->tris9.assign(tris + 9 * (size_t)toff[k], tris + 9 * (size_t)toff[k + 1]);
-            r->status = toff[k + 1] - toff[k];
+            int nt = galphaBatchTriangles(ctx, (int)k, nullptr, 0);
+            if (nt >= 0) {
+                r->tris9.resize(9 * (size_t)nt);
+                nt = galphaBatchTriangles(ctx, (int)k, r->tris9.data(), nt);
+            }
+            if (nt < 0) { r->status = -1; r->err = galphaLastError(ctx); continue; }
+            r->status = nt;
             if (r->with_alpha && galphaBatchCopyAlpha(ctx, (int)k, &r->alpha) < 0) {
                 r->status = -1;
                 r->err = galphaLastError(ctx);
             }
         }
-        galphaFree(tris);
-        galphaFree(toff);
         size_t held_now = galpha_ctx_device_bytes(ctx);
         {
             std::lock_guard<std::mutex> lk(S->mu);
