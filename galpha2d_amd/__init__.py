@@ -57,7 +57,7 @@ EXPORTS = ["gpudtComputeDT", "gpudtReleaseDTOutput", "gpudtGetLastError", "gpudt
            "galphaTextureFor", "galphaAutotune", "galphaBatch", "galphaBatchDevice", "galphaBatchTriangles", "galphaBatchCopyAlpha", "galphaFree", "galphaAutoTexture", "galphaKernelProbe",
            "galphaKernelProbeResult", "galphaServerCreate", "galphaServerDestroy", "galphaServerSubmit",
            "galphaServerWait", "galphaServerTriangles", "galphaServerAlpha", "galphaServerRelease",
-           "galphaServerStats"]
+           "galphaServerStats", "galphaAlphaRank", "galphaClassify", "galphaClassifyDevice"]
 
 
 def lib():
@@ -87,6 +87,12 @@ def lib():
     L.galphaVoronoi.restype = C.c_int
     L.galphaCopyAlpha.argtypes = [vp, C.POINTER(C.POINTER(_Alpha))]
     L.galphaCopyAlpha.restype = C.c_int
+    L.galphaAlphaRank.argtypes = [vp, C.c_double, ip]
+    L.galphaAlphaRank.restype = C.c_int
+    L.galphaClassify.argtypes = [vp, C.c_int, vp, vp, vp]
+    L.galphaClassify.restype = C.c_int
+    L.galphaClassifyDevice.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+    L.galphaClassifyDevice.restype = C.c_int
     L.gpudtReleaseAlpha.argtypes = [C.POINTER(_Alpha)]
     L.galphaKernelProbe.argtypes = [C.c_char_p]
     L.galphaKernelProbeResult.argtypes = [ip, C.POINTER(C.c_double)]
@@ -296,6 +302,28 @@ class Context:
             return _alpha_to_numpy(A)
         finally:
             self._L.gpudtReleaseAlpha(A)
+
+    # alpha-complex classes (galphaClassify): 0 exterior, 1 singular, 2 regular, 3 interior
+    EXTERIOR, SINGULAR, REGULAR, INTERIOR = 0, 1, 2, 3
+
+    def alpha_rank(self, alpha):
+        """Largest rank r with spectrum[r] <= alpha in the last single-set alpha result."""
+        r = C.c_int()
+        if self._L.galphaAlphaRank(self._h, float(alpha), C.byref(r)) < 0:
+            self._err("galphaAlphaRank")
+        return r.value
+
+    def classify(self, rank, alpha):
+        """Classes of every vertex, edge (alpha-result edge order) and triangle at `rank`:
+        (vert, edge, tri) uint8 arrays; `alpha` is this context's alpha() result (array sizes)."""
+        import numpy as np
+        v = np.zeros(len(alpha["vert_nu_min"]), np.uint8)
+        e = np.zeros(len(alpha["edge_t"]), np.uint8)
+        t = np.zeros(len(alpha["tri_ro_rank"]), np.uint8)
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p) if a.size else None
+        if self._L.galphaClassify(self._h, int(rank), ptr(v), ptr(e), ptr(t)) < 0:
+            self._err("galphaClassify")
+        return v, e, t
 
     def kernel_probe(self, kernel_name, fn):
         """Run fn() with HIP events around every launch of `kernel_name` (on its stream).

@@ -790,7 +790,7 @@ enum {
     AB_ID2, AB_LO, AB_HI, AB_SLO, AB_SHI, AB_FLAG, AB_RANK, AB_SPECBITS, AB_VMIN, AB_VMAX, AB_STATUS, AB_CNTRS,
     AB_TMP, AB_SCAN, AB_NEG, AB_PMAX, AB_NSMIN, AB_MKEY, AB_MVAL, AB_MFLAG, AB_MPOS, AB_OKEY, AB_OVAL,
     AB_SKEY, AB_SVAL, AB_SPEC, AB_SP, AB_MISC, AB_BIG0, AB_BIG1, AB_BIG2, AB_BIG3, AB_BIGL,
-    AB_BS0, AB_BS1, AB_BS2, AB_BS3, AB_BS4, AB_BSET, AB_MSET, AB_COUNT
+    AB_BS0, AB_BS1, AB_BS2, AB_BS3, AB_BS4, AB_BSET, AB_MSET, AB_VHULL, AB_RANKQ, AB_COUNT
 };
 
 static void ab(AlphaDev &d, int i, size_t bytes)
@@ -891,6 +891,7 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
                      hipStream_t st, galpha_stats *stats)
 {
     d.n = n; d.F = F; d.E = 0; d.S = 0; d.specSize = 1; d.M = 0; d.nsp = 1;
+    d.tris9 = d_tris9;
     const int nsets = info.nsets;
     d.nsets = nsets;
     d.poff = info.poff;
@@ -1260,4 +1261,86 @@ void alpha_free_host(PGPUDTALPHA A)
     free(A->edge_nu_min_rank); free(A->edge_nu_max_rank); free(A->vert_nu_min_rank); free(A->vert_nu_max_rank);
     free(A->master_simplex); free(A->master_st); free(A->master_at); free(A->spec_pointers);
     free(A);
+}
+
+// ---- alpha-complex classification at a rank ----------------------------------------------------
+// (no counterpart in the reference beyond setAlpha/getAlpha, Alpha_Shapes_2D.cu:739-747.)  Class
+// codes: 0 exterior, 1 singular, 2 regular, 3 interior, from the interval ranks:
+//   triangle  interior once r >= rho;
+//   edge      enters at rho (unattached) or nu_min (attached); singular until nu_min; regular
+//             until nu_max, for ever on the hull (nu_max = 0); then interior;
+//   vertex    not in the triangulation (nu_min = -1): exterior; singular while r < nu_min;
+//             regular until nu_max, for ever on the hull; then interior.
+__global__ void k_classify_edges(const int *erank, const int *enmin, const int *enmax, const int *et, const int *ev,
+                                 const int *tris9, int E, int r, unsigned char *ecls, unsigned char *vhull)
+{
+    GS(e, E) {
+        int ro = erank[e], mn = enmin[e], mx = enmax[e];
+        int enter = ro ? ro : mn;
+        unsigned char c = r < enter ? 0 : ((ro && r < mn) ? 1 : ((mx == 0 || r < mx) ? 2 : 3));
+        if (ecls) ecls[e] = c;
+        if (mx == 0) {   // hull edge: both vertices lie on the hull
+            int t = et[e], i = ev[e];
+            vhull[tris9[(size_t)t * 9 + 3 + pl1(i)]] = 1;
+            vhull[tris9[(size_t)t * 9 + 3 + mi1(i)]] = 1;
+        }
+    }
+}
+
+__global__ void k_classify_tris(const int *trank, int F, int r, unsigned char *tcls)
+{
+    GS(t, F) tcls[t] = r >= trank[t] ? 3 : 0;
+}
+
+__global__ void k_classify_verts(const int *vmin, const int *vmax, const unsigned char *vhull, int n, int r,
+                                 unsigned char *vcls)
+{
+    GS(v, n) {
+        int mn = vmin[v], mx = vmax[v];
+        vcls[v] = mn < 0 ? 0 : (r < mn ? 1 : ((vhull[v] || r < mx) ? 2 : 3));
+    }
+}
+
+// largest r with spectrum[r] <= alpha (-1 when there is none).  The FP64 spectrum need not be
+// monotone (a rank's value is the smallest FP64 value of its exact group, and neighbouring exact
+// values can round in the other order), so every rank is tested: one max per wave.
+__global__ void k_alpha_rank(const double *spec, int nspec, double alpha, int *out)
+{
+    int best = -1;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < ((nspec + 63) & ~63); r += gridDim.x * blockDim.x)
+        if (r < nspec && spec[r] <= alpha) best = r;
+    for (int d = 32; d > 0; d >>= 1) best = max(best, __shfl_xor(best, d, 64));
+    if ((threadIdx.x & 63) == 0 && best >= 0) atomicMax(out, best);
+}
+
+int alpha_rank_of(AlphaDev &d, double alpha, HostStage &stage, hipStream_t st)
+{
+    if (d.nsets != 1) throw GalphaError("alpha: rank lookup needs a single-set result");
+    ab(d, AB_RANKQ, 16);
+    launch_fill_i32(AP<int>(d, AB_RANKQ), -1, 1, st);
+    int g = std::max(1, std::min(1024, (d.specSize + 255) / 256));
+    GA_LAUNCH(k_alpha_rank, dim3(g), dim3(256), 0, st, (const double *)d.b[AB_SPEC].p, d.specSize, alpha,
+              AP<int>(d, AB_RANKQ));
+    int r = -1;
+    stage_d2h(stage, &r, d.b[AB_RANKQ].p, 4, st);
+    return r;
+}
+
+void alpha_classify(AlphaDev &d, int r, unsigned char *vcls, unsigned char *ecls, unsigned char *tcls,
+                    hipStream_t st)
+{
+    if (d.nsets != 1) throw GalphaError("alpha: classification needs a single-set result");
+    if (r < 0 || r >= d.specSize) throw GalphaError("alpha: rank out of range");
+    if (!d.tris9 && d.E > 0) throw GalphaError("alpha: no triangulation");
+    ab(d, AB_VHULL, (size_t)d.n + 16);
+    unsigned char *vhull = AP<unsigned char>(d, AB_VHULL);
+    chk(hipMemsetAsync(vhull, 0, (size_t)d.n + 16, st), "memset");
+    if (d.E > 0)
+        GA_LAUNCH(k_classify_edges, dim3(AGRID), dim3(256), 0, st, AP<const int>(d, AB_ERANK), AP<const int>(d, AB_ENMIN),
+                  AP<const int>(d, AB_ENMAX), AP<const int>(d, AB_ET), AP<const int>(d, AB_EV), d.tris9, d.E, r, ecls,
+                  vhull);
+    if (d.F > 0 && tcls) GA_LAUNCH(k_classify_tris, dim3(AGRID), dim3(256), 0, st, AP<const int>(d, AB_TRANK), d.F, r, tcls);
+    if (d.n > 0 && vcls)
+        GA_LAUNCH(k_classify_verts, dim3(AGRID), dim3(256), 0, st, AP<const int>(d, AB_VMIN), AP<const int>(d, AB_VMAX),
+                  (const unsigned char *)vhull, d.n, r, vcls);
 }

@@ -45,6 +45,7 @@ struct AlphaDev {
     // per set (host): first point, triangle, edge, spectrum slot and master entry; [nsets + 1]
     int nsets = 1;
     std::vector<int> poff, tfirst, efirst, specstart, mfirst;
+    const int *tris9 = nullptr;   // device triangles of the last run (the context's output)
 };
 
 void alpha_run(AlphaDev &a, const double *d_xy, int n, const int *d_tris9, int F, hipStream_t st,
@@ -57,4 +58,10 @@ PGPUDTALPHA alpha_copy_set_to_host(AlphaDev &a, int set, HostStage &stage, hipSt
 // alpha of a set without triangles: spectrum [0], every vertex interval -1
 PGPUDTALPHA alpha_empty_host(int npoints);
 void alpha_free_host(PGPUDTALPHA p);
+// alpha-complex query of a single-set result: rank of an alpha value, and the class
+// (0 exterior, 1 singular, 2 regular, 3 interior) of every vertex, edge and triangle at a rank;
+// the class arrays are device pointers (NULL: skipped), written on st
+int alpha_rank_of(AlphaDev &a, double alpha, HostStage &stage, hipStream_t st);
+void alpha_classify(AlphaDev &a, int rank, unsigned char *vcls, unsigned char *ecls, unsigned char *tcls,
+                    hipStream_t st);
 void alpha_release(AlphaDev &a);

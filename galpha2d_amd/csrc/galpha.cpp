@@ -70,6 +70,7 @@ struct galpha_ctx {
     // mesh
     DevBuf plo;   // phase-2 band range starts
     DevBuf perm, xys, pcell, ccount, ccursor;   // spatial numbering of the real points
+    DevBuf cls;                                  // alpha-complex classes (host queries)
     DevBuf tv, tn, remap, stamp, claim, modA, modB, cand, pos, vt, vstate, pix, deg, off, fill, fan,
         moved, decide, mx, mlist, loc, lt, scratch, hnext, hedge, keys, keys2, vals, vals2, rot, newid,
         out9, counters, status, xy, scanTmp, sortTmp, tileCnt, tileOff, dellist, dbg;
@@ -991,7 +992,7 @@ static void release_all(galpha_ctx *c)
                       &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
                       &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd, &c->plo,
-                      &c->perm, &c->xys, &c->pcell, &c->ccount, &c->ccursor};
+                      &c->perm, &c->xys, &c->pcell, &c->ccount, &c->ccursor, &c->cls};
     for (DevBuf *b : bufs) b->release();
     alpha_release(c->alpha);
 }
@@ -1009,7 +1010,7 @@ size_t galpha_ctx_device_bytes(galpha_ctx *c)
                       &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
                       &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd, &c->plo,
-                      &c->perm, &c->xys, &c->pcell, &c->ccount, &c->ccursor};
+                      &c->perm, &c->xys, &c->pcell, &c->ccount, &c->ccursor, &c->cls};
     size_t t = 0;
     for (DevBuf *b : bufs) t += b->bytes;
     for (auto &b : c->alpha.b) t += b.bytes;
@@ -1164,6 +1165,54 @@ int galphaCopyAlpha(galpha_ctx *c, PGPUDTALPHA *pA)
     try {
         if (!c->last_alpha) throw GalphaError("no alpha result on this context");
         *pA = alpha_copy_to_host(c->alpha, c->stage, c->stream);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+// ---- alpha-complex queries on the last single-set alpha result ---------------------------------
+int galphaAlphaRank(galpha_ctx *c, double alpha, int *rank)
+{
+    if (!c || !rank) return -1;
+    try {
+        if (!c->last_alpha) throw GalphaError("no alpha result on this context");
+        HIPCHK(hipSetDevice(c->device));
+        *rank = alpha_rank_of(c->alpha, alpha, c->stage, c->stream);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int galphaClassifyDevice(galpha_ctx *c, int rank, unsigned char *d_vert, unsigned char *d_edge, unsigned char *d_tri,
+                         void *stream)
+{
+    if (!c) return -1;
+    try {
+        if (!c->last_alpha) throw GalphaError("no alpha result on this context");
+        HIPCHK(hipSetDevice(c->device));
+        alpha_classify(c->alpha, rank, d_vert, d_edge, d_tri, stream ? (hipStream_t)stream : c->stream);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
+}
+
+int galphaClassify(galpha_ctx *c, int rank, unsigned char *vert, unsigned char *edge, unsigned char *tri)
+{
+    if (!c) return -1;
+    try {
+        if (!c->last_alpha) throw GalphaError("no alpha result on this context");
+        HIPCHK(hipSetDevice(c->device));
+        size_t n = (size_t)c->alpha.n, E = (size_t)c->alpha.E, F = (size_t)c->alpha.F;
+        c->cls.ensure(n + E + F + 64);
+        unsigned char *dv = c->cls.as<unsigned char>(), *de = dv + n, *dt = de + E;
+        alpha_classify(c->alpha, rank, vert ? dv : nullptr, edge ? de : nullptr, tri ? dt : nullptr, c->stream);
+        if (vert) stage_d2h(c->stage, vert, dv, n, c->stream);
+        if (edge) stage_d2h(c->stage, edge, de, E, c->stream);
+        if (tri) stage_d2h(c->stage, tri, dt, F, c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
         return 0;
     } catch (const std::exception &e) {
         return fail(c, e);

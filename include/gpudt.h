@@ -142,6 +142,19 @@ int galphaVoronoi(galpha_ctx *ctx, const double *xy, int n, int fboSize, int *la
  * allocated by the library (release with gpudtReleaseAlpha). */
 int galphaCopyAlpha(galpha_ctx *ctx, PGPUDTALPHA *pAlpha);
 
+/* Alpha-complex queries on the last single-set alpha result (the reference keeps an alpha value
+ * with setAlpha/getAlpha, AlphaShape/Alpha_Shapes_2D.cu:739-747, and leaves the query to callers
+ * walking its master list).  galphaAlphaRank: *rank = the largest r with spectrum[r] <= alpha
+ * (-1 when none; the FP64 spectrum values need not be monotone).  galphaClassify: the class of every vertex (nPoints), edge (nEdges, in
+ * the edge order of the alpha result) and triangle (nTris) at rank r: 0 exterior, 1 singular,
+ * 2 regular, 3 interior; a NULL array is skipped.  galphaClassifyDevice writes device arrays on
+ * stream (NULL = the context's stream). */
+int galphaAlphaRank(galpha_ctx *ctx, double alpha, int *rank);
+int galphaClassify(galpha_ctx *ctx, int rank, unsigned char *vert_class, unsigned char *edge_class,
+                   unsigned char *tri_class);
+int galphaClassifyDevice(galpha_ctx *ctx, int rank, unsigned char *d_vert_class, unsigned char *d_edge_class,
+                         unsigned char *d_tri_class, void *stream);
+
 /* ---- batches and serving (no reference counterpart) ----------------------------------- */
 /* Batched: nsets independent point sets, concatenated in xy (offsets[nsets+1] in points),
  * texture side per set (NULL or 0 = autotuned).  All sets go through one pass of the pipeline

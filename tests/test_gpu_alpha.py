@@ -71,3 +71,71 @@ def test_alpha_lattice_clusters(oracle):
     xy = lattice_clusters(30000, 30000, 200, 8, 0.005, 3)
     st = _check(oracle, xy, 2048)
     assert st["alphaExactCompares"] > 0
+
+
+def _classes_by_definition(tris9, a, r):
+    """Alpha-complex classes at rank r from the definition: a triangle is in when its rank <= r;
+    an edge is in when an incident triangle is, or when it is unattached with rank <= r; an edge
+    in the complex is singular with no incident triangle in, interior with both, else regular;
+    a vertex of the triangulation is singular with no incident edge in, interior when every
+    incident triangle is in and it is not on the hull, else regular; other points exterior."""
+    tri_in = a["tri_ro_rank"] <= r
+    et, ev, ero = a["edge_t"], a["edge_vopp"], a["edge_ro_rank"]
+    nb = tris9[et, ev]
+    has2 = nb >= 0
+    k = tri_in[et].astype(np.int64) + np.where(has2, tri_in[np.maximum(nb, 0) >> 2], False)
+    ein = (k >= 1) | ((ero != 0) & (ero <= r))
+    ecls = np.where(~ein, 0, np.where(k == 0, 1, np.where(has2 & (k == 2), 3, 2)))
+    n = len(a["vert_nu_min"])
+    u = tris9[et, 3 + (ev + 1) % 3]
+    w = tris9[et, 3 + (ev + 2) % 3]
+    in_dt = np.zeros(n, bool)
+    in_dt[tris9[:, 3:6].ravel()] = True
+    any_edge = np.zeros(n, bool)
+    any_edge[u[ein]] = True
+    any_edge[w[ein]] = True
+    hull = np.zeros(n, bool)
+    hull[u[~has2]] = True
+    hull[w[~has2]] = True
+    all_tri = np.ones(n, bool)
+    all_tri[tris9[~tri_in, 3:6].ravel()] = False
+    vcls = np.where(~in_dt, 0, np.where(~any_edge, 1, np.where(all_tri & ~hull, 3, 2)))
+    tcls = np.where(tri_in, 3, 0)
+    return vcls.astype(np.uint8), ecls.astype(np.uint8), tcls.astype(np.uint8)
+
+
+@pytest.mark.parametrize("case", ["uniform", "lattice", "bird"])
+def test_alpha_classify(oracle, case):
+    """Alpha-complex classes on the device (from the interval ranks) equal the classes derived
+    from the definition (triangle ranks, edge ranks, adjacency), at ranks across the spectrum;
+    the rank lookup returns the last rank of each spectrum value."""
+    import torch
+    import galpha2d_amd as G
+    print("case: alpha classification", case, flush=True)
+    if case == "uniform":
+        xy, T = oracle.gen_uniform(9, 1500), 512
+    elif case == "lattice":
+        xy, T = np.array([(i / 20.0, j / 20.0) for i in range(20) for j in range(20)]), 512
+    else:
+        xy, T = load_fixture("bird.txt"), 256
+    ctx = G.Context(0)
+    d = torch.from_numpy(np.ascontiguousarray(xy)).to("cuda:0")
+    torch.cuda.synchronize()
+    ctx.compute_device(d.data_ptr(), xy.shape[0], T, with_alpha=True,
+                       stream=torch.cuda.current_stream().cuda_stream)
+    tris, a = ctx.triangles(), ctx.alpha()
+    ref, _ = oracle.delaunay(xy)
+    assert np.array_equal(tris, ref)
+    spec = np.asarray(a["spectrum"])
+    R = len(spec) - 1
+    for r in sorted({0, 1, R // 7, R // 3, R // 2, R - 1, R}):
+        got = ctx.classify(r, a)
+        want = _classes_by_definition(tris, a, r)
+        for g, w_, name in zip(got, want, ("vertex", "edge", "triangle")):
+            assert np.array_equal(g, w_), (case, r, name)
+    for r in sorted({1, R // 2, R}):
+        # the FP64 spectrum need not be monotone (near-equal exact values can round either way)
+        last = int(np.nonzero(spec <= spec[r])[0].max())
+        assert ctx.alpha_rank(spec[r]) == last
+    assert ctx.alpha_rank(-1.0) == -1
+    ctx.close()

@@ -118,3 +118,18 @@ def test_autotune_picks_candidates_and_keeps_results(oracle):
         T = ctx.texture_for(n)
         assert 256 <= T <= 16384 and (T & (T - 1)) == 0
     ctx.close()
+
+
+def test_batch_repeated_sets(oracle):
+    """The same set several times in one batch (repeated requests): every copy equals the oracle
+    (equal values in different sets must not be merged or compared across sets)."""
+    import galpha2d_amd as G
+    print("case: batch of 6 copies of one set plus a lattice", flush=True)
+    u = oracle.gen_uniform(77, 3000)
+    g = np.array([(i / 25.0, j / 25.0) for i in range(25) for j in range(25)])
+    sets = [u, u, g, u, u, g, u, u]
+    ctx = G.Context(0)
+    tris = ctx.batch(sets, with_alpha=True)
+    for k, xy in enumerate(sets):
+        _check_set(oracle, xy, tris[k], ctx.batch_alpha(k))
+    ctx.close()
