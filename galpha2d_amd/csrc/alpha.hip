@@ -1332,9 +1332,11 @@ void alpha_classify(AlphaDev &d, int r, unsigned char *vcls, unsigned char *ecls
     if (d.nsets != 1) throw GalphaError("alpha: classification needs a single-set result");
     if (r < 0 || r >= d.specSize) throw GalphaError("alpha: rank out of range");
     if (!d.tris9 && d.E > 0) throw GalphaError("alpha: no triangulation");
-    ab(d, AB_VHULL, (size_t)d.n + 16);
+    // cleared by a fill kernel, not a memset node, so a caller may capture this on a stream
+    size_t hull_words = ((size_t)d.n + 16 + 3) / 4;
+    ab(d, AB_VHULL, hull_words * 4);
     unsigned char *vhull = AP<unsigned char>(d, AB_VHULL);
-    chk(hipMemsetAsync(vhull, 0, (size_t)d.n + 16, st), "memset");
+    launch_fill_i32((int *)vhull, 0, hull_words, st);
     if (d.E > 0)
         GA_LAUNCH(k_classify_edges, dim3(AGRID), dim3(256), 0, st, AP<const int>(d, AB_ERANK), AP<const int>(d, AB_ENMIN),
                   AP<const int>(d, AB_ENMAX), AP<const int>(d, AB_ET), AP<const int>(d, AB_EV), d.tris9, d.E, r, ecls,
