@@ -8,6 +8,7 @@ shared library or a GPU is missing, every entry point raises.
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -34,7 +35,8 @@ class _Stats(C.Structure):
                 ("ms_missing", C.c_double), ("ms_boundary", C.c_double), ("ms_flip", C.c_double),
                 ("ms_canon", C.c_double), ("ms_alpha", C.c_double), ("ms_total", C.c_double),
                 ("frameSeen", C.c_longlong), ("frameChain", C.c_longlong), ("flipListed", C.c_longlong),
-                ("frameBulk", C.c_int), ("earRounds", C.c_int)]
+                ("frameBulk", C.c_int), ("earRounds", C.c_int),
+                ("constraintRounds", C.c_int), ("ms_constraints", C.c_double)]
 
 
 class _Alpha(C.Structure):
@@ -57,7 +59,21 @@ EXPORTS = ["gpudtComputeDT", "gpudtReleaseDTOutput", "gpudtGetLastError", "gpudt
            "galphaTextureFor", "galphaAutotune", "galphaBatch", "galphaBatchDevice", "galphaBatchTriangles", "galphaBatchCopyAlpha", "galphaFree", "galphaAutoTexture", "galphaKernelProbe",
            "galphaKernelProbeResult", "galphaServerCreate", "galphaServerDestroy", "galphaServerSubmit",
            "galphaServerWait", "galphaServerTriangles", "galphaServerAlpha", "galphaServerRelease",
-           "galphaServerStats", "galphaAlphaRank", "galphaClassify", "galphaClassifyDevice"]
+           "galphaServerStats", "galphaAlphaRank", "galphaClassify", "galphaClassifyDevice",
+           "galphaReadPoints", "galphaReleaseParams", "galphaWritePoints", "galphaDumpValues",
+           "galphaPackResult", "galphaUnpackResult", "galphaSetConstraints"]
+
+
+class _Params(C.Structure):
+    """GPUDTPARAMS (gpudt/gpudt.h:106-119)."""
+    _fields_ = [("minX", C.c_double), ("minY", C.c_double), ("maxX", C.c_double), ("maxY", C.c_double),
+                ("nPoints", C.c_int), ("points", C.POINTER(C.c_double)), ("nConstraints", C.c_int),
+                ("constraints", C.POINTER(C.c_int)), ("fboSize", C.c_int)]
+
+
+class _Output(C.Structure):
+    """GPUDTOUTPUT (gpudt/gpudt.h:125-129)."""
+    _fields_ = [("triangles", C.POINTER(C.c_int)), ("nTris", C.c_int)]
 
 
 def lib():
@@ -67,6 +83,11 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise GalphaError("libgalpha.so not built: run __graft_entry__.build() or make -C galpha2d_amd/csrc")
+    # PyTorch-ROCm wheels carry their own HIP runtime.  With both runtimes in one process, torch's
+    # has to reach the device first: once this library's runtime has opened it, torch finds none.
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_available():
+        torch.cuda.init()
     L = C.CDLL(LIB_PATH)
     vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
     L.galphaCreate.argtypes = [C.c_int]
@@ -126,6 +147,19 @@ def lib():
     L.galphaServerRelease.restype = C.c_int
     L.galphaServerStats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_size_t)]
     L.galphaServerStats.restype = C.c_int
+    L.galphaSetConstraints.argtypes = [vp, ip, C.c_int]
+    L.galphaSetConstraints.restype = C.c_int
+    L.galphaReadPoints.argtypes = [C.c_char_p, C.POINTER(C.POINTER(_Params))]
+    L.galphaReadPoints.restype = C.c_int
+    L.galphaReleaseParams.argtypes = [C.POINTER(_Params)]
+    L.galphaWritePoints.argtypes = [C.c_char_p, dp, C.c_int]
+    L.galphaWritePoints.restype = C.c_int
+    L.galphaDumpValues.argtypes = [C.c_char_p, C.POINTER(_Params), C.POINTER(_Output), C.POINTER(_Alpha), C.c_int]
+    L.galphaDumpValues.restype = C.c_int
+    L.galphaPackResult.argtypes = [C.c_int, ip, C.c_int, C.POINTER(_Alpha), vp, C.c_longlong]
+    L.galphaPackResult.restype = C.c_longlong
+    L.galphaUnpackResult.argtypes = [vp, C.c_longlong, ip, C.POINTER(ip), ip, C.POINTER(C.POINTER(_Alpha))]
+    L.galphaUnpackResult.restype = C.c_int
     _lib = L
     return L
 
@@ -164,6 +198,103 @@ def _alpha_to_numpy(A):
         spec_pointers=arr(a.spec_pointers, a.nSpecPointers, np.int32))
 
 
+_ALPHA_ARRAYS = (("spectrum", "spectrum", C.c_double), ("tri_ro_rank", "tri_ro_rank", C.c_int),
+                 ("edge_t", "edge_t", C.c_int), ("edge_vopp", "edge_vopp", C.c_int),
+                 ("edge_ro_rank", "edge_ro_rank", C.c_int), ("edge_nu_min_rank", "edge_nu_min", C.c_int),
+                 ("edge_nu_max_rank", "edge_nu_max", C.c_int), ("vert_nu_min_rank", "vert_nu_min", C.c_int),
+                 ("vert_nu_max_rank", "vert_nu_max", C.c_int), ("master_simplex", "master_simplex", C.c_int),
+                 ("master_st", "master_st", C.c_int), ("master_at", "master_at", C.c_int),
+                 ("spec_pointers", "spec_pointers", C.c_int))
+
+
+def _alpha_struct(alpha, n_points, n_tris):
+    """GPUDTALPHA viewing the arrays of an alpha dict; returns (struct, arrays kept alive)."""
+    A = _Alpha()
+    keep = []
+    for field, key, ct in _ALPHA_ARRAYS:
+        a = np.ascontiguousarray(alpha[key], dtype=np.float64 if ct is C.c_double else np.int32)
+        keep.append(a)
+        setattr(A, field, a.ctypes.data_as(C.POINTER(ct)))
+    A.nPoints, A.nTris = int(n_points), int(n_tris)
+    A.nEdges, A.specSize = len(alpha["edge_t"]), len(alpha["spectrum"])
+    A.masterSize, A.nSpecPointers = len(alpha["master_simplex"]), len(alpha["spec_pointers"])
+    return A, keep
+
+
+def read_points(path):
+    """Reference point file (main.cu:117-161) -> (points [n, 2] float64, params dict with the
+    bounds, nConstraints and the texture side the library picks)."""
+    L = lib()
+    P = C.POINTER(_Params)()
+    if L.galphaReadPoints(os.fsencode(path), C.byref(P)) < 0:
+        raise GalphaError(L.gpudtGetLastError().decode())
+    try:
+        p = P.contents
+        n = p.nPoints
+        xy = np.ctypeslib.as_array(p.points, shape=(n, 2)).copy() if n else np.zeros((0, 2))
+        info = dict(minX=p.minX, minY=p.minY, maxX=p.maxX, maxY=p.maxY, nConstraints=p.nConstraints,
+                    fboSize=p.fboSize)
+    finally:
+        L.galphaReleaseParams(P)
+    return xy, info
+
+
+def write_points(path, points):
+    """Write points [n, 2] in the reference point-file format (17 significant digits)."""
+    L = lib()
+    xy = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    if L.galphaWritePoints(os.fsencode(path), _dp(xy), xy.shape[0]) < 0:
+        raise GalphaError(L.gpudtGetLastError().decode())
+
+
+def dump_values(path, points, tris, alpha, full=False):
+    """Alpha_Shapes_2D::dumpValues (AlphaShape/Alpha_Shapes_2D.cu:749-793) of a result."""
+    L = lib()
+    xy = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    t9 = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 9)
+    A, keep = _alpha_struct(alpha, xy.shape[0], t9.shape[0])
+    prm = _Params(0, 0, 0, 0, xy.shape[0], _dp(xy), 0, None, 0)
+    out = _Output(_ip(t9), t9.shape[0])
+    if L.galphaDumpValues(os.fsencode(path), C.byref(prm), C.byref(out), C.byref(A), int(bool(full))) < 0:
+        raise GalphaError(L.gpudtGetLastError().decode())
+    del keep
+
+
+def pack_result(n_points, tris, alpha=None):
+    """Binary result payload (bytes) of a triangulation and, optionally, its alpha result."""
+    L = lib()
+    t9 = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 9)
+    A, keep = _alpha_struct(alpha, n_points, t9.shape[0]) if alpha is not None else (None, None)
+    pa = C.byref(A) if A is not None else None
+    size = L.galphaPackResult(int(n_points), _ip(t9), t9.shape[0], pa, None, 0)
+    if size < 0:
+        raise GalphaError(L.gpudtGetLastError().decode())
+    buf = C.create_string_buffer(size)
+    if L.galphaPackResult(int(n_points), _ip(t9), t9.shape[0], pa, buf, size) != size:
+        raise GalphaError(L.gpudtGetLastError().decode())
+    del keep
+    return buf.raw
+
+
+def unpack_result(payload):
+    """Inverse of pack_result: (n_points, triangles [F, 9], alpha dict or None)."""
+    L = lib()
+    b = bytes(payload)
+    n, F = C.c_int(), C.c_int()
+    t = C.POINTER(C.c_int)()
+    A = C.POINTER(_Alpha)()
+    if L.galphaUnpackResult(b, len(b), C.byref(n), C.byref(t), C.byref(F), C.byref(A)) < 0:
+        raise GalphaError(L.gpudtGetLastError().decode())
+    try:
+        tris = np.ctypeslib.as_array(t, shape=(F.value, 9)).copy() if F.value else np.zeros((0, 9), np.int32)
+        alpha = _alpha_to_numpy(A) if A else None
+    finally:
+        L.galphaFree(C.cast(t, C.c_void_p))
+        if A:
+            L.gpudtReleaseAlpha(A)
+    return n.value, tris, alpha
+
+
 class Context:
     """One device context (buffers + HIP stream) -- the analogue of the reference's global
     GPU-DT state (gpudt.cpp:64-77, Cuda/cudaMain.cu:69-92), made reentrant."""
@@ -187,6 +318,14 @@ class Context:
 
     def _err(self, what):
         raise GalphaError(f"{what}: {self._L.galphaLastError(self._h).decode()}")
+
+    def set_constraints(self, pairs):
+        """Constraint edges (pairs of point indices, [k, 2]) for the following single-set runs of
+        this context; None or an empty array clears them.  The runs then give the constrained DT."""
+        c = np.ascontiguousarray(np.zeros((0, 2)) if pairs is None else pairs, dtype=np.int32).reshape(-1, 2)
+        self._cons = c   # the library copies the pairs; kept for reference
+        if self._L.galphaSetConstraints(self._h, _ip(c) if c.size else None, c.shape[0]) < 0:
+            self._err("galphaSetConstraints")
 
     def delaunay(self, xy, fbo_size=None):
         """Canonical DT of xy ([n, 2] float64) as int32 [F, 9] (tri[3], vtx[3], tmp[3])."""
@@ -434,9 +573,15 @@ def default_context():
     return _default
 
 
-def gpudt_compute_dt(points, fbo_size):
-    """Mirror of gpudtComputeDT (gpudt/gpudt.h:131): points [n, 2] -> triangles [F, 9]."""
-    return default_context().delaunay(points, fbo_size)
+def gpudt_compute_dt(points, fbo_size, constraints=None):
+    """Mirror of gpudtComputeDT (gpudt/gpudt.h:131): points [n, 2] -> triangles [F, 9]; with
+    constraints (pairs of point indices, GPUDTPARAMS::constraints) the constrained DT."""
+    ctx = default_context()
+    ctx.set_constraints(constraints)
+    try:
+        return ctx.delaunay(points, fbo_size)
+    finally:
+        ctx.set_constraints(None)
 
 
 def compute_alpha_shapes(points, fbo_size):

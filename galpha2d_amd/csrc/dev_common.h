@@ -461,3 +461,4 @@ GA_INLINE uint32_t hash32(uint32_t x)
 #define GA_ERR_CAPACITY 16
 #define GA_ERR_ISLANDS 32
 #define GA_ERR_PATCH 64
+// GA_ERR_CONS_* (constraint insertion): kernels.h

@@ -76,6 +76,11 @@ struct galpha_ctx {
         out9, counters, status, xy, scanTmp, sortTmp, tileCnt, tileOff, dellist, dbg;
     // batch plan on the device
     DevBuf setsD, bmD, vset, hullCnt, hullOff, hullStart, setFirst, offsD, bboxTmp, lmark, ffan, modF, badset, bnd;
+    // constrained DT: input pairs, vertex-id pairs, inverse numbering, flags, walk counts, region
+    // sizes and offsets, cavity regions, constraint key set
+    DevBuf kin, kv, kinv, kdone, kcnt, krs, kroff, kwork, khash;
+    std::vector<int> cons;            // constraint pairs (input indices) for single-set runs
+    int cons_rounds = 0;              // insertion rounds of the last constrained run
     std::vector<int> set_first;       // [nsets + 1] first output triangle of each planned set
     std::vector<SetDesc> plan_sets;   // host copy of the last plan
     int plan_ntot = 0;
@@ -793,6 +798,57 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     HIPCHK(hipEventRecord(c->ev[7], st));
     if (last.status()) throw GalphaError("device pipeline error (status " + std::to_string(last.status()) + ")");
 
+    // ---- constraint edges (single-set runs): cavity insertion rounds, then constrained flips
+    c->cons_rounds = 0;
+    if (!c->cons.empty()) {
+        if (nsets != 1) throw GalphaError("constraints apply to single-set runs only");
+        int nc = (int)c->cons.size() / 2;
+        int hs = 64;
+        while (hs < 2 * nc + 64) hs *= 2;
+        c->kin.ensure(8 * (size_t)nc + 64);
+        c->kv.ensure(8 * (size_t)nc + 64);
+        c->kinv.ensure(4 * (size_t)n + 64);
+        c->kdone.ensure(4 * (size_t)nc + 64);
+        c->kcnt.ensure(4 * (size_t)nc + 64);
+        c->krs.ensure(4 * (size_t)nc + 64);
+        c->kroff.ensure(4 * (size_t)nc + 64);
+        c->khash.ensure(8 * (size_t)hs);
+        c->scanTmp.ensure(scan_temp_bytes((size_t)nc) + 256);
+        stage_h2d(c->stage, c->kin.p, c->cons.data(), 8 * (size_t)nc, st);
+        launch_fill_i32(c->khash.as<int>(), 0, 2 * (size_t)hs, st);
+        m.chash = c->khash.as<unsigned long long>();
+        m.chmask = hs - 1;
+        launch_cons_setup(m, c->kin.as<int2>(), nc, c->kinv.as<int>(), c->kv.as<int2>(), c->kdone.as<int>(),
+                          c->khash.as<unsigned long long>(), hs - 1, st);
+        for (int r = 0;; r++) {
+            if (r > nc + 16) throw GalphaError("constraint insertion did not finish");
+            launch_reset_counters(c->counters.as<int>(), C_CONS_PEND, C_CONS_MOD, -1, -1, st);
+            launch_cons_count(m, c->kv.as<int2>(), nc, c->kdone.as<int>(), c->kcnt.as<int>(), c->krs.as<int>(), st);
+            launch_exclusive_scan(c->krs.as<int>(), c->kroff.as<int>(), (size_t)nc, c->scanTmp.p, st);
+            int tail[2];
+            stage_d2h(c->stage, &tail[0], c->kroff.as<int>() + nc - 1, 4, st);
+            stage_d2h(c->stage, &tail[1], c->krs.as<int>() + nc - 1, 4, st);
+            last = ctx_snapshot(c);
+            if (int s = last.status()) {
+                const char *why = (s & GA_ERR_CONS_POINT) ? "a point lies inside a constraint segment"
+                                  : (s & GA_ERR_CONS_CROSS) ? "two constraints cross"
+                                  : (s & GA_ERR_CONS_DUP)   ? "a constraint endpoint is a duplicate point"
+                                                            : "constraint insertion failed";
+                throw GalphaError(std::string(why) + " (status " + std::to_string(s) + ")");
+            }
+            if (last.v[C_CONS_PEND] == 0) break;
+            long long words = (long long)tail[0] + tail[1];
+            if (words <= 0 || words > (1LL << 31) - 64) throw GalphaError("constraint work space out of range");
+            c->kwork.ensure(4 * (size_t)words + 64);
+            launch_cons_insert(m, c->kv.as<int2>(), nc, c->kdone.as<int>(), c->kcnt.as<int>(), c->kroff.as<int>(),
+                               c->kwork.as<int>(), m.modA, c->op++, st);
+            c->cons_rounds++;
+        }
+        flip_all();
+        if (last.status()) throw GalphaError("constrained flips failed (status " + std::to_string(last.status()) + ")");
+    }
+    HIPCHK(hipEventRecord(c->ev[9], st));
+
     // ---- canonical output
     int nslots = std::min(last.v[40], cap);
     c->keys.ensure(8 * (size_t)nslots + 64);
@@ -852,7 +908,9 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     S.ms_missing = ev_ms(c, 4, 5);
     S.ms_boundary = ev_ms(c, 5, 6);
     S.ms_flip = ev_ms(c, 6, 7);
-    S.ms_canon = ev_ms(c, 7, 8);
+    S.ms_constraints = ev_ms(c, 7, 9);
+    S.ms_canon = ev_ms(c, 9, 8);
+    S.constraintRounds = c->cons_rounds;
     S.ms_total = ev_ms(c, 0, 8);
     return F;
 }
@@ -861,6 +919,13 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
 static int run_dt(galpha_ctx *c, const double *d_xy, int n, int T, const double *bb5)
 {
     check_texture(T);
+    for (size_t k = 0; k < c->cons.size(); k += 2) {
+        int a = c->cons[k], b = c->cons[k + 1];
+        if (a < 0 || a >= n || b < 0 || b >= n)
+            throw GalphaError("constraint " + std::to_string(k / 2) + ": endpoint outside [0, " + std::to_string(n) + ")");
+        if (a == b) throw GalphaError("constraint " + std::to_string(k / 2) + ": both endpoints are point " + std::to_string(a));
+    }
+    if (!c->cons.empty() && n < 3) throw GalphaError("constraints need at least 3 points");
     c->last_n = n;
     c->last_T = T;
     if (n < 3) {
@@ -899,6 +964,7 @@ static int batch_run(galpha_ctx *c, const double *xy, const int *offsets, int ns
                      int with_alpha, bool device_input = false)
 {
     hipStream_t st = c->stream;
+    if (!c->cons.empty()) throw GalphaError("constraints apply to single-set runs (clear them with galphaSetConstraints)");
     c->batch_sets.clear();
     c->batch_npts.assign(nsets, 0);
     std::vector<int> ns, Ts;
@@ -992,7 +1058,8 @@ static void release_all(galpha_ctx *c)
                       &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
                       &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd, &c->plo,
-                      &c->perm, &c->xys, &c->pcell, &c->ccount, &c->ccursor, &c->cls};
+                      &c->perm, &c->xys, &c->pcell, &c->ccount, &c->ccursor, &c->cls,
+                      &c->kin, &c->kv, &c->kinv, &c->kdone, &c->kcnt, &c->krs, &c->kroff, &c->kwork, &c->khash};
     for (DevBuf *b : bufs) b->release();
     alpha_release(c->alpha);
 }
@@ -1010,7 +1077,8 @@ size_t galpha_ctx_device_bytes(galpha_ctx *c)
                       &c->scanTmp, &c->sortTmp, &c->tileCnt, &c->tileOff, &c->dellist, &c->dbg,
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
                       &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd, &c->plo,
-                      &c->perm, &c->xys, &c->pcell, &c->ccount, &c->ccursor, &c->cls};
+                      &c->perm, &c->xys, &c->pcell, &c->ccount, &c->ccursor, &c->cls,
+                      &c->kin, &c->kv, &c->kinv, &c->kdone, &c->kcnt, &c->krs, &c->kroff, &c->kwork, &c->khash};
     size_t t = 0;
     for (DevBuf *b : bufs) t += b->bytes;
     for (auto &b : c->alpha.b) t += b.bytes;
@@ -1045,11 +1113,26 @@ void galphaDestroy(galpha_ctx *c)
 const char *galphaLastError(galpha_ctx *c) { return c ? c->err.c_str() : g_last_error.c_str(); }
 const char *gpudtGetLastError(void) { return g_last_error.c_str(); }
 
+// sets the thread-local message read by gpudtGetLastError (for the C-ABI calls in other files)
+__attribute__((visibility("hidden"))) void galpha_set_last_error(const char *msg) { g_last_error = msg ? msg : ""; }
+
 static int fail(galpha_ctx *c, const std::exception &e)
 {
     g_last_error = e.what();
     if (c) c->err = e.what();
     return -1;
+}
+
+int galphaSetConstraints(galpha_ctx *c, const int *pairs, int n)
+{
+    if (!c) return -1;
+    try {
+        if (n < 0 || (n > 0 && !pairs)) throw GalphaError("invalid constraint list");
+        c->cons.assign(pairs, pairs + 2 * (size_t)n);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(c, e);
+    }
 }
 
 int galphaComputeDevice(galpha_ctx *c, const double *d_xy, int n, int fboSize, int with_alpha, void *stream)
@@ -1059,20 +1142,26 @@ int galphaComputeDevice(galpha_ctx *c, const double *d_xy, int n, int fboSize, i
         HIPCHK(hipSetDevice(c->device));
         hipStream_t own = c->stream;
         if (stream) c->stream = (hipStream_t)stream;
-        double bb[5] = {0, 0, 1, 1, 0};
-        if (n > 0) {
-            c->bboxTmp.ensure(8 * 5 * 64);
-            bbox_device(d_xy, n, bb, c->bboxTmp.p, c->stream);
-        }
-        int F = run_dt(c, d_xy, n, fboSize, bb);
-        c->last_alpha = false;
-        if (with_alpha) {
-            auto t0 = std::chrono::steady_clock::now();
-            alpha_run(c->alpha, d_xy, n, c->out9.as<int>(), F, c->stream, &c->stats);
-            c->last_alpha = true;
-            HIPCHK(hipStreamSynchronize(c->stream));
-            auto t1 = std::chrono::steady_clock::now();
-            c->stats.ms_alpha = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        int F;
+        try {   // the context's own stream comes back on every exit
+            double bb[5] = {0, 0, 1, 1, 0};
+            if (n > 0) {
+                c->bboxTmp.ensure(8 * 5 * 64);
+                bbox_device(d_xy, n, bb, c->bboxTmp.p, c->stream);
+            }
+            F = run_dt(c, d_xy, n, fboSize, bb);
+            c->last_alpha = false;
+            if (with_alpha) {
+                auto t0 = std::chrono::steady_clock::now();
+                alpha_run(c->alpha, d_xy, n, c->out9.as<int>(), F, c->stream, &c->stats);
+                c->last_alpha = true;
+                HIPCHK(hipStreamSynchronize(c->stream));
+                auto t1 = std::chrono::steady_clock::now();
+                c->stats.ms_alpha = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            }
+        } catch (...) {
+            c->stream = own;
+            throw;
         }
         c->stream = own;
         return F;
@@ -1234,14 +1323,16 @@ PGPUDTOUTPUT gpudtComputeDT(PGPUDTPARAMS p)
     try {
         g_last_error.clear();
         if (!p) throw GalphaError("null parameters");
-        if (p->nConstraints > 0) throw GalphaError("constrained Delaunay triangulation is not supported");
         if (p->nPoints < 0 || (p->nPoints > 0 && !p->points)) throw GalphaError("invalid point list");
+        if (p->nConstraints < 0 || (p->nConstraints > 0 && !p->constraints)) throw GalphaError("invalid constraint list");
         galpha_ctx *c = default_ctx();
         if (!c) throw GalphaError(g_last_error);
         std::lock_guard<std::mutex> lk(g_default_mu);
         int n = p->nPoints;
         int cap = 0;
+        c->cons.assign(p->constraints, p->constraints + 2 * (size_t)std::max(p->nConstraints, 0));
         int F = galphaComputeHost(c, (const double *)p->points, n, p->fboSize, nullptr, &cap);
+        c->cons.clear();
         if (F < 0) throw GalphaError(c->err);
         PGPUDTOUTPUT out = new GPUDTOUTPUT;
         out->nTris = F;

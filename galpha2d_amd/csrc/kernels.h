@@ -106,6 +106,10 @@ struct MeshDev {
     int *lmark;          // [cap] operation that last put the slot in a flip list (one entry per pass)
     int *ffan;           // [2 * (V - n)] per frame vertex: first fan triangle (enc) and fan length
     int list_op;         // lmark tag of the list that frame removal and convexification fill
+    // constraint edges (set of (min << 32 | max) + 1 keys, open addressing, 0 = empty); NULL when
+    // the run has no constraints.  k_flip_test never flips an edge in this set.
+    const unsigned long long *chash;
+    int chmask;
 };
 
 // vertex states
@@ -149,6 +153,14 @@ struct MeshDev {
 #define C_PLIST 30            // entries of the pending-vertex list (MeshDev::plist)
 #define C_EAR_PREV 31         // ear candidates of the previous round
 #define C_STATUS 32           // device status word (GA_ERR_* bits); MeshDev::status points here
+#define C_CONS_PEND 33        // constraints still to insert (this round's walk)
+#define C_CONS_MOD 34         // slots rewritten by this round's constraint insertions
+// status bits of constraint insertion (the other GA_ERR_* bits are in dev_common.h)
+#define GA_ERR_CONS_POINT 128     // a point lies inside a constraint segment
+#define GA_ERR_CONS_CROSS 256     // two constraints cross
+#define GA_ERR_CONS_WALK 512      // the walk along a constraint left the mesh
+#define GA_ERR_CONS_DUP 1024      // a constraint endpoint is a dropped duplicate point
+#define GA_ERR_CONS_TRI 2048      // a retriangulated cavity produced a triangle that is not ccw
 #ifndef GA_STRIPES
 #define GA_STRIPES 16         // statistic stripes, GA_STRIPE_INTS ints apart (also in dev_common.h)
 #define GA_STRIPE_INTS 1024
@@ -202,6 +214,14 @@ void launch_fold_stats(int *counters, hipStream_t st);
 void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st);
 void launch_flip_pass(MeshDev &m, const int *list_in, int cnt_in_slot, int *list_out, int cnt_out_slot,
                       int prev_op, int op, hipStream_t st);
+// constrained DT (see mesh.hip).  Set-up: cv = constraints as vertex ids, done flags, the key set
+// (chash, size chmask + 1, cleared by the caller), vt rebuilt from the live slots.
+void launch_cons_setup(MeshDev &m, const int2 *cin, int nc, int *inv, int2 *cv, int *done,
+                       unsigned long long *chash, int chmask, hipStream_t st);
+// one insertion round: walk (counts, region sizes) -> scan -> collect + claim -> commit -> relink
+void launch_cons_count(MeshDev &m, const int2 *cv, int nc, int *done, int *cnt, int *rsize, hipStream_t st);
+void launch_cons_insert(MeshDev &m, const int2 *cv, int nc, int *done, const int *cnt, const int *roff,
+                        int *work, int *modlist, int op, hipStream_t st);
 void launch_frame_round(MeshDev &m, unsigned int *mx, int round, int *list_out, int cnt_out_slot, int op,
                         hipStream_t st);
 // bulk frame removal (see mesh.hip); bnd = 11 * B ints of work space once B is known

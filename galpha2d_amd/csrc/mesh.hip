@@ -874,6 +874,33 @@ void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, hipStre
     GA_LAUNCH(k_relink, dim3(FIXED_GRID), dim3(256), 0, st, m, modlist, cnt_slot, op);
 }
 
+// ---- constraint edge set ----------------------------------------------------------------------
+// key of the undirected edge {a, b}; never 0 (the empty mark)
+GA_INLINE unsigned long long cons_key(int a, int b)
+{
+    unsigned lo = (unsigned)min(a, b), hi = (unsigned)max(a, b);
+    return (((unsigned long long)lo << 32) | hi) + 1ull;
+}
+
+GA_INLINE unsigned long long cons_mix(unsigned long long k)
+{
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    return k ^ (k >> 33);
+}
+
+// true when {a, b} is a constraint; the table is at least twice the number of constraints, so
+// every probe sequence reaches an empty slot
+GA_INLINE bool cons_has(const MeshDev &m, int a, int b)
+{
+    unsigned long long k = cons_key(a, b);
+    for (unsigned long long h = cons_mix(k);; h++) {
+        unsigned long long s = m.chash[h & (unsigned long long)m.chmask];
+        if (s == k) return true;
+        if (s == 0) return false;
+    }
+}
+
 // ---- flipping ---------------------------------------------------------------------------------
 // every slot in use (dead ones are skipped by the test); no atomics: the list is the identity
 __global__ void k_flip_all(MeshDev m, int op, int *list, int cnt_slot)
@@ -921,6 +948,7 @@ __global__ __launch_bounds__(256) void k_flip_test(MeshDev m, const int *list, i
                 if (e < 0) continue;
                 int u = enc_t(e);
                 if (m.stamp[u] == prev_op && u < t) continue;
+                if (m.chash && cons_has(m, ic(tt, p1(i)), ic(tt, m1(i)))) continue;   // constraint edge
                 int d = ic(m.tv[u], enc_j(e));
                 tests++;
                 if (incircle_dev(m, tt.x, tt.y, tt.z, d) > 0) {
@@ -1698,4 +1726,319 @@ void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid,
     GA_LAUNCH(k_canon_newid, dim3(FIXED_GRID), dim3(256), 0, st, order, rot, dF, newid);
     GA_LAUNCH(k_canon_emit, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)newid, out9, rset);
     GA_LAUNCH(k_canon_set_first, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)rset, dF, set_first);
+}
+
+// ---- constrained DT ---------------------------------------------------------------------------
+// Constraint edges go into the finished DT by cavity retriangulation.  A constraint (a, b) walks
+// from a to b and collects the triangles it crosses (its cavity) with the chains of vertices
+// left and right of it.  Cavities claim their slots; every constraint that holds all of its slots
+// replaces its cavity by triangulations of the two pseudo-polygons on either side of the new edge.
+// Each is built by taking the chain vertex whose circle through the base edge holds no other chain
+// vertex (tie-broken incircle, the order the flip pass uses) and recursing on both sides.  A cavity
+// of K triangles has K + 2 vertices, so the new triangles reuse its K slots.  A constrained flip
+// pass afterwards (k_flip_test skips constraint edges) makes every other edge locally Delaunay.
+// The reference inserts by parallel flips instead (Cuda/cudaConstraint.cu:143-939) and marks the
+// constraint edges for its flip pass (Cuda/cudaFlipping.cu:111-220); the result, the constrained
+// DT, is unique under the tie-break.
+
+__global__ void k_cons_inv(MeshDev m, int *inv)
+{
+    GRID_STRIDE(v, m.n) inv[m.perm[v]] = v;
+}
+
+__global__ void k_cons_map(MeshDev m, const int2 *cin, int nc, const int *inv, int2 *cv, int *done,
+                           unsigned long long *chash, int chmask)
+{
+    GRID_STRIDE(c, nc) {
+        int a = inv[cin[c].x], b = inv[cin[c].y];
+        cv[c] = make_int2(a, b);
+        done[c] = 0;
+        if (m.vstate[a] == VS_DUP || m.vstate[b] == VS_DUP) {
+            atomicOr(m.status, GA_ERR_CONS_DUP);
+            done[c] = 1;
+            continue;
+        }
+        unsigned long long k = cons_key(a, b);
+        for (unsigned long long h = cons_mix(k);; h++) {
+            unsigned long long prev = atomicCAS(&chash[h & (unsigned long long)chmask], 0ull, k);
+            if (prev == 0 || prev == k) break;
+        }
+    }
+}
+
+// every vertex's anchor triangle from the live slots (frame removal leaves some anchors stale)
+__global__ void k_cons_vt(MeshDev m)
+{
+    int n = ntri_used(m);
+    GRID_STRIDE(t, n) {
+        int4 tt = m.tv[t];
+        if (tt.x == GA_DEAD) continue;
+        m.vt[tt.x] = t;
+        m.vt[tt.y] = t;
+        m.vt[tt.z] = t;
+    }
+}
+
+// ints of a cavity region of K triangles: K, ml, mr, pad; slots[K]; Lx[K + 2]; Rx[K + 2];
+// lb[K + 1]; rb[K + 1]; ol[K + 1]; orr[K + 1]; stack[3 (K + 2)]
+GA_INLINE int cons_region(int K) { return 10 * K + 18; }
+
+struct ConsRegion {
+    int *hdr, *slots, *Lx, *Rx, *lb, *rb, *ol, *orr, *stk;
+    __device__ ConsRegion(int *r, int K)
+    {
+        hdr = r;
+        slots = r + 4;
+        Lx = slots + K;
+        Rx = Lx + K + 2;
+        lb = Rx + K + 2;
+        rb = lb + K + 1;
+        ol = rb + K + 1;
+        orr = ol + K + 1;
+        stk = orr + K + 1;
+    }
+};
+
+// Walk of constraint (a, b).  Returns the cavity size K >= 1, 0 when the edge exists already, -1
+// on an error (status set).  With REC the cavity is written to the region sized for Kc triangles:
+// slots in walk order; Lx = a, left chain, b and Rx = a, right chain, b; lb[i] / rb[i] = the
+// cavity side (t << 2 | j) on the boundary edge between Lx[i], Lx[i + 1] / Rx[i], Rx[i + 1].
+template <bool REC>
+__device__ int cons_walk(const MeshDev &m, int a, int b, ConsRegion *R, int Kc)
+{
+    double2 A = m.pos[a], B = m.pos[b];
+    int t0 = m.vt[a], t = -1, k = 0;
+    if (t0 < 0 || t0 >= m.cap) {
+        atomicOr(m.status, GA_ERR_CONS_WALK);
+        return -1;
+    }
+    // the triangle (a, p, q) of a's star with b strictly inside its angle at a
+    for (int dir = 0; dir < 2 && t < 0; dir++) {
+        int s = t0;
+        for (int guard = 0; guard < 1000000; guard++) {
+            int4 tt = m.tv[s];
+            int ka = tt.x == a ? 0 : (tt.y == a ? 1 : 2);
+            int p = ic(tt, p1(ka)), q = ic(tt, m1(ka));
+            if (p == b || q == b) return 0;
+            double2 P = m.pos[p], Q = m.pos[q];
+            int op = orient_dev(m, A, P, B), oq = orient_dev(m, A, Q, B);
+            // a vertex on the ray from a towards b lies inside the segment (b itself is not on
+            // an edge of a's star)
+            if ((op == 0 && (P.x - A.x) * (B.x - A.x) + (P.y - A.y) * (B.y - A.y) > 0) ||
+                (oq == 0 && (Q.x - A.x) * (B.x - A.x) + (Q.y - A.y) * (B.y - A.y) > 0)) {
+                atomicOr(m.status, GA_ERR_CONS_POINT);
+                return -1;
+            }
+            if (op > 0 && oq < 0) {
+                t = s;
+                k = ka;
+                break;
+            }
+            // counter-clockwise: across (a, q), the side opposite p; clockwise: across (a, p)
+            int e = ic(m.tn[s], dir == 0 ? p1(ka) : m1(ka));
+            if (e < 0) break;
+            s = enc_t(e);
+            if (s == t0) break;
+        }
+    }
+    if (t < 0) {
+        atomicOr(m.status, GA_ERR_CONS_WALK);
+        return -1;
+    }
+    int4 tt = m.tv[t];
+    int p = ic(tt, p1(k)), q = ic(tt, m1(k));   // p right of a -> b, q left
+    int K = 1, ml = 1, mr = 1;
+    if (REC) {
+        R->slots[0] = t;
+        R->Lx[0] = a; R->Lx[1] = q;
+        R->Rx[0] = a; R->Rx[1] = p;
+        R->lb[0] = enc(t, p1(k));   // edge (q, a)
+        R->rb[0] = enc(t, m1(k));   // edge (a, p)
+    }
+    int cur = t, ex = k;
+    for (int guard = 0; guard < m.cap; guard++) {
+        int e = ic(m.tn[cur], ex);
+        if (e < 0) break;
+        int u = enc_t(e), j = enc_j(e);
+        int4 ut = m.tv[u];
+        // entered across (r1, r2): r1 left of a -> b, r2 right; w is the far vertex
+        int w = ic(ut, j), r1 = ic(ut, p1(j)), r2 = ic(ut, m1(j));
+        if (cons_has(m, r1, r2)) {
+            atomicOr(m.status, GA_ERR_CONS_CROSS);
+            return -1;
+        }
+        if (REC && K >= Kc) break;   // the mesh changed since the count: report below
+        if (REC) R->slots[K] = u;
+        K++;
+        if (w == b) {
+            if (REC) {
+                R->lb[ml] = enc(u, m1(j));   // edge (r1, b)
+                R->rb[mr] = enc(u, p1(j));   // edge (b, r2)
+                R->Lx[ml + 1] = b;
+                R->Rx[mr + 1] = b;
+                R->hdr[0] = K; R->hdr[1] = ml; R->hdr[2] = mr;
+            }
+            return K;
+        }
+        int o = orient_dev(m, A, B, m.pos[w]);
+        if (o == 0) {   // w is on the segment (the walk has not reached b)
+            atomicOr(m.status, GA_ERR_CONS_POINT);
+            return -1;
+        }
+        if (o > 0) {   // w joins the left chain; leave across (w, r2)
+            if (REC) { R->lb[ml] = enc(u, m1(j)); R->Lx[ml + 1] = w; }
+            ml++;
+            ex = p1(j);
+        } else {       // w joins the right chain; leave across (r1, w)
+            if (REC) { R->rb[mr] = enc(u, p1(j)); R->Rx[mr + 1] = w; }
+            mr++;
+            ex = m1(j);
+        }
+        cur = u;
+    }
+    atomicOr(m.status, GA_ERR_CONS_WALK);
+    return -1;
+}
+
+__global__ void k_cons_count(MeshDev m, const int2 *cv, int nc, int *done, int *cnt, int *rsize)
+{
+    GRID_STRIDE(c, nc) {
+        int K = 0;
+        if (!done[c]) {
+            K = cons_walk<false>(m, cv[c].x, cv[c].y, nullptr, 0);
+            if (K <= 0) {
+                done[c] = 1;   // present already, or an error (status set)
+                K = 0;
+            }
+        }
+        cnt[c] = K;
+        rsize[c] = K ? cons_region(K) : 0;
+        if (K) atomicAdd(&m.counters[C_CONS_PEND], 1);
+    }
+}
+
+// walks again, recording the cavity, and claims its slots (larger key = smaller constraint id)
+__global__ void k_cons_collect(MeshDev m, const int2 *cv, int nc, const int *cnt, const int *roff, int *work,
+                               int op)
+{
+    GRID_STRIDE(c, nc) {
+        int K = cnt[c];
+        if (!K) continue;
+        ConsRegion R(work + roff[c], K);
+        R.hdr[0] = 0;
+        if (cons_walk<true>(m, cv[c].x, cv[c].y, &R, K) != K) {
+            atomicOr(m.status, GA_ERR_CONS_WALK);
+            continue;
+        }
+        unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)c);
+        for (int i = 0; i < K; i++) atomicMax(&m.claim[R.slots[i]], key);
+    }
+}
+
+// boundary edge of the cavity on side k of new triangle T: the old cavity side `side` had the
+// outside neighbour `out` (-1 on the hull); k_relink repairs the outside end
+GA_INLINE void cons_boundary(const MeshDev &m, int T, int k, int side, int out)
+{
+    set_link(m, T, k, mark_old(out));
+    ((int *)(m.remap + enc_t(side)))[enc_j(side)] = enc(T, k);
+}
+
+// Triangulates the pseudo-polygon E(0), E(1..M), E(M + 1) lying left of E(0) -> E(M + 1), with
+// E(i) = X[i] (or X[M + 1 - i] when rev); bs / bo: cavity side and outside neighbour of the
+// boundary edge between E(i) and E(i + 1) at index i (M - i when rev).  New triangles take slots
+// from slots[*s]; returns the triangle on the base edge E(0) E(M + 1) (its side 2).
+__device__ int cons_fill(const MeshDev &m, const int *X, int M, bool rev, const int *bs, const int *bo,
+                         const int *slots, int *s, int *stk)
+{
+#define CE(i) (rev ? X[M + 1 - (i)] : X[(i)])
+#define CB(i) (rev ? M - (i) : (i))
+    int top = -1, sp = 0;
+    stk[0] = 1; stk[1] = M; stk[2] = -1;
+    sp = 1;
+    while (sp > 0) {
+        sp--;
+        int lo = stk[3 * sp], hi = stk[3 * sp + 1], pe = stk[3 * sp + 2];
+        int x = CE(lo - 1), y = CE(hi + 1);
+        int ci = lo;
+        for (int i = lo + 1; i <= hi; i++)
+            if (incircle_dev(m, x, y, CE(ci), CE(i)) > 0) ci = i;
+        int cv = CE(ci);
+        int T = slots[(*s)++];
+        m.tv[T] = make_int4(x, y, cv, 0);
+        if (orient_dev(m, m.pos[x], m.pos[y], m.pos[cv]) <= 0) atomicOr(m.status, GA_ERR_CONS_TRI);
+        if (pe >= 0) {
+            set_link(m, T, 2, pe);
+            set_link(m, enc_t(pe), enc_j(pe), enc(T, 2));
+        } else {
+            top = T;
+        }
+        // side 1: edge (cv, x); side 0: edge (y, cv)
+        if (ci > lo) {
+            stk[3 * sp] = lo; stk[3 * sp + 1] = ci - 1; stk[3 * sp + 2] = enc(T, 1);
+            sp++;
+        } else {
+            cons_boundary(m, T, 1, bs[CB(lo - 1)], bo[CB(lo - 1)]);
+        }
+        if (ci < hi) {
+            stk[3 * sp] = ci + 1; stk[3 * sp + 1] = hi; stk[3 * sp + 2] = enc(T, 0);
+            sp++;
+        } else {
+            cons_boundary(m, T, 0, bs[CB(hi)], bo[CB(hi)]);
+        }
+        m.vt[x] = T;
+        m.vt[y] = T;
+        m.vt[cv] = T;
+    }
+#undef CE
+#undef CB
+    return top;
+}
+
+__global__ void k_cons_commit(MeshDev m, const int2 *cv, int nc, int *done, const int *cnt, const int *roff,
+                              int *work, int *modlist, int op)
+{
+    GRID_STRIDE(c, nc) {
+        int K = cnt[c];
+        if (!K) continue;
+        ConsRegion R(work + roff[c], K);
+        if (R.hdr[0] != K) continue;   // collect failed (status set)
+        unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)c);
+        bool win = true;
+        for (int i = 0; i < K && win; i++) win = m.claim[R.slots[i]] == key;
+        if (!win) continue;
+        int ml = R.hdr[1], mr = R.hdr[2];
+        // outside neighbours of the boundary, read before any slot is rewritten
+        for (int i = 0; i <= ml; i++) R.ol[i] = ic(m.tn[enc_t(R.lb[i])], enc_j(R.lb[i]));
+        for (int i = 0; i <= mr; i++) R.orr[i] = ic(m.tn[enc_t(R.rb[i])], enc_j(R.rb[i]));
+        for (int i = 0; i < K; i++) m.stamp[R.slots[i]] = op;
+        int s = 0;
+        int tl = cons_fill(m, R.Lx, ml, false, R.lb, R.ol, R.slots, &s, R.stk);
+        int tr = cons_fill(m, R.Rx, mr, true, R.rb, R.orr, R.slots, &s, R.stk);
+        set_link(m, tl, 2, enc(tr, 2));   // the constraint edge (a, b)
+        set_link(m, tr, 2, enc(tl, 2));
+        int q = atomicAdd(&m.counters[C_CONS_MOD], K);
+        for (int i = 0; i < K; i++) modlist[q + i] = R.slots[i];
+        done[c] = 1;
+    }
+}
+
+void launch_cons_setup(MeshDev &m, const int2 *cin, int nc, int *inv, int2 *cv, int *done,
+                       unsigned long long *chash, int chmask, hipStream_t st)
+{
+    GA_LAUNCH(k_cons_inv, dim3(FIXED_GRID), dim3(256), 0, st, m, inv);
+    GA_LAUNCH(k_cons_map, dim3(FIXED_GRID), dim3(256), 0, st, m, cin, nc, (const int *)inv, cv, done, chash, chmask);
+    GA_LAUNCH(k_cons_vt, dim3(FIXED_GRID), dim3(256), 0, st, m);
+}
+
+void launch_cons_count(MeshDev &m, const int2 *cv, int nc, int *done, int *cnt, int *rsize, hipStream_t st)
+{
+    GA_LAUNCH(k_cons_count, dim3(FIXED_GRID), dim3(256), 0, st, m, cv, nc, done, cnt, rsize);
+}
+
+void launch_cons_insert(MeshDev &m, const int2 *cv, int nc, int *done, const int *cnt, const int *roff,
+                        int *work, int *modlist, int op, hipStream_t st)
+{
+    GA_LAUNCH(k_cons_collect, dim3(FIXED_GRID), dim3(256), 0, st, m, cv, nc, cnt, roff, work, op);
+    GA_LAUNCH(k_cons_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, cv, nc, done, cnt, roff, work, modlist, op);
+    GA_LAUNCH(k_relink, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)modlist, C_CONS_MOD, op);
 }

@@ -8,7 +8,9 @@
  * gpudt/gpudt.cpp:771-929).  Differences, all on error paths:
  *   - failures return NULL (or a negative code) and set a thread-local message readable
  *     with gpudtGetLastError(); the reference calls exit(-1) (Cuda/common.h:60-93);
- *   - nConstraints > 0 is rejected (the constrained path is not built);
+ *   - constraints that the reference leaves undefined are errors: an endpoint outside
+ *     [0, nPoints) or equal to the other, an endpoint that is a dropped duplicate point, a
+ *     point inside a constraint segment, and two constraints that cross;
  *   - minX/minY/maxX/maxY are recomputed from the points;
  *   - gpudtReleaseDTOutput frees the struct too (the reference leaks it, gpudt.cpp:926-929).
  * Output triangles are counter-clockwise, each rotated so vtx[0] is its smallest vertex and
@@ -16,6 +18,10 @@
  * neighbour across the edge opposite vtx[i] (t.vtx[j] is its far vertex), -1 on the hull;
  * tmp[] is zero.  Exact duplicate points keep their smallest index; the others are left out
  * (the reference drops duplicates too, Cuda/cudaMissing.cu:285-289).
+ * With nConstraints > 0 (constraints: pairs of point indices, gpudt/gpudt.h:113-114) the output
+ * is the constrained DT: every constraint is an edge and every other edge is locally Delaunay
+ * (Cuda/cudaConstraint.cu:828-939, Cuda/cudaFlipping.cu:111-220, gpudt.cpp:88-155); it is unique
+ * under the same tie-break, so the output does not depend on the constraint order.
  *
  * The alpha-shape entry points replace the C++ class Alpha_Shapes_2D
  * (AlphaShape/Alpha_Shapes_2D.h:43-121, computeAlphaShapes Alpha_Shapes_2D.cu:206-233) with
@@ -111,6 +117,10 @@ typedef struct {
     long long frameSeen, frameChain, flipListed;
     /* 1 when the frame was removed in bulk (one boundary cycle per set), 0 round by round */
     int frameBulk, earRounds;
+    /* constrained runs: cavity insertion rounds, and the time of the constraint stage (insertion
+       and the constrained flips) */
+    int constraintRounds;
+    double ms_constraints;
 } galpha_stats;
 
 galpha_ctx *galphaCreate(int device);
@@ -127,6 +137,10 @@ int galphaComputeHost(galpha_ctx *ctx, const double *xy, int n, int fboSize,
  * stream is a hipStream_t (NULL = the context's own stream). */
 int galphaComputeDevice(galpha_ctx *ctx, const double *d_xy, int n, int fboSize, int with_alpha,
                         void *stream);
+/* Constraint edges (n pairs of point indices, the layout of GPUDTPARAMS::constraints) for the
+ * following single-set runs of this context (galphaComputeHost / galphaComputeDevice), until set
+ * again; n = 0 clears them.  Batches and serving reject a context that holds constraints. */
+int galphaSetConstraints(galpha_ctx *ctx, const int *pairs, int n);
 /* device pointers to the last result: triangles in the 9-int layout, nTris */
 int galphaDeviceTriangles(galpha_ctx *ctx, const int **d_tris9, int *ntris);
 int galphaCopyTriangles(galpha_ctx *ctx, int *tris_out9, int capacity);
@@ -207,6 +221,34 @@ int galphaTextureFor(galpha_ctx *ctx, int n);
  * densities_out (optional): the 4 chosen densities.  Results do not depend on the texture. */
 int galphaAutotune(galpha_ctx *ctx, const double *xy, const int *offsets, int nsets, int reps,
                    double *densities_out);
+
+/* ---- formats ---------------------------------------------------------------------------- */
+/* Point file of the reference (main.cu:117-161, ReadDataFromFileBinary): "N", then N lines
+ * "idx x y", point idx stored at points[idx]; min/max computed as there (main.cu:139-159).
+ * *pParams is library-allocated (release with galphaReleaseParams), nConstraints = 0 and
+ * fboSize = galphaAutoTexture(N).  Unlike the reference, a missing file, a short file, a bad
+ * number, an index outside [0, N) or a point with no line is an error (-1, gpudtGetLastError). */
+int galphaReadPoints(const char *path, PGPUDTPARAMS *pParams);
+void galphaReleaseParams(PGPUDTPARAMS pParams);
+/* Writes points in the same format with 17 significant digits, so reading gives the same doubles. */
+int galphaWritePoints(const char *path, const gpudtVertex *points, int nPoints);
+/* Alpha_Shapes_2D::dumpValues (AlphaShape/Alpha_Shapes_2D.cu:749-793) to path: the spectrum size
+ * line.  full != 0 adds the sections that function keeps commented out (VERTICES: x y nu_min
+ * nu_max; EDGES: x1 y1 x2 y2 ro nu_min nu_max; TRIANGLES: x0 y0 x1 y1 x2 y2 ro; values are
+ * spectrum[rank], "nan" for rank -1) before the size line and the spectrum values after it, all
+ * with 17 significant digits (the reference's stream default is 6). */
+int galphaDumpValues(const char *path, PGPUDTPARAMS pParams, PGPUDTOUTPUT pDT, PGPUDTALPHA pAlpha, int full);
+/* Binary result payload (no reference counterpart): a 40-byte header (magic "G2DR", version 1,
+ * flags bit 0 = alpha present, nPoints, nTris, nEdges, specSize, masterSize, nSpecPointers, 0),
+ * the arrays of the triangles and of the alpha result, each padded to 8 bytes, then a 64-bit
+ * FNV-1a hash of all bytes before it; little-endian.  galphaPackResult returns the payload size
+ * and writes it when capacity suffices (alpha may be NULL); -1 on error.  galphaUnpackResult
+ * checks the header, size and hash and allocates *tris9 (galphaFree) and *pAlpha
+ * (gpudtReleaseAlpha; NULL when the payload has no alpha section). */
+long long galphaPackResult(int nPoints, const int *tris9, int nTris, PGPUDTALPHA alpha, void *buf,
+                           long long capacity);
+int galphaUnpackResult(const void *buf, long long size, int *nPoints, int **tris9, int *nTris,
+                       PGPUDTALPHA *pAlpha);
 
 #ifdef __cplusplus
 }

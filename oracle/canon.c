@@ -66,8 +66,56 @@ void ora_canonicalize(ora_tri *t, int nt)
 
 #define FAIL(code, ...) do { if (msg) snprintf(msg, msglen, __VA_ARGS__); rc = code; goto done; } while (0)
 
+static int cmp_u64(const void *a, const void *b)
+{
+    uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+static uint64_t edge_key(int a, int b)
+{
+    int lo = a < b ? a : b, hi = a < b ? b : a;
+    return (uint64_t)(uint32_t)lo << 32 | (uint32_t)hi;
+}
+
+/* ckeys: sorted constraint pairs (NULL: none); constraint edges are exempt from the incircle
+ * test and must all be present */
+static int validate_impl(const double *xy, int n, const ora_tri *t, int nt, const int *dup_of,
+                         const uint64_t *ckeys, int nck, char *msg, int msglen);
+
 int ora_validate(const double *xy, int n, const ora_tri *t, int nt,
                  const int *dup_of, char *msg, int msglen)
+{
+    return validate_impl(xy, n, t, nt, dup_of, NULL, 0, msg, msglen);
+}
+
+int ora_validate_constrained(const double *xy, int n, const ora_tri *t, int nt, const int *dup_of,
+                             const int *cons, int ncons, char *msg, int msglen)
+{
+    uint64_t *k = (uint64_t *)malloc(sizeof(uint64_t) * (ncons > 0 ? ncons : 1));
+    for (int c = 0; c < ncons; c++) k[c] = edge_key(cons[2 * c], cons[2 * c + 1]);
+    qsort(k, ncons, sizeof(uint64_t), cmp_u64);
+    int rc = validate_impl(xy, n, t, nt, dup_of, k, ncons, msg, msglen);
+    if (rc == 0) {
+        /* every constraint is an edge of the mesh (main.cu:218-286) */
+        int ne = 3 * nt, m = 0;
+        uint64_t *e = (uint64_t *)malloc(sizeof(uint64_t) * (ne > 0 ? ne : 1));
+        for (int q = 0; q < nt; q++)
+            for (int i = 0; i < 3; i++) e[m++] = edge_key(t[q].vtx[(i + 1) % 3], t[q].vtx[(i + 2) % 3]);
+        qsort(e, m, sizeof(uint64_t), cmp_u64);
+        for (int c = 0; c < ncons && rc == 0; c++)
+            if (!bsearch(&k[c], e, m, sizeof(uint64_t), cmp_u64)) {
+                if (msg) snprintf(msg, msglen, "constraint %d (%d, %d) is not an edge", c, cons[2 * c], cons[2 * c + 1]);
+                rc = 9;
+            }
+        free(e);
+    }
+    free(k);
+    return rc;
+}
+
+static int validate_impl(const double *xy, int n, const ora_tri *t, int nt, const int *dup_of,
+                         const uint64_t *ckeys, int nck, char *msg, int msglen)
 {
     int rc = 0;
     ora_exactinit();
@@ -104,7 +152,8 @@ int ora_validate(const double *xy, int n, const ora_tri *t, int nt,
             if (t[u].tri[j] != ((k << 2) | i)) FAIL(3, "triangle %d slot %d: link not reciprocal", k, i);
             if (t[u].vtx[(j + 1) % 3] != e1 || t[u].vtx[(j + 2) % 3] != e0)
                 FAIL(3, "triangle %d slot %d: shared edge mismatch", k, i);
-            if (u > k) {
+            uint64_t ek = edge_key(e0, e1);
+            if (u > k && !(ckeys && bsearch(&ek, ckeys, nck, sizeof(uint64_t), cmp_u64))) {
                 const double *d = xy + 2 * (size_t)t[u].vtx[j];
                 if (ora_incircle_sos(a, t[k].vtx[0], b, t[k].vtx[1], c, t[k].vtx[2], d, t[u].vtx[j]) > 0)
                     FAIL(7, "edge (%d,%d) between triangles %d and %d is not locally Delaunay", e0, e1, k, u);

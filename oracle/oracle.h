@@ -21,6 +21,10 @@
  *                 160-690, AlphaShape/Alpha_Shapes_2D.cu:127-727).
  *   canon.c       canonical triangle order and the validator that restates
  *                 checkResult (main.cu:163-343) with extra checks.
+ *   cdt.c         constrained DT: DT, then Sloan's edge-flip insertion of each
+ *                 constraint, then Lawson flips of the other edges (the result of
+ *                 Cuda/cudaConstraint.cu:828-939 + Cuda/cudaFlipping.cu:111-220 +
+ *                 gpudt.cpp:88-155).
  */
 #ifndef GALPHA_ORACLE_H
 #define GALPHA_ORACLE_H
@@ -64,6 +68,18 @@ void ora_canonicalize(ora_tri *t, int nt);
  * convex cycle, every interior edge locally Delaunay under the tie-break. */
 int ora_validate(const double *xy, int n, const ora_tri *t, int nt,
                  const int *dup_of, char *msg, int msglen);
+
+/* validator for a constrained DT: the checks above, with constraint edges exempt from the
+ * incircle test, and every constraint (cons: ncons pairs of point indices) an edge of the mesh
+ * (restates main.cu:218-286).  A mesh that passes is the constrained DT. */
+int ora_validate_constrained(const double *xy, int n, const ora_tri *t, int nt, const int *dup_of,
+                             const int *cons, int ncons, char *msg, int msglen);
+
+/* constrained DT (cdt.c): returns the number of canonical triangles written to *out (malloc'ed)
+ * or -1 with a message (invalid constraints: endpoint out of range, equal endpoints, duplicate
+ * endpoint, a point inside a constraint, crossing constraints). */
+int ora_cdt(const double *xy, int n, const int *cons, int ncons, ora_tri **out, int *dup_of,
+            char *msg, int msglen);
 
 /* alpha shapes on a canonical DT (layout of AlphaShape/headers.h:48-90) */
 typedef struct {

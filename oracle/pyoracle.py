@@ -49,6 +49,10 @@ def lib():
         L.ora_attached.argtypes = [dp, dp, dp]
         L.ora_attached.restype = C.c_int
         L.ora_pred_counters.argtypes = [C.POINTER(C.c_longlong)] * 5
+        L.ora_cdt.argtypes = [dp, C.c_int, ip, C.c_int, C.POINTER(C.c_void_p), ip, C.c_char_p, C.c_int]
+        L.ora_cdt.restype = C.c_int
+        L.ora_validate_constrained.argtypes = [dp, C.c_int, ip, C.c_int, ip, ip, C.c_int, C.c_char_p, C.c_int]
+        L.ora_validate_constrained.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -98,6 +102,37 @@ def validate(xy, tris, dup_of=None):
     msg = C.create_string_buffer(512)
     rc = lib().ora_validate(_dp(xy), xy.shape[0], _ip(t), t.shape[0],
                             _ip(d) if d is not None else None, msg, 512)
+    return rc, msg.value.decode()
+
+
+def cdt(xy, cons):
+    """Constrained DT (cdt.c): (tris int32 [F, 9] canonical, dup_of); raises ValueError with the
+    oracle's message for invalid constraints."""
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    c = np.ascontiguousarray(cons, dtype=np.int32).reshape(-1, 2)
+    n = xy.shape[0]
+    out = C.c_void_p()
+    dup = np.empty(max(n, 1), dtype=np.int32)
+    msg = C.create_string_buffer(512)
+    nt = lib().ora_cdt(_dp(xy), n, _ip(c), c.shape[0], C.byref(out), _ip(dup), msg, 512)
+    if nt < 0:
+        raise ValueError(msg.value.decode())
+    tris = np.zeros((nt, 9), dtype=np.int32)
+    if nt > 0:
+        C.memmove(tris.ctypes.data, out.value, nt * 36)
+    if out.value:
+        C.CDLL(None).free(out)
+    return tris, dup[:n].copy()
+
+
+def validate_constrained(xy, tris, dup_of, cons):
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    t = np.ascontiguousarray(tris, dtype=np.int32)
+    d = None if dup_of is None else np.ascontiguousarray(dup_of, dtype=np.int32)
+    c = np.ascontiguousarray(cons, dtype=np.int32).reshape(-1, 2)
+    msg = C.create_string_buffer(512)
+    rc = lib().ora_validate_constrained(_dp(xy), xy.shape[0], _ip(t), t.shape[0],
+                                        _ip(d) if d is not None else None, _ip(c), c.shape[0], msg, 512)
     return rc, msg.value.decode()
 
 

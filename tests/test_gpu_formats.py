@@ -1,0 +1,37 @@
+"""The command-line driver on the GPU (the counterpart of main.cu:364-440): it reads the
+reference's point files, runs DT + alpha through the reference-named entry points, checks the
+result and writes the payload and the full dump; both equal the oracle's result."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import galpha2d_amd as G
+from conftest import ALPHA_FIELDS, GOLDEN, ROOT, load_fixture
+
+CLI = os.path.join(ROOT, "galpha2d_amd", "galpha_cli")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["bird.txt", "alpha.txt", "1000_1.txt", "10000_1.txt"])
+def test_cli_reference_files(oracle, tmp_path, name):
+    import torch
+    print("case: galpha_cli", name, flush=True)
+    torch.cuda.init()   # the test process holds the device before the driver processes use it
+    pack, dump = tmp_path / "r.bin", tmp_path / "dump.txt"
+    r = subprocess.run([CLI, os.path.join(GOLDEN, name), "--check", "--pack", str(pack), "--dump-full", str(dump)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "GAlpha time:" in r.stdout and ", 0 errors," in r.stdout
+    xy = load_fixture(name)
+    want, _ = oracle.delaunay(xy)
+    wa = oracle.alpha(xy, want)
+    n, tris, alpha = G.unpack_result(pack.read_bytes())
+    assert n == len(xy) and np.array_equal(tris, want)
+    for k in ALPHA_FIELDS:
+        assert np.array_equal(alpha[k], np.asarray(wa[k])), k
+    # the full dump written by the driver equals the dump of the oracle's result
+    ref = tmp_path / "ref.txt"
+    G.dump_values(str(ref), xy, want, wa, full=True)
+    assert dump.read_text() == ref.read_text()
