@@ -78,6 +78,79 @@ def cpu_baseline(n, seed):
                       f"{dt:.2f} s on one core of {model or 'the host'} (nproc={os.cpu_count()})"}
 
 
+def cpu_batched(nsets, sample, seed=4):
+    """The oracle on the first `sample` sets of the batched workload, one set per task on up to 16
+    spawned host processes (they never touch the GPU; a worker that dies raises instead of
+    hanging the pool)."""
+    from concurrent.futures import ProcessPoolExecutor
+    import multiprocessing as mp
+    from galpha2d_amd import datagen
+    from oracle import pyoracle as O
+    sets = datagen.batch_sets(nsets, seed=seed)[:sample]
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as pool:
+        list(pool.map(O.dt_alpha_points, sets[:workers]))   # start-up and library load, not timed
+        t0 = time.perf_counter()
+        pts = sum(pool.map(O.dt_alpha_points, sets))
+        dt = time.perf_counter() - t0
+    return {"cpu_sets_per_s": round(len(sets) / dt, 1), "cpu_cores": workers, "cpu_sample_sets": len(sets),
+            "cpu_sample": f"first {len(sets)} sets ({pts} points), oracle DT + alpha, one set per task, "
+                          f"{workers} processes, {dt:.2f} s"}
+
+
+def host_to_host(G, xy, T, reps=3):
+    """The reference-named calls with host arrays in and out: gpudtComputeDT, then
+    gpudtComputeAlpha (which uploads the points and triangles again).  Median of `reps` runs after
+    one warm-up."""
+    import ctypes as C
+    import numpy as np
+    L = G.lib()
+    L.gpudtComputeDT.argtypes = [C.POINTER(G._Params)]
+    L.gpudtComputeDT.restype = C.POINTER(G._Output)
+    L.gpudtReleaseDTOutput.argtypes = [C.POINTER(G._Output)]
+    L.gpudtComputeAlpha.argtypes = [C.POINTER(G._Params), C.POINTER(G._Output), C.POINTER(C.POINTER(G._Alpha))]
+    L.gpudtComputeAlpha.restype = C.c_int
+    n = xy.shape[0]
+    prm = G._Params(0, 0, 1, 1, n, xy.ctypes.data_as(C.POINTER(C.c_double)), 0, None, T)
+    times = []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        out = L.gpudtComputeDT(C.byref(prm))
+        if not out:
+            raise RuntimeError(L.gpudtGetLastError().decode())
+        A = C.POINTER(G._Alpha)()
+        if L.gpudtComputeAlpha(C.byref(prm), out, C.byref(A)) < 0:
+            raise RuntimeError(L.gpudtGetLastError().decode())
+        dt = time.perf_counter() - t0
+        L.gpudtReleaseAlpha(A)
+        L.gpudtReleaseDTOutput(out)
+        if r:
+            times.append(dt)
+    med = float(np.median(times))
+    return {"value": round(n / med / 1e6, 3), "unit": "Mpoints/s", "ms": round(med * 1e3, 2),
+            "calls": "gpudtComputeDT + gpudtComputeAlpha, host arrays in and out (PCIe copies and the "
+                     "alpha stage's upload of the points and triangles included)"}
+
+
+def stage_gbs(st, a, n, T):
+    """Algorithmic bytes per stage by the scope table's formulas (P texels, N points, F triangles,
+    E edges, S = F + E spectrum entries, M master entries) over the measured stage time."""
+    P, F, E = T * T, st["nTris"], st["nEdges"]
+    S, M = F + E, len(a["master_simplex"])
+    model = {
+        "pba": ("S0-S2: 4P + 20N + 24P + 12P", 40 * P + 20 * n, st["ms_pba"]),
+        "recon": ("S3-S4: 4P + 12F + 48F", 4 * P + 60 * F, st["ms_recon"]),
+        "alpha": ("S7-S9: 24F + 48N + 8(F+E) + 12E + 24S + 8N + 24E + 4F + 24M",
+                  24 * F + 48 * n + 8 * (F + E) + 12 * E + 24 * S + 8 * n + 24 * E + 4 * F + 24 * M, st["ms_alpha"]),
+    }
+    out = {}
+    for k, (formula, b, ms) in model.items():
+        gbs = b / (ms * 1e-3) / 1e9
+        out[k] = {"formula": formula, "bytes": int(b), "ms": round(ms, 3), "gb_s": round(gbs, 1),
+                  "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def batched_vs_plain(G, torch, nsets, plain_sample, seed=4):
     """Batched workload: nsets point sets, sizes log-uniform in [256, 65536], each uniform or an
     8-cluster Gaussian mix (50/50), autotuned texture per set; DT + full alpha for every set.
@@ -266,6 +339,7 @@ def main():
     # workload): algorithmic bytes of the probed run / total time of its launches in that run
     probe = ctx.kernel_probe(args.probe, lambda: step())
     pst = ctx.stats()
+    stages = stage_gbs(pst, ctx.alpha(), n, T)
     roof = None
     if probe and probe["launches"] > 0:
         model = ALGO_BYTES[args.probe]
@@ -292,12 +366,16 @@ def main():
                       "orient_exact": st["orientExact"], "alpha_exact_compares": st["alphaExactCompares"],
                       "alpha_exact_attach": st["alphaExactAttach"]},
         "roofline": roof,
+        "stage_gbs": stages,
     }
     ctx.close()
     del d_xy
     torch.cuda.empty_cache()
+    out["host_to_host"] = host_to_host(G, xy, T)
     if args.batched_sets > 0:
         out["batched"] = batched_vs_plain(G, torch, args.batched_sets, args.plain_sample)
+        if not args.no_cpu_baseline:
+            out["batched"].update(cpu_batched(args.batched_sets, min(args.batched_sets, 1024)))
     if args.serve_qps:
         out["serving"] = serving(G, [float(q) for q in args.serve_qps.split(",")], args.serve_seconds)
     if not args.no_cpu_baseline:

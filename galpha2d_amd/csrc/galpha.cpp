@@ -486,7 +486,7 @@ static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t
     }
     if (tot != B) return false;
     (void)n;
-    c->bnd.ensure(4 * 11 * (size_t)B + 64);
+    c->bnd.ensure(4 * (size_t)((11 * (size_t)B + 3) & ~(size_t)3) + 16 * (size_t)B + 64);   // + cycle positions
     HIPCHK(hipMemcpyAsync(c->hullOff.p, off.data(), 4 * (size_t)nsets, hipMemcpyHostToDevice, st));
     int rounds = 1;
     while ((1 << rounds) < maxh) rounds++;

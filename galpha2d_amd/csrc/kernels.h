@@ -224,7 +224,8 @@ void launch_cons_insert(MeshDev &m, const int2 *cv, int nc, int *done, const int
                         int *work, int *modlist, int op, hipStream_t st);
 void launch_frame_round(MeshDev &m, unsigned int *mx, int round, int *list_out, int cnt_out_slot, int op,
                         hipStream_t st);
-// bulk frame removal (see mesh.hip); bnd = 11 * B ints of work space once B is known
+// bulk frame removal (see mesh.hip); bnd = 11 * B ints (rounded up to 4) + B double2 cycle positions of
+// work space once B is known
 void launch_bnd_detect(MeshDev &m, int *outdeg, int *cover, int *hull_cnt, int *hull_start, int *badset,
                        int *blist, int *bidx, hipStream_t st);
 void launch_bnd_rank(MeshDev &m, int B, int rounds, const int *blist, const int *bidx, const int *hull_start,

@@ -1411,7 +1411,7 @@ __global__ void k_bnd_rank_step(int B, const int *succ, const int *d, int *succ2
 // set may stay unwritten (verts -1, nxt/prv unset) and later kernels skip it
 __global__ void k_bnd_place(MeshDev m, int B, const int *blist, const int *succ, const int *d, const int *hull_cnt,
                             const int *hull_off, int *badset, int *verts, int *owner, int *nxt, int *prv,
-                            int *alive, int *pset)
+                            int *alive, int *pset, double2 *bpos)
 {
     GRID_STRIDE(k, B) {
         int a = blist[k];
@@ -1421,6 +1421,7 @@ __global__ void k_bnd_place(MeshDev m, int B, const int *blist, const int *succ,
         int pos = h - 1 - d[k];
         int p = hull_off[s] + pos;
         if (atomicExch(&verts[p], a) != -1) badset[s] = 1;
+        bpos[p] = m.pos[a];
         owner[p] = m.hedge[a];
         nxt[p] = hull_off[s] + (pos + 1 == h ? 0 : pos + 1);
         prv[p] = hull_off[s] + (pos == 0 ? h - 1 : pos - 1);
@@ -1485,7 +1486,7 @@ __global__ void k_ear_round_start(int *c)
     }
 }
 
-__global__ __launch_bounds__(256) void k_ear_cand(MeshDev m, int B, const int *verts, const int *nxt,
+__global__ __launch_bounds__(256) void k_ear_cand(MeshDev m, int B, const double2 *bpos, const int *nxt,
                                                   const int *prv, const int *alive, const int *pset,
                                                   const int *hull_cnt, const int *hull_off, int *cand,
                                                   int *ncand)
@@ -1498,16 +1499,18 @@ __global__ __launch_bounds__(256) void k_ear_cand(MeshDev m, int B, const int *v
         if (alive[p]) {
             int kp = prv[p], kn = nxt[p];
             if (kp != kn) {
-                double2 P = m.pos[verts[kp]], C = m.pos[verts[p]], N = m.pos[verts[kn]];
+                double2 P = bpos[kp], C = bpos[p], N = bpos[kn];
                 if (orient_dev(m, P, C, N) < 0) {
                     int s = pset[p], off = hull_off[s], h = hull_cnt[s];
                     double lx = fmin(P.x, fmin(C.x, N.x)), hx = fmax(P.x, fmax(C.x, N.x));
                     double ly = fmin(P.y, fmin(C.y, N.y)), hy = fmax(P.y, fmax(C.y, N.y));
                     bool blocked = false;
+                    // positions of the cycle in cycle order: one coalesced 16-byte read per lane;
+                    // the liveness test only for the few inside the ear's box
                     for (int q = off + lane; q < off + h; q += 64) {
-                        if (q == p || q == kp || q == kn || !alive[q]) continue;
-                        double2 Q = m.pos[verts[q]];
+                        double2 Q = bpos[q];
                         if (Q.x < lx || Q.x > hx || Q.y < ly || Q.y > hy) continue;
+                        if (q == p || q == kp || q == kn || !alive[q]) continue;
                         if (orient_dev(m, P, N, Q) >= 0 && orient_dev(m, N, C, Q) >= 0 && orient_dev(m, C, P, Q) >= 0)
                             blocked = true;
                     }
@@ -1572,7 +1575,9 @@ __global__ void k_ear_apply(MeshDev m, int B, const int *verts, int *nxt, int *p
     }
 }
 
-// bnd layout: succ[2][B], d[2][B], verts, owner, nxt, prv, alive, pset, cand, win
+// bnd layout: succ[2][B], d[2][B], verts, owner, nxt, prv, alive, pset, cand (win reuses d[0]), then
+// the positions of the cycle (double2[B]) from int offset bnd_pos_off(B), 16-byte aligned
+GA_INLINE_HOST_DEVICE int bnd_pos_off(int B) { return (11 * B + 3) & ~3; }
 void launch_bnd_detect(MeshDev &m, int *outdeg, int *cover, int *hull_cnt, int *hull_start, int *badset,
                        int *blist, int *bidx, hipStream_t st)
 {
@@ -1597,7 +1602,7 @@ void launch_bnd_rank(MeshDev &m, int B, int rounds, const int *blist, const int 
     }
     launch_fill_i32(verts, -1, (size_t)B, st);
     GA_LAUNCH(k_bnd_place, dim3(g), dim3(256), 0, st, m, B, blist, (const int *)succ[cur], (const int *)d[cur],
-              hull_cnt, hull_off, badset, verts, owner, nxt, prv, alive, pset);
+              hull_cnt, hull_off, badset, verts, owner, nxt, prv, alive, pset, (double2 *)(bnd + bnd_pos_off(B)));
     launch_reset_counters(m.counters, C_BND, -1, -1, -1, st);
     GA_LAUNCH(k_bnd_collinear, dim3(g), dim3(256), 0, st, m, B, (const int *)verts, (const int *)nxt,
               (const int *)prv, m.counters + C_BND);
@@ -1615,8 +1620,9 @@ void launch_ear_round(MeshDev &m, int B, int *bnd, const int *hull_cnt, const in
         *pset = bnd + 9 * B, *cand = bnd + 10 * B, *win = bnd + 2 * B;   // win reuses d[0] (ranking is done)
     GA_LAUNCH(k_ear_round_start, dim3(1), dim3(64), 0, st, m.counters);
     int gw = std::min(4096, (B + 3) / 4);   // 4 waves per block, one wave per cycle position
-    GA_LAUNCH(k_ear_cand, dim3(gw), dim3(256), 0, st, m, B, (const int *)verts, (const int *)nxt, (const int *)prv,
-              (const int *)alive, (const int *)pset, hull_cnt, hull_off, cand, m.counters + C_EAR);
+    GA_LAUNCH(k_ear_cand, dim3(gw), dim3(256), 0, st, m, B, (const double2 *)(bnd + bnd_pos_off(B)),
+              (const int *)nxt, (const int *)prv, (const int *)alive, (const int *)pset, hull_cnt, hull_off, cand,
+              m.counters + C_EAR);
     int g = std::min(4096, (B + 255) / 256);
     GA_LAUNCH(k_ear_select, dim3(g), dim3(256), 0, st, B, (const int *)nxt, (const int *)prv, (const int *)cand,
               round, win, (const int *)m.counters);

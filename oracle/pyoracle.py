@@ -95,6 +95,13 @@ def canonicalize(tris):
     return t
 
 
+def dt_alpha_points(xy):
+    """DT + alpha of one set, for a pool of worker processes; returns the number of points."""
+    tris, _ = delaunay(xy)
+    alpha(xy, tris)
+    return int(xy.shape[0])
+
+
 def validate(xy, tris, dup_of=None):
     xy = np.ascontiguousarray(xy, dtype=np.float64)
     t = np.ascontiguousarray(tris, dtype=np.int32)

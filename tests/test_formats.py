@@ -203,3 +203,29 @@ def test_cli_usage_and_errors():
     assert r.returncode == 2 and "usage" in r.stderr
     r = subprocess.run([cli, "/nonexistent.txt"], capture_output=True, text=True)
     assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+def test_render_svg_counts(oracle, tmp_path):
+    """The picture has one filled polygon per interior triangle, one bold line per regular edge,
+    one thin red line per singular edge and one dot per singular vertex (classes from the
+    definition, on the oracle's result)."""
+    import xml.etree.ElementTree as ET
+    from galpha2d_amd.visualize import render_svg
+    from test_gpu_alpha import _classes_by_definition
+    xy = load_fixture("bird.txt")
+    tris, _ = oracle.delaunay(xy)
+    a = oracle.alpha(xy, tris)
+    for r in (1, len(a["spectrum"]) // 3, len(a["spectrum"]) - 1):
+        v, e, t = _classes_by_definition(tris, a, r)
+        p = tmp_path / f"r{r}.svg"
+        render_svg(str(p), xy, tris, a, (v, e, t))
+        root = ET.parse(str(p)).getroot()
+        kinds = {}
+        for el in root:
+            key = (el.tag.split("}")[-1], el.get("class"))
+            kinds[key] = kinds.get(key, 0) + 1
+        assert kinds.get(("polygon", "interior"), 0) == int((t == 3).sum())
+        assert kinds.get(("line", "regular"), 0) == int((e == 2).sum())
+        assert kinds.get(("line", "singular"), 0) == int((e == 1).sum())
+        assert kinds.get(("circle", "singular"), 0) == int((v == 1).sum())
+        assert kinds.get(("line", "mesh"), 0) == int((e == 0).sum())

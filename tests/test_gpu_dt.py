@@ -116,10 +116,20 @@ def test_reference_api_entry_point(oracle):
     L.gpudtReleaseDTOutput(out)
     ref, _ = oracle.delaunay(xy)
     assert np.array_equal(got, ref)
-    # constrained input is rejected with a message, not a crash
+    # a constraint count without a list is an error with a message, not a crash
     p.nConstraints = 1
     assert not L.gpudtComputeDT(C.byref(p))
-    assert b"constrained" in L.gpudtGetLastError()
+    assert b"constraint list" in L.gpudtGetLastError()
+    # with a list: the constrained DT (a chord across the set, between its two extreme points)
+    c = np.array([int(np.argmin(xy[:, 0])), int(np.argmax(xy[:, 0]))], dtype=np.int32)
+    p.constraints = c.ctypes.data_as(C.POINTER(C.c_int))
+    out = L.gpudtComputeDT(C.byref(p))
+    assert out, L.gpudtGetLastError()
+    F = out.contents.nTris
+    got = np.ctypeslib.as_array(C.cast(out.contents.triangles, C.POINTER(C.c_int)), shape=(F * 9,)).reshape(F, 9).copy()
+    L.gpudtReleaseDTOutput(out)
+    assert np.array_equal(got, oracle.cdt(xy, c.reshape(1, 2))[0])
     p.nConstraints = 0
+    p.constraints = None
     p.fboSize = 1000
     assert not L.gpudtComputeDT(C.byref(p))

@@ -35,3 +35,26 @@ def test_cli_reference_files(oracle, tmp_path, name):
     ref = tmp_path / "ref.txt"
     G.dump_values(str(ref), xy, want, wa, full=True)
     assert dump.read_text() == ref.read_text()
+
+
+@pytest.mark.gpu
+def test_render_device_classes(tmp_path):
+    """The picture from the device's classes (galphaClassify) on the reference's bird input."""
+    import torch
+    import xml.etree.ElementTree as ET
+    from galpha2d_amd.visualize import render_svg
+    print("case: render bird", flush=True)
+    xy = load_fixture("bird.txt")
+    ctx = G.Context(0)
+    d = torch.from_numpy(xy).to("cuda")
+    torch.cuda.synchronize()
+    ctx.compute_device(d.data_ptr(), len(xy), 256, with_alpha=True, stream=torch.cuda.current_stream().cuda_stream)
+    tris, a = ctx.triangles(), ctx.alpha()
+    r = len(a["spectrum"]) // 2
+    v, e, t = ctx.classify(r, a)
+    ctx.close()
+    p = tmp_path / "bird.svg"
+    render_svg(str(p), xy, tris, a, (v, e, t))
+    root = ET.parse(str(p)).getroot()
+    n_poly = sum(1 for el in root if el.tag.endswith("polygon"))
+    assert n_poly == int((t == 3).sum()) > 0
