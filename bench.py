@@ -151,7 +151,7 @@ def stage_gbs(st, a, n, T):
     return out
 
 
-def batched_vs_plain(G, torch, nsets, plain_sample, seed=4):
+def batched_vs_plain(G, nsets, plain_sample, seed=4):
     """Batched workload: nsets point sets, sizes log-uniform in [256, 65536], each uniform or an
     8-cluster Gaussian mix (50/50), autotuned texture per set; DT + full alpha for every set.
     Points are resident in HBM before timing; results stay on the device.  Batched = one
@@ -164,15 +164,15 @@ def batched_vs_plain(G, torch, nsets, plain_sample, seed=4):
     sizes = np.array([x.shape[0] for x in sets], dtype=np.int64)
     offs = np.zeros(nsets + 1, dtype=np.int32)
     offs[1:] = np.cumsum(sizes)
-    d_xy = torch.from_numpy(np.ascontiguousarray(np.concatenate(sets))).to("cuda:0")
+    d_xy = G.DeviceBuffer(np.ascontiguousarray(np.concatenate(sets)))
+    sample = sets[:min(256, nsets)]
     del sets
     ctx = G.Context(0)
-    stream = torch.cuda.current_stream().cuda_stream
-    torch.cuda.synchronize()
-    base = d_xy.data_ptr()
+    stream = None   # the context's own stream
+    G.synchronize()
+    base = d_xy.ptr
     # texture autotuner on a sample (the first 256 sets), not timed; both modes use its choice
     t0 = time.perf_counter()
-    sample = [d_xy[int(offs[k]):int(offs[k + 1])].cpu().numpy() for k in range(min(256, nsets))]
     dens = ctx.autotune(sample, reps=1)
     t_tune = time.perf_counter() - t0
     del sample
@@ -181,26 +181,25 @@ def batched_vs_plain(G, torch, nsets, plain_sample, seed=4):
     for k in range(min(8, ns)):
         ctx.compute_device(base + 16 * int(offs[k]), int(sizes[k]), ctx.texture_for(int(sizes[k])),
                            with_alpha=True, stream=stream)
-    torch.cuda.synchronize()
+    G.synchronize()
     t0 = time.perf_counter()
     for k in range(ns):
         ctx.compute_device(base + 16 * int(offs[k]), int(sizes[k]), ctx.texture_for(int(sizes[k])),
                            with_alpha=True, stream=stream)
-    torch.cuda.synchronize()
+    G.synchronize()
     t_plain = time.perf_counter() - t0
     # batched: one warm-up pass (allocations), then timed passes
     ctx.batch_device(base, offs, with_alpha=True, stream=stream)
-    torch.cuda.synchronize()
+    G.synchronize()
     reps = 2
     t0 = time.perf_counter()
     for _ in range(reps):
         F = ctx.batch_device(base, offs, with_alpha=True, stream=stream)
-    torch.cuda.synchronize()
+    G.synchronize()
     t_batch = (time.perf_counter() - t0) / reps
     st = ctx.stats()
     ctx.close()
-    del d_xy
-    torch.cuda.empty_cache()
+    d_xy.free()
     plain_rate = ns / t_plain
     batch_rate = nsets / t_batch
     return {"sets": nsets, "points": int(sizes.sum()), "triangles": int(F),
@@ -297,30 +296,31 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
     import galpha2d_amd as G
     from galpha2d_amd import datagen
 
     n, T = args.n, args.texture
     xy = datagen.uniform(SEED, n)
     ctx = G.Context(0)
-    d_xy = torch.from_numpy(xy).to("cuda:0")
-    stream = torch.cuda.current_stream().cuda_stream
-    torch.cuda.synchronize()
+    # device memory and synchronization through the library's own HIP runtime
+    d_xy = G.DeviceBuffer(xy)
+    stream = None   # the context's own stream
+    G.synchronize()
     if args.copy_calibration:
-        src = torch.empty(64 * 1024 * 1024, dtype=torch.int32, device="cuda:0").fill_(1)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        src = G.DeviceBuffer(np.ones(64 * 1024 * 1024, dtype=np.int32))
+        dst = G.DeviceBuffer(nbytes=src.nbytes)
+        G.synchronize()
+        dst.copy_from(src)
+        G.synchronize()
+        src.free()
+        dst.free()
 
     def step():
-        return ctx.compute_device(d_xy.data_ptr(), n, T, with_alpha=True, stream=stream)
+        return ctx.compute_device(d_xy.ptr, n, T, with_alpha=True, stream=stream)
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    G.synchronize()
     stage_ms = {}
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -329,7 +329,7 @@ def main():
         for k, v in s.items():
             if k.startswith("ms_"):
                 stage_ms[k] = stage_ms.get(k, 0.0) + v
-    torch.cuda.synchronize()
+    G.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed / args.steps * 1e3
     value = n * args.steps / elapsed / 1e6
@@ -369,11 +369,10 @@ def main():
         "stage_gbs": stages,
     }
     ctx.close()
-    del d_xy
-    torch.cuda.empty_cache()
+    d_xy.free()
     out["host_to_host"] = host_to_host(G, xy, T)
     if args.batched_sets > 0:
-        out["batched"] = batched_vs_plain(G, torch, args.batched_sets, args.plain_sample)
+        out["batched"] = batched_vs_plain(G, args.batched_sets, args.plain_sample)
         if not args.no_cpu_baseline:
             out["batched"].update(cpu_batched(args.batched_sets, min(args.batched_sets, 1024)))
     if args.serve_qps:

@@ -8,7 +8,6 @@ shared library or a GPU is missing, every entry point raises.
 """
 import ctypes as C
 import os
-import sys
 
 import numpy as np
 
@@ -61,7 +60,8 @@ EXPORTS = ["gpudtComputeDT", "gpudtReleaseDTOutput", "gpudtGetLastError", "gpudt
            "galphaServerWait", "galphaServerTriangles", "galphaServerAlpha", "galphaServerRelease",
            "galphaServerStats", "galphaAlphaRank", "galphaClassify", "galphaClassifyDevice",
            "galphaReadPoints", "galphaReleaseParams", "galphaWritePoints", "galphaDumpValues",
-           "galphaPackResult", "galphaUnpackResult", "galphaSetConstraints"]
+           "galphaPackResult", "galphaUnpackResult", "galphaSetConstraints", "galphaDeviceCount",
+           "galphaDeviceMalloc", "galphaDeviceFree", "galphaDeviceCopy", "galphaDeviceSynchronize"]
 
 
 class _Params(C.Structure):
@@ -83,11 +83,6 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise GalphaError("libgalpha.so not built: run __graft_entry__.build() or make -C galpha2d_amd/csrc")
-    # PyTorch-ROCm wheels carry their own HIP runtime.  With both runtimes in one process, torch's
-    # has to reach the device first: once this library's runtime has opened it, torch finds none.
-    torch = sys.modules.get("torch")
-    if torch is not None and torch.cuda.is_available():
-        torch.cuda.init()
     L = C.CDLL(LIB_PATH)
     vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
     L.galphaCreate.argtypes = [C.c_int]
@@ -147,6 +142,13 @@ def lib():
     L.galphaServerRelease.restype = C.c_int
     L.galphaServerStats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_size_t)]
     L.galphaServerStats.restype = C.c_int
+    L.galphaDeviceCount.restype = C.c_int
+    L.galphaDeviceMalloc.argtypes = [C.c_int, C.c_size_t]
+    L.galphaDeviceMalloc.restype = vp
+    L.galphaDeviceFree.argtypes = [vp]
+    L.galphaDeviceCopy.argtypes = [vp, vp, C.c_size_t, C.c_int]
+    L.galphaDeviceCopy.restype = C.c_int
+    L.galphaDeviceSynchronize.restype = C.c_int
     L.galphaSetConstraints.argtypes = [vp, ip, C.c_int]
     L.galphaSetConstraints.restype = C.c_int
     L.galphaReadPoints.argtypes = [C.c_char_p, C.POINTER(C.POINTER(_Params))]
@@ -170,6 +172,60 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class DeviceBuffer:
+    """Device memory from the library's own HIP runtime, filled from a numpy array (or `nbytes`
+    uninitialised bytes): `.ptr` is the device address for the device-resident entry points."""
+
+    def __init__(self, array=None, nbytes=None, device=0):
+        self._L = lib()
+        self.array = None if array is None else np.ascontiguousarray(array)
+        self.nbytes = int(self.array.nbytes if array is not None else nbytes)
+        self.ptr = self._L.galphaDeviceMalloc(int(device), self.nbytes)
+        if not self.ptr:
+            raise GalphaError("galphaDeviceMalloc: " + self._L.gpudtGetLastError().decode())
+        if self.array is not None and self.nbytes:
+            self.upload(self.array)
+
+    def upload(self, array):
+        a = np.ascontiguousarray(array)
+        if a.nbytes > self.nbytes:
+            raise GalphaError("upload larger than the buffer")
+        if self._L.galphaDeviceCopy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes, 0) < 0:
+            raise GalphaError("galphaDeviceCopy: " + self._L.gpudtGetLastError().decode())
+
+    def download(self, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        if self._L.galphaDeviceCopy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), out.nbytes, 1) < 0:
+            raise GalphaError("galphaDeviceCopy: " + self._L.gpudtGetLastError().decode())
+        return out
+
+    def copy_from(self, other, nbytes=None):
+        """Device-to-device copy of `nbytes` (default: all of this buffer) from another buffer."""
+        nb = self.nbytes if nbytes is None else int(nbytes)
+        if self._L.galphaDeviceCopy(C.c_void_p(self.ptr), C.c_void_p(other.ptr), nb, 2) < 0:
+            raise GalphaError("galphaDeviceCopy: " + self._L.gpudtGetLastError().decode())
+
+    def free(self):
+        if self.ptr:
+            self._L.galphaDeviceFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_count():
+    return lib().galphaDeviceCount()
+
+
+def synchronize():
+    if lib().galphaDeviceSynchronize() < 0:
+        raise GalphaError(lib().gpudtGetLastError().decode())
 
 
 def auto_texture(n):
@@ -587,13 +643,11 @@ def gpudt_compute_dt(points, fbo_size, constraints=None):
 def compute_alpha_shapes(points, fbo_size):
     """Mirror of Alpha_Shapes_2D(pInput, pDelaunay).computeAlphaShapes()
     (AlphaShape/Alpha_Shapes_2D.cu:206-233): DT + alpha spectrum, intervals and master list."""
-    import ctypes
     ctx = default_context()
     xy = np.ascontiguousarray(points, dtype=np.float64)
-    import torch  # device memory for the device-resident entry point
-    d = torch.from_numpy(xy).to("cuda")
-    ctx.compute_device(d.data_ptr(), xy.shape[0], fbo_size, with_alpha=True)
+    d = DeviceBuffer(xy)
+    ctx.compute_device(d.ptr, xy.shape[0], fbo_size, with_alpha=True)
     tris = ctx.triangles()
     res = ctx.alpha()
-    del ctypes
+    d.free()
     return tris, res

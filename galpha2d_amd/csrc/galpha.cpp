@@ -1378,6 +1378,58 @@ void gpudtReleaseAlpha(PGPUDTALPHA a) { alpha_free_host(a); }
 
 void galphaFree(void *p) { free(p); }
 
+// ---- device memory for callers without a HIP runtime of their own ---------------------------
+int galphaDeviceCount(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void *galphaDeviceMalloc(int device, size_t bytes)
+{
+    try {
+        g_last_error.clear();
+        HIPCHK(hipSetDevice(device));
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, bytes ? bytes : 1));
+        return p;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return nullptr;
+    }
+}
+
+void galphaDeviceFree(void *d)
+{
+    if (d) hipFree(d);
+}
+
+// kind: 0 host -> device, 1 device -> host, 2 device -> device; synchronous
+int galphaDeviceCopy(void *dst, const void *src, size_t bytes, int kind)
+{
+    try {
+        g_last_error.clear();
+        if (kind < 0 || kind > 2) throw GalphaError("copy kind must be 0, 1 or 2");
+        hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+        if (bytes) HIPCHK(hipMemcpy(dst, src, bytes, k));
+        return 0;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return -1;
+    }
+}
+
+int galphaDeviceSynchronize(void)
+{
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        g_last_error = std::string("hipDeviceSynchronize: ") + hipGetErrorString(e);
+        return -1;
+    }
+    return 0;
+}
+
 // Batched entry point: all sets go through one pass of the pipeline on this context (every launch
 // covers every set).  Results are concatenated per input set with set-local vertex and triangle
 // indices.  tris9_out / tri_offsets_out are malloc'ed (release with galphaFree).

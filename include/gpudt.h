@@ -222,6 +222,17 @@ int galphaTextureFor(galpha_ctx *ctx, int n);
 int galphaAutotune(galpha_ctx *ctx, const double *xy, const int *offsets, int nsets, int reps,
                    double *densities_out);
 
+/* ---- device memory (no reference counterpart) -------------------------------------------
+ * For callers of the device-resident entry points that have no HIP runtime of their own (ctypes,
+ * plain C): the library's runtime allocates, copies and synchronizes.  galphaDeviceCopy kind:
+ * 0 host to device, 1 device to host, 2 device to device; synchronous.  Errors: NULL / -1 and
+ * gpudtGetLastError(). */
+int galphaDeviceCount(void);
+void *galphaDeviceMalloc(int device, size_t bytes);
+void galphaDeviceFree(void *d_ptr);
+int galphaDeviceCopy(void *dst, const void *src, size_t bytes, int kind);
+int galphaDeviceSynchronize(void);
+
 /* ---- formats ---------------------------------------------------------------------------- */
 /* Point file of the reference (main.cu:117-161, ReadDataFromFileBinary): "N", then N lines
  * "idx x y", point idx stored at points[idx]; min/max computed as there (main.cu:139-159).

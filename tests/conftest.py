@@ -15,14 +15,6 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path)")
 
 
-@pytest.fixture(autouse=True)
-def _torch_device_first(request):
-    """GPU tests: torch's HIP runtime (bundled with the wheel) opens the device before the
-    library's runtime does; in the other order torch finds no device."""
-    if request.node.get_closest_marker("gpu"):
-        import torch
-        torch.cuda.init()
-
 
 def load_fixture(name):
     """Reference input format (main.cu:117-161): N, then N lines 'idx x y'."""

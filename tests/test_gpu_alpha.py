@@ -12,13 +12,10 @@ FIELDS = ("spectrum", "tri_ro_rank", "edge_t", "edge_vopp", "edge_ro_rank", "edg
 
 
 def _alpha_gpu(xy, T):
-    import torch
     import galpha2d_amd as G
     ctx = G.Context(0)
-    d = torch.from_numpy(np.ascontiguousarray(xy)).to("cuda:0")
-    torch.cuda.synchronize()
-    ctx.compute_device(d.data_ptr(), xy.shape[0], T, with_alpha=True,
-                       stream=torch.cuda.current_stream().cuda_stream)
+    d = G.DeviceBuffer(np.ascontiguousarray(xy, dtype=np.float64))
+    ctx.compute_device(d.ptr, xy.shape[0], T, with_alpha=True)
     tris, a, st = ctx.triangles(), ctx.alpha(), ctx.stats()
     ctx.close()
     return tris, a, st
@@ -109,7 +106,6 @@ def test_alpha_classify(oracle, case):
     """Alpha-complex classes on the device (from the interval ranks) equal the classes derived
     from the definition (triangle ranks, edge ranks, adjacency), at ranks across the spectrum;
     the rank lookup returns the last rank of each spectrum value."""
-    import torch
     import galpha2d_amd as G
     print("case: alpha classification", case, flush=True)
     if case == "uniform":
@@ -119,10 +115,8 @@ def test_alpha_classify(oracle, case):
     else:
         xy, T = load_fixture("bird.txt"), 256
     ctx = G.Context(0)
-    d = torch.from_numpy(np.ascontiguousarray(xy)).to("cuda:0")
-    torch.cuda.synchronize()
-    ctx.compute_device(d.data_ptr(), xy.shape[0], T, with_alpha=True,
-                       stream=torch.cuda.current_stream().cuda_stream)
+    d = G.DeviceBuffer(np.ascontiguousarray(xy, dtype=np.float64))
+    ctx.compute_device(d.ptr, xy.shape[0], T, with_alpha=True)
     tris, a = ctx.triangles(), ctx.alpha()
     ref, _ = oracle.delaunay(xy)
     assert np.array_equal(tris, ref)
@@ -139,3 +133,28 @@ def test_alpha_classify(oracle, case):
         assert ctx.alpha_rank(spec[r]) == last
     assert ctx.alpha_rank(-1.0) == -1
     ctx.close()
+
+
+@pytest.mark.gpu
+def test_compute_on_caller_stream(oracle):
+    """galphaComputeDevice on a stream the caller owns (a PyTorch stream, with torch's device
+    memory as input) gives the result of the context's own stream."""
+    import torch
+    import galpha2d_amd as G
+    if not torch.cuda.is_available():
+        pytest.skip("torch's HIP runtime sees no GPU on this host (the library's own runtime does)")
+    print("case: caller stream", flush=True)
+    xy = oracle.gen_uniform(13, 3000)
+    want = _alpha_gpu(xy, 512)
+    ctx = G.Context(0)
+    d = torch.from_numpy(xy).to("cuda:0")
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        ctx.compute_device(d.data_ptr(), xy.shape[0], 512, with_alpha=True, stream=s.cuda_stream)
+    s.synchronize()
+    tris, a = ctx.triangles(), ctx.alpha()
+    ctx.close()
+    assert np.array_equal(tris, want[0])
+    for k in FIELDS:
+        assert np.array_equal(np.asarray(a[k]), np.asarray(want[1][k])), k

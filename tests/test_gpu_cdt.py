@@ -114,17 +114,14 @@ def test_cdt_reference_entry_point(oracle):
 def test_cdt_alpha(oracle):
     """Alpha shapes of a constrained DT: the device computes them from the constrained mesh,
     with the same results as the oracle's alpha stage on the oracle's constrained DT."""
-    import torch
     print("case: cdt alpha", flush=True)
     xy, cons = _case(oracle, "file_cells")
     want, _ = oracle.cdt(xy, cons)
     wa = oracle.alpha(xy, want)
     ctx = G.Context(0)
     ctx.set_constraints(cons)
-    d = torch.from_numpy(xy).to("cuda")
-    torch.cuda.synchronize()
-    ctx.compute_device(d.data_ptr(), len(xy), G.auto_texture(len(xy)), with_alpha=True,
-                       stream=torch.cuda.current_stream().cuda_stream)
+    d = G.DeviceBuffer(np.ascontiguousarray(xy, dtype=np.float64))
+    ctx.compute_device(d.ptr, len(xy), G.auto_texture(len(xy)), with_alpha=True)
     assert np.array_equal(ctx.triangles(), want)
     ga = ctx.alpha()
     ctx.close()

@@ -16,9 +16,7 @@ CLI = os.path.join(ROOT, "galpha2d_amd", "galpha_cli")
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["bird.txt", "alpha.txt", "1000_1.txt", "10000_1.txt"])
 def test_cli_reference_files(oracle, tmp_path, name):
-    import torch
     print("case: galpha_cli", name, flush=True)
-    torch.cuda.init()   # the test process holds the device before the driver processes use it
     pack, dump = tmp_path / "r.bin", tmp_path / "dump.txt"
     r = subprocess.run([CLI, os.path.join(GOLDEN, name), "--check", "--pack", str(pack), "--dump-full", str(dump)],
                        capture_output=True, text=True, timeout=120)
@@ -40,15 +38,13 @@ def test_cli_reference_files(oracle, tmp_path, name):
 @pytest.mark.gpu
 def test_render_device_classes(tmp_path):
     """The picture from the device's classes (galphaClassify) on the reference's bird input."""
-    import torch
     import xml.etree.ElementTree as ET
     from galpha2d_amd.visualize import render_svg
     print("case: render bird", flush=True)
     xy = load_fixture("bird.txt")
     ctx = G.Context(0)
-    d = torch.from_numpy(xy).to("cuda")
-    torch.cuda.synchronize()
-    ctx.compute_device(d.data_ptr(), len(xy), 256, with_alpha=True, stream=torch.cuda.current_stream().cuda_stream)
+    d = G.DeviceBuffer(np.ascontiguousarray(xy, dtype=np.float64))
+    ctx.compute_device(d.ptr, len(xy), 256, with_alpha=True)
     tris, a = ctx.triangles(), ctx.alpha()
     r = len(a["spectrum"]) // 2
     v, e, t = ctx.classify(r, a)

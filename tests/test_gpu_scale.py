@@ -10,13 +10,10 @@ pytestmark = pytest.mark.gpu
 
 
 def _run(xy, T):
-    import torch
     import galpha2d_amd as G
     ctx = G.Context(0)
-    d = torch.from_numpy(np.ascontiguousarray(xy)).to("cuda:0")
-    torch.cuda.synchronize()
-    ctx.compute_device(d.data_ptr(), xy.shape[0], T, with_alpha=True,
-                       stream=torch.cuda.current_stream().cuda_stream)
+    d = G.DeviceBuffer(np.ascontiguousarray(xy, dtype=np.float64))
+    ctx.compute_device(d.ptr, xy.shape[0], T, with_alpha=True)
     tris, a, st = ctx.triangles(), ctx.alpha(), ctx.stats()
     ctx.close()
     return tris, a, st
