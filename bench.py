@@ -212,7 +212,7 @@ def batched_vs_plain(G, nsets, plain_sample, seed=4):
                         f"(seed {seed}), autotuned textures, DT + alpha, device-resident"}
 
 
-def serving(G, qps_list, duration_s, seed=5, slots=4, budget_gb=96, max_group=64):
+def serving(G, qps_list, duration_s, seed=5, slots=4, budget_gb=192, max_group=64):
     """Serving workload: Poisson arrivals at a fixed rate, sizes log-uniform in [1,000, 2,000,000]
     i.i.d. uniform points (distinct windows of one seeded pool), DT + alpha per request, results
     copied to host.  Latency is measured inside the library from submit to completion."""
@@ -279,6 +279,8 @@ def main():
     ap.add_argument("--n", type=int, default=N_POINTS)
     ap.add_argument("--texture", type=int, default=TEXTURE)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-host-to-host", action="store_true",
+                    help="skip the host-to-host runs (profiled commands count only the timed workload)")
     ap.add_argument("--probe", default="k_flip_test", choices=sorted(ALGO_BYTES),
                     help="kernel timed for the roofline figure")
     ap.add_argument("--batched-sets", type=int, default=4096, help="sets of the batched workload (0: skip)")
@@ -370,7 +372,8 @@ def main():
     }
     ctx.close()
     d_xy.free()
-    out["host_to_host"] = host_to_host(G, xy, T)
+    if not args.no_host_to_host:
+        out["host_to_host"] = host_to_host(G, xy, T)
     if args.batched_sets > 0:
         out["batched"] = batched_vs_plain(G, args.batched_sets, args.plain_sample)
         if not args.no_cpu_baseline:

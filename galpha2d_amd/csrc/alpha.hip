@@ -1174,7 +1174,7 @@ PGPUDTALPHA alpha_copy_to_host(AlphaDev &d, HostStage &stage, hipStream_t st)
     A->nPoints = d.n; A->nEdges = d.E; A->nTris = d.F; A->specSize = d.specSize;
     A->masterSize = d.M; A->nSpecPointers = d.nsp;
     auto cp = [&](void **dst, int idx, size_t bytes) {
-        *dst = malloc(bytes > 0 ? bytes : 1);
+        *dst = galpha_host_alloc(bytes);
         if (!*dst) throw GalphaError("host allocation failed");
         if (bytes) stage_d2h(stage, *dst, d.b[idx].p, bytes, st);
     };
@@ -1208,7 +1208,7 @@ PGPUDTALPHA alpha_copy_set_to_host(AlphaDev &d, int set, HostStage &stage, hipSt
     A->nPoints = np; A->nEdges = ne; A->nTris = nt; A->specSize = ns;
     A->masterSize = nm; A->nSpecPointers = ns;
     auto cp = [&](void **dst, int idx, size_t elem, size_t first, size_t count) {
-        *dst = malloc(count * elem > 0 ? count * elem : 1);
+        *dst = galpha_host_alloc(count * elem);
         if (!*dst) throw GalphaError("host allocation failed");
         if (count) stage_d2h(stage, *dst, (char *)d.b[idx].p + first * elem, count * elem, st);
     };

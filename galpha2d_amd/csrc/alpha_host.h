@@ -4,6 +4,21 @@
 #include <stdexcept>
 #include <string>
 #include "../../include/gpudt.h"
+#include <cstdlib>
+#include <sys/mman.h>
+
+// Host arrays handed to callers (released with free()).  Large ones are 2 MiB aligned and marked
+// for transparent huge pages, so the first touch by the copy from the device faults once per
+// 2 MiB instead of once per 4 KiB page (the faults, not PCIe, dominate host-to-host calls).
+inline void *galpha_host_alloc(size_t bytes)
+{
+    const size_t huge = (size_t)2 << 20;
+    if (bytes < 2 * huge) return malloc(bytes ? bytes : 1);
+    void *p = nullptr;
+    if (posix_memalign(&p, huge, bytes) != 0) return nullptr;
+    madvise(p, (bytes + huge - 1) & ~(huge - 1), MADV_HUGEPAGE);
+    return p;
+}
 
 struct GalphaError : std::runtime_error {
     using std::runtime_error::runtime_error;

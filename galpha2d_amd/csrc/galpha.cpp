@@ -857,24 +857,20 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     c->vals2.ensure(4 * (size_t)nslots + 64);
     c->rot.ensure(4 * (size_t)nslots + 64);
     c->newid.ensure(4 * (size_t)nslots + 64);
-    // one 64-bit key (v0, v1, v2) needs ids below 2^21; otherwise two stable passes
-    int wide = (nsets > 1 || n > (1 << 21)) ? 1 : 0;
+    // one key (v0, v1) of 2B bits, input indices below 2^B (a batch's sets are consecutive ranges)
+    int kb = 2;
+    while ((1LL << kb) < (long long)n) kb++;
+    if (2 * kb > 64) throw GalphaError("too many points for the canonical key");
     unsigned long long *K = c->keys.as<unsigned long long>(), *K2 = c->keys2.as<unsigned long long>();
     int *vals = c->vals.as<int>(), *vals2 = c->vals2.as<int>();
-    launch_canon_keys(m, K, vals, c->rot.as<int>(), 1, wide, st);
+    launch_canon_keys(m, K, vals, c->rot.as<int>(), kb, st);
     size_t tb = 0;
-    sort_pairs_u64(K, vals, K2, vals2, nslots, 0, 64, nullptr, &tb, st);
+    sort_pairs_u64(K, vals, K2, vals2, nslots, 0, 2 * kb, nullptr, &tb, st);
     c->sortTmp.ensure(tb + 256);
     tb = c->sortTmp.bytes;
-    if (sort_pairs_u64(K, vals, K2, vals2, nslots, 0, 64, c->sortTmp.p, &tb, st)) throw GalphaError("radix sort failed");
+    if (sort_pairs_u64(K, vals, K2, vals2, nslots, 0, 2 * kb, c->sortTmp.p, &tb, st))
+        throw GalphaError("radix sort failed");
     const int *order = vals2;
-    if (wide) {
-        launch_canon_keys(m, K, vals2, c->rot.as<int>(), 2, wide, st);
-        tb = c->sortTmp.bytes;
-        if (sort_pairs_u64(K, vals2, K2, vals, nslots, 0, 64, c->sortTmp.p, &tb, st))
-            throw GalphaError("radix sort failed");
-        order = vals;
-    }
     // the live-triangle count stays on the device (folded statistic) until the final read
     c->out9.ensure(36 * (size_t)nslots + 64);
     launch_fold_stats(c->counters.as<int>(), st);
@@ -1336,7 +1332,11 @@ PGPUDTOUTPUT gpudtComputeDT(PGPUDTPARAMS p)
         if (F < 0) throw GalphaError(c->err);
         PGPUDTOUTPUT out = new GPUDTOUTPUT;
         out->nTris = F;
-        out->triangles = new gpudtTriangle[F > 0 ? F : 1];
+        out->triangles = (gpudtTriangle *)galpha_host_alloc(sizeof(gpudtTriangle) * (size_t)(F > 0 ? F : 1));
+        if (!out->triangles) {
+            delete out;
+            throw GalphaError("host allocation failed");
+        }
         if (F > 0) ctx_read(c, out->triangles, c->out9.p, 36 * (size_t)F);
         return out;
     } catch (const std::exception &e) {
@@ -1348,7 +1348,7 @@ PGPUDTOUTPUT gpudtComputeDT(PGPUDTPARAMS p)
 void gpudtReleaseDTOutput(PGPUDTOUTPUT o)
 {
     if (!o) return;
-    delete[] o->triangles;
+    free(o->triangles);
     delete o;
 }
 
@@ -1534,7 +1534,7 @@ int galphaBatch(galpha_ctx *c, const double *xy, const int *offsets, int nsets, 
                 }
             }
         }
-        *tris9_out = (int *)malloc(sizeof(int) * std::max<size_t>(all.size(), 1));
+        *tris9_out = (int *)galpha_host_alloc(sizeof(int) * std::max<size_t>(all.size(), 1));
         *tri_offsets_out = (int *)malloc(sizeof(int) * (nsets + 1));
         if (!*tris9_out || !*tri_offsets_out) throw GalphaError("host allocation failed");
         memcpy(*tris9_out, all.data(), sizeof(int) * all.size());

@@ -235,9 +235,8 @@ void launch_ear_round(MeshDev &m, int B, int *bnd, const int *hull_cnt, const in
                       int *list_out, int cnt_out_slot, int op, hipStream_t st);
 void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, int *hull_cnt, int *hull_off,
                       int *hull_start, void *scan_tmp, hipStream_t st);
-// canonical order: one 64-bit pass when wide == 0 (single set, ids < 2^21), else two passes
-void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot, int pass, int wide,
-                       hipStream_t st);
+// canonical order: key (v0 << B) | v1 of input indices below 2^B, sorted on its low 2B bits
+void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot, int B, hipStream_t st);
 void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, const int *dF, int *out9,
                        int *set_first, int *rset, hipStream_t st);
 int sort_pairs_u64(unsigned long long *keys_in, int *vals_in, unsigned long long *keys_out, int *vals_out,

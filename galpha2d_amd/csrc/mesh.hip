@@ -1631,10 +1631,11 @@ void launch_ear_round(MeshDev &m, int B, int *bnd, const int *hull_cnt, const in
 }
 
 // ---- canonical output ----------------------------------------------------------------------------
-// Output order: set, then (v0, v1, v2) with v0 the smallest vertex.  wide == 0 (one set, vertex ids
-// below 2^21): one 64-bit key (v0, v1, v2).  wide == 1: two stable passes, pass 1 on (v1, v2), pass 2
-// on (set, v0) over the order of pass 1 (order1 = vals of pass 1).  Dead slots sort last.
-__global__ void k_canon_keys(MeshDev m, unsigned long long *keys, int *vals, int *rot, int wide)
+// Output order: (v0, v1, v2) of input indices with v0 the smallest vertex; input indices of the sets
+// of a batch are consecutive ranges in set order, so this is also set order.  Every directed edge
+// belongs to one triangle, so (v0, v1) alone is unique: the key is v0 << B | v1 (indices below 2^B),
+// sorted on its low 2B bits.  Dead slots get all ones there, above every live key (v0 <= n - 3).
+__global__ void k_canon_keys(MeshDev m, unsigned long long *keys, int *vals, int *rot, int B)
 {
     int n = ntri_used(m);
     int alive = 0;
@@ -1648,29 +1649,15 @@ __global__ void k_canon_keys(MeshDev m, unsigned long long *keys, int *vals, int
         if (o.y < ic(o, r)) r = 1;
         if (o.z < ic(o, r)) r = 2;
         rot[t] = r;
-        unsigned long long a = (unsigned)ic(o, r), b = (unsigned)ic(o, (r + 1) % 3), c = (unsigned)ic(o, (r + 2) % 3);
-        keys[t] = wide ? ((b << 32) | c) : ((a << 42) | (b << 21) | c);
+        unsigned long long a = (unsigned)ic(o, r), b = (unsigned)ic(o, (r + 1) % 3);
+        keys[t] = (a << B) | b;
     }
     stat_add_full(m.stats, C_RECON_TRIS, alive);
 }
 
-__global__ void k_canon_keys2(MeshDev m, const int *order1, const int *rot, unsigned long long *keys)
+void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot, int B, hipStream_t st)
 {
-    int n = ntri_used(m);
-    GRID_STRIDE(r, n) {
-        int t = order1[r];
-        int4 v = m.tv[t];
-        if (v.x == GA_DEAD) { keys[r] = ~0ull; continue; }
-        int a = ic(v, rot[t]);
-        keys[r] = ((unsigned long long)m.vset[a] << 32) | (unsigned)oid(m, a);
-    }
-}
-
-void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot, int pass, int wide,
-                       hipStream_t st)
-{
-    if (pass == 1) GA_LAUNCH(k_canon_keys, dim3(FIXED_GRID), dim3(256), 0, st, m, keys, vals, rot, wide);
-    else GA_LAUNCH(k_canon_keys2, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)vals, (const int *)rot, keys);
+    GA_LAUNCH(k_canon_keys, dim3(FIXED_GRID), dim3(256), 0, st, m, keys, vals, rot, B);
 }
 
 // newid[slot] = output position << 2 | rotation of the slot's triangle: one gather per neighbour

@@ -46,7 +46,8 @@ struct Request {
     bool done = false;
     int status = 0;                    // >= 0 triangles, < 0 failure
     std::string err;
-    std::vector<int> tris9;
+    std::unique_ptr<int, void (*)(void *)> tris9{nullptr, free};   // 9 ints per triangle (galpha_host_alloc)
+    int ntris = 0;
     PGPUDTALPHA alpha = nullptr;
     int group = 0;                     // requests in the pass that served it
 };
@@ -152,8 +153,9 @@ static void worker_main(galpha_server *S, int w)
             if (rc < 0) { r->status = -1; r->err = err; continue; }
             int nt = galphaBatchTriangles(ctx, (int)k, nullptr, 0);
             if (nt >= 0) {
-                r->tris9.resize(9 * (size_t)nt);
-                nt = galphaBatchTriangles(ctx, (int)k, r->tris9.data(), nt);
+                r->tris9.reset((int *)galpha_host_alloc(36 * (size_t)nt));
+                nt = r->tris9 ? galphaBatchTriangles(ctx, (int)k, r->tris9.get(), nt) : -1;
+                r->ntris = nt;
             }
             if (nt < 0) { r->status = -1; r->err = galphaLastError(ctx); continue; }
             r->status = nt;
@@ -252,10 +254,10 @@ int galphaServerTriangles(galpha_server *S, long long ticket, int *tris9, int ca
     auto it = S->tickets.find(ticket);
     if (it == S->tickets.end() || !it->second->done || it->second->status < 0) return -1;
     auto &r = it->second;
-    int F = (int)(r->tris9.size() / 9);
+    int F = r->ntris;
     if (!tris9) return F;
     if (capacity < F) return -1;
-    std::copy(r->tris9.begin(), r->tris9.end(), tris9);
+    if (F > 0) memcpy(tris9, r->tris9.get(), 36 * (size_t)F);
     return F;
 }
 

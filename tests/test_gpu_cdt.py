@@ -181,3 +181,31 @@ def test_cdt_rejected_by_batches(oracle):
     with pytest.raises(G.GalphaError, match="single-set"):
         ctx.batch([xy, xy])
     ctx.close()
+
+
+@pytest.mark.gpu
+def test_cdt_long_rays(oracle):
+    """16 rays from the point nearest the centre of 200,000 uniform points to points near the
+    border: each crosses hundreds of triangles (one large cavity per ray, rays share an endpoint).
+    The device result equals the oracle's and satisfies the definition."""
+    print("case: cdt long rays", flush=True)
+    xy = oracle.gen_uniform(23, 200000)
+    c = int(np.argmin(((xy - 0.5) ** 2).sum(1)))
+    ang = np.arctan2(xy[:, 1] - xy[c, 1], xy[:, 0] - xy[c, 0])
+    far = np.nonzero(((xy - 0.5) ** 2).sum(1) > 0.2)[0]
+    picks = []
+    for k in range(16):   # one far point per angular sector, so rays never overlap
+        lo, hi = -np.pi + k * np.pi / 8, -np.pi + (k + 1) * np.pi / 8
+        sel = far[(ang[far] > lo + 0.05) & (ang[far] < hi - 0.05)]
+        picks.append(int(sel[0]))
+    cons = np.array([(c, p) for p in picks], dtype=np.int32)
+    ctx = G.Context(0)
+    ctx.set_constraints(cons)
+    a = ctx.delaunay(xy)
+    st = ctx.stats()
+    ctx.close()
+    want, dup = oracle.cdt(xy, cons)
+    assert np.array_equal(a, want)
+    rc, msg = oracle.validate_constrained(xy, a, dup, cons)
+    assert rc == 0, msg
+    print(f"rays: rounds {st['constraintRounds']}, {st['ms_constraints']:.2f} ms", flush=True)
