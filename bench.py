@@ -212,7 +212,7 @@ def batched_vs_plain(G, nsets, plain_sample, seed=4):
                         f"(seed {seed}), autotuned textures, DT + alpha, device-resident"}
 
 
-def serving(G, qps_list, duration_s, seed=5, slots=4, budget_gb=192, max_group=64):
+def serving(G, qps_list, duration_s, seed=5, slots=4, budget_gb=192, max_group=64, max_group_points=2_000_000):
     """Serving workload: Poisson arrivals at a fixed rate, sizes log-uniform in [1,000, 2,000,000]
     i.i.d. uniform points (distinct windows of one seeded pool), DT + alpha per request, results
     copied to host.  Latency is measured inside the library from submit to completion."""
@@ -221,7 +221,8 @@ def serving(G, qps_list, duration_s, seed=5, slots=4, budget_gb=192, max_group=6
     from galpha2d_amd import datagen
     rng = np.random.default_rng(seed)
     pool = datagen.uniform(seed, 2_000_000 + 1)
-    srv = G.Server(0, slots=slots, hbm_budget_bytes=budget_gb << 30, max_group=max_group)
+    srv = G.Server(0, slots=slots, hbm_budget_bytes=budget_gb << 30, max_group=max_group,
+                   max_group_points=max_group_points)
     # warm-up: one large and a few small requests (buffer growth)
     for n in (2_000_000, 300_000, 3000):
         t = srv.submit(pool[:n], with_alpha=True)
@@ -258,14 +259,21 @@ def serving(G, qps_list, duration_s, seed=5, slots=4, budget_gb=192, max_group=6
         wall = time.perf_counter() - t0
         lat = np.array([r[0] for r in res])
         grp = np.array([r[2] for r in res])
+        by_size = {}
+        for name, lo, hi in (("<100K", 0, 100_000), ("100K-1M", 100_000, 1_000_000), (">=1M", 1_000_000, 1 << 62)):
+            sel = (sizes >= lo) & (sizes < hi)
+            if sel.any():
+                by_size[name] = {"requests": int(sel.sum()), "p50_ms": round(float(np.percentile(lat[sel], 50)), 2),
+                                 "p99_ms": round(float(np.percentile(lat[sel], 99)), 2)}
         out.append({"qps_offered": qps, "requests": nreq, "points": int(sizes.sum()),
                     "sets_per_s": round(nreq / wall, 1), "p50_ms": round(float(np.percentile(lat, 50)), 2),
                     "p99_ms": round(float(np.percentile(lat, 99)), 2),
                     "mean_queue_ms": round(float(np.mean([r[1] for r in res])), 2),
-                    "mean_group": round(float(grp.mean()), 2)})
+                    "mean_group": round(float(grp.mean()), 2), "by_size": by_size})
     st = srv.stats()
     srv.close()
     return {"runs": out, "slots": slots, "hbm_budget_gb": budget_gb, "max_group": max_group,
+            "max_group_points": max_group_points,
             "held_gb_end": round(st["held_bytes"] / 2**30, 2),
             "workload": f"Poisson arrivals, sizes log-uniform in [1000, 2000000] uniform points (seed {seed}), "
                         f"DT + alpha, results copied to host"}

@@ -61,7 +61,8 @@ EXPORTS = ["gpudtComputeDT", "gpudtReleaseDTOutput", "gpudtGetLastError", "gpudt
            "galphaServerStats", "galphaAlphaRank", "galphaClassify", "galphaClassifyDevice",
            "galphaReadPoints", "galphaReleaseParams", "galphaWritePoints", "galphaDumpValues",
            "galphaPackResult", "galphaUnpackResult", "galphaSetConstraints", "galphaDeviceCount",
-           "galphaDeviceMalloc", "galphaDeviceFree", "galphaDeviceCopy", "galphaDeviceSynchronize"]
+           "galphaDeviceMalloc", "galphaDeviceFree", "galphaDeviceCopy", "galphaDeviceSynchronize",
+           "galphaStreamCreate", "galphaStreamDestroy", "galphaStreamSynchronize", "galphaServerSetGroupPoints"]
 
 
 class _Params(C.Structure):
@@ -142,6 +143,8 @@ def lib():
     L.galphaServerRelease.restype = C.c_int
     L.galphaServerStats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_size_t)]
     L.galphaServerStats.restype = C.c_int
+    L.galphaServerSetGroupPoints.argtypes = [vp, C.c_longlong]
+    L.galphaServerSetGroupPoints.restype = C.c_int
     L.galphaDeviceCount.restype = C.c_int
     L.galphaDeviceMalloc.argtypes = [C.c_int, C.c_size_t]
     L.galphaDeviceMalloc.restype = vp
@@ -149,6 +152,12 @@ def lib():
     L.galphaDeviceCopy.argtypes = [vp, vp, C.c_size_t, C.c_int]
     L.galphaDeviceCopy.restype = C.c_int
     L.galphaDeviceSynchronize.restype = C.c_int
+    L.galphaStreamCreate.argtypes = [C.c_int]
+    L.galphaStreamCreate.restype = vp
+    L.galphaStreamDestroy.argtypes = [vp]
+    L.galphaStreamDestroy.restype = C.c_int
+    L.galphaStreamSynchronize.argtypes = [vp]
+    L.galphaStreamSynchronize.restype = C.c_int
     L.galphaSetConstraints.argtypes = [vp, ip, C.c_int]
     L.galphaSetConstraints.restype = C.c_int
     L.galphaReadPoints.argtypes = [C.c_char_p, C.POINTER(C.POINTER(_Params))]
@@ -215,6 +224,31 @@ class DeviceBuffer:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    """A HIP stream of the library's own runtime (`.handle` for the `stream` arguments)."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        self.handle = self._L.galphaStreamCreate(int(device))
+        if not self.handle:
+            raise GalphaError("galphaStreamCreate: " + self._L.gpudtGetLastError().decode())
+
+    def synchronize(self):
+        if self._L.galphaStreamSynchronize(C.c_void_p(self.handle)) < 0:
+            raise GalphaError(self._L.gpudtGetLastError().decode())
+
+    def close(self):
+        if self.handle:
+            self._L.galphaStreamDestroy(C.c_void_p(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 
@@ -559,11 +593,13 @@ class Server:
     """Request loop on one device: `slots` contexts with their own streams, admission by HBM
     footprint, continuous batching of queued requests (see include/gpudt.h, galphaServer*)."""
 
-    def __init__(self, device=0, slots=4, hbm_budget_bytes=64 << 30, max_group=64):
+    def __init__(self, device=0, slots=4, hbm_budget_bytes=64 << 30, max_group=64, max_group_points=0):
         self._L = lib()
         self._h = self._L.galphaServerCreate(int(device), int(slots), int(hbm_budget_bytes), int(max_group))
         if not self._h:
             raise GalphaError("galphaServerCreate failed")
+        if max_group_points and self._L.galphaServerSetGroupPoints(self._h, int(max_group_points)) < 0:
+            raise GalphaError("galphaServerSetGroupPoints failed")
 
     def submit(self, xy, fbo_size=0, with_alpha=True):
         """Queues one request; returns its ticket."""

@@ -1420,6 +1420,36 @@ int galphaDeviceCopy(void *dst, const void *src, size_t bytes, int kind)
     }
 }
 
+void *galphaStreamCreate(int device)
+{
+    try {
+        g_last_error.clear();
+        HIPCHK(hipSetDevice(device));
+        hipStream_t s = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));   // default priority
+        return (void *)s;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return nullptr;
+    }
+}
+
+int galphaStreamDestroy(void *stream)
+{
+    if (!stream) return 0;
+    return hipStreamDestroy((hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+
+int galphaStreamSynchronize(void *stream)
+{
+    hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) {
+        g_last_error = std::string("hipStreamSynchronize: ") + hipGetErrorString(e);
+        return -1;
+    }
+    return 0;
+}
+
 int galphaDeviceSynchronize(void)
 {
     hipError_t e = hipDeviceSynchronize();

@@ -153,8 +153,8 @@ struct MeshDev {
 #define C_PLIST 30            // entries of the pending-vertex list (MeshDev::plist)
 #define C_EAR_PREV 31         // ear candidates of the previous round
 #define C_STATUS 32           // device status word (GA_ERR_* bits); MeshDev::status points here
-#define C_CONS_PEND 33        // constraints still to insert (this round's walk)
-#define C_CONS_MOD 34         // slots rewritten by this round's constraint insertions
+#define C_CONS_PEND 33        // constraints still to insert (the current insertion round's walk)
+#define C_CONS_MOD 34         // slots rewritten by the current insertion round
 // status bits of constraint insertion (the other GA_ERR_* bits are in dev_common.h)
 #define GA_ERR_CONS_POINT 128     // a point lies inside a constraint segment
 #define GA_ERR_CONS_CROSS 256     // two constraints cross

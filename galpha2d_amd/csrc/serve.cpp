@@ -5,7 +5,8 @@
 // kept resident and reused across requests).  Requests are queued in arrival order.  A free
 // worker admits the longest prefix of the queue that fits the HBM budget -- the bytes that all
 // contexts hold, with its own holdings grown to what the admitted requests need -- up to
-// `max_group` requests, and runs them as one batched pass (continuous batching: requests that
+// `max_group` requests (and, when set, up to a number of points), and runs them as one batched
+// pass (continuous batching: requests that
 // arrive while others are on the GPU join the next pass of a free slot).  When the head request
 // does not fit, an idle worker gives back its own buffers and waits for other slots to finish;
 // a request larger than the whole budget is refused at submission.  Results (canonical triangles, alpha arrays) are copied to host
@@ -56,6 +57,8 @@ struct Request {
 
 struct galpha_server {
     int device = 0, slots = 0, max_group = 0;
+    long long max_points = 0;          // points admitted into one pass (0: no limit); a larger
+                                       // request still runs, alone
     size_t budget = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -103,6 +106,7 @@ static void worker_main(galpha_server *S, int w)
                     long long npts = 0, pix = 0;
                     for (auto &r : S->queue) {
                         if ((int)grp.size() >= S->max_group) break;
+                        if (!grp.empty() && S->max_points > 0 && npts + r->n > S->max_points) break;
                         size_t nd = group_footprint(npts + r->n, pix + (long long)r->T * r->T);
                         size_t mine = std::max(S->held[w], nd);
                         if (others + mine > S->budget) break;
@@ -282,6 +286,16 @@ int galphaServerRelease(galpha_server *S, long long ticket)
     if (it == S->tickets.end() || !it->second->done) return -1;
     if (it->second->alpha) gpudtReleaseAlpha(it->second->alpha);
     S->tickets.erase(it);
+    return 0;
+}
+
+// Largest number of points one pass admits (0: no limit).  Short passes free slots sooner, so
+// small requests wait less behind large ones; a request above the limit still runs, alone.
+int galphaServerSetGroupPoints(galpha_server *S, long long max_points)
+{
+    if (!S || max_points < 0) return -1;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->max_points = max_points;
     return 0;
 }
 

@@ -204,6 +204,9 @@ int galphaServerTriangles(galpha_server *server, long long ticket, int *tris9, i
 int galphaServerAlpha(galpha_server *server, long long ticket, PGPUDTALPHA *pAlpha);
 int galphaServerRelease(galpha_server *server, long long ticket);
 int galphaServerStats(galpha_server *server, long long *passes, long long *served, size_t *held_bytes);
+/* At most max_points points per pass (0: no limit, the default); a larger request still runs,
+ * alone.  Shorter passes free slots sooner, so small requests wait less behind large ones. */
+int galphaServerSetGroupPoints(galpha_server *server, long long max_points);
 
 /* Kernel timing probe: while enabled, HIP events are recorded on the launch stream around every
  * launch of the named kernel; NULL disables.  Result: number of launches and total ms. */
@@ -232,6 +235,10 @@ void *galphaDeviceMalloc(int device, size_t bytes);
 void galphaDeviceFree(void *d_ptr);
 int galphaDeviceCopy(void *dst, const void *src, size_t bytes, int kind);
 int galphaDeviceSynchronize(void);
+/* a HIP stream of the library's runtime (non-blocking, default priority) to pass as `stream` */
+void *galphaStreamCreate(int device);
+int galphaStreamDestroy(void *stream);
+int galphaStreamSynchronize(void *stream);
 
 /* ---- formats ---------------------------------------------------------------------------- */
 /* Point file of the reference (main.cu:117-161, ReadDataFromFileBinary): "N", then N lines

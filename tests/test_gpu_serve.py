@@ -91,3 +91,30 @@ def test_oversize_request_refused(oracle):
     with pytest.raises(G.GalphaError):
         srv.submit(oracle.gen_uniform(1, 100000))
     srv.close()
+
+
+def test_group_points_cap(oracle):
+    """With a cap on points per pass, no pass holds more than the cap unless it holds a single
+    request, and every request's result is the one it gets alone."""
+    import galpha2d_amd as G
+    print("case: serving with 40,000 points per pass", flush=True)
+    reqs = _requests(oracle)
+    cap = 40000
+    srv = G.Server(0, slots=2, hbm_budget_bytes=24 << 30, max_group=8, max_group_points=cap)
+    tickets = [srv.submit(xy, with_alpha=True) for xy in reqs]
+    groups = []
+    for t in tickets:
+        F, lat, q, g = srv.wait(t)
+        groups.append(g)
+    mixed = [srv.result(t) for t in tickets]
+    srv.close()
+    for xy, g in zip(reqs, groups):
+        if xy.shape[0] > cap:
+            assert g == 1, "a request above the cap shares its pass"
+    assert max(groups) > 1
+    solo = G.Server(0, slots=1, hbm_budget_bytes=24 << 30, max_group=1)
+    for xy, m in zip(reqs, mixed):
+        t = solo.submit(xy, with_alpha=True)
+        solo.wait(t)
+        _same(solo.result(t), m)
+    solo.close()
