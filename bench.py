@@ -373,6 +373,7 @@ def main():
                    "triangles": st["nTris"], "spectrum": st["specSize"]},
         "stage_ms": {k: round(v / args.steps, 3) for k, v in stage_ms.items()},
         "precision": {"incircle_tests": st["incircleTests"], "incircle_exact": st["incircleExact"],
+                      "incircle_exact_rate": round(st["incircleExact"] / max(1, st["incircleTests"]), 8),
                       "orient_exact": st["orientExact"], "alpha_exact_compares": st["alphaExactCompares"],
                       "alpha_exact_attach": st["alphaExactAttach"]},
         "roofline": roof,

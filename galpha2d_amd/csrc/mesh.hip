@@ -1940,8 +1940,12 @@ GA_INLINE void cons_boundary(const MeshDev &m, int T, int k, int side, int out)
 // E(i) = X[i] (or X[M + 1 - i] when rev); bs / bo: cavity side and outside neighbour of the
 // boundary edge between E(i) and E(i + 1) at index i (M - i when rev).  New triangles take slots
 // from slots[*s]; returns the triangle on the base edge E(0) E(M + 1) (its side 2).
+// Run by one thread (width 1) or one wave (width 64).  In a wave every lane runs the same
+// sequential control flow on the same values (the stack in stk is written with identical values
+// by all lanes, and each lane reads its own writes); only the choice of the chain vertex is split
+// across lanes, and only lane 0 writes the mesh.
 __device__ int cons_fill(const MeshDev &m, const int *X, int M, bool rev, const int *bs, const int *bo,
-                         const int *slots, int *s, int *stk)
+                         const int *slots, int *s, int *stk, int lane, int width)
 {
 #define CE(i) (rev ? X[M + 1 - (i)] : X[(i)])
 #define CB(i) (rev ? M - (i) : (i))
@@ -1952,66 +1956,100 @@ __device__ int cons_fill(const MeshDev &m, const int *X, int M, bool rev, const 
         sp--;
         int lo = stk[3 * sp], hi = stk[3 * sp + 1], pe = stk[3 * sp + 2];
         int x = CE(lo - 1), y = CE(hi + 1);
-        int ci = lo;
-        for (int i = lo + 1; i <= hi; i++)
-            if (incircle_dev(m, x, y, CE(ci), CE(i)) > 0) ci = i;
+        // the chain vertex whose circle through x and y holds no other chain vertex: the minimum
+        // of the order "inside the circle of" (the circles through x and y on one side are
+        // nested, and the tie-break makes the order strict).  Each lane scans a strided part,
+        // a butterfly combines them, and every lane ends with the index a sequential scan finds.
+        int ci = -1;
+        for (int i = lo + lane; i <= hi; i += width)
+            if (ci < 0 || incircle_dev(m, x, y, CE(ci), CE(i)) > 0) ci = i;
+        if (width == 64)
+            for (int d = 32; d > 0; d >>= 1) {
+                int o = __shfl_xor(ci, d, 64);
+                if (o >= 0 && (ci < 0 || incircle_dev(m, x, y, CE(ci), CE(o)) > 0)) ci = o;
+            }
         int cv = CE(ci);
         int T = slots[(*s)++];
-        m.tv[T] = make_int4(x, y, cv, 0);
-        if (orient_dev(m, m.pos[x], m.pos[y], m.pos[cv]) <= 0) atomicOr(m.status, GA_ERR_CONS_TRI);
-        if (pe >= 0) {
-            set_link(m, T, 2, pe);
-            set_link(m, enc_t(pe), enc_j(pe), enc(T, 2));
-        } else {
-            top = T;
+        if (lane == 0) {
+            m.tv[T] = make_int4(x, y, cv, 0);
+            if (orient_dev(m, m.pos[x], m.pos[y], m.pos[cv]) <= 0) atomicOr(m.status, GA_ERR_CONS_TRI);
+            if (pe >= 0) {
+                set_link(m, T, 2, pe);
+                set_link(m, enc_t(pe), enc_j(pe), enc(T, 2));
+            }
+            // side 1: edge (cv, x); side 0: edge (y, cv)
+            if (ci == lo) cons_boundary(m, T, 1, bs[CB(lo - 1)], bo[CB(lo - 1)]);
+            if (ci == hi) cons_boundary(m, T, 0, bs[CB(hi)], bo[CB(hi)]);
+            m.vt[x] = T;
+            m.vt[y] = T;
+            m.vt[cv] = T;
         }
-        // side 1: edge (cv, x); side 0: edge (y, cv)
+        if (pe < 0) top = T;
         if (ci > lo) {
             stk[3 * sp] = lo; stk[3 * sp + 1] = ci - 1; stk[3 * sp + 2] = enc(T, 1);
             sp++;
-        } else {
-            cons_boundary(m, T, 1, bs[CB(lo - 1)], bo[CB(lo - 1)]);
         }
         if (ci < hi) {
             stk[3 * sp] = ci + 1; stk[3 * sp + 1] = hi; stk[3 * sp + 2] = enc(T, 0);
             sp++;
-        } else {
-            cons_boundary(m, T, 0, bs[CB(hi)], bo[CB(hi)]);
         }
-        m.vt[x] = T;
-        m.vt[y] = T;
-        m.vt[cv] = T;
     }
 #undef CE
 #undef CB
     return top;
 }
 
-__global__ void k_cons_commit(MeshDev m, const int2 *cv, int nc, int *done, const int *cnt, const int *roff,
-                              int *work, int *modlist, int op)
+// Cavities below CONS_WAVE_K triangles: one thread per constraint; larger ones: one wave per
+// constraint (wave-uniform control flow throughout).  A slot belongs to at most one winning
+// cavity, so the two kernels never touch the same slots.
+#define CONS_WAVE_K 32
+
+__device__ void cons_commit_one(const MeshDev &m, int c, int K, int *done, const int *roff, int *work,
+                                int *modlist, int op, int lane, int width)
+{
+    ConsRegion R(work + roff[c], K);
+    if (R.hdr[0] != K) return;   // collect failed (status set)
+    unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)c);
+    bool lost = false;
+    for (int i = lane; i < K; i += width) lost |= m.claim[R.slots[i]] != key;
+    if (width == 64 ? __ballot(lost) != 0 : lost) return;
+    int ml = R.hdr[1], mr = R.hdr[2];
+    // outside neighbours of the boundary, read before any slot is rewritten (every lane writes
+    // the whole list, so it reads back only its own writes)
+    for (int i = 0; i <= ml; i++) R.ol[i] = ic(m.tn[enc_t(R.lb[i])], enc_j(R.lb[i]));
+    for (int i = 0; i <= mr; i++) R.orr[i] = ic(m.tn[enc_t(R.rb[i])], enc_j(R.rb[i]));
+    for (int i = lane; i < K; i += width) m.stamp[R.slots[i]] = op;
+    int s = 0;
+    int tl = cons_fill(m, R.Lx, ml, false, R.lb, R.ol, R.slots, &s, R.stk, lane, width);
+    int tr = cons_fill(m, R.Rx, mr, true, R.rb, R.orr, R.slots, &s, R.stk, lane, width);
+    int q = 0;
+    if (lane == 0) {
+        set_link(m, tl, 2, enc(tr, 2));   // the constraint edge (a, b)
+        set_link(m, tr, 2, enc(tl, 2));
+        q = atomicAdd(&m.counters[C_CONS_MOD], K);
+        done[c] = 1;
+    }
+    if (width == 64) q = __shfl(q, 0, 64);
+    for (int i = lane; i < K; i += width) modlist[q + i] = R.slots[i];
+}
+
+__global__ void k_cons_commit_small(MeshDev m, int nc, int *done, const int *cnt, const int *roff, int *work,
+                                    int *modlist, int op)
 {
     GRID_STRIDE(c, nc) {
         int K = cnt[c];
-        if (!K) continue;
-        ConsRegion R(work + roff[c], K);
-        if (R.hdr[0] != K) continue;   // collect failed (status set)
-        unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)c);
-        bool win = true;
-        for (int i = 0; i < K && win; i++) win = m.claim[R.slots[i]] == key;
-        if (!win) continue;
-        int ml = R.hdr[1], mr = R.hdr[2];
-        // outside neighbours of the boundary, read before any slot is rewritten
-        for (int i = 0; i <= ml; i++) R.ol[i] = ic(m.tn[enc_t(R.lb[i])], enc_j(R.lb[i]));
-        for (int i = 0; i <= mr; i++) R.orr[i] = ic(m.tn[enc_t(R.rb[i])], enc_j(R.rb[i]));
-        for (int i = 0; i < K; i++) m.stamp[R.slots[i]] = op;
-        int s = 0;
-        int tl = cons_fill(m, R.Lx, ml, false, R.lb, R.ol, R.slots, &s, R.stk);
-        int tr = cons_fill(m, R.Rx, mr, true, R.rb, R.orr, R.slots, &s, R.stk);
-        set_link(m, tl, 2, enc(tr, 2));   // the constraint edge (a, b)
-        set_link(m, tr, 2, enc(tl, 2));
-        int q = atomicAdd(&m.counters[C_CONS_MOD], K);
-        for (int i = 0; i < K; i++) modlist[q + i] = R.slots[i];
-        done[c] = 1;
+        if (K > 0 && K < CONS_WAVE_K) cons_commit_one(m, c, K, done, roff, work, modlist, op, 0, 1);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cons_commit_wave(MeshDev m, int nc, int *done, const int *cnt,
+                                                          const int *roff, int *work, int *modlist, int op)
+{
+    int lane = threadIdx.x & 63;
+    int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
+    for (int c = wave; c < nc; c += nw) {
+        int K = cnt[c];
+        if (K >= CONS_WAVE_K) cons_commit_one(m, c, K, done, roff, work, modlist, op, lane, 64);
     }
 }
 
@@ -2032,6 +2070,7 @@ void launch_cons_insert(MeshDev &m, const int2 *cv, int nc, int *done, const int
                         int *work, int *modlist, int op, hipStream_t st)
 {
     GA_LAUNCH(k_cons_collect, dim3(FIXED_GRID), dim3(256), 0, st, m, cv, nc, cnt, roff, work, op);
-    GA_LAUNCH(k_cons_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, cv, nc, done, cnt, roff, work, modlist, op);
+    GA_LAUNCH(k_cons_commit_small, dim3(FIXED_GRID), dim3(256), 0, st, m, nc, done, cnt, roff, work, modlist, op);
+    GA_LAUNCH(k_cons_commit_wave, dim3(FIXED_GRID), dim3(256), 0, st, m, nc, done, cnt, roff, work, modlist, op);
     GA_LAUNCH(k_relink, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)modlist, C_CONS_MOD, op);
 }
