@@ -1504,26 +1504,17 @@ __global__ __launch_bounds__(256) void k_ear_cand(MeshDev m, int B, const double
                     int s = pset[p], off = hull_off[s], h = hull_cnt[s];
                     double lx = fmin(P.x, fmin(C.x, N.x)), hx = fmax(P.x, fmax(C.x, N.x));
                     double ly = fmin(P.y, fmin(C.y, N.y)), hy = fmax(P.y, fmax(C.y, N.y));
-                    // positions of the cycle in chunks of 64 (one coalesced 16-byte read per lane),
-                    // starting around p, where the vertices of a pocket that block its ears lie;
-                    // the wave stops at the first chunk with a blocker.  The liveness test runs
-                    // only for the few inside the ear's box.
                     bool blocked = false;
-                    int start = ((p - off) - 32) % h;
-                    if (start < 0) start += h;   // in [0, h): every index below stays in the cycle
-                    for (int k0 = 0; k0 < h && !blocked; k0 += 64) {
-                        bool hit = false;
-                        if (k0 + lane < h) {
-                            int q = off + (start + k0 + lane) % h;
-                            double2 Q = bpos[q];
-                            if (!(Q.x < lx || Q.x > hx || Q.y < ly || Q.y > hy) && q != p && q != kp && q != kn &&
-                                alive[q] && orient_dev(m, P, N, Q) >= 0 && orient_dev(m, N, C, Q) >= 0 &&
-                                orient_dev(m, C, P, Q) >= 0)
-                                hit = true;
-                        }
-                        blocked = __ballot(hit) != 0;
+                    // positions of the cycle in cycle order: one coalesced 16-byte read per lane;
+                    // the liveness test only for the few inside the ear's box
+                    for (int q = off + lane; q < off + h; q += 64) {
+                        double2 Q = bpos[q];
+                        if (Q.x < lx || Q.x > hx || Q.y < ly || Q.y > hy) continue;
+                        if (q == p || q == kp || q == kn || !alive[q]) continue;
+                        if (orient_dev(m, P, N, Q) >= 0 && orient_dev(m, N, C, Q) >= 0 && orient_dev(m, C, P, Q) >= 0)
+                            blocked = true;
                     }
-                    c = blocked ? 0 : 1;
+                    c = __ballot(blocked) ? 0 : 1;
                 }
             }
         }
