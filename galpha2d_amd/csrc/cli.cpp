@@ -83,6 +83,10 @@ int check(const PGPUDTPARAMS in, const PGPUDTOUTPUT out)
                 if (errors++ < 10) printf("*** ERROR *** vertex %d is not in the triangulation\n", v);
         }
     }
+    if (errors) {   // the geometric checks below follow the links: only on a sound structure
+        printf("check: %d triangles, %d structural errors\n", F, errors);
+        return 1;
+    }
     for (int t = 0; t < F; t++) {
         int o = orient_filtered(P[T[t].vtx[0]], P[T[t].vtx[1]], P[T[t].vtx[2]]);
         if (o < 0) {

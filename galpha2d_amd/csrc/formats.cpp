@@ -254,11 +254,16 @@ int galphaDumpValues(const char *path, PGPUDTPARAMS pParams, PGPUDTOUTPUT pDT, P
             fprintf(f.f, " %.17g", A->spectrum[r]);
         };
         if (full) {
-            const gpudtVertex *P = pParams->points;
+            const gpudtVertex *Pts = pParams->points;
             const gpudtTriangle *T = pDT->triangles;
+            const int np = pParams->nPoints;
+            auto P = [&](int v) -> const gpudtVertex & {
+                if (v < 0 || v >= np) throw GalphaError("triangle vertex " + std::to_string(v) + " outside the points");
+                return Pts[v];
+            };
             fprintf(f.f, "VERTICES\n");
             for (int v = 0; v < A->nPoints; v++) {
-                fprintf(f.f, "%.17g %.17g", P[v].x, P[v].y);
+                fprintf(f.f, "%.17g %.17g", P(v).x, P(v).y);
                 val(A->vert_nu_min_rank[v]);
                 val(A->vert_nu_max_rank[v]);
                 fputc('\n', f.f);
@@ -268,7 +273,7 @@ int galphaDumpValues(const char *path, PGPUDTPARAMS pParams, PGPUDTOUTPUT pDT, P
                 int t = A->edge_t[e], i = A->edge_vopp[e];
                 if (t < 0 || t >= pDT->nTris || i < 0 || i > 2) throw GalphaError("edge " + std::to_string(e) + " is invalid");
                 int p1 = T[t].vtx[(i + 1) % 3], p2 = T[t].vtx[(i + 2) % 3];
-                fprintf(f.f, "%.17g %.17g %.17g %.17g", P[p1].x, P[p1].y, P[p2].x, P[p2].y);
+                fprintf(f.f, "%.17g %.17g %.17g %.17g", P(p1).x, P(p1).y, P(p2).x, P(p2).y);
                 val(A->edge_ro_rank[e]);
                 val(A->edge_nu_min_rank[e]);
                 val(A->edge_nu_max_rank[e]);
@@ -276,7 +281,7 @@ int galphaDumpValues(const char *path, PGPUDTPARAMS pParams, PGPUDTOUTPUT pDT, P
             }
             fprintf(f.f, "TRIANGLES\n");
             for (int t = 0; t < A->nTris; t++) {
-                for (int k = 0; k < 3; k++) fprintf(f.f, k ? " %.17g %.17g" : "%.17g %.17g", P[T[t].vtx[k]].x, P[T[t].vtx[k]].y);
+                for (int k = 0; k < 3; k++) fprintf(f.f, k ? " %.17g %.17g" : "%.17g %.17g", P(T[t].vtx[k]).x, P(T[t].vtx[k]).y);
                 val(A->tri_ro_rank[t]);
                 fputc('\n', f.f);
             }
