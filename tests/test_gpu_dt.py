@@ -3,7 +3,7 @@ adjacency, plus size-independent validity checks at full scale."""
 import numpy as np
 import pytest
 
-from conftest import load_fixture, lattice_clusters
+from conftest import ALPHA_FIELDS, load_fixture, lattice_clusters
 
 pytestmark = pytest.mark.gpu
 
@@ -133,3 +133,64 @@ def test_reference_api_entry_point(oracle):
     p.constraints = None
     p.fboSize = 1000
     assert not L.gpudtComputeDT(C.byref(p))
+
+
+def _degenerate_cases(oracle):
+    """Exactly co-circular integer points (x^2 + y^2 = 25^2, 65^2), a fan over collinear points,
+    a large offset, a tiny scale, a full integer grid and the grid with duplicates."""
+    out = {}
+    r = 25
+    pts = sorted({(x, y) for x in range(-r, r + 1) for y in range(-r, r + 1) if x * x + y * y == r * r})
+    out["circle25"] = np.array(pts, dtype=np.float64)
+    out["circle25_center"] = np.array(pts + [(0, 0), (1, 2)], dtype=np.float64)
+    r = 65
+    pts65 = sorted({(x, y) for x in range(-r, r + 1) for y in range(-r, r + 1) if x * x + y * y == r * r})
+    out["circle65_two"] = np.array(pts65 + [(x // 5, y // 5) for x, y in pts if (x // 5, y // 5) not in pts65],
+                                   dtype=np.float64)
+    out["fan"] = np.array([(i, 0.0) for i in range(40)] + [(19.5, 7.0)], dtype=np.float64)
+    u = oracle.gen_uniform(91, 3000)
+    out["offset"] = u + 1.0e6
+    out["scaled_tiny"] = u * 1e-9
+    g = np.array([(i, j) for i in range(30) for j in range(30)], dtype=np.float64)
+    out["grid_full"] = g
+    out["grid_dups"] = np.concatenate([g, g[::7]])
+    return out
+
+
+def _same_alpha(a, ra, name):
+    for k in ALPHA_FIELDS:
+        g, r = np.asarray(a[k]), np.asarray(ra[k])
+        assert g.shape == r.shape, (name, k)
+        if k == "spectrum":
+            assert np.array_equal(g.view(np.uint64), r.view(np.uint64)), (name, "spectrum differs in ulps")
+        else:
+            assert np.array_equal(g, r), (name, k)
+
+
+def test_degenerate_geometry(oracle):
+    """Every case through the single-set path and all of them through one batch: triangles and
+    every alpha array bit-exact against the oracle."""
+    import galpha2d_amd as G
+    cases = _degenerate_cases(oracle)
+    ctx = G.Context(0)
+    refs = {}
+    try:
+        for name, xy in cases.items():
+            print("case: degenerate", name, xy.shape[0], flush=True)
+            ref, _ = oracle.delaunay(xy)
+            ra = oracle.alpha(xy, ref)
+            refs[name] = (ref, ra)
+            d = G.DeviceBuffer(np.ascontiguousarray(xy))
+            ctx.compute_device(d.ptr, xy.shape[0], G.auto_texture(xy.shape[0]), with_alpha=True)
+            tris, a = ctx.triangles(), ctx.alpha()
+            d.free()
+            assert np.array_equal(tris, ref), name
+            _same_alpha(a, ra, name)
+        print("case: degenerate batch", len(cases), flush=True)
+        got = ctx.batch(list(cases.values()), with_alpha=True)
+        for k, name in enumerate(cases):
+            ref, ra = refs[name]
+            assert np.array_equal(got[k], ref), name
+            _same_alpha(ctx.batch_alpha(k), ra, name)
+    finally:
+        ctx.close()
