@@ -87,14 +87,20 @@ __device__ __noinline__ int incircle_slow(double2 a, int ia, double2 b, int ib, 
     return s;
 }
 
-GA_INLINE int incircle_dev(const MeshDev &m, int ia, int ib, int ic_, int id)
+// a, b, c, d: the positions of vertices ia, ib, ic_, id, already loaded by the caller
+GA_INLINE int incircle_at(const MeshDev &m, double2 a, double2 b, double2 c, double2 d, int ia, int ib,
+                          int ic_, int id)
 {
-    double2 a = m.pos[ia], b = m.pos[ib], c = m.pos[ic_], d = m.pos[id];
     int s = incircle_filter(a, b, c, d);
     if (s != 0) return s;
     stat_inc(m.stats, C_EXACT_INCIRCLE, true);
     // the symbolic tie-break ranks vertices by input index
     return incircle_slow(a, oid(m, ia), b, oid(m, ib), c, oid(m, ic_), d, oid(m, id), m.status);
+}
+
+GA_INLINE int incircle_dev(const MeshDev &m, int ia, int ib, int ic_, int id)
+{
+    return incircle_at(m, m.pos[ia], m.pos[ib], m.pos[ic_], m.pos[id], ia, ib, ic_, id);
 }
 
 // ---- reconstruction --------------------------------------------------------------------
@@ -943,18 +949,37 @@ __global__ __launch_bounds__(256) void k_flip_test(MeshDev m, const int *list, i
             int4 tt = m.tv[t];
             if (tt.x == GA_DEAD) continue;
             int4 nn = m.tn[t];
+            // The edges to test first, then every gather they need before any predicate: the
+            // corner positions once, the three opposite vertices, then their positions, so the
+            // three dependent chains are in flight together instead of one after another behind
+            // the claim atomics.
+            int uu[3];
+#pragma unroll
             for (int i = 0; i < 3; i++) {
                 int e = ic(nn, i);
+                uu[i] = -1;
                 if (e < 0) continue;
                 int u = enc_t(e);
                 if (m.stamp[u] == prev_op && u < t) continue;
                 if (m.chash && cons_has(m, ic(tt, p1(i)), ic(tt, m1(i)))) continue;   // constraint edge
-                int d = ic(m.tv[u], enc_j(e));
+                uu[i] = u;
+            }
+            if ((uu[0] & uu[1] & uu[2]) < 0) continue;   // no edge to test
+            int dd[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) dd[i] = uu[i] >= 0 ? ic(m.tv[uu[i]], enc_j(ic(nn, i))) : 0;
+            double2 pa = m.pos[tt.x], pb = m.pos[tt.y], pc = m.pos[tt.z];
+            double2 pd[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) pd[i] = uu[i] >= 0 ? m.pos[dd[i]] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                if (uu[i] < 0) continue;
                 tests++;
-                if (incircle_dev(m, tt.x, tt.y, tt.z, d) > 0) {
+                if (incircle_at(m, pa, pb, pc, pd[i], tt.x, tt.y, tt.z, dd[i]) > 0) {
                     unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)(t * 3 + i));
                     atomicMax(&m.claim[t], key);
-                    atomicMax(&m.claim[u], key);
+                    atomicMax(&m.claim[uu[i]], key);
                     bits |= 1u << (3 * j + i);
                 }
             }
