@@ -167,18 +167,50 @@ struct AlphaK {
 #define A_CLUSTERS 3
 #define A_MAXCLUSTER 4
 #define A_BIGCLUSTER 5
+#define A_BADMESH 6      // caller triangles that fail k_mesh_check
 
 GA_INLINE int tv(const AlphaK &a, int t, int i) { return a.tris[(size_t)t * 9 + 3 + i]; }
 GA_INLINE int tn(const AlphaK &a, int t, int i) { return a.tris[(size_t)t * 9 + i]; }
 GA_INLINE int pl1(int i) { return i == 2 ? 0 : i + 1; }
 GA_INLINE int mi1(int i) { return i == 0 ? 2 : i - 1; }
 
+// Every edge is listed once, by its owner: the triangle in which it runs from the smaller vertex
+// id to the larger, or the only triangle of a hull edge.  Edges of one owner are consecutive,
+// in slot order, from eoff[owner].
+GA_INLINE bool edge_owned(const AlphaK &a, int t, int i)
+{
+    return tv(a, t, pl1(i)) < tv(a, t, mi1(i)) || tn(a, t, i) == -1;
+}
+
+// Caller triangles, before any kernel follows a link: three distinct vertex ids in [0, n), and
+// every neighbour either -1 or a slot of a triangle in [0, F) that links back over the same two
+// vertices in the other direction.  Counts the triangles that fail in A_BADMESH.
+__global__ void k_mesh_check(AlphaK a)
+{
+    GS(t, a.F) {
+        bool bad = false;
+        int v[3];
+        for (int i = 0; i < 3; i++) {
+            v[i] = tv(a, t, i);
+            bad |= v[i] < 0 || v[i] >= a.n;
+        }
+        bad |= v[0] == v[1] || v[1] == v[2] || v[2] == v[0];
+        for (int i = 0; i < 3 && !bad; i++) {
+            int nb = tn(a, t, i);
+            if (nb == -1) continue;
+            int u = nb >> 2, j = nb & 3;
+            if (nb < 0 || u >= a.F || j > 2) { bad = true; break; }
+            bad |= tn(a, u, j) != t * 4 + i || tv(a, u, pl1(j)) != v[mi1(i)] || tv(a, u, mi1(j)) != v[pl1(i)];
+        }
+        if (bad) atomicAdd(&a.counters[A_BADMESH], 1);
+    }
+}
+
 __global__ void k_edge_count(AlphaK a)
 {
     GS(t, a.F) {
         int c = 0;
-        for (int i = 0; i < 3; i++)
-            c += (tv(a, t, pl1(i)) < tv(a, t, mi1(i))) || tn(a, t, i) == -1;
+        for (int i = 0; i < 3; i++) c += edge_owned(a, t, i);
         a.cnt[t] = c;
     }
 }
@@ -188,7 +220,7 @@ __global__ void k_edges(AlphaK a)
     GS(t, a.F) {
         int e = a.eoff[t];
         for (int i = 0; i < 3; i++)
-            if ((tv(a, t, pl1(i)) < tv(a, t, mi1(i))) || tn(a, t, i) == -1) {
+            if (edge_owned(a, t, i)) {
                 a.edge_t[e] = t;
                 a.edge_v[e] = i;
                 e++;
@@ -878,12 +910,14 @@ static void big_clusters(AlphaDev &d, AlphaK &a, int S0, hipStream_t st)
     GA_LAUNCH(k_big_fix, dim3(AGRID), dim3(256), 0, st, a, b, Mb, (const int *)closef, (const int *)inv, claimed);
 }
 
-void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F, hipStream_t st, galpha_stats *stats)
+void alpha_run(AlphaDev &d, const double *d_xy, int n, const int *d_tris9, int F, hipStream_t st, galpha_stats *stats,
+               bool check_mesh)
 {
     AlphaBatchInfo one;
     one.nsets = 1;
     one.poff = {0, n};
     one.tfirst = {0, F};
+    one.check_mesh = check_mesh;
     alpha_run_batch(d, d_xy, n, d_tris9, F, one, st, stats);
 }
 
@@ -926,12 +960,18 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     a.xy = (const double2 *)d_xy; a.tris = d_tris9; a.n = n; a.F = F;
     a.cnt = AP<int>(d, AB_CNT); a.eoff = AP<int>(d, AB_EOFF);
     a.status = AP<int>(d, AB_STATUS); a.counters = AP<int>(d, AB_CNTRS);
+    if (info.check_mesh) GA_LAUNCH(k_mesh_check, dim3(AGRID), dim3(256), 0, st, a);
     GA_LAUNCH(k_edge_count, dim3(AGRID), dim3(256), 0, st, a);
     launch_exclusive_scan(a.cnt, a.eoff, F, d.b[AB_SCAN].p, st);
-    int h2[2];
+    int h2[3] = {0, 0, 0};
     chk(hipMemcpyAsync(&h2[0], a.eoff + F - 1, 4, hipMemcpyDeviceToHost, st), "copy");
     chk(hipMemcpyAsync(&h2[1], a.cnt + F - 1, 4, hipMemcpyDeviceToHost, st), "copy");
+    if (info.check_mesh) chk(hipMemcpyAsync(&h2[2], a.counters + A_BADMESH, 4, hipMemcpyDeviceToHost, st), "copy");
     chk(hipStreamSynchronize(st), "sync");
+    if (h2[2])
+        throw GalphaError("alpha: the triangles are not a mesh of the points (" + std::to_string(h2[2]) +
+                          " triangles with a vertex or neighbour out of range, or a neighbour link that does not "
+                          "link back over the same edge)");
     int E = h2[0] + h2[1];
     d.E = E;
     size_t NE = (size_t)F + E;

@@ -52,6 +52,7 @@ struct AlphaBatchInfo {
     const int *vset = nullptr;     // device [n]: set of every real point
     std::vector<int> poff;         // [nsets + 1] first real point of every set
     std::vector<int> tfirst;       // [nsets + 1] first triangle of every set
+    bool check_mesh = false;       // caller triangles: reject ids and links that are not a mesh
 };
 
 struct AlphaDev {
@@ -64,7 +65,7 @@ struct AlphaDev {
 };
 
 void alpha_run(AlphaDev &a, const double *d_xy, int n, const int *d_tris9, int F, hipStream_t st,
-               galpha_stats *stats);
+               galpha_stats *stats, bool check_mesh = false);
 void alpha_run_batch(AlphaDev &a, const double *d_xy, int n, const int *d_tris9, int F, const AlphaBatchInfo &info,
                      hipStream_t st, galpha_stats *stats);
 PGPUDTALPHA alpha_copy_to_host(AlphaDev &a, HostStage &stage, hipStream_t st);

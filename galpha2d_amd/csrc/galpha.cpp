@@ -1361,11 +1361,15 @@ int gpudtComputeAlpha(PGPUDTPARAMS p, PGPUDTOUTPUT dt, PGPUDTALPHA *pA)
         if (!c) throw GalphaError(g_last_error);
         std::lock_guard<std::mutex> lk(g_default_mu);
         int n = p->nPoints;
+        if (n < 0 || dt->nTris < 0 || dt->nTris >= (1 << 29))
+            throw GalphaError("alpha: point or triangle count out of range");
+        if ((n > 0 && !p->points) || (dt->nTris > 0 && !dt->triangles))
+            throw GalphaError("alpha: null point or triangle array");
         c->xy.ensure(16 * (size_t)std::max(n, 1));
         if (n > 0) stage_h2d(c->stage, c->xy.p, p->points, 16 * (size_t)n, c->stream);
         c->out9.ensure(36 * (size_t)std::max(dt->nTris, 1));
         if (dt->nTris > 0) stage_h2d(c->stage, c->out9.p, dt->triangles, 36 * (size_t)dt->nTris, c->stream);
-        alpha_run(c->alpha, c->xy.as<double>(), n, c->out9.as<int>(), dt->nTris, c->stream, &c->stats);
+        alpha_run(c->alpha, c->xy.as<double>(), n, c->out9.as<int>(), dt->nTris, c->stream, &c->stats, true);
         *pA = alpha_copy_to_host(c->alpha, c->stage, c->stream);
         return 0;
     } catch (const std::exception &e) {

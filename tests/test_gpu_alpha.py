@@ -170,3 +170,49 @@ def test_compute_on_caller_stream(oracle):
     else:
         print("torch's HIP runtime sees no GPU here: PyTorch stream part not run", flush=True)
     ctx.close()
+
+
+def test_alpha_of_caller_triangles(oracle):
+    """gpudtComputeAlpha over triangles the caller passes in: the oracle's DT gives the oracle's
+    alpha result; triangles that are not a mesh of the points are an error with a message."""
+    print("case: gpudtComputeAlpha with caller triangles", flush=True)
+    import ctypes as C
+    import galpha2d_amd as G
+    L = G.lib()
+    L.gpudtComputeAlpha.argtypes = [C.POINTER(G._Params), C.POINTER(G._Output), C.POINTER(C.POINTER(G._Alpha))]
+    L.gpudtComputeAlpha.restype = C.c_int
+    xy = np.ascontiguousarray(load_fixture("1000_1.txt"))
+    n = xy.shape[0]
+    prm = G._Params(0, 0, 1, 1, n, xy.ctypes.data_as(C.POINTER(C.c_double)), 0, None, 512)
+    ref, _ = oracle.delaunay(xy)
+
+    def run(tris):
+        tris = np.ascontiguousarray(tris, dtype=np.int32)
+        out = G._Output(tris.ctypes.data_as(C.POINTER(C.c_int)), tris.shape[0])
+        A = C.POINTER(G._Alpha)()
+        if L.gpudtComputeAlpha(C.byref(prm), C.byref(out), C.byref(A)) < 0:
+            return L.gpudtGetLastError().decode()
+        try:
+            return G._alpha_to_numpy(A)
+        finally:
+            L.gpudtReleaseAlpha(A)
+
+    got, ra = run(ref), oracle.alpha(xy, ref)
+    for k in FIELDS:
+        g, r = np.asarray(got[k]), np.asarray(ra[k])
+        if k == "spectrum":
+            assert np.array_equal(g.view(np.uint64), r.view(np.uint64)), k
+        else:
+            assert np.array_equal(g, r), k
+    t = 10 if ref[10, 0] >= 0 else int(np.argmax(ref[:, 0] >= 0))
+    bad = {}
+    b = ref.copy(); b[5, 3] = n; bad["vertex out of range"] = b
+    b = ref.copy(); b[5, 4] = b[5, 3]; bad["repeated vertex"] = b
+    b = ref.copy(); b[t, 0] = 4 * ref.shape[0]; bad["neighbour out of range"] = b
+    b = ref.copy(); b[t, 0] = b[t, 0] ^ 1 if (b[t, 0] & 3) != 2 else b[t, 0] - 2; bad["link that does not link back"] = b
+    for name, b in bad.items():
+        msg = run(b)
+        assert isinstance(msg, str) and "not a mesh" in msg, (name, msg)
+    # the context still works after a rejected call
+    got = run(ref)
+    assert np.array_equal(np.asarray(got["vert_nu_min"]), np.asarray(ra["vert_nu_min"]))

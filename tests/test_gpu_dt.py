@@ -148,6 +148,11 @@ def _degenerate_cases(oracle):
     out["circle65_two"] = np.array(pts65 + [(x // 5, y // 5) for x, y in pts if (x // 5, y // 5) not in pts65],
                                    dtype=np.float64)
     out["fan"] = np.array([(i, 0.0) for i in range(40)] + [(19.5, 7.0)], dtype=np.float64)
+    # vertices of degree 2000: an apex over 2000 collinear points (open fan, on the hull) and
+    # the centre of a ring of 2000 (closed fan)
+    out["fan_2000"] = np.array([(i, 0.0) for i in range(2000)] + [(999.5, 7.0)], dtype=np.float64)
+    th = 2 * np.pi * np.arange(2000) / 2000
+    out["wheel_2000"] = np.concatenate([np.stack([np.cos(th), np.sin(th)], 1), [[0.001, 0.002]]])
     u = oracle.gen_uniform(91, 3000)
     out["offset"] = u + 1.0e6
     out["scaled_tiny"] = u * 1e-9
