@@ -96,6 +96,11 @@ typedef struct {
     int *spec_pointers;      /* [nSpecPointers] [0] = -1, then first master index of each rank */
 } GPUDTALPHA, *PGPUDTALPHA;
 
+/* Replaces Alpha_Shapes_2D(pInput, pDT) + computeAlphaShapes() (AlphaShape/Alpha_Shapes_2D.cu:58-74,
+ * 206-233).  pDT may be any triangle array of the points in the gpudtComputeDT layout: it is checked
+ * on the device first (distinct vertex ids in [0, nPoints), neighbours -1 or a triangle slot that
+ * links back over the same edge), and one that fails returns -1 with a message, before any kernel
+ * follows its links.  Returns 0 and sets *pAlpha (free with gpudtReleaseAlpha). */
 int gpudtComputeAlpha(PGPUDTPARAMS pParam, PGPUDTOUTPUT pDT, PGPUDTALPHA *pAlpha);
 void gpudtReleaseAlpha(PGPUDTALPHA pAlpha);
 
