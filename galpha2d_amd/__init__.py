@@ -62,7 +62,8 @@ EXPORTS = ["gpudtComputeDT", "gpudtReleaseDTOutput", "gpudtGetLastError", "gpudt
            "galphaReadPoints", "galphaReleaseParams", "galphaWritePoints", "galphaDumpValues",
            "galphaPackResult", "galphaUnpackResult", "galphaSetConstraints", "galphaDeviceCount",
            "galphaDeviceMalloc", "galphaDeviceFree", "galphaDeviceCopy", "galphaDeviceSynchronize",
-           "galphaStreamCreate", "galphaStreamDestroy", "galphaStreamSynchronize", "galphaServerSetGroupPoints"]
+           "galphaStreamCreate", "galphaStreamDestroy", "galphaStreamSynchronize", "galphaServerSetGroupPoints",
+           "galphaServerFootprint"]
 
 
 class _Params(C.Structure):
@@ -145,6 +146,8 @@ def lib():
     L.galphaServerStats.restype = C.c_int
     L.galphaServerSetGroupPoints.argtypes = [vp, C.c_longlong]
     L.galphaServerSetGroupPoints.restype = C.c_int
+    L.galphaServerFootprint.argtypes = [C.c_int, C.c_int]
+    L.galphaServerFootprint.restype = C.c_size_t
     L.galphaDeviceCount.restype = C.c_int
     L.galphaDeviceMalloc.argtypes = [C.c_int, C.c_size_t]
     L.galphaDeviceMalloc.restype = vp
@@ -589,6 +592,11 @@ class Context:
 _default = None
 
 
+def server_footprint(n, fbo_size=0):
+    """Device bytes the serving admission rule counts for one request of n points."""
+    return int(lib().galphaServerFootprint(int(n), int(fbo_size)))
+
+
 class Server:
     """Request loop on one device: `slots` contexts with their own streams, admission by HBM
     footprint, continuous batching of queued requests (see include/gpudt.h, galphaServer*)."""
@@ -597,7 +605,7 @@ class Server:
         self._L = lib()
         self._h = self._L.galphaServerCreate(int(device), int(slots), int(hbm_budget_bytes), int(max_group))
         if not self._h:
-            raise GalphaError("galphaServerCreate failed")
+            raise GalphaError("galphaServerCreate failed: " + self._L.gpudtGetLastError().decode())
         if max_group_points and self._L.galphaServerSetGroupPoints(self._h, int(max_group_points)) < 0:
             raise GalphaError("galphaServerSetGroupPoints failed")
 

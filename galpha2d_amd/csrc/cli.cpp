@@ -5,12 +5,14 @@
 //   galpha_cli POINTS.txt [--fbo N] [--no-alpha] [--check] [--repeat K]
 //              [--dump FILE] [--dump-full FILE] [--pack FILE]
 //
-// --check runs the structural part of main.cu's checkResult (main.cu:163-343) on the host: every
-// neighbour link is symmetric, every point that is not an exact duplicate is a vertex, and the
-// orientation and the empty-circle property hold wherever the FP64 filter is certain (Shewchuk's
-// static error bounds).  Cases the filter leaves open are counted, not decided: the library's own
-// predicates are exact, so this check is a sanity check of the output, not a proof.
+// --check runs main.cu's checkResult (main.cu:163-343) on the host, walking the result with the
+// mesh-navigation macros of include/gpudt.h (navcheck.h): vertex indices, every point that is not
+// an exact duplicate is a vertex, sym across every edge, the vertex fans (onext / oprev), Euler's
+// formula, and the orientation and empty-circle properties wherever the FP64 filter is certain
+// (Shewchuk's static error bounds).  Cases the filter leaves open are counted, not decided: the
+// library's own predicates are exact, so this check is a sanity check of the output, not a proof.
 #include "../../include/gpudt.h"
+#include "navcheck.h"
 
 #include <algorithm>
 #include <chrono>
@@ -23,92 +25,14 @@
 
 namespace {
 
-const double EPS = 1.1102230246251565e-16;   // 2^-53
-const double CCW_BOUND = (3.0 + 16.0 * EPS) * EPS;
-const double ICC_BOUND = (10.0 + 96.0 * EPS) * EPS;
-
-// +1 / -1 when the FP64 sign is certain, 0 when the filter cannot decide
-int orient_filtered(const gpudtVertex &a, const gpudtVertex &b, const gpudtVertex &c)
-{
-    double l = (a.x - c.x) * (b.y - c.y), r = (a.y - c.y) * (b.x - c.x), det = l - r;
-    double bound = CCW_BOUND * (std::fabs(l) + std::fabs(r));
-    return det > bound ? 1 : (det < -bound ? -1 : 0);
-}
-
-int incircle_filtered(const gpudtVertex &a, const gpudtVertex &b, const gpudtVertex &c, const gpudtVertex &d)
-{
-    double adx = a.x - d.x, ady = a.y - d.y, bdx = b.x - d.x, bdy = b.y - d.y, cdx = c.x - d.x, cdy = c.y - d.y;
-    double bc = bdx * cdy - cdx * bdy, ca = cdx * ady - adx * cdy, ab = adx * bdy - bdx * ady;
-    double al = adx * adx + ady * ady, bl = bdx * bdx + bdy * bdy, cl = cdx * cdx + cdy * cdy;
-    double det = al * bc + bl * ca + cl * ab;
-    double perm = (std::fabs(bdx * cdy) + std::fabs(cdx * bdy)) * al + (std::fabs(cdx * ady) + std::fabs(adx * cdy)) * bl +
-                  (std::fabs(adx * bdy) + std::fabs(bdx * ady)) * cl;
-    double bound = ICC_BOUND * perm;
-    return det > bound ? 1 : (det < -bound ? -1 : 0);
-}
-
 int check(const PGPUDTPARAMS in, const PGPUDTOUTPUT out)
 {
-    int F = out->nTris, n = in->nPoints, errors = 0;
-    long long uncertain = 0;
-    const gpudtTriangle *T = out->triangles;
-    const gpudtVertex *P = in->points;
-    std::vector<char> used(n, 0);
-    for (int t = 0; t < F; t++)
-        for (int i = 0; i < 3; i++) {
-            int v = T[t].vtx[i];
-            if (v < 0 || v >= n) {
-                printf("*** ERROR *** triangle %d has vertex %d outside [0, %d)\n", t, v, n);
-                return 1;
-            }
-            used[v] = 1;
-            int nb = T[t].tri[i];
-            if (nb < 0) continue;
-            int u = nb >> 2, j = nb & 3;
-            if (u >= F || j > 2 || T[u].tri[j] != (t << 2 | i)) {
-                if (errors++ < 10) printf("*** ERROR *** neighbour link %d.%d is not symmetric\n", t, i);
-            }
-        }
-    // a point may only be missing when an earlier point has the same coordinates
-    {
-        std::vector<int> idx(n);
-        for (int i = 0; i < n; i++) idx[i] = i;
-        std::sort(idx.begin(), idx.end(), [&](int a, int b) {
-            return P[a].x != P[b].x ? P[a].x < P[b].x : (P[a].y != P[b].y ? P[a].y < P[b].y : a < b);
-        });
-        for (int k = 0; k < n; k++) {
-            int v = idx[k];
-            bool dup = k > 0 && P[idx[k - 1]].x == P[v].x && P[idx[k - 1]].y == P[v].y;
-            if (!used[v] && !dup && n >= 3)
-                if (errors++ < 10) printf("*** ERROR *** vertex %d is not in the triangulation\n", v);
-        }
-    }
-    if (errors) {   // the geometric checks below follow the links: only on a sound structure
-        printf("check: %d triangles, %d structural errors\n", F, errors);
-        return 1;
-    }
-    for (int t = 0; t < F; t++) {
-        int o = orient_filtered(P[T[t].vtx[0]], P[T[t].vtx[1]], P[T[t].vtx[2]]);
-        if (o < 0) {
-            if (errors++ < 10) printf("*** ERROR *** wrong orientation at triangle %d\n", t);
-        } else if (o == 0) {
-            uncertain++;
-        }
-        for (int i = 0; i < 3; i++) {
-            int nb = T[t].tri[i];
-            if (nb < 0) continue;
-            int far = T[nb >> 2].vtx[nb & 3];
-            int s = incircle_filtered(P[T[t].vtx[0]], P[T[t].vtx[1]], P[T[t].vtx[2]], P[far]);
-            if (s > 0) {
-                if (errors++ < 10) printf("*** ERROR *** edge %d.%d is not locally Delaunay\n", t, i);
-            } else if (s == 0) {
-                uncertain++;
-            }
-        }
-    }
-    printf("check: %d triangles, %d errors, %lld predicate evaluations left open by the FP64 filter\n", F, errors,
-           uncertain);
-    return errors ? 1 : 0;
+    NavReport r = galpha_nav_check(in->points, in->nPoints, out->triangles, out->nTris);
+    printf("check: %d triangles, %d errors, %lld predicate evaluations left open by the FP64 filter\n",
+           out->nTris, r.errors, r.open);
+    printf("check: V = %d, E = %d (%d on the hull), F = %d + 1, Euler characteristic %d, fan corners %lld\n",
+           r.vertices, r.edges, r.hullEdges, out->nTris, r.euler, r.fanCorners);
+    return r.errors ? 1 : 0;
 }
 
 bool write_file(const char *path, const std::vector<char> &b)

@@ -254,6 +254,19 @@ static void check_texture(int T)
     if (T < 64 || T > 16384 || (T & (T - 1))) throw GalphaError("fboSize must be a power of two in [64, 16384]");
 }
 
+// Triangle slots are named by (t << 2) | j in int32 links, so slot ids stay below 2^29.  A pass
+// reserves 3 V + 8192 slots (V = real points + frame sites of all its sets): at most 178,954,240
+// vertices per pass.  Checked from the sizes alone, before any device work touches the points.
+static const long long kMaxSlots = 1LL << 29;
+static long long frame_count(int T) { return (long long)frame_pixels(T).size(); }
+static void check_slot_capacity(long long V)
+{
+    if (3 * V + 8192 > kMaxSlots)
+        throw GalphaError("too many points in one pass: " + std::to_string(V) + " vertices need " +
+                          std::to_string(3 * V + 8192) + " triangle slots, and slot ids (t << 2 | j) must stay "
+                          "below 2^29 (at most 178954240 vertices, frame sites included)");
+}
+
 // smallest power-of-two side in [256, 16384] with at least `density` pixels per point
 static int texture_for_density(int n, double density)
 {
@@ -320,7 +333,7 @@ static BatchPlan plan_batch(const int *n, const int *T, const double *bb5, int n
         roff += T[s];
         toff += (size_t)T[s] * T[s];
     }
-    if (poff + nftot > 0x7fffffffLL / 4) throw GalphaError("batch too large (vertex ids)");
+    check_slot_capacity(poff + nftot);
     b.ntot = (int)poff;
     b.V = (int)(poff + nftot);
     b.sumT = (int)roff;
@@ -981,6 +994,11 @@ static int batch_run(galpha_ctx *c, const double *xy, const int *offsets, int ns
         total += n;
     }
     if (total > 0x7fffffffLL / 8) throw GalphaError("batch too large");
+    {
+        long long V = total;
+        for (int T : Ts) V += frame_count(T);
+        check_slot_capacity(V);
+    }
     int na = (int)ns.size();
     c->xy.ensure(16 * (size_t)std::max<long long>(total, 1));
     std::vector<int> poffs(na + 1, 0);
@@ -1144,6 +1162,8 @@ int galphaComputeDevice(galpha_ctx *c, const double *d_xy, int n, int fboSize, i
         int F;
         try {   // the context's own stream comes back on every exit
             double bb[5] = {0, 0, 1, 1, 0};
+            check_texture(fboSize);
+            if (n >= 3) check_slot_capacity((long long)n + frame_count(fboSize));
             if (n > 0) {
                 c->bboxTmp.ensure(8 * 5 * 64);
                 bbox_device(d_xy, n, bb, c->bboxTmp.p, c->stream);
@@ -1203,6 +1223,8 @@ int galphaComputeHost(galpha_ctx *c, const double *xy, int n, int fboSize, int *
     try {
         HIPCHK(hipSetDevice(c->device));
         double bb[5] = {0, 0, 1, 1, 0};
+        check_texture(fboSize);
+        if (n >= 3) check_slot_capacity((long long)n + frame_count(fboSize));
         if (n > 0) bbox_host(xy, n, bb);
         c->xy.ensure(16 * (size_t)std::max(n, 1));
         if (n > 0) stage_h2d(c->stage, c->xy.p, xy, 16 * (size_t)n, c->stream);

@@ -65,6 +65,8 @@ struct galpha_server {
     std::deque<std::shared_ptr<Request>> queue;
     std::unordered_map<long long, std::shared_ptr<Request>> tickets;
     std::vector<size_t> held;          // device bytes held by each worker's context
+    std::vector<char> busy;            // worker is running a pass (its holdings cannot be trimmed)
+    std::vector<galpha_ctx *> ctxs;    // each worker's context, created before the workers start
     std::vector<std::thread> workers;
     long long next_id = 1;
     bool stop = false;
@@ -87,7 +89,7 @@ static size_t footprint(long long n, long long T) { return group_footprint(n, T 
 static void worker_main(galpha_server *S, int w)
 {
     hipSetDevice(S->device);
-    galpha_ctx *ctx = galphaCreate(S->device);
+    galpha_ctx *ctx = S->ctxs[w];
     for (;;) {
         std::vector<std::shared_ptr<Request>> grp;
         {
@@ -118,6 +120,7 @@ static void worker_main(galpha_server *S, int w)
                     if (!grp.empty()) {
                         S->queue.erase(S->queue.begin(), S->queue.begin() + grp.size());
                         S->held[w] = std::max(S->held[w], need);
+                        S->busy[w] = 1;
                         break;
                     }
                     if (S->held[w] > 0) {
@@ -130,6 +133,13 @@ static void worker_main(galpha_server *S, int w)
                         S->cv_work.notify_all();
                         continue;
                     }
+                    // this slot holds nothing, yet the head does not fit: wake the idle slots
+                    // that hold buffers so that they give them back (busy slots wake everyone
+                    // when their pass ends).  Once they have trimmed, none is left to wake.
+                    bool idle_holder = false;
+                    for (int q = 0; q < S->slots; q++)
+                        if (q != w && !S->busy[q] && S->held[q] > 0) idle_holder = true;
+                    if (idle_holder) S->cv_work.notify_all();
                 }
                 S->cv_work.wait(lk);
             }
@@ -174,6 +184,7 @@ static void worker_main(galpha_server *S, int w)
             auto t1 = clk::now();
             for (auto &r : grp) { r->finished = t1; r->done = true; r->xy.clear(); r->xy.shrink_to_fit(); }
             S->held[w] = held_now;
+            S->busy[w] = 0;
             S->passes++;
             S->served += (long long)grp.size();
         }
@@ -184,15 +195,33 @@ static void worker_main(galpha_server *S, int w)
 
 extern "C" {
 
+extern "C" void galpha_set_last_error(const char *msg);   // galpha.cpp (library-internal)
+
 galpha_server *galphaServerCreate(int device, int slots, size_t hbm_budget_bytes, int max_group)
 {
-    if (slots < 1 || slots > 64 || max_group < 1) return nullptr;
+    if (slots < 1 || slots > 64 || max_group < 1) {
+        galpha_set_last_error("server: slots must be in [1, 64] and max_group at least 1");
+        return nullptr;
+    }
     galpha_server *S = new galpha_server();
     S->device = device;
     S->slots = slots;
     S->max_group = max_group;
     S->budget = hbm_budget_bytes;
     S->held.assign(slots, 0);
+    S->busy.assign(slots, 0);
+    // every slot's context exists before any worker runs: a device error fails the creation
+    for (int w = 0; w < slots; w++) {
+        galpha_ctx *c = galphaCreate(device);
+        if (!c) {
+            std::string why = std::string("server: ") + gpudtGetLastError();
+            for (galpha_ctx *x : S->ctxs) galphaDestroy(x);
+            delete S;
+            galpha_set_last_error(why.c_str());
+            return nullptr;
+        }
+        S->ctxs.push_back(c);
+    }
     for (int w = 0; w < slots; w++) S->workers.emplace_back(worker_main, S, w);
     return S;
 }
@@ -230,7 +259,9 @@ long long galphaServerSubmit(galpha_server *S, const double *xy, int n, int fboS
         S->tickets[id] = r;
         S->queue.push_back(r);
     }
-    S->cv_work.notify_one();
+    // every waiting slot looks at the queue: the first that can admit the head takes it, and one
+    // that holds nothing cannot be the only one woken while idle slots hold the memory it needs
+    S->cv_work.notify_all();
     return id;
 }
 
@@ -297,6 +328,15 @@ int galphaServerSetGroupPoints(galpha_server *S, long long max_points)
     std::lock_guard<std::mutex> lk(S->mu);
     S->max_points = max_points;
     return 0;
+}
+
+// Device bytes the admission rule counts for one request of n points on a fboSize texture (0: the
+// default texture for n).
+size_t galphaServerFootprint(int n, int fboSize)
+{
+    if (n < 0) return 0;
+    long long T = fboSize > 0 ? fboSize : galphaAutoTexture(n);
+    return footprint(n, T);
 }
 
 // Passes run, requests served, and device bytes held by all slots.

@@ -46,6 +46,22 @@ def gaussian_clusters(rng, n, nclusters, sigma):
     return pts
 
 
+def serving_sizes(nreq, seed=5, lo=1000, hi=2_000_000):
+    """Request sizes log-uniform in [lo, hi] and window starts into a pool of hi + 1 points."""
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), nreq))).astype(np.int64)
+    starts = rng.integers(0, hi + 1 - sizes + 1)
+    return sizes, starts
+
+
+def serving_requests(nreq=24, seed=5, lo=1000, hi=2_000_000):
+    """Serving workload in arrival order: windows of one seeded pool, uniform(seed, hi + 1), with
+    sizes log-uniform in [lo, hi] (serving_sizes)."""
+    pool = uniform(seed, hi + 1)
+    sizes, starts = serving_sizes(nreq, seed, lo, hi)
+    return [pool[s:s + n].copy() for s, n in zip(starts, sizes)]
+
+
 def batch_sets(nsets=4096, lo=256, hi=65536, seed=4):
     """Batched workload: sizes log-uniform in [lo, hi], each set uniform or an 8-cluster Gaussian
     mix (sigma 0.01, points outside [0,1)^2 resampled), 50/50."""

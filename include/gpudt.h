@@ -73,6 +73,65 @@ void gpudtReleaseDTOutput(PGPUDTOUTPUT pGpudtOutput);
 /* Thread-local message of the last failure ("" if none). */
 const char *gpudtGetLastError(void);
 
+#ifdef __cplusplus
+}
+#endif
+
+/* ---- mesh navigation (gpudt/gpudt.h:137-244) ------------------------------------------------
+ * Same names, arguments and results as the reference, so a caller written against the reference
+ * header (its validator checkResult, main.cu:163-343) compiles unchanged.  An oriented triangle
+ * (tri, orient) names one edge of triangle `tri`: the edge opposite vtx[orient], directed from
+ * org = vtx[(orient + 1) % 3] to dest = vtx[(orient + 2) % 3]; apex = vtx[orient].  It packs into
+ * one int as (tri << 2) | orient, the encoding of gpudtTriangle::tri[], so -1 (a hull edge)
+ * decodes to tri = -1.
+ *   lnext / lprev   the next / previous edge of the same triangle (counter-clockwise / clockwise)
+ *   sym             the same edge seen from the neighbouring triangle (tri < 0 on the hull)
+ *   onext / oprev   the next edge around org counter-clockwise / clockwise
+ *   setorg / setdest / setapex, bond   write a vertex, or link two oriented triangles both ways
+ * As in the reference, sym, symself, onext, onextself, oprev and oprevself assign the packed
+ * neighbour to a variable named `ptr` (an int) that the caller declares.  Each macro here is one
+ * expression, so it also works as the body of an unbraced if or loop. */
+typedef struct {
+    int tri;
+    int orient;
+} gpudtOrientedTriangle;
+
+static const int gpudt_plus1mod3[3] = {1, 2, 0};
+static const int gpudt_minus1mod3[3] = {2, 0, 1};
+
+#define Encode(otri) (((otri).tri << 2) | (otri).orient)
+#define decode(code, otri) ((otri).orient = (int)((unsigned int)(code) & 3u), (otri).tri = (code) >> 2)
+#define otricopy(from, to) ((to).tri = (from).tri, (to).orient = (from).orient)
+#define otriequal(a, b) ((a).tri == (b).tri && (a).orient == (b).orient)
+
+#define org(triList, otri, v) ((v) = (triList)[(otri).tri].vtx[gpudt_plus1mod3[(otri).orient]])
+#define dest(triList, otri, v) ((v) = (triList)[(otri).tri].vtx[gpudt_minus1mod3[(otri).orient]])
+#define apex(triList, otri, v) ((v) = (triList)[(otri).tri].vtx[(otri).orient])
+#define setorg(triList, otri, v) ((triList)[(otri).tri].vtx[gpudt_plus1mod3[(otri).orient]] = (v))
+#define setdest(triList, otri, v) ((triList)[(otri).tri].vtx[gpudt_minus1mod3[(otri).orient]] = (v))
+#define setapex(triList, otri, v) ((triList)[(otri).tri].vtx[(otri).orient] = (v))
+
+#define lnext(from, to) ((to).tri = (from).tri, (to).orient = gpudt_plus1mod3[(from).orient])
+#define lnextself(otri) ((otri).orient = gpudt_plus1mod3[(otri).orient])
+#define lprev(from, to) ((to).tri = (from).tri, (to).orient = gpudt_minus1mod3[(from).orient])
+#define lprevself(otri) ((otri).orient = gpudt_minus1mod3[(otri).orient])
+
+#define sym(triList, from, to) (ptr = (triList)[(from).tri].tri[(from).orient], decode(ptr, to))
+#define symself(triList, otri) (ptr = (triList)[(otri).tri].tri[(otri).orient], decode(ptr, otri))
+#define onext(triList, from, to) (lprev(from, to), symself(triList, to))
+#define onextself(triList, otri) (lprevself(otri), symself(triList, otri))
+#define oprev(triList, from, to) (sym(triList, from, to), lnextself(to))
+#define oprevself(triList, otri) (symself(triList, otri), lnextself(otri))
+
+/* links a and b across their shared edge; a side with tri < 0 (the hull) is not written */
+#define bond(triList, a, b)                                                                   \
+    (((a).tri >= 0 ? ((triList)[(a).tri].tri[(a).orient] = Encode(b)) : 0),                  \
+     ((b).tri >= 0 ? ((triList)[(b).tri].tri[(b).orient] = Encode(a)) : 0))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
 /* Alpha shapes of a DT returned by gpudtComputeDT (AlphaShape/headers.h:48-90 layouts).
  * Ranks index spectrum[]; rank 0 is the attached/void slot (spectrum[0] = 0). */
 typedef struct {
@@ -209,6 +268,9 @@ int galphaServerTriangles(galpha_server *server, long long ticket, int *tris9, i
 int galphaServerAlpha(galpha_server *server, long long ticket, PGPUDTALPHA *pAlpha);
 int galphaServerRelease(galpha_server *server, long long ticket);
 int galphaServerStats(galpha_server *server, long long *passes, long long *served, size_t *held_bytes);
+/* device bytes the admission rule counts for one request of n points (fboSize 0: the default
+ * texture for n): textures, mesh and alpha buffers of a context that runs it alone */
+size_t galphaServerFootprint(int n, int fboSize);
 /* At most max_points points per pass (0: no limit, the default); a larger request still runs,
  * alone.  Shorter passes free slots sooner, so small requests wait less behind large ones. */
 int galphaServerSetGroupPoints(galpha_server *server, long long max_points);

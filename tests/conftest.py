@@ -70,6 +70,16 @@ def digest(tris, alpha=None):
     return out
 
 
+def set_digest(tris, alpha):
+    """One SHA-256 over a set's canonical triangles (int32 [F, 9]) and every alpha array in
+    ALPHA_FIELDS order (spectrum little-endian float64, the rest little-endian int32)."""
+    import hashlib
+    h = hashlib.sha256(np.ascontiguousarray(tris, dtype="<i4").tobytes())
+    for k in ALPHA_FIELDS:
+        h.update(np.ascontiguousarray(alpha[k], dtype="<f8" if k == "spectrum" else "<i4").tobytes())
+    return h.hexdigest()
+
+
 def golden_digests():
     import json
     with open(os.path.join(GOLDEN, "digests.json")) as f:

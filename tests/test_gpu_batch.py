@@ -133,3 +133,51 @@ def test_batch_repeated_sets(oracle):
     for k, xy in enumerate(sets):
         _check_set(oracle, xy, tris[k], ctx.batch_alpha(k))
     ctx.close()
+
+
+def _digest_sets(ctx, nsets, workers=8):
+    """Full set digests (conftest.set_digest) of every set of the context's last batch; the
+    copies run on this thread, the hashing on a pool (hashlib releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from conftest import set_digest
+    out = []
+    with ThreadPoolExecutor(workers) as pool:
+        pending = []
+        for k in range(nsets):
+            pending.append(pool.submit(set_digest, ctx.batch_triangles(k), ctx.batch_alpha(k)))
+            if len(pending) > 4 * workers:   # bound the copied results held in host memory
+                out.append(pending.pop(0).result())
+        out += [f.result() for f in pending]
+    return out
+
+
+def test_batched_workload_full_size():
+    """The batched workload at full size: 4,096 sets of 256..65,536 points (uniform or 8-cluster
+    Gaussian, seed 4, 49.5M points), autotuned textures, one device-resident batched pass with
+    alpha.  Every set's triangles and alpha arrays equal the oracle's (committed digests)."""
+    import hashlib
+    import galpha2d_amd as G
+    from galpha2d_amd import datagen
+    from conftest import golden_digests
+    want = golden_digests()["batched_4096"]
+    print("case: batched workload, 4096 sets, autotune + galphaBatchDevice", flush=True)
+    sets = datagen.batch_sets(4096, seed=4)
+    sizes = np.array([x.shape[0] for x in sets], dtype=np.int64)
+    assert int(sizes.sum()) == want["points"]
+    offs = np.zeros(len(sets) + 1, dtype=np.int32)
+    offs[1:] = np.cumsum(sizes)
+    ctx = G.Context(0)
+    ctx.autotune(sets[:256], reps=1)
+    d = G.DeviceBuffer(np.ascontiguousarray(np.concatenate(sets)))
+    del sets
+    try:
+        F = ctx.batch_device(d.ptr, offs, with_alpha=True)
+        G.synchronize()
+        print(f"batch done: {F} triangles; hashing 4096 sets", flush=True)
+        got = _digest_sets(ctx, len(offs) - 1)
+    finally:
+        d.free()
+        ctx.close()
+    bad = [k for k, g in enumerate(got) if g[:16] != want["per_set16"][k]]
+    assert not bad, f"{len(bad)} sets differ from the oracle, first {bad[:10]}"
+    assert hashlib.sha256("".join(got).encode()).hexdigest() == want["all"]

@@ -199,3 +199,26 @@ def test_degenerate_geometry(oracle):
             _same_alpha(ctx.batch_alpha(k), ra, name)
     finally:
         ctx.close()
+
+
+def test_slot_capacity_guard():
+    """Links name triangle slots as (t << 2) | j in int32, so one pass holds at most 3 V + 8192 <
+    2^29 slots.  Sizes past that bound are refused from the sizes alone, before any kernel reads
+    the (here tiny) point buffer."""
+    import galpha2d_amd as G
+    print("case: slot capacity guard", flush=True)
+    ctx = G.Context(0)
+    d = G.DeviceBuffer(np.zeros((16, 2)))
+    try:
+        with pytest.raises(G.GalphaError, match="2\\^29"):
+            ctx.compute_device(d.ptr, 179_000_000, 16384)
+        with pytest.raises(G.GalphaError, match="2\\^29"):
+            ctx.batch_device(d.ptr, np.array([0, 90_000_000, 180_000_000]), fbo_sizes=[16384, 16384])
+        # the context still works afterwards
+        xy = np.random.default_rng(3).random((500, 2))
+        d2 = G.DeviceBuffer(xy)
+        assert ctx.compute_device(d2.ptr, 500, 256) > 0
+        d2.free()
+    finally:
+        d.free()
+        ctx.close()

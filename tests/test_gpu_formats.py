@@ -22,6 +22,8 @@ def test_cli_reference_files(oracle, tmp_path, name):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "GAlpha time:" in r.stdout and ", 0 errors," in r.stdout
+    # the macro walk (include/gpudt.h navigation) closed every vertex fan and Euler's formula holds
+    assert "Euler characteristic 2," in r.stdout
     xy = load_fixture(name)
     want, _ = oracle.delaunay(xy)
     wa = oracle.alpha(xy, want)

@@ -46,18 +46,61 @@ def test_antisymmetry_and_cyclic(oracle):
         assert oracle.incircle_sign(a, b, d, c) == -i   # row swap of the lifted determinant
 
 
+def _perm_sign(p):
+    s, p = 1, list(p)
+    for i in range(len(p)):
+        while p[i] != i:
+            j = p[i]
+            p[i], p[j] = p[j], p[i]
+            s = -s
+    return s
+
+
+def _det(m):
+    """Exact determinant (integers or Fractions) by cofactor expansion."""
+    if len(m) == 1:
+        return m[0][0]
+    return sum((-1) ** c * m[0][c] * _det([row[:c] + row[c + 1:] for row in m[1:]]) for c in range(len(m)))
+
+
+def _sos_by_definition(P, idx, s0):
+    """Sign of det[x, y, x^2 + y^2 + eps^(i + 1), 1] for an infinitesimal eps > 0: the determinant
+    is linear in each lifted height, so the coefficient of eps^(i + 1) is the cofactor of row i's
+    height, and the first non-zero one in index order decides.  s0 maps the 4x4 determinant's
+    sign to incircle's."""
+    rows = [[x, y, x * x + y * y, 1] for x, y in P]
+    assert _det(rows) == 0
+    for r in sorted(range(4), key=lambda k: idx[k]):
+        minor = [row[:2] + row[3:] for k, row in enumerate(rows) if k != r]
+        cof = (-1) ** (r + 2) * _det(minor)
+        if cof:
+            return s0 * (1 if cof > 0 else -1)
+    return 0
+
+
 def test_sos_tiebreak_consistent(oracle):
-    """Exactly co-circular quadruples: the tie-break never returns 0 and is a determinant sign
-    (odd permutations flip it)."""
+    """Exactly co-circular quadruples: the tie-break never returns 0, is a determinant sign over
+    all 24 argument orders (the sign of the permutation times the identity order's), and equals
+    the sign of the height-perturbed lifted determinant worked out exactly."""
     circ = [(5.0, 0.0), (4.0, 3.0), (3.0, 4.0), (0.0, 5.0), (-3.0, 4.0), (-4.0, 3.0),
             (-5.0, 0.0), (-4.0, -3.0), (-3.0, -4.0), (0.0, -5.0), (3.0, -4.0), (4.0, -3.0)]
+    # the 4x4 lifted determinant's sign against incircle's, on a point strictly inside
+    a, b, c, d = (5, 0), (0, 5), (-5, 0), (0, 0)
+    s0 = oracle.incircle_sign(a, b, c, d) * (1 if _det([[x, y, x * x + y * y, 1] for x, y in (a, b, c, d)]) > 0 else -1)
+    rnd = random.Random(11)
+    labels = list(range(40))
     for q in itertools.combinations(range(12), 4):
         P = [circ[i] for i in q]
         assert oracle.incircle_sign(*P) == 0
-        s = oracle.incircle_sos(P[0], q[0], P[1], q[1], P[2], q[2], P[3], q[3])
+        idx = rnd.sample(labels, 4)   # point indices in any order relative to the positions
+        s = oracle.incircle_sos(P[0], idx[0], P[1], idx[1], P[2], idx[2], P[3], idx[3])
         assert s != 0
-        s2 = oracle.incircle_sos(P[1], q[1], P[0], q[0], P[2], q[2], P[3], q[3])
-        assert s2 == -s
+        assert s == _sos_by_definition([tuple(map(int, p)) for p in P], idx, s0)
+        for perm in itertools.permutations(range(4)):
+            args = []
+            for k in perm:
+                args += [P[k], idx[k]]
+            assert oracle.incircle_sos(*args) == _perm_sign(perm) * s, (q, idx, perm)
 
 
 def _ref_lib():
