@@ -63,7 +63,7 @@ EXPORTS = ["gpudtComputeDT", "gpudtReleaseDTOutput", "gpudtGetLastError", "gpudt
            "galphaPackResult", "galphaUnpackResult", "galphaSetConstraints", "galphaDeviceCount",
            "galphaDeviceMalloc", "galphaDeviceFree", "galphaDeviceCopy", "galphaDeviceSynchronize",
            "galphaStreamCreate", "galphaStreamDestroy", "galphaStreamSynchronize", "galphaServerSetGroupPoints",
-           "galphaServerFootprint"]
+           "galphaServerFootprint", "galphaComputeDeviceNoAlloc"]
 
 
 class _Params(C.Structure):
@@ -97,6 +97,8 @@ def lib():
     L.galphaComputeHost.restype = C.c_int
     L.galphaComputeDevice.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, vp]
     L.galphaComputeDevice.restype = C.c_int
+    L.galphaComputeDeviceNoAlloc.argtypes = [C.c_int, vp, C.c_int, C.c_int, vp, C.c_int, vp]
+    L.galphaComputeDeviceNoAlloc.restype = C.c_int
     L.galphaDeviceTriangles.argtypes = [vp, C.POINTER(vp), ip]
     L.galphaCopyTriangles.argtypes = [vp, ip, C.c_int]
     L.galphaCopyTriangles.restype = C.c_int
@@ -263,6 +265,19 @@ def device_count():
 def synchronize():
     if lib().galphaDeviceSynchronize() < 0:
         raise GalphaError(lib().gpudtGetLastError().decode())
+
+
+def compute_device_noalloc(d_xy_ptr, n, fbo_size, d_tris_ptr, capacity, device=0, stream=None):
+    """Delaunay triangulation of n device points (x, y doubles at d_xy_ptr) that allocates no
+    device memory: the run's buffers come from the library's static per-device scratch and the
+    canonical triangles (9 ints each) go to d_tris_ptr (room for `capacity` of them).  Returns the
+    triangle count."""
+    L = lib()
+    F = L.galphaComputeDeviceNoAlloc(int(device), C.c_void_p(d_xy_ptr), int(n), int(fbo_size),
+                                     C.c_void_p(d_tris_ptr), int(capacity), C.c_void_p(stream or 0))
+    if F < 0:
+        raise GalphaError("galphaComputeDeviceNoAlloc: " + L.gpudtGetLastError().decode())
+    return F
 
 
 def auto_texture(n):

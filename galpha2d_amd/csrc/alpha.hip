@@ -829,6 +829,7 @@ static void ab(AlphaDev &d, int i, size_t bytes)
 {
     AlphaBuf &b = d.b[i];
     if (bytes <= b.bytes) return;
+    if (galpha_arena_active()) throw GalphaError("alpha: no device allocation is allowed in this call");
     if (b.p) hipFree(b.p);
     b.p = nullptr;
     b.bytes = 0;

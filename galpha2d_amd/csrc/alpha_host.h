@@ -24,6 +24,10 @@ struct GalphaError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
 
+// true while a no-allocation run (buffers carved from a fixed device region) is active on this
+// thread: nothing may call hipMalloc then
+bool galpha_arena_active();
+
 struct AlphaBuf {
     void *p = nullptr;
     size_t bytes = 0;

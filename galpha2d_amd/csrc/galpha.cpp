@@ -32,24 +32,55 @@ static thread_local std::string g_last_error;
         }                                                                                      \
     } while (0)
 
+// Fixed device region that buffers are carved from instead of hipMalloc while a no-allocation
+// run is active on this thread (bump allocation, reset at the start of each run).
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0, high = 0;
+};
+static thread_local Arena *t_arena = nullptr;
+
+struct ArenaScope {
+    Arena *prev;
+    explicit ArenaScope(Arena *a) : prev(t_arena) { t_arena = a; }
+    ~ArenaScope() { t_arena = prev; }
+};
+
+__attribute__((visibility("hidden"))) bool galpha_arena_active() { return t_arena != nullptr; }
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool carved = false;   // p lies in an arena: never freed
     void ensure(size_t b)
     {
         if (b <= bytes) return;
-        if (p) hipFree(p);
+        if (p && !carved) hipFree(p);
         p = nullptr;
         bytes = 0;
+        carved = false;
+        if (t_arena) {
+            size_t nb = (b + 255) & ~(size_t)255;
+            if (t_arena->used + nb > t_arena->cap)
+                throw GalphaError("scratch exhausted: the run needs more than the " +
+                                  std::to_string(t_arena->cap >> 20) + " MiB of the no-allocation scratch");
+            p = t_arena->base + t_arena->used;
+            t_arena->used += nb;
+            t_arena->high = std::max(t_arena->high, t_arena->used);
+            bytes = nb;
+            carved = true;
+            return;
+        }
         size_t nb = b + b / 8 + 256;
         HIPCHK(hipMalloc(&p, nb));
         bytes = nb;
     }
     void release()
     {
-        if (p) hipFree(p);
+        if (p && !carved) hipFree(p);
         p = nullptr;
         bytes = 0;
+        carved = false;
     }
     template <class T> T *as() { return (T *)p; }
 };
@@ -92,9 +123,10 @@ struct galpha_ctx {
     int last_n = 0, last_F = 0, last_T = 0;
     bool last_alpha = false;
     galpha_stats stats{};
-    hipEvent_t ev[16];
+    hipEvent_t ev[16] = {};
     int op = 1;
     int *hsnap = nullptr;   // pinned staging for counter reads (SNAP_INTS ints)
+    std::vector<int> hsnap_pageable;   // counter reads of a no-allocation context
     HostStage stage;        // pinned staging for large host <-> device copies
 };
 
@@ -104,6 +136,7 @@ static const size_t STAGE_HALF = (size_t)32 << 20;
 static void stage_ensure(HostStage &s)
 {
     if (s.p) return;
+    if (t_arena) throw GalphaError("pinned staging is not available in a no-allocation call");
     HIPCHK(hipHostMalloc((void **)&s.p, 2 * STAGE_HALF, hipHostMallocDefault));
     s.half = STAGE_HALF;
     for (int k = 0; k < 2; k++) HIPCHK(hipEventCreateWithFlags(&s.ev[k], hipEventDisableTiming));
@@ -443,8 +476,14 @@ static void run_pba(galpha_ctx *c, const double *d_xy, const BatchDev &bd, int *
 static float ev_ms(galpha_ctx *c, int a, int b)
 {
     float ms = 0;
-    hipEventElapsedTime(&ms, c->ev[a], c->ev[b]);
+    if (c->ev[a] && c->ev[b]) hipEventElapsedTime(&ms, c->ev[a], c->ev[b]);
     return ms;
+}
+
+// stage boundary k on stream st (contexts created without events skip the timing)
+static void ev_mark(galpha_ctx *c, int k, hipStream_t st)
+{
+    if (c->ev[k]) HIPCHK(hipEventRecord(c->ev[k], st));
 }
 
 // box of host points into bb5 (min x, min y, max x, max y, non-finite flag)
@@ -571,10 +610,10 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     HIPCHK(hipMemsetAsync(c->counters.p, 0, COUNTER_BYTES, st));
     HIPCHK(hipMemsetAsync(c->status.p, 0, sizeof(int) * 4, st));
     BatchDev bd = upload_plan(c, plan);
-    HIPCHK(hipEventRecord(c->ev[0], st));
+    ev_mark(c, 0, st);
     d_xy = renumber_points(c, d_xy, bd, plan);
     run_pba(c, d_xy, bd, &S.pbaIterations, &S.islandSites, c->perm.as<int>());
-    HIPCHK(hipEventRecord(c->ev[1], st));
+    ev_mark(c, 1, st);
 
     // ---- mesh buffers
     // dual triangles <= 2V; each re-inserted point adds 2 slots, each deleted vertex leaves 2 dead
@@ -659,7 +698,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
         if (cv[C_STATUS])
             throw GalphaError("reconstruction produced an invalid triangle (status " + std::to_string(cv[C_STATUS]) + ")");
     }
-    HIPCHK(hipEventRecord(c->ev[2], st));
+    ev_mark(c, 2, st);
     // ---- shifting
     // revert moved vertices of non-ccw triangles until every triangle is ccw (monotone)
     // Round loops launch several rounds per host read of their counters: a round after
@@ -730,7 +769,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
         }
     }
     S.deleted = deleted_before;
-    HIPCHK(hipEventRecord(c->ev[3], st));
+    ev_mark(c, 3, st);
 
     // ---- flips to a Delaunay mesh of the current vertices
     // Lawson passes over the list of triangles to test; a pass with no flip has no candidate
@@ -762,7 +801,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
         flip_until_done(m.modA, C_MODN, m.modB, C_MODN2, opn);
     };
     flip_all();
-    HIPCHK(hipEventRecord(c->ev[4], st));
+    ev_mark(c, 4, st);
 
     // ---- missing sites
     ctx_counter_set(c, C_MISS, 0);
@@ -786,7 +825,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     }
     S.inserted = last.stat(C_INSERTED);
     S.duplicates = last.stat(C_DUPS);
-    HIPCHK(hipEventRecord(c->ev[5], st));
+    ev_mark(c, 5, st);
 
     // ---- frame removal + hull convexification
     // Triangles created by frame removal and convexification go to one list (each slot once);
@@ -809,9 +848,9 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     launch_convexify(m, c->modF.as<int>(), C_FRAME_LIST, c->op++, c->hullCnt.as<int>(), c->hullOff.as<int>(),
                      c->hullStart.as<int>(), c->scanTmp.p, st);
     }
-    HIPCHK(hipEventRecord(c->ev[6], st));
+    ev_mark(c, 6, st);
     flip_until_done(c->modF.as<int>(), C_FRAME_LIST, m.modB, C_MODN2, m.list_op);
-    HIPCHK(hipEventRecord(c->ev[7], st));
+    ev_mark(c, 7, st);
     if (last.status()) throw GalphaError("device pipeline error (status " + std::to_string(last.status()) + ")");
 
     // ---- constraint edges (single-set runs): cavity insertion rounds, then constrained flips
@@ -863,7 +902,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
         flip_all();
         if (last.status()) throw GalphaError("constrained flips failed (status " + std::to_string(last.status()) + ")");
     }
-    HIPCHK(hipEventRecord(c->ev[9], st));
+    ev_mark(c, 9, st);
 
     // ---- canonical output
     int nslots = std::min(last.v[40], cap);
@@ -893,7 +932,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     launch_canon_emit(m, order, c->rot.as<int>(), c->newid.as<int>(),
                       c->counters.as<int>() + C_FOLD + C_RECON_TRIS, c->out9.as<int>(), c->setFirst.as<int>(),
                       c->keys2.as<int>(), st);   // keys2 is free after the sort: output sets
-    HIPCHK(hipEventRecord(c->ev[8], st));
+    ev_mark(c, 8, st);
     std::vector<int> sf(nsets);
     Snap fin;
     HIPCHK(hipMemcpyAsync(sf.data(), c->setFirst.p, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st));
@@ -1110,6 +1149,78 @@ void galpha_ctx_trim(galpha_ctx *c)
     release_all(c);
     c->last_F = 0;
     c->last_alpha = false;
+}
+
+// ---- runs that allocate nothing --------------------------------------------------------------
+// A context for runs whose device buffers are carved from a fixed region: no stream, no events
+// and no pinned memory of its own (counter reads go to pageable host memory).
+__attribute__((visibility("hidden"))) galpha_ctx *galpha_noalloc_ctx_new(int device)
+{
+    galpha_ctx *c = new galpha_ctx();
+    c->device = device;
+    c->hsnap_pageable.assign(SNAP_INTS, 0);
+    c->hsnap = c->hsnap_pageable.data();
+    return c;
+}
+
+// Upper bound of the bytes a single-set DT run of n points on a T x T texture carves: every
+// buffer the run ensures, at its largest size (slots at the capacity 3 V + 8192, boundary cycle
+// at V vertices), the scan and sort temporaries as rocPRIM reports them, and 256-byte rounding.
+__attribute__((visibility("hidden"))) size_t galpha_noalloc_need(int n, int T)
+{
+    if (n < 3) return 4096;
+    long long P = (long long)T * T, nf = frame_count(T), V = n + nf, cap = 3 * V + 8192;
+    long long ntiles = (long long)(T - 1) * ((T - 1 + 255) / 256), nc = P / 256 + 1;
+    long long kb = 2;
+    while ((1LL << kb) < (long long)n) kb++;
+    size_t sort_tmp = 0;
+    sort_pairs_u64(nullptr, nullptr, nullptr, nullptr, (int)cap, 0, (int)(2 * kb), nullptr, &sort_tmp, 0);
+    double b = 0;
+    b += 18.0 * P + 516.0 * T + 8.0 * ntiles + 8.0 * nc;          // textures, band ranges, tiles, cells
+    b += (116.0 + 68.0) * cap;                                     // mesh slots, canonical keys + output
+    b += (89.0 + 60.0 + 12.0) * V + 36.0 * n + 4.0 * nf;           // vertex arrays, boundary cycle, plan
+    b += (double)COUNTER_BYTES + 4096.0 + 8.0 * 5 * 64;           // counters, status, box partials
+    b += (double)scan_temp_bytes((size_t)std::max(ntiles, V + 2)) + (double)scan_temp_bytes((size_t)nc) + 8192.0;
+    b += (double)sort_tmp + 256.0 + 96.0 * 256.0 + 65536.0;        // sort temporary, rounding, slack
+    return (size_t)b;
+}
+
+// The DT of galphaComputeDevice (no alpha) with every device buffer carved from
+// [scratch, scratch + scratch_bytes) and the canonical triangles copied to d_tris9 (9 ints each,
+// capacity records).  Calls no hipMalloc or hipFree; returns the triangle count.
+__attribute__((visibility("hidden"))) int galpha_noalloc_run(galpha_ctx *c, void *scratch, size_t scratch_bytes,
+                                                             const double *d_xy, int n, int fboSize, int *d_tris9,
+                                                             int capacity, void *stream, size_t *high_water)
+{
+    check_texture(fboSize);
+    if (n < 0) throw GalphaError("negative point count");
+    if (n > 0 && !d_xy) throw GalphaError("null point array");
+    if (capacity < 0 || (capacity > 0 && !d_tris9)) throw GalphaError("invalid output buffer");
+    if (n >= 3) check_slot_capacity((long long)n + frame_count(fboSize));
+    size_t need = galpha_noalloc_need(n, fboSize);
+    if (need > scratch_bytes)
+        throw GalphaError("no-allocation run of " + std::to_string(n) + " points on a " + std::to_string(fboSize) +
+                          "^2 texture needs up to " + std::to_string(need >> 20) + " MiB of scratch; the limit is " +
+                          std::to_string(scratch_bytes >> 20) + " MiB");
+    Arena ar;
+    ar.base = (char *)scratch;
+    ar.cap = scratch_bytes;
+    ArenaScope scope(&ar);
+    release_all(c);   // forget the previous run's carving: the buffers are carved again in order
+    c->stream = (hipStream_t)stream;
+    c->last_alpha = false;
+    double bb[5] = {0, 0, 1, 1, 0};
+    if (n > 0) {
+        c->bboxTmp.ensure(8 * 5 * 64);
+        bbox_device(d_xy, n, bb, c->bboxTmp.p, c->stream);
+    }
+    int F = run_dt(c, d_xy, n, fboSize, bb);
+    if (F > capacity)
+        throw GalphaError("output holds " + std::to_string(capacity) + " triangles; the run made " + std::to_string(F));
+    if (F > 0) HIPCHK(hipMemcpyAsync(d_tris9, c->out9.p, 36 * (size_t)F, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (high_water) *high_water = ar.high;
+    return F;
 }
 
 extern "C" {

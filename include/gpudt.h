@@ -233,6 +233,22 @@ int galphaClassify(galpha_ctx *ctx, int rank, unsigned char *vert_class, unsigne
 int galphaClassifyDevice(galpha_ctx *ctx, int rank, unsigned char *d_vert_class, unsigned char *d_edge_class,
                          unsigned char *d_tri_class, void *stream);
 
+/* ---- Delaunay without device allocation (no reference counterpart) ------------------------
+ * For callers that cannot allocate device memory: code between HIP graph captures, tight serving
+ * loops, harnesses that forbid hipMalloc.  Same triangulation as galphaComputeDevice with
+ * with_alpha = 0, but every device buffer of the run is carved from a static device array of
+ * GALPHA_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the process ends;
+ * the call makes no hipMalloc, hipFree or other device or pinned allocation.  d_xy: n (x, y)
+ * doubles on `device`; d_tris9: device array of `capacity` records in the gpudtTriangle layout
+ * (9 ints each; 2 n records always suffice).  stream: a hipStream_t of `device` (NULL: the null
+ * stream); the call returns when its device work has finished.  Returns the triangle count, or
+ * -1 with gpudtGetLastError() set, e.g. when the run could need more than the scratch holds
+ * (textures up to 1024^2 with up to 65,536 points always fit).  Calls on one GPU share its
+ * scratch and run one at a time (a per-device lock). */
+#define GALPHA_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int galphaComputeDeviceNoAlloc(int device, const double *d_xy, int n, int fboSize, int *d_tris9, int capacity,
+                               void *stream);
+
 /* ---- batches and serving (no reference counterpart) ----------------------------------- */
 /* Batched: nsets independent point sets, concatenated in xy (offsets[nsets+1] in points),
  * texture side per set (NULL or 0 = autotuned).  All sets go through one pass of the pipeline

@@ -91,7 +91,10 @@ def test_auto_texture_rule():
 def test_no_process_global_work_state():
     """Contexts run concurrently (one per serving slot), so the library may keep no work buffers or
     pointers in process globals: the only writable globals are the mutex-protected default context
-    and kernel-probe state, the thread-local error text and C++ runtime bookkeeping."""
+    and kernel-probe state, the thread-local error text, the no-allocation entry point's per-device
+    scratch table (one lock per device), the host-side stand-in of its device array, the
+    thread-local pointer to the region a no-allocation run carves from, and C++ runtime
+    bookkeeping."""
     import re
     import shutil
     import subprocess
@@ -102,7 +105,8 @@ def test_no_process_global_work_state():
                          check=True).stdout
     allowed = {"g_default", "g_default_mu", "g_last_error[abi:cxx11]", "(anonymous namespace)::g_ev",
                "(anonymous namespace)::g_mu", "_ZN12_GLOBAL__N_14g_onE.0", "(anonymous namespace)::g_name[abi:cxx11]",
-               "(anonymous namespace)::g_used", "std::__ioinit", "__tls_guard", "__fini", "__init"}
+               "(anonymous namespace)::g_used", "std::__ioinit", "__tls_guard", "__fini", "__init",
+               "(anonymous namespace)::g_dev", "scratch", "t_arena"}
     runtime = re.compile(r"rocprim|__hip|^_ZZ|guard variable|typeinfo|vtable|__dso_handle|__TMC_END__|completed\.|"
                          r"__init_array|__fini_array|__frame_dummy|__do_global_dtors|DW\.ref|^_DYNAMIC|"
                          r"^_GLOBAL_OFFSET_TABLE_|^__do_(init|fini)\.")
