@@ -35,9 +35,12 @@ ALGO_BYTES = {
                  "slot 32 written, anchor and back-link 8 written)",
         "bytes": lambda st, T: 12.0 * st["frameSeen"] + 88.0 * st["frameChain"]},
     "k_flip_test": {
-        "model": "156 B per listed triangle (list entry 4, vertices + links 32, three neighbour "
-                 "stamps and far vertices 24, six positions 96)",
-        "bytes": lambda st, T: 156.0 * st["flipListed"]},
+        "model": "32 B per listed triangle: the flipping stage's 24F + 16V per pass over F triangles "
+                 "and V = F / 2 vertices (24 B vertex + neighbour record, 16 B position)",
+        "bytes": lambda st, T: 32.0 * st["flipListed"],
+        "gather_model": "156 B per listed triangle (list entry 4, vertices + links 32, three neighbour "
+                        "stamps and far vertices 24, six positions 96)",
+        "gather_bytes": lambda st, T: 156.0 * st["flipListed"]},
     "k_pba_color": {
         "model": "8 B per pixel (envelope read 4 + label written 4)",
         "bytes": lambda st, T: 8.0 * T * T * st["pbaIterations"]},
@@ -58,44 +61,71 @@ def measured_traffic(kernel):
     return None
 
 
-def cpu_baseline(n, seed):
-    """The oracle (sequential C: Shewchuk predicates, Bowyer-Watson DT, exact alpha ranks) on the
-    full workload, one core, one run (about 13 s)."""
-    from oracle import pyoracle as O
-    xy = O.gen_uniform(seed, n)
-    t0 = time.perf_counter()
-    tris, _ = O.delaunay(xy)
-    O.alpha(xy, tris)
-    dt = time.perf_counter() - t0
-    model = ""
+def cpu_model():
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
-        model = next((l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")), "")
+        return next((l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")), "")
     except Exception:
-        pass
-    return {"value": round(n / dt / 1e6, 4), "unit": "Mpoints/s", "cores": 1, "kind": "port",
-            "sample": f"full workload: {n} uniform points seed {seed}, DT + alpha spectrum, 1 run, "
-                      f"{dt:.2f} s on one core of {model or 'the host'} (nproc={os.cpu_count()})"}
+        return ""
+
+
+def cpu_baseline(n, seed, runs=5):
+    """The oracle (sequential C: Shewchuk predicates, Bowyer-Watson DT, exact alpha ranks) on the
+    full workload, one core: one warm-up run, then the median of `runs` timed runs (about 3 s
+    each).  Also returns the last run's triangles and alpha arrays for the parity check."""
+    import numpy as np
+    from oracle import pyoracle as O
+    xy = O.gen_uniform(seed, n)
+    times = []
+    for r in range(runs + 1):
+        t0 = time.perf_counter()
+        tris, _ = O.delaunay(xy)
+        alpha = O.alpha(xy, tris)
+        if r:
+            times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    base = {"value": round(n / med / 1e6, 4), "unit": "Mpoints/s", "cores": 1, "kind": "port",
+            "sample": f"full workload: {n} uniform points seed {seed}, DT + alpha spectrum, median of "
+                      f"{runs} runs after one warm-up ({min(times):.2f}-{max(times):.2f} s) on one core of "
+                      f"{cpu_model() or 'the host'} (nproc={os.cpu_count()})"}
+    return base, tris, alpha
+
+
+def oracle_check(gpu_tris, gpu_alpha, ref_tris, ref_alpha):
+    """The device result of the timed workload against the oracle run of the CPU baseline:
+    canonical triangles, spectrum values in ulps, and every rank array."""
+    import numpy as np
+    out = {"triangles_equal": bool(np.array_equal(gpu_tris, ref_tris))}
+    g, r = np.asarray(gpu_alpha["spectrum"]), np.asarray(ref_alpha["spectrum"])
+    if g.shape == r.shape:
+        ulp = np.abs(g.view(np.int64) - r.view(np.int64))
+        out["rho_max_ulp"] = int(ulp.max()) if ulp.size else 0
+    else:
+        out["rho_max_ulp"] = None
+        out["spectrum_sizes"] = [int(g.size), int(r.size)]
+    ranks = ("tri_ro_rank", "edge_t", "edge_vopp", "edge_ro_rank", "edge_nu_min", "edge_nu_max", "vert_nu_min",
+             "vert_nu_max", "master_simplex", "master_st", "master_at", "spec_pointers")
+    out["ranks_equal"] = all(np.array_equal(np.asarray(gpu_alpha[k]), np.asarray(ref_alpha[k])) for k in ranks)
+    return out
 
 
 def cpu_batched(nsets, sample, seed=4):
     """The oracle on the first `sample` sets of the batched workload, one set per task on up to 16
-    spawned host processes (they never touch the GPU; a worker that dies raises instead of
-    hanging the pool)."""
-    from concurrent.futures import ProcessPoolExecutor
-    import multiprocessing as mp
+    host threads (the oracle is C called through ctypes, which releases the GIL, so the threads
+    run in parallel; no child process is started)."""
+    from concurrent.futures import ThreadPoolExecutor
     from galpha2d_amd import datagen
     from oracle import pyoracle as O
     sets = datagen.batch_sets(nsets, seed=seed)[:sample]
     workers = max(1, min(16, os.cpu_count() or 1))
-    with ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as pool:
-        list(pool.map(O.dt_alpha_points, sets[:workers]))   # start-up and library load, not timed
+    O.dt_alpha_points(sets[0])   # library load and predicate set-up, not timed
+    with ThreadPoolExecutor(workers) as pool:
         t0 = time.perf_counter()
         pts = sum(pool.map(O.dt_alpha_points, sets))
         dt = time.perf_counter() - t0
     return {"cpu_sets_per_s": round(len(sets) / dt, 1), "cpu_cores": workers, "cpu_sample_sets": len(sets),
             "cpu_sample": f"first {len(sets)} sets ({pts} points), oracle DT + alpha, one set per task, "
-                          f"{workers} processes, {dt:.2f} s"}
+                          f"{workers} threads, {dt:.2f} s"}
 
 
 def host_to_host(G, xy, T, reps=3):
@@ -133,21 +163,89 @@ def host_to_host(G, xy, T, reps=3):
 
 
 def stage_gbs(st, a, n, T):
-    """Algorithmic bytes per stage by the scope table's formulas (P texels, N points, F triangles,
-    E edges, S = F + E spectrum entries, M master entries) over the measured stage time."""
+    """Algorithmic bytes of every stage by the scope table's formulas (P texels, N points, V ~ N
+    vertices, F triangles, E edges, S = F + E spectrum entries, M master entries; a mesh pass
+    reads and writes 36F + 32V, a flip test 32 B per listed triangle) over the measured stage
+    time.  The flip stage holds every flip pass: the passes after shifting and missing-site
+    insertion and those after frame removal."""
     P, F, E = T * T, st["nTris"], st["nEdges"]
-    S, M = F + E, len(a["master_simplex"])
-    model = {
-        "pba": ("S0-S2: 4P + 20N + 24P + 12P", 40 * P + 20 * n, st["ms_pba"]),
-        "recon": ("S3-S4: 4P + 12F + 48F", 4 * P + 60 * F, st["ms_recon"]),
-        "alpha": ("S7-S9: 24F + 48N + 8(F+E) + 12E + 24S + 8N + 24E + 4F + 24M",
-                  24 * F + 48 * n + 8 * (F + E) + 12 * E + 24 * S + 8 * n + 24 * E + 4 * F + 24 * M, st["ms_alpha"]),
-    }
+    S, M, V = F + E, len(a["master_simplex"]), n
+    listed = st.get("stageListed", [0, 0, 0])
+    mesh_pass = 36 * F + 32 * V
+    model = [
+        ("pba", "S0-S2: 4P + 20N + 24P + 12P", 40 * P + 20 * n, st["ms_pba"]),
+        ("recon", "S3-S4: 4P + 12F + 48F", 4 * P + 60 * F, st["ms_recon"]),
+        ("shift", "S5 one pass 36F + 32V", mesh_pass, st["ms_shift"]),
+        ("missing", "S5 one pass 36F + 32V", mesh_pass, st["ms_missing"]),
+        ("boundary", "S5 one pass 36F + 32V", mesh_pass, st["ms_boundary"]),
+        ("flip", "S6 32 B per triangle listed for a flip test (all passes)", 32 * (listed[0] + listed[1]),
+         st["ms_flip"]),
+        ("canon", "triangle records read once (24F) and output records written once (36F)", 60 * F, st["ms_canon"]),
+    ]
+    if st.get("ms_alpha_rho", 0) > 0:
+        model += [
+            ("alpha_rho", "S7: 24F + 16N*3 + 8(F+E) + 12E", 24 * F + 48 * n + 8 * (F + E) + 12 * E, st["ms_alpha_rho"]),
+            ("alpha_order", "S8: 24S", 24 * S, st["ms_alpha_order"]),
+            ("alpha_lists", "S9: 8N + 24E + 4F + 24M", 8 * n + 24 * E + 4 * F + 24 * M, st["ms_alpha_lists"]),
+        ]
+    else:
+        model.append(("alpha", "S7-S9: 24F + 48N + 8(F+E) + 12E + 24S + 8N + 24E + 4F + 24M",
+                      24 * F + 48 * n + 8 * (F + E) + 12 * E + 24 * S + 8 * n + 24 * E + 4 * F + 24 * M,
+                      st["ms_alpha"]))
     out = {}
-    for k, (formula, b, ms) in model.items():
-        gbs = b / (ms * 1e-3) / 1e9
+    for k, formula, b, ms in model:
+        gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         out[k] = {"formula": formula, "bytes": int(b), "ms": round(ms, 3), "gb_s": round(gbs, 1),
                   "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
+def stress_c3(G, reps=3, seed=3):
+    """The degenerate / clustered stress workload: 2,000,000 FP64 points, half drawn without
+    replacement from a 1414 x 1414 lattice scaled to [0,1)^2 (many co-circular quadruples), half
+    from 64 Gaussian clusters (sigma 0.005, centres uniform); seed 3; texture 8192 x 8192; DT +
+    full alpha, points resident in HBM.  Median of `reps` runs after one warm-up, and the exact
+    fallback rate of every predicate (exact evaluations / filtered ones), overall and per mesh
+    stage."""
+    import numpy as np
+    from galpha2d_amd import datagen
+    n, T = 2_000_000, 8192
+    xy = datagen.lattice_clusters(n, seed=seed)
+    ctx = G.Context(0)
+    d = G.DeviceBuffer(xy)
+    ctx.compute_device(d.ptr, n, T, with_alpha=True)
+    G.synchronize()
+    times, stage = [], {}
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ctx.compute_device(d.ptr, n, T, with_alpha=True)
+        G.synchronize()
+        times.append(time.perf_counter() - t0)
+        for k, v in ctx.stats().items():
+            if k.startswith("ms_"):
+                stage.setdefault(k, []).append(v)
+    st = ctx.stats()
+    ctx.close()
+    d.free()
+    med = float(np.median(times))
+
+    def rate(exact, tests, per="tests"):
+        return {"exact": int(exact), per: int(tests), "rate": round(exact / tests, 8) if tests else None}
+    out = {"workload": f"{n} points: half from a 1414 x 1414 lattice (without replacement), half from 64 "
+                       f"Gaussian clusters (sigma 0.005); seed {seed}; texture {T}x{T}; DT + full alpha",
+           "ms": round(med * 1e3, 2), "mpoints_per_s": round(n / med / 1e6, 3), "runs": reps,
+           "stage_ms": {k: round(float(np.median(v)), 3) for k, v in stage.items()},
+           "triangles": st["nTris"], "frame_bulk": st["frameBulk"],
+           "fallback": {
+               "incircle": rate(st["incircleExact"], st["incircleTests"]),
+               "orient_exact": int(st["orientExact"]),
+               "alpha_compare": rate(st["alphaExactCompares"], st["nTris"] + st["nEdges"], "entries"),
+               "attachment": rate(st["alphaExactAttach"], st["nEdges"], "edges")}}
+    if "stageIncircleTests" in st:
+        names = ("shift, insertion and flips", "frame removal and final flips", "constraints and canonical output")
+        out["fallback"]["per_stage"] = {
+            nm: {"incircle": rate(st["stageIncircleExact"][i], st["stageIncircleTests"][i]),
+                 "orient_exact": int(st["stageOrientExact"][i])} for i, nm in enumerate(names)}
     return out
 
 
@@ -295,6 +393,7 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=256, help="sets timed for the plain per-set loop")
     ap.add_argument("--serve-qps", default="50,150", help="offered request rates of the serving runs ('' skips)")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--no-c3", action="store_true", help="skip the degenerate / clustered stress workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device once, so that a counter run sees a plain "
                          "copy of known size beside the pipeline kernels")
@@ -349,7 +448,9 @@ def main():
     # workload): algorithmic bytes of the probed run / total time of its launches in that run
     probe = ctx.kernel_probe(args.probe, lambda: step())
     pst = ctx.stats()
-    stages = stage_gbs(pst, ctx.alpha(), n, T)
+    gpu_alpha = ctx.alpha()
+    gpu_tris = ctx.triangles() if not args.no_cpu_baseline else None
+    stages = stage_gbs(pst, gpu_alpha, n, T)
     roof = None
     if probe and probe["launches"] > 0:
         model = ALGO_BYTES[args.probe]
@@ -361,7 +462,13 @@ def main():
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": args.probe, "launches": probe["launches"], "avg_launch_ms": round(avg_ms, 4),
-                "algorithmic_bytes_per_launch": round(per_launch), "algorithmic_model": model["model"]}
+                "algorithmic_bytes_per_launch": round(per_launch), "algorithmic_model": model["model"],
+                "traffic_source": "profiles/c2_hbm_traffic.txt (FETCH_SIZE and WRITE_SIZE passes, scaled by "
+                                  "known-size streaming kernels)"}
+        if "gather_bytes" in model:
+            gb = model["gather_bytes"](pst, T) / probe["launches"]
+            roof["gather_model"] = model["gather_model"]
+            roof["gather_model_achieved"] = round(gb / (avg_ms * 1e-3) / 1e9, 2)
 
     out = {
         "metric": "Mpoints/s DT+alpha at 1M pts; batched sets/s and p50/p99 latency vs plain",
@@ -389,8 +496,12 @@ def main():
             out["batched"].update(cpu_batched(args.batched_sets, min(args.batched_sets, 1024)))
     if args.serve_qps:
         out["serving"] = serving(G, [float(q) for q in args.serve_qps.split(",")], args.serve_seconds)
+    if not args.no_c3:
+        out["c3"] = stress_c3(G)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(n, SEED)
+        base, ref_tris, ref_alpha = cpu_baseline(n, SEED)
+        out["cpu_baseline"] = base
+        out["oracle_check"] = oracle_check(gpu_tris, gpu_alpha, ref_tris, ref_alpha)
     print(json.dumps(out))
 
 

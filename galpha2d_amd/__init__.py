@@ -35,7 +35,10 @@ class _Stats(C.Structure):
                 ("ms_canon", C.c_double), ("ms_alpha", C.c_double), ("ms_total", C.c_double),
                 ("frameSeen", C.c_longlong), ("frameChain", C.c_longlong), ("flipListed", C.c_longlong),
                 ("frameBulk", C.c_int), ("earRounds", C.c_int),
-                ("constraintRounds", C.c_int), ("ms_constraints", C.c_double)]
+                ("constraintRounds", C.c_int), ("ms_constraints", C.c_double),
+                ("ms_alpha_rho", C.c_double), ("ms_alpha_order", C.c_double), ("ms_alpha_lists", C.c_double),
+                ("stageListed", C.c_longlong * 3), ("stageIncircleTests", C.c_longlong * 3),
+                ("stageIncircleExact", C.c_longlong * 3), ("stageOrientExact", C.c_longlong * 3)]
 
 
 class _Alpha(C.Structure):
@@ -589,7 +592,11 @@ class Context:
     def stats(self):
         s = _Stats()
         self._L.galphaGetStats(self._h, C.byref(s))
-        return {k: getattr(s, k) for k, _ in _Stats._fields_}
+        out = {}
+        for k, _ in _Stats._fields_:
+            v = getattr(s, k)
+            out[k] = list(v) if isinstance(v, C.Array) else v
+        return out
 
     def voronoi(self, xy, fbo_size):
         """Discrete Voronoi texture: (labels [T, T] x-major, pix [n + frame], n_frame)."""

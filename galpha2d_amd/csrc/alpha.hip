@@ -845,6 +845,12 @@ void alpha_release(AlphaDev &d)
         if (b.p) { hipFree(b.p); b.p = nullptr; b.bytes = 0; }
 }
 
+void alpha_destroy_events(AlphaDev &d)
+{
+    for (auto &e : d.ev)
+        if (e) { hipEventDestroy(e); e = nullptr; }
+}
+
 static void chk(hipError_t e, const char *w)
 {
     if (e != hipSuccess) throw GalphaError(std::string("alpha: ") + w + ": " + hipGetErrorString(e));
@@ -954,6 +960,9 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
         chk(hipStreamSynchronize(st), "sync");
         return;
     }
+    for (int k = 0; k < 4; k++)
+        if (!d.ev[k]) chk(hipEventCreate(&d.ev[k]), "event");
+    chk(hipEventRecord(d.ev[0], st), "event");
     ab(d, AB_CNT, 4 * (size_t)F);
     ab(d, AB_EOFF, 4 * (size_t)F);
     ab(d, AB_SCAN, scan_temp_bytes((size_t)3 * F + 2 * (size_t)n + 16) + 4096);
@@ -1011,6 +1020,7 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     int S0 = F + nun;
     d.S = S0;
     GA_LAUNCH(k_edge_entries, dim3(AGRID), dim3(256), 0, st, a, E);
+    chk(hipEventRecord(d.ev[1], st), "event");
     // radix sort by the top 40 bits of the FP64 value (nonnegative doubles order like their bit
     // patterns; 5 passes instead of 8; see sort_key).  The cluster boundaries below only need
     // "everything before is certainly smaller than everything after", which holds for any order,
@@ -1122,6 +1132,7 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     ab(d, AB_SPEC, 8 * (size_t)(nslots + 1));
     GA_LAUNCH(k_spec_values, dim3(AGRID), dim3(256), 0, st, (const unsigned long long *)a.specbits, nslots,
                        AP<double>(d, AB_SPEC));
+    chk(hipEventRecord(d.ev[2], st), "event");
     // intervals
     launch_fill_i32(a.vnumin, 0x7fffffff, (size_t)n, st);
     launch_fill_i32(a.vnumax, -1, (size_t)n, st);
@@ -1193,6 +1204,7 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
         d.mfirst = {0, Mk};
         d.efirst = {0, E};
     }
+    chk(hipEventRecord(d.ev[3], st), "event");
     int hc[8], hs = 0;
     chk(hipMemcpyAsync(hc, a.counters, sizeof(hc), hipMemcpyDeviceToHost, st), "copy");
     chk(hipMemcpyAsync(&hs, a.status, 4, hipMemcpyDeviceToHost, st), "copy");
@@ -1204,6 +1216,11 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
         stats->alphaExactAttach = hc[A_EXACT_ATT];
         stats->alphaClusters = hc[A_CLUSTERS];
         stats->alphaMaxCluster = hc[A_MAXCLUSTER];
+        float ms[3] = {0, 0, 0};
+        for (int k = 0; k < 3; k++) chk(hipEventElapsedTime(&ms[k], d.ev[k], d.ev[k + 1]), "event");
+        stats->ms_alpha_rho = ms[0];
+        stats->ms_alpha_order = ms[1];
+        stats->ms_alpha_lists = ms[2];
     }
     if (hs) throw GalphaError("alpha: device status " + std::to_string(hs) +
                               (hc[A_BIGCLUSTER] ? " (exact cluster larger than supported)" : ""));

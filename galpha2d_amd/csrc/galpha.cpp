@@ -387,7 +387,7 @@ static BatchPlan plan_batch(const int *n, const int *T, const double *bb5, int n
         acc[BM_MERGE] += (Ts / 64) * std::max(1LL, m / 2);
         acc[BM_COLOR] += (Ts / 64) * m;
         acc[BM_PIX] += Ts * Ts / 256;
-        acc[BM_RECON] += (Ts - 1) * ((Ts - 1 + 255) / 256);
+        acc[BM_RECON] += (Ts / 16) * (Ts / 16);   // one tile per 16 x 16-pixel cell
     }
     for (int k = 0; k < BM_COUNT; k++) b.blocks[k] = (int)acc[k];
     return b;
@@ -771,8 +771,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     S.deleted = deleted_before;
     ev_mark(c, 3, st);
 
-    // ---- flips to a Delaunay mesh of the current vertices
-    // Lawson passes over the list of triangles to test; a pass with no flip has no candidate
+    // Flip helpers: Lawson passes over the list of triangles to test; a pass with no flip has no candidate
     // left (the highest candidate always wins), so three passes run per host read.
     // `last` keeps the snapshot of the final check (counters, statistics and status after the
     // passes), which the callers use instead of another read
@@ -800,17 +799,18 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
         launch_flip_all(m, opn, m.modA, st);
         flip_until_done(m.modA, C_MODN, m.modB, C_MODN2, opn);
     };
-    flip_all();
-    ev_mark(c, 4, st);
-
     // ---- missing sites
+    // Inserted into the mesh as it is after shifting (every triangle counter-clockwise, not yet
+    // Delaunay): the walks and splits only need a valid triangulation, and one flip phase over
+    // all triangles afterwards makes the mesh Delaunay, instead of flip passes after every
+    // insertion round.
     ctx_counter_set(c, C_MISS, 0);
     launch_missing_init(m, c->vor.as<int>(), c->mlist.as<int>(), c->loc.as<int>(), st);
     last = ctx_snapshot(c);
     int nmiss = last.v[C_MISS];
     S.missing = nmiss;
     for (int r = 0; r < 100000; r++) {
-        // `last`: the snapshot after the previous round's flips (or after the list was built)
+        // `last`: the snapshot after the previous round (or after the list was built)
         int left = nmiss - last.stat(C_INSERTED) - last.stat(C_DUPS);
         if (left <= 0) break;
         if (last.status())
@@ -821,11 +821,16 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
         launch_insert_round(m, c->mlist.as<int>(), n, c->loc.as<int>(), opn, st);
         launch_relink(m, m.modA, C_MODN, opn, st);
         S.insertRounds++;
-        flip_until_done(m.modA, C_MODN, m.modB, C_MODN2, opn);
+        last = ctx_snapshot(c);
     }
     S.inserted = last.stat(C_INSERTED);
     S.duplicates = last.stat(C_DUPS);
+    ev_mark(c, 4, st);
+
+    // ---- flips to a Delaunay mesh of the current vertices
+    flip_all();
     ev_mark(c, 5, st);
+    const Snap after_main = last;   // per-stage statistics: shifting, insertion and these flips
 
     // ---- frame removal + hull convexification
     // Triangles created by frame removal and convexification go to one list (each slot once);
@@ -851,6 +856,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     ev_mark(c, 6, st);
     flip_until_done(c->modF.as<int>(), C_FRAME_LIST, m.modB, C_MODN2, m.list_op);
     ev_mark(c, 7, st);
+    const Snap after_frame = last;   // per-stage statistics: frame removal and the final flips
     if (last.status()) throw GalphaError("device pipeline error (status " + std::to_string(last.status()) + ")");
 
     // ---- constraint edges (single-set runs): cavity insertion rounds, then constrained flips
@@ -955,10 +961,24 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     S.orientExact = fin.stat(C_EXACT_ORIENT);
     S.ms_pba = ev_ms(c, 0, 1);
     S.ms_recon = ev_ms(c, 1, 2);
-    S.ms_shift = ev_ms(c, 2, 3) + ev_ms(c, 3, 4);
-    S.ms_missing = ev_ms(c, 4, 5);
+    S.ms_shift = ev_ms(c, 2, 3);
+    S.ms_missing = ev_ms(c, 3, 4);
     S.ms_boundary = ev_ms(c, 5, 6);
-    S.ms_flip = ev_ms(c, 6, 7);
+    S.ms_flip = ev_ms(c, 4, 5) + ev_ms(c, 6, 7);
+    {
+        const Snap *cut[4] = {nullptr, &after_main, &after_frame, &fin};
+        for (int k = 0; k < 3; k++) {
+            auto d = [&](bool striped, int slot) -> long long {
+                long long b = striped ? cut[k + 1]->stat(slot) : cut[k + 1]->v[slot];
+                long long a = cut[k] ? (striped ? cut[k]->stat(slot) : cut[k]->v[slot]) : 0;
+                return b - a;
+            };
+            S.stageListed[k] = d(false, C_FLIP_LISTED);
+            S.stageIncircleTests[k] = d(true, C_TESTS_INCIRCLE);
+            S.stageIncircleExact[k] = d(true, C_EXACT_INCIRCLE);
+            S.stageOrientExact[k] = d(true, C_EXACT_ORIENT);
+        }
+    }
     S.ms_constraints = ev_ms(c, 7, 9);
     S.ms_canon = ev_ms(c, 9, 8);
     S.constraintRounds = c->cons_rounds;
@@ -1170,7 +1190,7 @@ __attribute__((visibility("hidden"))) size_t galpha_noalloc_need(int n, int T)
 {
     if (n < 3) return 4096;
     long long P = (long long)T * T, nf = frame_count(T), V = n + nf, cap = 3 * V + 8192;
-    long long ntiles = (long long)(T - 1) * ((T - 1 + 255) / 256), nc = P / 256 + 1;
+    long long ntiles = (long long)(T / 16) * (T / 16), nc = P / 256 + 1;
     long long kb = 2;
     while ((1LL << kb) < (long long)n) kb++;
     size_t sort_tmp = 0;
@@ -1231,7 +1251,9 @@ void galphaDestroy(galpha_ctx *c)
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     release_all(c);
-    for (int i = 0; i < 16; i++) hipEventDestroy(c->ev[i]);
+    for (int i = 0; i < 16; i++)
+        if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    alpha_destroy_events(c->alpha);
     if (c->stream) hipStreamDestroy(c->stream);
     if (c->hsnap) hipHostFree(c->hsnap);
     stage_free(c->stage);

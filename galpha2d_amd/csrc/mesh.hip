@@ -25,7 +25,16 @@
 #include "kernels.h"
 #include <algorithm>
 
-#define GRID_STRIDE(i, n) for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+// Workgroups go to the 8 XCDs round-robin by blockIdx.x, and each XCD has its own L2.  Grid-
+// stride loops renumber their blocks so that XCD k runs the blocks [k G / 8, (k + 1) G / 8) of
+// every sweep: neighbouring indices (triangle slots and vertices are numbered in spatial order)
+// then share one L2 instead of being spread over all eight.
+GA_INLINE int xcd_block()
+{
+    unsigned g = gridDim.x, b = blockIdx.x;
+    return (g & 7u) ? (int)b : (int)((b & 7u) * (g >> 3) + (b >> 3));
+}
+#define GRID_STRIDE(i, n) for (int i = xcd_block() * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 #define FIXED_GRID 2048
 
 GA_INLINE int ic(const int4 &a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
@@ -119,8 +128,27 @@ GA_INLINE long long iorient(int pa, int pb, int pc)
     return (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
 }
 
-// tiles of 256 consecutive rows of one column of 2x2 blocks; set s owns tiles
-// [bm[s], bm[s + 1]) of the BM_RECON map
+// bits at the even positions of v, packed (inverse of the cell numbering's bit spread)
+GA_INLINE unsigned compact_bits(unsigned v)
+{
+    v &= 0x55555555u;
+    v = (v | (v >> 1)) & 0x33333333u;
+    v = (v | (v >> 2)) & 0x0f0f0f0fu;
+    v = (v | (v >> 4)) & 0x00ff00ffu;
+    v = (v | (v >> 8)) & 0x0000ffffu;
+    return v;
+}
+
+// Tiles of 16 x 16 2x2-block origins, one per 16 x 16-pixel cell, in the Morton order of the
+// cells (the order the real points are numbered in), so triangles are emitted in the spatial
+// order of their vertices.  Set s owns tiles [bm[s], bm[s + 1]) of the BM_RECON map; thread k of
+// a tile takes the block at (16 cx + k / 16, 16 cy + k % 16).
+GA_INLINE void recon_block(int lt, int *x, int *y)
+{
+    *x = (int)(compact_bits((unsigned)lt >> 1) << 4) + (int)(threadIdx.x >> 4);
+    *y = (int)(compact_bits((unsigned)lt) << 4) + (int)(threadIdx.x & 15);
+}
+
 __global__ __launch_bounds__(256) void k_recon_count(const int *__restrict__ vor_all, BatchDev bd,
                                                      int *__restrict__ tile_cnt)
 {
@@ -128,11 +156,11 @@ __global__ __launch_bounds__(256) void k_recon_count(const int *__restrict__ vor
     int set = set_of(bm, bd.nsets, blockIdx.x);
     const SetDesc &sd = bd.sets[set];
     int logT = sd.logT;
-    int T = 1 << logT, nyb = (T - 1 + 255) / 256;
+    int T = 1 << logT;
     const int *vor = vor_all + sd.toff;
     int tile = blockIdx.x;
-    int lt = tile - bm[set];
-    int x = lt / nyb, y = (lt % nyb) * 256 + threadIdx.x;
+    int x, y;
+    recon_block(tile - bm[set], &x, &y);
     int c = 0;
     if (x < T - 1 && y < T - 1) {
         size_t i0 = ((size_t)x << logT) + y, i1 = i0 + T;
@@ -155,11 +183,11 @@ __global__ __launch_bounds__(256) void k_recon_emit(const int *__restrict__ vor_
     int set = set_of(bm, bd.nsets, blockIdx.x);
     const SetDesc &sd = bd.sets[set];
     int logT = sd.logT;
-    int T = 1 << logT, nyb = (T - 1 + 255) / 256;
+    int T = 1 << logT;
     const int *vor = vor_all + sd.toff;
     int tile = blockIdx.x;
-    int lt = tile - bm[set];
-    int x = lt / nyb, y = (lt % nyb) * 256 + threadIdx.x;
+    int x, y;
+    recon_block(tile - bm[set], &x, &y);
     int c = 0, a = 0, b = 0, cc = 0, d = 0;
     if (x < T - 1 && y < T - 1) {
         size_t i0 = ((size_t)x << logT) + y, i1 = i0 + T;
@@ -935,7 +963,7 @@ __global__ __launch_bounds__(256) void k_flip_test(MeshDev m, const int *list, i
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&m.counters[C_FLIP_LISTED], n);
     int tests = 0;
     int per = n > 4 * (int)(gridDim.x * 256) ? 4 : 1;
-    for (int b0 = blockIdx.x * 256 * per; b0 < n; b0 += gridDim.x * 256 * per) {
+    for (int b0 = xcd_block() * 256 * per; b0 < n; b0 += gridDim.x * 256 * per) {
         unsigned bits = 0;   // bit 3j + i: edge i of the j-th triangle of this thread wants a flip
         int tj[4];
 #pragma unroll

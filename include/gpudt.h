@@ -185,6 +185,13 @@ typedef struct {
        and the constrained flips) */
     int constraintRounds;
     double ms_constraints;
+    /* alpha sub-stages: edges, rho and attachment; spectrum order, ranks and values; intervals
+       and the master list */
+    double ms_alpha_rho, ms_alpha_order, ms_alpha_lists;
+    /* per mesh stage (0: shifting, missing-site insertion and the flips after them; 1: frame
+       removal and the final flips; 2: constraints and canonical output): triangles listed for
+       flip tests, incircle tests, exact incircle and exact orientation evaluations */
+    long long stageListed[3], stageIncircleTests[3], stageIncircleExact[3], stageOrientExact[3];
 } galpha_stats;
 
 galpha_ctx *galphaCreate(int device);

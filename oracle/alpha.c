@@ -56,7 +56,7 @@ double ora_rho_edge(const double *pa, const double *pb)
     return (ax + ay) / 4;
 }
 
-static long long g_exact_attach;
+static __thread long long g_exact_attach;   /* per thread: the oracle may run on several threads */
 
 /* filtered attachment: the filter decides when |d1 - d2| exceeds a bound on its rounding
  * error; otherwise the exact expansion decides (>= 0 means attached). */
@@ -82,10 +82,10 @@ typedef struct {
     int st, idx;     /* st 0 = edge, 1 = triangle */
 } sent;
 
-static const double *g_xy;
-static const ora_tri *g_t;
-static const int *g_et, *g_ev;
-static long long g_exact_cmp;
+static __thread const double *g_xy;
+static __thread const ora_tri *g_t;
+static __thread const int *g_et, *g_ev;
+static __thread long long g_exact_cmp;
 
 static void ent_pts(const sent *e, const double **p0, const double **p1, const double **p2)
 {

@@ -129,9 +129,9 @@ static int insert_point(mesh_t *m, int ip, int iter, int *first_of, int nvert)
     if (!in_conflict(m, &m->t[t0], ip)) return -2;  /* cannot happen with exact predicates */
 
     /* flood the conflict region */
-    static int *stack = NULL; static int scap = 0;
-    static bedge *bnd = NULL; static int bcap = 0;
-    static int *dead = NULL; static int dcap = 0;
+    static __thread int *stack = NULL; static __thread int scap = 0;   /* per thread, kept across calls */
+    static __thread bedge *bnd = NULL; static __thread int bcap = 0;
+    static __thread int *dead = NULL; static __thread int dcap = 0;
     int sp = 0, nb = 0, nd = 0;
     if (!scap) { scap = 1024; stack = (int *)malloc(sizeof(int) * scap); }
     if (!bcap) { bcap = 1024; bnd = (bedge *)malloc(sizeof(bedge) * bcap); }
@@ -214,7 +214,7 @@ static int cmp_keyidx(const void *a, const void *b)
     if (p->key != q->key) return p->key < q->key ? -1 : 1;
     return p->idx - q->idx;
 }
-static const double *g_xy;
+static __thread const double *g_xy;
 static int cmp_xy(const void *a, const void *b)
 {
     int i = *(const int *)a, j = *(const int *)b;

@@ -20,7 +20,7 @@
 static double splitter, epsilon;
 static double ccwerrboundA, iccerrboundA;
 static int inited = 0;
-static long long n_orient, n_orient_exact, n_incircle, n_incircle_exact, n_sos;
+static __thread long long n_orient, n_orient_exact, n_incircle, n_incircle_exact, n_sos;   /* per thread */
 
 void ora_exactinit(void)
 {
