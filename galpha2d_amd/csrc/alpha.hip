@@ -28,6 +28,7 @@
 #include "kernels.h"
 #include "alpha_host.h"
 #include <algorithm>
+#include <initializer_list>
 #include <vector>
 
 #define GS(i, n) for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
@@ -277,10 +278,11 @@ __global__ void k_tri_rho(AlphaK a)
     }
 }
 
-__global__ void k_attached(AlphaK a)
+__global__ void k_attached(AlphaK a, int Emax)
 {
     int E = a.eoff[a.F - 1] + a.cnt[a.F - 1];
-    GS(e, E) {
+    GS(e, Emax) {
+        if (e >= E) { a.flag[e] = 0; continue; }   // the scan of unattached edges runs to Emax
         int t = a.edge_t[e], i = a.edge_v[e];
         double2 p = a.xy[tv(a, t, pl1(i))], q = a.xy[tv(a, t, mi1(i))];
         int nb = tn(a, t, i);
@@ -784,9 +786,12 @@ __global__ void k_spec_pointers(int Mk, const int *pflag, const int *ppos, int *
     if (blockIdx.x == 0 && threadIdx.x == 0) sp[0] = -1;
 }
 
-// slots without a member (rank 0 of every set) read as 0; ~0 is a NaN pattern, never a value
-__global__ void k_spec_values(const unsigned long long *bits, int nslots, double *spec)
+// slots without a member (rank 0 of every set) read as 0; ~0 is a NaN pattern, never a value.
+// nslots < 0: a single set whose rank count R is incl[0] (the inclusive rank of the last entry),
+// nslots = R + 1
+__global__ void k_spec_values(const unsigned long long *bits, int nslots, const int *incl_last, double *spec)
 {
+    if (nslots < 0) nslots = *incl_last + 1;
     GS(r, nslots) spec[r] = bits[r] == ~0ull ? 0.0 : __longlong_as_double((long long)bits[r]);
 }
 
@@ -822,7 +827,7 @@ enum {
     AB_ID2, AB_LO, AB_HI, AB_SLO, AB_SHI, AB_FLAG, AB_RANK, AB_SPECBITS, AB_VMIN, AB_VMAX, AB_STATUS, AB_CNTRS,
     AB_TMP, AB_SCAN, AB_NEG, AB_PMAX, AB_NSMIN, AB_MKEY, AB_MVAL, AB_MFLAG, AB_MPOS, AB_OKEY, AB_OVAL,
     AB_SKEY, AB_SVAL, AB_SPEC, AB_SP, AB_MISC, AB_BIG0, AB_BIG1, AB_BIG2, AB_BIG3, AB_BIGL,
-    AB_BS0, AB_BS1, AB_BS2, AB_BS3, AB_BS4, AB_BSET, AB_MSET, AB_VHULL, AB_RANKQ, AB_COUNT
+    AB_BS0, AB_BS1, AB_BS2, AB_BS3, AB_BS4, AB_BSET, AB_MSET, AB_VHULL, AB_RANKQ, AB_RD, AB_COUNT
 };
 
 static void ab(AlphaDev &d, int i, size_t bytes)
@@ -849,11 +854,40 @@ void alpha_destroy_events(AlphaDev &d)
 {
     for (auto &e : d.ev)
         if (e) { hipEventDestroy(e); e = nullptr; }
+    if (d.hpin) { hipHostFree(d.hpin); d.hpin = nullptr; }
 }
 
 static void chk(hipError_t e, const char *w)
 {
     if (e != hipSuccess) throw GalphaError(std::string("alpha: ") + w + ": " + hipGetErrorString(e));
+}
+
+// A few device ints in one transfer: one wave gathers them into a device buffer and one copy
+// brings them to pinned host memory (a copy into pageable memory waits for a round trip of its
+// own, about 25 us each).
+struct IntReads {
+    const int *p[16];
+    int n;
+};
+__global__ void k_gather_ints(IntReads r, int *dst)
+{
+    int i = threadIdx.x;
+    if (i < r.n) dst[i] = *r.p[i];
+}
+
+static void read_ints(AlphaDev &d, hipStream_t st, std::initializer_list<const int *> ptrs, int *out)
+{
+    IntReads r{};
+    for (const int *q : ptrs) {
+        if (r.n == 16) throw GalphaError("alpha: too many values in one read");
+        r.p[r.n++] = q;
+    }
+    if (!d.hpin) chk(hipHostMalloc((void **)&d.hpin, 16 * sizeof(int), hipHostMallocDefault), "pinned alloc");
+    ab(d, AB_RD, 16 * sizeof(int));
+    GA_LAUNCH(k_gather_ints, dim3(1), dim3(64), 0, st, r, AP<int>(d, AB_RD));
+    chk(hipMemcpyAsync(d.hpin, d.b[AB_RD].p, 4 * (size_t)r.n, hipMemcpyDeviceToHost, st), "copy");
+    chk(hipStreamSynchronize(st), "sync");
+    memcpy(out, d.hpin, 4 * (size_t)r.n);
 }
 
 static void big_clusters(AlphaDev &d, AlphaK &a, int S0, hipStream_t st)
@@ -973,18 +1007,21 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     if (info.check_mesh) GA_LAUNCH(k_mesh_check, dim3(AGRID), dim3(256), 0, st, a);
     GA_LAUNCH(k_edge_count, dim3(AGRID), dim3(256), 0, st, a);
     launch_exclusive_scan(a.cnt, a.eoff, F, d.b[AB_SCAN].p, st);
-    int h2[3] = {0, 0, 0};
-    chk(hipMemcpyAsync(&h2[0], a.eoff + F - 1, 4, hipMemcpyDeviceToHost, st), "copy");
-    chk(hipMemcpyAsync(&h2[1], a.cnt + F - 1, 4, hipMemcpyDeviceToHost, st), "copy");
-    if (info.check_mesh) chk(hipMemcpyAsync(&h2[2], a.counters + A_BADMESH, 4, hipMemcpyDeviceToHost, st), "copy");
-    chk(hipStreamSynchronize(st), "sync");
-    if (h2[2])
-        throw GalphaError("alpha: the triangles are not a mesh of the points (" + std::to_string(h2[2]) +
-                          " triangles with a vertex or neighbour out of range, or a neighbour link that does not "
-                          "link back over the same edge)");
-    int E = h2[0] + h2[1];
-    d.E = E;
-    size_t NE = (size_t)F + E;
+    if (info.check_mesh) {   // caller triangles: no kernel follows their links before the check
+        int bad = 0;
+        chk(hipMemcpyAsync(&bad, a.counters + A_BADMESH, 4, hipMemcpyDeviceToHost, st), "copy");
+        chk(hipStreamSynchronize(st), "sync");
+        if (bad)
+            throw GalphaError("alpha: the triangles are not a mesh of the points (" + std::to_string(bad) +
+                              " triangles with a vertex or neighbour out of range, or a neighbour link that does "
+                              "not link back over the same edge)");
+    }
+    // Edge arrays are sized by a bound and the edge count E is read with the unattached count
+    // below: a triangulation of n points with F triangles has E = n' + F - 1 <= n + F edges (n'
+    // points in it); a caller mesh has at most 3F.
+    const int Emax = (int)std::min<long long>(3LL * F, info.check_mesh ? 3LL * F : (long long)n + F);
+    int E = Emax;
+    size_t NE = (size_t)F + Emax;
     ab(d, AB_ET, 4 * (size_t)E); ab(d, AB_EV, 4 * (size_t)E); ab(d, AB_EATT, 4 * (size_t)E);
     ab(d, AB_ERANK, 4 * (size_t)E); ab(d, AB_ENMIN, 4 * (size_t)E); ab(d, AB_ENMAX, 4 * (size_t)E);
     ab(d, AB_TRANK, 4 * (size_t)F);
@@ -1009,14 +1046,15 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     a.vnumin = AP<int>(d, AB_VMIN); a.vnumax = AP<int>(d, AB_VMAX);
     GA_LAUNCH(k_edges, dim3(AGRID), dim3(256), 0, st, a);
     GA_LAUNCH(k_tri_rho, dim3(AGRID), dim3(256), 0, st, a);
-    GA_LAUNCH(k_attached, dim3(AGRID), dim3(256), 0, st, a);
-    // positions of unattached edges
-    launch_exclusive_scan(a.flag, a.rank, E, d.b[AB_SCAN].p, st);
-    int h3[2];
-    chk(hipMemcpyAsync(&h3[0], a.rank + E - 1, 4, hipMemcpyDeviceToHost, st), "copy");
-    chk(hipMemcpyAsync(&h3[1], a.flag + E - 1, 4, hipMemcpyDeviceToHost, st), "copy");
-    chk(hipStreamSynchronize(st), "sync");
+    GA_LAUNCH(k_attached, dim3(AGRID), dim3(256), 0, st, a, Emax);
+    // positions of unattached edges (flags past E are zero)
+    launch_exclusive_scan(a.flag, a.rank, Emax, d.b[AB_SCAN].p, st);
+    int h3[4];
+    read_ints(d, st, {a.rank + Emax - 1, a.flag + Emax - 1, a.eoff + F - 1, a.cnt + F - 1}, h3);
     int nun = h3[0] + h3[1];
+    E = h3[2] + h3[3];
+    if (E > Emax) throw GalphaError("alpha: more edges than the Euler bound (" + std::to_string(E) + ")");
+    d.E = E;
     int S0 = F + nun;
     d.S = S0;
     GA_LAUNCH(k_edge_entries, dim3(AGRID), dim3(256), 0, st, a, E);
@@ -1072,18 +1110,17 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     GA_LAUNCH(k_copy_i32, dim3(AGRID), dim3(256), 0, st, (const int *)a.flag, a.gflag, S0);
     GA_LAUNCH(k_clusters, dim3(AGRID), dim3(256), 0, st, a, S0);
     big_clusters(d, a, S0, st);
-    // ranks: inclusive scan of group starts
+    // ranks: inclusive scan of group starts.  A single set reads its rank count R with the final
+    // counters (buffers are sized by R <= S0); a batch reads it here.
     launch_exclusive_scan(a.gflag, a.rank, S0, d.b[AB_SCAN].p, st);
-    int h4[2];
-    chk(hipMemcpyAsync(&h4[0], a.rank + S0 - 1, 4, hipMemcpyDeviceToHost, st), "copy");
-    chk(hipMemcpyAsync(&h4[1], a.gflag + S0 - 1, 4, hipMemcpyDeviceToHost, st), "copy");
-    chk(hipStreamSynchronize(st), "sync");
-    int R = h4[0] + h4[1];
-    int nslots = R + 1;
+    int R = -1, nslots = -1;
     if (nsets == 1) {
-        d.specSize = R + 1;
-        d.specstart = {0, R + 1};
-        chk(hipMemsetAsync(a.specbits, 0xff, 8 * (size_t)(R + 2), st), "memset");
+        chk(hipMemsetAsync(a.specbits, 0xff, 8 * (size_t)(S0 + 2), st), "memset");
+    } else {
+        int h4[2];
+        read_ints(d, st, {a.rank + S0 - 1, a.gflag + S0 - 1}, h4);
+        R = h4[0] + h4[1];
+        nslots = R + 1;
     }
     // incl = excl + flag: reuse flag buffer as inclusive ranks
     int *incl = a.gflag;
@@ -1129,9 +1166,10 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
         GA_LAUNCH(k_ranks_batch, dim3(AGRID), dim3(256), 0, st, a, S0, (const unsigned *)sset2, (const int *)order2,
                   (const int *)incl2, (const int *)first);
     }
-    ab(d, AB_SPEC, 8 * (size_t)(nslots + 1));
+    const int slot_bound = nsets == 1 ? S0 + 1 : nslots;   // spectrum slots, or a bound of them
+    ab(d, AB_SPEC, 8 * (size_t)(slot_bound + 1));
     GA_LAUNCH(k_spec_values, dim3(AGRID), dim3(256), 0, st, (const unsigned long long *)a.specbits, nslots,
-                       AP<double>(d, AB_SPEC));
+              (const int *)(incl + S0 - 1), AP<double>(d, AB_SPEC));
     chk(hipEventRecord(d.ev[2], st), "event");
     // intervals
     launch_fill_i32(a.vnumin, 0x7fffffff, (size_t)n, st);
@@ -1149,9 +1187,7 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     GA_LAUNCH(k_master_keys, dim3(AGRID), dim3(256), 0, st, a, E, M, mkey, mval, mflag);
     launch_exclusive_scan(mflag, mpos, (size_t)M, d.b[AB_SCAN].p, st);
     int h5[2];
-    chk(hipMemcpyAsync(&h5[0], mpos + M - 1, 4, hipMemcpyDeviceToHost, st), "copy");
-    chk(hipMemcpyAsync(&h5[1], mflag + M - 1, 4, hipMemcpyDeviceToHost, st), "copy");
-    chk(hipStreamSynchronize(st), "sync");
+    read_ints(d, st, {mpos + M - 1, mflag + M - 1}, h5);
     int Mk = h5[0] + h5[1];
     d.M = Mk;
     ab(d, AB_OKEY, 4 * (size_t)Mk + 16); ab(d, AB_OVAL, 4 * (size_t)Mk + 16);
@@ -1161,7 +1197,7 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     GA_LAUNCH(k_master_compact, dim3(AGRID), dim3(256), 0, st, M, (const unsigned *)mkey, (const int *)mval,
                        (const int *)mflag, (const int *)mpos, okey, oval);
     int bits = 1;
-    while ((1LL << bits) <= (long long)nslots) bits++;
+    while ((1LL << bits) <= (long long)slot_bound) bits++;
     tb = 0;
     chk(rocprim::radix_sort_pairs(nullptr, tb, okey, skey, oval, sval, (unsigned)Mk, 0u, (unsigned)bits, st), "sort");
     ab(d, AB_TMP, tb + 256);
@@ -1172,15 +1208,10 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     GA_LAUNCH(k_master_decode, dim3(AGRID), dim3(256), 0, st, n, E, Mk, (const unsigned *)skey, (const int *)sval,
                        msimp, mst, mat, mflag);
     launch_exclusive_scan(mflag, mpos, (size_t)Mk, d.b[AB_SCAN].p, st);
-    int h6[2] = {0, 0};
-    if (Mk > 0) {
-        chk(hipMemcpyAsync(&h6[0], mpos + Mk - 1, 4, hipMemcpyDeviceToHost, st), "copy");
-        chk(hipMemcpyAsync(&h6[1], mflag + Mk - 1, 4, hipMemcpyDeviceToHost, st), "copy");
-    }
-    chk(hipStreamSynchronize(st), "sync");
-    int np = h6[0] + h6[1];
-    d.nsp = np + 1;
-    ab(d, AB_SP, 4 * (size_t)(np + 2));
+    // spec_pointers: one entry per distinct key, and every rank >= 1 of a set holds at least the
+    // entry of its own value, so there are slot_bound - nsets of them (read with the final
+    // counters for a single set)
+    ab(d, AB_SP, 4 * (size_t)(slot_bound + 2));
     GA_LAUNCH(k_spec_pointers, dim3(AGRID), dim3(256), 0, st, Mk, (const int *)mflag, (const int *)mpos,
                        AP<int>(d, AB_SP));
     if (nsets > 1) {
@@ -1205,10 +1236,22 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
         d.efirst = {0, E};
     }
     chk(hipEventRecord(d.ev[3], st), "event");
-    int hc[8], hs = 0;
-    chk(hipMemcpyAsync(hc, a.counters, sizeof(hc), hipMemcpyDeviceToHost, st), "copy");
-    chk(hipMemcpyAsync(&hs, a.status, 4, hipMemcpyDeviceToHost, st), "copy");
-    chk(hipStreamSynchronize(st), "sync");
+    // counters, status, the single set's rank count and the spec-pointer count in one read
+    int fin[12];
+    const int *zero = a.counters + A_BADMESH;   // 0 here: only caller meshes with errors set it
+    read_ints(d, st,
+              {a.counters + 0, a.counters + 1, a.counters + 2, a.counters + 3, a.counters + 4, a.counters + 5,
+               a.counters + 6, a.counters + 7, a.status, nsets == 1 ? incl + S0 - 1 : zero,
+               Mk > 0 ? mpos + Mk - 1 : zero, Mk > 0 ? mflag + Mk - 1 : zero},
+              fin);
+    int *hc = fin, hs = fin[8], hr[3] = {fin[9], fin[10], fin[11]};
+    if (nsets == 1) {
+        R = hr[0];
+        nslots = R + 1;
+        d.specSize = R + 1;
+        d.specstart = {0, R + 1};
+    }
+    d.nsp = hr[1] + hr[2] + 1;
     if (stats) {
         stats->nEdges = E;
         stats->specSize = R + 1;

@@ -67,6 +67,7 @@ struct AlphaDev {
     std::vector<int> poff, tfirst, efirst, specstart, mfirst;
     const int *tris9 = nullptr;   // device triangles of the last run (the context's output)
     hipEvent_t ev[4] = {};        // sub-stage boundaries (created on the first run)
+    int *hpin = nullptr;          // pinned host ints for counter reads (allocated on the first run)
 };
 
 void alpha_run(AlphaDev &a, const double *d_xy, int n, const int *d_tris9, int F, hipStream_t st,
@@ -86,4 +87,5 @@ int alpha_rank_of(AlphaDev &a, double alpha, HostStage &stage, hipStream_t st);
 void alpha_classify(AlphaDev &a, int rank, unsigned char *vcls, unsigned char *ecls, unsigned char *tcls,
                     hipStream_t st);
 void alpha_release(AlphaDev &a);
+// events and pinned host memory of the alpha stage
 void alpha_destroy_events(AlphaDev &a);

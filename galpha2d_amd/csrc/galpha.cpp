@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -110,6 +111,7 @@ struct galpha_ctx {
     // constrained DT: input pairs, vertex-id pairs, inverse numbering, flags, walk counts, region
     // sizes and offsets, cavity regions, constraint key set
     DevBuf kin, kv, kinv, kdone, kcnt, krs, kroff, kwork, khash;
+    DevBuf rdbuf;   // segments gathered for one host read
     std::vector<int> cons;            // constraint pairs (input indices) for single-set runs
     int cons_rounds = 0;              // insertion rounds of the last constrained run
     std::vector<int> set_first;       // [nsets + 1] first output triangle of each planned set
@@ -209,6 +211,33 @@ static void ctx_read(galpha_ctx *c, void *dst, const void *src, size_t bytes)
     stage_d2h(c->stage, dst, src, bytes, c->stream);
 }
 static int ctx_status(galpha_ctx *c) { return ctx_counter(c, C_STATUS); }
+
+// Device int segments (up to four) in one host read: gathered into one device array, then one copy
+// through pinned memory and one wait (each copy into pageable memory waits a round trip of its own).
+static void ctx_read_segments(galpha_ctx *c, std::initializer_list<std::pair<const int *, int>> segs, int *dst)
+{
+    IntSegments g{};
+    size_t tot = 0;
+    for (auto &sg : segs) {
+        if (g.n == 4) throw GalphaError("too many segments in one read");
+        g.src[g.n] = sg.first;
+        g.len[g.n++] = sg.second;
+        tot += (size_t)sg.second;
+    }
+    if (tot == 0) return;
+    c->rdbuf.ensure(4 * tot + 64);
+    launch_gather_segments(g, c->rdbuf.as<int>(), c->stream);
+    if (tot <= (size_t)SNAP_INTS) {
+        HIPCHK(hipMemcpyAsync(c->hsnap, c->rdbuf.p, 4 * tot, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        memcpy(dst, c->hsnap, 4 * tot);
+    } else if (t_arena) {   // no pinned staging in a no-allocation run
+        HIPCHK(hipMemcpyAsync(dst, c->rdbuf.p, 4 * tot, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } else {
+        stage_d2h(c->stage, dst, c->rdbuf.p, 4 * tot, c->stream);
+    }
+}
 static void ctx_counter_set(galpha_ctx *c, int slot, int v)
 {
     HIPCHK(hipMemsetD32Async((hipDeviceptr_t)(c->counters.as<int>() + slot), v, 1, c->stream));
@@ -519,12 +548,10 @@ static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t
     launch_fill_i32(hstart, 0x7fffffff, (size_t)nsets, st);
     launch_reset_counters(c->counters.as<int>(), C_BND, -1, -1, -1, st);
     launch_bnd_detect(m, outdeg, cover, hcnt, hstart, badset, blist, bidx, st);
-    std::vector<int> bad(nsets), cnt(nsets);
-    int B = 0;
-    HIPCHK(hipMemcpyAsync(bad.data(), badset, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(cnt.data(), hcnt, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&B, c->counters.as<int>() + C_BND, 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    std::vector<int> rd(2 * (size_t)nsets + 1);
+    ctx_read_segments(c, {{badset, nsets}, {hcnt, nsets}, {c->counters.as<int>() + C_BND, 1}}, rd.data());
+    std::vector<int> bad(rd.begin(), rd.begin() + nsets), cnt(rd.begin() + nsets, rd.begin() + 2 * nsets);
+    int B = rd[2 * (size_t)nsets];
     long long tot = 0;
     int maxh = 0;
     std::vector<int> off(nsets);
@@ -538,31 +565,40 @@ static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t
     }
     if (tot != B) return false;
     (void)n;
-    c->bnd.ensure(4 * (size_t)((11 * (size_t)B + 3) & ~(size_t)3) + 16 * (size_t)B + 64);   // + cycle positions
+    // cycle arrays, the cycle's positions, then 5 doubles of grid parameters per set
+    c->bnd.ensure(4 * (size_t)((11 * (size_t)B + 3) & ~(size_t)3) + 16 * (size_t)B + 40 * (size_t)nsets + 64);
     HIPCHK(hipMemcpyAsync(c->hullOff.p, off.data(), 4 * (size_t)nsets, hipMemcpyHostToDevice, st));
     int rounds = 1;
     while ((1 << rounds) < maxh) rounds++;
     rounds += 1;
     launch_bnd_rank(m, B, rounds, blist, bidx, hstart, hcnt, c->hullOff.as<int>(), badset, c->bnd.as<int>(), st);
-    int collinear = 0;
-    HIPCHK(hipMemcpyAsync(bad.data(), badset, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&collinear, c->counters.as<int>() + C_BND, 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    ctx_read_segments(c, {{badset, nsets}, {c->counters.as<int>() + C_BND, 1}}, rd.data());
+    std::copy(rd.begin(), rd.begin() + nsets, bad.begin());
+    int collinear = rd[nsets];
     for (int s = 0; s < nsets; s++)
         if (bad[s]) return false;
     // many exactly collinear boundary triples (lattices): ears would mostly be blocked
     if (collinear > std::max(8, B / 100)) return false;
     // from here on the mesh changes
     launch_kill_frame(m, st);
-    // rounds after one without candidates return at once, so four run per read
-    ctx_counter_set(c, C_EAR, -1);
+    int *bnd = c->bnd.as<int>(), *hoff = c->hullOff.as<int>();
+    if (nsets >= 32) {
+        // many sets: every set's rounds in one workgroup, one launch (the most rounds a set ran is
+        // read with the next snapshot)
+        launch_reset_counters(c->counters.as<int>(), C_EAR, -1, -1, -1, st);
+        launch_ear_sets(m, B, bnd, hcnt, hoff, nsets, c->modF.as<int>(), C_FRAME_LIST, c->op++, st);
+        *ear_rounds = -1;
+        return true;
+    }
+    // few sets: grid-wide rounds, four per host read (a round after one without candidates
+    // returns at once)
+    launch_ear_grid(m, B, bnd, hcnt, hoff, nsets, st);
+    launch_reset_counters(c->counters.as<int>(), C_EAR, -1, -1, -1, st);
     int round = 0;
     for (;; round += 4) {
         if (round > 200000) throw GalphaError("hull convexification did not finish");
-        for (int k = 0; k < 4; k++)
-            launch_ear_round(m, B, c->bnd.as<int>(), hcnt, c->hullOff.as<int>(), round + k, c->modF.as<int>(),
-                             C_FRAME_LIST, c->op++, st);
-        if (ctx_counter(c, C_EAR) == 0) break;
+        launch_ear_rounds(m, B, bnd, hoff, round, 4, c->modF.as<int>(), C_FRAME_LIST, c->op++, st);
+        if (ctx_counter(c, ((round + 3) & 1) ? C_EAR_PREV : C_EAR) == 0) break;
     }
     *ear_rounds = round + 4;
     return true;
@@ -857,6 +893,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     flip_until_done(c->modF.as<int>(), C_FRAME_LIST, m.modB, C_MODN2, m.list_op);
     ev_mark(c, 7, st);
     const Snap after_frame = last;   // per-stage statistics: frame removal and the final flips
+    if (S.frameBulk && S.earRounds < 0) S.earRounds = after_frame.v[C_EAR];
     if (last.status()) throw GalphaError("device pipeline error (status " + std::to_string(last.status()) + ")");
 
     // ---- constraint edges (single-set runs): cavity insertion rounds, then constrained flips
@@ -939,12 +976,12 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
                       c->counters.as<int>() + C_FOLD + C_RECON_TRIS, c->out9.as<int>(), c->setFirst.as<int>(),
                       c->keys2.as<int>(), st);   // keys2 is free after the sort: output sets
     ev_mark(c, 8, st);
-    std::vector<int> sf(nsets);
     Snap fin;
-    HIPCHK(hipMemcpyAsync(sf.data(), c->setFirst.p, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(c->hsnap, c->counters.p, sizeof(fin.v), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    memcpy(fin.v, c->hsnap, sizeof(fin.v));
+    std::vector<int> sfv((size_t)nsets + sizeof(fin.v) / 4);   // first triangle of every set, then the counters
+    ctx_read_segments(c, {{c->setFirst.as<int>(), nsets}, {c->counters.as<int>(), (int)(sizeof(fin.v) / 4)}},
+                      sfv.data());
+    memcpy(fin.v, sfv.data() + nsets, sizeof(fin.v));
+    const int *sf = sfv.data();
     if (fin.status()) throw GalphaError("canonical output failed (status " + std::to_string(fin.status()) + ")");
     int F = fin.stat(C_RECON_TRIS);
     // sets without triangles start where the next set starts
@@ -1135,7 +1172,7 @@ static void release_all(galpha_ctx *c)
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
                       &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd, &c->plo,
                       &c->perm, &c->xys, &c->pcell, &c->ccount, &c->ccursor, &c->cls,
-                      &c->kin, &c->kv, &c->kinv, &c->kdone, &c->kcnt, &c->krs, &c->kroff, &c->kwork, &c->khash};
+                      &c->kin, &c->kv, &c->kinv, &c->kdone, &c->kcnt, &c->krs, &c->kroff, &c->kwork, &c->khash, &c->rdbuf};
     for (DevBuf *b : bufs) b->release();
     alpha_release(c->alpha);
 }
@@ -1154,7 +1191,7 @@ size_t galpha_ctx_device_bytes(galpha_ctx *c)
                       &c->setsD, &c->bmD, &c->vset, &c->hullCnt, &c->hullOff, &c->hullStart, &c->setFirst,
                       &c->offsD, &c->bboxTmp, &c->lmark, &c->ffan, &c->modF, &c->badset, &c->bnd, &c->plo,
                       &c->perm, &c->xys, &c->pcell, &c->ccount, &c->ccursor, &c->cls,
-                      &c->kin, &c->kv, &c->kinv, &c->kdone, &c->kcnt, &c->krs, &c->kroff, &c->kwork, &c->khash};
+                      &c->kin, &c->kv, &c->kinv, &c->kdone, &c->kcnt, &c->krs, &c->kroff, &c->kwork, &c->khash, &c->rdbuf};
     size_t t = 0;
     for (DevBuf *b : bufs) t += b->bytes;
     for (auto &b : c->alpha.b) t += b.bytes;

@@ -186,6 +186,14 @@ void launch_pba(const PbaBuffers &p, const BatchDev &b, hipStream_t st);
 void launch_islands(const int *vor, const int *pix, const BatchDev &b, int *bad, int *flag, hipStream_t st);
 void launch_remove_bad(int *bad, const int *pix, const BatchDev &b, int *smap, int *removed, hipStream_t st);
 
+// util.hip: up to four segments of device ints copied into one contiguous device array
+struct IntSegments {
+    const int *src[4];
+    int len[4];
+    int n;
+};
+void launch_gather_segments(const IntSegments &g, int *dst, hipStream_t st);
+
 // scan.hip
 size_t scan_temp_bytes(size_t n);
 void launch_exclusive_scan(const int *in, int *out, size_t n, void *temp, hipStream_t st);
@@ -231,8 +239,14 @@ void launch_bnd_detect(MeshDev &m, int *outdeg, int *cover, int *hull_cnt, int *
 void launch_bnd_rank(MeshDev &m, int B, int rounds, const int *blist, const int *bidx, const int *hull_start,
                      const int *hull_cnt, const int *hull_off, int *badset, int *bnd, hipStream_t st);
 void launch_kill_frame(MeshDev &m, hipStream_t st);
-void launch_ear_round(MeshDev &m, int B, int *bnd, const int *hull_cnt, const int *hull_off, int round,
-                      int *list_out, int cnt_out_slot, int op, hipStream_t st);
+// bulk frame removal's ear rounds: the per-set grids, then rounds [r0, r0 + k) as grid-wide kernels
+// (counters[C_EAR] / [C_EAR_PREV] count the candidates of even / odd rounds)
+void launch_ear_grid(MeshDev &m, int B, int *bnd, const int *hull_cnt, const int *hull_off, int nsets, hipStream_t st);
+void launch_ear_rounds(MeshDev &m, int B, int *bnd, const int *hull_off, int r0, int k, int *list_out,
+                       int cnt_out_slot, int op, hipStream_t st);
+// every set's ear rounds in one workgroup per set (bulk frame removal); counters[C_EAR] = most rounds
+void launch_ear_sets(MeshDev &m, int B, int *bnd, const int *hull_cnt, const int *hull_off, int nsets,
+                     int *list_out, int cnt_out_slot, int op, hipStream_t st);
 void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, int *hull_cnt, int *hull_off,
                       int *hull_start, void *scan_tmp, hipStream_t st);
 // canonical order: key (v0 << B) | v1 of input indices below 2^B, sorted on its low 2B bits

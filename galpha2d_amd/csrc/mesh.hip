@@ -1527,57 +1527,18 @@ __global__ void k_kill_frame(MeshDev m)
     }
 }
 
+// bnd layout: succ[2][B], d[2][B], verts, owner, nxt, prv, alive, pset, cand (win reuses d[0]), then
+// the positions of the cycle (double2[B]) from int offset bnd_pos_off(B), 16-byte aligned
+GA_INLINE_HOST_DEVICE int bnd_pos_off(int B) { return (11 * B + 3) & ~3; }
+
 // Ear rounds.  A cycle position p (vertex C between P = prev and N = next) is a candidate when C
 // is reflex (P, C, N clockwise) and no other live vertex of its cycle lies in the closed triangle
-// (P, N, C); one wave tests emptiness.  A round after one without candidates has nothing to do
-// (nothing changed), so its kernels return at once and the host checks only every few rounds.
-__global__ void k_ear_round_start(int *c)
-{
-    if (threadIdx.x == 0) {
-        c[C_EAR_PREV] = c[C_EAR];
-        c[C_EAR] = 0;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_ear_cand(MeshDev m, int B, const double2 *bpos, const int *nxt,
-                                                  const int *prv, const int *alive, const int *pset,
-                                                  const int *hull_cnt, const int *hull_off, int *cand,
-                                                  int *ncand)
-{
-    if (m.counters[C_EAR_PREV] == 0) return;
-    int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
-    int nw = (gridDim.x * blockDim.x) >> 6;
-    for (int p = wave; p < B; p += nw) {
-        int c = 0;
-        if (alive[p]) {
-            int kp = prv[p], kn = nxt[p];
-            if (kp != kn) {
-                double2 P = bpos[kp], C = bpos[p], N = bpos[kn];
-                if (orient_dev(m, P, C, N) < 0) {
-                    int s = pset[p], off = hull_off[s], h = hull_cnt[s];
-                    double lx = fmin(P.x, fmin(C.x, N.x)), hx = fmax(P.x, fmax(C.x, N.x));
-                    double ly = fmin(P.y, fmin(C.y, N.y)), hy = fmax(P.y, fmax(C.y, N.y));
-                    bool blocked = false;
-                    // positions of the cycle in cycle order: one coalesced 16-byte read per lane;
-                    // the liveness test only for the few inside the ear's box
-                    for (int q = off + lane; q < off + h; q += 64) {
-                        double2 Q = bpos[q];
-                        if (Q.x < lx || Q.x > hx || Q.y < ly || Q.y > hy) continue;
-                        if (q == p || q == kp || q == kn || !alive[q]) continue;
-                        if (orient_dev(m, P, N, Q) >= 0 && orient_dev(m, N, C, Q) >= 0 && orient_dev(m, C, P, Q) >= 0)
-                            blocked = true;
-                    }
-                    c = __ballot(blocked) ? 0 : 1;
-                }
-            }
-        }
-        if (lane == 0) {
-            cand[p] = c;
-            if (c) atomicAdd(ncand, 1);
-        }
-    }
-}
-
+// (P, N, C); winners are candidates above every candidate within cycle distance 2 (so two winners
+// share no vertex), and each winner cuts its ear.  Rounds repeat until one has no candidate: the
+// cycle is then convex.  The emptiness test looks only at the vertices in the grid cells the ear's
+// bounding box covers: a uniform grid over each set's cycle, about two vertices per cell, built by
+// a counting sort before the first round.  Few large sets run their rounds as grid-wide kernels
+// (three per round); many small sets run all their rounds in one launch, one workgroup per set.
 GA_INLINE uint32_t ear_prio(int p, int round) { return hash32((uint32_t)p * 0x9E3779B1u ^ (uint32_t)round); }
 GA_INLINE bool ear_beats(int p, int q, int round)
 {
@@ -1585,52 +1546,291 @@ GA_INLINE bool ear_beats(int p, int q, int round)
     return a > b || (a == b && p > q);
 }
 
-// winners: candidates above every candidate within cycle distance 2 (so two winners share no vertex)
-__global__ void k_ear_select(int B, const int *nxt, const int *prv, const int *cand, int round, int *win,
-                             const int *counters)
+GA_INLINE int ear_cell(double v, double lo, double sc, int G)
 {
-    if (counters[C_EAR] == 0) return;
-    GRID_STRIDE(p, B) {
-        int w = 0;
-        if (cand[p]) {
-            int a = prv[p], b = nxt[p];
-            int q[4] = {a, b, prv[a], nxt[b]};
-            w = 1;
-            for (int k = 0; k < 4; k++)
-                if (q[k] != p && cand[q[k]] && !ear_beats(p, q[k], round)) w = 0;
+    int c = (int)((v - lo) * sc);
+    return c < 0 ? 0 : (c >= G ? G - 1 : c);
+}
+
+// The cycle arrays of bulk frame removal (bnd layout: succ[2][B], d[2][B], verts, owner, nxt, prv,
+// alive, pset, cand, positions) and the grid of set s: cell ends in succ (2 h ints from 2 off; the
+// ranking is done), positions by cell in d[1], and (lx, ly, sx, sy, G) after the positions.
+struct EarCycle {
+    const int *verts;
+    int *owner, *nxt, *prv, *alive, *pset, *cand, *win;
+    const double2 *bpos;
+    double *param;
+    int *cells;
+    int B;
+};
+GA_INLINE EarCycle ear_cycle(int *bnd, int B)
+{
+    EarCycle e;
+    e.verts = bnd + 4 * B; e.owner = bnd + 5 * B; e.nxt = bnd + 6 * B; e.prv = bnd + 7 * B;
+    e.alive = bnd + 8 * B; e.pset = bnd + 9 * B; e.cand = bnd + 10 * B; e.win = bnd + 2 * B;
+    e.bpos = (const double2 *)(bnd + bnd_pos_off(B));
+    e.param = (double *)(bnd + bnd_pos_off(B) + 4 * B);
+    e.cells = bnd;
+    e.B = B;
+    return e;
+}
+GA_INLINE int *ear_cend(const EarCycle &e, int off) { return e.cells + 2 * (size_t)off; }
+GA_INLINE int *ear_citem(const EarCycle &e, int off) { return e.cells + 3 * (size_t)e.B + off; }
+
+#define EAR_THREADS 1024
+// The grid of the set of this workgroup (h positions from off): bounding box, counting sort of the
+// positions by cell (counts, exclusive starts, then a fill that counts each start up to its cell's
+// end: cell c holds [c ? cend[c - 1] : 0, cend[c])), parameters stored for other kernels.
+GA_INLINE void ear_build_grid(const EarCycle &e, int s, int h, int off)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    __shared__ double red[4][EAR_THREADS / 64];
+    __shared__ int s_total[EAR_THREADS / 64];
+    int *cend = ear_cend(e, off), *citem = ear_citem(e, off);
+    double lx = INFINITY, ly = INFINITY, hx = -INFINITY, hy = -INFINITY;
+    for (int k = tid; k < h; k += EAR_THREADS) {
+        double2 q = e.bpos[off + k];
+        lx = fmin(lx, q.x); ly = fmin(ly, q.y); hx = fmax(hx, q.x); hy = fmax(hy, q.y);
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        lx = fmin(lx, __shfl_xor(lx, d, 64)); ly = fmin(ly, __shfl_xor(ly, d, 64));
+        hx = fmax(hx, __shfl_xor(hx, d, 64)); hy = fmax(hy, __shfl_xor(hy, d, 64));
+    }
+    if (lane == 0) { red[0][wv] = lx; red[1][wv] = ly; red[2][wv] = hx; red[3][wv] = hy; }
+    __syncthreads();
+    lx = red[0][0]; ly = red[1][0]; hx = red[2][0]; hy = red[3][0];
+    for (int w = 1; w < EAR_THREADS / 64; w++) {
+        lx = fmin(lx, red[0][w]); ly = fmin(ly, red[1][w]); hx = fmax(hx, red[2][w]); hy = fmax(hy, red[3][w]);
+    }
+    int G = 1;
+    while (2 * G * G < h) G++;   // about two vertices per cell; G * G <= 2 h for h >= 3
+    const int ncell = G * G;
+    const double sx = hx > lx ? G / (hx - lx) : 0.0, sy = hy > ly ? G / (hy - ly) : 0.0;
+    if (tid == 0) {
+        double *pr = e.param + 5 * (size_t)s;
+        pr[0] = lx; pr[1] = ly; pr[2] = sx; pr[3] = sy; pr[4] = (double)G;
+    }
+    for (int c = tid; c < ncell; c += EAR_THREADS) cend[c] = 0;
+    __syncthreads();
+    for (int k = tid; k < h; k += EAR_THREADS) {
+        double2 q = e.bpos[off + k];
+        atomicAdd(&cend[ear_cell(q.y, ly, sy, G) * G + ear_cell(q.x, lx, sx, G)], 1);
+    }
+    __syncthreads();
+    {
+        int per = (ncell + EAR_THREADS - 1) / EAR_THREADS, c0 = tid * per, c1 = min(ncell, c0 + per);
+        int sum = 0;
+        for (int c = c0; c < c1; c++) sum += cend[c];
+        int inc = sum;
+        for (int d = 1; d < 64; d <<= 1) {
+            int t = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += t;
         }
-        win[p] = w;
+        if (lane == 63) s_total[wv] = inc;
+        __syncthreads();
+        int base = inc - sum;
+        for (int w = 0; w < wv; w++) base += s_total[w];
+        for (int c = c0; c < c1; c++) {
+            int v = cend[c];
+            cend[c] = base;
+            base += v;
+        }
     }
+    __syncthreads();
+    for (int k = tid; k < h; k += EAR_THREADS) {
+        double2 q = e.bpos[off + k];
+        int c = ear_cell(q.y, ly, sy, G) * G + ear_cell(q.x, lx, sx, G);
+        citem[atomicAdd(&cend[c], 1)] = off + k;
+    }
+    __syncthreads();
 }
 
-__global__ void k_ear_apply(MeshDev m, int B, const int *verts, int *nxt, int *prv, int *alive, int *owner,
-                            const int *win, int *list_out, int cnt_out_slot, int op)
+// 1 when live position p of set s is reflex and its ear is empty
+GA_INLINE int ear_candidate(const MeshDev &m, const EarCycle &e, int p, int off)
 {
-    if (m.counters[C_EAR] == 0) return;
-    GRID_STRIDE(p, B) {
-        if (!win[p]) continue;
-        int kp = prv[p], kn = nxt[p];
-        int T = atomicAdd(m.ntri, 1);
-        if (T >= m.cap) { atomicOr(m.status, GA_ERR_CAPACITY); continue; }
-        int vp = verts[kp], vc = verts[p], vn = verts[kn];
-        m.tv[T] = make_int4(vp, vn, vc, 0);
-        m.tn[T] = make_int4(-1, -1, -1, 0);
-        bond(m, T, 0, owner[p]);    // edge (vn, vc) = reverse of boundary edge (vc -> vn)
-        bond(m, T, 1, owner[kp]);   // edge (vc, vp) = reverse of boundary edge (vp -> vc)
-        m.stamp[T] = op;
-        if (atomicExch(&m.lmark[T], m.list_op) != m.list_op)
-            list_out[atomicAdd(&m.counters[cnt_out_slot], 1)] = T;
-        owner[kp] = enc(T, 2);      // new boundary edge (vp -> vn)
-        nxt[kp] = kn;
-        prv[kn] = kp;
-        alive[p] = 0;
-        m.vt[vp] = T; m.vt[vc] = T; m.vt[vn] = T;
+    if (!e.alive[p]) return 0;
+    int kp = e.prv[p], kn = e.nxt[p];
+    if (kp == kn) return 0;
+    double2 P = e.bpos[kp], C = e.bpos[p], N = e.bpos[kn];
+    if (orient_dev(m, P, C, N) >= 0) return 0;
+    const double *pr = e.param + 5 * (size_t)e.pset[p];
+    const double lx = pr[0], ly = pr[1], sx = pr[2], sy = pr[3];
+    const int G = (int)pr[4];
+    const int *cend = ear_cend(e, off), *citem = ear_citem(e, off);
+    double bx0 = fmin(P.x, fmin(C.x, N.x)), bx1 = fmax(P.x, fmax(C.x, N.x));
+    double by0 = fmin(P.y, fmin(C.y, N.y)), by1 = fmax(P.y, fmax(C.y, N.y));
+    int cx0 = ear_cell(bx0, lx, sx, G), cx1 = ear_cell(bx1, lx, sx, G);
+    int cy0 = ear_cell(by0, ly, sy, G), cy1 = ear_cell(by1, ly, sy, G);
+    for (int cy = cy0; cy <= cy1; cy++)
+        for (int cx = cx0; cx <= cx1; cx++) {
+            int cc = cy * G + cx;
+            for (int j = cc ? cend[cc - 1] : 0, j1 = cend[cc]; j < j1; j++) {
+                int q = citem[j];
+                if (q == p || q == kp || q == kn || !e.alive[q]) continue;
+                double2 Q = e.bpos[q];
+                if (Q.x < bx0 || Q.x > bx1 || Q.y < by0 || Q.y > by1) continue;
+                if (orient_dev(m, P, N, Q) >= 0 && orient_dev(m, N, C, Q) >= 0 && orient_dev(m, C, P, Q) >= 0)
+                    return 0;
+            }
+        }
+    return 1;
+}
+
+// winner: a candidate above every candidate within cycle distance 2
+GA_INLINE int ear_winner(const EarCycle &e, int p, int round)
+{
+    if (!e.cand[p]) return 0;
+    int a = e.prv[p], b = e.nxt[p];
+    int q[4] = {a, b, e.prv[a], e.nxt[b]};
+    for (int j = 0; j < 4; j++)
+        if (q[j] != p && e.cand[q[j]] && !ear_beats(p, q[j], round)) return 0;
+    return 1;
+}
+
+// cut the ear of winner p: triangle (prev, next, p), new boundary edge prev -> next; false when the
+// triangle slots are exhausted (status set)
+GA_INLINE bool ear_cut(const MeshDev &m, const EarCycle &e, int p, int *list_out, int cnt_out_slot, int op)
+{
+    int kp = e.prv[p], kn = e.nxt[p];
+    int T = atomicAdd(m.ntri, 1);
+    if (T >= m.cap) {
+        atomicOr(m.status, GA_ERR_CAPACITY);
+        return false;
+    }
+    int vp = e.verts[kp], vc = e.verts[p], vn = e.verts[kn];
+    m.tv[T] = make_int4(vp, vn, vc, 0);
+    m.tn[T] = make_int4(-1, -1, -1, 0);
+    bond(m, T, 0, e.owner[p]);    // edge (vn, vc) = reverse of boundary edge (vc -> vn)
+    bond(m, T, 1, e.owner[kp]);   // edge (vc, vp) = reverse of boundary edge (vp -> vc)
+    m.stamp[T] = op;
+    if (atomicExch(&m.lmark[T], m.list_op) != m.list_op)
+        list_out[atomicAdd(&m.counters[cnt_out_slot], 1)] = T;
+    e.owner[kp] = enc(T, 2);      // new boundary edge (vp -> vn)
+    e.nxt[kp] = kn;
+    e.prv[kn] = kp;
+    e.alive[p] = 0;
+    m.vt[vp] = T; m.vt[vc] = T; m.vt[vn] = T;
+    return true;
+}
+
+// one workgroup per set: the grid
+__global__ __launch_bounds__(EAR_THREADS) void k_ear_grid(MeshDev m, int B, int *bnd, const int *hull_cnt,
+                                                          const int *hull_off)
+{
+    int s = blockIdx.x, h = hull_cnt[s];
+    if (h < 3) return;
+    ear_build_grid(ear_cycle(bnd, B), s, h, hull_off[s]);
+}
+
+// Grid-wide rounds: round r counts its candidates in counters[C_EAR + ...] by parity; a round
+// after one without candidates returns at once (so several run per host read), and the apply of
+// round r clears the counter round r + 1 will use.
+GA_INLINE int *ear_count(const MeshDev &m, int round) { return m.counters + ((round & 1) ? C_EAR_PREV : C_EAR); }
+
+// One wave per cycle position: the lanes test the vertices of the grid rows the ear's box covers
+// (a row's cells are consecutive, so its vertices are one contiguous range of the cell order).
+__global__ __launch_bounds__(256) void k_ear_cand(MeshDev m, int B, int *bnd, const int *hull_off, int round)
+{
+    if (round > 0 && *ear_count(m, round - 1) == 0) return;
+    EarCycle e = ear_cycle(bnd, B);
+    const int lane = threadIdx.x & 63;
+    const int wave = (xcd_block() * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
+    for (int p = wave; p < B; p += nw) {
+        int c = 0;
+        if (e.alive[p]) {
+            int kp = e.prv[p], kn = e.nxt[p];
+            if (kp != kn) {
+                double2 P = e.bpos[kp], C = e.bpos[p], N = e.bpos[kn];
+                if (orient_dev(m, P, C, N) < 0) {
+                    int off = hull_off[e.pset[p]];
+                    const double *pr = e.param + 5 * (size_t)e.pset[p];
+                    const double lx = pr[0], ly = pr[1], sx = pr[2], sy = pr[3];
+                    const int G = (int)pr[4];
+                    const int *cend = ear_cend(e, off), *citem = ear_citem(e, off);
+                    double bx0 = fmin(P.x, fmin(C.x, N.x)), bx1 = fmax(P.x, fmax(C.x, N.x));
+                    double by0 = fmin(P.y, fmin(C.y, N.y)), by1 = fmax(P.y, fmax(C.y, N.y));
+                    int cx0 = ear_cell(bx0, lx, sx, G), cx1 = ear_cell(bx1, lx, sx, G);
+                    int cy0 = ear_cell(by0, ly, sy, G), cy1 = ear_cell(by1, ly, sy, G);
+                    bool blocked = false;
+                    for (int cy = cy0; cy <= cy1 && !__any(blocked); cy++) {
+                        int c0 = cy * G + cx0, c1 = cy * G + cx1;
+                        int j0 = c0 ? cend[c0 - 1] : 0, j1 = cend[c1];
+                        for (int j = j0 + lane; j < j1; j += 64) {
+                            int q = citem[j];
+                            if (q == p || q == kp || q == kn || !e.alive[q]) continue;
+                            double2 Q = e.bpos[q];
+                            if (Q.x < bx0 || Q.x > bx1 || Q.y < by0 || Q.y > by1) continue;
+                            if (orient_dev(m, P, N, Q) >= 0 && orient_dev(m, N, C, Q) >= 0 &&
+                                orient_dev(m, C, P, Q) >= 0)
+                                blocked = true;
+                        }
+                    }
+                    c = __any(blocked) ? 0 : 1;
+                }
+            }
+        }
+        if (lane == 0) {
+            e.cand[p] = c;
+            if (c) atomicAdd(ear_count(m, round), 1);
+        }
     }
 }
 
-// bnd layout: succ[2][B], d[2][B], verts, owner, nxt, prv, alive, pset, cand (win reuses d[0]), then
-// the positions of the cycle (double2[B]) from int offset bnd_pos_off(B), 16-byte aligned
-GA_INLINE_HOST_DEVICE int bnd_pos_off(int B) { return (11 * B + 3) & ~3; }
+__global__ void k_ear_select(MeshDev m, int B, int *bnd, int round)
+{
+    if (*ear_count(m, round) == 0) return;
+    EarCycle e = ear_cycle(bnd, B);
+    GRID_STRIDE(p, B) e.win[p] = ear_winner(e, p, round);
+}
+
+__global__ void k_ear_apply(MeshDev m, int B, int *bnd, int round, int *list_out, int cnt_out_slot, int op)
+{
+    bool any = *ear_count(m, round) != 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ear_count(m, round + 1) = 0;
+    if (!any) return;
+    EarCycle e = ear_cycle(bnd, B);
+    GRID_STRIDE(p, B) {
+        if (e.win[p]) ear_cut(m, e, p, list_out, cnt_out_slot, op);
+    }
+}
+
+// All rounds of every set in one launch, one workgroup per set (batches of many small sets);
+// counters[C_EAR] = the most rounds a set ran.
+__global__ __launch_bounds__(EAR_THREADS) void k_ear_sets(MeshDev m, int B, int *bnd, const int *hull_cnt,
+                                                          const int *hull_off, int *list_out, int cnt_out_slot,
+                                                          int op)
+{
+    const int s = blockIdx.x, h = hull_cnt[s], off = hull_off[s];
+    if (h < 3) return;
+    EarCycle e = ear_cycle(bnd, B);
+    ear_build_grid(e, s, h, off);
+    const int tid = threadIdx.x;
+    __shared__ int s_cut, s_stop;
+    for (int round = 0;; round++) {
+        if (round > h + 8) {   // every round with a candidate cuts at least one ear
+            if (tid == 0) atomicOr(m.status, GA_ERR_WALK);
+            break;
+        }
+        for (int k = tid; k < h; k += EAR_THREADS) e.cand[off + k] = ear_candidate(m, e, off + k, off);
+        __syncthreads();
+        if (tid == 0) { s_cut = 0; s_stop = 0; }
+        for (int k = tid; k < h; k += EAR_THREADS) e.win[off + k] = ear_winner(e, off + k, round);
+        __syncthreads();
+        for (int k = tid; k < h; k += EAR_THREADS) {
+            if (!e.win[off + k]) continue;
+            if (ear_cut(m, e, off + k, list_out, cnt_out_slot, op)) s_cut = 1;
+            else s_stop = 1;
+        }
+        __syncthreads();
+        bool done = !s_cut || s_stop;
+        if (done) {
+            if (tid == 0) atomicMax(&m.counters[C_EAR], round + 1);
+            break;
+        }
+        __syncthreads();
+    }
+}
+
 void launch_bnd_detect(MeshDev &m, int *outdeg, int *cover, int *hull_cnt, int *hull_start, int *badset,
                        int *blist, int *bidx, hipStream_t st)
 {
@@ -1666,21 +1866,32 @@ void launch_kill_frame(MeshDev &m, hipStream_t st)
     GA_LAUNCH(k_kill_frame, dim3(FIXED_GRID), dim3(256), 0, st, m);
 }
 
-void launch_ear_round(MeshDev &m, int B, int *bnd, const int *hull_cnt, const int *hull_off, int round,
-                      int *list_out, int cnt_out_slot, int op, hipStream_t st)
+// the grid of every set, then rounds r0 .. r0 + k - 1 as grid-wide kernels
+void launch_ear_grid(MeshDev &m, int B, int *bnd, const int *hull_cnt, const int *hull_off, int nsets, hipStream_t st)
 {
-    int *verts = bnd + 4 * B, *owner = bnd + 5 * B, *nxt = bnd + 6 * B, *prv = bnd + 7 * B, *alive = bnd + 8 * B,
-        *pset = bnd + 9 * B, *cand = bnd + 10 * B, *win = bnd + 2 * B;   // win reuses d[0] (ranking is done)
-    GA_LAUNCH(k_ear_round_start, dim3(1), dim3(64), 0, st, m.counters);
-    int gw = std::min(4096, (B + 3) / 4);   // 4 waves per block, one wave per cycle position
-    GA_LAUNCH(k_ear_cand, dim3(gw), dim3(256), 0, st, m, B, (const double2 *)(bnd + bnd_pos_off(B)),
-              (const int *)nxt, (const int *)prv, (const int *)alive, (const int *)pset, hull_cnt, hull_off, cand,
-              m.counters + C_EAR);
+    GA_LAUNCH(k_ear_grid, dim3(nsets), dim3(EAR_THREADS), 0, st, m, B, bnd, hull_cnt, hull_off);
+}
+
+void launch_ear_rounds(MeshDev &m, int B, int *bnd, const int *hull_off, int r0, int k, int *list_out,
+                       int cnt_out_slot, int op, hipStream_t st)
+{
     int g = std::min(4096, (B + 255) / 256);
-    GA_LAUNCH(k_ear_select, dim3(g), dim3(256), 0, st, B, (const int *)nxt, (const int *)prv, (const int *)cand,
-              round, win, (const int *)m.counters);
-    GA_LAUNCH(k_ear_apply, dim3(g), dim3(256), 0, st, m, B, (const int *)verts, nxt, prv, alive, owner,
-              (const int *)win, list_out, cnt_out_slot, op);
+    g = (g + 7) & ~7;
+    int gw = std::min(4096, (B + 3) / 4);   // one wave per cycle position
+    gw = (gw + 7) & ~7;
+    for (int r = r0; r < r0 + k; r++) {
+        GA_LAUNCH(k_ear_cand, dim3(gw), dim3(256), 0, st, m, B, bnd, hull_off, r);
+        GA_LAUNCH(k_ear_select, dim3(g), dim3(256), 0, st, m, B, bnd, r);
+        GA_LAUNCH(k_ear_apply, dim3(g), dim3(256), 0, st, m, B, bnd, r, list_out, cnt_out_slot, op);
+    }
+}
+
+// every set's ear rounds in one workgroup; counters[C_EAR] = the most rounds any set ran
+void launch_ear_sets(MeshDev &m, int B, int *bnd, const int *hull_cnt, const int *hull_off, int nsets,
+                     int *list_out, int cnt_out_slot, int op, hipStream_t st)
+{
+    GA_LAUNCH(k_ear_sets, dim3(nsets), dim3(EAR_THREADS), 0, st, m, B, bnd, hull_cnt, hull_off, list_out,
+              cnt_out_slot, op);
 }
 
 // ---- canonical output ----------------------------------------------------------------------------

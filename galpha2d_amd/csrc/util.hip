@@ -2,6 +2,7 @@
 // for non-finite coordinates, which the host rejects).
 #include "dev_common.h"
 #include "kernels.h"
+#include <algorithm>
 
 // reduce one block's box; returns it in thread 0
 GA_INLINE void block_box(double &a, double &b, double &c, double &d, double &bad)
@@ -80,4 +81,23 @@ void bbox_sets_device(const double *d_xy, const int *d_offs, int nsets, double *
     GA_LAUNCH(k_bbox_sets, dim3(nsets), dim3(256), 0, st, (const double2 *)d_xy, d_offs, (double *)tmp);
     hipMemcpyAsync(out5, tmp, sizeof(double) * 5 * (size_t)nsets, hipMemcpyDeviceToHost, st);
     hipStreamSynchronize(st);
+}
+
+// segments of device ints gathered into one contiguous array (for a single host read)
+__global__ void k_gather_segments(IntSegments g, int *dst)
+{
+    int base = 0;
+    for (int k = 0; k < g.n; k++) {
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.len[k]; i += gridDim.x * blockDim.x)
+            dst[base + i] = g.src[k][i];
+        base += g.len[k];
+    }
+}
+
+void launch_gather_segments(const IntSegments &g, int *dst, hipStream_t st)
+{
+    int tot = 0;
+    for (int k = 0; k < g.n; k++) tot += g.len[k];
+    int blocks = std::max(1, std::min(256, (tot + 255) / 256));
+    GA_LAUNCH(k_gather_segments, dim3(blocks), dim3(256), 0, st, g, dst);
 }
