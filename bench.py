@@ -485,6 +485,9 @@ def main():
                       "alpha_exact_attach": st["alphaExactAttach"]},
         "roofline": roof,
         "stage_gbs": stages,
+        "work": {k: st[k] for k in ("reconTris", "shiftBad", "deleted", "missing", "inserted", "duplicates",
+                                     "insertRounds", "flipPasses", "flips", "flipListed", "frameBulk", "earRounds",
+                                     "islandSites", "alphaClusters", "alphaMaxCluster")},
     }
     ctx.close()
     d_xy.free()

@@ -766,8 +766,9 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     // Eight greedy rounds (two batches of four per host read), then deletion takes the rest.
     // Later rounds move few vertices each and cost more than deleting and reinserting them:
     // on 1M uniform points the shifting stage took 1.68 ms with up to 48 rounds, 1.40 ms with 8.
+    const int pgrid = round_grid(pend);   // the pending list keeps its length (done entries are skipped)
     for (int r = 0; r < 8 && pend > 0; r += 4) {
-        for (int k = 0; k < 4; k++) launch_shift_greedy(m, r + k, st);
+        for (int k = 0; k < 4; k++) launch_shift_greedy(m, r + k, pgrid, st);
         int moved_total = ctx_stat(c, C_MOVED);
         int movedn = moved_total - moved_before;
         moved_before = moved_total;
@@ -779,7 +780,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
         if (r > 10000) throw GalphaError("vertex deletion did not finish");
         for (int k = 0; k < 2; k++) {
             launch_reset_counters(c->counters.as<int>(), C_SCR, C_MODN, -1, -1, st);
-            launch_delete_round(m, c->mx.as<unsigned int>(), next_prio_round(), c->op++, st);
+            launch_delete_round(m, c->mx.as<unsigned int>(), next_prio_round(), c->op++, pgrid, st);
         }
         Snap sn = ctx_snapshot(c);
         int after = sn.stat(C_DELETED);
@@ -812,13 +813,16 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     // `last` keeps the snapshot of the final check (counters, statistics and status after the
     // passes), which the callers use instead of another read
     Snap last;
-    auto flip_until_done = [&](int *listA, int cntA, int *listB, int cntB, int prev) {
+    // `hint`: the length of the first list when the caller knows it (else a full grid); later
+    // passes size their grids by the list length of the last read (lists shrink from pass to pass).
+    auto flip_until_done = [&](int *listA, int cntA, int *listB, int cntB, int prev, long long hint) {
         int *lin = listA, *lout = listB, sin = cntA, sout = cntB;
+        int grid = hint >= 0 ? round_grid(2 * hint) : 2048;
         for (int pass = 0; pass < 100000; pass += 3) {
             for (int k = 0; k < 3; k++) {
                 int opn = c->op++;
                 launch_reset_counters(c->counters.as<int>(), C_CAND, sout, C_FLIPS, -1, st);
-                launch_flip_pass(m, lin, sin, lout, sout, prev, opn, st);
+                launch_flip_pass(m, lin, sin, lout, sout, prev, opn, grid, st);
                 prev = opn;
                 std::swap(lin, lout);
                 std::swap(sin, sout);
@@ -826,6 +830,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
             S.flipPasses += 3;
             last = ctx_snapshot(c);
             if (last.v[C_FLIPS] == 0) return;
+            grid = round_grid(2LL * last.v[sin]);
         }
         throw GalphaError("flipping did not converge");
     };
@@ -833,7 +838,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
         int opn = c->op++;
         ctx_counter_set(c, C_MODN, 0);
         launch_flip_all(m, opn, m.modA, st);
-        flip_until_done(m.modA, C_MODN, m.modB, C_MODN2, opn);
+        flip_until_done(m.modA, C_MODN, m.modB, C_MODN2, opn, -1);
     };
     // ---- missing sites
     // Inserted into the mesh as it is after shifting (every triangle counter-clockwise, not yet
@@ -854,8 +859,8 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
         if (r > 5000) throw GalphaError("missing-site insertion did not finish");
         int opn = c->op++;
         ctx_counter_set(c, C_MODN, 0);
-        launch_insert_round(m, c->mlist.as<int>(), n, c->loc.as<int>(), opn, st);
-        launch_relink(m, m.modA, C_MODN, opn, st);
+        launch_insert_round(m, c->mlist.as<int>(), c->loc.as<int>(), opn, round_grid(nmiss), st);
+        launch_relink(m, m.modA, C_MODN, opn, round_grid(4LL * left), st);
         S.insertRounds++;
         last = ctx_snapshot(c);
     }
@@ -890,7 +895,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
                      c->hullStart.as<int>(), c->scanTmp.p, st);
     }
     ev_mark(c, 6, st);
-    flip_until_done(c->modF.as<int>(), C_FRAME_LIST, m.modB, C_MODN2, m.list_op);
+    flip_until_done(c->modF.as<int>(), C_FRAME_LIST, m.modB, C_MODN2, m.list_op, -1);
     ev_mark(c, 7, st);
     const Snap after_frame = last;   // per-stage statistics: frame removal and the final flips
     if (S.frameBulk && S.earRounds < 0) S.earRounds = after_frame.v[C_EAR];

@@ -211,17 +211,26 @@ void launch_shift_star(MeshDev &m, unsigned char *moved, int *mark, const int *w
 // lists the vertices still pending into m.plist (count in counters[C_PLIST])
 void launch_pending_list(MeshDev &m, hipStream_t st);
 void launch_shift_commit(MeshDev &m, const unsigned char *moved, hipStream_t st);
-void launch_shift_greedy(MeshDev &m, int round, hipStream_t st);
-void launch_delete_round(MeshDev &m, unsigned int *mx, int round, int op, hipStream_t st);
+void launch_shift_greedy(MeshDev &m, int round, int grid, hipStream_t st);
+void launch_delete_round(MeshDev &m, unsigned int *mx, int round, int op, int grid, hipStream_t st);
 void launch_missing_init(MeshDev &m, const int *vor, int *mlist, int *loc, hipStream_t st);
-void launch_insert_round(MeshDev &m, int *mlist, int nmiss_cap, int *loc, int op, hipStream_t st);
-void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, hipStream_t st);
+void launch_insert_round(MeshDev &m, int *mlist, int *loc, int op, int grid, hipStream_t st);
+void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, int grid, hipStream_t st);
 void launch_reset_counters(int *counters, int a, int b, int d, int e, hipStream_t st);
 // counters[C_FOLD + k] = sum over the stripes of statistic k, for k < 64
 void launch_fold_stats(int *counters, hipStream_t st);
 void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st);
 void launch_flip_pass(MeshDev &m, const int *list_in, int cnt_in_slot, int *list_out, int cnt_out_slot,
-                      int prev_op, int op, hipStream_t st);
+                      int prev_op, int op, int grid, hipStream_t st);
+// Blocks of 256 threads for a round over about `items` list entries: enough for one entry per
+// thread, a multiple of 8 (one share per XCD), at most 2048 (grid-stride loops cover any rest).
+// Lists that shrink from round to round then do not launch thousands of idle blocks.
+inline int round_grid(long long items)
+{
+    long long b = (items + 255) / 256;
+    b = (b + 7) & ~7LL;
+    return (int)(b < 8 ? 8 : (b > 2048 ? 2048 : b));
+}
 // constrained DT (see mesh.hip).  Set-up: cv = constraints as vertex ids, done flags, the key set
 // (chash, size chmask + 1, cleared by the caller), vt rebuilt from the live slots.
 void launch_cons_setup(MeshDev &m, const int2 *cin, int nc, int *inv, int2 *cv, int *done,

@@ -515,10 +515,10 @@ __global__ void k_shift_greedy_commit(MeshDev m, const int *decide)
     }
 }
 
-void launch_shift_greedy(MeshDev &m, int round, hipStream_t st)
+void launch_shift_greedy(MeshDev &m, int round, int grid, hipStream_t st)
 {
-    GA_LAUNCH(k_shift_greedy, dim3(FIXED_GRID), dim3(256), 0, st, m, round, m.decide);
-    GA_LAUNCH(k_shift_greedy_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)m.decide);
+    GA_LAUNCH(k_shift_greedy, dim3(grid), dim3(256), 0, st, m, round, m.decide);
+    GA_LAUNCH(k_shift_greedy_commit, dim3(grid), dim3(256), 0, st, m, (const int *)m.decide);
 }
 
 // ---- deletion of vertices that cannot move (star-shaped hole, ear clipping) --------------------
@@ -669,11 +669,11 @@ __global__ void k_del_apply(MeshDev m, const unsigned long long *mx, int round, 
     }
 }
 
-void launch_delete_round(MeshDev &m, unsigned int *mxraw, int round, int op, hipStream_t st)
+void launch_delete_round(MeshDev &m, unsigned int *mxraw, int round, int op, int grid, hipStream_t st)
 {
     unsigned long long *mx = (unsigned long long *)mxraw;
-    GA_LAUNCH(k_del_prio, dim3(FIXED_GRID), dim3(256), 0, st, m, mx, round);
-    GA_LAUNCH(k_del_apply, dim3(FIXED_GRID), dim3(256), 0, st, m, (const unsigned long long *)mx, round, op,
+    GA_LAUNCH(k_del_prio, dim3(grid), dim3(256), 0, st, m, mx, round);
+    GA_LAUNCH(k_del_apply, dim3(grid), dim3(256), 0, st, m, (const unsigned long long *)mx, round, op,
                        m.dellist);
 }
 
@@ -869,11 +869,10 @@ __global__ void k_insert(MeshDev m, const int *mlist, const int *loc, const int 
     }
 }
 
-void launch_insert_round(MeshDev &m, int *mlist, int nmiss_cap, int *loc, int op, hipStream_t st)
+void launch_insert_round(MeshDev &m, int *mlist, int *loc, int op, int grid, hipStream_t st)
 {
-    (void)nmiss_cap;
-    GA_LAUNCH(k_locate, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)mlist, loc, m.lt, op);
-    GA_LAUNCH(k_insert, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)mlist, (const int *)loc,
+    GA_LAUNCH(k_locate, dim3(grid), dim3(256), 0, st, m, (const int *)mlist, loc, m.lt, op);
+    GA_LAUNCH(k_insert, dim3(grid), dim3(256), 0, st, m, (const int *)mlist, (const int *)loc,
                        (const int *)m.lt, op, m.modA);
 }
 
@@ -903,9 +902,9 @@ __global__ void k_relink(MeshDev m, const int *list, int cnt_slot, int op)
     }
 }
 
-void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, hipStream_t st)
+void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, int grid, hipStream_t st)
 {
-    GA_LAUNCH(k_relink, dim3(FIXED_GRID), dim3(256), 0, st, m, modlist, cnt_slot, op);
+    GA_LAUNCH(k_relink, dim3(grid), dim3(256), 0, st, m, modlist, cnt_slot, op);
 }
 
 // ---- constraint edge set ----------------------------------------------------------------------
@@ -1069,11 +1068,11 @@ __global__ void k_flip_commit(MeshDev m, int op, int *list_out, int cnt_out_slot
 }
 
 void launch_flip_pass(MeshDev &m, const int *list_in, int cnt_in_slot, int *list_out, int cnt_out_slot,
-                      int prev_op, int op, hipStream_t st)
+                      int prev_op, int op, int grid, hipStream_t st)
 {
-    GA_LAUNCH(k_flip_test, dim3(FIXED_GRID), dim3(256), 0, st, m, list_in, cnt_in_slot, prev_op, op);
-    GA_LAUNCH(k_flip_commit, dim3(FIXED_GRID), dim3(256), 0, st, m, op, list_out, cnt_out_slot);
-    GA_LAUNCH(k_relink, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)list_out, cnt_out_slot, op);
+    GA_LAUNCH(k_flip_test, dim3(grid), dim3(256), 0, st, m, list_in, cnt_in_slot, prev_op, op);
+    GA_LAUNCH(k_flip_commit, dim3(grid), dim3(256), 0, st, m, op, list_out, cnt_out_slot);
+    GA_LAUNCH(k_relink, dim3(grid), dim3(256), 0, st, m, (const int *)list_out, cnt_out_slot, op);
 }
 
 // ---- frame removal ---------------------------------------------------------------------------
