@@ -37,16 +37,24 @@ struct AlphaBuf {
 // through two halves used alternately, so the DMA of one chunk overlaps the host copy of the
 // other.  (Copies from pageable memory go through the runtime's shared staging buffer and
 // serialise the copies of all threads, e.g. of concurrent serving slots.)
+struct CopyPool;                 // host threads that split large memcpys (galpha.cpp)
 struct HostStage {
     char *p = nullptr;           // 2 * half bytes of pinned memory, allocated on first use
     size_t half = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    CopyPool *pool = nullptr;    // started on the first large copy, joined by stage_free
 };
 // dst (host) <- src (device); returns when dst holds the data
 void stage_d2h(HostStage &s, void *dst, const void *src, size_t bytes, hipStream_t st);
 // dst (device) <- src (host); returns when src may be reused (the copies are ordered on st)
 void stage_h2d(HostStage &s, void *dst, const void *src, size_t bytes, hipStream_t st);
 void stage_free(HostStage &s);
+// the same, adding the content hash of the host array to *hash (galpha.cpp)
+#include <atomic>
+void stage_d2h_hash(HostStage &s, void *dst, const void *src, size_t bytes, hipStream_t st,
+                    std::atomic<unsigned long long> *hash);
+void stage_h2d_hash(HostStage &s, void *dst, const void *src, size_t bytes, hipStream_t st,
+                    std::atomic<unsigned long long> *hash);
 
 #include <vector>
 

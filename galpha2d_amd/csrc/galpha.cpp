@@ -12,13 +12,16 @@
 #include "alpha_host.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <initializer_list>
+#include <condition_variable>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -129,11 +132,150 @@ struct galpha_ctx {
     int op = 1;
     int *hsnap = nullptr;   // pinned staging for counter reads (SNAP_INTS ints)
     std::vector<int> hsnap_pageable;   // counter reads of a no-allocation context
+    // the last gpudtComputeDT on this (default) context: the caller's point array and the returned
+    // triangle array with their sizes and content hashes, so that gpudtComputeAlpha of that output
+    // runs on the points and triangles still resident on the device
+    struct {
+        const void *points = nullptr, *tris = nullptr;
+        int n = -1, F = -1;
+        unsigned long long hp = 0, ht = 0;
+    } last_dt;
     HostStage stage;        // pinned staging for large host <-> device copies
 };
 
 // ---- pinned staging ------------------------------------------------------------------------
 static const size_t STAGE_HALF = (size_t)32 << 20;
+
+// Position-dependent hash of bytes [base, base + n) of an array whose bytes from `base` are at p:
+// a sum of mixed 8-byte words (and the tail bytes), each mixed with its offset, so that parts of an
+// array hashed separately add up to the hash of the whole.  Detects changed, moved or swapped
+// data; not meant against adversaries.
+static unsigned long long content_hash(const char *p, size_t n, size_t base)
+{
+    auto mix = [](unsigned long long x) {
+        x ^= x >> 31;
+        x *= 0x7fb5d329728ea185ull;
+        x ^= x >> 27;
+        x *= 0x81dadef4bc2dd44dull;
+        return x ^ (x >> 33);
+    };
+    unsigned long long h = 0;
+    size_t i = 0;
+    // words at offsets that are multiples of 8 in the whole array (base is a multiple of 4096)
+    for (; i + 8 <= n; i += 8) {
+        unsigned long long w;
+        memcpy(&w, p + i, 8);
+        h += mix(w ^ ((base + i) * 0x9E3779B97F4A7C15ull));
+    }
+    for (; i < n; i++) h += mix((unsigned long long)(unsigned char)p[i] ^ ((base + i) * 0xC2B2AE3D27D4EB4Full));
+    return h;
+}
+
+// Host copies between the pinned staging halves and caller arrays run on several threads: one
+// thread copies about 10 GB/s, far below what the host's memory and PCIe give, and the results of
+// a 1M-point run (triangles and alpha arrays) are about 330 MB.  The pool belongs to one context
+// (serving slots copy concurrently, each with its own).
+struct CopyPool {
+    static const int kThreads = 7;      // workers; the calling thread takes a share too
+    std::vector<std::thread> th;
+    std::mutex mu;
+    std::condition_variable go, done;
+    char *dst = nullptr;
+    const char *src = nullptr;
+    size_t len = 0, part = 0;
+    int nparts = 0, next = 0, pending = 0;
+    unsigned gen = 0;
+    bool stop = false;
+    size_t hbase = 0;                        // offset of src in the hashed array
+    std::atomic<unsigned long long> *hash = nullptr;
+
+    CopyPool()
+    {
+        for (int k = 0; k < kThreads; k++) th.emplace_back([this] { work(); });
+    }
+    ~CopyPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        go.notify_all();
+        for (auto &t : th) t.join();
+    }
+    // takes parts of the current copy until none is left
+    void drain()
+    {
+        for (;;) {
+            int k;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (next >= nparts) return;
+                k = next++;
+            }
+            size_t off = (size_t)k * part, n = std::min(part, len - off);
+            if (dst) memcpy(dst + off, src + off, n);
+            if (hash) hash->fetch_add(content_hash(src + off, n, hbase + off));
+            std::lock_guard<std::mutex> lk(mu);
+            if (--pending == 0) done.notify_all();
+        }
+    }
+    void work()
+    {
+        unsigned seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                go.wait(lk, [&] { return stop || gen != seen; });
+                if (stop) return;
+                seen = gen;
+            }
+            drain();
+        }
+    }
+    // dst NULL: hash only; h non-NULL: add the content hash of src (at offset base of its array)
+    void copy(void *d, const void *s_, size_t n, std::atomic<unsigned long long> *h = nullptr, size_t base = 0)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            dst = (char *)d;
+            src = (const char *)s_;
+            hash = h;
+            hbase = base;
+            len = n;
+            nparts = kThreads + 1;
+            part = (n + nparts - 1) / nparts;
+            part = (part + 4095) & ~(size_t)4095;
+            nparts = (int)((n + part - 1) / part);
+            next = 0;
+            pending = nparts;
+            gen++;
+        }
+        go.notify_all();
+        drain();
+        std::unique_lock<std::mutex> lk(mu);
+        done.wait(lk, [&] { return pending == 0; });
+    }
+};
+
+static void host_copy(HostStage &s, void *dst, const void *src, size_t n,
+                      std::atomic<unsigned long long> *hash = nullptr, size_t base = 0)
+{
+    if (n < ((size_t)4 << 20)) {
+        if (dst) memcpy(dst, src, n);
+        if (hash) hash->fetch_add(content_hash((const char *)src, n, base));
+        return;
+    }
+    if (!s.pool) s.pool = new CopyPool();
+    s.pool->copy(dst, src, n, hash, base);
+}
+
+// content hash of a host array (sum of per-part hashes, so any split gives the same value)
+static unsigned long long host_hash(HostStage &s, const void *p, size_t n)
+{
+    std::atomic<unsigned long long> h{0};
+    host_copy(s, nullptr, p, n, &h, 0);
+    return h.load();
+}
 
 static void stage_ensure(HostStage &s)
 {
@@ -146,6 +288,7 @@ static void stage_ensure(HostStage &s)
 
 void stage_free(HostStage &s)
 {
+    delete s.pool;
     if (s.p) hipHostFree(s.p);
     for (int k = 0; k < 2; k++)
         if (s.ev[k]) hipEventDestroy(s.ev[k]);
@@ -153,6 +296,12 @@ void stage_free(HostStage &s)
 }
 
 void stage_d2h(HostStage &s, void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    stage_d2h_hash(s, dst, src, bytes, st, nullptr);
+}
+
+void stage_d2h_hash(HostStage &s, void *dst, const void *src, size_t bytes, hipStream_t st,
+                    std::atomic<unsigned long long> *hash)
 {
     if (bytes == 0) return;
     stage_ensure(s);
@@ -166,12 +315,18 @@ void stage_d2h(HostStage &s, void *dst, const void *src, size_t bytes, hipStream
         if (i > 0) {     // host copy of chunk i - 1 while chunk i is in flight
             size_t j = i - 1, off = j * s.half, len = std::min(s.half, bytes - off);
             HIPCHK(hipEventSynchronize(s.ev[j & 1]));
-            memcpy((char *)dst + off, s.p + (j & 1) * s.half, len);
+            host_copy(s, (char *)dst + off, s.p + (j & 1) * s.half, len, hash, off);
         }
     }
 }
 
 void stage_h2d(HostStage &s, void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    stage_h2d_hash(s, dst, src, bytes, st, nullptr);
+}
+
+void stage_h2d_hash(HostStage &s, void *dst, const void *src, size_t bytes, hipStream_t st,
+                    std::atomic<unsigned long long> *hash)
 {
     if (bytes == 0) return;
     stage_ensure(s);
@@ -179,7 +334,7 @@ void stage_h2d(HostStage &s, void *dst, const void *src, size_t bytes, hipStream
     for (size_t i = 0; i < nch; i++) {
         size_t off = i * s.half, len = std::min(s.half, bytes - off);
         if (i >= 2) HIPCHK(hipEventSynchronize(s.ev[i & 1]));   // the DMA of chunk i - 2 left this half
-        memcpy(s.p + (i & 1) * s.half, (const char *)src + off, len);
+        host_copy(s, s.p + (i & 1) * s.half, (const char *)src + off, len, hash, off);
         HIPCHK(hipMemcpyAsync((char *)dst + off, s.p + (i & 1) * s.half, len, hipMemcpyHostToDevice, st));
         HIPCHK(hipEventRecord(s.ev[i & 1], st));
     }
@@ -1524,12 +1679,25 @@ PGPUDTOUTPUT gpudtComputeDT(PGPUDTPARAMS p)
         galpha_ctx *c = default_ctx();
         if (!c) throw GalphaError(g_last_error);
         std::lock_guard<std::mutex> lk(g_default_mu);
+        c->last_dt = {};
         int n = p->nPoints;
-        int cap = 0;
         c->cons.assign(p->constraints, p->constraints + 2 * (size_t)std::max(p->nConstraints, 0));
-        int F = galphaComputeHost(c, (const double *)p->points, n, p->fboSize, nullptr, &cap);
+        int F;
+        std::atomic<unsigned long long> hp{0}, ht{0};
+        try {
+            HIPCHK(hipSetDevice(c->device));
+            double bb[5] = {0, 0, 1, 1, 0};
+            check_texture(p->fboSize);
+            if (n >= 3) check_slot_capacity((long long)n + frame_count(p->fboSize));
+            if (n > 0) bbox_host((const double *)p->points, n, bb);
+            c->xy.ensure(16 * (size_t)std::max(n, 1));
+            if (n > 0) stage_h2d_hash(c->stage, c->xy.p, p->points, 16 * (size_t)n, c->stream, &hp);
+            F = run_dt(c, c->xy.as<double>(), n, p->fboSize, bb);
+        } catch (...) {
+            c->cons.clear();
+            throw;
+        }
         c->cons.clear();
-        if (F < 0) throw GalphaError(c->err);
         PGPUDTOUTPUT out = new GPUDTOUTPUT;
         out->nTris = F;
         out->triangles = (gpudtTriangle *)galpha_host_alloc(sizeof(gpudtTriangle) * (size_t)(F > 0 ? F : 1));
@@ -1537,7 +1705,13 @@ PGPUDTOUTPUT gpudtComputeDT(PGPUDTPARAMS p)
             delete out;
             throw GalphaError("host allocation failed");
         }
-        if (F > 0) ctx_read(c, out->triangles, c->out9.p, 36 * (size_t)F);
+        if (F > 0) stage_d2h_hash(c->stage, out->triangles, c->out9.p, 36 * (size_t)F, c->stream, &ht);
+        c->last_dt.points = p->points;
+        c->last_dt.tris = out->triangles;
+        c->last_dt.n = n;
+        c->last_dt.F = F;
+        c->last_dt.hp = hp.load();
+        c->last_dt.ht = ht.load();
         return out;
     } catch (const std::exception &e) {
         g_last_error = e.what();
@@ -1565,11 +1739,21 @@ int gpudtComputeAlpha(PGPUDTPARAMS p, PGPUDTOUTPUT dt, PGPUDTALPHA *pA)
             throw GalphaError("alpha: point or triangle count out of range");
         if ((n > 0 && !p->points) || (dt->nTris > 0 && !dt->triangles))
             throw GalphaError("alpha: null point or triangle array");
-        c->xy.ensure(16 * (size_t)std::max(n, 1));
-        if (n > 0) stage_h2d(c->stage, c->xy.p, p->points, 16 * (size_t)n, c->stream);
-        c->out9.ensure(36 * (size_t)std::max(dt->nTris, 1));
-        if (dt->nTris > 0) stage_h2d(c->stage, c->out9.p, dt->triangles, 36 * (size_t)dt->nTris, c->stream);
-        alpha_run(c->alpha, c->xy.as<double>(), n, c->out9.as<int>(), dt->nTris, c->stream, &c->stats, true);
+        HIPCHK(hipSetDevice(c->device));
+        // The output of the last gpudtComputeDT, unchanged (same arrays, sizes and content): its
+        // points and triangles are still on the device, and the mesh is the pipeline's own.
+        bool resident = c->last_dt.tris == (const void *)dt->triangles && c->last_dt.F == dt->nTris &&
+                        c->last_dt.points == (const void *)p->points && c->last_dt.n == n &&
+                        host_hash(c->stage, p->points, 16 * (size_t)std::max(n, 0)) == c->last_dt.hp &&
+                        host_hash(c->stage, dt->triangles, 36 * (size_t)std::max(dt->nTris, 0)) == c->last_dt.ht;
+        if (!resident) {
+            c->last_dt = {};
+            c->xy.ensure(16 * (size_t)std::max(n, 1));
+            if (n > 0) stage_h2d(c->stage, c->xy.p, p->points, 16 * (size_t)n, c->stream);
+            c->out9.ensure(36 * (size_t)std::max(dt->nTris, 1));
+            if (dt->nTris > 0) stage_h2d(c->stage, c->out9.p, dt->triangles, 36 * (size_t)dt->nTris, c->stream);
+        }
+        alpha_run(c->alpha, c->xy.as<double>(), n, c->out9.as<int>(), dt->nTris, c->stream, &c->stats, !resident);
         *pA = alpha_copy_to_host(c->alpha, c->stage, c->stream);
         return 0;
     } catch (const std::exception &e) {

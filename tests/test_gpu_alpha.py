@@ -216,3 +216,62 @@ def test_alpha_of_caller_triangles(oracle):
     # the context still works after a rejected call
     got = run(ref)
     assert np.array_equal(np.asarray(got["vert_nu_min"]), np.asarray(ra["vert_nu_min"]))
+
+
+def test_alpha_of_own_output_and_of_an_edited_copy(oracle):
+    """gpudtComputeAlpha on gpudtComputeDT's output uses the points and triangles still on the
+    device; after the caller edits that output in place (here: the vertex order of one triangle
+    rotated, with its links) the edited mesh is the one analysed."""
+    print("case: gpudtComputeDT then gpudtComputeAlpha, unchanged and edited in place", flush=True)
+    import ctypes as C
+    import galpha2d_amd as G
+    L = G.lib()
+    L.gpudtComputeDT.argtypes = [C.POINTER(G._Params)]
+    L.gpudtComputeDT.restype = C.POINTER(G._Output)
+    L.gpudtReleaseDTOutput.argtypes = [C.POINTER(G._Output)]
+    L.gpudtComputeAlpha.argtypes = [C.POINTER(G._Params), C.POINTER(G._Output), C.POINTER(C.POINTER(G._Alpha))]
+    L.gpudtComputeAlpha.restype = C.c_int
+    xy = np.ascontiguousarray(oracle.gen_uniform(41, 3000))
+    n = xy.shape[0]
+    prm = G._Params(0, 0, 1, 1, n, xy.ctypes.data_as(C.POINTER(C.c_double)), 0, None, 512)
+    out = L.gpudtComputeDT(C.byref(prm))
+    assert out
+    F = out.contents.nTris
+    view = np.ctypeslib.as_array(C.cast(out.contents.triangles, C.POINTER(C.c_int)), shape=(F * 9,)).reshape(F, 9)
+    ref, _ = oracle.delaunay(xy)
+    assert np.array_equal(view, ref)
+
+    def alpha():
+        A = C.POINTER(G._Alpha)()
+        assert L.gpudtComputeAlpha(C.byref(prm), out, C.byref(A)) == 0, L.gpudtGetLastError().decode()
+        try:
+            return G._alpha_to_numpy(A)
+        finally:
+            L.gpudtReleaseAlpha(A)
+
+    def check(got, tris):
+        want = oracle.alpha(xy, tris)
+        for k in FIELDS:
+            g, r = np.asarray(got[k]), np.asarray(want[k])
+            assert g.shape == r.shape, k
+            if k == "spectrum":
+                assert np.array_equal(g.view(np.uint64), r.view(np.uint64)), k
+            else:
+                assert np.array_equal(g, r), k
+
+    check(alpha(), ref)
+    # rotate triangle t's corners by one (vtx and tri together) and fix its neighbours' back-links
+    t = F // 2
+    old = view[t].copy()
+    for i in range(3):
+        view[t, i] = old[(i + 1) % 3]
+        view[t, 3 + i] = old[3 + (i + 1) % 3]
+    for i in range(3):
+        e = view[t, i]
+        if e >= 0:
+            u, j = e >> 2, e & 3
+            view[u, j] = (t << 2) | i
+    edited = view.copy()
+    assert not np.array_equal(edited, ref)
+    check(alpha(), edited)
+    L.gpudtReleaseDTOutput(out)
