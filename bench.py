@@ -235,6 +235,9 @@ def stress_c3(G, reps=3, seed=3):
            "ms": round(med * 1e3, 2), "mpoints_per_s": round(n / med / 1e6, 3), "runs": reps,
            "stage_ms": {k: round(float(np.median(v)), 3) for k, v in stage.items()},
            "triangles": st["nTris"], "frame_bulk": st["frameBulk"],
+           "work": {k: st[k] for k in ("shiftBad", "deleted", "missing", "inserted", "duplicates", "insertRounds",
+                                        "flipPasses", "flips", "flipListed", "earRounds", "alphaClusters",
+                                        "alphaMaxCluster")},
            "fallback": {
                "incircle": rate(st["incircleExact"], st["incircleTests"]),
                "orient_exact": int(st["orientExact"]),

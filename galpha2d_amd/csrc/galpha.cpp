@@ -720,20 +720,17 @@ static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t
     }
     if (tot != B) return false;
     (void)n;
-    // cycle arrays, the cycle's positions, then 5 doubles of grid parameters per set
-    c->bnd.ensure(4 * (size_t)((11 * (size_t)B + 3) & ~(size_t)3) + 16 * (size_t)B + 40 * (size_t)nsets + 64);
+    // cycle arrays, the cycle's positions, 5 doubles of grid parameters per set, then two arrays of
+    // claim keys
+    c->bnd.ensure(4 * (size_t)((11 * (size_t)B + 3) & ~(size_t)3) + 32 * (size_t)B + 40 * (size_t)nsets + 64);
     HIPCHK(hipMemcpyAsync(c->hullOff.p, off.data(), 4 * (size_t)nsets, hipMemcpyHostToDevice, st));
     int rounds = 1;
     while ((1 << rounds) < maxh) rounds++;
     rounds += 1;
     launch_bnd_rank(m, B, rounds, blist, bidx, hstart, hcnt, c->hullOff.as<int>(), badset, c->bnd.as<int>(), st);
-    ctx_read_segments(c, {{badset, nsets}, {c->counters.as<int>() + C_BND, 1}}, rd.data());
-    std::copy(rd.begin(), rd.begin() + nsets, bad.begin());
-    int collinear = rd[nsets];
+    ctx_read_segments(c, {{badset, nsets}}, rd.data());
     for (int s = 0; s < nsets; s++)
-        if (bad[s]) return false;
-    // many exactly collinear boundary triples (lattices): ears would mostly be blocked
-    if (collinear > std::max(8, B / 100)) return false;
+        if (rd[s]) return false;
     // from here on the mesh changes
     launch_kill_frame(m, st);
     int *bnd = c->bnd.as<int>(), *hoff = c->hullOff.as<int>();
@@ -752,7 +749,7 @@ static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t
     int round = 0;
     for (;; round += 4) {
         if (round > 200000) throw GalphaError("hull convexification did not finish");
-        launch_ear_rounds(m, B, bnd, hoff, round, 4, c->modF.as<int>(), C_FRAME_LIST, c->op++, st);
+        launch_ear_rounds(m, B, bnd, hcnt, hoff, nsets, round, 4, c->modF.as<int>(), C_FRAME_LIST, c->op++, st);
         if (ctx_counter(c, ((round + 3) & 1) ? C_EAR_PREV : C_EAR) == 0) break;
     }
     *ear_rounds = round + 4;
@@ -1395,7 +1392,7 @@ __attribute__((visibility("hidden"))) size_t galpha_noalloc_need(int n, int T)
     double b = 0;
     b += 18.0 * P + 516.0 * T + 8.0 * ntiles + 8.0 * nc;          // textures, band ranges, tiles, cells
     b += (116.0 + 68.0) * cap;                                     // mesh slots, canonical keys + output
-    b += (89.0 + 60.0 + 12.0) * V + 36.0 * n + 4.0 * nf;           // vertex arrays, boundary cycle, plan
+    b += (89.0 + 76.0 + 12.0) * V + 36.0 * n + 4.0 * nf;           // vertex arrays, boundary cycle, plan
     b += (double)COUNTER_BYTES + 4096.0 + 8.0 * 5 * 64;           // counters, status, box partials
     b += (double)scan_temp_bytes((size_t)std::max(ntiles, V + 2)) + (double)scan_temp_bytes((size_t)nc) + 8192.0;
     b += (double)sort_tmp + 256.0 + 96.0 * 256.0 + 65536.0;        // sort temporary, rounding, slack

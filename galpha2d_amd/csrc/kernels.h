@@ -251,8 +251,8 @@ void launch_kill_frame(MeshDev &m, hipStream_t st);
 // bulk frame removal's ear rounds: the per-set grids, then rounds [r0, r0 + k) as grid-wide kernels
 // (counters[C_EAR] / [C_EAR_PREV] count the candidates of even / odd rounds)
 void launch_ear_grid(MeshDev &m, int B, int *bnd, const int *hull_cnt, const int *hull_off, int nsets, hipStream_t st);
-void launch_ear_rounds(MeshDev &m, int B, int *bnd, const int *hull_off, int r0, int k, int *list_out,
-                       int cnt_out_slot, int op, hipStream_t st);
+void launch_ear_rounds(MeshDev &m, int B, int *bnd, const int *hull_cnt, const int *hull_off, int nsets, int r0,
+                       int k, int *list_out, int cnt_out_slot, int op, hipStream_t st);
 // every set's ear rounds in one workgroup per set (bulk frame removal); counters[C_EAR] = most rounds
 void launch_ear_sets(MeshDev &m, int B, int *bnd, const int *hull_cnt, const int *hull_off, int nsets,
                      int *list_out, int cnt_out_slot, int op, hipStream_t st);

@@ -1482,23 +1482,6 @@ __global__ void k_bnd_place(MeshDev m, int B, const int *blist, const int *succ,
     }
 }
 
-// exactly collinear consecutive boundary triples (a degenerate boundary: most ears would be blocked);
-// positions left unwritten by k_bnd_place (marked sets) are skipped
-__global__ void k_bnd_collinear(MeshDev m, int B, const int *verts, const int *nxt, const int *prv, int *count)
-{
-    int c = 0;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < ((B + 63) & ~63); p += gridDim.x * blockDim.x) {
-        if (p >= B) continue;
-        int vc = verts[p];
-        if (vc < 0) continue;
-        int vp = verts[prv[p]], vn = verts[nxt[p]];
-        if (vp < 0 || vn < 0) continue;
-        double2 P = m.pos[vp], C = m.pos[vc], N = m.pos[vn];
-        if (orient_dev(m, P, C, N) == 0) c++;
-    }
-    wave_count_full(count, c);
-}
-
 // drop every frame triangle; R's triangles across a dropped edge get a hull link; frame vertices
 // leave the mesh; boundary vertices take their outgoing boundary triangle as anchor
 __global__ void k_kill_frame(MeshDev m)
@@ -1537,7 +1520,8 @@ GA_INLINE_HOST_DEVICE int bnd_pos_off(int B) { return (11 * B + 3) & ~3; }
 // cycle is then convex.  The emptiness test looks only at the vertices in the grid cells the ear's
 // bounding box covers: a uniform grid over each set's cycle, about two vertices per cell, built by
 // a counting sort before the first round.  Few large sets run their rounds as grid-wide kernels
-// (three per round); many small sets run all their rounds in one launch, one workgroup per set.
+// (two per round, with fan ears over straight runs of the cycle, below); many small sets run all
+// their rounds in one launch, one workgroup per set.
 GA_INLINE uint32_t ear_prio(int p, int round) { return hash32((uint32_t)p * 0x9E3779B1u ^ (uint32_t)round); }
 GA_INLINE bool ear_beats(int p, int q, int round)
 {
@@ -1726,70 +1710,312 @@ __global__ __launch_bounds__(EAR_THREADS) void k_ear_grid(MeshDev m, int B, int 
 // round r clears the counter round r + 1 will use.
 GA_INLINE int *ear_count(const MeshDev &m, int round) { return m.counters + ((round & 1) ? C_EAR_PREV : C_EAR); }
 
-// One wave per cycle position: the lanes test the vertices of the grid rows the ear's box covers
-// (a row's cells are consecutive, so its vertices are one contiguous range of the cell order).
-__global__ __launch_bounds__(256) void k_ear_cand(MeshDev m, int B, int *bnd, const int *hull_off, int round)
+// Fan ears.  Where the cycle runs straight (C, N, N2, ... collinear) only one of the run's vertices
+// is reflex at a time, and plain ears would take the run one vertex per round (a long thin pocket
+// between a straight side of lattice points and a hull edge takes hundreds).  A candidate C whose
+// neighbour N starts such a run instead cuts the fan (P, N, C), (P, N2, N), ... up to the farthest
+// run vertex E for which the closed triangle (P, E, C) holds no live vertex of the cycle outside
+// the positions P .. E (the run lies on its edge C E) and no fan triangle has P's other
+// neighbour in its circumcircle; the triangles nest, so E is found by binary search.  Mirrored for a run through P, with apex N.  A candidate's span is the positions from its
+// first to its last vertex; a winner holds the largest claim key (plain ear before fan, priority,
+// position) at every position of its span, so the spans of two winners never meet.  Plain ears
+// first: the apex of a pocket's fan is then its last reflex vertex left, not the first one met
+// (a fan from a shallow apex is far from Delaunay, and the flips that repair it run one
+// triangle per pass along the fan).
+#define FAN_CAP 1024    // most run vertices one fan takes (a longer run takes more rounds)
+#define EAR_WAVES 4     // waves per workgroup of the wave-per-position round kernels
+
+GA_INLINE unsigned long long ear_key(int p, int round, bool fan)
+{
+    return ((unsigned long long)!fan << 63) | ((unsigned long long)(ear_prio(p, round) >> 1) << 32) |
+           (unsigned)(p + 1);
+}
+// position k of the set's cyclic range [off, off + h), k within one turn of it
+GA_INLINE int ear_wrap(int k, int off, int h) { return k >= off + h ? k - h : (k < off ? k + h : k); }
+// positions from lo forward to hi in the cyclic range (both included)
+GA_INLINE int ear_span(int lo, int hi, int h) { return (hi >= lo ? hi - lo : hi - lo + h) + 1; }
+
+// wave-uniform: no live vertex of the set at a position outside [lo, hi] lies in the closed
+// triangle (A, Bq, C), clockwise order A, C, Bq
+GA_INLINE bool ear_empty_wave(const MeshDev &m, const EarCycle &e, int off, int h, const double *pr, double2 A,
+                              double2 Bq, double2 C, int lo, int hi)
+{
+    const int lane = threadIdx.x & 63;
+    const double lx = pr[0], ly = pr[1], sx = pr[2], sy = pr[3];
+    const int G = (int)pr[4];
+    const int *cend = ear_cend(e, off), *citem = ear_citem(e, off);
+    const int last = ear_span(lo, hi, h) - 1;
+    double bx0 = fmin(A.x, fmin(C.x, Bq.x)), bx1 = fmax(A.x, fmax(C.x, Bq.x));
+    double by0 = fmin(A.y, fmin(C.y, Bq.y)), by1 = fmax(A.y, fmax(C.y, Bq.y));
+    int cx0 = ear_cell(bx0, lx, sx, G), cx1 = ear_cell(bx1, lx, sx, G);
+    int cy0 = ear_cell(by0, ly, sy, G), cy1 = ear_cell(by1, ly, sy, G);
+    bool blocked = false;
+    for (int cy = cy0; cy <= cy1 && !__any(blocked); cy++) {
+        int c0 = cy * G + cx0, c1 = cy * G + cx1;
+        int j0 = c0 ? cend[c0 - 1] : 0, j1 = cend[c1];
+        for (int j = j0 + lane; j < j1; j += 64) {
+            int q = citem[j];
+            int dq = q - lo;
+            if (dq < 0) dq += h;
+            if (dq <= last || !e.alive[q]) continue;
+            double2 Q = e.bpos[q];
+            if (Q.x < bx0 || Q.x > bx1 || Q.y < by0 || Q.y > by1) continue;
+            if (orient_dev(m, A, Bq, Q) >= 0 && orient_dev(m, Bq, C, Q) >= 0 && orient_dev(m, C, A, Q) >= 0)
+                blocked = true;
+        }
+    }
+    return !__any(blocked);
+}
+
+// The run beyond X on the line (C, X): the live positions after x in direction d (+1 forward)
+// while they lie on the line beyond X; written to run[] in order (at most FAN_CAP).  Returns
+// their count (wave-uniform).  The scan ends at the latest at the candidate's other neighbour,
+// which is off the line.
+GA_INLINE int ear_run_wave(const MeshDev &m, const EarCycle &e, int off, int h, int x, int d, double2 C, double2 X,
+                           int *run)
+{
+    const int lane = threadIdx.x & 63;
+    const double dx = X.x - C.x, dy = X.y - C.y;
+    int R = 0;
+    for (int base = 1; base < h && R < FAN_CAP; base += 64) {
+        int k = base + lane, q = -1;
+        bool live = false, brk = false;
+        if (k < h) {
+            q = ear_wrap(x + d * k, off, h);
+            live = e.alive[q] != 0;
+            if (live) {
+                double2 Q = e.bpos[q];
+                brk = orient_dev(m, C, X, Q) != 0 || (Q.x - X.x) * dx + (Q.y - X.y) * dy <= 0.0;
+            }
+        }
+        unsigned long long bm = __ballot(brk), lm = __ballot(live);
+        if (bm) lm &= (1ull << (__ffsll((long long)bm) - 1)) - 1;   // live lanes before the first break
+        int idx = R + __popcll(lm & ((1ull << lane) - 1));
+        if (((lm >> lane) & 1) && idx < FAN_CAP) run[idx] = q;
+        R += __popcll(lm);
+        if (bm) break;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return min(R, FAN_CAP);
+}
+
+// The usable part of a run: the fan's triangles up to run[k] keep the apex's other neighbour W
+// outside their circumcircles (exact, with the mesh's symbolic tie-break).  Going on past that
+// point would give the apex run edges that belong to W or to vertices beyond it in the final
+// mesh, and the flips that hand them over run one triangle per pass along the fan; stopped
+// there, the next rounds cut the apex and continue the run with W as apex.  d = +1: triangle
+// (apex, run[k], run[k - 1]); d = -1: (run[k], apex, run[k - 1]); run[-1] = x.
+GA_INLINE int ear_fan_delaunay(const MeshDev &m, const EarCycle &e, const int *run, int R, int d, int apex, int w,
+                               int x)
+{
+    const int lane = threadIdx.x & 63;
+    const double2 A = e.bpos[apex], W = e.bpos[w];
+    const int ia = e.verts[apex], iw = e.verts[w];
+    for (int b = 0; b < R; b += 64) {
+        int k = b + lane;
+        bool bad = false;
+        if (k < R) {
+            int qa = run[k], qb = k ? run[k - 1] : x;
+            double2 Qa = e.bpos[qa], Qb = e.bpos[qb];
+            int ja = e.verts[qa], jb = e.verts[qb];
+            bad = d > 0 ? incircle_at(m, A, Qa, Qb, W, ia, ja, jb, iw) > 0
+                        : incircle_at(m, Qa, A, Qb, W, ja, ia, jb, iw) > 0;
+        }
+        unsigned long long bm = __ballot(bad);
+        if (bm) return b + __ffsll((long long)bm) - 1;
+    }
+    return R;
+}
+
+// Largest k < R whose fan (end run[k]) is empty; -1 when even run[0]'s is not.  d = +1: apex P,
+// triangle (P, E, C), span kp .. E; d = -1: apex N, triangle (E, N, C), span E .. kn.
+GA_INLINE int ear_fan_end(const MeshDev &m, const EarCycle &e, int off, int h, const double *pr, const int *run,
+                          int R, int d, double2 P, double2 C, double2 N, int kp, int kn)
+{
+    auto empty = [&](int k) {
+        int q = run[k];
+        double2 E = e.bpos[q];
+        return d > 0 ? ear_empty_wave(m, e, off, h, pr, P, E, C, kp, q)
+                     : ear_empty_wave(m, e, off, h, pr, E, N, C, q, kn);
+    };
+    if (empty(R - 1)) return R - 1;
+    int lo = -1, hi = R - 1;   // run[lo] empty (-1: N or P itself), run[hi] blocked
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (empty(mid)) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One wave per cycle position: cand[p] = 0 (none), 1 (plain ear), 2 (fan through N) or 3 (fan
+// through P), fan[p] = the fan's far end; candidates claim their spans in claims[round & 1].
+__global__ __launch_bounds__(64 * EAR_WAVES) void k_ear_cand(MeshDev m, int B, int *bnd, const int *hull_off,
+                                                            const int *hull_cnt, unsigned long long *claims,
+                                                            int round)
 {
     if (round > 0 && *ear_count(m, round - 1) == 0) return;
+    __shared__ int s_run[EAR_WAVES][FAN_CAP];
     EarCycle e = ear_cycle(bnd, B);
     const int lane = threadIdx.x & 63;
+    int *run = s_run[threadIdx.x >> 6];
+    unsigned long long *claim = claims + (size_t)(round & 1) * B;
     const int wave = (xcd_block() * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
     for (int p = wave; p < B; p += nw) {
-        int c = 0;
+        int c = 0, lo = 0, hi = 0, end = -1, off = 0, h = 0;
         if (e.alive[p]) {
             int kp = e.prv[p], kn = e.nxt[p];
             if (kp != kn) {
                 double2 P = e.bpos[kp], C = e.bpos[p], N = e.bpos[kn];
                 if (orient_dev(m, P, C, N) < 0) {
-                    int off = hull_off[e.pset[p]];
-                    const double *pr = e.param + 5 * (size_t)e.pset[p];
-                    const double lx = pr[0], ly = pr[1], sx = pr[2], sy = pr[3];
-                    const int G = (int)pr[4];
-                    const int *cend = ear_cend(e, off), *citem = ear_citem(e, off);
-                    double bx0 = fmin(P.x, fmin(C.x, N.x)), bx1 = fmax(P.x, fmax(C.x, N.x));
-                    double by0 = fmin(P.y, fmin(C.y, N.y)), by1 = fmax(P.y, fmax(C.y, N.y));
-                    int cx0 = ear_cell(bx0, lx, sx, G), cx1 = ear_cell(bx1, lx, sx, G);
-                    int cy0 = ear_cell(by0, ly, sy, G), cy1 = ear_cell(by1, ly, sy, G);
-                    bool blocked = false;
-                    for (int cy = cy0; cy <= cy1 && !__any(blocked); cy++) {
-                        int c0 = cy * G + cx0, c1 = cy * G + cx1;
-                        int j0 = c0 ? cend[c0 - 1] : 0, j1 = cend[c1];
-                        for (int j = j0 + lane; j < j1; j += 64) {
-                            int q = citem[j];
-                            if (q == p || q == kp || q == kn || !e.alive[q]) continue;
-                            double2 Q = e.bpos[q];
-                            if (Q.x < bx0 || Q.x > bx1 || Q.y < by0 || Q.y > by1) continue;
-                            if (orient_dev(m, P, N, Q) >= 0 && orient_dev(m, N, C, Q) >= 0 &&
-                                orient_dev(m, C, P, Q) >= 0)
-                                blocked = true;
+                    const int s = e.pset[p];
+                    off = hull_off[s];
+                    h = hull_cnt[s];
+                    const double *pr = e.param + 5 * (size_t)s;
+                    if (ear_empty_wave(m, e, off, h, pr, P, N, C, kp, kn)) {
+                        c = 1; lo = kp; hi = kn;
+                        int k2 = e.nxt[kn];
+                        if (k2 != kp && orient_dev(m, C, N, e.bpos[k2]) == 0) {
+                            int R = ear_run_wave(m, e, off, h, kn, 1, C, N, run);
+                            if (R > 0) R = ear_fan_delaunay(m, e, run, R, 1, kp, e.prv[kp], kn);
+                            int f = R > 0 ? ear_fan_end(m, e, off, h, pr, run, R, 1, P, C, N, kp, kn) : -1;
+                            if (f >= 0) { c = 2; end = run[f]; hi = end; }
+                        }
+                        int k0 = e.prv[kp];
+                        if (c == 1 && k0 != kn && orient_dev(m, C, P, e.bpos[k0]) == 0) {
+                            int R = ear_run_wave(m, e, off, h, kp, -1, C, P, run);
+                            if (R > 0) R = ear_fan_delaunay(m, e, run, R, -1, kn, e.nxt[kn], kp);
+                            int f = R > 0 ? ear_fan_end(m, e, off, h, pr, run, R, -1, P, C, N, kp, kn) : -1;
+                            if (f >= 0) { c = 3; end = run[f]; lo = end; }
                         }
                     }
-                    c = __any(blocked) ? 0 : 1;
                 }
             }
         }
+        if (c) {
+            const unsigned long long key = ear_key(p, round, c > 1);
+            const int len = ear_span(lo, hi, h);
+            for (int k = lane; k < len; k += 64) atomicMax(&claim[ear_wrap(lo + k, off, h)], key);
+        }
         if (lane == 0) {
             e.cand[p] = c;
+            e.win[p] = end;
             if (c) atomicAdd(ear_count(m, round), 1);
         }
     }
 }
 
-__global__ void k_ear_select(MeshDev m, int B, int *bnd, int round)
+// The fan of winner p in direction d (+1: run through N, apex P; -1: run through P, apex N) up to
+// position `end`, by the whole wave: the run's live positions q_1 .. q_R after p, triangle i
+// between q_i and q_{i+1} (q_0 = p), consecutive triangles linked across their shared apex edges.
+GA_INLINE void ear_fan_cut(const MeshDev &m, const EarCycle &e, int p, int d, int end, int off, int h, int kp, int kn,
+                           int *list_out, int cnt_out_slot, int op)
 {
-    if (*ear_count(m, round) == 0) return;
-    EarCycle e = ear_cycle(bnd, B);
-    GRID_STRIDE(p, B) e.win[p] = ear_winner(e, p, round);
+    const int lane = threadIdx.x & 63;
+    const int len = ear_span(d > 0 ? p : end, d > 0 ? end : p, h) - 1;   // positions p + d .. end
+    int R = 0;
+    for (int b = 0; b < len; b += 64) {
+        int k = b + lane + 1;
+        bool live = k <= len && e.alive[ear_wrap(p + d * k, off, h)];
+        R += __popcll(__ballot(live));
+    }
+    int T0 = 0;
+    if (lane == 0) T0 = atomicAdd(m.ntri, R);
+    T0 = __shfl(T0, 0, 64);
+    if (T0 + R > m.cap) {
+        if (lane == 0) atomicOr(m.status, GA_ERR_CAPACITY);
+        return;
+    }
+    const int apex = d > 0 ? kp : kn, va = e.verts[apex];
+    int carry = p, i0 = 0;
+    for (int b = 0; b < len; b += 64) {
+        int k = b + lane + 1;
+        int q = k <= len ? ear_wrap(p + d * k, off, h) : -1;
+        bool live = q >= 0 && e.alive[q];
+        unsigned long long lm = __ballot(live), below = lm & ((1ull << lane) - 1);
+        int src = below ? 63 - __clzll((long long)below) : lane;
+        int qprev = __shfl(q, src, 64);
+        if (live) {
+            const int i = i0 + __popcll(below), T = T0 + i;
+            const int qi = below ? qprev : carry;
+            const int vi = e.verts[qi], vn = e.verts[q];
+            int4 tn;
+            if (d > 0) {
+                // (P, q_{i+1}, q_i): edge 0 faces the boundary edge q_i -> q_{i+1}, edge 1 the apex edge
+                const int X0 = e.owner[qi], X1 = i == 0 ? e.owner[kp] : enc(T - 1, 2);
+                m.tv[T] = make_int4(va, vn, vi, 0);
+                tn = make_int4(X0, X1, i == R - 1 ? -1 : enc(T + 1, 1), 0);
+                if (X0 >= 0) set_link(m, enc_t(X0), enc_j(X0), enc(T, 0));
+                if (i == 0 && X1 >= 0) set_link(m, enc_t(X1), enc_j(X1), enc(T, 1));
+            } else {
+                // (q_{i+1}, N, q_i): edge 1 faces the boundary edge q_{i+1} -> q_i, edge 0 the apex edge
+                const int Y0 = i == 0 ? e.owner[p] : enc(T - 1, 2), Y1 = e.owner[q];
+                m.tv[T] = make_int4(vn, va, vi, 0);
+                tn = make_int4(Y0, Y1, i == R - 1 ? -1 : enc(T + 1, 0), 0);
+                if (Y1 >= 0) set_link(m, enc_t(Y1), enc_j(Y1), enc(T, 1));
+                if (i == 0 && Y0 >= 0) set_link(m, enc_t(Y0), enc_j(Y0), enc(T, 0));
+            }
+            m.tn[T] = tn;
+            m.stamp[T] = op;
+            if (atomicExch(&m.lmark[T], m.list_op) != m.list_op)
+                list_out[atomicAdd(&m.counters[cnt_out_slot], 1)] = T;
+            e.alive[qi] = 0;
+            m.vt[vi] = T;
+            m.vt[vn] = T;
+            if (i == 0) m.vt[va] = T;
+        }
+        if (lm) carry = __shfl(q, 63 - __clzll((long long)lm), 64);
+        i0 += __popcll(lm);
+    }
+    if (lane == 0) {
+        const int Tl = T0 + R - 1;
+        if (d > 0) {
+            e.owner[kp] = enc(Tl, 2);
+            e.nxt[kp] = end;
+            e.prv[end] = kp;
+        } else {
+            e.owner[end] = enc(Tl, 2);
+            e.nxt[end] = kn;
+            e.prv[kn] = end;
+        }
+    }
 }
 
-__global__ void k_ear_apply(MeshDev m, int B, int *bnd, int round, int *list_out, int cnt_out_slot, int op)
+// One wave per candidate: a winner (its key at every position of its span) cuts its ear or fan.
+// Clears the claims of the next round (written two rounds ago, read by the previous apply).
+__global__ __launch_bounds__(64 * EAR_WAVES) void k_ear_apply(MeshDev m, int B, int *bnd, const int *hull_off,
+                                                             const int *hull_cnt, unsigned long long *claims,
+                                                             int round, int *list_out, int cnt_out_slot, int op)
 {
     bool any = *ear_count(m, round) != 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) *ear_count(m, round + 1) = 0;
     if (!any) return;
     EarCycle e = ear_cycle(bnd, B);
-    GRID_STRIDE(p, B) {
-        if (e.win[p]) ear_cut(m, e, p, list_out, cnt_out_slot, op);
+    const unsigned long long *claim = claims + (size_t)(round & 1) * B;
+    unsigned long long *other = claims + (size_t)((round + 1) & 1) * B;
+    GRID_STRIDE(k, B) other[k] = 0;
+    const int lane = threadIdx.x & 63;
+    const int wave = (xcd_block() * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
+    for (int p = wave; p < B; p += nw) {
+        const int c = e.cand[p];
+        if (!c) continue;
+        const unsigned long long key = ear_key(p, round, c > 1);
+        // a winner holds p, so no other winner changes p's links or its span
+        if (claim[p] != key) continue;
+        const int s = e.pset[p], off = hull_off[s], h = hull_cnt[s];
+        const int kp = e.prv[p], kn = e.nxt[p], end = e.win[p];
+        const int lo = c == 3 ? end : kp, hi = c == 2 ? end : kn, len = ear_span(lo, hi, h);
+        bool lost = false;
+        for (int k = lane; k < len; k += 64)
+            if (claim[ear_wrap(lo + k, off, h)] != key) lost = true;
+        if (__any(lost)) continue;
+        if (c == 1) {
+            if (lane == 0) ear_cut(m, e, p, list_out, cnt_out_slot, op);
+        } else {
+            ear_fan_cut(m, e, p, c == 2 ? 1 : -1, end, off, h, kp, kn, list_out, cnt_out_slot, op);
+        }
     }
 }
 
@@ -1855,9 +2081,6 @@ void launch_bnd_rank(MeshDev &m, int B, int rounds, const int *blist, const int 
     launch_fill_i32(verts, -1, (size_t)B, st);
     GA_LAUNCH(k_bnd_place, dim3(g), dim3(256), 0, st, m, B, blist, (const int *)succ[cur], (const int *)d[cur],
               hull_cnt, hull_off, badset, verts, owner, nxt, prv, alive, pset, (double2 *)(bnd + bnd_pos_off(B)));
-    launch_reset_counters(m.counters, C_BND, -1, -1, -1, st);
-    GA_LAUNCH(k_bnd_collinear, dim3(g), dim3(256), 0, st, m, B, (const int *)verts, (const int *)nxt,
-              (const int *)prv, m.counters + C_BND);
 }
 
 void launch_kill_frame(MeshDev &m, hipStream_t st)
@@ -1865,23 +2088,30 @@ void launch_kill_frame(MeshDev &m, hipStream_t st)
     GA_LAUNCH(k_kill_frame, dim3(FIXED_GRID), dim3(256), 0, st, m);
 }
 
-// the grid of every set, then rounds r0 .. r0 + k - 1 as grid-wide kernels
+// claim keys of the grid-wide rounds: two arrays of B, after the grid parameters of the sets
+static unsigned long long *ear_claims(int *bnd, int B, int nsets)
+{
+    return (unsigned long long *)(bnd + ((bnd_pos_off(B) + 4 * B + 10 * nsets + 1) & ~1));
+}
+
+// the grid of every set, and cleared claims for the grid-wide rounds
 void launch_ear_grid(MeshDev &m, int B, int *bnd, const int *hull_cnt, const int *hull_off, int nsets, hipStream_t st)
 {
     GA_LAUNCH(k_ear_grid, dim3(nsets), dim3(EAR_THREADS), 0, st, m, B, bnd, hull_cnt, hull_off);
+    launch_fill_i32((int *)ear_claims(bnd, B, nsets), 0, 4 * (size_t)B, st);
 }
 
-void launch_ear_rounds(MeshDev &m, int B, int *bnd, const int *hull_off, int r0, int k, int *list_out,
-                       int cnt_out_slot, int op, hipStream_t st)
+// rounds r0 .. r0 + k - 1 as grid-wide kernels, one wave per cycle position
+void launch_ear_rounds(MeshDev &m, int B, int *bnd, const int *hull_cnt, const int *hull_off, int nsets, int r0,
+                       int k, int *list_out, int cnt_out_slot, int op, hipStream_t st)
 {
-    int g = std::min(4096, (B + 255) / 256);
-    g = (g + 7) & ~7;
-    int gw = std::min(4096, (B + 3) / 4);   // one wave per cycle position
+    unsigned long long *claims = ear_claims(bnd, B, nsets);
+    int gw = std::min(4096, (B + EAR_WAVES - 1) / EAR_WAVES);
     gw = (gw + 7) & ~7;
     for (int r = r0; r < r0 + k; r++) {
-        GA_LAUNCH(k_ear_cand, dim3(gw), dim3(256), 0, st, m, B, bnd, hull_off, r);
-        GA_LAUNCH(k_ear_select, dim3(g), dim3(256), 0, st, m, B, bnd, r);
-        GA_LAUNCH(k_ear_apply, dim3(g), dim3(256), 0, st, m, B, bnd, r, list_out, cnt_out_slot, op);
+        GA_LAUNCH(k_ear_cand, dim3(gw), dim3(64 * EAR_WAVES), 0, st, m, B, bnd, hull_off, hull_cnt, claims, r);
+        GA_LAUNCH(k_ear_apply, dim3(gw), dim3(64 * EAR_WAVES), 0, st, m, B, bnd, hull_off, hull_cnt, claims, r,
+                  list_out, cnt_out_slot, op);
     }
 }
 

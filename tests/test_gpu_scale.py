@@ -49,3 +49,6 @@ def test_lattice_2m():
     st = _check("lattice_2m", datagen.lattice_clusters(2000000, side=1414, nclusters=64, sigma=0.005, seed=3),
                 8192)
     assert st["incircleExact"] > 0
+    # the lattice's straight hull sides leave long thin pockets under hull edges to cluster points;
+    # fan ears fill each in a few rounds instead of one round per lattice point
+    assert st["frameBulk"] == 1 and 0 < st["earRounds"] <= 64
