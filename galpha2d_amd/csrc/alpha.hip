@@ -153,7 +153,7 @@ struct AlphaK {
     int *vnumin, *vnumax;  // [n]
     int *status;
     int *counters;       // [16]
-    int *bigstart, *bigsize;   // large clusters (start position, size)
+    int *bigstart;             // large clusters (start position)
     int bigcap;
     int nS0;
     // batches: set of every vertex, first spectrum slot and rank base of every set
@@ -392,7 +392,11 @@ GA_INLINE int cmp_ids(const AlphaK &a, int ia, int ib)
     return r;
 }
 
-#define MAX_SMALL_CLUSTER 64
+// Clusters of up to MAX_SMALL_CLUSTER entries are put in order by one thread (exact insertion
+// sort: up to m (m - 1) / 2 exact comparisons in sequence); larger ones by the exact-key path,
+// which works on all their entries in parallel.  On the 2M lattice + cluster set a threshold of
+// 64 left one thread with thousands of exact comparisons (36 ms); at 8 the worst is 28.
+#define MAX_SMALL_CLUSTER 8
 
 // ---- large clusters: exact 256-bit keys --------------------------------------------------------
 // For every entry of a large cluster, rho = N / (4 D) * 2^(2s) exactly (big integers).  The key is
@@ -493,8 +497,9 @@ struct BigK {
 
 // one thread per entry of all large clusters (grid-stride); the cluster is found by binary
 // search over the entry offsets coff[0..nbig) (coff[0] = 0, increasing)
-__global__ void k_big_elems(AlphaK a, BigK b, int nbig, int Mb)
+__global__ void k_big_elems(AlphaK a, BigK b, int nbig, int Mb, int maxsize)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&a.counters[A_MAXCLUSTER], maxsize);
     GS(k, Mb) {
         int lo = 0, hi = nbig - 1;
         while (lo < hi) {
@@ -585,17 +590,21 @@ __global__ void k_clusters(AlphaK a, int S0)
     GS(k, S0) {
         if (!a.flag[k]) continue;
         if (k + 1 >= S0 || a.flag[k + 1]) { continue; }
+        // the walk stops after MAX_SMALL_CLUSTER entries: larger clusters get their ends from
+        // k_big_ends (one cluster can hold most of the spectrum, e.g. the equal circumradii of a
+        // lattice's triangles)
         int end = k + 1;
-        while (end < S0 && !a.flag[end]) end++;
-        int m = end - k;
+        const int lim = min(S0, k + MAX_SMALL_CLUSTER);
+        while (end < lim && !a.flag[end]) end++;
         atomicAdd(&a.counters[A_CLUSTERS], 1);
-        atomicMax(&a.counters[A_MAXCLUSTER], m);
-        if (m > MAX_SMALL_CLUSTER) {
+        if (end < S0 && !a.flag[end]) {
             int q = atomicAdd(&a.counters[A_BIGCLUSTER], 1);
-            if (q < a.bigcap) { a.bigstart[q] = k; a.bigsize[q] = m; }
+            if (q < a.bigcap) a.bigstart[q] = k;
             else atomicOr(a.status, GA_ERR_CAPACITY);
             continue;
         }
+        int m = end - k;
+        atomicMax(&a.counters[A_MAXCLUSTER], m);
         int *ids = a.id2 + k;
         for (int i = 1; i < m; i++) {
             int x = ids[i], j = i - 1;
@@ -604,6 +613,28 @@ __global__ void k_clusters(AlphaK a, int S0)
         }
         // group starts inside the cluster (gflag[k] stays 1)
         for (int i = 1; i < m; i++) a.gflag[k + i] = cmp_ids(a, ids[i - 1], ids[i]) != 0;
+    }
+}
+
+// End (one past the last entry) of every large cluster, cstart[0..nbig) in increasing order: a
+// cluster start j (or j = S0) ends a large cluster when no start lies in the MAX_SMALL_CLUSTER
+// entries before it; that cluster is the last large one starting before j.  The backward walks
+// stop at the first start, so they read each small cluster once.
+__global__ void k_big_ends(const int *flag, int S0, const int *cstart, int nbig, int *cend)
+{
+    GS(j, S0 + 1) {
+        if (j == 0 || (j < S0 && !flag[j])) continue;
+        int i = j - 1;
+        const int lim = j - MAX_SMALL_CLUSTER;
+        while (i >= lim && i >= 0 && !flag[i]) i--;
+        if (i >= lim) continue;   // the cluster before j has at most MAX_SMALL_CLUSTER entries
+        int lo = 0, hi = nbig - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (cstart[mid] < j) lo = mid;
+            else hi = mid - 1;
+        }
+        cend[lo] = j;
     }
 }
 
@@ -633,23 +664,42 @@ GA_INLINE double entry_value(const AlphaK &a, int id)
 // of its members
 // An entry alone in its certified cluster is never moved by the exact sorts, so key2[k] still
 // holds its FP64 value, and it is alone in its rank (plain store).  Entries of larger clusters
-// recompute their value and take the minimum.
-GA_INLINE void spec_value(const AlphaK &a, int S0, int k, int id, int slot)
+// recompute their value and take the minimum: first over the lanes of the wave that hold the same
+// slot (slots do not decrease from lane to lane), then one atomicMin per slot and wave (a
+// lattice's triangles can put a million entries in one rank).  Called by every lane of the wave;
+// `act` is false past the end.
+GA_INLINE void spec_value(const AlphaK &a, int S0, int k, int id, int slot, bool act)
 {
-    bool alone = a.flag[k] && (k + 1 == S0 || a.flag[k + 1]);
+    const bool alone = act && a.flag[k] && (k + 1 == S0 || a.flag[k + 1]);
     if (alone && !a.vset) a.specbits[slot] = key_bits(a.key2[k]);
     else if (alone) a.specbits[slot] = (unsigned long long)__double_as_longlong(entry_value(a, id));
-    else atomicMin(&a.specbits[slot], (unsigned long long)__double_as_longlong(entry_value(a, id)));
+    const bool need = act && !alone;
+    if (!__any(need)) return;
+    const int lane = threadIdx.x & 63;
+    unsigned long long v = need ? (unsigned long long)__double_as_longlong(entry_value(a, id)) : ~0ull;
+    const int sl = need ? slot : -1;
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned long long ov = __shfl_up(v, d, 64);
+        int os = __shfl_up(sl, d, 64);
+        if (lane >= d && os == sl && ov < v) v = ov;
+    }
+    const int next = __shfl_down(sl, 1, 64);
+    if (need && (lane == 63 || next != sl)) atomicMin(&a.specbits[slot], v);
 }
 
 __global__ void k_ranks(AlphaK a, int S0, const int *incl)
 {
-    GS(k, S0) {
-        int r = incl[k];
-        int id = a.id2[k];
-        if (id < a.F) a.tri_rank[id] = r;
-        else a.edge_rank[id - a.F] = r;
-        spec_value(a, S0, k, id, r);
+    for (int base = blockIdx.x * blockDim.x; base < S0; base += gridDim.x * blockDim.x) {
+        const int k = base + threadIdx.x;
+        const bool act = k < S0;
+        int r = 0, id = 0;
+        if (act) {
+            r = incl[k];
+            id = a.id2[k];
+            if (id < a.F) a.tri_rank[id] = r;
+            else a.edge_rank[id - a.F] = r;
+        }
+        spec_value(a, S0, k, id, r, act);
     }
 }
 
@@ -682,14 +732,20 @@ __global__ void k_set_bounds(int S0, const unsigned int *sset2, const int *incl2
 __global__ void k_ranks_batch(AlphaK a, int S0, const unsigned int *sset2, const int *order2, const int *incl2,
                               const int *first)
 {
-    GS(j, S0) {
-        int s = (int)sset2[j];
-        int r = incl2[j] - first[s] + 1;
-        int k = order2[j];
-        int id = a.id2[k];
-        if (id < a.F) a.tri_rank[id] = r;
-        else a.edge_rank[id - a.F] = r;
-        spec_value(a, S0, k, id, a.specstart[s] + r);
+    for (int base = blockIdx.x * blockDim.x; base < S0; base += gridDim.x * blockDim.x) {
+        const int j = base + threadIdx.x;
+        const bool act = j < S0;
+        int k = 0, id = 0, slot = 0;
+        if (act) {
+            int s = (int)sset2[j];
+            int r = incl2[j] - first[s] + 1;
+            k = order2[j];
+            id = a.id2[k];
+            if (id < a.F) a.tri_rank[id] = r;
+            else a.edge_rank[id - a.F] = r;
+            slot = a.specstart[s] + r;
+        }
+        spec_value(a, S0, k, id, slot, act);
     }
 }
 
@@ -897,20 +953,25 @@ static void big_clusters(AlphaDev &d, AlphaK &a, int S0, hipStream_t st)
     chk(hipStreamSynchronize(st), "sync");
     if (nbig == 0) return;
     if (nbig > a.bigcap) throw GalphaError("alpha: too many large clusters");
-    std::vector<int> cs(nbig), sz(nbig), off(nbig + 1, 0);
-    chk(hipMemcpyAsync(cs.data(), a.bigstart, 4 * (size_t)nbig, hipMemcpyDeviceToHost, st), "copy");
-    chk(hipMemcpyAsync(sz.data(), a.bigsize, 4 * (size_t)nbig, hipMemcpyDeviceToHost, st), "copy");
+    std::vector<int> cs2(nbig), ce(nbig), sz2(nbig), off(nbig + 1, 0);
+    chk(hipMemcpyAsync(cs2.data(), a.bigstart, 4 * (size_t)nbig, hipMemcpyDeviceToHost, st), "copy");
     chk(hipStreamSynchronize(st), "sync");
-    // order clusters by position so that cluster ids follow spectrum order
-    std::vector<int> ord(nbig);
-    for (int i = 0; i < nbig; i++) ord[i] = i;
-    std::sort(ord.begin(), ord.end(), [&](int x, int y) { return cs[x] < cs[y]; });
-    std::vector<int> cs2(nbig), sz2(nbig);
-    for (int i = 0; i < nbig; i++) { cs2[i] = cs[ord[i]]; sz2[i] = sz[ord[i]]; off[i + 1] = off[i] + sz2[i]; }
-    int Mb = off[nbig];
+    // order clusters by position so that cluster ids follow spectrum order; then their ends
+    std::sort(cs2.begin(), cs2.end());
     ab(d, AB_BIG0, 4 * (size_t)nbig * 3 + 64);
     int *cstart = AP<int>(d, AB_BIG0), *csize = cstart + nbig, *coff = csize + nbig;
     chk(hipMemcpyAsync(cstart, cs2.data(), 4 * (size_t)nbig, hipMemcpyHostToDevice, st), "copy");
+    GA_LAUNCH(k_big_ends, dim3(AGRID), dim3(256), 0, st, (const int *)a.flag, S0, (const int *)cstart, nbig, csize);
+    chk(hipMemcpyAsync(ce.data(), csize, 4 * (size_t)nbig, hipMemcpyDeviceToHost, st), "copy");
+    chk(hipStreamSynchronize(st), "sync");
+    int maxsize = 0;
+    for (int i = 0; i < nbig; i++) {
+        sz2[i] = ce[i] - cs2[i];
+        if (sz2[i] <= MAX_SMALL_CLUSTER || ce[i] > S0) throw GalphaError("alpha: large cluster bounds");
+        off[i + 1] = off[i] + sz2[i];
+        maxsize = std::max(maxsize, sz2[i]);
+    }
+    int Mb = off[nbig];
     chk(hipMemcpyAsync(csize, sz2.data(), 4 * (size_t)nbig, hipMemcpyHostToDevice, st), "copy");
     chk(hipMemcpyAsync(coff, off.data(), 4 * (size_t)nbig, hipMemcpyHostToDevice, st), "copy");
     ab(d, AB_BIG1, (size_t)Mb * (4 * 5 + 8 * 7) + 1024);
@@ -926,7 +987,7 @@ static void big_clusters(AlphaDev &d, AlphaK &a, int S0, hipStream_t st)
     for (int w = 0; w < 5; w++) b.key[w] = (unsigned long long *)take(8 * (size_t)Mb);
     b.kt = (unsigned long long *)take(8 * (size_t)Mb);
     b.kt2 = (unsigned long long *)take(8 * (size_t)Mb);
-    GA_LAUNCH(k_big_elems, dim3((Mb + 255) / 256), dim3(256), 0, st, a, b, nbig, Mb);
+    GA_LAUNCH(k_big_elems, dim3((Mb + 255) / 256), dim3(256), 0, st, a, b, nbig, Mb, maxsize);
     // LSD radix passes: mantissa word 0 (lowest) .. 3, then cluster | exponent
     size_t tb = 0;
     chk(rocprim::radix_sort_pairs(nullptr, tb, b.kt, b.kt2, b.ej, b.vj, (unsigned)Mb, 0u, 64u, st), "sort");
@@ -1037,8 +1098,7 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     a.iv = AP<double2>(d, AB_LO);
     a.slo = AP<double>(d, AB_SLO); a.shi = AP<double>(d, AB_SHI);
     a.flag = AP<int>(d, AB_FLAG); a.rank = AP<int>(d, AB_RANK); a.gflag = AP<int>(d, AB_MISC);
-    ab(d, AB_BIGL, 8 * 65536);
-    a.bigstart = AP<int>(d, AB_BIGL); a.bigsize = a.bigstart + 65536; a.bigcap = 65536;
+
     a.specbits = AP<unsigned long long>(d, AB_SPECBITS);
     a.vset = nsets > 1 ? info.vset : nullptr;
     a.nsets = nsets;
@@ -1108,6 +1168,10 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     GA_LAUNCH(k_cluster_flags, dim3(AGRID), dim3(256), 0, st, a, S0, (const double *)pmax, (const double *)nsmin,
               (const int *)sset);
     GA_LAUNCH(k_copy_i32, dim3(AGRID), dim3(256), 0, st, (const int *)a.flag, a.gflag, S0);
+    // every large cluster has more than MAX_SMALL_CLUSTER entries
+    a.bigcap = S0 / (MAX_SMALL_CLUSTER + 1) + 1;
+    ab(d, AB_BIGL, 4 * (size_t)a.bigcap + 64);
+    a.bigstart = AP<int>(d, AB_BIGL);
     GA_LAUNCH(k_clusters, dim3(AGRID), dim3(256), 0, st, a, S0);
     big_clusters(d, a, S0, st);
     // ranks: inclusive scan of group starts.  A single set reads its rank count R with the final
