@@ -584,9 +584,11 @@ __global__ void k_big_fix(AlphaK a, BigK b, int Mbig, const int *closef, const i
     }
 }
 
-// one thread per cluster start with size >= 2: exact insertion sort, then group flags
+// one thread per cluster start with size >= 2: exact insertion sort, then group flags; the
+// cluster count and largest size go to the counters once per wave
 __global__ void k_clusters(AlphaK a, int S0)
 {
+    int nclu = 0, mmax = 0;
     GS(k, S0) {
         if (!a.flag[k]) continue;
         if (k + 1 >= S0 || a.flag[k + 1]) { continue; }
@@ -596,7 +598,7 @@ __global__ void k_clusters(AlphaK a, int S0)
         int end = k + 1;
         const int lim = min(S0, k + MAX_SMALL_CLUSTER);
         while (end < lim && !a.flag[end]) end++;
-        atomicAdd(&a.counters[A_CLUSTERS], 1);
+        nclu++;
         if (end < S0 && !a.flag[end]) {
             int q = atomicAdd(&a.counters[A_BIGCLUSTER], 1);
             if (q < a.bigcap) a.bigstart[q] = k;
@@ -604,7 +606,7 @@ __global__ void k_clusters(AlphaK a, int S0)
             continue;
         }
         int m = end - k;
-        atomicMax(&a.counters[A_MAXCLUSTER], m);
+        mmax = max(mmax, m);
         int *ids = a.id2 + k;
         for (int i = 1; i < m; i++) {
             int x = ids[i], j = i - 1;
@@ -614,6 +616,9 @@ __global__ void k_clusters(AlphaK a, int S0)
         // group starts inside the cluster (gflag[k] stays 1)
         for (int i = 1; i < m; i++) a.gflag[k + i] = cmp_ids(a, ids[i - 1], ids[i]) != 0;
     }
+    for (int d = 32; d > 0; d >>= 1) mmax = max(mmax, __shfl_xor(mmax, d, 64));
+    if (__lane_id() == 0 && mmax) atomicMax(&a.counters[A_MAXCLUSTER], mmax);
+    wave_count_full(&a.counters[A_CLUSTERS], nclu);
 }
 
 // End (one past the last entry) of every large cluster, cstart[0..nbig) in increasing order: a
@@ -1038,8 +1043,12 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     for (int s = 0; s <= nsets; s++) d.specstart[s] = s;   // empty spectra until ranks exist
     ab(d, AB_STATUS, 16);
     ab(d, AB_CNTRS, 64 * 4);
-    chk(hipMemsetAsync(d.b[AB_STATUS].p, 0, 16, st), "memset");
-    chk(hipMemsetAsync(d.b[AB_CNTRS].p, 0, 256, st), "memset");
+    {
+        IntFills f;
+        f.add(d.b[AB_STATUS].p, 4, 0);
+        f.add(d.b[AB_CNTRS].p, 64, 0);
+        launch_fill_multi(f, st);
+    }
     ab(d, AB_VMIN, 4 * (size_t)(n + 1));
     ab(d, AB_VMAX, 4 * (size_t)(n + 1));
     ab(d, AB_SPEC, 8 * 2);
@@ -1179,7 +1188,12 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     launch_exclusive_scan(a.gflag, a.rank, S0, d.b[AB_SCAN].p, st);
     int R = -1, nslots = -1;
     if (nsets == 1) {
-        chk(hipMemsetAsync(a.specbits, 0xff, 8 * (size_t)(S0 + 2), st), "memset");
+        // spectrum slots (minimum by atomics) and the vertex intervals' extremes
+        IntFills f;
+        f.add(a.specbits, 2 * (size_t)(S0 + 2), -1);
+        f.add(a.vnumin, (size_t)n, 0x7fffffff);
+        f.add(a.vnumax, (size_t)n, -1);
+        launch_fill_multi(f, st);
     } else {
         int h4[2];
         read_ints(d, st, {a.rank + S0 - 1, a.gflag + S0 - 1}, h4);
@@ -1226,7 +1240,11 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
         d.specSize = nslots;
         chk(hipMemcpyAsync(sstart, d.specstart.data(), 4 * (size_t)nsets, hipMemcpyHostToDevice, st), "copy");
         a.specstart = sstart;
-        chk(hipMemsetAsync(a.specbits, 0xff, 8 * (size_t)nslots, st), "memset");
+        IntFills f;
+        f.add(a.specbits, 2 * (size_t)nslots, -1);
+        f.add(a.vnumin, (size_t)n, 0x7fffffff);
+        f.add(a.vnumax, (size_t)n, -1);
+        launch_fill_multi(f, st);
         GA_LAUNCH(k_ranks_batch, dim3(AGRID), dim3(256), 0, st, a, S0, (const unsigned *)sset2, (const int *)order2,
                   (const int *)incl2, (const int *)first);
     }
@@ -1235,9 +1253,7 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     GA_LAUNCH(k_spec_values, dim3(AGRID), dim3(256), 0, st, (const unsigned long long *)a.specbits, nslots,
               (const int *)(incl + S0 - 1), AP<double>(d, AB_SPEC));
     chk(hipEventRecord(d.ev[2], st), "event");
-    // intervals
-    launch_fill_i32(a.vnumin, 0x7fffffff, (size_t)n, st);
-    launch_fill_i32(a.vnumax, -1, (size_t)n, st);
+    // intervals (vnumin / vnumax were set with the spectrum slots)
     GA_LAUNCH(k_edge_intervals, dim3(AGRID), dim3(256), 0, st, a, E);
     GA_LAUNCH(k_vert_fix, dim3(AGRID), dim3(256), 0, st, a);
     // master list

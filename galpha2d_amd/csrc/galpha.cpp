@@ -130,6 +130,11 @@ struct galpha_ctx {
     galpha_stats stats{};
     hipEvent_t ev[16] = {};
     int op = 1;
+    // the slot-tag arrays (stamp, claim, lmark) and the per-vertex removal maxima (mx) as last
+    // cleared, and the removal round number, which grows across runs like op
+    void *tag_p[3] = {nullptr, nullptr, nullptr};
+    void *mx_p = nullptr;
+    int prio_round = 0;
     int *hsnap = nullptr;   // pinned staging for counter reads (SNAP_INTS ints)
     std::vector<int> hsnap_pageable;   // counter reads of a no-allocation context
     // the last gpudtComputeDT on this (default) context: the caller's point array and the returned
@@ -694,14 +699,18 @@ static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t
     int *outdeg = m.decide, *cover = m.deg, *bidx = c->fill.as<int>(), *blist = c->mlist.as<int>();
     c->badset.ensure(4 * (size_t)nsets + 16);
     int *badset = c->badset.as<int>(), *hcnt = c->hullCnt.as<int>(), *hstart = c->hullStart.as<int>();
-    HIPCHK(hipMemsetAsync(m.hnext, 0xff, 4 * (size_t)V, st));
-    HIPCHK(hipMemsetAsync(outdeg, 0, 4 * (size_t)V, st));
-    HIPCHK(hipMemsetAsync(cover, 0, 4 * (size_t)V, st));
-    HIPCHK(hipMemsetAsync(bidx, 0xff, 4 * (size_t)V, st));
-    HIPCHK(hipMemsetAsync(hcnt, 0, 4 * (size_t)nsets, st));
-    HIPCHK(hipMemsetAsync(badset, 0, 4 * (size_t)nsets, st));
-    launch_fill_i32(hstart, 0x7fffffff, (size_t)nsets, st);
-    launch_reset_counters(c->counters.as<int>(), C_BND, -1, -1, -1, st);
+    {
+        IntFills f;
+        f.add(m.hnext, (size_t)V, -1);
+        f.add(outdeg, (size_t)V, 0);
+        f.add(cover, (size_t)V, 0);
+        f.add(bidx, (size_t)V, -1);
+        f.add(hcnt, (size_t)nsets, 0);
+        f.add(badset, (size_t)nsets, 0);
+        f.add(hstart, (size_t)nsets, 0x7fffffff);
+        f.add(c->counters.as<int>() + C_BND, 1, 0);
+        launch_fill_multi(f, st);
+    }
     launch_bnd_detect(m, outdeg, cover, hcnt, hstart, badset, blist, bidx, st);
     std::vector<int> rd(2 * (size_t)nsets + 1);
     ctx_read_segments(c, {{badset, nsets}, {hcnt, nsets}, {c->counters.as<int>() + C_BND, 1}}, rd.data());
@@ -795,8 +804,12 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     if (nsets == 0) return 0;
     int V = plan.V, n = plan.ntot;
     ensure_pba(c, plan);
-    HIPCHK(hipMemsetAsync(c->counters.p, 0, COUNTER_BYTES, st));
-    HIPCHK(hipMemsetAsync(c->status.p, 0, sizeof(int) * 4, st));
+    {
+        IntFills f;
+        f.add(c->counters.p, COUNTER_BYTES / sizeof(int), 0);
+        f.add(c->status.p, 4, 0);
+        launch_fill_multi(f, st);
+    }
     BatchDev bd = upload_plan(c, plan);
     ev_mark(c, 0, st);
     d_xy = renumber_points(c, d_xy, bd, plan);
@@ -835,17 +848,33 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     c->lmark.ensure(4 * (size_t)cap);
     c->ffan.ensure(8 * (size_t)(V - n) + 16);
     c->modF.ensure(4 * (size_t)cap);
-    HIPCHK(hipMemsetAsync(c->stamp.p, 0, 4 * (size_t)cap, st));
-    HIPCHK(hipMemsetAsync(c->claim.p, 0, 8 * (size_t)cap, st));
-    HIPCHK(hipMemsetAsync(c->lmark.p, 0, 4 * (size_t)cap, st));
-    HIPCHK(hipMemsetAsync(c->vt.p, 0xff, 4 * (size_t)V, st));
-    // removal rounds (deletion, frame) tag their priorities with an increasing round number, so
-    // the per-vertex maxima are cleared once per run
-    HIPCHK(hipMemsetAsync(c->mx.p, 0, 8 * (size_t)V, st));
-    int prio_round = 0;
+    // Slot tags (stamp, claim keys, list marks) carry the context's op number, and the removal
+    // rounds (deletion, frame) tag their per-vertex priority maxima with a round number; both
+    // only grow, across runs too, so values left by earlier runs never match or beat later ones.
+    // The arrays are cleared only when their memory is new (or carved from a shared scratch
+    // region) and when the numbers would run out.
+    {
+        IntFills f;
+        const bool arena = galpha_arena_active();
+        if (arena || c->op > (1 << 30) || c->tag_p[0] != c->stamp.p || c->tag_p[1] != c->claim.p ||
+            c->tag_p[2] != c->lmark.p) {
+            f.add(c->stamp.p, c->stamp.bytes / 4, 0);
+            f.add(c->claim.p, c->claim.bytes / 4, 0);
+            f.add(c->lmark.p, c->lmark.bytes / 4, 0);
+            c->tag_p[0] = c->stamp.p; c->tag_p[1] = c->claim.p; c->tag_p[2] = c->lmark.p;
+            c->op = 1;
+        }
+        if (arena || c->prio_round > (1 << 19) || c->mx_p != c->mx.p) {
+            f.add(c->mx.p, c->mx.bytes / 4, 0);
+            c->mx_p = c->mx.p;
+            c->prio_round = 0;
+        }
+        f.add(c->vt.p, (size_t)V, -1);
+        launch_fill_multi(f, st);
+    }
     auto next_prio_round = [&]() {
-        if (++prio_round >= (1 << 20)) throw GalphaError("too many removal rounds");
-        return prio_round;
+        if (++c->prio_round >= (1 << 20)) throw GalphaError("too many removal rounds");
+        return c->prio_round;
     };
 
     MeshDev m{};
@@ -970,10 +999,10 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     auto flip_until_done = [&](int *listA, int cntA, int *listB, int cntB, int prev, long long hint) {
         int *lin = listA, *lout = listB, sin = cntA, sout = cntB;
         int grid = hint >= 0 ? round_grid(2 * hint) : 2048;
+        launch_reset_counters(c->counters.as<int>(), C_CAND, -1, -1, -1, st);
         for (int pass = 0; pass < 100000; pass += 3) {
             for (int k = 0; k < 3; k++) {
                 int opn = c->op++;
-                launch_reset_counters(c->counters.as<int>(), C_CAND, sout, C_FLIPS, -1, st);
                 launch_flip_pass(m, lin, sin, lout, sout, prev, opn, grid, st);
                 prev = opn;
                 std::swap(lin, lout);

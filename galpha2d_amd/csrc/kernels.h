@@ -193,6 +193,21 @@ struct IntSegments {
     int n;
 };
 void launch_gather_segments(const IntSegments &g, int *dst, hipStream_t st);
+// up to eight device int arrays, each set to one value, in one launch
+struct IntFills {
+    int *p[8];
+    size_t len[8];
+    int v[8];
+    int n = 0;
+    void add(void *ptr, size_t ints, int value)
+    {
+        p[n] = (int *)ptr;
+        len[n] = ints;
+        v[n] = value;
+        n++;
+    }
+};
+void launch_fill_multi(const IntFills &f, hipStream_t st);
 
 // scan.hip
 size_t scan_temp_bytes(size_t n);

@@ -877,9 +877,10 @@ void launch_insert_round(MeshDev &m, int *mlist, int *loc, int op, int grid, hip
 }
 
 // ---- generic link repair -------------------------------------------------------------------
-__global__ void k_relink(MeshDev m, const int *list, int cnt_slot, int op)
+__global__ void k_relink(MeshDev m, const int *list, int cnt_slot, int op, int zero_slot)
 {
     int n = m.counters[cnt_slot];
+    if (zero_slot >= 0 && blockIdx.x == 0 && threadIdx.x == 0) m.counters[zero_slot] = 0;
     GRID_STRIDE(q, n) {
         int t = list[q];
         if (t < 0) continue;
@@ -904,7 +905,7 @@ __global__ void k_relink(MeshDev m, const int *list, int cnt_slot, int op)
 
 void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, int grid, hipStream_t st)
 {
-    GA_LAUNCH(k_relink, dim3(grid), dim3(256), 0, st, m, modlist, cnt_slot, op);
+    GA_LAUNCH(k_relink, dim3(grid), dim3(256), 0, st, m, modlist, cnt_slot, op, -1);
 }
 
 // ---- constraint edge set ----------------------------------------------------------------------
@@ -956,10 +957,16 @@ void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st)
 // loop over 256 * per list entries per block and step with one reservation of candidate slots
 // per step: per = 4 for lists longer than four times the grid (fewer reservations), else 1 (more
 // waves in flight for the latency-bound gathers).
-__global__ __launch_bounds__(256) void k_flip_test(MeshDev m, const int *list, int cnt_slot, int prev_op, int op)
+__global__ __launch_bounds__(256) void k_flip_test(MeshDev m, const int *list, int cnt_slot, int prev_op, int op,
+                                                   int cnt_out_slot)
 {
     int n = m.counters[cnt_slot];
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&m.counters[C_FLIP_LISTED], n);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&m.counters[C_FLIP_LISTED], n);
+        // the commit of this pass counts its output list and its flips from zero
+        m.counters[cnt_out_slot] = 0;
+        m.counters[C_FLIPS] = 0;
+    }
     int tests = 0;
     int per = n > 4 * (int)(gridDim.x * 256) ? 4 : 1;
     for (int b0 = xcd_block() * 256 * per; b0 < n; b0 += gridDim.x * 256 * per) {
@@ -1067,12 +1074,15 @@ __global__ void k_flip_commit(MeshDev m, int op, int *list_out, int cnt_out_slot
     stat_add_full(m.stats, C_FLIPS_TOTAL, flips);
 }
 
+// Three launches per pass and no counter resets of their own: the test clears the output list's
+// count and the flip flag, and the relink clears the candidate count for the next pass (it must
+// be zero before the first pass).
 void launch_flip_pass(MeshDev &m, const int *list_in, int cnt_in_slot, int *list_out, int cnt_out_slot,
                       int prev_op, int op, int grid, hipStream_t st)
 {
-    GA_LAUNCH(k_flip_test, dim3(grid), dim3(256), 0, st, m, list_in, cnt_in_slot, prev_op, op);
+    GA_LAUNCH(k_flip_test, dim3(grid), dim3(256), 0, st, m, list_in, cnt_in_slot, prev_op, op, cnt_out_slot);
     GA_LAUNCH(k_flip_commit, dim3(grid), dim3(256), 0, st, m, op, list_out, cnt_out_slot);
-    GA_LAUNCH(k_relink, dim3(grid), dim3(256), 0, st, m, (const int *)list_out, cnt_out_slot, op);
+    GA_LAUNCH(k_relink, dim3(grid), dim3(256), 0, st, m, (const int *)list_out, cnt_out_slot, op, C_CAND);
 }
 
 // ---- frame removal ---------------------------------------------------------------------------
@@ -2565,5 +2575,5 @@ void launch_cons_insert(MeshDev &m, const int2 *cv, int nc, int *done, const int
     GA_LAUNCH(k_cons_collect, dim3(FIXED_GRID), dim3(256), 0, st, m, cv, nc, cnt, roff, work, op);
     GA_LAUNCH(k_cons_commit_small, dim3(FIXED_GRID), dim3(256), 0, st, m, nc, done, cnt, roff, work, modlist, op);
     GA_LAUNCH(k_cons_commit_wave, dim3(FIXED_GRID), dim3(256), 0, st, m, nc, done, cnt, roff, work, modlist, op);
-    GA_LAUNCH(k_relink, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)modlist, C_CONS_MOD, op);
+    GA_LAUNCH(k_relink, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)modlist, C_CONS_MOD, op, -1);
 }

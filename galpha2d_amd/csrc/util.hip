@@ -101,3 +101,32 @@ void launch_gather_segments(const IntSegments &g, int *dst, hipStream_t st)
     int blocks = std::max(1, std::min(256, (tot + 255) / 256));
     GA_LAUNCH(k_gather_segments, dim3(blocks), dim3(256), 0, st, g, dst);
 }
+
+// several device int arrays, each set to one value, in one launch (16-byte stores where a
+// segment's start is 16-byte aligned)
+__global__ void k_fill_multi(IntFills f)
+{
+    const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x, nth = (size_t)gridDim.x * blockDim.x;
+    for (int k = 0; k < f.n; k++) {
+        int *p = f.p[k];
+        const size_t n = f.len[k];
+        const int v = f.v[k];
+        size_t head = 0;
+        if (((uintptr_t)p & 15) == 0) {
+            const size_t n4 = n / 4;
+            int4 *p4 = (int4 *)p;
+            for (size_t i = tid; i < n4; i += nth) p4[i] = make_int4(v, v, v, v);
+            head = 4 * n4;
+        }
+        for (size_t i = head + tid; i < n; i += nth) p[i] = v;
+    }
+}
+
+void launch_fill_multi(const IntFills &f, hipStream_t st)
+{
+    size_t tot = 0;
+    for (int k = 0; k < f.n; k++) tot += f.len[k];
+    if (!tot) return;
+    int blocks = (int)std::max((size_t)1, std::min((size_t)2048, (tot / 4 + 255) / 256));
+    GA_LAUNCH(k_fill_multi, dim3(blocks), dim3(256), 0, st, f);
+}
