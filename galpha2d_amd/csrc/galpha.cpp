@@ -751,17 +751,19 @@ static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t
         *ear_rounds = -1;
         return true;
     }
-    // few sets: grid-wide rounds, four per host read (a round after one without candidates
-    // returns at once)
+    // few sets: grid-wide rounds, eight per host read (a round after one without candidates
+    // returns at once; 1M uniform points take about 20 rounds)
     launch_ear_grid(m, B, bnd, hcnt, hoff, nsets, st);
     launch_reset_counters(c->counters.as<int>(), C_EAR, -1, -1, -1, st);
+    const int per_read = 8;
     int round = 0;
-    for (;; round += 4) {
+    for (;; round += per_read) {
         if (round > 200000) throw GalphaError("hull convexification did not finish");
-        launch_ear_rounds(m, B, bnd, hcnt, hoff, nsets, round, 4, c->modF.as<int>(), C_FRAME_LIST, c->op++, st);
-        if (ctx_counter(c, ((round + 3) & 1) ? C_EAR_PREV : C_EAR) == 0) break;
+        launch_ear_rounds(m, B, bnd, hcnt, hoff, nsets, round, per_read, c->modF.as<int>(), C_FRAME_LIST, c->op++,
+                          st);
+        if (ctx_counter(c, ((round + per_read - 1) & 1) ? C_EAR_PREV : C_EAR) == 0) break;
     }
-    *ear_rounds = round + 4;
+    *ear_rounds = round + per_read;
     return true;
 }
 

@@ -1468,6 +1468,35 @@ __global__ void k_bnd_rank_step(int B, const int *succ, const int *d, int *succ2
     }
 }
 
+// the same rounds in one workgroup, in place (each round reads every entry's successor into
+// registers, then writes), for cycles of up to RANK_BLOCK * RANK_PER positions: one launch instead
+// of one per round
+#define RANK_BLOCK 1024
+#define RANK_PER 16
+__global__ __launch_bounds__(RANK_BLOCK) void k_bnd_rank_block(int B, int rounds, int *succ, int *d)
+{
+    int sn[RANK_PER], dn[RANK_PER];
+    for (int r = 0; r < rounds; r++) {
+#pragma unroll
+        for (int j = 0; j < RANK_PER; j++) {
+            int k = threadIdx.x + j * RANK_BLOCK;
+            if (k >= B) continue;
+            int sk = succ[k], dk = d[k];
+            sn[j] = sk < 0 ? -1 : succ[sk];
+            dn[j] = sk < 0 ? dk : dk + d[sk];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RANK_PER; j++) {
+            int k = threadIdx.x + j * RANK_BLOCK;
+            if (k >= B) continue;
+            succ[k] = sn[j];
+            d[k] = dn[j];
+        }
+        __syncthreads();
+    }
+}
+
 // a cycle that does not return to its start within the rounds (a second cycle in the set), or two
 // vertices ranked to one position, marks the set; verts[] starts at -1, so a position of a marked
 // set may stay unwritten (verts -1, nxt/prv unset) and later kernels skip it
@@ -2083,10 +2112,14 @@ void launch_bnd_rank(MeshDev &m, int B, int rounds, const int *blist, const int 
     int g = std::min(4096, (B + 255) / 256);
     GA_LAUNCH(k_bnd_rank_init, dim3(g), dim3(256), 0, st, m, B, blist, bidx, hull_start, succ[0], d[0], badset);
     int cur = 0;
-    for (int r = 0; r < rounds; r++) {
-        GA_LAUNCH(k_bnd_rank_step, dim3(g), dim3(256), 0, st, B, (const int *)succ[cur], (const int *)d[cur],
-                  succ[1 - cur], d[1 - cur]);
-        cur = 1 - cur;
+    if (B <= RANK_BLOCK * RANK_PER) {
+        GA_LAUNCH(k_bnd_rank_block, dim3(1), dim3(RANK_BLOCK), 0, st, B, rounds, succ[0], d[0]);
+    } else {
+        for (int r = 0; r < rounds; r++) {
+            GA_LAUNCH(k_bnd_rank_step, dim3(g), dim3(256), 0, st, B, (const int *)succ[cur], (const int *)d[cur],
+                      succ[1 - cur], d[1 - cur]);
+            cur = 1 - cur;
+        }
     }
     launch_fill_i32(verts, -1, (size_t)B, st);
     GA_LAUNCH(k_bnd_place, dim3(g), dim3(256), 0, st, m, B, blist, (const int *)succ[cur], (const int *)d[cur],

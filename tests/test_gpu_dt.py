@@ -153,6 +153,10 @@ def _degenerate_cases(oracle):
     out["fan_2000"] = np.array([(i, 0.0) for i in range(2000)] + [(999.5, 7.0)], dtype=np.float64)
     th = 2 * np.pi * np.arange(2000) / 2000
     out["wheel_2000"] = np.concatenate([np.stack([np.cos(th), np.sin(th)], 1), [[0.001, 0.002]]])
+    # the same with the high-degree vertex first: it is the smallest vertex of all its triangles,
+    # which overflows the per-vertex buckets of the canonical order (the keys are sorted instead)
+    out["wheel_2000_centre_first"] = np.concatenate([[[0.001, 0.002]], np.stack([np.cos(th), np.sin(th)], 1)])
+    out["fan_2000_apex_first"] = np.array([(999.5, 7.0)] + [(i, 0.0) for i in range(2000)], dtype=np.float64)
     u = oracle.gen_uniform(91, 3000)
     out["offset"] = u + 1.0e6
     out["scaled_tiny"] = u * 1e-9
