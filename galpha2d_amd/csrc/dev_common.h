@@ -419,11 +419,11 @@ GA_INLINE int wave_reserve_n(int *counter, int c)
     return __shfl(base, leader, 64) + pre;
 }
 
-// c slots per thread for a whole 256-thread block: every thread of the block calls it at the
-// same site (block-uniform control flow); one atomic per block
+// c slots per thread for a whole block (up to 1024 threads): every thread of the block calls it at
+// the same site (block-uniform control flow); one atomic per block
 GA_INLINE int block_reserve(int *counter, int c)
 {
-    __shared__ int s_w[4];
+    __shared__ int s_w[16];
     __shared__ int s_base;
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int inc = c;
@@ -434,7 +434,8 @@ GA_INLINE int block_reserve(int *counter, int c)
     if (lane == 63) s_w[w] = inc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        int tot = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) tot += s_w[i];
         s_base = tot ? atomicAdd(counter, tot) : 0;
     }
     __syncthreads();

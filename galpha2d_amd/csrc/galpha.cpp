@@ -693,8 +693,12 @@ void bbox_sets_device(const double *d_xy, const int *d_offs, int nsets, double *
 // Bulk frame removal: when the real region of every set is bounded by one simple cycle through all
 // of its meshed points, drop all frame triangles and convexify each cycle by parallel ear rounds.
 // Returns false (mesh untouched) when some set does not qualify.
-static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t st, int *ear_rounds)
+// On success *ear_rounds = the rounds run (-1: read later) and *frame_list = the length of the
+// list of triangles it created (-1: not read).
+static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t st, int *ear_rounds,
+                               long long *frame_list)
 {
+    *frame_list = -1;
     int V = m.V, n = m.n;
     int *outdeg = m.decide, *cover = m.deg, *bidx = c->fill.as<int>(), *blist = c->mlist.as<int>();
     c->badset.ensure(4 * (size_t)nsets + 16);
@@ -761,7 +765,12 @@ static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t
         if (round > 200000) throw GalphaError("hull convexification did not finish");
         launch_ear_rounds(m, B, bnd, hcnt, hoff, nsets, round, per_read, c->modF.as<int>(), C_FRAME_LIST, c->op++,
                           st);
-        if (ctx_counter(c, ((round + per_read - 1) & 1) ? C_EAR_PREV : C_EAR) == 0) break;
+        int cv[C_EAR_PREV - C_FRAME_LIST + 1];
+        ctx_counters_read(c, cv, C_FRAME_LIST, C_EAR_PREV - C_FRAME_LIST + 1);
+        if (cv[(((round + per_read - 1) & 1) ? C_EAR_PREV : C_EAR) - C_FRAME_LIST] == 0) {
+            *frame_list = cv[0];
+            break;
+        }
     }
     *ear_rounds = round + per_read;
     return true;
@@ -998,11 +1007,34 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     Snap last;
     // `hint`: the length of the first list when the caller knows it (else a full grid); later
     // passes size their grids by the list length of the last read (lists shrink from pass to pass).
+    // Lists of at most TAIL_LIST triangles go to the single-workgroup tail, which runs passes
+    // back to back without launches or host reads until one flips nothing.
     auto flip_until_done = [&](int *listA, int cntA, int *listB, int cntB, int prev, long long hint) {
         int *lin = listA, *lout = listB, sin = cntA, sout = cntB;
         int grid = hint >= 0 ? round_grid(2 * hint) : 2048;
+        long long next = hint;   // length of the next pass's list, when known
         launch_reset_counters(c->counters.as<int>(), C_CAND, -1, -1, -1, st);
-        for (int pass = 0; pass < 100000; pass += 3) {
+        for (int pass = 0; pass < 100000;) {
+            if (next >= 0 && next <= TAIL_LIST) {
+                const int maxp = 512;
+                const int op0 = c->op;
+                c->op += maxp;
+                launch_flip_tail(m, lin, sin, lout, sout, prev, op0, maxp, st);
+                last = ctx_snapshot(c);
+                const int p = last.v[C_TAIL_PASSES];
+                S.flipPasses += p;
+                pass += p;
+                if (p & 1) {
+                    std::swap(lin, lout);
+                    std::swap(sin, sout);
+                }
+                if (p > 0) prev = op0 + p - 1;
+                if (last.v[C_TAIL_DONE]) return;
+                next = last.v[sin];
+                grid = round_grid(2LL * next);
+                if (p == maxp && next <= TAIL_LIST) continue;
+                next = TAIL_LIST + 1;   // the list outgrew the tail: grid passes first
+            }
             for (int k = 0; k < 3; k++) {
                 int opn = c->op++;
                 launch_flip_pass(m, lin, sin, lout, sout, prev, opn, grid, st);
@@ -1011,9 +1043,11 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
                 std::swap(sin, sout);
             }
             S.flipPasses += 3;
+            pass += 3;
             last = ctx_snapshot(c);
             if (last.v[C_FLIPS] == 0) return;
-            grid = round_grid(2LL * last.v[sin]);
+            next = last.v[sin];
+            grid = round_grid(2LL * next);
         }
         throw GalphaError("flipping did not converge");
     };
@@ -1061,7 +1095,8 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     // the mesh was Delaunay before, so only their edges can need flips afterwards.
     m.list_op = c->op++;
     launch_reset_counters(c->counters.as<int>(), C_FRAME_LIST, -1, -1, -1, st);
-    if (bulk_frame_removal(c, m, nsets, st, &S.earRounds)) {
+    long long frame_list = -1;   // triangles listed by frame removal, when known
+    if (bulk_frame_removal(c, m, nsets, st, &S.earRounds, &frame_list)) {
         S.frameBulk = 1;
     } else {
     for (int r = 0;; r += 2) {
@@ -1078,7 +1113,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
                      c->hullStart.as<int>(), c->scanTmp.p, st);
     }
     ev_mark(c, 6, st);
-    flip_until_done(c->modF.as<int>(), C_FRAME_LIST, m.modB, C_MODN2, m.list_op, -1);
+    flip_until_done(c->modF.as<int>(), C_FRAME_LIST, m.modB, C_MODN2, m.list_op, frame_list);
     ev_mark(c, 7, st);
     const Snap after_frame = last;   // per-stage statistics: frame removal and the final flips
     if (S.frameBulk && S.earRounds < 0) S.earRounds = after_frame.v[C_EAR];

@@ -155,6 +155,9 @@ struct MeshDev {
 #define C_STATUS 32           // device status word (GA_ERR_* bits); MeshDev::status points here
 #define C_CONS_PEND 33        // constraints still to insert (the current insertion round's walk)
 #define C_CONS_MOD 34         // slots rewritten by the current insertion round
+#define C_TAIL_PASSES 35      // passes run by the last single-workgroup flip tail
+#define C_TAIL_DONE 36        // 1 when that tail ended with a pass without flips
+#define TAIL_LIST 2048        // longest flip list the single-workgroup tail takes
 // status bits of constraint insertion (the other GA_ERR_* bits are in dev_common.h)
 #define GA_ERR_CONS_POINT 128     // a point lies inside a constraint segment
 #define GA_ERR_CONS_CROSS 256     // two constraints cross
@@ -235,6 +238,9 @@ void launch_reset_counters(int *counters, int a, int b, int d, int e, hipStream_
 // counters[C_FOLD + k] = sum over the stripes of statistic k, for k < 64
 void launch_fold_stats(int *counters, hipStream_t st);
 void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st);
+// flip passes on short lists in one workgroup (see k_flip_tail)
+void launch_flip_tail(MeshDev &m, int *lA, int sA, int *lB, int sB, int prev_op, int op0, int max_passes,
+                      hipStream_t st);
 void launch_flip_pass(MeshDev &m, const int *list_in, int cnt_in_slot, int *list_out, int cnt_out_slot,
                       int prev_op, int op, int grid, hipStream_t st);
 // Blocks of 256 threads for a round over about `items` list entries: enough for one entry per

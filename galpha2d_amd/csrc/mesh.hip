@@ -877,10 +877,8 @@ void launch_insert_round(MeshDev &m, int *mlist, int *loc, int op, int grid, hip
 }
 
 // ---- generic link repair -------------------------------------------------------------------
-__global__ void k_relink(MeshDev m, const int *list, int cnt_slot, int op, int zero_slot)
+GA_INLINE void relink_body(const MeshDev &m, const int *list, int n, int op)
 {
-    int n = m.counters[cnt_slot];
-    if (zero_slot >= 0 && blockIdx.x == 0 && threadIdx.x == 0) m.counters[zero_slot] = 0;
     GRID_STRIDE(q, n) {
         int t = list[q];
         if (t < 0) continue;
@@ -901,6 +899,13 @@ __global__ void k_relink(MeshDev m, const int *list, int cnt_slot, int op, int z
         }
         if (ch) m.tn[t] = r;
     }
+}
+
+__global__ void k_relink(MeshDev m, const int *list, int cnt_slot, int op, int zero_slot)
+{
+    int n = m.counters[cnt_slot];
+    if (zero_slot >= 0 && blockIdx.x == 0 && threadIdx.x == 0) m.counters[zero_slot] = 0;
+    relink_body(m, list, n, op);
 }
 
 void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, int grid, hipStream_t st)
@@ -957,26 +962,20 @@ void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st)
 // loop over 256 * per list entries per block and step with one reservation of candidate slots
 // per step: per = 4 for lists longer than four times the grid (fewer reservations), else 1 (more
 // waves in flight for the latency-bound gathers).
-__global__ __launch_bounds__(256) void k_flip_test(MeshDev m, const int *list, int cnt_slot, int prev_op, int op,
-                                                   int cnt_out_slot)
+// block-uniform: every thread of the block runs the same iterations (block_reserve inside)
+GA_INLINE void flip_test_body(const MeshDev &m, const int *list, int n, int prev_op, int op)
 {
-    int n = m.counters[cnt_slot];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        atomicAdd(&m.counters[C_FLIP_LISTED], n);
-        // the commit of this pass counts its output list and its flips from zero
-        m.counters[cnt_out_slot] = 0;
-        m.counters[C_FLIPS] = 0;
-    }
     int tests = 0;
-    int per = n > 4 * (int)(gridDim.x * 256) ? 4 : 1;
-    for (int b0 = xcd_block() * 256 * per; b0 < n; b0 += gridDim.x * 256 * per) {
+    const int bd = (int)blockDim.x;
+    int per = n > 4 * (int)(gridDim.x * bd) ? 4 : 1;
+    for (int b0 = xcd_block() * bd * per; b0 < n; b0 += gridDim.x * bd * per) {
         unsigned bits = 0;   // bit 3j + i: edge i of the j-th triangle of this thread wants a flip
         int tj[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             tj[j] = -1;
             if (j >= per) continue;
-            int q = b0 + j * 256 + (int)threadIdx.x;
+            int q = b0 + j * bd + (int)threadIdx.x;
             int t = q < n ? list[q] : -1;
             tj[j] = t;
             if (t < 0) continue;
@@ -1027,9 +1026,21 @@ __global__ __launch_bounds__(256) void k_flip_test(MeshDev m, const int *list, i
     stat_add_full(m.stats, C_TESTS_INCIRCLE, tests);
 }
 
-__global__ void k_flip_commit(MeshDev m, int op, int *list_out, int cnt_out_slot)
+__global__ __launch_bounds__(256) void k_flip_test(MeshDev m, const int *list, int cnt_slot, int prev_op, int op,
+                                                   int cnt_out_slot)
 {
-    int n = m.counters[C_CAND];
+    int n = m.counters[cnt_slot];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&m.counters[C_FLIP_LISTED], n);
+        // the commit of this pass counts its output list and its flips from zero
+        m.counters[cnt_out_slot] = 0;
+        m.counters[C_FLIPS] = 0;
+    }
+    flip_test_body(m, list, n, prev_op, op);
+}
+
+GA_INLINE void flip_commit_body(const MeshDev &m, int n, int op, int *list_out, int cnt_out_slot)
+{
     int flips = 0;
     GRID_STRIDE(q, n) {
         int c = m.cand[q];
@@ -1072,6 +1083,63 @@ __global__ void k_flip_commit(MeshDev m, int op, int *list_out, int cnt_out_slot
     for (int d = 32; d > 0; d >>= 1) wf += __shfl_xor(wf, d, 64);
     if (__lane_id() == 0 && wf) m.counters[C_FLIPS] = 1;
     stat_add_full(m.stats, C_FLIPS_TOTAL, flips);
+}
+
+__global__ void k_flip_commit(MeshDev m, int op, int *list_out, int cnt_out_slot)
+{
+    flip_commit_body(m, m.counters[C_CAND], op, list_out, cnt_out_slot);
+}
+
+// Short lists: the remaining passes in one workgroup, phases separated by block barriers instead
+// of launches, until a pass flips nothing (counters[C_TAIL_DONE] = 1) or the list outgrows
+// TAIL_LIST or max_passes run out (the host goes on with grid passes).  Pass k uses op op0 + k;
+// counters[C_TAIL_PASSES] = passes run (odd: the lists have swapped).
+#define TAIL_THREADS 1024
+__global__ __launch_bounds__(TAIL_THREADS) void k_flip_tail(MeshDev m, int *lA, int sA, int *lB, int sB,
+                                                            int prev_op, int op0, int max_passes)
+{
+    int *lin = lA, *lout = lB, sin = sA, sout = sB;
+    int prev = prev_op, pass = 0, done = 0;
+    for (; pass < max_passes; pass++) {
+        const int n = m.counters[sin];
+        if (n > TAIL_LIST) break;
+        const int op = op0 + pass;
+        __syncthreads();   // every thread has read n before the counters are reset
+        if (threadIdx.x == 0) {
+            atomicAdd(&m.counters[C_FLIP_LISTED], n);
+            m.counters[sout] = 0;
+            m.counters[C_CAND] = 0;
+            m.counters[C_FLIPS] = 0;
+        }
+        __syncthreads();
+        flip_test_body(m, lin, n, prev, op);
+        __syncthreads();
+        flip_commit_body(m, m.counters[C_CAND], op, lout, sout);
+        __syncthreads();
+        relink_body(m, lout, m.counters[sout], op);
+        __syncthreads();
+        const bool any = m.counters[C_FLIPS] != 0;
+        prev = op;
+        int *tl = lin; lin = lout; lout = tl;
+        int ts = sin; sin = sout; sout = ts;
+        if (!any) {
+            pass++;
+            done = 1;
+            break;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m.counters[C_CAND] = 0;
+        m.counters[C_TAIL_PASSES] = pass;
+        m.counters[C_TAIL_DONE] = done;
+    }
+}
+
+void launch_flip_tail(MeshDev &m, int *lA, int sA, int *lB, int sB, int prev_op, int op0, int max_passes,
+                      hipStream_t st)
+{
+    GA_LAUNCH(k_flip_tail, dim3(1), dim3(TAIL_THREADS), 0, st, m, lA, sA, lB, sB, prev_op, op0, max_passes);
 }
 
 // Three launches per pass and no counter resets of their own: the test clears the output list's
