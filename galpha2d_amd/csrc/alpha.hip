@@ -154,6 +154,8 @@ struct AlphaK {
     int *status;
     int *counters;       // [16]
     int *bigstart;             // large clusters (start position)
+    int *xclu;                 // small clusters that need exact comparisons (start position)
+    int *xatt;                 // edges whose attachment needs the exact predicate
     int bigcap;
     int nS0;
     // batches: set of every vertex, first spectrum slot and rank base of every set
@@ -169,6 +171,8 @@ struct AlphaK {
 #define A_MAXCLUSTER 4
 #define A_BIGCLUSTER 5
 #define A_BADMESH 6      // caller triangles that fail k_mesh_check
+#define A_EXCLU 7        // small clusters left to the exact kernel (list in xclu)
+#define A_ATT_LIST 8     // edges whose attachment the filter leaves open (list in xatt)
 
 GA_INLINE int tv(const AlphaK &a, int t, int i) { return a.tris[(size_t)t * 9 + 3 + i]; }
 GA_INLINE int tn(const AlphaK &a, int t, int i) { return a.tris[(size_t)t * 9 + i]; }
@@ -278,6 +282,23 @@ __global__ void k_tri_rho(AlphaK a)
     }
 }
 
+// attachment of edge (p, q) to the apex k by the FP64 filter: 1 attached, 0 not, -1 open
+GA_INLINE int attached_filter(double2 p, double2 q, double2 k)
+{
+    double dx = p.x - q.x, dy = p.y - q.y;
+    double d1 = dx * dx + dy * dy;
+    double sx = p.x + q.x - 2 * k.x, sy = p.y + q.y - 2 * k.y;
+    double d2 = sx * sx + sy * sy;
+    double diff = d1 - d2;
+    double px = fabs(p.x + q.x) + 2 * fabs(k.x), py = fabs(p.y + q.y) + 2 * fabs(k.y);
+    double eb = 32.0 * U53 * (d1 + px * px + py * py);
+    if (diff > eb) return 1;
+    if (-diff > eb) return 0;
+    return -1;
+}
+
+// Edges decided by the filter; an edge it leaves open goes to xatt for k_attached_exact (which
+// carries the expansion arithmetic and its scratch, so this kernel runs at full occupancy).
 __global__ void k_attached(AlphaK a, int Emax)
 {
     int E = a.eoff[a.F - 1] + a.cnt[a.F - 1];
@@ -285,6 +306,30 @@ __global__ void k_attached(AlphaK a, int Emax)
         if (e >= E) { a.flag[e] = 0; continue; }   // the scan of unattached edges runs to Emax
         int t = a.edge_t[e], i = a.edge_v[e];
         double2 p = a.xy[tv(a, t, pl1(i))], q = a.xy[tv(a, t, mi1(i))];
+        int nb = tn(a, t, i);
+        int att = attached_filter(p, q, a.xy[tv(a, t, i)]);
+        if (att == 0 && nb >= 0) att = attached_filter(p, q, a.xy[tv(a, nb >> 2, nb & 3)]);
+        const bool open = att < 0;
+        int slot = wave_reserve(&a.counters[A_ATT_LIST], open);
+        if (open) {
+            a.xatt[slot] = e;
+            att = 0;   // set by k_attached_exact before the flags are scanned again
+        }
+        a.edge_att[e] = att;
+        a.flag[e] = !att;
+        double v = rho_edge(p, q);
+        a.iv[a.F + e] = make_double2(v * (1.0 - 8.0 * U53), v * (1.0 + 8.0 * U53));
+    }
+}
+
+// the edges left open by k_attached: both sides with the exact predicate where the filter fails
+__global__ void k_attached_exact(AlphaK a)
+{
+    const int nx = a.counters[A_ATT_LIST];
+    GS(q, nx) {
+        int e = a.xatt[q];
+        int t = a.edge_t[e], i = a.edge_v[e];
+        double2 p = a.xy[tv(a, t, pl1(i))], pq = a.xy[tv(a, t, mi1(i))];
         int nb = tn(a, t, i);
         int att = 0;
         for (int side = 0; side < 2 && !att; side++) {
@@ -294,26 +339,16 @@ __global__ void k_attached(AlphaK a, int Emax)
                 if (nb < 0) break;
                 k = a.xy[tv(a, nb >> 2, nb & 3)];
             }
-            double dx = p.x - q.x, dy = p.y - q.y;
-            double d1 = dx * dx + dy * dy;
-            double sx = p.x + q.x - 2 * k.x, sy = p.y + q.y - 2 * k.y;
-            double d2 = sx * sx + sy * sy;
-            double diff = d1 - d2;
-            double px = fabs(p.x + q.x) + 2 * fabs(k.x), py = fabs(p.y + q.y) + 2 * fabs(k.y);
-            double eb = 32.0 * U53 * (d1 + px * px + py * py);
-            if (diff > eb) att = 1;
-            else if (-diff > eb) att = 0;
-            else {
+            att = attached_filter(p, pq, k);
+            if (att < 0) {
                 atomicAdd(&a.counters[A_EXACT_ATT], 1);
                 int ovf = 0;
-                att = attached_exact(p, q, k, &ovf);
+                att = attached_exact(p, pq, k, &ovf);
                 if (ovf) atomicOr(a.status, GA_ERR_EXACT_OVERFLOW);
             }
         }
         a.edge_att[e] = att;
         a.flag[e] = !att;
-        double v = rho_edge(p, q);
-        a.iv[a.F + e] = make_double2(v * (1.0 - 8.0 * U53), v * (1.0 + 8.0 * U53));
     }
 }
 
@@ -584,8 +619,11 @@ __global__ void k_big_fix(AlphaK a, BigK b, int Mbig, const int *closef, const i
     }
 }
 
-// one thread per cluster start with size >= 2: exact insertion sort, then group flags; the
-// cluster count and largest size go to the counters once per wave
+// One thread per cluster start with size >= 2.  A small cluster whose certified intervals are
+// pairwise disjoint is ordered by them (its values are distinct: every entry starts a group);
+// one with overlapping intervals goes to xclu for k_clusters_exact, which holds the expansion
+// arithmetic (so this kernel needs no scratch).  The cluster count and largest size go to the
+// counters once per wave.
 __global__ void k_clusters(AlphaK a, int S0)
 {
     int nclu = 0, mmax = 0;
@@ -605,20 +643,64 @@ __global__ void k_clusters(AlphaK a, int S0)
             else atomicOr(a.status, GA_ERR_CAPACITY);
             continue;
         }
-        int m = end - k;
+        const int m = end - k;
         mmax = max(mmax, m);
+        int *ids = a.id2 + k;
+        int idv[MAX_SMALL_CLUSTER];
+        double lov[MAX_SMALL_CLUSTER], hiv[MAX_SMALL_CLUSTER];
+#pragma unroll
+        for (int i = 0; i < MAX_SMALL_CLUSTER; i++) {
+            idv[i] = 0; lov[i] = 0.0; hiv[i] = 0.0;
+            if (i < m) {
+                idv[i] = ids[i];
+                double2 w = a.iv[idv[i]];
+                lov[i] = w.x;
+                hiv[i] = w.y;
+            }
+        }
+        bool open = false;
+#pragma unroll
+        for (int i = 0; i < MAX_SMALL_CLUSTER; i++)
+#pragma unroll
+            for (int j = i + 1; j < MAX_SMALL_CLUSTER; j++)
+                if (j < m && !(hiv[i] < lov[j] || hiv[j] < lov[i])) open = true;
+        if (open) {
+            a.xclu[atomicAdd(&a.counters[A_EXCLU], 1)] = k;
+            continue;
+        }
+#pragma unroll
+        for (int i = 0; i < MAX_SMALL_CLUSTER; i++) {
+            if (i >= m) continue;
+            int r = 0;
+#pragma unroll
+            for (int j = 0; j < MAX_SMALL_CLUSTER; j++)
+                if (j < m && lov[j] < lov[i]) r++;
+            ids[r] = idv[i];
+            if (i > 0) a.gflag[k + i] = 1;
+        }
+    }
+    for (int d = 32; d > 0; d >>= 1) mmax = max(mmax, __shfl_xor(mmax, d, 64));
+    if (__lane_id() == 0 && mmax) atomicMax(&a.counters[A_MAXCLUSTER], mmax);
+    wave_count_full(&a.counters[A_CLUSTERS], nclu);
+}
+
+// the small clusters with overlapping intervals: exact insertion sort, then group flags
+__global__ void k_clusters_exact(AlphaK a, int S0)
+{
+    const int nx = a.counters[A_EXCLU];
+    GS(q, nx) {
+        const int k = a.xclu[q];
+        int end = k + 1;
+        while (end < S0 && !a.flag[end]) end++;
+        const int m = end - k;
         int *ids = a.id2 + k;
         for (int i = 1; i < m; i++) {
             int x = ids[i], j = i - 1;
             while (j >= 0 && cmp_ids(a, ids[j], x) > 0) { ids[j + 1] = ids[j]; j--; }
             ids[j + 1] = x;
         }
-        // group starts inside the cluster (gflag[k] stays 1)
         for (int i = 1; i < m; i++) a.gflag[k + i] = cmp_ids(a, ids[i - 1], ids[i]) != 0;
     }
-    for (int d = 32; d > 0; d >>= 1) mmax = max(mmax, __shfl_xor(mmax, d, 64));
-    if (__lane_id() == 0 && mmax) atomicMax(&a.counters[A_MAXCLUSTER], mmax);
-    wave_count_full(&a.counters[A_CLUSTERS], nclu);
 }
 
 // End (one past the last entry) of every large cluster, cstart[0..nbig) in increasing order: a
@@ -888,7 +970,7 @@ enum {
     AB_ID2, AB_LO, AB_HI, AB_SLO, AB_SHI, AB_FLAG, AB_RANK, AB_SPECBITS, AB_VMIN, AB_VMAX, AB_STATUS, AB_CNTRS,
     AB_TMP, AB_SCAN, AB_NEG, AB_PMAX, AB_NSMIN, AB_MKEY, AB_MVAL, AB_MFLAG, AB_MPOS, AB_OKEY, AB_OVAL,
     AB_SKEY, AB_SVAL, AB_SPEC, AB_SP, AB_MISC, AB_BIG0, AB_BIG1, AB_BIG2, AB_BIG3, AB_BIGL,
-    AB_BS0, AB_BS1, AB_BS2, AB_BS3, AB_BS4, AB_BSET, AB_MSET, AB_VHULL, AB_RANKQ, AB_RD, AB_COUNT
+    AB_BS0, AB_BS1, AB_BS2, AB_BS3, AB_BS4, AB_BSET, AB_MSET, AB_VHULL, AB_RANKQ, AB_RD, AB_XATT, AB_XCLU, AB_COUNT
 };
 
 static void ab(AlphaDev &d, int i, size_t bytes)
@@ -953,9 +1035,10 @@ static void read_ints(AlphaDev &d, hipStream_t st, std::initializer_list<const i
 
 static void big_clusters(AlphaDev &d, AlphaK &a, int S0, hipStream_t st)
 {
-    int nbig = 0;
-    chk(hipMemcpyAsync(&nbig, a.counters + A_BIGCLUSTER, 4, hipMemcpyDeviceToHost, st), "copy");
-    chk(hipStreamSynchronize(st), "sync");
+    int cv[3];
+    read_ints(d, st, {a.counters + A_BIGCLUSTER, a.counters + A_EXCLU}, cv);
+    const int nbig = cv[0];
+    if (cv[1] > 0) GA_LAUNCH(k_clusters_exact, dim3(std::min(AGRID, (cv[1] + 255) / 256)), dim3(256), 0, st, a, S0);
     if (nbig == 0) return;
     if (nbig > a.bigcap) throw GalphaError("alpha: too many large clusters");
     std::vector<int> cs2(nbig), ce(nbig), sz2(nbig), off(nbig + 1, 0);
@@ -1115,11 +1198,20 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     a.vnumin = AP<int>(d, AB_VMIN); a.vnumax = AP<int>(d, AB_VMAX);
     GA_LAUNCH(k_edges, dim3(AGRID), dim3(256), 0, st, a);
     GA_LAUNCH(k_tri_rho, dim3(AGRID), dim3(256), 0, st, a);
+    ab(d, AB_XATT, 4 * (size_t)Emax + 64);
+    a.xatt = AP<int>(d, AB_XATT);
     GA_LAUNCH(k_attached, dim3(AGRID), dim3(256), 0, st, a, Emax);
     // positions of unattached edges (flags past E are zero)
     launch_exclusive_scan(a.flag, a.rank, Emax, d.b[AB_SCAN].p, st);
-    int h3[4];
-    read_ints(d, st, {a.rank + Emax - 1, a.flag + Emax - 1, a.eoff + F - 1, a.cnt + F - 1}, h3);
+    int h3[5];
+    read_ints(d, st, {a.rank + Emax - 1, a.flag + Emax - 1, a.eoff + F - 1, a.cnt + F - 1, a.counters + A_ATT_LIST},
+              h3);
+    if (h3[4] > 0) {
+        // edges the filter left open: the exact predicate, then the positions again
+        GA_LAUNCH(k_attached_exact, dim3(std::min(AGRID, (h3[4] + 255) / 256)), dim3(256), 0, st, a);
+        launch_exclusive_scan(a.flag, a.rank, Emax, d.b[AB_SCAN].p, st);
+        read_ints(d, st, {a.rank + Emax - 1, a.flag + Emax - 1, a.eoff + F - 1, a.cnt + F - 1}, h3);
+    }
     int nun = h3[0] + h3[1];
     E = h3[2] + h3[3];
     if (E > Emax) throw GalphaError("alpha: more edges than the Euler bound (" + std::to_string(E) + ")");
@@ -1177,10 +1269,12 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     GA_LAUNCH(k_cluster_flags, dim3(AGRID), dim3(256), 0, st, a, S0, (const double *)pmax, (const double *)nsmin,
               (const int *)sset);
     GA_LAUNCH(k_copy_i32, dim3(AGRID), dim3(256), 0, st, (const int *)a.flag, a.gflag, S0);
-    // every large cluster has more than MAX_SMALL_CLUSTER entries
+    // every large cluster has more than MAX_SMALL_CLUSTER entries, every cluster at least two
     a.bigcap = S0 / (MAX_SMALL_CLUSTER + 1) + 1;
     ab(d, AB_BIGL, 4 * (size_t)a.bigcap + 64);
     a.bigstart = AP<int>(d, AB_BIGL);
+    ab(d, AB_XCLU, 4 * ((size_t)S0 / 2 + 1) + 64);
+    a.xclu = AP<int>(d, AB_XCLU);
     GA_LAUNCH(k_clusters, dim3(AGRID), dim3(256), 0, st, a, S0);
     big_clusters(d, a, S0, st);
     // ranks: inclusive scan of group starts.  A single set reads its rank count R with the final
