@@ -36,11 +36,12 @@ ALGO_BYTES = {
         "bytes": lambda st, T: 12.0 * st["frameSeen"] + 88.0 * st["frameChain"]},
     "k_flip_test": {
         "model": "32 B per listed triangle: the flipping stage's 24F + 16V per pass over F triangles "
-                 "and V = F / 2 vertices (24 B vertex + neighbour record, 16 B position)",
-        "bytes": lambda st, T: 32.0 * st["flipListed"],
+                 "and V = F / 2 vertices (24 B vertex + neighbour record, 16 B position); the "
+                 "triangles listed by the grid passes (the single-workgroup tail's are not k_flip_test's)",
+        "bytes": lambda st, T: 32.0 * (st["flipListed"] - st["flipTailListed"]),
         "gather_model": "156 B per listed triangle (list entry 4, vertices + links 32, three neighbour "
                         "stamps and far vertices 24, six positions 96)",
-        "gather_bytes": lambda st, T: 156.0 * st["flipListed"]},
+        "gather_bytes": lambda st, T: 156.0 * (st["flipListed"] - st["flipTailListed"])},
     "k_pba_color": {
         "model": "8 B per pixel (envelope read 4 + label written 4)",
         "bytes": lambda st, T: 8.0 * T * T * st["pbaIterations"]},

@@ -38,7 +38,8 @@ class _Stats(C.Structure):
                 ("constraintRounds", C.c_int), ("ms_constraints", C.c_double),
                 ("ms_alpha_rho", C.c_double), ("ms_alpha_order", C.c_double), ("ms_alpha_lists", C.c_double),
                 ("stageListed", C.c_longlong * 3), ("stageIncircleTests", C.c_longlong * 3),
-                ("stageIncircleExact", C.c_longlong * 3), ("stageOrientExact", C.c_longlong * 3)]
+                ("stageIncircleExact", C.c_longlong * 3), ("stageOrientExact", C.c_longlong * 3),
+                ("flipTailPasses", C.c_int), ("flipTailListed", C.c_longlong)]
 
 
 class _Alpha(C.Structure):

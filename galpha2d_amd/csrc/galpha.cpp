@@ -1023,6 +1023,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
                 last = ctx_snapshot(c);
                 const int p = last.v[C_TAIL_PASSES];
                 S.flipPasses += p;
+                S.flipTailPasses += p;
                 pass += p;
                 if (p & 1) {
                     std::swap(lin, lout);
@@ -1216,6 +1217,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     S.frameSeen = fin.stat(C_FRAME_SEEN);
     S.frameChain = fin.stat(C_FRAME_CHAIN);
     S.flipListed = fin.v[C_FLIP_LISTED];
+    S.flipTailListed = fin.v[C_TAIL_LISTED];
     S.incircleTests = fin.stat(C_TESTS_INCIRCLE);
     S.incircleExact = fin.stat(C_EXACT_INCIRCLE);
     S.orientExact = fin.stat(C_EXACT_ORIENT);

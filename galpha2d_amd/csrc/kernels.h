@@ -157,6 +157,7 @@ struct MeshDev {
 #define C_CONS_MOD 34         // slots rewritten by the current insertion round
 #define C_TAIL_PASSES 35      // passes run by the last single-workgroup flip tail
 #define C_TAIL_DONE 36        // 1 when that tail ended with a pass without flips
+#define C_TAIL_LISTED 37      // triangles listed by flip tails (also counted in C_FLIP_LISTED)
 #define TAIL_LIST 2048        // longest flip list the single-workgroup tail takes
 // status bits of constraint insertion (the other GA_ERR_* bits are in dev_common.h)
 #define GA_ERR_CONS_POINT 128     // a point lies inside a constraint segment

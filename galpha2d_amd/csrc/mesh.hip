@@ -1039,44 +1039,63 @@ __global__ __launch_bounds__(256) void k_flip_test(MeshDev m, const int *list, i
     flip_test_body(m, list, n, prev_op, op);
 }
 
+// Block-uniform loop (block_reserve inside): up to four candidates per thread and step, and one
+// reservation of output-list slots per block and step (a reservation per wave put thousands of
+// atomics on the one counter).
 GA_INLINE void flip_commit_body(const MeshDev &m, int n, int op, int *list_out, int cnt_out_slot)
 {
     int flips = 0;
-    GRID_STRIDE(q, n) {
-        int c = m.cand[q];
-        int t = c / 3, i = c % 3;
-        unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)c);
-        bool win = m.claim[t] == key;
-        int4 tt = m.tv[t], nt = m.tn[t];
-        int e = ic(nt, i);
-        int u = enc_t(e), j = enc_j(e);
-        win = win && m.claim[u] == key;
-        // Next pass's list: both triangles of a flip, and the first triangle of a losing
-        // candidate (its edge may stay non-Delaunay when the candidate that beat it lost too);
-        // lmark keeps every slot at most once per list, so no candidate key repeats.
-        bool lt_ = atomicExch(&m.lmark[t], op) != op;
-        bool lu_ = win && atomicExch(&m.lmark[u], op) != op;
-        int q1, q2;
-        wave_reserve2(&m.counters[cnt_out_slot], lt_, lu_, &q1, &q2);
-        if (lt_) list_out[q1] = t;
-        if (lu_) list_out[q2] = u;
-        if (!win) continue;
-        int4 ut = m.tv[u], un = m.tn[u];
-        int cc = ic(tt, i), x = ic(tt, p1(i)), y = ic(tt, m1(i));
-        int d = ic(ut, j);
-        int nt1 = ic(nt, p1(i)), nt2 = ic(nt, m1(i));   // (y,c) , (c,x)
-        int nu1 = ic(un, p1(j)), nu2 = ic(un, m1(j));   // (x,d) , (d,y)
-        m.tv[t] = make_int4(cc, x, d, 0);
-        m.tv[u] = make_int4(d, y, cc, 0);
-        m.tn[t] = make_int4(mark_old(nu1), enc(u, 1), mark_old(nt2), 0);
-        m.tn[u] = make_int4(mark_old(nt1), enc(t, 1), mark_old(nu2), 0);
-        int4 rt = make_int4(0, 0, 0, 0), ru = make_int4(0, 0, 0, 0);
-        sc(rt, i, -1); sc(rt, p1(i), enc(u, 0)); sc(rt, m1(i), enc(t, 2));
-        sc(ru, j, -1); sc(ru, p1(j), enc(t, 0)); sc(ru, m1(j), enc(u, 2));
-        m.remap[t] = rt; m.remap[u] = ru;
-        m.stamp[t] = op; m.stamp[u] = op;
-        m.vt[x] = t; m.vt[cc] = t; m.vt[y] = u; m.vt[d] = u;
-        flips++;
+    const int bd = (int)blockDim.x;
+    const int per = n > 4 * (int)(gridDim.x * bd) ? 4 : 1;
+    for (int b0 = xcd_block() * bd * per; b0 < n; b0 += gridDim.x * bd * per) {
+        int tq[4], uq[4];
+        unsigned outs = 0;   // bit 2j: slot t of the j-th candidate joins the next list, 2j + 1: slot u
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            tq[j] = 0;
+            uq[j] = 0;
+            if (j >= per) continue;
+            int q = b0 + j * bd + (int)threadIdx.x;
+            if (q >= n) continue;
+            int c = m.cand[q];
+            int t = c / 3, i = c % 3;
+            unsigned long long key = ((unsigned long long)op << 32) | (0xffffffffu - (unsigned)c);
+            bool win = m.claim[t] == key;
+            int4 tt = m.tv[t], nt = m.tn[t];
+            int e = ic(nt, i);
+            int u = enc_t(e), jj = enc_j(e);
+            win = win && m.claim[u] == key;
+            // Next pass's list: both triangles of a flip, and the first triangle of a losing
+            // candidate (its edge may stay non-Delaunay when the candidate that beat it lost too);
+            // lmark keeps every slot at most once per list, so no candidate key repeats.
+            if (atomicExch(&m.lmark[t], op) != op) outs |= 1u << (2 * j);
+            if (win && atomicExch(&m.lmark[u], op) != op) outs |= 2u << (2 * j);
+            tq[j] = t;
+            uq[j] = u;
+            if (!win) continue;
+            int4 ut = m.tv[u], un = m.tn[u];
+            int cc = ic(tt, i), x = ic(tt, p1(i)), y = ic(tt, m1(i));
+            int d = ic(ut, jj);
+            int nt1 = ic(nt, p1(i)), nt2 = ic(nt, m1(i));   // (y,c) , (c,x)
+            int nu1 = ic(un, p1(jj)), nu2 = ic(un, m1(jj));   // (x,d) , (d,y)
+            m.tv[t] = make_int4(cc, x, d, 0);
+            m.tv[u] = make_int4(d, y, cc, 0);
+            m.tn[t] = make_int4(mark_old(nu1), enc(u, 1), mark_old(nt2), 0);
+            m.tn[u] = make_int4(mark_old(nt1), enc(t, 1), mark_old(nu2), 0);
+            int4 rt = make_int4(0, 0, 0, 0), ru = make_int4(0, 0, 0, 0);
+            sc(rt, i, -1); sc(rt, p1(i), enc(u, 0)); sc(rt, m1(i), enc(t, 2));
+            sc(ru, jj, -1); sc(ru, p1(jj), enc(t, 0)); sc(ru, m1(jj), enc(u, 2));
+            m.remap[t] = rt; m.remap[u] = ru;
+            m.stamp[t] = op; m.stamp[u] = op;
+            m.vt[x] = t; m.vt[cc] = t; m.vt[y] = u; m.vt[d] = u;
+            flips++;
+        }
+        int slot = block_reserve(&m.counters[cnt_out_slot], __popc(outs));
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if ((outs >> (2 * j)) & 1) list_out[slot++] = tq[j];
+            if ((outs >> (2 * j + 1)) & 1) list_out[slot++] = uq[j];
+        }
     }
     // C_FLIPS only tells the host whether the pass flipped anything: a plain store
     int wf = flips;
@@ -1107,6 +1126,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_flip_tail(MeshDev m, int *lA, 
         __syncthreads();   // every thread has read n before the counters are reset
         if (threadIdx.x == 0) {
             atomicAdd(&m.counters[C_FLIP_LISTED], n);
+            atomicAdd(&m.counters[C_TAIL_LISTED], n);
             m.counters[sout] = 0;
             m.counters[C_CAND] = 0;
             m.counters[C_FLIPS] = 0;

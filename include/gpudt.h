@@ -192,6 +192,10 @@ typedef struct {
        removal and the final flips; 2: constraints and canonical output): triangles listed for
        flip tests, incircle tests, exact incircle and exact orientation evaluations */
     long long stageListed[3], stageIncircleTests[3], stageIncircleExact[3], stageOrientExact[3];
+    /* flip passes run back to back by the single-workgroup tail on short lists, and the triangles
+       they listed (flipPasses and flipListed include both) */
+    int flipTailPasses;
+    long long flipTailListed;
 } galpha_stats;
 
 galpha_ctx *galphaCreate(int device);
