@@ -151,6 +151,7 @@ struct AlphaK {
     int *rank;           // [S0]
     unsigned long long *specbits;  // [S0 + 1]
     int *vnumin, *vnumax;  // [n]
+    int2 *vnum;            // [n] (vnumin, vnumax) side by side while k_edge_intervals builds them
     int *status;
     int *counters;       // [16]
     int *bigstart;             // large clusters (start position)
@@ -856,18 +857,28 @@ __global__ void k_edge_intervals(AlphaK a, int E)
         int u = tv(a, t, pl1(i)), w = tv(a, t, mi1(i));
         // vnumin only decreases and vnumax only increases, so a plain read that already shows
         // a value at or beyond this edge's (stale reads are larger / smaller) makes the atomic
-        // a no-op and it is skipped
-        if (a.vnumin[u] > key) atomicMin(&a.vnumin[u], key);
-        if (a.vnumin[w] > key) atomicMin(&a.vnumin[w], key);
-        if (a.vnumax[u] < mx) atomicMax(&a.vnumax[u], mx);
-        if (a.vnumax[w] < mx) atomicMax(&a.vnumax[w], mx);
+        // a no-op and it is skipped; both extremes of a vertex are one 8-byte read
+        const int2 vu = a.vnum[u], vw = a.vnum[w];
+        if (vu.x > key) atomicMin(&a.vnum[u].x, key);
+        if (vw.x > key) atomicMin(&a.vnum[w].x, key);
+        if (vu.y < mx) atomicMax(&a.vnum[u].y, mx);
+        if (vw.y < mx) atomicMax(&a.vnum[w].y, mx);
     }
 }
 
+__global__ void k_vnum_init(int2 *vnum, int n)
+{
+    GS(v, n) vnum[v] = make_int2(0x7fffffff, -1);
+}
+
+// vertex intervals from their side-by-side working copy; a vertex on no edge gets -1, -1
 __global__ void k_vert_fix(AlphaK a)
 {
     GS(v, a.n) {
-        if (a.vnumin[v] == 0x7fffffff) { a.vnumin[v] = -1; a.vnumax[v] = -1; }
+        int2 w = a.vnum[v];
+        if (w.x == 0x7fffffff) w = make_int2(-1, -1);
+        a.vnumin[v] = w.x;
+        a.vnumax[v] = w.y;
     }
 }
 
@@ -970,7 +981,7 @@ enum {
     AB_ID2, AB_LO, AB_HI, AB_SLO, AB_SHI, AB_FLAG, AB_RANK, AB_SPECBITS, AB_VMIN, AB_VMAX, AB_STATUS, AB_CNTRS,
     AB_TMP, AB_SCAN, AB_NEG, AB_PMAX, AB_NSMIN, AB_MKEY, AB_MVAL, AB_MFLAG, AB_MPOS, AB_OKEY, AB_OVAL,
     AB_SKEY, AB_SVAL, AB_SPEC, AB_SP, AB_MISC, AB_BIG0, AB_BIG1, AB_BIG2, AB_BIG3, AB_BIGL,
-    AB_BS0, AB_BS1, AB_BS2, AB_BS3, AB_BS4, AB_BSET, AB_MSET, AB_VHULL, AB_RANKQ, AB_RD, AB_XATT, AB_XCLU, AB_COUNT
+    AB_BS0, AB_BS1, AB_BS2, AB_BS3, AB_BS4, AB_BSET, AB_MSET, AB_VHULL, AB_RANKQ, AB_RD, AB_XATT, AB_XCLU, AB_VNUM, AB_COUNT
 };
 
 static void ab(AlphaDev &d, int i, size_t bytes)
@@ -1196,6 +1207,8 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     a.nsets = nsets;
     a.specstart = nullptr;
     a.vnumin = AP<int>(d, AB_VMIN); a.vnumax = AP<int>(d, AB_VMAX);
+    ab(d, AB_VNUM, 8 * (size_t)n + 64);
+    a.vnum = AP<int2>(d, AB_VNUM);
     GA_LAUNCH(k_edges, dim3(AGRID), dim3(256), 0, st, a);
     GA_LAUNCH(k_tri_rho, dim3(AGRID), dim3(256), 0, st, a);
     ab(d, AB_XATT, 4 * (size_t)Emax + 64);
@@ -1282,11 +1295,9 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     launch_exclusive_scan(a.gflag, a.rank, S0, d.b[AB_SCAN].p, st);
     int R = -1, nslots = -1;
     if (nsets == 1) {
-        // spectrum slots (minimum by atomics) and the vertex intervals' extremes
+        // spectrum slots (minimum by atomics)
         IntFills f;
         f.add(a.specbits, 2 * (size_t)(S0 + 2), -1);
-        f.add(a.vnumin, (size_t)n, 0x7fffffff);
-        f.add(a.vnumax, (size_t)n, -1);
         launch_fill_multi(f, st);
     } else {
         int h4[2];
@@ -1336,8 +1347,6 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
         a.specstart = sstart;
         IntFills f;
         f.add(a.specbits, 2 * (size_t)nslots, -1);
-        f.add(a.vnumin, (size_t)n, 0x7fffffff);
-        f.add(a.vnumax, (size_t)n, -1);
         launch_fill_multi(f, st);
         GA_LAUNCH(k_ranks_batch, dim3(AGRID), dim3(256), 0, st, a, S0, (const unsigned *)sset2, (const int *)order2,
                   (const int *)incl2, (const int *)first);
@@ -1347,7 +1356,8 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     GA_LAUNCH(k_spec_values, dim3(AGRID), dim3(256), 0, st, (const unsigned long long *)a.specbits, nslots,
               (const int *)(incl + S0 - 1), AP<double>(d, AB_SPEC));
     chk(hipEventRecord(d.ev[2], st), "event");
-    // intervals (vnumin / vnumax were set with the spectrum slots)
+    // intervals
+    GA_LAUNCH(k_vnum_init, dim3(AGRID), dim3(256), 0, st, a.vnum, n);
     GA_LAUNCH(k_edge_intervals, dim3(AGRID), dim3(256), 0, st, a, E);
     GA_LAUNCH(k_vert_fix, dim3(AGRID), dim3(256), 0, st, a);
     // master list
