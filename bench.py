@@ -7,9 +7,9 @@ Voronoi diagram, reconstruction, shifting, missing-site insertion, boundary remo
 canonical output, then rho, attachment, exact spectrum ranks, intervals and the master list.
 Inputs are resident in HBM when the timed region starts; outputs stay on the device.
 
-Prints one JSON line.  `roofline` covers the dominant kernel (timed with HIP events on the stream
-it runs on, through the library's kernel probe); `cpu_baseline` is the sequential CPU oracle on
-the same set, one core.
+Prints one JSON line.  `roofline` covers the dominant hand-written kernel (the largest share of
+GPU time after rocPRIM's sort passes, timed with HIP events on the stream it runs on, through the
+library's kernel probe); `cpu_baseline` is the sequential CPU oracle on the same set, one core.
 """
 import argparse
 import json
@@ -29,22 +29,27 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic (compulsory) bytes of one run of a kernel, from the work counters of that run.
 ALGO_BYTES = {
+    "k_edge_intervals": {
+        "model": "8N + 24E + 4F: the interval half of the intervals + master-list stage (vertex "
+                 "interval pair 8 B per point; rank, lower and upper interval ends 24 B per edge read "
+                 "or written once; triangle rank 4 B)",
+        "bytes": lambda st, T, n: 8.0 * n + 24.0 * st["nEdges"] + 4.0 * st["nTris"]},
     "k_frame_remove": {
         "model": "12 B per frame vertex examined (state 4 + priority 8) + 88 B per fan triangle of a "
                  "removed frame vertex (triangle record 32 read, chain position 16 read, fill or dead "
                  "slot 32 written, anchor and back-link 8 written)",
-        "bytes": lambda st, T: 12.0 * st["frameSeen"] + 88.0 * st["frameChain"]},
+        "bytes": lambda st, T, n: 12.0 * st["frameSeen"] + 88.0 * st["frameChain"]},
     "k_flip_test": {
         "model": "32 B per listed triangle: the flipping stage's 24F + 16V per pass over F triangles "
                  "and V = F / 2 vertices (24 B vertex + neighbour record, 16 B position); the "
                  "triangles listed by the grid passes (the single-workgroup tail's are not k_flip_test's)",
-        "bytes": lambda st, T: 32.0 * (st["flipListed"] - st["flipTailListed"]),
+        "bytes": lambda st, T, n: 32.0 * (st["flipListed"] - st["flipTailListed"]),
         "gather_model": "156 B per listed triangle (list entry 4, vertices + links 32, three neighbour "
                         "stamps and far vertices 24, six positions 96)",
-        "gather_bytes": lambda st, T: 156.0 * (st["flipListed"] - st["flipTailListed"])},
+        "gather_bytes": lambda st, T, n: 156.0 * (st["flipListed"] - st["flipTailListed"])},
     "k_pba_color": {
         "model": "8 B per pixel (envelope read 4 + label written 4)",
-        "bytes": lambda st, T: 8.0 * T * T * st["pbaIterations"]},
+        "bytes": lambda st, T, n: 8.0 * T * T * st["pbaIterations"]},
 }
 
 
@@ -390,8 +395,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-host-to-host", action="store_true",
                     help="skip the host-to-host runs (profiled commands count only the timed workload)")
-    ap.add_argument("--probe", default="k_flip_test", choices=sorted(ALGO_BYTES),
-                    help="kernel timed for the roofline figure")
+    ap.add_argument("--probe", default="k_edge_intervals", choices=sorted(ALGO_BYTES),
+                    help="kernel timed for the roofline figure (default: the hand-written kernel with "
+                         "the largest share of GPU time in profiles/c2_kernel_stats.txt)")
     ap.add_argument("--batched-sets", type=int, default=4096, help="sets of the batched workload (0: skip)")
     ap.add_argument("--plain-sample", type=int, default=256, help="sets timed for the plain per-set loop")
     ap.add_argument("--serve-qps", default="50,150", help="offered request rates of the serving runs ('' skips)")
@@ -447,8 +453,9 @@ def main():
     value = n * args.steps / elapsed / 1e6
     st = ctx.stats()
 
-    # roofline of the dominant kernel (largest share of GPU time in the rocprof summary of this
-    # workload): algorithmic bytes of the probed run / total time of its launches in that run
+    # roofline of the dominant hand-written kernel (largest share of GPU time after rocPRIM's
+    # sort passes in the rocprof summary of this workload): algorithmic bytes of the probed run /
+    # total time of its launches in that run
     probe = ctx.kernel_probe(args.probe, lambda: step())
     pst = ctx.stats()
     gpu_alpha = ctx.alpha()
@@ -457,7 +464,7 @@ def main():
     roof = None
     if probe and probe["launches"] > 0:
         model = ALGO_BYTES[args.probe]
-        total_bytes = model["bytes"](pst, T)
+        total_bytes = model["bytes"](pst, T, n)
         per_launch = total_bytes / probe["launches"]
         avg_ms = probe["total_ms"] / probe["launches"]
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
@@ -469,7 +476,7 @@ def main():
                 "traffic_source": "profiles/c2_hbm_traffic.txt (FETCH_SIZE and WRITE_SIZE passes, scaled by "
                                   "known-size streaming kernels)"}
         if "gather_bytes" in model:
-            gb = model["gather_bytes"](pst, T) / probe["launches"]
+            gb = model["gather_bytes"](pst, T, n) / probe["launches"]
             roof["gather_model"] = model["gather_model"]
             roof["gather_model_achieved"] = round(gb / (avg_ms * 1e-3) / 1e9, 2)
 
