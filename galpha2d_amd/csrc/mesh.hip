@@ -254,32 +254,121 @@ void launch_recon(const int *vor, const int *pix, const BatchDev &b, MeshDev &m,
 }
 
 // ---- adjacency from scratch ----------------------------------------------------------------
-__global__ void k_deg(const int4 *__restrict__ tv, const int *__restrict__ ntri, int *__restrict__ deg)
+// Degrees and fans with the per-vertex atomics combined in LDS first.  A block takes 1024
+// consecutive triangle slots; slots and vertices are both in spatial (Morton) order, so their
+// vertices fall in a short id range.  Counts for ids in [vmin, vmin + ADJ_WIN) go to an LDS
+// window and reach HBM as one atomic per distinct vertex and block; other ids take the global
+// atomic directly.  (Device-scope atomics are performed beyond the per-XCD L2, about 11 ns each
+// on one address.)
+#define ADJ_WIN 8192
+#define ADJ_CHUNK 1024
+
+// smallest live vertex id of slots [c0, c0 + ADJ_CHUNK) (block-uniform; every thread calls it)
+GA_INLINE int adj_chunk_min(const int4 *__restrict__ tv, int c0, int n, int *s_red)
 {
-    int n = *ntri;
-    GRID_STRIDE(t, n) {
+    int mn = 0x7fffffff;
+    for (int j = 0; j < ADJ_CHUNK / 256; j++) {
+        int t = c0 + j * 256 + (int)threadIdx.x;
+        if (t >= n) break;
         int4 v = tv[t];
         if (v.x == GA_DEAD) continue;
-        atomicAdd(&deg[v.x], 1);
-        atomicAdd(&deg[v.y], 1);
-        atomicAdd(&deg[v.z], 1);
+        mn = min(mn, min(v.x, min(v.y, v.z)));
+    }
+    for (int d = 32; d > 0; d >>= 1) mn = min(mn, __shfl_xor(mn, d, 64));
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = mn;
+    __syncthreads();
+    mn = min(min(s_red[0], s_red[1]), min(s_red[2], s_red[3]));
+    return mn;
+}
+
+__global__ __launch_bounds__(256) void k_deg(const int4 *__restrict__ tv, const int *__restrict__ ntri,
+                                             int *__restrict__ deg)
+{
+    __shared__ int cnt[ADJ_WIN];
+    __shared__ int s_red[4];
+    const int n = *ntri;
+    for (int c0 = xcd_block() * ADJ_CHUNK; c0 < n; c0 += gridDim.x * ADJ_CHUNK) {
+        const int base = adj_chunk_min(tv, c0, n, s_red);
+        for (int i = threadIdx.x; i < ADJ_WIN; i += 256) cnt[i] = 0;
+        __syncthreads();
+        for (int j = 0; j < ADJ_CHUNK / 256; j++) {
+            int t = c0 + j * 256 + (int)threadIdx.x;
+            if (t >= n) break;
+            int4 v = tv[t];
+            if (v.x == GA_DEAD) continue;
+            for (int i = 0; i < 3; i++) {
+                int a = ic(v, i);
+                unsigned r = (unsigned)(a - base);
+                if (r < ADJ_WIN) atomicAdd(&cnt[r], 1);
+                else atomicAdd(&deg[a], 1);
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < ADJ_WIN; i += 256) {
+            int c = cnt[i];
+            if (c) atomicAdd(&deg[base + i], c);
+        }
+        __syncthreads();
     }
 }
 
-// fan entry of corner i of triangle t: (enc(t, i), the vertex that follows it counter-clockwise)
-__global__ void k_fanfill(const int4 *__restrict__ tv, const int *__restrict__ ntri, const int *__restrict__ off,
-                          int *__restrict__ fill, int2 *__restrict__ fan, int *__restrict__ vt)
+// fan entry of corner i of triangle t: (enc(t, i), the vertex that follows it counter-clockwise);
+// a corner's place in its vertex's fan is its block's base for the vertex (one global atomic per
+// distinct vertex) plus its rank among the block's corners of that vertex (an LDS atomic)
+__global__ __launch_bounds__(256) void k_fanfill(const int4 *__restrict__ tv, const int *__restrict__ ntri,
+                                                 const int *__restrict__ off, int *__restrict__ fill,
+                                                 int2 *__restrict__ fan, int *__restrict__ vt)
 {
-    int n = *ntri;
-    GRID_STRIDE(t, n) {
-        int4 v = tv[t];
-        if (v.x == GA_DEAD) continue;
-        for (int i = 0; i < 3; i++) {
-            int a = ic(v, i);
-            int k = atomicAdd(&fill[a], 1);
-            fan[off[a] + k] = make_int2(enc(t, i), ic(v, p1(i)));
-            if (k == 0) vt[a] = t;
+    __shared__ int cnt[ADJ_WIN];
+    __shared__ int s_red[4];
+    const int n = *ntri;
+    for (int c0 = xcd_block() * ADJ_CHUNK; c0 < n; c0 += gridDim.x * ADJ_CHUNK) {
+        const int base = adj_chunk_min(tv, c0, n, s_red);
+        for (int i = threadIdx.x; i < ADJ_WIN; i += 256) cnt[i] = 0;
+        __syncthreads();
+        int lr[ADJ_CHUNK / 256][3];
+#pragma unroll
+        for (int j = 0; j < ADJ_CHUNK / 256; j++) {
+            lr[j][0] = lr[j][1] = lr[j][2] = -1;
+            int t = c0 + j * 256 + (int)threadIdx.x;
+            if (t >= n) continue;
+            int4 v = tv[t];
+            if (v.x == GA_DEAD) continue;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                int a = ic(v, i);
+                unsigned r = (unsigned)(a - base);
+                if (r < ADJ_WIN) {
+                    lr[j][i] = atomicAdd(&cnt[r], 1);
+                } else {
+                    int k = atomicAdd(&fill[a], 1);
+                    fan[off[a] + k] = make_int2(enc(t, i), ic(v, p1(i)));
+                    if (k == 0) vt[a] = t;
+                }
+            }
         }
+        __syncthreads();
+        for (int i = threadIdx.x; i < ADJ_WIN; i += 256) {
+            int c = cnt[i];
+            if (c) cnt[i] = atomicAdd(&fill[base + i], c);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < ADJ_CHUNK / 256; j++) {
+            int t = c0 + j * 256 + (int)threadIdx.x;
+            if (t >= n) continue;
+            if (lr[j][0] < 0 && lr[j][1] < 0 && lr[j][2] < 0) continue;
+            int4 v = tv[t];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                if (lr[j][i] < 0) continue;
+                int a = ic(v, i);
+                int k = cnt[a - base] + lr[j][i];
+                fan[off[a] + k] = make_int2(enc(t, i), ic(v, p1(i)));
+                if (k == 0) vt[a] = t;
+            }
+        }
+        __syncthreads();
     }
 }
 
