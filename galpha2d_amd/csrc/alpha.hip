@@ -445,27 +445,59 @@ GA_INLINE int bi_bitlen(const BigInt<NL> &b)
     if (b.n == 0) return 0;
     return 32 * (b.n - 1) + (32 - __clz(b.l[b.n - 1]));
 }
-template <int NL>
-GA_INLINE int bi_bit(const BigInt<NL> &b, int i)
+// q = floor(u / v) for u of m limbs and v of n limbs (1 <= n <= m <= DIV_MAXU, v[n - 1] != 0); q gets
+// m - n + 1 limbs.  Knuth's algorithm D with 32-bit digits: normalise v's top bit, estimate each
+// quotient digit from the top two remainder digits, correct it at most twice, multiply-subtract,
+// and add back on the rare overdraft.
+#define DIV_MAXU 52
+GA_INLINE void bi_divq(const uint32_t *u, int m, const uint32_t *v, int n, uint32_t *q)
 {
-    int li = i >> 5;
-    return li < b.n ? (int)((b.l[li] >> (i & 31)) & 1u) : 0;
-}
-// r = 2 r + bit
-template <int NL>
-GA_INLINE void bi_shl1_add(BigInt<NL> &r, int bit, int *ovf)
-{
-    uint32_t carry = (uint32_t)bit;
-    for (int i = 0; i < r.n; i++) {
-        uint32_t v = r.l[i];
-        r.l[i] = (v << 1) | carry;
-        carry = v >> 31;
+    if (n == 1) {
+        uint64_t r = 0;
+        for (int j = m - 1; j >= 0; j--) {
+            uint64_t cur = (r << 32) | u[j];
+            q[j] = (uint32_t)(cur / v[0]);
+            r = cur - (uint64_t)q[j] * v[0];
+        }
+        return;
     }
-    if (carry) {
-        if (r.n < NL) r.l[r.n++] = carry;
-        else *ovf = 1;
+    const uint64_t b = 1ull << 32;
+    uint32_t un[DIV_MAXU + 1], vn[ANL];
+    const int s = __clz(v[n - 1]);
+    for (int i = n - 1; i > 0; i--) vn[i] = (uint32_t)(((uint64_t)v[i] << s) | ((uint64_t)v[i - 1] >> (32 - s)));
+    vn[0] = v[0] << s;
+    un[m] = (uint32_t)((uint64_t)u[m - 1] >> (32 - s));
+    for (int i = m - 1; i > 0; i--) un[i] = (uint32_t)(((uint64_t)u[i] << s) | ((uint64_t)u[i - 1] >> (32 - s)));
+    un[0] = u[0] << s;
+    for (int j = m - n; j >= 0; j--) {
+        const uint64_t top = ((uint64_t)un[j + n] << 32) | un[j + n - 1];
+        uint64_t qhat = top / vn[n - 1], rhat = top - qhat * vn[n - 1];
+        while (qhat >= b || qhat * vn[n - 2] > ((rhat << 32) | un[j + n - 2])) {
+            qhat--;
+            rhat += vn[n - 1];
+            if (rhat >= b) break;
+        }
+        int64_t t, kk = 0;
+        for (int i = 0; i < n; i++) {
+            const uint64_t pr = qhat * vn[i];
+            t = (int64_t)un[i + j] - kk - (int64_t)(pr & 0xffffffffull);
+            un[i + j] = (uint32_t)t;
+            kk = (int64_t)(pr >> 32) - (t >> 32);
+        }
+        t = (int64_t)un[j + n] - kk;
+        un[j + n] = (uint32_t)t;
+        q[j] = (uint32_t)qhat;
+        if (t < 0) {
+            q[j]--;
+            uint64_t c = 0;
+            for (int i = 0; i < n; i++) {
+                const uint64_t s2 = (uint64_t)un[i + j] + vn[i] + c;
+                un[i + j] = (uint32_t)s2;
+                c = s2 >> 32;
+            }
+            un[j + n] = (uint32_t)((uint64_t)un[j + n] + c);
+        }
     }
-    if (r.n == 0 && bit) { r.l[0] = 1; r.n = 1; }
 }
 
 __device__ __noinline__ void rho_key_exact(int st, const double2 *p, unsigned long long *q4, int *E, int *ovf)
@@ -494,24 +526,34 @@ __device__ __noinline__ void rho_key_exact(int st, const double2 *p, unsigned lo
         bi_sub(t1, t1, t2, ovf);
         bi_mul(den, t1, t1, ovf);
     }
-    // binary long division of N * 2^k by D, 257 quotient bits at most
+    // long division of N * 2^k by D in 32-bit digits (Knuth's algorithm D); the quotient has 256
+    // or 257 bits
     int Ln = bi_bitlen(num), Ld = bi_bitlen(den);
     int k = 256 - Ln + Ld;
-    if (k < 0) { *ovf = 1; k = 0; }
     unsigned long long q[5] = {0, 0, 0, 0, 0};
-    BI r;
-    bi_zero(r);
-    int total = Ln + k;
-    for (int i = total - 1; i >= 0; i--) {
-        int bit = i >= k ? bi_bit(num, i - k) : 0;
-        bi_shl1_add(r, bit, ovf);
-        for (int w = 4; w > 0; w--) q[w] = (q[w] << 1) | (q[w - 1] >> 63);
-        q[0] <<= 1;
-        if (bi_cmpmag(r, den) >= 0) {
-            bi_submag(r, r, den);
-            r.neg = 0;
-            q[0] |= 1ull;
-        }
+    const int ls = k >> 5, bs = k & 31;
+    int m = num.n + ls + 1;
+    if (k < 0 || den.n == 0 || m > DIV_MAXU) {
+        *ovf = 1;
+        for (int w = 0; w < 4; w++) q4[w] = 0;
+        *E = 0;
+        return;
+    }
+    uint32_t u[DIV_MAXU], qd[DIV_MAXU];
+    for (int i = 0; i < ls; i++) u[i] = 0;
+    uint32_t carry = 0;
+    for (int i = 0; i < num.n; i++) {
+        uint64_t w = ((uint64_t)num.l[i] << bs) | carry;
+        u[ls + i] = (uint32_t)w;
+        carry = (uint32_t)(w >> 32);
+    }
+    u[ls + num.n] = carry;
+    while (m > 1 && u[m - 1] == 0) m--;
+    bi_divq(u, m, den.l, den.n, qd);
+    const int qn = m - den.n + 1;
+    for (int i = 0; i < qn; i++) {
+        if (i < 10) q[i >> 1] |= (unsigned long long)qd[i] << (32 * (i & 1));
+        else if (qd[i]) *ovf = 1;
     }
     // normalise to exactly 256 bits (top bit 255)
     if (q[4] & 1ull) {
