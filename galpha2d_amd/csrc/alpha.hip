@@ -174,6 +174,7 @@ struct AlphaK {
 #define A_BADMESH 6      // caller triangles that fail k_mesh_check
 #define A_EXCLU 7        // small clusters left to the exact kernel (list in xclu)
 #define A_ATT_LIST 8     // edges whose attachment the filter leaves open (list in xatt)
+#define A_MK 9           // master-list entries with a rank (the sorted list's length)
 
 GA_INLINE int tv(const AlphaK &a, int t, int i) { return a.tris[(size_t)t * 9 + 3 + i]; }
 GA_INLINE int tn(const AlphaK &a, int t, int i) { return a.tris[(size_t)t * 9 + i]; }
@@ -924,9 +925,11 @@ __global__ void k_vert_fix(AlphaK a)
     }
 }
 
-// master keys: q < 2n vertices, < 2n + 3E edges, then triangles
-__global__ void k_master_keys(AlphaK a, int E, long long M, unsigned int *mkey, int *mval, int *mflag)
+// master keys: q < 2n vertices, < 2n + 3E edges, then triangles; entries without a rank get the
+// key `none`, above every rank, so they sort to the end; counters[A_MK] counts the others
+__global__ void k_master_keys(AlphaK a, int E, long long M, unsigned int *mkey, int *mval, unsigned int none)
 {
+    int valid = 0;
     for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < M; q += (long long)gridDim.x * blockDim.x) {
         int r;
         long long n2 = 2LL * a.n, e3 = 3LL * E;
@@ -944,27 +947,20 @@ __global__ void k_master_keys(AlphaK a, int E, long long M, unsigned int *mkey, 
             else set = tri_set(a, (int)(q - n2 - e3));
             base = a.specstart[set];
         }
-        mflag[q] = r >= 1;
-        mkey[q] = r >= 1 ? (unsigned)(base + r) : 0u;
+        valid += r >= 1;
+        mkey[q] = r >= 1 ? (unsigned)(base + r) : none;
         mval[q] = (int)q;
     }
+    wave_count_full(&a.counters[A_MK], valid);
 }
 
-__global__ void k_master_compact(long long M, const unsigned int *mkey, const int *mval, const int *mflag,
-                                 const int *mpos, unsigned int *okey, int *oval)
+// the first *pMk sorted entries (the ranked ones); pflag is 0 for the rest
+__global__ void k_master_decode(int n, int E, int M, const int *pMk, const unsigned int *skey, const int *sval,
+                                int *simplex, int *st, int *at, int *pflag)
 {
-    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < M; q += (long long)gridDim.x * blockDim.x) {
-        if (!mflag[q]) continue;
-        int d = mpos[q];
-        okey[d] = mkey[q];
-        oval[d] = mval[q];
-    }
-}
-
-__global__ void k_master_decode(int n, int E, int Mk, const unsigned int *skey, const int *sval, int *simplex, int *st,
-                                int *at, int *pflag)
-{
-    GS(k, Mk) {
+    const int Mk = *pMk;
+    GS(k, M) {
+        if (k >= Mk) { pflag[k] = 0; continue; }
         long long q = sval[k];
         long long n2 = 2LL * n, e3 = 3LL * E;
         if (q < n2) { simplex[k] = (int)(q >> 1); st[k] = 2; at[k] = (int)(q & 1) + 1; }
@@ -994,10 +990,12 @@ __global__ void k_spec_values(const unsigned long long *bits, int nslots, const 
 // first master entry of every set: keys are sorted and the keys of set s lie in
 // (specstart[s], specstart[s + 1]), so it is the first position with key > specstart[s]
 // (Mk when the set has no master entry); one binary search per set
-__global__ void k_master_set_first(int Mk, const unsigned int *skey, const int *specstart, int nsets, int *mfirst)
+__global__ void k_master_set_first(const int *pMk, const unsigned int *skey, const int *specstart, int nsets,
+                                   int *mfirst)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsets) return;
+    const int Mk = *pMk;
     unsigned int key = (unsigned)specstart[s];
     int lo = 0, hi = Mk;
     while (lo < hi) {
@@ -1410,66 +1408,68 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     ab(d, AB_SCAN, scan_temp_bytes((size_t)M) + 4096);
     unsigned *mkey = AP<unsigned>(d, AB_MKEY);
     int *mval = AP<int>(d, AB_MVAL), *mflag = AP<int>(d, AB_MFLAG), *mpos = AP<int>(d, AB_MPOS);
-    GA_LAUNCH(k_master_keys, dim3(AGRID), dim3(256), 0, st, a, E, M, mkey, mval, mflag);
-    launch_exclusive_scan(mflag, mpos, (size_t)M, d.b[AB_SCAN].p, st);
-    int h5[2];
-    read_ints(d, st, {mpos + M - 1, mflag + M - 1}, h5);
-    int Mk = h5[0] + h5[1];
-    d.M = Mk;
-    ab(d, AB_OKEY, 4 * (size_t)Mk + 16); ab(d, AB_OVAL, 4 * (size_t)Mk + 16);
-    ab(d, AB_SKEY, 4 * (size_t)Mk + 16); ab(d, AB_SVAL, 4 * (size_t)Mk + 16);
-    unsigned *okey = AP<unsigned>(d, AB_OKEY), *skey = AP<unsigned>(d, AB_SKEY);
-    int *oval = AP<int>(d, AB_OVAL), *sval = AP<int>(d, AB_SVAL);
-    GA_LAUNCH(k_master_compact, dim3(AGRID), dim3(256), 0, st, M, (const unsigned *)mkey, (const int *)mval,
-                       (const int *)mflag, (const int *)mpos, okey, oval);
+    // every entry is sorted, the unranked ones with a key above every rank (no compaction and no
+    // host read of the ranked count before the sort)
     int bits = 1;
     while ((1LL << bits) <= (long long)slot_bound) bits++;
+    const unsigned none = (unsigned)((1LL << bits) - 1);
+    GA_LAUNCH(k_master_keys, dim3(AGRID), dim3(256), 0, st, a, E, M, mkey, mval, none);
+    ab(d, AB_OKEY, 4 * (size_t)M + 16); ab(d, AB_OVAL, 4 * (size_t)M + 16);
+    ab(d, AB_SKEY, 4 * (size_t)M + 16); ab(d, AB_SVAL, 4 * (size_t)M + 16);
+    unsigned *skey = AP<unsigned>(d, AB_SKEY);
+    int *sval = AP<int>(d, AB_SVAL);
     tb = 0;
-    chk(rocprim::radix_sort_pairs(nullptr, tb, okey, skey, oval, sval, (unsigned)Mk, 0u, (unsigned)bits, st), "sort");
+    chk(rocprim::radix_sort_pairs(nullptr, tb, mkey, skey, mval, sval, (unsigned)M, 0u, (unsigned)bits, st), "sort");
     ab(d, AB_TMP, tb + 256);
     tb = d.b[AB_TMP].bytes;
-    chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, okey, skey, oval, sval, (unsigned)Mk, 0u, (unsigned)bits, st), "sort");
-    // decode into mval/mflag/mpos reused: simplex -> okey (as int), st -> oval, at -> mkey ; pflag -> mflag
-    int *msimp = (int *)okey, *mst = oval, *mat = (int *)mkey;
-    GA_LAUNCH(k_master_decode, dim3(AGRID), dim3(256), 0, st, n, E, Mk, (const unsigned *)skey, (const int *)sval,
-                       msimp, mst, mat, mflag);
-    launch_exclusive_scan(mflag, mpos, (size_t)Mk, d.b[AB_SCAN].p, st);
+    chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, mkey, skey, mval, sval, (unsigned)M, 0u, (unsigned)bits, st),
+        "sort");
+    // decode: simplex -> AB_OKEY, st -> AB_OVAL, at -> AB_MKEY (free after the sort); pflag -> mflag
+    int *msimp = AP<int>(d, AB_OKEY), *mst = AP<int>(d, AB_OVAL), *mat = (int *)mkey;
+    const int *pMk = a.counters + A_MK;
+    GA_LAUNCH(k_master_decode, dim3(AGRID), dim3(256), 0, st, n, E, (int)M, pMk, (const unsigned *)skey,
+              (const int *)sval, msimp, mst, mat, mflag);
+    launch_exclusive_scan(mflag, mpos, (size_t)M, d.b[AB_SCAN].p, st);
     // spec_pointers: one entry per distinct key, and every rank >= 1 of a set holds at least the
     // entry of its own value, so there are slot_bound - nsets of them (read with the final
     // counters for a single set)
     ab(d, AB_SP, 4 * (size_t)(slot_bound + 2));
-    GA_LAUNCH(k_spec_pointers, dim3(AGRID), dim3(256), 0, st, Mk, (const int *)mflag, (const int *)mpos,
-                       AP<int>(d, AB_SP));
+    GA_LAUNCH(k_spec_pointers, dim3(AGRID), dim3(256), 0, st, (int)M, (const int *)mflag, (const int *)mpos,
+              AP<int>(d, AB_SP));
+    int Mk = -1;   // read below
     if (nsets > 1) {
         // first master entry and first edge of every set
         ab(d, AB_MSET, 4 * 3 * (size_t)(nsets + 1));
         int *mf = AP<int>(d, AB_MSET), *tf = mf + (nsets + 1), *ef = tf + (nsets + 1);
         chk(hipMemcpyAsync(tf, d.tfirst.data(), 4 * (size_t)nsets, hipMemcpyHostToDevice, st), "copy");
-        GA_LAUNCH(k_master_set_first, dim3((nsets + 255) / 256), dim3(256), 0, st, Mk, (const unsigned *)skey,
+        GA_LAUNCH(k_master_set_first, dim3((nsets + 255) / 256), dim3(256), 0, st, pMk, (const unsigned *)skey,
                   a.specstart, nsets, mf);
         GA_LAUNCH(k_edge_set_first, dim3((nsets + 255) / 256), dim3(256), 0, st, (const int *)tf, nsets, F,
                   (const int *)a.eoff, E, ef);
         std::vector<int> hm(nsets), he(nsets);
         chk(hipMemcpyAsync(hm.data(), mf, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st), "copy");
         chk(hipMemcpyAsync(he.data(), ef, 4 * (size_t)nsets, hipMemcpyDeviceToHost, st), "copy");
+        chk(hipMemcpyAsync(&Mk, pMk, 4, hipMemcpyDeviceToHost, st), "copy");
         chk(hipStreamSynchronize(st), "sync");
         d.mfirst[nsets] = Mk;
         for (int q = 0; q < nsets; q++) d.mfirst[q] = hm[q];
         for (int q = 0; q < nsets; q++) d.efirst[q] = he[q];
         d.efirst[nsets] = E;
     } else {
-        d.mfirst = {0, Mk};
         d.efirst = {0, E};
     }
     chk(hipEventRecord(d.ev[3], st), "event");
     // counters, status, the single set's rank count and the spec-pointer count in one read
-    int fin[12];
+    int fin[13];
     const int *zero = a.counters + A_BADMESH;   // 0 here: only caller meshes with errors set it
     read_ints(d, st,
               {a.counters + 0, a.counters + 1, a.counters + 2, a.counters + 3, a.counters + 4, a.counters + 5,
                a.counters + 6, a.counters + 7, a.status, nsets == 1 ? incl + S0 - 1 : zero,
-               Mk > 0 ? mpos + Mk - 1 : zero, Mk > 0 ? mflag + Mk - 1 : zero},
+               M > 0 ? mpos + M - 1 : zero, M > 0 ? mflag + M - 1 : zero, pMk},
               fin);
+    Mk = fin[12];
+    d.M = Mk;
+    if (nsets == 1) d.mfirst = {0, Mk};
     int *hc = fin, hs = fin[8], hr[3] = {fin[9], fin[10], fin[11]};
     if (nsets == 1) {
         R = hr[0];
