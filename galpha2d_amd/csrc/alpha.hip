@@ -925,55 +925,88 @@ __global__ void k_vert_fix(AlphaK a)
     }
 }
 
-// master keys: q < 2n vertices, < 2n + 3E edges, then triangles; entries without a rank get the
-// key `none`, above every rank, so they sort to the end; counters[A_MK] counts the others
-__global__ void k_master_keys(AlphaK a, int E, long long M, unsigned int *mkey, int *mval, unsigned int none)
+// master keys: q < 2n vertices (2v: vnumin, 2v + 1: vnumax), < 2n + 3E edges (2n + 3e + 0/1/2: rank,
+// numin, numax), then triangles; entries without a rank get the key `none`, above every rank, so
+// they sort to the end; counters[A_MK] counts the others.  One loop per simplex kind, so no index
+// is divided (the host keeps 2n + 3E + F below 2^31 - 2^20).
+GA_INLINE unsigned master_key(const AlphaK &a, int r, int set, unsigned none)
+{
+    if (r < 1) return none;
+    return (unsigned)((a.vset ? a.specstart[set] : 0) + r);
+}
+
+__global__ void k_master_keys(AlphaK a, int E, unsigned int *mkey, int *mval, unsigned int none)
 {
     int valid = 0;
-    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < M; q += (long long)gridDim.x * blockDim.x) {
-        int r;
-        long long n2 = 2LL * a.n, e3 = 3LL * E;
-        if (q < n2) r = (q & 1) ? a.vnumax[q >> 1] : a.vnumin[q >> 1];
-        else if (q < n2 + e3) {
-            long long w = q - n2;
-            int e = (int)(w / 3), at = (int)(w % 3);
-            r = at == 0 ? a.edge_rank[e] : (at == 1 ? a.edge_numin[e] : a.edge_numax[e]);
-        } else r = a.tri_rank[q - n2 - e3];
-        int base = 0;
-        if (a.vset && r >= 1) {
-            int set;
-            if (q < n2) set = a.vset[q >> 1];
-            else if (q < n2 + e3) set = tri_set(a, a.edge_t[(q - n2) / 3]);
-            else set = tri_set(a, (int)(q - n2 - e3));
-            base = a.specstart[set];
-        }
+    const int n2 = 2 * a.n, e0 = n2 + 3 * E;
+    GS(v, a.n) {
+        const int set = a.vset ? a.vset[v] : 0;
+        const int r0 = a.vnumin[v], r1 = a.vnumax[v];
+        valid += (r0 >= 1) + (r1 >= 1);
+        *(uint2 *)&mkey[2 * v] = make_uint2(master_key(a, r0, set, none), master_key(a, r1, set, none));
+        *(int2 *)&mval[2 * v] = make_int2(2 * v, 2 * v + 1);
+    }
+    GS(e, E) {
+        const int set = a.vset ? tri_set(a, a.edge_t[e]) : 0;
+        const int r0 = a.edge_rank[e], r1 = a.edge_numin[e], r2 = a.edge_numax[e];
+        valid += (r0 >= 1) + (r1 >= 1) + (r2 >= 1);
+        const int q = n2 + 3 * e;
+        mkey[q] = master_key(a, r0, set, none);
+        mkey[q + 1] = master_key(a, r1, set, none);
+        mkey[q + 2] = master_key(a, r2, set, none);
+        mval[q] = q;
+        mval[q + 1] = q + 1;
+        mval[q + 2] = q + 2;
+    }
+    GS(t, a.F) {
+        const int r = a.tri_rank[t];
         valid += r >= 1;
-        mkey[q] = r >= 1 ? (unsigned)(base + r) : none;
-        mval[q] = (int)q;
+        mkey[e0 + t] = master_key(a, r, tri_set(a, t), none);
+        mval[e0 + t] = e0 + t;
     }
     wave_count_full(&a.counters[A_MK], valid);
 }
 
-// the first *pMk sorted entries (the ranked ones); pflag is 0 for the rest
-__global__ void k_master_decode(int n, int E, int M, const int *pMk, const unsigned int *skey, const int *sval,
-                                int *simplex, int *st, int *at, int *pflag)
+// the first *pMk sorted entries (the ranked ones): simplex id, type (2 vertex, 0 edge, 1 triangle)
+// and attribute of each
+__global__ void k_master_decode(int n, int E, const int *pMk, const int *sval, int *simplex, int *st, int *at)
 {
     const int Mk = *pMk;
-    GS(k, M) {
-        if (k >= Mk) { pflag[k] = 0; continue; }
-        long long q = sval[k];
-        long long n2 = 2LL * n, e3 = 3LL * E;
-        if (q < n2) { simplex[k] = (int)(q >> 1); st[k] = 2; at[k] = (int)(q & 1) + 1; }
-        else if (q < n2 + e3) { long long w = q - n2; simplex[k] = (int)(w / 3); st[k] = 0; at[k] = (int)(w % 3); }
-        else { simplex[k] = (int)(q - n2 - e3); st[k] = 1; at[k] = 0; }
-        pflag[k] = (k == 0) || (skey[k] != skey[k - 1]);
+    const unsigned n2 = 2u * (unsigned)n, e3 = 3u * (unsigned)E;
+    GS(k, Mk) {
+        const unsigned q = (unsigned)sval[k];
+        int sp, ty, att;
+        if (q < n2) { sp = (int)(q >> 1); ty = 2; att = (int)(q & 1) + 1; }
+        else if (q < n2 + e3) { const unsigned w = q - n2, e = w / 3u; sp = (int)e; ty = 0; att = (int)(w - 3u * e); }
+        else { sp = (int)(q - n2 - e3); ty = 1; att = 0; }
+        simplex[k] = sp;
+        st[k] = ty;
+        at[k] = att;
     }
 }
 
-__global__ void k_spec_pointers(int Mk, const int *pflag, const int *ppos, int *sp)
+// sp[1 + i] = first master entry of the i-th distinct key.  Every rank r >= 1 of a set holds at
+// least the entry of its own value, so the keys of set s are exactly specstart[s] + 1 ..
+// specstart[s + 1] - 1 (single set: 1 .. R), and key K of set s is distinct key number
+// K - s - 1: no scan over the entries is needed.  specstart[0] = 0.
+__global__ void k_spec_pointers(const int *pMk, const unsigned int *skey, const int *specstart, int nsets, int *sp)
 {
+    const int Mk = *pMk;
     GS(k, Mk) {
-        if (pflag[k]) sp[1 + ppos[k]] = k;
+        const unsigned K = skey[k];
+        if (k > 0 && skey[k - 1] == K) continue;
+        int s = 0;
+        if (nsets > 1) {
+            // largest s with specstart[s] < K
+            int lo = 0, hi = nsets - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if ((unsigned)specstart[mid] < K) lo = mid;
+                else hi = mid - 1;
+            }
+            s = lo;
+        }
+        sp[(int)K - s] = k;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) sp[0] = -1;
 }
@@ -1402,18 +1435,17 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     GA_LAUNCH(k_vert_fix, dim3(AGRID), dim3(256), 0, st, a);
     // master list
     long long M = 2LL * n + 3LL * E + F;
-    if (M > 0x7fffffffLL) throw GalphaError("alpha: master list too large");
-    ab(d, AB_MKEY, 4 * (size_t)M); ab(d, AB_MVAL, 4 * (size_t)M); ab(d, AB_MFLAG, 4 * (size_t)M);
-    ab(d, AB_MPOS, 4 * (size_t)M);
-    ab(d, AB_SCAN, scan_temp_bytes((size_t)M) + 4096);
+    // (grid-stride loops over int indices: keep the last index plus the grid below 2^31)
+    if (M > 0x7fffffffLL - (1LL << 20)) throw GalphaError("alpha: master list too large");
+    ab(d, AB_MKEY, 4 * (size_t)M); ab(d, AB_MVAL, 4 * (size_t)M);
     unsigned *mkey = AP<unsigned>(d, AB_MKEY);
-    int *mval = AP<int>(d, AB_MVAL), *mflag = AP<int>(d, AB_MFLAG), *mpos = AP<int>(d, AB_MPOS);
+    int *mval = AP<int>(d, AB_MVAL);
     // every entry is sorted, the unranked ones with a key above every rank (no compaction and no
     // host read of the ranked count before the sort)
     int bits = 1;
     while ((1LL << bits) <= (long long)slot_bound) bits++;
     const unsigned none = (unsigned)((1LL << bits) - 1);
-    GA_LAUNCH(k_master_keys, dim3(AGRID), dim3(256), 0, st, a, E, M, mkey, mval, none);
+    GA_LAUNCH(k_master_keys, dim3(AGRID), dim3(256), 0, st, a, E, mkey, mval, none);
     ab(d, AB_OKEY, 4 * (size_t)M + 16); ab(d, AB_OVAL, 4 * (size_t)M + 16);
     ab(d, AB_SKEY, 4 * (size_t)M + 16); ab(d, AB_SVAL, 4 * (size_t)M + 16);
     unsigned *skey = AP<unsigned>(d, AB_SKEY);
@@ -1424,18 +1456,16 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     tb = d.b[AB_TMP].bytes;
     chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, mkey, skey, mval, sval, (unsigned)M, 0u, (unsigned)bits, st),
         "sort");
-    // decode: simplex -> AB_OKEY, st -> AB_OVAL, at -> AB_MKEY (free after the sort); pflag -> mflag
+    // decode: simplex -> AB_OKEY, st -> AB_OVAL, at -> AB_MKEY (free after the sort)
     int *msimp = AP<int>(d, AB_OKEY), *mst = AP<int>(d, AB_OVAL), *mat = (int *)mkey;
     const int *pMk = a.counters + A_MK;
-    GA_LAUNCH(k_master_decode, dim3(AGRID), dim3(256), 0, st, n, E, (int)M, pMk, (const unsigned *)skey,
-              (const int *)sval, msimp, mst, mat, mflag);
-    launch_exclusive_scan(mflag, mpos, (size_t)M, d.b[AB_SCAN].p, st);
+    GA_LAUNCH(k_master_decode, dim3(AGRID), dim3(256), 0, st, n, E, pMk, (const int *)sval, msimp, mst, mat);
     // spec_pointers: one entry per distinct key, and every rank >= 1 of a set holds at least the
     // entry of its own value, so there are slot_bound - nsets of them (read with the final
     // counters for a single set)
     ab(d, AB_SP, 4 * (size_t)(slot_bound + 2));
-    GA_LAUNCH(k_spec_pointers, dim3(AGRID), dim3(256), 0, st, (int)M, (const int *)mflag, (const int *)mpos,
-              AP<int>(d, AB_SP));
+    GA_LAUNCH(k_spec_pointers, dim3(AGRID), dim3(256), 0, st, pMk, (const unsigned *)skey,
+              (const int *)a.specstart, nsets, AP<int>(d, AB_SP));
     int Mk = -1;   // read below
     if (nsets > 1) {
         // first master entry and first edge of every set
@@ -1459,25 +1489,25 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
         d.efirst = {0, E};
     }
     chk(hipEventRecord(d.ev[3], st), "event");
-    // counters, status, the single set's rank count and the spec-pointer count in one read
-    int fin[13];
+    // counters, status, the single set's rank count and the ranked master entries in one read
+    int fin[11];
     const int *zero = a.counters + A_BADMESH;   // 0 here: only caller meshes with errors set it
     read_ints(d, st,
               {a.counters + 0, a.counters + 1, a.counters + 2, a.counters + 3, a.counters + 4, a.counters + 5,
-               a.counters + 6, a.counters + 7, a.status, nsets == 1 ? incl + S0 - 1 : zero,
-               M > 0 ? mpos + M - 1 : zero, M > 0 ? mflag + M - 1 : zero, pMk},
+               a.counters + 6, a.counters + 7, a.status, nsets == 1 ? incl + S0 - 1 : zero, pMk},
               fin);
-    Mk = fin[12];
+    Mk = fin[10];
     d.M = Mk;
     if (nsets == 1) d.mfirst = {0, Mk};
-    int *hc = fin, hs = fin[8], hr[3] = {fin[9], fin[10], fin[11]};
+    int *hc = fin, hs = fin[8];
     if (nsets == 1) {
-        R = hr[0];
+        R = fin[9];
         nslots = R + 1;
         d.specSize = R + 1;
         d.specstart = {0, R + 1};
     }
-    d.nsp = hr[1] + hr[2] + 1;
+    // one spec pointer per distinct key (every rank >= 1 of every set), after the leading -1
+    d.nsp = (nsets == 1 ? R : nslots - nsets) + 1;
     if (stats) {
         stats->nEdges = E;
         stats->specSize = R + 1;
