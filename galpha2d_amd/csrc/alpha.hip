@@ -33,6 +33,14 @@
 
 #define GS(i, n) for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 #define AGRID 2048
+
+// Radix sort of 32-bit keys with 32-bit values: 1024-thread blocks of 8 items and 8-bit digits.
+// rocPRIM carries no tuned onesweep configuration for gfx950; this one sorts 13.5M rank keys in
+// 332 us against 391 us for its default on MI355X.
+using SortCfg32 = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 8>, rocprim::kernel_config<1024, 8>, 8,
+                                        rocprim::block_radix_rank_algorithm::match>>;
 #define U53 1.1102230246251565e-16
 
 GA_INLINE double2 P(const double2 *xy, int v) { return xy[v]; }
@@ -134,6 +142,11 @@ __device__ __noinline__ int attached_exact(double2 i, double2 j, double2 k, int 
     return bi_sign(t) >= 0;
 }
 
+// per spectrum entry id: certified interval of its value and the value (32 bytes, one line)
+struct alignas(32) EntryIv {
+    double lo, hi, v, pad;
+};
+
 struct AlphaK {
     const double2 *xy;
     const int *tris;     // F x 9
@@ -141,10 +154,10 @@ struct AlphaK {
     int *cnt, *eoff;     // [F]
     int *edge_t, *edge_v, *edge_att, *edge_rank, *edge_numin, *edge_numax;  // [E]
     int *tri_rank;       // [F]
-    unsigned long long *key;   // [S0] sort keys (sort_key of the value)
+    unsigned long long *key;   // [S0] single set: FP64 value bits of the triangles; batches: sort keys
     int *id, *id2;       // [S0] entry ids (t or F + e)
     unsigned long long *key2;
-    double2 *iv;         // [F + E] per entry id: certified interval (lo, hi) of its value
+    EntryIv *iv;         // [F + E] per entry id: certified interval (lo, hi) and FP64 value
     double *slo, *shi;   // [S0] sorted
     int *flag;           // [S0] cluster starts
     int *gflag;          // [S0] group starts (exact)
@@ -153,7 +166,7 @@ struct AlphaK {
     int *vnumin, *vnumax;  // [n]
     int2 *vnum;            // [n] (vnumin, vnumax) side by side while k_edge_intervals builds them
     int *status;
-    int *counters;       // [16]
+    int *counters;       // [ALPHA_COUNTER_INTS]: 64 counters, then the stripes
     int *bigstart;             // large clusters (start position)
     int *xclu;                 // small clusters that need exact comparisons (start position)
     int *xatt;                 // edges whose attachment needs the exact predicate
@@ -174,7 +187,33 @@ struct AlphaK {
 #define A_BADMESH 6      // caller triangles that fail k_mesh_check
 #define A_EXCLU 7        // small clusters left to the exact kernel (list in xclu)
 #define A_ATT_LIST 8     // edges whose attachment the filter leaves open (list in xatt)
-#define A_MK 9           // master-list entries with a rank (the sorted list's length)
+#define A_MK 9           // master-list entries with a rank (the sorted list's length), set by k_master_decode
+// Counters that every wave of a large grid updates live in stripes 128 bytes apart (atomics on
+// one address or line from many waves serialise at about 11 ns each), chosen by the wave's
+// global index.  After the 64 counters (ints): the single set's largest FP64 bit pattern of a
+// value and the largest complement of one (u64 stripes), then the statistics ST_* (int stripes),
+// all zero before the run; k_fold_stats folds the statistics into their counters.
+#define URANGE_STRIPES 64
+#define STRIPE_U64 16    // u64 between stripes
+#define STRIPE_INT 32    // ints between stripes
+#define UR_MAX_OFF 64                                            // ints from counters
+#define UR_MINC_OFF (UR_MAX_OFF + 2 * STRIPE_U64 * URANGE_STRIPES)
+#define ST_OFF (UR_MINC_OFF + 2 * STRIPE_U64 * URANGE_STRIPES)
+#define ALPHA_COUNTER_INTS (ST_OFF + STRIPE_INT * URANGE_STRIPES)
+#define ST_CLUSTERS 0    // sum -> A_CLUSTERS
+#define ST_MAXCLUSTER 1  // max -> A_MAXCLUSTER
+#define ST_EXACT_CMP 2   // sum -> A_EXACT_CMP
+#define ST_EXACT_ATT 3   // sum -> A_EXACT_ATT
+GA_INLINE int wave_stripe()
+{
+    return (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % URANGE_STRIPES);
+}
+// wave sum of v into statistic k (call with every lane of the wave active)
+GA_INLINE void stat_add(int *counters, int k, int v)
+{
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (__lane_id() == 0 && v) atomicAdd(&counters[ST_OFF + wave_stripe() * STRIPE_INT + k], v);
+}
 
 GA_INLINE int tv(const AlphaK &a, int t, int i) { return a.tris[(size_t)t * 9 + 3 + i]; }
 GA_INLINE int tn(const AlphaK &a, int t, int i) { return a.tris[(size_t)t * 9 + i]; }
@@ -235,17 +274,54 @@ __global__ void k_edges(AlphaK a)
     }
 }
 
-// Sort keys: the FP64 bits of a value rotated right by 24, so that bits [0, 40) hold its top 40
-// bits (the part the radix sort orders: sign, exponent and 28 mantissa bits) and bits [40, 64)
-// the rest; rotating back gives the value.  (Sorting bits [0, 40) of these keys instead of bits
-// [24, 64) of the plain bits: rocprim's radix sort with a nonzero begin_bit misorders 64-bit keys
-// at some sizes.)
-GA_INLINE unsigned long long sort_key(double v)
+// Single set: the radix sort orders 32-bit keys (key32).  Nonnegative doubles order like their
+// bit patterns u, so key32 = (u >> sh) - (umin >> sh), with the smallest shift sh that fits the
+// value range [umin, umax] of the run in 32 bits, is monotone in the value; values within 2^sh
+// of each other's bits may tie.  The cluster boundaries below hold for any order, so ties only
+// make clusters, which are put in exact order there.  (Uniform points keep about 26 mantissa
+// bits.)  The range comes from a wave maximum of u and of ~u, one atomic per wave.
+// wave maxima of the thread maxima (call with every lane of the wave active), one atomic each
+GA_INLINE void range_note(const AlphaK &a, unsigned long long mx, unsigned long long mc)
 {
-    unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    return (u >> 24) | (u << 40);
+    for (int d = 32; d > 0; d >>= 1) {
+        const unsigned long long ox = __shfl_xor(mx, d, 64), oc = __shfl_xor(mc, d, 64);
+        mx = ox > mx ? ox : mx;
+        mc = oc > mc ? oc : mc;
+    }
+    if (__lane_id() == 0) {
+        const int sl = wave_stripe();
+        if (mx) atomicMax((unsigned long long *)(a.counters + UR_MAX_OFF) + sl * STRIPE_U64, mx);
+        if (mc) atomicMax((unsigned long long *)(a.counters + UR_MINC_OFF) + sl * STRIPE_U64, mc);
+    }
 }
-GA_INLINE unsigned long long key_bits(unsigned long long k) { return (k << 24) | (k >> 40); }
+struct Key32Map {
+    unsigned long long base;
+    int sh;
+};
+// the map of the run from the stripes; every thread of the block calls it (one barrier)
+__device__ Key32Map key32_map(const AlphaK &a)
+{
+    __shared__ unsigned long long s_r[2];
+    if (threadIdx.x < 64) {
+        unsigned long long mx = 0, mc = 0;
+        for (int i = threadIdx.x; i < URANGE_STRIPES; i += 64) {
+            mx = max(mx, ((const unsigned long long *)(a.counters + UR_MAX_OFF))[i * STRIPE_U64]);
+            mc = max(mc, ((const unsigned long long *)(a.counters + UR_MINC_OFF))[i * STRIPE_U64]);
+        }
+        for (int d = 32; d > 0; d >>= 1) {
+            mx = max(mx, (unsigned long long)__shfl_xor(mx, d, 64));
+            mc = max(mc, (unsigned long long)__shfl_xor(mc, d, 64));
+        }
+        if (threadIdx.x == 0) { s_r[0] = mx; s_r[1] = mc; }
+    }
+    __syncthreads();
+    const unsigned long long umax = s_r[0], umin = ~s_r[1];
+    int sh = 0;
+    while (sh < 32 && (umax >> sh) - (umin >> sh) > 0xffffffffull) sh++;
+    return Key32Map{umin >> sh, sh};
+}
+GA_INLINE unsigned key32(const Key32Map &m, unsigned long long u) { return (unsigned)((u >> m.sh) - m.base); }
+GA_INLINE unsigned long long value_bits(double v) { return (unsigned long long)__double_as_longlong(v); }
 // Batches order entries by (set, value): the set above the value's top 40 bits, so certified
 // clusters never span sets (equal values in different sets, e.g. repeated requests, are never
 // compared).  The low value bits are not kept; they are recomputed where needed.
@@ -257,31 +333,38 @@ GA_INLINE unsigned long long batch_key(int set, double v)
 // relative error bound of the certified rho estimate used for ordering
 __global__ void k_tri_rho(AlphaK a)
 {
+    unsigned long long mx = 0, mc = 0;   // value-bit range of this thread's triangles
     GS(t, a.F) {
-        double2 A = a.xy[tv(a, t, 0)], B = a.xy[tv(a, t, 1)], Cc = a.xy[tv(a, t, 2)];
-        double v = rho_tri(A, B, Cc);
-        a.key[t] = a.vset ? batch_key(a.vset[tv(a, t, 0)], v) : sort_key(v);
-        a.id[t] = t;
-        // ar with Shewchuk's orient2d bound: |ar - det| <= eb
-        double detleft = (A.x - Cc.x) * (B.y - Cc.y), detright = (A.y - Cc.y) * (B.x - Cc.x);
-        double det = detleft - detright;
-        double eb = GA_CCWERRA * (fabs(detleft) + fabs(detright));
-        double la = (A.x - B.x) * (A.x - B.x) + (A.y - B.y) * (A.y - B.y);
-        double lb = (B.x - Cc.x) * (B.x - Cc.x) + (B.y - Cc.y) * (B.y - Cc.y);
-        double lc = (Cc.x - A.x) * (Cc.x - A.x) + (Cc.y - A.y) * (Cc.y - A.y);
-        double lo, hi;
-        if (eb < 0.25 * fabs(det)) {
-            double r = eb / fabs(det);
-            double est = (la * lb * lc) / (4.0 * det * det);
-            double rel = 32.0 * U53 + 3.0 * r;
-            lo = est * (1.0 - rel);
-            hi = est * (1.0 + rel);
-        } else {
-            lo = 0.0;
-            hi = INFINITY;
+        {
+            double2 A = a.xy[tv(a, t, 0)], B = a.xy[tv(a, t, 1)], Cc = a.xy[tv(a, t, 2)];
+            double v = rho_tri(A, B, Cc);
+            const unsigned long long u = value_bits(v);
+            a.key[t] = a.vset ? batch_key(a.vset[tv(a, t, 0)], v) : u;
+            a.id[t] = t;
+            // ar with Shewchuk's orient2d bound: |ar - det| <= eb
+            double detleft = (A.x - Cc.x) * (B.y - Cc.y), detright = (A.y - Cc.y) * (B.x - Cc.x);
+            double det = detleft - detright;
+            double eb = GA_CCWERRA * (fabs(detleft) + fabs(detright));
+            double la = (A.x - B.x) * (A.x - B.x) + (A.y - B.y) * (A.y - B.y);
+            double lb = (B.x - Cc.x) * (B.x - Cc.x) + (B.y - Cc.y) * (B.y - Cc.y);
+            double lc = (Cc.x - A.x) * (Cc.x - A.x) + (Cc.y - A.y) * (Cc.y - A.y);
+            double lo, hi;
+            if (eb < 0.25 * fabs(det)) {
+                double r = eb / fabs(det);
+                double est = (la * lb * lc) / (4.0 * det * det);
+                double rel = 32.0 * U53 + 3.0 * r;
+                lo = est * (1.0 - rel);
+                hi = est * (1.0 + rel);
+            } else {
+                lo = 0.0;
+                hi = INFINITY;
+            }
+            a.iv[t] = EntryIv{lo, hi, v, 0.0};
+            mx = max(mx, u);
+            mc = max(mc, ~u);
         }
-        a.iv[t] = make_double2(lo, hi);
     }
+    if (!a.vset) range_note(a, mx, mc);
 }
 
 // attachment of edge (p, q) to the apex k by the FP64 filter: 1 attached, 0 not, -1 open
@@ -304,30 +387,38 @@ GA_INLINE int attached_filter(double2 p, double2 q, double2 k)
 __global__ void k_attached(AlphaK a, int Emax)
 {
     int E = a.eoff[a.F - 1] + a.cnt[a.F - 1];
+    unsigned long long mx = 0, mc = 0;   // value-bit range of every edge (a superset of the entries)
     GS(e, Emax) {
-        if (e >= E) { a.flag[e] = 0; continue; }   // the scan of unattached edges runs to Emax
-        int t = a.edge_t[e], i = a.edge_v[e];
-        double2 p = a.xy[tv(a, t, pl1(i))], q = a.xy[tv(a, t, mi1(i))];
-        int nb = tn(a, t, i);
-        int att = attached_filter(p, q, a.xy[tv(a, t, i)]);
-        if (att == 0 && nb >= 0) att = attached_filter(p, q, a.xy[tv(a, nb >> 2, nb & 3)]);
-        const bool open = att < 0;
-        int slot = wave_reserve(&a.counters[A_ATT_LIST], open);
-        if (open) {
-            a.xatt[slot] = e;
-            att = 0;   // set by k_attached_exact before the flags are scanned again
+        if (e >= E) a.flag[e] = 0;   // the scan of unattached edges runs to Emax
+        else {
+            int t = a.edge_t[e], i = a.edge_v[e];
+            double2 p = a.xy[tv(a, t, pl1(i))], q = a.xy[tv(a, t, mi1(i))];
+            int nb = tn(a, t, i);
+            int att = attached_filter(p, q, a.xy[tv(a, t, i)]);
+            if (att == 0 && nb >= 0) att = attached_filter(p, q, a.xy[tv(a, nb >> 2, nb & 3)]);
+            const bool open = att < 0;
+            int slot = wave_reserve(&a.counters[A_ATT_LIST], open);
+            if (open) {
+                a.xatt[slot] = e;
+                att = 0;   // set by k_attached_exact before the flags are scanned again
+            }
+            a.edge_att[e] = att;
+            a.flag[e] = !att;
+            double v = rho_edge(p, q);
+            const unsigned long long u = value_bits(v);
+            a.iv[a.F + e] = EntryIv{v * (1.0 - 8.0 * U53), v * (1.0 + 8.0 * U53), v, 0.0};
+            mx = max(mx, u);
+            mc = max(mc, ~u);
         }
-        a.edge_att[e] = att;
-        a.flag[e] = !att;
-        double v = rho_edge(p, q);
-        a.iv[a.F + e] = make_double2(v * (1.0 - 8.0 * U53), v * (1.0 + 8.0 * U53));
     }
+    if (!a.vset) range_note(a, mx, mc);
 }
 
 // the edges left open by k_attached: both sides with the exact predicate where the filter fails
 __global__ void k_attached_exact(AlphaK a)
 {
     const int nx = a.counters[A_ATT_LIST];
+    int natt = 0;
     GS(q, nx) {
         int e = a.xatt[q];
         int t = a.edge_t[e], i = a.edge_v[e];
@@ -343,7 +434,7 @@ __global__ void k_attached_exact(AlphaK a)
             }
             att = attached_filter(p, pq, k);
             if (att < 0) {
-                atomicAdd(&a.counters[A_EXACT_ATT], 1);
+                natt++;
                 int ovf = 0;
                 att = attached_exact(p, pq, k, &ovf);
                 if (ovf) atomicOr(a.status, GA_ERR_EXACT_OVERFLOW);
@@ -352,17 +443,26 @@ __global__ void k_attached_exact(AlphaK a)
         a.edge_att[e] = att;
         a.flag[e] = !att;
     }
+    stat_add(a.counters, ST_EXACT_ATT, natt);
 }
 
-// unattached edges appended after the triangles (flag scanned into rank[] positions)
-__global__ void k_edge_entries(AlphaK a, int E)
+// unattached edges appended after the triangles (flag scanned into rank[] positions).  Single
+// set: the 32-bit sort keys of every entry go to k32 (the triangles' from their value bits in
+// key[]); batches: 64-bit (set, value) keys in key[].
+__global__ void k_edge_entries(AlphaK a, int E, unsigned *k32)
 {
+    Key32Map m{0, 0};
+    if (!a.vset) {
+        m = key32_map(a);
+        GS(t, a.F) k32[t] = key32(m, a.key[t]);
+    }
     GS(e, E) {
         if (a.edge_att[e]) continue;
         int k = a.F + a.rank[e];
         int t = a.edge_t[e], i = a.edge_v[e];
         double v = rho_edge(a.xy[tv(a, t, pl1(i))], a.xy[tv(a, t, mi1(i))]);
-        a.key[k] = a.vset ? batch_key(a.vset[tv(a, t, 0)], v) : sort_key(v);
+        if (a.vset) a.key[k] = batch_key(a.vset[tv(a, t, 0)], v);
+        else k32[k] = key32(m, value_bits(v));
         a.id[k] = a.F + e;
     }
 }
@@ -376,26 +476,29 @@ __global__ void k_gather_iv(AlphaK a, int S0, double *neg_lo_rev, int *sset, int
             sset_rev[S0 - 1 - k] = s;
         }
         int id = a.id2[k];
-        double2 w = a.iv[id];
-        a.shi[k] = w.y;
-        double l = w.x;
-        a.slo[k] = l;
-        neg_lo_rev[S0 - 1 - k] = -l;
+        const EntryIv w = a.iv[id];
+        a.shi[k] = w.hi;
+        a.slo[k] = w.lo;
+        neg_lo_rev[S0 - 1 - k] = -w.lo;
+        if (!sset) a.key2[k] = value_bits(w.v);   // single set: the value, for spec_value
     }
 }
 
 // cluster starts: boundary before k when max(hi[0..k-1]) < min(lo[k..]); with sets (scans
 // segmented by set), also where the set changes
+GA_INLINE int cluster_flag(int k, int S0, const double *pmax, const double *neg_smin_rev, const int *sset)
+{
+    if (k == 0 || (sset && sset[k] != sset[k - 1])) return 1;
+    return pmax[k - 1] < -neg_smin_rev[S0 - 1 - k];
+}
+
+// cluster flags, into flag and gflag (the group flags start from them)
 __global__ void k_cluster_flags(AlphaK a, int S0, const double *pmax, const double *neg_smin_rev, const int *sset)
 {
     GS(k, S0) {
-        int f = 1;
-        if (k > 0 && !(sset && sset[k] != sset[k - 1])) {
-            double m = pmax[k - 1];
-            double s = -neg_smin_rev[S0 - 1 - k];
-            f = m < s;
-        }
+        const int f = cluster_flag(k, S0, pmax, neg_smin_rev, sset);
         a.flag[k] = f;
+        a.gflag[k] = f;
     }
 }
 
@@ -412,13 +515,13 @@ GA_INLINE void entry_pts(const AlphaK &a, int id, int *st, double2 *p)
     }
 }
 
-GA_INLINE int cmp_ids(const AlphaK &a, int ia, int ib)
+// exact comparisons are counted in *ncmp (the caller adds its count with stat_add)
+GA_INLINE int cmp_ids(const AlphaK &a, int ia, int ib, int *ncmp)
 {
-    double2 wa = a.iv[ia], wb = a.iv[ib];
-    double la = wa.x, ha = wa.y, lb = wb.x, hb = wb.y;
+    const double la = a.iv[ia].lo, ha = a.iv[ia].hi, lb = a.iv[ib].lo, hb = a.iv[ib].hi;
     if (ha < lb) return -1;
     if (hb < la) return 1;
-    atomicAdd(&a.counters[A_EXACT_CMP], 1);
+    (*ncmp)++;
     int sa, sb;
     double2 pa[3], pb[3];
     entry_pts(a, ia, &sa, pa);
@@ -620,6 +723,7 @@ __global__ void k_big_scatter(AlphaK a, BigK b, int Mbig, const int *sorted_j)
 // inversion (values closer than 2^-255 relative) marks the pair for the fix-up.
 __global__ void k_big_pairs(AlphaK a, BigK b, int Mbig, const int *sorted_j, int *closef, int *inv)
 {
+    int ncmp = 0;
     GS(k, Mbig) {
         int gpos = b.epos[k];
         int jc = sorted_j[k];
@@ -633,17 +737,19 @@ __global__ void k_big_pairs(AlphaK a, BigK b, int Mbig, const int *sorted_j, int
         int f = 1;
         if (same) {
             closef[gpos] = 1;
-            int r = cmp_ids(a, b.eid[jp], b.eid[jc]);
+            int r = cmp_ids(a, b.eid[jp], b.eid[jc], &ncmp);
             if (r > 0) inv[gpos] = 1;
             f = r != 0;
         }
         a.gflag[gpos] = f;
     }
+    stat_add(a.counters, ST_EXACT_CMP, ncmp);
 }
 
 // runs of key-equal neighbours containing an exact inversion: exact insertion sort (one thread)
 __global__ void k_big_fix(AlphaK a, BigK b, int Mbig, const int *closef, const int *inv, int *claimed)
 {
+    int ncmp = 0;
     GS(k, Mbig) {
         int gpos = b.epos[k];
         if (!inv[gpos]) continue;
@@ -656,11 +762,12 @@ __global__ void k_big_fix(AlphaK a, BigK b, int Mbig, const int *closef, const i
         int m = e - s + 1;
         for (int i = 1; i < m; i++) {
             int x = ids[i], j = i - 1;
-            while (j >= 0 && cmp_ids(a, ids[j], x) > 0) { ids[j + 1] = ids[j]; j--; }
+            while (j >= 0 && cmp_ids(a, ids[j], x, &ncmp) > 0) { ids[j + 1] = ids[j]; j--; }
             ids[j + 1] = x;
         }
-        for (int i = 1; i < m; i++) a.gflag[s + i] = cmp_ids(a, ids[i - 1], ids[i]) != 0;
+        for (int i = 1; i < m; i++) a.gflag[s + i] = cmp_ids(a, ids[i - 1], ids[i], &ncmp) != 0;
     }
+    stat_add(a.counters, ST_EXACT_CMP, ncmp);
 }
 
 // One thread per cluster start with size >= 2.  A small cluster whose certified intervals are
@@ -697,9 +804,8 @@ __global__ void k_clusters(AlphaK a, int S0)
             idv[i] = 0; lov[i] = 0.0; hiv[i] = 0.0;
             if (i < m) {
                 idv[i] = ids[i];
-                double2 w = a.iv[idv[i]];
-                lov[i] = w.x;
-                hiv[i] = w.y;
+                lov[i] = a.slo[k + i];
+                hiv[i] = a.shi[k + i];
             }
         }
         bool open = false;
@@ -723,15 +829,47 @@ __global__ void k_clusters(AlphaK a, int S0)
             if (i > 0) a.gflag[k + i] = 1;
         }
     }
-    for (int d = 32; d > 0; d >>= 1) mmax = max(mmax, __shfl_xor(mmax, d, 64));
-    if (__lane_id() == 0 && mmax) atomicMax(&a.counters[A_MAXCLUSTER], mmax);
-    wave_count_full(&a.counters[A_CLUSTERS], nclu);
+    for (int d = 32; d > 0; d >>= 1) {
+        mmax = max(mmax, __shfl_xor(mmax, d, 64));
+        nclu += __shfl_xor(nclu, d, 64);
+    }
+    if (__lane_id() == 0) {
+        int *row = a.counters + ST_OFF + wave_stripe() * STRIPE_INT;
+        if (mmax) atomicMax(row + ST_MAXCLUSTER, mmax);
+        if (nclu) atomicAdd(row + ST_CLUSTERS, nclu);
+    }
+}
+
+// the statistics from their stripes into their counters (one wave, at the end of the run)
+__global__ void k_fold_stats(AlphaK a)
+{
+    int c = 0, m = 0, x = 0, t = 0;
+    for (int i = threadIdx.x; i < URANGE_STRIPES; i += 64) {
+        const int *row = a.counters + ST_OFF + i * STRIPE_INT;
+        c += row[ST_CLUSTERS];
+        m = max(m, row[ST_MAXCLUSTER]);
+        x += row[ST_EXACT_CMP];
+        t += row[ST_EXACT_ATT];
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        c += __shfl_xor(c, d, 64);
+        m = max(m, __shfl_xor(m, d, 64));
+        x += __shfl_xor(x, d, 64);
+        t += __shfl_xor(t, d, 64);
+    }
+    if (threadIdx.x == 0) {
+        a.counters[A_CLUSTERS] += c;
+        a.counters[A_MAXCLUSTER] = max(a.counters[A_MAXCLUSTER], m);
+        a.counters[A_EXACT_CMP] += x;
+        a.counters[A_EXACT_ATT] += t;
+    }
 }
 
 // the small clusters with overlapping intervals: exact insertion sort, then group flags
 __global__ void k_clusters_exact(AlphaK a, int S0)
 {
     const int nx = a.counters[A_EXCLU];
+    int ncmp = 0;
     GS(q, nx) {
         const int k = a.xclu[q];
         int end = k + 1;
@@ -740,11 +878,12 @@ __global__ void k_clusters_exact(AlphaK a, int S0)
         int *ids = a.id2 + k;
         for (int i = 1; i < m; i++) {
             int x = ids[i], j = i - 1;
-            while (j >= 0 && cmp_ids(a, ids[j], x) > 0) { ids[j + 1] = ids[j]; j--; }
+            while (j >= 0 && cmp_ids(a, ids[j], x, &ncmp) > 0) { ids[j + 1] = ids[j]; j--; }
             ids[j + 1] = x;
         }
-        for (int i = 1; i < m; i++) a.gflag[k + i] = cmp_ids(a, ids[i - 1], ids[i]) != 0;
+        for (int i = 1; i < m; i++) a.gflag[k + i] = cmp_ids(a, ids[i - 1], ids[i], &ncmp) != 0;
     }
+    stat_add(a.counters, ST_EXACT_CMP, ncmp);
 }
 
 // End (one past the last entry) of every large cluster, cstart[0..nbig) in increasing order: a
@@ -769,11 +908,6 @@ __global__ void k_big_ends(const int *flag, int S0, const int *cstart, int nbig,
     }
 }
 
-__global__ void k_copy_i32(const int *a, int *b, int n)
-{
-    GS(i, n) b[i] = a[i];
-}
-
 // inclusive = exclusive + flag (in place over the flag array)
 __global__ void k_incl_from_excl(const int *excl, int *flag_incl, int n)
 {
@@ -793,8 +927,8 @@ GA_INLINE double entry_value(const AlphaK &a, int id)
 
 // single set: rank = incl[k] (1-based); the spectrum value of a rank is the smallest FP64 value
 // of its members
-// An entry alone in its certified cluster is never moved by the exact sorts, so key2[k] still
-// holds its FP64 value, and it is alone in its rank (plain store).  Entries of larger clusters
+// An entry alone in its certified cluster is never moved by the exact sorts, so key2[k] (single
+// set) still holds its FP64 value bits, and it is alone in its rank (plain store).  Entries of larger clusters
 // recompute their value and take the minimum: first over the lanes of the wave that hold the same
 // slot (slots do not decrease from lane to lane), then one atomicMin per slot and wave (a
 // lattice's triangles can put a million entries in one rank).  Called by every lane of the wave;
@@ -802,7 +936,7 @@ GA_INLINE double entry_value(const AlphaK &a, int id)
 GA_INLINE void spec_value(const AlphaK &a, int S0, int k, int id, int slot, bool act)
 {
     const bool alone = act && a.flag[k] && (k + 1 == S0 || a.flag[k + 1]);
-    if (alone && !a.vset) a.specbits[slot] = key_bits(a.key2[k]);
+    if (alone && !a.vset) a.specbits[slot] = a.key2[k];
     else if (alone) a.specbits[slot] = (unsigned long long)__double_as_longlong(entry_value(a, id));
     const bool need = act && !alone;
     if (!__any(need)) return;
@@ -927,8 +1061,8 @@ __global__ void k_vert_fix(AlphaK a)
 
 // master keys: q < 2n vertices (2v: vnumin, 2v + 1: vnumax), < 2n + 3E edges (2n + 3e + 0/1/2: rank,
 // numin, numax), then triangles; entries without a rank get the key `none`, above every rank, so
-// they sort to the end; counters[A_MK] counts the others.  One loop per simplex kind, so no index
-// is divided (the host keeps 2n + 3E + F below 2^31 - 2^20).
+// they sort to the end.  One loop per simplex kind, so no index is divided (the host keeps
+// 2n + 3E + F below 2^31 - 2^20).
 GA_INLINE unsigned master_key(const AlphaK &a, int r, int set, unsigned none)
 {
     if (r < 1) return none;
@@ -937,19 +1071,16 @@ GA_INLINE unsigned master_key(const AlphaK &a, int r, int set, unsigned none)
 
 __global__ void k_master_keys(AlphaK a, int E, unsigned int *mkey, int *mval, unsigned int none)
 {
-    int valid = 0;
     const int n2 = 2 * a.n, e0 = n2 + 3 * E;
     GS(v, a.n) {
         const int set = a.vset ? a.vset[v] : 0;
         const int r0 = a.vnumin[v], r1 = a.vnumax[v];
-        valid += (r0 >= 1) + (r1 >= 1);
         *(uint2 *)&mkey[2 * v] = make_uint2(master_key(a, r0, set, none), master_key(a, r1, set, none));
         *(int2 *)&mval[2 * v] = make_int2(2 * v, 2 * v + 1);
     }
     GS(e, E) {
         const int set = a.vset ? tri_set(a, a.edge_t[e]) : 0;
         const int r0 = a.edge_rank[e], r1 = a.edge_numin[e], r2 = a.edge_numax[e];
-        valid += (r0 >= 1) + (r1 >= 1) + (r2 >= 1);
         const int q = n2 + 3 * e;
         mkey[q] = master_key(a, r0, set, none);
         mkey[q + 1] = master_key(a, r1, set, none);
@@ -960,18 +1091,36 @@ __global__ void k_master_keys(AlphaK a, int E, unsigned int *mkey, int *mval, un
     }
     GS(t, a.F) {
         const int r = a.tri_rank[t];
-        valid += r >= 1;
         mkey[e0 + t] = master_key(a, r, tri_set(a, t), none);
         mval[e0 + t] = e0 + t;
     }
-    wave_count_full(&a.counters[A_MK], valid);
 }
 
-// the first *pMk sorted entries (the ranked ones): simplex id, type (2 vertex, 0 edge, 1 triangle)
-// and attribute of each
-__global__ void k_master_decode(int n, int E, const int *pMk, const int *sval, int *simplex, int *st, int *at)
+// number of sorted keys below `none` (the ranked entries), by binary search; every block
+// computes it, block 0 stores it in *pMk
+__device__ int ranked_count(int M, const unsigned int *skey, unsigned int none, int *pMk)
 {
-    const int Mk = *pMk;
+    __shared__ int s_mk;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = M;
+        while (lo < hi) {
+            const int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
+            if (skey[mid] < none) lo = mid + 1;
+            else hi = mid;
+        }
+        s_mk = lo;
+        if (blockIdx.x == 0) *pMk = lo;
+    }
+    __syncthreads();
+    return s_mk;
+}
+
+// the ranked sorted entries (keys below `none`; their count goes to *pMk): simplex id, type
+// (2 vertex, 0 edge, 1 triangle) and attribute of each
+__global__ void k_master_decode(int n, int E, int M, unsigned int none, const unsigned int *skey, int *pMk,
+                                const int *sval, int *simplex, int *st, int *at)
+{
+    const int Mk = ranked_count(M, skey, none, pMk);
     const unsigned n2 = 2u * (unsigned)n, e3 = 3u * (unsigned)E;
     GS(k, Mk) {
         const unsigned q = (unsigned)sval[k];
@@ -989,12 +1138,12 @@ __global__ void k_master_decode(int n, int E, const int *pMk, const int *sval, i
 // least the entry of its own value, so the keys of set s are exactly specstart[s] + 1 ..
 // specstart[s + 1] - 1 (single set: 1 .. R), and key K of set s is distinct key number
 // K - s - 1: no scan over the entries is needed.  specstart[0] = 0.
-__global__ void k_spec_pointers(const int *pMk, const unsigned int *skey, const int *specstart, int nsets, int *sp)
+__global__ void k_spec_pointers(int M, unsigned int none, const unsigned int *skey, const int *specstart, int nsets,
+                                int *sp)
 {
-    const int Mk = *pMk;
-    GS(k, Mk) {
+    GS(k, M) {
         const unsigned K = skey[k];
-        if (k > 0 && skey[k - 1] == K) continue;
+        if (K >= none || (k > 0 && skey[k - 1] == K)) continue;
         int s = 0;
         if (nsets > 1) {
             // largest s with specstart[s] < K
@@ -1209,11 +1358,11 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     d.mfirst.assign(nsets + 1, 0);
     for (int s = 0; s <= nsets; s++) d.specstart[s] = s;   // empty spectra until ranks exist
     ab(d, AB_STATUS, 16);
-    ab(d, AB_CNTRS, 64 * 4);
+    ab(d, AB_CNTRS, 4 * (size_t)ALPHA_COUNTER_INTS);
     {
         IntFills f;
         f.add(d.b[AB_STATUS].p, 4, 0);
-        f.add(d.b[AB_CNTRS].p, 64, 0);
+        f.add(d.b[AB_CNTRS].p, ALPHA_COUNTER_INTS, 0);
         launch_fill_multi(f, st);
     }
     ab(d, AB_VMIN, 4 * (size_t)(n + 1));
@@ -1263,7 +1412,7 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     ab(d, AB_ERANK, 4 * (size_t)E); ab(d, AB_ENMIN, 4 * (size_t)E); ab(d, AB_ENMAX, 4 * (size_t)E);
     ab(d, AB_TRANK, 4 * (size_t)F);
     ab(d, AB_KEY, 8 * NE); ab(d, AB_KEY2, 8 * NE); ab(d, AB_ID, 4 * NE); ab(d, AB_ID2, 4 * NE);
-    ab(d, AB_LO, 16 * NE); ab(d, AB_SLO, 8 * NE); ab(d, AB_SHI, 8 * NE);
+    ab(d, AB_LO, 32 * NE); ab(d, AB_SLO, 8 * NE); ab(d, AB_SHI, 8 * NE);
     ab(d, AB_FLAG, 4 * NE); ab(d, AB_RANK, 4 * NE); ab(d, AB_SPECBITS, 8 * (NE + (size_t)nsets + 2));
     ab(d, AB_NEG, 8 * NE); ab(d, AB_PMAX, 8 * NE); ab(d, AB_NSMIN, 8 * NE); ab(d, AB_MISC, 4 * NE);
     a.edge_t = AP<int>(d, AB_ET); a.edge_v = AP<int>(d, AB_EV); a.edge_att = AP<int>(d, AB_EATT);
@@ -1271,7 +1420,7 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     a.tri_rank = AP<int>(d, AB_TRANK);
     a.key = AP<unsigned long long>(d, AB_KEY); a.key2 = AP<unsigned long long>(d, AB_KEY2);
     a.id = AP<int>(d, AB_ID); a.id2 = AP<int>(d, AB_ID2);
-    a.iv = AP<double2>(d, AB_LO);
+    a.iv = AP<EntryIv>(d, AB_LO);
     a.slo = AP<double>(d, AB_SLO); a.shi = AP<double>(d, AB_SHI);
     a.flag = AP<int>(d, AB_FLAG); a.rank = AP<int>(d, AB_RANK); a.gflag = AP<int>(d, AB_MISC);
 
@@ -1304,22 +1453,33 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     d.E = E;
     int S0 = F + nun;
     d.S = S0;
-    GA_LAUNCH(k_edge_entries, dim3(AGRID), dim3(256), 0, st, a, E);
+    // single set: 32-bit keys in and out side by side in key2 (free until the gather below)
+    unsigned *k32 = (unsigned *)a.key2;
+    GA_LAUNCH(k_edge_entries, dim3(AGRID), dim3(256), 0, st, a, E, k32);
     chk(hipEventRecord(d.ev[1], st), "event");
-    // radix sort by the top 40 bits of the FP64 value (nonnegative doubles order like their bit
-    // patterns; 5 passes instead of 8; see sort_key).  The cluster boundaries below only need
+    // radix sort by value: single set, 32-bit monotone keys (4 passes; see key32); batches, the
+    // set above the top 40 bits of the FP64 value.  The cluster boundaries below only need
     // "everything before is certainly smaller than everything after", which holds for any order,
-    // so entries that tie on those bits (values within 2^-28 relative) end in one cluster and are
-    // put in exact order there.
+    // so entries that tie on their key end in one cluster and are put in exact order there.
     size_t tb = 0;
-    unsigned sbits = 0;   // batches: set bits above the 40 value bits
-    if (nsets > 1)
+    if (nsets == 1) {
+        chk(rocprim::radix_sort_pairs<SortCfg32>(nullptr, tb, k32, k32 + S0, a.id, a.id2, (unsigned)S0, 0u, 32u, st),
+            "sort");
+        ab(d, AB_TMP, tb + 256);
+        tb = d.b[AB_TMP].bytes;
+        chk(rocprim::radix_sort_pairs<SortCfg32>(d.b[AB_TMP].p, tb, k32, k32 + S0, a.id, a.id2, (unsigned)S0, 0u,
+                                                 32u, st),
+            "sort");
+    } else {
+        unsigned sbits = 0;   // set bits above the 40 value bits
         while ((1LL << sbits) < (long long)nsets) sbits++;
-    const unsigned kbits = 40u + sbits;
-    chk(rocprim::radix_sort_pairs(nullptr, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, kbits, st), "sort");
-    ab(d, AB_TMP, tb + 256);
-    tb = d.b[AB_TMP].bytes;
-    chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, kbits, st), "sort");
+        const unsigned kbits = 40u + sbits;
+        chk(rocprim::radix_sort_pairs(nullptr, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, kbits, st), "sort");
+        ab(d, AB_TMP, tb + 256);
+        tb = d.b[AB_TMP].bytes;
+        chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, a.key, a.key2, a.id, a.id2, (unsigned)S0, 0u, kbits, st),
+            "sort");
+    }
     // certified intervals in sorted order, prefix max of hi, suffix min of lo (per set in batches)
     double *neg = AP<double>(d, AB_NEG), *pmax = AP<double>(d, AB_PMAX), *nsmin = AP<double>(d, AB_NSMIN);
     int *sset = nullptr, *sset_rev = nullptr;
@@ -1354,7 +1514,6 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     }
     GA_LAUNCH(k_cluster_flags, dim3(AGRID), dim3(256), 0, st, a, S0, (const double *)pmax, (const double *)nsmin,
               (const int *)sset);
-    GA_LAUNCH(k_copy_i32, dim3(AGRID), dim3(256), 0, st, (const int *)a.flag, a.gflag, S0);
     // every large cluster has more than MAX_SMALL_CLUSTER entries, every cluster at least two
     a.bigcap = S0 / (MAX_SMALL_CLUSTER + 1) + 1;
     ab(d, AB_BIGL, 4 * (size_t)a.bigcap + 64);
@@ -1451,20 +1610,23 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     unsigned *skey = AP<unsigned>(d, AB_SKEY);
     int *sval = AP<int>(d, AB_SVAL);
     tb = 0;
-    chk(rocprim::radix_sort_pairs(nullptr, tb, mkey, skey, mval, sval, (unsigned)M, 0u, (unsigned)bits, st), "sort");
+    chk(rocprim::radix_sort_pairs<SortCfg32>(nullptr, tb, mkey, skey, mval, sval, (unsigned)M, 0u, (unsigned)bits, st),
+        "sort");
     ab(d, AB_TMP, tb + 256);
     tb = d.b[AB_TMP].bytes;
-    chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, tb, mkey, skey, mval, sval, (unsigned)M, 0u, (unsigned)bits, st),
+    chk(rocprim::radix_sort_pairs<SortCfg32>(d.b[AB_TMP].p, tb, mkey, skey, mval, sval, (unsigned)M, 0u,
+                                             (unsigned)bits, st),
         "sort");
     // decode: simplex -> AB_OKEY, st -> AB_OVAL, at -> AB_MKEY (free after the sort)
     int *msimp = AP<int>(d, AB_OKEY), *mst = AP<int>(d, AB_OVAL), *mat = (int *)mkey;
-    const int *pMk = a.counters + A_MK;
-    GA_LAUNCH(k_master_decode, dim3(AGRID), dim3(256), 0, st, n, E, pMk, (const int *)sval, msimp, mst, mat);
+    int *pMk = a.counters + A_MK;
+    GA_LAUNCH(k_master_decode, dim3(AGRID), dim3(256), 0, st, n, E, (int)M, none, (const unsigned *)skey, pMk,
+              (const int *)sval, msimp, mst, mat);
     // spec_pointers: one entry per distinct key, and every rank >= 1 of a set holds at least the
     // entry of its own value, so there are slot_bound - nsets of them (read with the final
     // counters for a single set)
     ab(d, AB_SP, 4 * (size_t)(slot_bound + 2));
-    GA_LAUNCH(k_spec_pointers, dim3(AGRID), dim3(256), 0, st, pMk, (const unsigned *)skey,
+    GA_LAUNCH(k_spec_pointers, dim3(AGRID), dim3(256), 0, st, (int)M, none, (const unsigned *)skey,
               (const int *)a.specstart, nsets, AP<int>(d, AB_SP));
     int Mk = -1;   // read below
     if (nsets > 1) {
@@ -1488,6 +1650,7 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     } else {
         d.efirst = {0, E};
     }
+    GA_LAUNCH(k_fold_stats, dim3(1), dim3(64), 0, st, a);
     chk(hipEventRecord(d.ev[3], st), "event");
     // counters, status, the single set's rank count and the ranked master entries in one read
     int fin[11];
