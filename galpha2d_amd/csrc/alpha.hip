@@ -1245,10 +1245,12 @@ struct IntReads {
     const int *p[16];
     int n;
 };
-__global__ void k_gather_ints(IntReads r, int *dst)
+// into pinned host memory (the host waits on the stream before it reads)
+__global__ void k_gather_ints(IntReads r, int *host)
 {
     int i = threadIdx.x;
-    if (i < r.n) dst[i] = *r.p[i];
+    if (i < r.n) host[i] = *r.p[i];
+    __threadfence_system();
 }
 
 static void read_ints(AlphaDev &d, hipStream_t st, std::initializer_list<const int *> ptrs, int *out)
@@ -1259,9 +1261,7 @@ static void read_ints(AlphaDev &d, hipStream_t st, std::initializer_list<const i
         r.p[r.n++] = q;
     }
     if (!d.hpin) chk(hipHostMalloc((void **)&d.hpin, 16 * sizeof(int), hipHostMallocDefault), "pinned alloc");
-    ab(d, AB_RD, 16 * sizeof(int));
-    GA_LAUNCH(k_gather_ints, dim3(1), dim3(64), 0, st, r, AP<int>(d, AB_RD));
-    chk(hipMemcpyAsync(d.hpin, d.b[AB_RD].p, 4 * (size_t)r.n, hipMemcpyDeviceToHost, st), "copy");
+    GA_LAUNCH(k_gather_ints, dim3(1), dim3(64), 0, st, r, d.hpin);
     chk(hipStreamSynchronize(st), "sync");
     memcpy(out, d.hpin, 4 * (size_t)r.n);
 }

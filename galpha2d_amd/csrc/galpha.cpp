@@ -135,7 +135,8 @@ struct galpha_ctx {
     void *tag_p[3] = {nullptr, nullptr, nullptr};
     void *mx_p = nullptr;
     int prio_round = 0;
-    int *hsnap = nullptr;   // pinned staging for counter reads (SNAP_INTS ints)
+    int *hsnap = nullptr;   // staging for counter reads (SNAP_INTS ints)
+    bool hsnap_pinned = false;   // pinned (kernels write it directly), else pageable
     std::vector<int> hsnap_pageable;   // counter reads of a no-allocation context
     // the last gpudtComputeDT on this (default) context: the caller's point array and the returned
     // triangle array with their sizes and content hashes, so that gpudtComputeAlpha of that output
@@ -350,12 +351,19 @@ void stage_h2d_hash(HostStage &s, void *dst, const void *src, size_t bytes, hipS
 
 static const int SNAP_INTS = 256;
 
-// counter reads go through pinned host memory: one device-to-host copy, one wait
-static void ctx_counters_read(galpha_ctx *c, int *dst, int slot, int n)
+// Counter reads: a kernel writes the counters into pinned host memory, then one wait (a no-
+// allocation context has pageable staging: one device-to-host copy instead).  fold: the slots
+// C_FOLD + k come from the statistics stripes.
+static void ctx_counters_read(galpha_ctx *c, int *dst, int slot, int n, bool fold = false)
 {
     if (n > SNAP_INTS) throw GalphaError("counter read too large");
-    HIPCHK(hipMemcpyAsync(c->hsnap, c->counters.as<int>() + slot, sizeof(int) * n, hipMemcpyDeviceToHost,
-                          c->stream));
+    if (c->hsnap_pinned) {
+        launch_counters_to_host(c->counters.as<int>(), slot, n, fold ? 1 : 0, c->hsnap, c->stream);
+    } else {
+        if (fold) launch_fold_stats(c->counters.as<int>(), c->stream);
+        HIPCHK(hipMemcpyAsync(c->hsnap, c->counters.as<int>() + slot, sizeof(int) * n, hipMemcpyDeviceToHost,
+                              c->stream));
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     memcpy(dst, c->hsnap, sizeof(int) * n);
 }
@@ -385,6 +393,12 @@ static void ctx_read_segments(galpha_ctx *c, std::initializer_list<std::pair<con
         tot += (size_t)sg.second;
     }
     if (tot == 0) return;
+    if (c->hsnap_pinned && tot <= (size_t)SNAP_INTS) {   // gathered straight into pinned memory
+        launch_gather_segments(g, c->hsnap, c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        memcpy(dst, c->hsnap, 4 * tot);
+        return;
+    }
     c->rdbuf.ensure(4 * tot + 64);
     launch_gather_segments(g, c->rdbuf.as<int>(), c->stream);
     if (tot <= (size_t)SNAP_INTS) {
@@ -405,8 +419,9 @@ static void ctx_counter_set(galpha_ctx *c, int slot, int v)
 // striped statistics (see kernels.h): fold the stripes on the device, then read
 static int ctx_stat(galpha_ctx *c, int k)
 {
-    launch_fold_stats(c->counters.as<int>(), c->stream);
-    return ctx_counter(c, C_FOLD + k);
+    int v = 0;
+    ctx_counters_read(c, &v, C_FOLD + k, 1, true);
+    return v;
 }
 // all single-word counters, the status word and the folded statistics in one read:
 // snap[k] = counter k, snap[C_FOLD + k] = statistic k
@@ -418,8 +433,7 @@ struct Snap {
 static Snap ctx_snapshot(galpha_ctx *c)
 {
     Snap s;
-    launch_fold_stats(c->counters.as<int>(), c->stream);
-    ctx_counters_read(c, s.v, 0, C_FOLD + 64);
+    ctx_counters_read(c, s.v, 0, C_FOLD + 64, true);
     return s;
 }
 static const size_t COUNTER_BYTES = sizeof(int) * GA_STRIPE_INTS * (1 + GA_STRIPES);
@@ -757,8 +771,7 @@ static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t
     }
     // few sets: grid-wide rounds, eight per host read (a round after one without candidates
     // returns at once; 1M uniform points take about 20 rounds)
-    launch_ear_grid(m, B, bnd, hcnt, hoff, nsets, st);
-    launch_reset_counters(c->counters.as<int>(), C_EAR, -1, -1, -1, st);
+    launch_ear_grid(m, B, bnd, hcnt, hoff, nsets, st);   // also zeroes counters[C_EAR]
     const int per_read = 8;
     int round = 0;
     for (;; round += per_read) {
@@ -971,7 +984,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     for (int r = 0; pend > 0; r += 2) {
         if (r > 10000) throw GalphaError("vertex deletion did not finish");
         for (int k = 0; k < 2; k++) {
-            launch_reset_counters(c->counters.as<int>(), C_SCR, C_MODN, -1, -1, st);
+            // (k_del_prio zeroes C_SCR and C_MODN)
             launch_delete_round(m, c->mx.as<unsigned int>(), next_prio_round(), c->op++, pgrid, st);
         }
         Snap sn = ctx_snapshot(c);
@@ -1379,6 +1392,7 @@ galpha_ctx *galphaCreate(int device)
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (int i = 0; i < 16; i++) HIPCHK(hipEventCreate(&c->ev[i]));
         HIPCHK(hipHostMalloc((void **)&c->hsnap, sizeof(int) * SNAP_INTS, hipHostMallocDefault));
+        c->hsnap_pinned = true;
         return c;
     } catch (const std::exception &e) {
         g_last_error = e.what();
@@ -1517,7 +1531,7 @@ void galphaDestroy(galpha_ctx *c)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
     alpha_destroy_events(c->alpha);
     if (c->stream) hipStreamDestroy(c->stream);
-    if (c->hsnap) hipHostFree(c->hsnap);
+    if (c->hsnap && c->hsnap_pinned) hipHostFree(c->hsnap);
     stage_free(c->stage);
     delete c;
 }

@@ -238,6 +238,8 @@ void launch_relink(MeshDev &m, const int *modlist, int cnt_slot, int op, int gri
 void launch_reset_counters(int *counters, int a, int b, int d, int e, hipStream_t st);
 // counters[C_FOLD + k] = sum over the stripes of statistic k, for k < 64
 void launch_fold_stats(int *counters, hipStream_t st);
+// counters[slot, slot + n) (n <= 256) into pinned host memory, statistics folded when fold is set
+void launch_counters_to_host(int *counters, int slot, int n, int fold, int *host, hipStream_t st);
 void launch_flip_all(MeshDev &m, int op, int *list, hipStream_t st);
 // flip passes on short lists in one workgroup (see k_flip_tail)
 void launch_flip_tail(MeshDev &m, int *lA, int sA, int *lB, int sB, int prev_op, int op0, int max_passes,

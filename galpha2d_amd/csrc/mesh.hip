@@ -395,8 +395,10 @@ __global__ void k_match(const int4 *__restrict__ tv, const int *__restrict__ ntr
 
 void launch_adjacency(MeshDev &m, int *deg, int *off, int *fill, int2 *fan, void *scan_tmp, hipStream_t st)
 {
-    hipMemsetAsync(deg, 0, sizeof(int) * (m.V + 1), st);
-    hipMemsetAsync(fill, 0, sizeof(int) * (m.V + 1), st);
+    IntFills f;
+    f.add(deg, (size_t)m.V + 1, 0);
+    f.add(fill, (size_t)m.V + 1, 0);
+    launch_fill_multi(f, st);
     GA_LAUNCH(k_deg, dim3(FIXED_GRID), dim3(256), 0, st, m.tv, m.ntri, deg);
     launch_exclusive_scan(deg, off, m.V + 1, scan_tmp, st);
     GA_LAUNCH(k_fanfill, dim3(FIXED_GRID), dim3(256), 0, st, m.tv, m.ntri, off, fill, fan, m.vt);
@@ -440,8 +442,10 @@ GA_INLINE double2 cand_pos(const MeshDev &m, const unsigned char *moved, int v)
 // triangle does.  Reverting only ever clears moved[], so the first pass looks at every triangle
 // and later passes only at the stars of the vertices reverted by the pass before (a worklist;
 // mark[] lists each vertex once).
-__global__ void k_shift_init(MeshDev m, unsigned char *moved)
+// also zeroes the count of k_shift_bulk's output list
+__global__ void k_shift_init(MeshDev m, unsigned char *moved, int cnt_out_slot)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) m.counters[cnt_out_slot] = 0;
     GRID_STRIDE(v, m.n) moved[v] = (m.vstate[v] == VS_PENDING) ? 1 : 0;
 }
 
@@ -501,8 +505,7 @@ __global__ void k_shift_star(MeshDev m, unsigned char *moved, int *mark, const i
 
 void launch_shift_bulk(MeshDev &m, unsigned char *moved, int *mark, int *wl_out, int cnt_out_slot, hipStream_t st)
 {
-    GA_LAUNCH(k_shift_init, dim3(FIXED_GRID), dim3(256), 0, st, m, moved);
-    launch_reset_counters(m.counters, cnt_out_slot, -1, -1, -1, st);
+    GA_LAUNCH(k_shift_init, dim3(FIXED_GRID), dim3(256), 0, st, m, moved, cnt_out_slot);
     GA_LAUNCH(k_shift_bulk, dim3(FIXED_GRID), dim3(256), 0, st, m, moved, mark, wl_out, cnt_out_slot);
 }
 
@@ -620,8 +623,13 @@ GA_INLINE unsigned long long prio64(int v, int round)
 }
 
 // distance-2 independence: every vertex records the largest priority in its closed neighbourhood
+// also zeroes the scratch and modified-list counts that k_del_apply of this round fills
 __global__ void k_del_prio(MeshDev m, unsigned long long *mx, int round)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        m.counters[C_SCR] = 0;
+        m.counters[C_MODN] = 0;
+    }
     int np = m.counters[C_PLIST];
     GRID_STRIDE(k, np) {
         int v = m.plist[k];
@@ -1542,6 +1550,32 @@ void launch_fold_stats(int *counters, hipStream_t st)
     GA_LAUNCH(k_fold_stats, dim3(1), dim3(64), 0, st, counters);
 }
 
+// counters[slot, slot + n) straight into pinned host memory (n <= 256); with fold, the slots
+// C_FOLD + k hold the stripe sum of statistic k (also stored back on the device, as k_fold_stats
+// does).  The host waits on the stream before it reads.
+__global__ void k_counters_to_host(int *c, int slot, int n, int fold, int *host)
+{
+    const int i = threadIdx.x;
+    if (i < n) {
+        const int q = slot + i;
+        int v;
+        if (fold && q >= C_FOLD && q < C_FOLD + 64) {
+            v = 0;
+            for (int s = 0; s < GA_STRIPES; s++) v += c[GA_STRIPE_INTS * (1 + s) + (q - C_FOLD)];
+            c[q] = v;
+        } else {
+            v = c[q];
+        }
+        host[i] = v;
+    }
+    __threadfence_system();
+}
+
+void launch_counters_to_host(int *counters, int slot, int n, int fold, int *host, hipStream_t st)
+{
+    GA_LAUNCH(k_counters_to_host, dim3(1), dim3(256), 0, st, counters, slot, n, fold, host);
+}
+
 __global__ void k_hull_edges(MeshDev m, int *hnext, int *hedge, int *hull_cnt, int *hull_start)
 {
     int n = ntri_used(m);
@@ -1564,9 +1598,11 @@ __global__ void k_hull_edges(MeshDev m, int *hnext, int *hedge, int *hull_cnt, i
 void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, int *hull_cnt, int *hull_off,
                       int *hull_start, void *scan_tmp, hipStream_t st)
 {
-    hipMemsetAsync(m.hnext, 0xff, sizeof(int) * m.V, st);
-    hipMemsetAsync(hull_cnt, 0, sizeof(int) * m.nsets, st);
-    launch_fill_i32(hull_start, 0x7fffffff, (size_t)m.nsets, st);
+    IntFills f;
+    f.add(m.hnext, (size_t)m.V, -1);
+    f.add(hull_cnt, (size_t)m.nsets, 0);
+    f.add(hull_start, (size_t)m.nsets, 0x7fffffff);
+    launch_fill_multi(f, st);
     GA_LAUNCH(k_hull_edges, dim3(FIXED_GRID), dim3(256), 0, st, m, m.hnext, m.hedge, hull_cnt, hull_start);
     launch_exclusive_scan(hull_cnt, hull_off, m.nsets, scan_tmp, st);
     GA_LAUNCH(k_convexify, dim3(m.nsets), dim3(256), 0, st, m, (const int *)m.hnext, (const int *)m.hedge,
@@ -2318,7 +2354,11 @@ static unsigned long long *ear_claims(int *bnd, int B, int nsets)
 void launch_ear_grid(MeshDev &m, int B, int *bnd, const int *hull_cnt, const int *hull_off, int nsets, hipStream_t st)
 {
     GA_LAUNCH(k_ear_grid, dim3(nsets), dim3(EAR_THREADS), 0, st, m, B, bnd, hull_cnt, hull_off);
-    launch_fill_i32((int *)ear_claims(bnd, B, nsets), 0, 4 * (size_t)B, st);
+    // claims zero, and the first round's candidate count
+    IntFills f;
+    f.add(ear_claims(bnd, B, nsets), 4 * (size_t)B, 0);
+    f.add(m.counters + C_EAR, 1, 0);
+    launch_fill_multi(f, st);
 }
 
 // rounds r0 .. r0 + k - 1 as grid-wide kernels, one wave per cycle position

@@ -84,6 +84,7 @@ void bbox_sets_device(const double *d_xy, const int *d_offs, int nsets, double *
 }
 
 // segments of device ints gathered into one contiguous array (for a single host read)
+// dst may be pinned host memory (the host waits on the stream before it reads)
 __global__ void k_gather_segments(IntSegments g, int *dst)
 {
     int base = 0;
@@ -92,6 +93,7 @@ __global__ void k_gather_segments(IntSegments g, int *dst)
             dst[base + i] = g.src[k][i];
         base += g.len[k];
     }
+    __threadfence_system();
 }
 
 void launch_gather_segments(const IntSegments &g, int *dst, hipStream_t st)
