@@ -467,6 +467,8 @@ __global__ void k_edge_entries(AlphaK a, int E, unsigned *k32)
     }
 }
 
+// batches: sorted intervals, the set of every position (and reversed) and the negated lo values
+// reversed, for the segmented scans of the cluster flags
 __global__ void k_gather_iv(AlphaK a, int S0, double *neg_lo_rev, int *sset, int *sset_rev)
 {
     GS(k, S0) {
@@ -480,12 +482,112 @@ __global__ void k_gather_iv(AlphaK a, int S0, double *neg_lo_rev, int *sset, int
         a.shi[k] = w.hi;
         a.slo[k] = w.lo;
         neg_lo_rev[S0 - 1 - k] = -w.lo;
-        if (!sset) a.key2[k] = value_bits(w.v);   // single set: the value, for spec_value
     }
 }
 
 // cluster starts: boundary before k when max(hi[0..k-1]) < min(lo[k..]); with sets (scans
 // segmented by set), also where the set changes
+// ---- single set: cluster flags from tile bounds instead of full-length scans ----------------
+// Tiles of CF_TILE sorted entries; a block of 256 threads holds CF_ITEMS consecutive entries per
+// thread.  k_gather_iv_tiles writes the sorted intervals and each tile's max hi and min lo;
+// k_cluster_flags_tiles sets flag[k] = max(hi[0..k-1]) < min(lo[k..S0-1]) from the bounds of the
+// tiles before and after, a block scan of the thread bounds and the thread's own entries.
+#define CF_ITEMS 16
+#define CF_TILE (256 * CF_ITEMS)
+
+__global__ __launch_bounds__(256) void k_gather_iv_tiles(AlphaK a, int S0, double *tmax, double *tmin)
+{
+    __shared__ double s_mx[4], s_mn[4];
+    const int base = blockIdx.x * CF_TILE;
+    double mx = -INFINITY, mn = INFINITY;
+    for (int i = 0; i < CF_ITEMS; i++) {
+        const int k = base + i * 256 + threadIdx.x;
+        if (k >= S0) break;
+        const EntryIv w = a.iv[a.id2[k]];
+        a.shi[k] = w.hi;
+        a.slo[k] = w.lo;
+        a.key2[k] = value_bits(w.v);   // the value, for spec_value
+        mx = fmax(mx, w.hi);
+        mn = fmin(mn, w.lo);
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, d, 64));
+        mn = fmin(mn, __shfl_xor(mn, d, 64));
+    }
+    if ((threadIdx.x & 63) == 0) { s_mx[threadIdx.x >> 6] = mx; s_mn[threadIdx.x >> 6] = mn; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; w++) { mx = fmax(mx, s_mx[w]); mn = fmin(mn, s_mn[w]); }
+        tmax[blockIdx.x] = mx;
+        tmin[blockIdx.x] = mn;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cluster_flags_tiles(AlphaK a, int S0, int ntiles, const double *tmax,
+                                                             const double *tmin)
+{
+    __shared__ double s_a[4], s_b[4];
+    __shared__ double s_wpre[4], s_wsuf[4];
+    const int tile = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // bounds of the tiles before (max hi) and after (min lo)
+    double P = -INFINITY, Q = INFINITY;
+    for (int j = threadIdx.x; j < ntiles; j += 256) {
+        if (j < tile) P = fmax(P, tmax[j]);
+        if (j > tile) Q = fmin(Q, tmin[j]);
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        P = fmax(P, __shfl_xor(P, d, 64));
+        Q = fmin(Q, __shfl_xor(Q, d, 64));
+    }
+    if (lane == 0) { s_a[w] = P; s_b[w] = Q; }
+    __syncthreads();
+    P = fmax(fmax(s_a[0], s_a[1]), fmax(s_a[2], s_a[3]));
+    Q = fmin(fmin(s_b[0], s_b[1]), fmin(s_b[2], s_b[3]));
+    // this thread's entries; past S0 neutral bounds
+    const int k0 = tile * CF_TILE + threadIdx.x * CF_ITEMS;
+    double hi[CF_ITEMS], lo[CF_ITEMS];
+#pragma unroll
+    for (int i = 0; i < CF_ITEMS; i++) {
+        const bool in = k0 + i < S0;
+        hi[i] = in ? a.shi[k0 + i] : -INFINITY;
+        lo[i] = in ? a.slo[k0 + i] : INFINITY;
+    }
+    // lo[i] <- min(lo[i..]) within the thread; thread bounds H (max hi), L (min lo)
+    double H = -INFINITY;
+#pragma unroll
+    for (int i = 0; i < CF_ITEMS; i++) H = fmax(H, hi[i]);
+#pragma unroll
+    for (int i = CF_ITEMS - 2; i >= 0; i--) lo[i] = fmin(lo[i], lo[i + 1]);
+    const double L = lo[0];
+    // exclusive prefix max of H and exclusive suffix min of L over the threads of the block
+    double pre = H, suf = L;
+    for (int d = 1; d < 64; d <<= 1) {
+        const double u = __shfl_up(pre, d, 64), v = __shfl_down(suf, d, 64);
+        if (lane >= d) pre = fmax(pre, u);
+        if (lane + d < 64) suf = fmin(suf, v);
+    }
+    if (lane == 63) s_wpre[w] = pre;   // wave totals
+    if (lane == 0) s_wsuf[w] = suf;
+    double ex_pre = __shfl_up(pre, 1, 64), ex_suf = __shfl_down(suf, 1, 64);
+    if (lane == 0) ex_pre = -INFINITY;
+    if (lane == 63) ex_suf = INFINITY;
+    __syncthreads();
+    for (int v = 0; v < w; v++) ex_pre = fmax(ex_pre, s_wpre[v]);
+    for (int v = w + 1; v < 4; v++) ex_suf = fmin(ex_suf, s_wsuf[v]);
+    double before = fmax(P, ex_pre);       // max hi of every entry before this thread's first
+    const double after_t = fmin(Q, ex_suf);  // min lo of every entry after this thread's last
+#pragma unroll
+    for (int i = 0; i < CF_ITEMS; i++) {
+        const int k = k0 + i;
+        if (k < S0) {
+            const int f = k == 0 ? 1 : (before < fmin(after_t, lo[i]));
+            a.flag[k] = f;
+            a.gflag[k] = f;
+        }
+        before = fmax(before, hi[i]);
+    }
+}
+
 GA_INLINE int cluster_flag(int k, int S0, const double *pmax, const double *neg_smin_rev, const int *sset)
 {
     if (k == 0 || (sset && sset[k] != sset[k - 1])) return 1;
@@ -1489,8 +1591,14 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
         sset = AP<int>(d, AB_BS0);
         sset_rev = AP<int>(d, AB_BS1);
     }
-    GA_LAUNCH(k_gather_iv, dim3(AGRID), dim3(256), 0, st, a, S0, neg, sset, sset_rev);
-    if (nsets > 1) {
+    if (nsets == 1) {
+        const int ntiles = (S0 + CF_TILE - 1) / CF_TILE;
+        double *tmax = pmax, *tmin = nsmin;   // tile bounds (ntiles each)
+        GA_LAUNCH(k_gather_iv_tiles, dim3(ntiles), dim3(256), 0, st, a, S0, tmax, tmin);
+        GA_LAUNCH(k_cluster_flags_tiles, dim3(ntiles), dim3(256), 0, st, a, S0, ntiles, (const double *)tmax,
+                  (const double *)tmin);
+    } else {
+        GA_LAUNCH(k_gather_iv, dim3(AGRID), dim3(256), 0, st, a, S0, neg, sset, sset_rev);
         tb = 0;
         chk(rocprim::inclusive_scan_by_key(nullptr, tb, sset, a.shi, pmax, (size_t)S0, rocprim::maximum<double>(),
                                            rocprim::equal_to<int>(), st), "scan");
@@ -1501,19 +1609,9 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
         tb = d.b[AB_TMP].bytes;
         chk(rocprim::inclusive_scan_by_key(d.b[AB_TMP].p, tb, sset_rev, neg, nsmin, (size_t)S0,
                                            rocprim::maximum<double>(), rocprim::equal_to<int>(), st), "scan");
-    } else {
-        tb = 0;
-        chk(rocprim::inclusive_scan(nullptr, tb, a.shi, pmax, (size_t)S0, rocprim::maximum<double>(), st), "scan");
-        ab(d, AB_TMP, tb + 256);
-        tb = d.b[AB_TMP].bytes;
-        chk(rocprim::inclusive_scan(d.b[AB_TMP].p, tb, a.shi, pmax, (size_t)S0, rocprim::maximum<double>(), st),
-            "scan");
-        tb = d.b[AB_TMP].bytes;
-        chk(rocprim::inclusive_scan(d.b[AB_TMP].p, tb, neg, nsmin, (size_t)S0, rocprim::maximum<double>(), st),
-            "scan");
+        GA_LAUNCH(k_cluster_flags, dim3(AGRID), dim3(256), 0, st, a, S0, (const double *)pmax,
+                  (const double *)nsmin, (const int *)sset);
     }
-    GA_LAUNCH(k_cluster_flags, dim3(AGRID), dim3(256), 0, st, a, S0, (const double *)pmax, (const double *)nsmin,
-              (const int *)sset);
     // every large cluster has more than MAX_SMALL_CLUSTER entries, every cluster at least two
     a.bigcap = S0 / (MAX_SMALL_CLUSTER + 1) + 1;
     ab(d, AB_BIGL, 4 * (size_t)a.bigcap + 64);
