@@ -1192,26 +1192,41 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     c->vals2.ensure(4 * (size_t)nslots + 64);
     c->rot.ensure(4 * (size_t)nslots + 64);
     c->newid.ensure(4 * (size_t)nslots + 64);
-    // one key (v0, v1) of 2B bits, input indices below 2^B (a batch's sets are consecutive ranges)
+    // one key (v0, v1) of 2B bits, input indices below 2^B (a batch's sets are consecutive ranges);
+    // up to B = 20 a 32-bit key (v0, top bits of v1) whose rare ties k_canon_newid orders
     int kb = 2;
     while ((1LL << kb) < (long long)n) kb++;
     if (2 * kb > 64) throw GalphaError("too many points for the canonical key");
-    unsigned long long *K = c->keys.as<unsigned long long>(), *K2 = c->keys2.as<unsigned long long>();
     int *vals = c->vals.as<int>(), *vals2 = c->vals2.as<int>();
-    launch_canon_keys(m, K, vals, c->rot.as<int>(), kb, st);
+    const unsigned *skey32 = nullptr, *v1s = nullptr;
     size_t tb = 0;
-    sort_pairs_u64(K, vals, K2, vals2, nslots, 0, 2 * kb, nullptr, &tb, st);
-    c->sortTmp.ensure(tb + 256);
-    tb = c->sortTmp.bytes;
-    if (sort_pairs_u64(K, vals, K2, vals2, nslots, 0, 2 * kb, c->sortTmp.p, &tb, st))
-        throw GalphaError("radix sort failed");
-    const int *order = vals2;
+    if (kb <= CANON_KEY32_BITS) {
+        unsigned *K = c->keys.as<unsigned>(), *Ko = K + nslots, *V1 = c->keys2.as<unsigned>();
+        launch_canon_keys32(m, K, V1, vals, c->rot.as<int>(), kb, st);
+        const int bits = std::min(32, 2 * kb);
+        sort_pairs_u32(K, vals, Ko, vals2, nslots, 0, bits, nullptr, &tb, st);
+        c->sortTmp.ensure(tb + 256);
+        tb = c->sortTmp.bytes;
+        if (sort_pairs_u32(K, vals, Ko, vals2, nslots, 0, bits, c->sortTmp.p, &tb, st))
+            throw GalphaError("radix sort failed");
+        skey32 = Ko;
+        v1s = V1;
+    } else {
+        unsigned long long *K = c->keys.as<unsigned long long>(), *K2 = c->keys2.as<unsigned long long>();
+        launch_canon_keys(m, K, vals, c->rot.as<int>(), kb, st);
+        sort_pairs_u64(K, vals, K2, vals2, nslots, 0, 2 * kb, nullptr, &tb, st);
+        c->sortTmp.ensure(tb + 256);
+        tb = c->sortTmp.bytes;
+        if (sort_pairs_u64(K, vals, K2, vals2, nslots, 0, 2 * kb, c->sortTmp.p, &tb, st))
+            throw GalphaError("radix sort failed");
+    }
+    int *order = vals2;
     // the live-triangle count stays on the device (folded statistic) until the final read
     c->out9.ensure(36 * (size_t)nslots + 64);
     launch_fold_stats(c->counters.as<int>(), st);
-    launch_canon_emit(m, order, c->rot.as<int>(), c->newid.as<int>(),
+    launch_canon_emit(m, order, skey32, v1s, c->rot.as<int>(), c->newid.as<int>(),
                       c->counters.as<int>() + C_FOLD + C_RECON_TRIS, c->out9.as<int>(), c->setFirst.as<int>(),
-                      c->keys2.as<int>(), st);   // keys2 is free after the sort: output sets
+                      c->keys2.as<int>(), st);   // keys2 (v1s) is free after k_canon_newid: output sets
     ev_mark(c, 8, st);
     Snap fin;
     std::vector<int> sfv((size_t)nsets + sizeof(fin.v) / 4);   // first triangle of every set, then the counters
@@ -1469,8 +1484,10 @@ __attribute__((visibility("hidden"))) size_t galpha_noalloc_need(int n, int T)
     long long ntiles = (long long)(T / 16) * (T / 16), nc = P / 256 + 1;
     long long kb = 2;
     while ((1LL << kb) < (long long)n) kb++;
-    size_t sort_tmp = 0;
+    size_t sort_tmp = 0, sort_tmp32 = 0;
     sort_pairs_u64(nullptr, nullptr, nullptr, nullptr, (int)cap, 0, (int)(2 * kb), nullptr, &sort_tmp, 0);
+    sort_pairs_u32(nullptr, nullptr, nullptr, nullptr, (int)cap, 0, (int)std::min(32LL, 2 * kb), nullptr, &sort_tmp32, 0);
+    sort_tmp = std::max(sort_tmp, sort_tmp32);
     double b = 0;
     b += 18.0 * P + 516.0 * T + 8.0 * ntiles + 8.0 * nc;          // textures, band ranges, tiles, cells
     b += (116.0 + 68.0) * cap;                                     // mesh slots, canonical keys + output

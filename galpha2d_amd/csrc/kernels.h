@@ -284,7 +284,11 @@ void launch_convexify(MeshDev &m, int *list_out, int cnt_out_slot, int op, int *
                       int *hull_start, void *scan_tmp, hipStream_t st);
 // canonical order: key (v0 << B) | v1 of input indices below 2^B, sorted on its low 2B bits
 void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot, int B, hipStream_t st);
-void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, const int *dF, int *out9,
+// canonical keys of 32 bits up to input indices below 2^CANON_KEY32_BITS (ties within 256 triangles)
+#define CANON_KEY32_BITS 20
+void launch_canon_keys32(MeshDev &m, unsigned *keys, unsigned *v1s, int *vals, int *rot, int B, hipStream_t st);
+void launch_canon_emit(MeshDev &m, int *order, const unsigned *skey, const unsigned *v1s, const int *rot, int *newid,
+                       const int *dF, int *out9,
                        int *set_first, int *rset, hipStream_t st);
 int sort_pairs_u64(unsigned long long *keys_in, int *vals_in, unsigned long long *keys_out, int *vals_out,
                    int n, int begin_bit, int end_bit, void *temp, size_t *temp_bytes, hipStream_t st);

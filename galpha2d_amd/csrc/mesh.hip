@@ -2413,14 +2413,66 @@ void launch_canon_keys(MeshDev &m, unsigned long long *keys, int *vals, int *rot
     GA_LAUNCH(k_canon_keys, dim3(FIXED_GRID), dim3(256), 0, st, m, keys, vals, rot, B);
 }
 
+// 32-bit keys (indices below 2^B, B <= 20): v0 in the top B bits, the top 32 - B bits of v1
+// (of its B bits) below, so one sort of 4 radix passes (B <= 16: the whole key, 2B bits) orders
+// the triangles up to runs of equal keys: the same v0 and v1 within 2^(2B - 32) of each other, at
+// most 256 triangles (every directed edge is in one triangle).  v1[t] keeps the whole v1 for
+// k_canon_newid, which orders those runs.  Dead slots get all ones, above every live key.
+__global__ void k_canon_keys32(MeshDev m, unsigned *keys, unsigned *v1s, int *vals, int *rot, int B)
+{
+    int n = ntri_used(m);
+    int alive = 0;
+    const int lo = 2 * B > 32 ? 2 * B - 32 : 0;   // v1 bits dropped from the key
+    GRID_STRIDE(t, n) {
+        int4 v = m.tv[t];
+        vals[t] = t;
+        if (v.x == GA_DEAD) { keys[t] = 0xffffffffu; rot[t] = 0; continue; }
+        alive++;
+        int4 o = make_int4(oid(m, v.x), oid(m, v.y), oid(m, v.z), 0);   // input indices
+        int r = 0;
+        if (o.y < ic(o, r)) r = 1;
+        if (o.z < ic(o, r)) r = 2;
+        rot[t] = r;
+        const unsigned a = (unsigned)ic(o, r), b = (unsigned)ic(o, (r + 1) % 3);
+        keys[t] = (a << (B - lo)) | (b >> lo);
+        v1s[t] = b;
+    }
+    stat_add_full(m.stats, C_RECON_TRIS, alive);
+}
+
+void launch_canon_keys32(MeshDev &m, unsigned *keys, unsigned *v1s, int *vals, int *rot, int B, hipStream_t st)
+{
+    GA_LAUNCH(k_canon_keys32, dim3(FIXED_GRID), dim3(256), 0, st, m, keys, v1s, vals, rot, B);
+}
+
 // newid[slot] = output position << 2 | rotation of the slot's triangle: one gather per neighbour
-// in k_canon_emit
-__global__ void k_canon_newid(const int *order, const int *rot, const int *dF, int *newid)
+// in k_canon_emit.  With 32-bit keys (skey, v1s non-null) a run of equal sorted keys is put in
+// v1 order by the thread of its first position (insertion sort in place, at most 256 entries).
+__global__ void k_canon_newid(int *order, const unsigned *skey, const unsigned *v1s, const int *rot, const int *dF,
+                              int *newid)
 {
     int F = *dF;
     GRID_STRIDE(r, F) {
-        int s = order[r];
-        newid[s] = (r << 2) | rot[s];
+        if (!skey) {
+            int s = order[r];
+            newid[s] = (r << 2) | rot[s];
+            continue;
+        }
+        const unsigned key = skey[r];
+        if (r > 0 && skey[r - 1] == key) continue;   // inside a run: its first position orders it
+        int e = r + 1;
+        while (e < F && skey[e] == key) e++;
+        for (int i = r + 1; i < e; i++) {
+            const int x = order[i];
+            const unsigned vx = v1s[x];
+            int j = i - 1;
+            while (j >= r && v1s[order[j]] > vx) { order[j + 1] = order[j]; j--; }
+            order[j + 1] = x;
+        }
+        for (int i = r; i < e; i++) {
+            const int s = order[i];
+            newid[s] = (i << 2) | rot[s];
+        }
     }
 }
 
@@ -2465,11 +2517,11 @@ __global__ void k_canon_set_first(MeshDev m, const int *rset, const int *dF, int
     }
 }
 
-void launch_canon_emit(MeshDev &m, const int *order, const int *rot, int *newid, const int *dF, int *out9,
-                       int *set_first, int *rset, hipStream_t st)
+void launch_canon_emit(MeshDev &m, int *order, const unsigned *skey, const unsigned *v1s, const int *rot, int *newid,
+                       const int *dF, int *out9, int *set_first, int *rset, hipStream_t st)
 {
     launch_fill_i32(set_first, -1, (size_t)m.nsets, st);
-    GA_LAUNCH(k_canon_newid, dim3(FIXED_GRID), dim3(256), 0, st, order, rot, dF, newid);
+    GA_LAUNCH(k_canon_newid, dim3(FIXED_GRID), dim3(256), 0, st, order, skey, v1s, rot, dF, newid);
     GA_LAUNCH(k_canon_emit, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)newid, out9, rset);
     GA_LAUNCH(k_canon_set_first, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)rset, dF, set_first);
 }

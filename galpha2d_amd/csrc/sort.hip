@@ -12,10 +12,17 @@ int sort_pairs_u64(unsigned long long *keys_in, int *vals_in, unsigned long long
     return (int)e;
 }
 
+// 32-bit keys: 1024-thread blocks of 8 items, 8-bit digits (rocPRIM carries no tuned onesweep
+// configuration for gfx950; this one is 15-25% faster than its default on MI355X)
+using SortCfgU32 = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 8>, rocprim::kernel_config<1024, 8>, 8,
+                                        rocprim::block_radix_rank_algorithm::match>>;
+
 int sort_pairs_u32(unsigned int *keys_in, int *vals_in, unsigned int *keys_out, int *vals_out,
                    int n, int begin_bit, int end_bit, void *temp, size_t *temp_bytes, hipStream_t st)
 {
-    hipError_t e = rocprim::radix_sort_pairs(temp, *temp_bytes, keys_in, keys_out, vals_in, vals_out,
-                                             (unsigned int)n, (unsigned)begin_bit, (unsigned)end_bit, st);
+    hipError_t e = rocprim::radix_sort_pairs<SortCfgU32>(temp, *temp_bytes, keys_in, keys_out, vals_in, vals_out,
+                                                         (unsigned int)n, (unsigned)begin_bit, (unsigned)end_bit, st);
     return (int)e;
 }
