@@ -149,6 +149,7 @@ struct alignas(32) EntryIv {
 
 struct AlphaK {
     const double2 *xy;
+    double2 *cxy;        // [3 F] corner points of every triangle in vertex order (k_tri_rho)
     const int *tris;     // F x 9
     int n, F;
     int *cnt, *eoff;     // [F]
@@ -218,6 +219,9 @@ GA_INLINE void stat_add(int *counters, int k, int v)
 GA_INLINE int tv(const AlphaK &a, int t, int i) { return a.tris[(size_t)t * 9 + 3 + i]; }
 GA_INLINE int tn(const AlphaK &a, int t, int i) { return a.tris[(size_t)t * 9 + i]; }
 GA_INLINE int pl1(int i) { return i == 2 ? 0 : i + 1; }
+// corner i of triangle t (its point, from the per-triangle copy: one read near the triangle's
+// other data instead of a gather at the vertex id)
+GA_INLINE double2 cpt(const AlphaK &a, int t, int i) { return a.cxy[(size_t)t * 3 + i]; }
 GA_INLINE int mi1(int i) { return i == 0 ? 2 : i - 1; }
 
 // Every edge is listed once, by its owner: the triangle in which it runs from the smaller vertex
@@ -337,6 +341,9 @@ __global__ void k_tri_rho(AlphaK a)
     GS(t, a.F) {
         {
             double2 A = a.xy[tv(a, t, 0)], B = a.xy[tv(a, t, 1)], Cc = a.xy[tv(a, t, 2)];
+            a.cxy[(size_t)t * 3] = A;
+            a.cxy[(size_t)t * 3 + 1] = B;
+            a.cxy[(size_t)t * 3 + 2] = Cc;
             double v = rho_tri(A, B, Cc);
             const unsigned long long u = value_bits(v);
             a.key[t] = a.vset ? batch_key(a.vset[tv(a, t, 0)], v) : u;
@@ -392,10 +399,10 @@ __global__ void k_attached(AlphaK a, int Emax)
         if (e >= E) a.flag[e] = 0;   // the scan of unattached edges runs to Emax
         else {
             int t = a.edge_t[e], i = a.edge_v[e];
-            double2 p = a.xy[tv(a, t, pl1(i))], q = a.xy[tv(a, t, mi1(i))];
+            double2 p = cpt(a, t, pl1(i)), q = cpt(a, t, mi1(i));
             int nb = tn(a, t, i);
-            int att = attached_filter(p, q, a.xy[tv(a, t, i)]);
-            if (att == 0 && nb >= 0) att = attached_filter(p, q, a.xy[tv(a, nb >> 2, nb & 3)]);
+            int att = attached_filter(p, q, cpt(a, t, i));
+            if (att == 0 && nb >= 0) att = attached_filter(p, q, cpt(a, nb >> 2, nb & 3));
             const bool open = att < 0;
             int slot = wave_reserve(&a.counters[A_ATT_LIST], open);
             if (open) {
@@ -422,15 +429,15 @@ __global__ void k_attached_exact(AlphaK a)
     GS(q, nx) {
         int e = a.xatt[q];
         int t = a.edge_t[e], i = a.edge_v[e];
-        double2 p = a.xy[tv(a, t, pl1(i))], pq = a.xy[tv(a, t, mi1(i))];
+        double2 p = cpt(a, t, pl1(i)), pq = cpt(a, t, mi1(i));
         int nb = tn(a, t, i);
         int att = 0;
         for (int side = 0; side < 2 && !att; side++) {
             double2 k;
-            if (side == 0) k = a.xy[tv(a, t, i)];
+            if (side == 0) k = cpt(a, t, i);
             else {
                 if (nb < 0) break;
-                k = a.xy[tv(a, nb >> 2, nb & 3)];
+                k = cpt(a, nb >> 2, nb & 3);
             }
             att = attached_filter(p, pq, k);
             if (att < 0) {
@@ -460,7 +467,7 @@ __global__ void k_edge_entries(AlphaK a, int E, unsigned *k32)
         if (a.edge_att[e]) continue;
         int k = a.F + a.rank[e];
         int t = a.edge_t[e], i = a.edge_v[e];
-        double v = rho_edge(a.xy[tv(a, t, pl1(i))], a.xy[tv(a, t, mi1(i))]);
+        double v = rho_edge(cpt(a, t, pl1(i)), cpt(a, t, mi1(i)));
         if (a.vset) a.key[k] = batch_key(a.vset[tv(a, t, 0)], v);
         else k32[k] = key32(m, value_bits(v));
         a.id[k] = a.F + e;
@@ -608,12 +615,12 @@ GA_INLINE void entry_pts(const AlphaK &a, int id, int *st, double2 *p)
 {
     if (id < a.F) {
         *st = 1;
-        p[0] = a.xy[tv(a, id, 0)]; p[1] = a.xy[tv(a, id, 1)]; p[2] = a.xy[tv(a, id, 2)];
+        p[0] = cpt(a, id, 0); p[1] = cpt(a, id, 1); p[2] = cpt(a, id, 2);
     } else {
         int e = id - a.F;
         int t = a.edge_t[e], i = a.edge_v[e];
         *st = 0;
-        p[0] = a.xy[tv(a, t, pl1(i))]; p[1] = a.xy[tv(a, t, mi1(i))];
+        p[0] = cpt(a, t, pl1(i)); p[1] = cpt(a, t, mi1(i));
     }
 }
 
@@ -1021,10 +1028,10 @@ GA_INLINE int tri_set(const AlphaK &a, int t) { return a.vset ? a.vset[tv(a, t, 
 
 GA_INLINE double entry_value(const AlphaK &a, int id)
 {
-    if (id < a.F) return rho_tri(a.xy[tv(a, id, 0)], a.xy[tv(a, id, 1)], a.xy[tv(a, id, 2)]);
+    if (id < a.F) return rho_tri(cpt(a, id, 0), cpt(a, id, 1), cpt(a, id, 2));
     int e = id - a.F;
     int t = a.edge_t[e], i = a.edge_v[e];
-    return rho_edge(a.xy[tv(a, t, pl1(i))], a.xy[tv(a, t, mi1(i))]);
+    return rho_edge(cpt(a, t, pl1(i)), cpt(a, t, mi1(i)));
 }
 
 // single set: rank = incl[k] (1-based); the spectrum value of a rank is the smallest FP64 value
@@ -1305,8 +1312,9 @@ enum {
     AB_ID2, AB_LO, AB_HI, AB_SLO, AB_SHI, AB_FLAG, AB_RANK, AB_SPECBITS, AB_VMIN, AB_VMAX, AB_STATUS, AB_CNTRS,
     AB_TMP, AB_SCAN, AB_NEG, AB_PMAX, AB_NSMIN, AB_MKEY, AB_MVAL, AB_MFLAG, AB_MPOS, AB_OKEY, AB_OVAL,
     AB_SKEY, AB_SVAL, AB_SPEC, AB_SP, AB_MISC, AB_BIG0, AB_BIG1, AB_BIG2, AB_BIG3, AB_BIGL,
-    AB_BS0, AB_BS1, AB_BS2, AB_BS3, AB_BS4, AB_BSET, AB_MSET, AB_VHULL, AB_RANKQ, AB_RD, AB_XATT, AB_XCLU, AB_VNUM, AB_COUNT
+    AB_BS0, AB_BS1, AB_BS2, AB_BS3, AB_BS4, AB_BSET, AB_MSET, AB_VHULL, AB_RANKQ, AB_RD, AB_XATT, AB_XCLU, AB_VNUM, AB_CXY, AB_COUNT
 };
+static_assert(AB_COUNT <= 64, "AlphaDev holds 64 buffers");
 
 static void ab(AlphaDev &d, int i, size_t bytes)
 {
@@ -1534,6 +1542,8 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     ab(d, AB_VNUM, 8 * (size_t)n + 64);
     a.vnum = AP<int2>(d, AB_VNUM);
     GA_LAUNCH(k_edges, dim3(AGRID), dim3(256), 0, st, a);
+    ab(d, AB_CXY, 48 * (size_t)F + 64);
+    a.cxy = AP<double2>(d, AB_CXY);
     GA_LAUNCH(k_tri_rho, dim3(AGRID), dim3(256), 0, st, a);
     ab(d, AB_XATT, 4 * (size_t)Emax + 64);
     a.xatt = AP<int>(d, AB_XATT);
