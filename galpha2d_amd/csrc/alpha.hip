@@ -499,7 +499,7 @@ __global__ void k_gather_iv(AlphaK a, int S0, double *neg_lo_rev, int *sset, int
 // thread.  k_gather_iv_tiles writes the sorted intervals and each tile's max hi and min lo;
 // k_cluster_flags_tiles sets flag[k] = max(hi[0..k-1]) < min(lo[k..S0-1]) from the bounds of the
 // tiles before and after, a block scan of the thread bounds and the thread's own entries.
-#define CF_ITEMS 16
+#define CF_ITEMS 8
 #define CF_TILE (256 * CF_ITEMS)
 
 __global__ __launch_bounds__(256) void k_gather_iv_tiles(AlphaK a, int S0, double *tmax, double *tmin)
@@ -530,34 +530,88 @@ __global__ __launch_bounds__(256) void k_gather_iv_tiles(AlphaK a, int S0, doubl
     }
 }
 
-__global__ __launch_bounds__(256) void k_cluster_flags_tiles(AlphaK a, int S0, int ntiles, const double *tmax,
-                                                             const double *tmin)
+// tile bounds -> exclusive bounds: tmax[t] <- max hi of the tiles before t, tmin[t] <- min lo of
+// the tiles after t (one block, chunks of 1024 tiles with a carry)
+__global__ __launch_bounds__(1024) void k_tile_bounds_scan(int ntiles, double *tmax, double *tmin)
 {
-    __shared__ double s_a[4], s_b[4];
+    __shared__ double s_w[16];
+    __shared__ double s_carry;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // prefix max, forward
+    if (threadIdx.x == 0) s_carry = -INFINITY;
+    __syncthreads();
+    for (int c0 = 0; c0 < ntiles; c0 += 1024) {
+        const int j = c0 + threadIdx.x;
+        const double x = j < ntiles ? tmax[j] : -INFINITY;
+        double inc = x;
+        for (int d = 1; d < 64; d <<= 1) {
+            const double y = __shfl_up(inc, d, 64);
+            if (lane >= d) inc = fmax(inc, y);
+        }
+        if (lane == 63) s_w[w] = inc;
+        __syncthreads();
+        double ex = __shfl_up(inc, 1, 64);
+        if (lane == 0) ex = -INFINITY;
+        for (int v = 0; v < w; v++) ex = fmax(ex, s_w[v]);
+        ex = fmax(ex, s_carry);
+        double tot = s_carry;
+        for (int v = 0; v < 16; v++) tot = fmax(tot, s_w[v]);
+        __syncthreads();
+        if (j < ntiles) tmax[j] = ex;
+        if (threadIdx.x == 0) s_carry = tot;
+        __syncthreads();
+    }
+    // suffix min, backward
+    if (threadIdx.x == 0) s_carry = INFINITY;
+    __syncthreads();
+    for (int c1 = ntiles; c1 > 0; c1 -= 1024) {
+        const int j = c1 - 1 - threadIdx.x;   // thread 0 takes the chunk's last tile
+        const double x = j >= 0 ? tmin[j] : INFINITY;
+        double inc = x;
+        for (int d = 1; d < 64; d <<= 1) {
+            const double y = __shfl_up(inc, d, 64);
+            if (lane >= d) inc = fmin(inc, y);
+        }
+        if (lane == 63) s_w[w] = inc;
+        __syncthreads();
+        double ex = __shfl_up(inc, 1, 64);
+        if (lane == 0) ex = INFINITY;
+        for (int v = 0; v < w; v++) ex = fmin(ex, s_w[v]);
+        ex = fmin(ex, s_carry);
+        double tot = s_carry;
+        for (int v = 0; v < 16; v++) tot = fmin(tot, s_w[v]);
+        __syncthreads();
+        if (j >= 0) tmin[j] = ex;
+        if (threadIdx.x == 0) s_carry = tot;
+        __syncthreads();
+    }
+}
+
+// The tile's hi and lo go through LDS (coalesced loads, then each thread's CF_ITEMS consecutive
+// entries; one pad slot per CF_ITEMS keeps the per-thread reads off one bank), and the flags go
+// back the same way.  P / Q: max hi before and min lo after the tile (k_tile_bounds_scan).
+#define CF_PAD(i) ((i) + (i) / CF_ITEMS)
+__global__ __launch_bounds__(256) void k_cluster_flags_tiles(AlphaK a, int S0, const double *texmax,
+                                                             const double *texmin)
+{
+    __shared__ double s_hi[CF_PAD(CF_TILE)], s_lo[CF_PAD(CF_TILE)];
     __shared__ double s_wpre[4], s_wsuf[4];
     const int tile = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    // bounds of the tiles before (max hi) and after (min lo)
-    double P = -INFINITY, Q = INFINITY;
-    for (int j = threadIdx.x; j < ntiles; j += 256) {
-        if (j < tile) P = fmax(P, tmax[j]);
-        if (j > tile) Q = fmin(Q, tmin[j]);
+    const double P = texmax[tile], Q = texmin[tile];
+    const int base = tile * CF_TILE;
+#pragma unroll
+    for (int j = 0; j < CF_ITEMS; j++) {
+        const int idx = j * 256 + threadIdx.x, k = base + idx;
+        const bool in = k < S0;
+        s_hi[CF_PAD(idx)] = in ? a.shi[k] : -INFINITY;
+        s_lo[CF_PAD(idx)] = in ? a.slo[k] : INFINITY;
     }
-    for (int d = 32; d > 0; d >>= 1) {
-        P = fmax(P, __shfl_xor(P, d, 64));
-        Q = fmin(Q, __shfl_xor(Q, d, 64));
-    }
-    if (lane == 0) { s_a[w] = P; s_b[w] = Q; }
     __syncthreads();
-    P = fmax(fmax(s_a[0], s_a[1]), fmax(s_a[2], s_a[3]));
-    Q = fmin(fmin(s_b[0], s_b[1]), fmin(s_b[2], s_b[3]));
-    // this thread's entries; past S0 neutral bounds
-    const int k0 = tile * CF_TILE + threadIdx.x * CF_ITEMS;
     double hi[CF_ITEMS], lo[CF_ITEMS];
 #pragma unroll
     for (int i = 0; i < CF_ITEMS; i++) {
-        const bool in = k0 + i < S0;
-        hi[i] = in ? a.shi[k0 + i] : -INFINITY;
-        lo[i] = in ? a.slo[k0 + i] : INFINITY;
+        hi[i] = s_hi[CF_PAD(threadIdx.x * CF_ITEMS + i)];
+        lo[i] = s_lo[CF_PAD(threadIdx.x * CF_ITEMS + i)];
     }
     // lo[i] <- min(lo[i..]) within the thread; thread bounds H (max hi), L (min lo)
     double H = -INFINITY;
@@ -578,20 +632,27 @@ __global__ __launch_bounds__(256) void k_cluster_flags_tiles(AlphaK a, int S0, i
     double ex_pre = __shfl_up(pre, 1, 64), ex_suf = __shfl_down(suf, 1, 64);
     if (lane == 0) ex_pre = -INFINITY;
     if (lane == 63) ex_suf = INFINITY;
-    __syncthreads();
+    __syncthreads();   // also: every thread holds its entries, s_hi is free for the flags
     for (int v = 0; v < w; v++) ex_pre = fmax(ex_pre, s_wpre[v]);
     for (int v = w + 1; v < 4; v++) ex_suf = fmin(ex_suf, s_wsuf[v]);
-    double before = fmax(P, ex_pre);       // max hi of every entry before this thread's first
+    double before = fmax(P, ex_pre);         // max hi of every entry before this thread's first
     const double after_t = fmin(Q, ex_suf);  // min lo of every entry after this thread's last
+    int *s_f = (int *)s_hi;
 #pragma unroll
     for (int i = 0; i < CF_ITEMS; i++) {
-        const int k = k0 + i;
+        const int k = base + threadIdx.x * CF_ITEMS + i;
+        s_f[CF_PAD(threadIdx.x * CF_ITEMS + i)] = k == 0 ? 1 : (before < fmin(after_t, lo[i]));
+        before = fmax(before, hi[i]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CF_ITEMS; j++) {
+        const int idx = j * 256 + threadIdx.x, k = base + idx;
         if (k < S0) {
-            const int f = k == 0 ? 1 : (before < fmin(after_t, lo[i]));
+            const int f = s_f[CF_PAD(idx)];
             a.flag[k] = f;
             a.gflag[k] = f;
         }
-        before = fmax(before, hi[i]);
     }
 }
 
@@ -1605,7 +1666,8 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
         const int ntiles = (S0 + CF_TILE - 1) / CF_TILE;
         double *tmax = pmax, *tmin = nsmin;   // tile bounds (ntiles each)
         GA_LAUNCH(k_gather_iv_tiles, dim3(ntiles), dim3(256), 0, st, a, S0, tmax, tmin);
-        GA_LAUNCH(k_cluster_flags_tiles, dim3(ntiles), dim3(256), 0, st, a, S0, ntiles, (const double *)tmax,
+        GA_LAUNCH(k_tile_bounds_scan, dim3(1), dim3(1024), 0, st, ntiles, tmax, tmin);
+        GA_LAUNCH(k_cluster_flags_tiles, dim3(ntiles), dim3(256), 0, st, a, S0, (const double *)tmax,
                   (const double *)tmin);
     } else {
         GA_LAUNCH(k_gather_iv, dim3(AGRID), dim3(256), 0, st, a, S0, neg, sset, sset_rev);
