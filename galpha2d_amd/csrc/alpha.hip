@@ -173,6 +173,7 @@ struct AlphaK {
     int *xatt;                 // edges whose attachment needs the exact predicate
     int bigcap;
     int nS0;
+    int skip_if_big;     // first pass of a single set: kernels after k_clusters return when it found large clusters
     // batches: set of every vertex, first spectrum slot and rank base of every set
     const int *vset;     // [n] (NULL for a single set)
     int nsets;
@@ -193,7 +194,7 @@ struct AlphaK {
 // one address or line from many waves serialise at about 11 ns each), chosen by the wave's
 // global index.  After the 64 counters (ints): the single set's largest FP64 bit pattern of a
 // value and the largest complement of one (u64 stripes), then the statistics ST_* (int stripes),
-// all zero before the run; k_fold_stats folds the statistics into their counters.
+// all zero before the run; the final read folds the statistics into their counters.
 #define URANGE_STRIPES 64
 #define STRIPE_U64 16    // u64 between stripes
 #define STRIPE_INT 32    // ints between stripes
@@ -219,6 +220,8 @@ GA_INLINE void stat_add(int *counters, int k, int v)
 GA_INLINE int tv(const AlphaK &a, int t, int i) { return a.tris[(size_t)t * 9 + 3 + i]; }
 GA_INLINE int tn(const AlphaK &a, int t, int i) { return a.tris[(size_t)t * 9 + i]; }
 GA_INLINE int pl1(int i) { return i == 2 ? 0 : i + 1; }
+// first pass of a single set that found large clusters: the rest is redone after their order
+GA_INLINE bool big_pending(const AlphaK &a) { return a.skip_if_big && a.counters[A_BIGCLUSTER] > 0; }
 // corner i of triangle t (its point, from the per-triangle copy: one read near the triangle's
 // other data instead of a gather at the vertex id)
 GA_INLINE double2 cpt(const AlphaK &a, int t, int i) { return a.cxy[(size_t)t * 3 + i]; }
@@ -1010,8 +1013,9 @@ __global__ void k_clusters(AlphaK a, int S0)
     }
 }
 
-// the statistics from their stripes into their counters (one wave, at the end of the run)
-__global__ void k_fold_stats(AlphaK a)
+// the statistics from their stripes into their counters (one wave, at the end of the run); every
+// lane returns the folded (clusters, largest cluster, exact compares, exact attachments)
+__device__ int4 fold_stats(const AlphaK &a)
 {
     int c = 0, m = 0, x = 0, t = 0;
     for (int i = threadIdx.x; i < URANGE_STRIPES; i += 64) {
@@ -1027,12 +1031,14 @@ __global__ void k_fold_stats(AlphaK a)
         x += __shfl_xor(x, d, 64);
         t += __shfl_xor(t, d, 64);
     }
-    if (threadIdx.x == 0) {
-        a.counters[A_CLUSTERS] += c;
-        a.counters[A_MAXCLUSTER] = max(a.counters[A_MAXCLUSTER], m);
-        a.counters[A_EXACT_CMP] += x;
-        a.counters[A_EXACT_ATT] += t;
+    m = max(m, a.counters[A_MAXCLUSTER]);   // also k_big_elems' atomicMax
+    if (threadIdx.x == 0) {   // these counters change only through their stripes: folding twice is safe
+        a.counters[A_CLUSTERS] = c;
+        a.counters[A_MAXCLUSTER] = m;
+        a.counters[A_EXACT_CMP] = x;
+        a.counters[A_EXACT_ATT] = t;
     }
+    return make_int4(c, m, x, t);
 }
 
 // the small clusters with overlapping intervals: exact insertion sort, then group flags
@@ -1079,9 +1085,10 @@ __global__ void k_big_ends(const int *flag, int S0, const int *cstart, int nbig,
 }
 
 // inclusive = exclusive + flag (in place over the flag array)
-__global__ void k_incl_from_excl(const int *excl, int *flag_incl, int n)
+// incl[i] = excl[i] + flag[i] (incl may be flag)
+__global__ void k_incl_from_excl(const int *excl, const int *flag, int *incl, int n)
 {
-    GS(i, n) flag_incl[i] = excl[i] + flag_incl[i];
+    GS(i, n) incl[i] = excl[i] + flag[i];
 }
 
 GA_INLINE int entry_tri(const AlphaK &a, int id) { return id < a.F ? id : a.edge_t[id - a.F]; }
@@ -1124,6 +1131,7 @@ GA_INLINE void spec_value(const AlphaK &a, int S0, int k, int id, int slot, bool
 
 __global__ void k_ranks(AlphaK a, int S0, const int *incl)
 {
+    if (big_pending(a)) return;
     for (int base = blockIdx.x * blockDim.x; base < S0; base += gridDim.x * blockDim.x) {
         const int k = base + threadIdx.x;
         const bool act = k < S0;
@@ -1186,6 +1194,7 @@ __global__ void k_ranks_batch(AlphaK a, int S0, const unsigned int *sset2, const
 
 __global__ void k_edge_intervals(AlphaK a, int E)
 {
+    if (big_pending(a)) return;
     GS(e, E) {
         int t = a.edge_t[e], i = a.edge_v[e];
         if (a.edge_att[e]) a.edge_rank[e] = 0;
@@ -1213,22 +1222,6 @@ __global__ void k_edge_intervals(AlphaK a, int E)
     }
 }
 
-__global__ void k_vnum_init(int2 *vnum, int n)
-{
-    GS(v, n) vnum[v] = make_int2(0x7fffffff, -1);
-}
-
-// vertex intervals from their side-by-side working copy; a vertex on no edge gets -1, -1
-__global__ void k_vert_fix(AlphaK a)
-{
-    GS(v, a.n) {
-        int2 w = a.vnum[v];
-        if (w.x == 0x7fffffff) w = make_int2(-1, -1);
-        a.vnumin[v] = w.x;
-        a.vnumax[v] = w.y;
-    }
-}
-
 // master keys: q < 2n vertices (2v: vnumin, 2v + 1: vnumax), < 2n + 3E edges (2n + 3e + 0/1/2: rank,
 // numin, numax), then triangles; entries without a rank get the key `none`, above every rank, so
 // they sort to the end.  One loop per simplex kind, so no index is divided (the host keeps
@@ -1241,10 +1234,16 @@ GA_INLINE unsigned master_key(const AlphaK &a, int r, int set, unsigned none)
 
 __global__ void k_master_keys(AlphaK a, int E, unsigned int *mkey, int *mval, unsigned int none)
 {
+    if (big_pending(a)) return;
     const int n2 = 2 * a.n, e0 = n2 + 3 * E;
     GS(v, a.n) {
+        // the vertex's interval from its working pair (a vertex on no edge: -1, -1)
+        int2 wp = a.vnum[v];
+        if (wp.x == 0x7fffffff) wp = make_int2(-1, -1);
+        a.vnumin[v] = wp.x;
+        a.vnumax[v] = wp.y;
         const int set = a.vset ? a.vset[v] : 0;
-        const int r0 = a.vnumin[v], r1 = a.vnumax[v];
+        const int r0 = wp.x, r1 = wp.y;
         *(uint2 *)&mkey[2 * v] = make_uint2(master_key(a, r0, set, none), master_key(a, r1, set, none));
         *(int2 *)&mval[2 * v] = make_int2(2 * v, 2 * v + 1);
     }
@@ -1308,9 +1307,10 @@ __global__ void k_master_decode(int n, int E, int M, unsigned int none, const un
 // least the entry of its own value, so the keys of set s are exactly specstart[s] + 1 ..
 // specstart[s + 1] - 1 (single set: 1 .. R), and key K of set s is distinct key number
 // K - s - 1: no scan over the entries is needed.  specstart[0] = 0.
-__global__ void k_spec_pointers(int M, unsigned int none, const unsigned int *skey, const int *specstart, int nsets,
-                                int *sp)
+__global__ void k_spec_pointers(AlphaK a, int M, unsigned int none, const unsigned int *skey, const int *specstart,
+                                int nsets, int *sp, int cap)
 {
+    if (big_pending(a)) return;
     GS(k, M) {
         const unsigned K = skey[k];
         if (K >= none || (k > 0 && skey[k - 1] == K)) continue;
@@ -1325,7 +1325,7 @@ __global__ void k_spec_pointers(int M, unsigned int none, const unsigned int *sk
             }
             s = lo;
         }
-        sp[(int)K - s] = k;
+        if ((int)K - s < cap) sp[(int)K - s] = k;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) sp[0] = -1;
 }
@@ -1333,10 +1333,13 @@ __global__ void k_spec_pointers(int M, unsigned int none, const unsigned int *sk
 // slots without a member (rank 0 of every set) read as 0; ~0 is a NaN pattern, never a value.
 // nslots < 0: a single set whose rank count R is incl[0] (the inclusive rank of the last entry),
 // nslots = R + 1
-__global__ void k_spec_values(const unsigned long long *bits, int nslots, const int *incl_last, double *spec)
+// also sets every vertex's working interval pair to (INT_MAX, -1) for k_edge_intervals
+__global__ void k_spec_values(const unsigned long long *bits, int nslots, const int *incl_last, double *spec,
+                              int2 *vnum, int n)
 {
     if (nslots < 0) nslots = *incl_last + 1;
     GS(r, nslots) spec[r] = bits[r] == ~0ull ? 0.0 : __longlong_as_double((long long)bits[r]);
+    GS(v, n) vnum[v] = make_int2(0x7fffffff, -1);
 }
 
 // first master entry of every set: keys are sorted and the keys of set s lie in
@@ -1416,15 +1419,27 @@ struct IntReads {
     const int *p[16];
     int n;
 };
-// into pinned host memory (the host waits on the stream before it reads)
-__global__ void k_gather_ints(IntReads r, int *host)
+// into pinned host memory (the host waits on the stream before it reads); with fold, the
+// statistics are folded from their stripes first and read as folded (one wave)
+__device__ int4 fold_stats(const AlphaK &a);
+__global__ void k_gather_ints(IntReads r, int *host, AlphaK a, int fold)
 {
+    const int4 f = fold ? fold_stats(a) : make_int4(0, 0, 0, 0);
     int i = threadIdx.x;
-    if (i < r.n) host[i] = *r.p[i];
+    if (i < r.n) {
+        const int *p = r.p[i];
+        int v = *p;
+        if (fold && p == a.counters + A_CLUSTERS) v = f.x;
+        if (fold && p == a.counters + A_MAXCLUSTER) v = f.y;
+        if (fold && p == a.counters + A_EXACT_CMP) v = f.z;
+        if (fold && p == a.counters + A_EXACT_ATT) v = f.w;
+        host[i] = v;
+    }
     __threadfence_system();
 }
 
-static void read_ints(AlphaDev &d, hipStream_t st, std::initializer_list<const int *> ptrs, int *out)
+static void read_ints(AlphaDev &d, hipStream_t st, std::initializer_list<const int *> ptrs, int *out,
+                      const AlphaK *fold = nullptr)
 {
     IntReads r{};
     for (const int *q : ptrs) {
@@ -1432,17 +1447,14 @@ static void read_ints(AlphaDev &d, hipStream_t st, std::initializer_list<const i
         r.p[r.n++] = q;
     }
     if (!d.hpin) chk(hipHostMalloc((void **)&d.hpin, 16 * sizeof(int), hipHostMallocDefault), "pinned alloc");
-    GA_LAUNCH(k_gather_ints, dim3(1), dim3(64), 0, st, r, d.hpin);
+    GA_LAUNCH(k_gather_ints, dim3(1), dim3(64), 0, st, r, d.hpin, fold ? *fold : AlphaK{}, fold ? 1 : 0);
     chk(hipStreamSynchronize(st), "sync");
     memcpy(out, d.hpin, 4 * (size_t)r.n);
 }
 
-static void big_clusters(AlphaDev &d, AlphaK &a, int S0, hipStream_t st)
+// the clusters above MAX_SMALL_CLUSTER entries (nbig of them, listed by k_clusters in bigstart)
+static void big_clusters(AlphaDev &d, AlphaK &a, int S0, int nbig, hipStream_t st)
 {
-    int cv[3];
-    read_ints(d, st, {a.counters + A_BIGCLUSTER, a.counters + A_EXCLU}, cv);
-    const int nbig = cv[0];
-    if (cv[1] > 0) GA_LAUNCH(k_clusters_exact, dim3(std::min(AGRID, (cv[1] + 255) / 256)), dim3(256), 0, st, a, S0);
     if (nbig == 0) return;
     if (nbig > a.bigcap) throw GalphaError("alpha: too many large clusters");
     std::vector<int> cs2(nbig), ce(nbig), sz2(nbig), off(nbig + 1, 0);
@@ -1691,7 +1703,20 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     ab(d, AB_XCLU, 4 * ((size_t)S0 / 2 + 1) + 64);
     a.xclu = AP<int>(d, AB_XCLU);
     GA_LAUNCH(k_clusters, dim3(AGRID), dim3(256), 0, st, a, S0);
-    big_clusters(d, a, S0, st);
+    // small clusters with overlapping intervals (the kernel reads their count)
+    GA_LAUNCH(k_clusters_exact, dim3(256), dim3(256), 0, st, a, S0);
+    // Large clusters: a batch reads their count here.  A single set goes on as if there were none
+    // and reads the count with its final counters; when there are some, it runs their exact
+    // order and everything after it again (the first pass's kernels below k_clusters return at
+    // once, but its library sorts and scans run).
+    if (nsets > 1) {
+        int nb = 0;
+        read_ints(d, st, {a.counters + A_BIGCLUSTER}, &nb);
+        big_clusters(d, a, S0, nb, st);
+    }
+    bool big_done = nsets > 1;
+    for (;;) {
+    a.skip_if_big = big_done ? 0 : 1;
     // ranks: inclusive scan of group starts.  A single set reads its rank count R with the final
     // counters (buffers are sized by R <= S0); a batch reads it here.
     launch_exclusive_scan(a.gflag, a.rank, S0, d.b[AB_SCAN].p, st);
@@ -1707,9 +1732,10 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
         R = h4[0] + h4[1];
         nslots = R + 1;
     }
-    // incl = excl + flag: reuse flag buffer as inclusive ranks
-    int *incl = a.gflag;
-    GA_LAUNCH(k_incl_from_excl, dim3(AGRID), dim3(256), 0, st, (const int *)a.rank, incl, S0);
+    // incl = excl + flag: single set into the free NEG buffer (gflag stays for a second pass),
+    // batches into the group-flag buffer
+    int *incl = nsets == 1 ? AP<int>(d, AB_NEG) : a.gflag;
+    GA_LAUNCH(k_incl_from_excl, dim3(AGRID), dim3(256), 0, st, (const int *)a.rank, (const int *)a.gflag, incl, S0);
     if (nsets == 1) {
         GA_LAUNCH(k_ranks, dim3(AGRID), dim3(256), 0, st, a, S0, (const int *)incl);
     } else {
@@ -1726,7 +1752,8 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
                   (const int *)incl, flags2);
         int *incl2 = pos;   // pos is free after the sort
         launch_exclusive_scan(flags2, incl2, S0, d.b[AB_SCAN].p, st);
-        GA_LAUNCH(k_incl_from_excl, dim3(AGRID), dim3(256), 0, st, (const int *)incl2, flags2, S0);
+        GA_LAUNCH(k_incl_from_excl, dim3(AGRID), dim3(256), 0, st, (const int *)incl2, (const int *)flags2, flags2,
+                  S0);
         incl2 = flags2;
         launch_fill_i32(first, -1, 2 * (size_t)(nsets + 1), st);
         GA_LAUNCH(k_set_bounds, dim3(AGRID), dim3(256), 0, st, S0, (const unsigned *)sset2, (const int *)incl2,
@@ -1756,12 +1783,10 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     const int slot_bound = nsets == 1 ? S0 + 1 : nslots;   // spectrum slots, or a bound of them
     ab(d, AB_SPEC, 8 * (size_t)(slot_bound + 1));
     GA_LAUNCH(k_spec_values, dim3(AGRID), dim3(256), 0, st, (const unsigned long long *)a.specbits, nslots,
-              (const int *)(incl + S0 - 1), AP<double>(d, AB_SPEC));
+              (const int *)(incl + S0 - 1), AP<double>(d, AB_SPEC), a.vnum, n);
     chk(hipEventRecord(d.ev[2], st), "event");
-    // intervals
-    GA_LAUNCH(k_vnum_init, dim3(AGRID), dim3(256), 0, st, a.vnum, n);
+    // intervals (k_master_keys splits the vertex pairs into the output arrays)
     GA_LAUNCH(k_edge_intervals, dim3(AGRID), dim3(256), 0, st, a, E);
-    GA_LAUNCH(k_vert_fix, dim3(AGRID), dim3(256), 0, st, a);
     // master list
     long long M = 2LL * n + 3LL * E + F;
     // (grid-stride loops over int indices: keep the last index plus the grid below 2^31)
@@ -1796,8 +1821,8 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     // entry of its own value, so there are slot_bound - nsets of them (read with the final
     // counters for a single set)
     ab(d, AB_SP, 4 * (size_t)(slot_bound + 2));
-    GA_LAUNCH(k_spec_pointers, dim3(AGRID), dim3(256), 0, st, (int)M, none, (const unsigned *)skey,
-              (const int *)a.specstart, nsets, AP<int>(d, AB_SP));
+    GA_LAUNCH(k_spec_pointers, dim3(AGRID), dim3(256), 0, st, a, (int)M, none, (const unsigned *)skey,
+              (const int *)a.specstart, nsets, AP<int>(d, AB_SP), slot_bound + 2);
     int Mk = -1;   // read below
     if (nsets > 1) {
         // first master entry and first edge of every set
@@ -1820,7 +1845,6 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     } else {
         d.efirst = {0, E};
     }
-    GA_LAUNCH(k_fold_stats, dim3(1), dim3(64), 0, st, a);
     chk(hipEventRecord(d.ev[3], st), "event");
     // counters, status, the single set's rank count and the ranked master entries in one read
     int fin[11];
@@ -1828,7 +1852,12 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     read_ints(d, st,
               {a.counters + 0, a.counters + 1, a.counters + 2, a.counters + 3, a.counters + 4, a.counters + 5,
                a.counters + 6, a.counters + 7, a.status, nsets == 1 ? incl + S0 - 1 : zero, pMk},
-              fin);
+              fin, &a);   // the statistics folded on the way
+    if (!big_done && fin[A_BIGCLUSTER] > 0) {
+        big_clusters(d, a, S0, fin[A_BIGCLUSTER], st);
+        big_done = true;
+        continue;
+    }
     Mk = fin[10];
     d.M = Mk;
     if (nsets == 1) d.mfirst = {0, Mk};
@@ -1856,6 +1885,8 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     }
     if (hs) throw GalphaError("alpha: device status " + std::to_string(hs) +
                               (hc[A_BIGCLUSTER] ? " (exact cluster larger than supported)" : ""));
+    return;
+    }
 }
 
 PGPUDTALPHA alpha_copy_to_host(AlphaDev &d, HostStage &stage, hipStream_t st)
