@@ -947,18 +947,18 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     {
         // worklists in the flip lists (free until the flips), marks in decide[] (free until the
         // greedy rounds)
+        // Lists alternate between two buffers; their counts rotate through three slots, so each
+        // pass zeroes the next pass's output count (the slot the pass before it read) itself.
         unsigned char *moved = c->moved.as<unsigned char>();
-        HIPCHK(hipMemsetAsync(m.decide, 0, 4 * (size_t)V, st));
-        int *wl_in = m.modA, *wl_out = m.modB, sin = C_MODN, sout = C_MODN2;
-        launch_shift_bulk(m, moved, m.decide, wl_in, sin, st);
+        int *wl[2] = {m.modA, m.modB};
+        const int slot[3] = {C_MODN, C_MODN2, C_MODN3};
+        launch_shift_bulk(m, moved, m.decide, wl[0], slot[0], slot[1], st);   // (zeroes decide[])
         for (int it = 0;; it += 4) {
             if (it > 100000) throw GalphaError("bulk shifting did not converge");
-            for (int k = 0; k < 4; k++) {
-                launch_shift_star(m, moved, m.decide, wl_in, sin, wl_out, sout, st);
-                std::swap(wl_in, wl_out);
-                std::swap(sin, sout);
-            }
-            if (ctx_counter(c, sin) == 0) break;
+            for (int k = it; k < it + 4; k++)
+                launch_shift_star(m, moved, m.decide, wl[k & 1], slot[k % 3], wl[(k + 1) & 1], slot[(k + 1) % 3],
+                                  slot[(k + 2) % 3], st);
+            if (ctx_counter(c, slot[(it + 4) % 3]) == 0) break;
         }
     }
     launch_shift_commit(m, c->moved.as<unsigned char>(), st);

@@ -158,6 +158,7 @@ struct MeshDev {
 #define C_TAIL_PASSES 35      // passes run by the last single-workgroup flip tail
 #define C_TAIL_DONE 36        // 1 when that tail ended with a pass without flips
 #define C_TAIL_LISTED 37      // triangles listed by flip tails (also counted in C_FLIP_LISTED)
+#define C_MODN3 38            // third list count of the bulk shifting passes (with C_MODN, C_MODN2)
 #define TAIL_LIST 2048        // longest flip list the single-workgroup tail takes
 // status bits of constraint insertion (the other GA_ERR_* bits are in dev_common.h)
 #define GA_ERR_CONS_POINT 128     // a point lies inside a constraint segment
@@ -224,9 +225,10 @@ void launch_adjacency(MeshDev &m, int *deg, int *off, int *fill, int2 *fan, void
 void launch_init_positions(MeshDev &m, hipStream_t st);
 // bulk shifting: one pass over all triangles (reverted vertices listed in wl_out, mark[] zero on
 // entry), then passes over the stars of the vertices the previous pass listed
-void launch_shift_bulk(MeshDev &m, unsigned char *moved, int *mark, int *wl_out, int cnt_out_slot, hipStream_t st);
+void launch_shift_bulk(MeshDev &m, unsigned char *moved, int *mark, int *wl_out, int cnt_out_slot,
+                       int cnt_next_slot, hipStream_t st);
 void launch_shift_star(MeshDev &m, unsigned char *moved, int *mark, const int *wl_in, int cnt_in_slot, int *wl_out,
-                       int cnt_out_slot, hipStream_t st);
+                       int cnt_out_slot, int cnt_next_slot, hipStream_t st);
 // lists the vertices still pending into m.plist (count in counters[C_PLIST])
 void launch_pending_list(MeshDev &m, hipStream_t st);
 void launch_shift_commit(MeshDev &m, const unsigned char *moved, hipStream_t st);

@@ -442,11 +442,16 @@ GA_INLINE double2 cand_pos(const MeshDev &m, const unsigned char *moved, int v)
 // triangle does.  Reverting only ever clears moved[], so the first pass looks at every triangle
 // and later passes only at the stars of the vertices reverted by the pass before (a worklist;
 // mark[] lists each vertex once).
-// also zeroes the count of k_shift_bulk's output list
-__global__ void k_shift_init(MeshDev m, unsigned char *moved, int cnt_out_slot)
+// also zeroes the revert marks, the count of k_shift_bulk's output list and that of the first
+// star pass's
+__global__ void k_shift_init(MeshDev m, unsigned char *moved, int *mark, int cnt_out_slot, int cnt_next_slot)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) m.counters[cnt_out_slot] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        m.counters[cnt_out_slot] = 0;
+        m.counters[cnt_next_slot] = 0;
+    }
     GRID_STRIDE(v, m.n) moved[v] = (m.vstate[v] == VS_PENDING) ? 1 : 0;
+    GRID_STRIDE(v, m.V) mark[v] = 0;
 }
 
 // test triangle v (vertex ids) with the current choice; revert and list its moved vertices when
@@ -485,9 +490,12 @@ __global__ void k_shift_bulk(MeshDev m, unsigned char *moved, int *mark, int *wl
     }
 }
 
+// cnt_next_slot: the output count of the next pass (neither this pass's input nor its output),
+// zeroed here so that no launch is needed between passes
 __global__ void k_shift_star(MeshDev m, unsigned char *moved, int *mark, const int *wl_in, int cnt_in_slot,
-                             int *wl_out, int cnt_out_slot)
+                             int *wl_out, int cnt_out_slot, int cnt_next_slot)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) m.counters[cnt_next_slot] = 0;
     int nw = m.counters[cnt_in_slot];
     GRID_STRIDE(k, nw) {
         int v = wl_in[k];
@@ -503,18 +511,18 @@ __global__ void k_shift_star(MeshDev m, unsigned char *moved, int *mark, const i
     }
 }
 
-void launch_shift_bulk(MeshDev &m, unsigned char *moved, int *mark, int *wl_out, int cnt_out_slot, hipStream_t st)
+void launch_shift_bulk(MeshDev &m, unsigned char *moved, int *mark, int *wl_out, int cnt_out_slot,
+                       int cnt_next_slot, hipStream_t st)
 {
-    GA_LAUNCH(k_shift_init, dim3(FIXED_GRID), dim3(256), 0, st, m, moved, cnt_out_slot);
+    GA_LAUNCH(k_shift_init, dim3(FIXED_GRID), dim3(256), 0, st, m, moved, mark, cnt_out_slot, cnt_next_slot);
     GA_LAUNCH(k_shift_bulk, dim3(FIXED_GRID), dim3(256), 0, st, m, moved, mark, wl_out, cnt_out_slot);
 }
 
 void launch_shift_star(MeshDev &m, unsigned char *moved, int *mark, const int *wl_in, int cnt_in_slot, int *wl_out,
-                       int cnt_out_slot, hipStream_t st)
+                       int cnt_out_slot, int cnt_next_slot, hipStream_t st)
 {
-    launch_reset_counters(m.counters, cnt_out_slot, -1, -1, -1, st);
     GA_LAUNCH(k_shift_star, dim3(FIXED_GRID), dim3(256), 0, st, m, moved, mark, wl_in, cnt_in_slot, wl_out,
-              cnt_out_slot);
+              cnt_out_slot, cnt_next_slot);
 }
 
 __global__ void k_shift_commit(MeshDev m, const unsigned char *moved)

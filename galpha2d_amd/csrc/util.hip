@@ -12,12 +12,12 @@ GA_INLINE void block_box(double &a, double &b, double &c, double &d, double &bad
         c = fmax(c, __shfl_down(c, s, 64)); d = fmax(d, __shfl_down(d, s, 64));
         bad = fmax(bad, __shfl_down(bad, s, 64));
     }
-    __shared__ double sh[4][5];
+    __shared__ double sh[16][5];
     int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) { sh[w][0] = a; sh[w][1] = b; sh[w][2] = c; sh[w][3] = d; sh[w][4] = bad; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int k = 1; k < 4; k++) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
             a = fmin(a, sh[k][0]); b = fmin(b, sh[k][1]); c = fmax(c, sh[k][2]); d = fmax(d, sh[k][3]);
             bad = fmax(bad, sh[k][4]);
         }
@@ -30,12 +30,21 @@ GA_INLINE void box_add(double2 p, double &a, double &b, double &c, double &d, do
     a = fmin(a, p.x); b = fmin(b, p.y); c = fmax(c, p.x); d = fmax(d, p.y);
 }
 
-// one set, many blocks: partial boxes out5[5 * block]
-__global__ __launch_bounds__(256) void k_bbox(const double2 *__restrict__ xy, int n, double *out5)
+// one set, 64 blocks of 1,024 threads: partial boxes out5[5 * block]; four loads in flight per
+// thread (the box is read before anything else can run, so its latency is all exposed)
+__global__ __launch_bounds__(1024) void k_bbox(const double2 *__restrict__ xy, int n, double *out5)
 {
     double a = INFINITY, b = INFINITY, c = -INFINITY, d = -INFINITY, bad = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        box_add(xy[i], a, b, c, d, bad);
+    const int stride = gridDim.x * blockDim.x;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const double2 p0 = xy[i], p1 = xy[i + stride], p2 = xy[i + 2 * stride], p3 = xy[i + 3 * stride];
+        box_add(p0, a, b, c, d, bad);
+        box_add(p1, a, b, c, d, bad);
+        box_add(p2, a, b, c, d, bad);
+        box_add(p3, a, b, c, d, bad);
+    }
+    for (; i < n; i += stride) box_add(xy[i], a, b, c, d, bad);
     block_box(a, b, c, d, bad);
     if (threadIdx.x == 0) {
         double *o = out5 + 5 * blockIdx.x;
@@ -62,7 +71,7 @@ void bbox_device(const double *d_xy, int n, double *out5, void *tmp, hipStream_t
 {
     const int blocks = 64;
     double *part = (double *)tmp;
-    GA_LAUNCH(k_bbox, dim3(blocks), dim3(256), 0, st, (const double2 *)d_xy, n, part);
+    GA_LAUNCH(k_bbox, dim3(blocks), dim3(1024), 0, st, (const double2 *)d_xy, n, part);
     double h[5 * blocks];
     hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, st);
     hipStreamSynchronize(st);
