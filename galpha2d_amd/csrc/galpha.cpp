@@ -655,14 +655,19 @@ static void run_pba(galpha_ctx *c, const double *d_xy, const BatchDev &bd, int *
     launch_fill_i32(c->smap.as<int>(), 0x7fffffff, bd.P, st);
     launch_vset(bd, c->vset.as<int>(), st);
     launch_splat((const double2 *)d_xy, bd, perm, c->smap.as<int>(), c->pix.as<int>(), c->fpix.as<int>(), st);
-    HIPCHK(hipMemsetAsync(c->bad.p, 0, 4 * (size_t)(bd.V + 1), st));
+    {
+        IntFills f;   // island marks, and the island counters of the first pass
+        f.add(c->bad.p, (size_t)bd.V + 1, 0);
+        f.add(c->counters.as<int>() + C_ISLAND_SITES, 1, 0);
+        f.add(c->counters.as<int>() + C_FLAG, 1, 0);
+        launch_fill_multi(f, st);
+    }
     PbaBuffers b{c->smap.as<int>(), c->g.as<short>(), c->S.as<int>(), c->cnt.as<int>(), c->plo.as<int>(),
                  c->E.as<int>(), c->ecnt.as<int>(), c->vor.as<int>()};
     int it = 0, isl = 0;
-    ctx_counter_set(c, C_ISLAND_SITES, 0);
     for (; it < 16; it++) {
         launch_pba(b, bd, st);
-        ctx_counter_set(c, C_FLAG, 0);
+        if (it > 0) ctx_counter_set(c, C_FLAG, 0);
         launch_islands(c->vor.as<int>(), c->pix.as<int>(), bd, c->bad.as<int>(), c->counters.as<int>() + C_FLAG, st);
         int cv[C_ISLAND_SITES + 1];   // the flag and the island count in one read
         ctx_counters_read(c, cv, 0, C_ISLAND_SITES + 1);
@@ -1076,7 +1081,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     // Delaunay): the walks and splits only need a valid triangulation, and one flip phase over
     // all triangles afterwards makes the mesh Delaunay, instead of flip passes after every
     // insertion round.
-    ctx_counter_set(c, C_MISS, 0);
+    // (C_MISS is zero since the run's start: only the missing stage counts it)
     launch_missing_init(m, c->vor.as<int>(), c->mlist.as<int>(), c->loc.as<int>(), st);
     last = ctx_snapshot(c);
     int nmiss = last.v[C_MISS];
@@ -1089,8 +1094,7 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
             throw GalphaError("missing-site insertion failed (status " + std::to_string(last.status()) + ")");
         if (r > 5000) throw GalphaError("missing-site insertion did not finish");
         int opn = c->op++;
-        ctx_counter_set(c, C_MODN, 0);
-        launch_insert_round(m, c->mlist.as<int>(), c->loc.as<int>(), opn, round_grid(nmiss), st);
+        launch_insert_round(m, c->mlist.as<int>(), c->loc.as<int>(), opn, round_grid(nmiss), st);   // (zeroes C_MODN)
         launch_relink(m, m.modA, C_MODN, opn, round_grid(4LL * left), st);
         S.insertRounds++;
         last = ctx_snapshot(c);

@@ -844,8 +844,11 @@ void launch_missing_init(MeshDev &m, const int *vor, int *mlist, int *loc, hipSt
 }
 
 // located: lt = (type << 2) | edge ; type 1 inside, 2 on edge, 3 duplicate ; -1 not located
+// also zeroes the modified-slot count that k_insert of this round fills (the previous round's
+// relink, which read it, has finished)
 __global__ void k_locate(MeshDev m, const int *mlist, int *loc, int *lt, int op)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) m.counters[C_MODN] = 0;
     int nm = m.counters[C_MISS];
     GRID_STRIDE(k, nm) {
         int v = mlist[k];
