@@ -977,18 +977,17 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     // Later rounds move few vertices each and cost more than deleting and reinserting them:
     // on 1M uniform points the shifting stage took 1.68 ms with up to 48 rounds, 1.40 ms with 8.
     const int pgrid = round_grid(pend);   // the pending list keeps its length (done entries are skipped)
-    for (int r = 0; r < 8 && pend > 0; r += 4) {
-        for (int k = 0; k < 4; k++) launch_shift_greedy(m, r + k, pgrid, st);
+    // (all eight before one read: a round after convergence moves nothing)
+    if (pend > 0) {
+        for (int k = 0; k < 8; k++) launch_shift_greedy(m, k, pgrid, st);
         int moved_total = ctx_stat(c, C_MOVED);
-        int movedn = moved_total - moved_before;
+        pend -= moved_total - moved_before;
         moved_before = moved_total;
-        pend -= movedn;
-        if (movedn == 0) break;
     }
     int deleted_before = 0;
-    for (int r = 0; pend > 0; r += 2) {
+    for (int r = 0; pend > 0; r += 3) {   // three rounds per read (C2 takes six)
         if (r > 10000) throw GalphaError("vertex deletion did not finish");
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < 3; k++) {
             // (k_del_prio zeroes C_SCR and C_MODN)
             launch_delete_round(m, c->mx.as<unsigned int>(), next_prio_round(), c->op++, pgrid, st);
         }
@@ -1093,10 +1092,13 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
         if (last.status())
             throw GalphaError("missing-site insertion failed (status " + std::to_string(last.status()) + ")");
         if (r > 5000) throw GalphaError("missing-site insertion did not finish");
-        int opn = c->op++;
-        launch_insert_round(m, c->mlist.as<int>(), c->loc.as<int>(), opn, round_grid(nmiss), st);   // (zeroes C_MODN)
-        launch_relink(m, m.modA, C_MODN, opn, round_grid(4LL * left), st);
-        S.insertRounds++;
+        // two rounds per read (a round after the last insertion finds every entry done)
+        for (int k = 0; k < 2; k++) {
+            int opn = c->op++;
+            launch_insert_round(m, c->mlist.as<int>(), c->loc.as<int>(), opn, round_grid(nmiss), st);   // (zeroes C_MODN)
+            launch_relink(m, m.modA, C_MODN, opn, round_grid(4LL * left), st);
+            S.insertRounds++;
+        }
         last = ctx_snapshot(c);
     }
     S.inserted = last.stat(C_INSERTED);
