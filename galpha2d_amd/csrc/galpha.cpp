@@ -1028,26 +1028,33 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     // back to back without launches or host reads until one flips nothing.
     auto flip_until_done = [&](int *listA, int cntA, int *listB, int cntB, int prev, long long hint) {
         int *lin = listA, *lout = listB, sin = cntA, sout = cntB;
+        int pass_count = 0;
         int grid = hint >= 0 ? round_grid(2 * hint) : 2048;
         long long next = hint;   // length of the next pass's list, when known
         launch_reset_counters(c->counters.as<int>(), C_CAND, -1, -1, -1, st);
-        for (int pass = 0; pass < 100000;) {
+        const int maxp = 512;
+        // the tail's passes after a read: returns true when the flips are done
+        auto tail_result = [&](int op0) {
+            const int p = last.v[C_TAIL_PASSES];
+            c->op = op0 + p;   // tags of passes that did not run are not used
+            S.flipPasses += p;
+            S.flipTailPasses += p;
+            pass_count += p;
+            if (p & 1) {
+                std::swap(lin, lout);
+                std::swap(sin, sout);
+            }
+            if (p > 0) prev = op0 + p - 1;
+            return last.v[C_TAIL_DONE] != 0;
+        };
+        for (pass_count = 0; pass_count < 100000;) {
             if (next >= 0 && next <= TAIL_LIST) {
-                const int maxp = 512;
                 const int op0 = c->op;
                 c->op += maxp;
                 launch_flip_tail(m, lin, sin, lout, sout, prev, op0, maxp, st);
                 last = ctx_snapshot(c);
                 const int p = last.v[C_TAIL_PASSES];
-                S.flipPasses += p;
-                S.flipTailPasses += p;
-                pass += p;
-                if (p & 1) {
-                    std::swap(lin, lout);
-                    std::swap(sin, sout);
-                }
-                if (p > 0) prev = op0 + p - 1;
-                if (last.v[C_TAIL_DONE]) return;
+                if (tail_result(op0)) return;
                 next = last.v[sin];
                 grid = round_grid(2LL * next);
                 if (p == maxp && next <= TAIL_LIST) continue;
@@ -1061,10 +1068,24 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
                 std::swap(sin, sout);
             }
             S.flipPasses += 3;
-            pass += 3;
+            pass_count += 3;
+            // the tail right behind the grid passes, before the read: it returns at once while the
+            // list is longer than TAIL_LIST (no passes), else it usually finishes the flips
+            const int op0 = c->op;
+            c->op += maxp;
+            launch_flip_tail(m, lin, sin, lout, sout, prev, op0, maxp, st);
             last = ctx_snapshot(c);
-            if (last.v[C_FLIPS] == 0) return;
-            next = last.v[sin];
+            const int p = last.v[C_TAIL_PASSES];
+            if (p == 0) {
+                c->op = op0;
+                if (last.v[C_FLIPS] == 0) return;
+                next = last.v[sin];
+            } else {
+                if (tail_result(op0)) return;
+                next = last.v[sin];
+                if (p == maxp && next <= TAIL_LIST) continue;
+                next = TAIL_LIST + 1;
+            }
             grid = round_grid(2LL * next);
         }
         throw GalphaError("flipping did not converge");
