@@ -1631,9 +1631,13 @@ GA_INLINE bool is_frame_tri(const MeshDev &m, int4 v) { return v.x >= m.n || v.y
 
 // boundary edges of R (a -> b with R on the left), their count and smallest tail per set; a vertex
 // with two outgoing boundary edges (a pinch) or a meshed real point in no R triangle marks its set
+// Each set's boundary-edge count and smallest boundary vertex are accumulated per thread (while
+// the set stays the same) and added once per wave when the wave's edges share one set: atomics
+// of every boundary edge on the set's two words would serialise.
 __global__ void k_bnd_edges(MeshDev m, int *outdeg, int *cover, int *hull_cnt, int *hull_start, int *badset)
 {
     int n = ntri_used(m);
+    int cs = -1, cc = 0, cmin = 0x7fffffff;   // pending count and minimum of set cs
     GRID_STRIDE(t, n) {
         int4 v = m.tv[t];
         if (v.x == GA_DEAD || is_frame_tri(m, v)) continue;
@@ -1648,9 +1652,38 @@ __global__ void k_bnd_edges(MeshDev m, int *outdeg, int *cover, int *hull_cnt, i
             m.hedge[a] = enc(t, i);
             int s = m.vset[a];
             if (atomicAdd(&outdeg[a], 1) != 0) badset[s] = 1;
-            atomicMin(&hull_start[s], a);
-            atomicAdd(&hull_cnt[s], 1);
+            if (s != cs) {
+                if (cc) {
+                    atomicMin(&hull_start[cs], cmin);
+                    atomicAdd(&hull_cnt[cs], cc);
+                }
+                cs = s;
+                cc = 0;
+                cmin = 0x7fffffff;
+            }
+            cc++;
+            cmin = min(cmin, a);
         }
+    }
+    // every lane of the wave is here
+    const bool has = cc > 0;
+    const unsigned long long bal = __ballot(has);
+    if (!bal) return;
+    const int lane = threadIdx.x & 63, leader = __ffsll((long long)bal) - 1;
+    const int s0 = __shfl(cs, leader, 64);
+    if (__all(!has || cs == s0)) {
+        int sum = has ? cc : 0, mn = has ? cmin : 0x7fffffff;
+        for (int d = 32; d > 0; d >>= 1) {
+            sum += __shfl_xor(sum, d, 64);
+            mn = min(mn, __shfl_xor(mn, d, 64));
+        }
+        if (lane == leader) {
+            atomicMin(&hull_start[s0], mn);
+            atomicAdd(&hull_cnt[s0], sum);
+        }
+    } else if (has) {
+        atomicMin(&hull_start[cs], cmin);
+        atomicAdd(&hull_cnt[cs], cc);
     }
 }
 
@@ -1824,10 +1857,13 @@ GA_INLINE int *ear_cend(const EarCycle &e, int off) { return e.cells + 2 * (size
 GA_INLINE int *ear_citem(const EarCycle &e, int off) { return e.cells + 3 * (size_t)e.B + off; }
 
 #define EAR_THREADS 1024
+#define EAR_LDS_CELLS 16384   // grid cells k_ear_grid counts in LDS (64 KB); larger grids count in memory
 // The grid of the set of this workgroup (h positions from off): bounding box, counting sort of the
 // positions by cell (counts, exclusive starts, then a fill that counts each start up to its cell's
 // end: cell c holds [c ? cend[c - 1] : 0, cend[c])), parameters stored for other kernels.
-GA_INLINE void ear_build_grid(const EarCycle &e, int s, int h, int off)
+// lcells: LDS cells of the calling kernel (NULL: none); the counting sort runs there when the
+// grid fits in lcap cells, and its result is copied out
+GA_INLINE void ear_build_grid(const EarCycle &e, int s, int h, int off, int *lcells = nullptr, int lcap = 0)
 {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     __shared__ double red[4][EAR_THREADS / 64];
@@ -1856,17 +1892,18 @@ GA_INLINE void ear_build_grid(const EarCycle &e, int s, int h, int off)
         double *pr = e.param + 5 * (size_t)s;
         pr[0] = lx; pr[1] = ly; pr[2] = sx; pr[3] = sy; pr[4] = (double)G;
     }
-    for (int c = tid; c < ncell; c += EAR_THREADS) cend[c] = 0;
+    int *cnt = (lcells && ncell <= lcap) ? lcells : cend;
+    for (int c = tid; c < ncell; c += EAR_THREADS) cnt[c] = 0;
     __syncthreads();
     for (int k = tid; k < h; k += EAR_THREADS) {
         double2 q = e.bpos[off + k];
-        atomicAdd(&cend[ear_cell(q.y, ly, sy, G) * G + ear_cell(q.x, lx, sx, G)], 1);
+        atomicAdd(&cnt[ear_cell(q.y, ly, sy, G) * G + ear_cell(q.x, lx, sx, G)], 1);
     }
     __syncthreads();
     {
         int per = (ncell + EAR_THREADS - 1) / EAR_THREADS, c0 = tid * per, c1 = min(ncell, c0 + per);
         int sum = 0;
-        for (int c = c0; c < c1; c++) sum += cend[c];
+        for (int c = c0; c < c1; c++) sum += cnt[c];
         int inc = sum;
         for (int d = 1; d < 64; d <<= 1) {
             int t = __shfl_up(inc, d, 64);
@@ -1877,8 +1914,8 @@ GA_INLINE void ear_build_grid(const EarCycle &e, int s, int h, int off)
         int base = inc - sum;
         for (int w = 0; w < wv; w++) base += s_total[w];
         for (int c = c0; c < c1; c++) {
-            int v = cend[c];
-            cend[c] = base;
+            int v = cnt[c];
+            cnt[c] = base;
             base += v;
         }
     }
@@ -1886,9 +1923,11 @@ GA_INLINE void ear_build_grid(const EarCycle &e, int s, int h, int off)
     for (int k = tid; k < h; k += EAR_THREADS) {
         double2 q = e.bpos[off + k];
         int c = ear_cell(q.y, ly, sy, G) * G + ear_cell(q.x, lx, sx, G);
-        citem[atomicAdd(&cend[c], 1)] = off + k;
+        citem[atomicAdd(&cnt[c], 1)] = off + k;
     }
     __syncthreads();
+    if (cnt != cend)
+        for (int c = tid; c < ncell; c += EAR_THREADS) cend[c] = cnt[c];
 }
 
 // 1 when live position p of set s is reflex and its ear is empty
@@ -1963,9 +2002,10 @@ GA_INLINE bool ear_cut(const MeshDev &m, const EarCycle &e, int p, int *list_out
 __global__ __launch_bounds__(EAR_THREADS) void k_ear_grid(MeshDev m, int B, int *bnd, const int *hull_cnt,
                                                           const int *hull_off)
 {
+    __shared__ int s_cells[EAR_LDS_CELLS];
     int s = blockIdx.x, h = hull_cnt[s];
     if (h < 3) return;
-    ear_build_grid(ear_cycle(bnd, B), s, h, hull_off[s]);
+    ear_build_grid(ear_cycle(bnd, B), s, h, hull_off[s], s_cells, EAR_LDS_CELLS);
 }
 
 // Grid-wide rounds: round r counts its candidates in counters[C_EAR + ...] by parity; a round
