@@ -319,10 +319,83 @@ GA_INLINE int incircle_sos_tie(double2 a, int ia, double2 b, int ib, double2 c, 
     return 0;
 }
 
+// Exact zero test of the incircle determinant by residues: 1 when it is zero, 0 when it is not,
+// -1 when the coordinates need more than 60 bits at their common scale (then the big-integer
+// path decides).  Each coordinate is m * 2^e with m odd; dropping the common factor 2^t (t the
+// smallest e) changes neither the sign nor the zero-ness, and leaves integers X below 2^B.
+// Then |det| < 2^(4B + 8), and a determinant that vanishes modulo primes whose product exceeds
+// that bound is zero (the primes lie just below 2^31, so residue products fit 64 bits).  Snapped
+// coordinates are multiples of the snapping unit, so B <= 52 and eight primes suffice.
+// Co-circular lattice quadruples, the exact path's common case, skip the big integers.
+__host__ __device__ __forceinline__ int incircle_zero_res(double2 a, double2 b, double2 c, double2 d)
+{
+    const double v[8] = {a.x, a.y, b.x, b.y, c.x, c.y, d.x, d.y};
+    unsigned long long mm[8];
+    int ex[8], t = 1 << 30;
+    for (int i = 0; i < 8; i++) {
+        mm[i] = 0;
+        ex[i] = 0;
+        if (v[i] == 0.0) continue;
+        int e;
+        const double f = frexp(fabs(v[i]), &e);            // |v| = f * 2^e, 0.5 <= f < 1
+        unsigned long long m = (unsigned long long)ldexp(f, 53);   // exact, below 2^53
+        int x = e - 53;
+        const int tz = __builtin_ctzll(m);
+        m >>= tz;
+        x += tz;
+        mm[i] = m;
+        ex[i] = x;
+        t = x < t ? x : t;
+    }
+    if (t == (1 << 30)) return 1;   // every coordinate zero
+    long long X[8];
+    int B = 1;
+    for (int i = 0; i < 8; i++) {
+        X[i] = 0;
+        if (mm[i] == 0) continue;
+        const int k = ex[i] - t;
+        const int bl = 64 - __builtin_clzll(mm[i]) + k;
+        if (bl > 60) return -1;
+        X[i] = (long long)(mm[i] << k);
+        if (v[i] < 0.0) X[i] = -X[i];
+        B = bl > B ? bl : B;
+    }
+    const int np = (4 * B + 8 + 29) / 30;   // primes > 2^30 each
+    const long long dx[3] = {X[0] - X[6], X[2] - X[6], X[4] - X[6]};
+    const long long dy[3] = {X[1] - X[7], X[3] - X[7], X[5] - X[7]};
+    constexpr long long P[9] = {2147483647LL, 2147483629LL, 2147483587LL, 2147483579LL, 2147483563LL,
+                                2147483549LL, 2147483543LL, 2147483497LL, 2147483489LL};
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        if (q >= np) break;
+        const long long p = P[q];
+        unsigned long long rx[3], ry[3], lf[3];
+        for (int k = 0; k < 3; k++) {
+            long long r = dx[k] % p;
+            rx[k] = (unsigned long long)(r < 0 ? r + p : r);
+            r = dy[k] % p;
+            ry[k] = (unsigned long long)(r < 0 ? r + p : r);
+            lf[k] = (rx[k] * rx[k] + ry[k] * ry[k]) % (unsigned long long)p;
+        }
+        unsigned long long det = 0;
+        for (int k = 0; k < 3; k++) {
+            const int i = (k + 1) % 3, j = (k + 2) % 3;
+            // minor = rx[i] * ry[j] - rx[j] * ry[i] (mod p)
+            const unsigned long long t1 = (rx[i] * ry[j]) % (unsigned long long)p;
+            const unsigned long long t2 = (rx[j] * ry[i]) % (unsigned long long)p;
+            const unsigned long long mn = t1 >= t2 ? t1 - t2 : t1 + (unsigned long long)p - t2;
+            det = (det + lf[k] * mn) % (unsigned long long)p;
+        }
+        if (det != 0) return 0;
+    }
+    return 1;
+}
+
 GA_INLINE int incircle_exact_sos(double2 a, int ia, double2 b, int ib, double2 c, int ic,
                                  double2 d, int id, int *ovf)
 {
-    int s = incircle_exact(a, b, c, d, ovf);
+    // a determinant the residues show to be zero skips the big integers
+    int s = incircle_zero_res(a, b, c, d) == 1 ? 0 : incircle_exact(a, b, c, d, ovf);
     if (s != 0) return s;
     return incircle_sos_tie(a, ia, b, ib, c, ic, d, id, ovf);
 }
