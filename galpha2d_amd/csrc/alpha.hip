@@ -81,6 +81,7 @@ GA_INLINE void bi_sq_dist(BI &r, const BI &ax, const BI &ay, const BI &bx, const
 // entries: st 0 = edge (p, q), 1 = triangle (a, b, c).  Returns sign(rho_A - rho_B).
 __device__ __noinline__ int rho_cmp_exact(int sta, const double2 *pa, int stb, const double2 *pb, int *ovf)
 {
+    if (rho_equal_res(sta, pa, stb, pb) == 1) return 0;   // equal values skip the big integers
     double v[12];
     int na = sta ? 3 : 2, nb = stb ? 3 : 2, k = 0;
     for (int i = 0; i < na; i++) { v[k++] = pa[i].x; v[k++] = pa[i].y; }
@@ -124,6 +125,7 @@ __device__ __noinline__ int rho_cmp_exact(int sta, const double2 *pa, int stb, c
 // attachment: exact sign of |i-j|^2 - |i+j-2k|^2 (>= 0 attached)
 __device__ __noinline__ int attached_exact(double2 i, double2 j, double2 k, int *ovf)
 {
+    if (attached_zero_res(i, j, k) == 1) return 1;   // on the circle: attached (>= 0)
     double v[6] = {i.x, i.y, j.x, j.y, k.x, k.y};
     int s = min_ulp_exp(v, 6) - 1;   // -1: 2k stays an integer multiple of the unit
     BigInt<16> ix, iy, jx, jy, kx, ky, d1, sx, sy, t;

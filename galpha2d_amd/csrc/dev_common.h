@@ -319,74 +319,186 @@ GA_INLINE int incircle_sos_tie(double2 a, int ia, double2 b, int ib, double2 c, 
     return 0;
 }
 
-// Exact zero test of the incircle determinant by residues: 1 when it is zero, 0 when it is not,
-// -1 when the coordinates need more than 60 bits at their common scale (then the big-integer
-// path decides).  Each coordinate is m * 2^e with m odd; dropping the common factor 2^t (t the
-// smallest e) changes neither the sign nor the zero-ness, and leaves integers X below 2^B.
-// Then |det| < 2^(4B + 8), and a determinant that vanishes modulo primes whose product exceeds
-// that bound is zero (the primes lie just below 2^31, so residue products fit 64 bits).  Snapped
-// coordinates are multiples of the snapping unit, so B <= 52 and eight primes suffice.
-// Co-circular lattice quadruples, the exact path's common case, skip the big integers.
-__host__ __device__ __forceinline__ int incircle_zero_res(double2 a, double2 b, double2 c, double2 d)
+// ---- exact zero tests by residues --------------------------------------------------------
+// Coordinates as integers: each is m * 2^e with m odd; dropping the common factor 2^t (t the
+// smallest e) changes neither the sign nor the zero-ness of a homogeneous polynomial in them, and
+// leaves integers X below 2^B (B returned).  False when some X would need more than maxb bits.
+#define RES_NP 20   // primes just below 2^31 (each above 2^30): residue products fit 64 bits
+__host__ __device__ __forceinline__ long long res_prime(int q)
 {
-    const double v[8] = {a.x, a.y, b.x, b.y, c.x, c.y, d.x, d.y};
-    unsigned long long mm[8];
-    int ex[8], t = 1 << 30;
-    for (int i = 0; i < 8; i++) {
-        mm[i] = 0;
-        ex[i] = 0;
+    constexpr long long P[RES_NP] = {
+        2147483647LL, 2147483629LL, 2147483587LL, 2147483579LL, 2147483563LL, 2147483549LL, 2147483543LL,
+        2147483497LL, 2147483489LL, 2147483477LL, 2147483423LL, 2147483399LL, 2147483353LL, 2147483323LL,
+        2147483269LL, 2147483249LL, 2147483237LL, 2147483179LL, 2147483171LL, 2147483137LL};
+    return P[q];
+}
+__host__ __device__ __forceinline__ bool res_ints(const double *v, int n, long long *X, int *Bout, int maxb)
+{
+    // two passes (the smallest exponent, then the integers) keep few values live
+    int t = 1 << 30;
+    for (int i = 0; i < n; i++) {
         if (v[i] == 0.0) continue;
         int e;
         const double f = frexp(fabs(v[i]), &e);            // |v| = f * 2^e, 0.5 <= f < 1
-        unsigned long long m = (unsigned long long)ldexp(f, 53);   // exact, below 2^53
-        int x = e - 53;
-        const int tz = __builtin_ctzll(m);
-        m >>= tz;
-        x += tz;
-        mm[i] = m;
-        ex[i] = x;
+        const unsigned long long m = (unsigned long long)ldexp(f, 53);   // exact, below 2^53
+        const int x = e - 53 + __builtin_ctzll(m);
         t = x < t ? x : t;
     }
-    if (t == (1 << 30)) return 1;   // every coordinate zero
-    long long X[8];
     int B = 1;
-    for (int i = 0; i < 8; i++) {
+    for (int i = 0; i < n; i++) {
         X[i] = 0;
-        if (mm[i] == 0) continue;
-        const int k = ex[i] - t;
-        const int bl = 64 - __builtin_clzll(mm[i]) + k;
-        if (bl > 60) return -1;
-        X[i] = (long long)(mm[i] << k);
+        if (v[i] == 0.0) continue;
+        int e;
+        const double f = frexp(fabs(v[i]), &e);
+        unsigned long long m = (unsigned long long)ldexp(f, 53);
+        const int tz = __builtin_ctzll(m);
+        m >>= tz;
+        const int k = e - 53 + tz - t;
+        const int bl = 64 - __builtin_clzll(m) + k;
+        if (bl > maxb) return false;
+        X[i] = (long long)(m << k);
         if (v[i] < 0.0) X[i] = -X[i];
         B = bl > B ? bl : B;
     }
-    const int np = (4 * B + 8 + 29) / 30;   // primes > 2^30 each
+    *Bout = B;
+    return true;
+}
+// Reduction by Barrett's method (no division): with m = floor((2^64 - 1) / p), q = floor(a m / 2^64)
+// is floor(a / p) or at most two less for a < 2^63, so a - q p needs at most two subtractions.
+struct ResMod {
+    unsigned long long p, m;
+};
+__host__ __device__ __forceinline__ ResMod res_mod(int q)
+{
+    constexpr unsigned long long M[RES_NP] = {
+        8589934596ULL, 8589934668ULL, 8589934836ULL, 8589934868ULL,
+        8589934932ULL, 8589934988ULL, 8589935012ULL, 8589935196ULL,
+        8589935228ULL, 8589935276ULL, 8589935492ULL, 8589935588ULL,
+        8589935772ULL, 8589935892ULL, 8589936108ULL, 8589936188ULL,
+        8589936236ULL, 8589936468ULL, 8589936500ULL, 8589936636ULL};
+    return ResMod{(unsigned long long)res_prime(q), M[q]};
+}
+__host__ __device__ __forceinline__ unsigned long long res_red(unsigned long long a, const ResMod &r)   // a < 2^63
+{
+    const unsigned long long q = (unsigned long long)(((unsigned __int128)a * r.m) >> 64);
+    unsigned long long x = a - q * r.p;
+    if (x >= r.p) x -= r.p;
+    if (x >= r.p) x -= r.p;
+    return x;
+}
+__host__ __device__ __forceinline__ unsigned long long res_of(long long x, const ResMod &r)   // |x| < 2^63
+{
+    const unsigned long long a = res_red((unsigned long long)(x < 0 ? -x : x), r);
+    return x < 0 && a ? r.p - a : a;
+}
+__host__ __device__ __forceinline__ unsigned long long res_mul(unsigned long long a, unsigned long long b,
+                                                                const ResMod &r)
+{
+    return res_red(a * b, r);
+}
+__host__ __device__ __forceinline__ unsigned long long res_sub(unsigned long long a, unsigned long long b,
+                                                                const ResMod &r)
+{
+    return a >= b ? a - b : a + r.p - b;
+}
+
+// Exact zero test of the incircle determinant by residues: 1 when it is zero, 0 when it is not,
+// -1 when the coordinates need more than 60 bits at their common scale (then the big-integer
+// path decides).  With integers below 2^B, |det| < 2^(4B + 8), and a determinant that vanishes
+// modulo primes whose product exceeds that bound is zero.  Snapped coordinates are multiples of
+// the snapping unit, so B <= 52 and eight primes suffice.  Co-circular lattice quadruples, the
+// exact path's common case, skip the big integers.
+__host__ __device__ __forceinline__ int incircle_zero_res(double2 a, double2 b, double2 c, double2 d)
+{
+    const double v[8] = {a.x, a.y, b.x, b.y, c.x, c.y, d.x, d.y};
+    long long X[8];
+    int B;
+    if (!res_ints(v, 8, X, &B, 60)) return -1;
+    const int np = (4 * B + 8 + 29) / 30;   // primes above 2^30 whose product exceeds 2^(4B + 8)
     const long long dx[3] = {X[0] - X[6], X[2] - X[6], X[4] - X[6]};
     const long long dy[3] = {X[1] - X[7], X[3] - X[7], X[5] - X[7]};
-    constexpr long long P[9] = {2147483647LL, 2147483629LL, 2147483587LL, 2147483579LL, 2147483563LL,
-                                2147483549LL, 2147483543LL, 2147483497LL, 2147483489LL};
-#pragma unroll
-    for (int q = 0; q < 9; q++) {
-        if (q >= np) break;
-        const long long p = P[q];
+    for (int q = 0; q < np; q++) {
+        const ResMod p = res_mod(q);
         unsigned long long rx[3], ry[3], lf[3];
         for (int k = 0; k < 3; k++) {
-            long long r = dx[k] % p;
-            rx[k] = (unsigned long long)(r < 0 ? r + p : r);
-            r = dy[k] % p;
-            ry[k] = (unsigned long long)(r < 0 ? r + p : r);
-            lf[k] = (rx[k] * rx[k] + ry[k] * ry[k]) % (unsigned long long)p;
+            rx[k] = res_of(dx[k], p);
+            ry[k] = res_of(dy[k], p);
+            lf[k] = res_red(rx[k] * rx[k] + ry[k] * ry[k], p);
         }
         unsigned long long det = 0;
         for (int k = 0; k < 3; k++) {
             const int i = (k + 1) % 3, j = (k + 2) % 3;
-            // minor = rx[i] * ry[j] - rx[j] * ry[i] (mod p)
-            const unsigned long long t1 = (rx[i] * ry[j]) % (unsigned long long)p;
-            const unsigned long long t2 = (rx[j] * ry[i]) % (unsigned long long)p;
-            const unsigned long long mn = t1 >= t2 ? t1 - t2 : t1 + (unsigned long long)p - t2;
-            det = (det + lf[k] * mn) % (unsigned long long)p;
+            const unsigned long long mn = res_sub(res_mul(rx[i], ry[j], p), res_mul(rx[j], ry[i], p), p);
+            det = res_red(det + lf[k] * mn, p);
         }
         if (det != 0) return 0;
+    }
+    return 1;
+}
+
+// Exact equality of two alpha-shape values (squared circumradius of a triangle, squared half
+// length of an edge, in the order of rho_cmp_exact) by residues (res_ints): 1 equal, 0 different, -1 unknown.  value =
+// num / den with edge num = |pq|^2, den = 1 and triangle num = l1 l2 l3, den = ar^2; with
+// integers below 2^B, |num_a den_b - num_b den_a| < 2^(10B + 16), so primes whose product exceeds
+// that decide it.  The lattice's equal circumradii, the exact comparisons' common case, skip the
+// big integers.
+__host__ __device__ __forceinline__ int rho_equal_res(int sta, const double2 *pa, int stb, const double2 *pb)
+{
+    double v[12];   // entity e's point i at slots 2 (3e + i), 2 (3e + i) + 1; unused slots 0
+    for (int q = 0; q < 12; q++) v[q] = 0.0;
+    const int na = sta ? 3 : 2, nb = stb ? 3 : 2;
+    for (int i = 0; i < na; i++) { v[2 * i] = pa[i].x; v[2 * i + 1] = pa[i].y; }
+    for (int i = 0; i < nb; i++) { v[6 + 2 * i] = pb[i].x; v[6 + 2 * i + 1] = pb[i].y; }
+    long long X[12];
+    int B;
+    if (!res_ints(v, 12, X, &B, 58)) return -1;
+    const int np = (10 * B + 16 + 29) / 30;
+    if (np > RES_NP) return -1;
+    const int stv[2] = {sta, stb};
+    for (int q = 0; q < np; q++) {
+        const ResMod p = res_mod(q);
+        unsigned long long num[2], den[2];
+        for (int e = 0; e < 2; e++) {
+            const long long *P = X + 6 * e;   // x0 y0 x1 y1 x2 y2
+            if (!stv[e]) {
+                const unsigned long long dx = res_of(P[0] - P[2], p), dy = res_of(P[1] - P[3], p);
+                num[e] = res_red(dx * dx + dy * dy, p);
+                den[e] = 1;
+            } else {
+                unsigned long long l[3];
+                for (int i = 0; i < 3; i++) {
+                    const int j = i == 2 ? 0 : i + 1;
+                    const unsigned long long dx = res_of(P[2 * i] - P[2 * j], p);
+                    const unsigned long long dy = res_of(P[2 * i + 1] - P[2 * j + 1], p);
+                    l[i] = res_red(dx * dx + dy * dy, p);
+                }
+                num[e] = res_mul(res_mul(l[0], l[1], p), l[2], p);
+                const unsigned long long ux = res_of(P[2] - P[0], p), uy = res_of(P[3] - P[1], p);
+                const unsigned long long wx = res_of(P[4] - P[0], p), wy = res_of(P[5] - P[1], p);
+                const unsigned long long ar = res_sub(res_mul(ux, wy, p), res_mul(uy, wx, p), p);
+                den[e] = res_mul(ar, ar, p);
+            }
+        }
+        if (res_mul(num[0], den[1], p) != res_mul(num[1], den[0], p)) return 0;
+    }
+    return 1;
+}
+
+// |i-j|^2 - |i+j-2k|^2 is zero (k on the circle with diameter ij), by residues: 1 zero, 0 not,
+// -1 unknown; with integers below 2^B it is below 2^(2B + 7) in magnitude
+__host__ __device__ __forceinline__ int attached_zero_res(double2 i, double2 j, double2 k)
+{
+    const double v[6] = {i.x, i.y, j.x, j.y, k.x, k.y};
+    long long X[6];
+    int B;
+    if (!res_ints(v, 6, X, &B, 58)) return -1;
+    const int np = (2 * B + 7 + 29) / 30;
+    const long long dx = X[0] - X[2], dy = X[1] - X[3];
+    const long long sx = X[0] + X[2] - 2 * X[4], sy = X[1] + X[3] - 2 * X[5];
+    for (int q = 0; q < np; q++) {   // np <= 5 for B <= 58
+        const ResMod p = res_mod(q);
+        const unsigned long long a = res_of(dx, p), b = res_of(dy, p), c = res_of(sx, p), d = res_of(sy, p);
+        if (res_red(a * a + b * b, p) != res_red(c * c + d * d, p)) return 0;
     }
     return 1;
 }
