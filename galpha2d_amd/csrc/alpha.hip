@@ -125,7 +125,8 @@ __device__ __noinline__ int rho_cmp_exact(int sta, const double2 *pa, int stb, c
 // attachment: exact sign of |i-j|^2 - |i+j-2k|^2 (>= 0 attached)
 __device__ __noinline__ int attached_exact(double2 i, double2 j, double2 k, int *ovf)
 {
-    if (attached_zero_res(i, j, k) == 1) return 1;   // on the circle: attached (>= 0)
+    const int q = attached_i128(i, j, k);
+    if (q >= 0) return q;
     double v[6] = {i.x, i.y, j.x, j.y, k.x, k.y};
     int s = min_ulp_exp(v, 6) - 1;   // -1: 2k stays an integer multiple of the unit
     BigInt<16> ix, iy, jx, jy, kx, ky, d1, sx, sy, t;

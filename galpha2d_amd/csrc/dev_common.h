@@ -484,23 +484,18 @@ __host__ __device__ __forceinline__ int rho_equal_res(int sta, const double2 *pa
     return 1;
 }
 
-// |i-j|^2 - |i+j-2k|^2 is zero (k on the circle with diameter ij), by residues: 1 zero, 0 not,
-// -1 unknown; with integers below 2^B it is below 2^(2B + 7) in magnitude
-__host__ __device__ __forceinline__ int attached_zero_res(double2 i, double2 j, double2 k)
+// Attachment in 128-bit integers: |i-j|^2 - |i+j-2k|^2 >= 0 (k in or on the circle with diameter
+// ij) with the coordinates as integers below 2^60 (res_ints): its terms stay below 2^125.  1
+// attached, 0 not, -1 when the coordinates need more bits (then the big integers decide).
+__host__ __device__ __forceinline__ int attached_i128(double2 i, double2 j, double2 k)
 {
     const double v[6] = {i.x, i.y, j.x, j.y, k.x, k.y};
     long long X[6];
     int B;
-    if (!res_ints(v, 6, X, &B, 58)) return -1;
-    const int np = (2 * B + 7 + 29) / 30;
-    const long long dx = X[0] - X[2], dy = X[1] - X[3];
-    const long long sx = X[0] + X[2] - 2 * X[4], sy = X[1] + X[3] - 2 * X[5];
-    for (int q = 0; q < np; q++) {   // np <= 5 for B <= 58
-        const ResMod p = res_mod(q);
-        const unsigned long long a = res_of(dx, p), b = res_of(dy, p), c = res_of(sx, p), d = res_of(sy, p);
-        if (res_red(a * a + b * b, p) != res_red(c * c + d * d, p)) return 0;
-    }
-    return 1;
+    if (!res_ints(v, 6, X, &B, 60)) return -1;
+    const __int128 dx = X[0] - X[2], dy = X[1] - X[3];
+    const __int128 sx = (__int128)X[0] + X[2] - 2 * (__int128)X[4], sy = (__int128)X[1] + X[3] - 2 * (__int128)X[5];
+    return dx * dx + dy * dy >= sx * sx + sy * sy ? 1 : 0;
 }
 
 GA_INLINE int incircle_exact_sos(double2 a, int ia, double2 b, int ib, double2 c, int ic,

@@ -4,7 +4,8 @@ compiled for the host and checked against exact rational arithmetic:
   lattice squares at a binary snapping unit) come out zero, perturbed and random ones nonzero;
 - `rho_equal_res`: equal alpha values (congruent lattice triangles moved and turned, a right
   triangle against its hypotenuse) come out equal, perturbed and random pairs different;
-- `attached_zero_res`: apexes on the circle over an edge come out zero, others not;
+- `attached_i128` (an exact sign in 128-bit integers): apexes on, inside and outside the circle
+  over an edge;
 and coordinate spreads beyond their bit limits defer (-1) instead of answering.  A wrong prime, a
 dropped term or a bound that lets a nonzero value vanish fails here."""
 import math
@@ -39,7 +40,7 @@ int main(int argc, char **argv)
         for (int k = 0; k < 6; k++) P[k] = make_double2(v[2 * k], v[2 * k + 1]);
         int r;
         if (kind == 0) r = incircle_zero_res(P[0], P[1], P[2], P[3]);
-        else if (kind == 1) r = attached_zero_res(P[0], P[1], P[2]);
+        else if (kind == 1) r = attached_i128(P[0], P[1], P[2]);
         else r = rho_equal_res((kind - 2) & 1, P, ((kind - 2) >> 1) & 1, P + 3);   // kind 2 + sta + 2 stb
         printf("%d\n", r);
     }
@@ -141,14 +142,14 @@ def make_alpha_cases(seed=12, n=800):
                 continue
             if len(b) == 3 and (b[1][0] - b[0][0]) * (b[2][1] - b[0][1]) == (b[1][1] - b[0][1]) * (b[2][0] - b[0][0]):
                 continue
-        else:   # attachment: apex on the circle over the edge (a right angle there), or one unit off
+        else:   # attachment: apex on the circle over the edge (a right angle there) or a unit in or out
             x0, y0, p, q = rng.randint(0, 2 ** 21), rng.randint(0, 2 ** 21), rng.randint(1, 999), rng.randint(1, 999)
             i, j, k = (x0 + p, y0), (x0, y0 + q), (x0, y0)
-            if rng.random() < 0.4:
-                k = (x0 + rng.choice([-1, 1]), y0)
+            if rng.random() < 0.6:
+                k = (x0 + rng.choice([-1, 1]), y0 + rng.choice([-1, 0, 1]))
             q3 = [(Fraction(x) * unit, Fraction(y) * unit) for x, y in (i, j, k)]
             (ix, iy), (jx, jy), (kx, ky) = q3
-            z = 1 if (ix - jx) ** 2 + (iy - jy) ** 2 == (ix + jx - 2 * kx) ** 2 + (iy + jy - 2 * ky) ** 2 else 0
+            z = 1 if (ix - jx) ** 2 + (iy - jy) ** 2 >= (ix + jx - 2 * kx) ** 2 + (iy + jy - 2 * ky) ** 2 else 0
             cases.append((1, [float(c) for pt in q3 for c in pt] + [0.0] * 6, z))
             continue
         qa = [(Fraction(x) * unit, Fraction(y) * unit) for x, y in a]
@@ -181,6 +182,6 @@ def test_incircle_zero_residues():
     want = [z for _, _, z in cases]
     bad = [(i, cases[i][0], got[i], want[i]) for i in range(len(cases)) if got[i] != want[i]]
     assert not bad, bad[:5]
-    for kind in (0, 1):   # zero and nonzero outcomes of every test are exercised
+    for kind in (0, 1):   # both outcomes of every test are exercised
         assert {z for k, _, z in cases if k == kind} >= {0, 1}
-    assert {z for k, _, z in cases if k >= 2} == {0, 1}
+    assert {z for k, _, z in cases if 2 <= k <= 5} == {0, 1}
