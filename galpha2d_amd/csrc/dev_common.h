@@ -402,21 +402,30 @@ __host__ __device__ __forceinline__ unsigned long long res_sub(unsigned long lon
     return a >= b ? a - b : a + r.p - b;
 }
 
+// bit length of |x| (0 for 0)
+__host__ __device__ __forceinline__ int res_bits(long long x)
+{
+    const unsigned long long u = (unsigned long long)(x < 0 ? -x : x);
+    return u ? 64 - __builtin_clzll(u) : 0;
+}
+
 // Exact zero test of the incircle determinant by residues: 1 when it is zero, 0 when it is not,
 // -1 when the coordinates need more than 60 bits at their common scale (then the big-integer
-// path decides).  With integers below 2^B, |det| < 2^(4B + 8), and a determinant that vanishes
-// modulo primes whose product exceeds that bound is zero.  Snapped coordinates are multiples of
-// the snapping unit, so B <= 52 and eight primes suffice.  Co-circular lattice quadruples, the
-// exact path's common case, skip the big integers.
+// path decides).  The determinant is a polynomial in the differences to d; with those below 2^D,
+// |det| < 3 * 2^(4D + 2) < 2^(4D + 4), and a determinant that vanishes modulo primes whose product
+// exceeds that bound is zero.  Co-circular lattice quadruples, the exact path's common case, skip
+// the big integers; their differences are far shorter than their coordinates.
 __host__ __device__ __forceinline__ int incircle_zero_res(double2 a, double2 b, double2 c, double2 d)
 {
     const double v[8] = {a.x, a.y, b.x, b.y, c.x, c.y, d.x, d.y};
     long long X[8];
     int B;
     if (!res_ints(v, 8, X, &B, 60)) return -1;
-    const int np = (4 * B + 8 + 29) / 30;   // primes above 2^30 whose product exceeds 2^(4B + 8)
     const long long dx[3] = {X[0] - X[6], X[2] - X[6], X[4] - X[6]};
     const long long dy[3] = {X[1] - X[7], X[3] - X[7], X[5] - X[7]};
+    int D = 0;
+    for (int k = 0; k < 3; k++) D = max(D, max(res_bits(dx[k]), res_bits(dy[k])));
+    const int np = (4 * D + 4 + 29) / 30;   // primes above 2^30 whose product exceeds 2^(4D + 4)
     for (int q = 0; q < np; q++) {
         const ResMod p = res_mod(q);
         unsigned long long rx[3], ry[3], lf[3];
@@ -438,9 +447,9 @@ __host__ __device__ __forceinline__ int incircle_zero_res(double2 a, double2 b, 
 
 // Exact equality of two alpha-shape values (squared circumradius of a triangle, squared half
 // length of an edge, in the order of rho_cmp_exact) by residues (res_ints): 1 equal, 0 different, -1 unknown.  value =
-// num / den with edge num = |pq|^2, den = 1 and triangle num = l1 l2 l3, den = ar^2; with
-// integers below 2^B, |num_a den_b - num_b den_a| < 2^(10B + 16), so primes whose product exceeds
-// that decide it.  The lattice's equal circumradii, the exact comparisons' common case, skip the
+// num / den with edge num = |pq|^2, den = 1 and triangle num = l1 l2 l3, den = ar^2, all in the
+// differences of an entity's points; with those below 2^D, num < 2^(6D + 3) and den < 2^(4D + 2),
+// so |num_a den_b - num_b den_a| < 2^(10D + 5) and primes whose product exceeds that decide it.  The lattice's equal circumradii, the exact comparisons' common case, skip the
 // big integers.
 __host__ __device__ __forceinline__ int rho_equal_res(int sta, const double2 *pa, int stb, const double2 *pb)
 {
@@ -451,10 +460,19 @@ __host__ __device__ __forceinline__ int rho_equal_res(int sta, const double2 *pa
     for (int i = 0; i < nb; i++) { v[6 + 2 * i] = pb[i].x; v[6 + 2 * i + 1] = pb[i].y; }
     long long X[12];
     int B;
-    if (!res_ints(v, 12, X, &B, 58)) return -1;
-    const int np = (10 * B + 16 + 29) / 30;
-    if (np > RES_NP) return -1;
+    if (!res_ints(v, 12, X, &B, 61)) return -1;   // differences below 2^62
     const int stv[2] = {sta, stb};
+    int D = 0;
+    for (int e = 0; e < 2; e++) {
+        const long long *P = X + 6 * e;
+        const int npt = stv[e] ? 3 : 2;
+        for (int i = 0; i < npt; i++) {
+            const int j = i + 1 == npt ? 0 : i + 1;
+            D = max(D, max(res_bits(P[2 * i] - P[2 * j]), res_bits(P[2 * i + 1] - P[2 * j + 1])));
+        }
+    }
+    const int np = (10 * D + 5 + 29) / 30;
+    if (np > RES_NP) return -1;
     for (int q = 0; q < np; q++) {
         const ResMod p = res_mod(q);
         unsigned long long num[2], den[2];
