@@ -332,7 +332,8 @@ __host__ __device__ __forceinline__ long long res_prime(int q)
         2147483269LL, 2147483249LL, 2147483237LL, 2147483179LL, 2147483171LL, 2147483137LL};
     return P[q];
 }
-__host__ __device__ __forceinline__ bool res_ints(const double *v, int n, long long *X, int *Bout, int maxb)
+__host__ __device__ __forceinline__ bool res_ints(const double *v, int n, long long *X, int *Bout, int maxb,
+                                             int *tout = nullptr)
 {
     // two passes (the smallest exponent, then the integers) keep few values live
     int t = 1 << 30;
@@ -361,6 +362,7 @@ __host__ __device__ __forceinline__ bool res_ints(const double *v, int n, long l
         B = bl > B ? bl : B;
     }
     *Bout = B;
+    if (tout) *tout = t;   // v[i] = X[i] 2^t
     return true;
 }
 // Reduction by Barrett's method (no division): with m = floor((2^64 - 1) / p), q = floor(a m / 2^64)
@@ -451,26 +453,38 @@ __host__ __device__ __forceinline__ int incircle_zero_res(double2 a, double2 b, 
 // differences of an entity's points; with those below 2^D, num < 2^(6D + 3) and den < 2^(4D + 2),
 // so |num_a den_b - num_b den_a| < 2^(10D + 5) and primes whose product exceeds that decide it.  The lattice's equal circumradii, the exact comparisons' common case, skip the
 // big integers.
-__host__ __device__ __forceinline__ int rho_equal_res(int sta, const double2 *pa, int stb, const double2 *pb)
+// the points of two alpha-value entities (st 0: edge of 2 points, 1: triangle of 3) as integers at
+// their common scale, X = x0 y0 x1 y1 x2 y2 of each (unused slots 0), and D, the largest bit length
+// of the differences between points of one entity; false when a coordinate needs more than 61 bits
+__host__ __device__ __forceinline__ bool rho_pair_ints(int sta, const double2 *pa, int stb, const double2 *pb,
+                                                       long long *X, int *Dout)
 {
-    double v[12];   // entity e's point i at slots 2 (3e + i), 2 (3e + i) + 1; unused slots 0
+    double v[12];
     for (int q = 0; q < 12; q++) v[q] = 0.0;
     const int na = sta ? 3 : 2, nb = stb ? 3 : 2;
     for (int i = 0; i < na; i++) { v[2 * i] = pa[i].x; v[2 * i + 1] = pa[i].y; }
     for (int i = 0; i < nb; i++) { v[6 + 2 * i] = pb[i].x; v[6 + 2 * i + 1] = pb[i].y; }
-    long long X[12];
     int B;
-    if (!res_ints(v, 12, X, &B, 61)) return -1;   // differences below 2^62
-    const int stv[2] = {sta, stb};
+    if (!res_ints(v, 12, X, &B, 61)) return false;   // differences below 2^62
     int D = 0;
     for (int e = 0; e < 2; e++) {
         const long long *P = X + 6 * e;
-        const int npt = stv[e] ? 3 : 2;
+        const int npt = (e ? stb : sta) ? 3 : 2;
         for (int i = 0; i < npt; i++) {
             const int j = i + 1 == npt ? 0 : i + 1;
             D = max(D, max(res_bits(P[2 * i] - P[2 * j]), res_bits(P[2 * i + 1] - P[2 * j + 1])));
         }
     }
+    *Dout = D;
+    return true;
+}
+
+__host__ __device__ __forceinline__ int rho_equal_res(int sta, const double2 *pa, int stb, const double2 *pb)
+{
+    long long X[12];
+    int D;
+    if (!rho_pair_ints(sta, pa, stb, pb, X, &D)) return -1;
+    const int stv[2] = {sta, stb};
     const int np = (10 * D + 5 + 29) / 30;
     if (np > RES_NP) return -1;
     for (int q = 0; q < np; q++) {
@@ -500,6 +514,203 @@ __host__ __device__ __forceinline__ int rho_equal_res(int sta, const double2 *pa
         if (res_mul(num[0], den[1], p) != res_mul(num[1], den[0], p)) return 0;
     }
     return 1;
+}
+
+// ---- alpha values in fixed-width integers ----------------------------------------------------
+// Limbs of 64 bits, least significant first.  Every size is a compile-time constant and every loop
+// is unrolled, so the arrays stay in registers (the big integers above live in scratch memory).
+template <int NA, int NB>
+__host__ __device__ __forceinline__ void mw_mul(const unsigned long long *a, const unsigned long long *b,
+                                                unsigned long long *r)   // r[NA + NB] = a b
+{
+#pragma unroll
+    for (int i = 0; i < NA + NB; i++) r[i] = 0;
+#pragma unroll
+    for (int i = 0; i < NA; i++) {
+        unsigned long long c = 0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            // a b + r + c <= (2^64 - 1)^2 + 2 (2^64 - 1) = 2^128 - 1
+            const unsigned __int128 t = (unsigned __int128)a[i] * b[j] + r[i + j] + c;
+            r[i + j] = (unsigned long long)t;
+            c = (unsigned long long)(t >> 64);
+        }
+        r[i + NB] = c;
+    }
+}
+template <int N>
+__host__ __device__ __forceinline__ int mw_bitlen(const unsigned long long *a)
+{
+    int L = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        if (a[i]) L = 64 * i + 64 - __builtin_clzll(a[i]);
+    return L;
+}
+// a <<= sh (0 <= sh < 64 N, bits shifted out are lost): whole limbs in three conditional stages of
+// 1, 2 and 4 limbs (register moves, no indexing by a variable), then the bits
+template <int N>
+__host__ __device__ __forceinline__ void mw_shl(unsigned long long *a, int sh)
+{
+    const int ws = sh >> 6, bs = sh & 63;
+#pragma unroll
+    for (int st = 1; st < N && st <= 4; st <<= 1) {
+        if (ws & st) {
+#pragma unroll
+            for (int i = N - 1; i >= 0; i--) a[i] = i >= st ? a[i - st] : 0;
+        }
+    }
+    if (bs) {
+#pragma unroll
+        for (int i = N - 1; i > 0; i--) a[i] = (a[i] << bs) | (a[i - 1] >> (64 - bs));
+        a[0] <<= bs;
+    }
+}
+
+// num and den of an entity's value rho = num / (4 den) (times the squared unit) from its integer
+// points P (x0 y0 x1 y1 x2 y2): edge num = |pq|^2, den = 1; triangle num = l1 l2 l3, den = ar^2.
+// With differences below 2^58: l < 2^117, num < 2^351 (6 limbs), |ar| < 2^117, den < 2^234 (4 limbs).
+__host__ __device__ __forceinline__ void rho_numden(int st, const long long *P, unsigned long long *num,
+                                                    unsigned long long *den)
+{
+    unsigned long long l[3][2];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int j = i == 2 ? 0 : i + 1;
+        const __int128 dx = (__int128)(P[2 * i] - P[2 * j]), dy = (__int128)(P[2 * i + 1] - P[2 * j + 1]);
+        const unsigned __int128 q = (unsigned __int128)(dx * dx) + (unsigned __int128)(dy * dy);
+        l[i][0] = (unsigned long long)q;
+        l[i][1] = (unsigned long long)(q >> 64);
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) num[i] = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) den[i] = 0;
+    if (!st) {   // edge P0 P1: its length is l[0]
+        num[0] = l[0][0];
+        num[1] = l[0][1];
+        den[0] = 1;
+        return;
+    }
+    unsigned long long t[4];
+    mw_mul<2, 2>(l[0], l[1], t);
+    mw_mul<4, 2>(t, l[2], num);
+    const __int128 ar = (__int128)(P[2] - P[0]) * (P[5] - P[1]) - (__int128)(P[3] - P[1]) * (P[4] - P[0]);
+    const unsigned __int128 aa = (unsigned __int128)(ar < 0 ? -ar : ar);
+    const unsigned long long A[2] = {(unsigned long long)aa, (unsigned long long)(aa >> 64)};
+    mw_mul<2, 2>(A, A, den);
+}
+
+// Exact sign of rho_a - rho_b in fixed-width integers (num_a den_b against num_b den_a, 10 limbs
+// each); 2 when the differences need more than 58 bits (then the big integers decide).
+__host__ __device__ __forceinline__ int rho_cmp_fixed(int sta, const double2 *pa, int stb, const double2 *pb)
+{
+    long long X[12];
+    int D;
+    if (!rho_pair_ints(sta, pa, stb, pb, X, &D) || D > 58) return 2;
+    unsigned long long na[6], da[4], nb[6], db[4], l[10], r[10];
+    rho_numden(sta, X, na, da);
+    rho_numden(stb, X + 6, nb, db);
+    mw_mul<6, 4>(na, db, l);
+    mw_mul<6, 4>(nb, da, r);
+    int c = 0;
+#pragma unroll
+    for (int i = 0; i < 10; i++)
+        if (l[i] != r[i]) c = l[i] > r[i] ? 1 : -1;   // the highest differing limb decides
+    return c;
+}
+
+// Knuth's algorithm D with 32-bit digits and fixed sizes: qd[0..8] = floor(U / V) for U = un[0..16]
+// (un[16] = 0) and V = vn[0..7] with its top bit set; un is left holding the remainder.  Each
+// quotient digit is estimated from the top two remainder digits, corrected at most twice,
+// multiplied back, and added back on an overdraft.
+__host__ __device__ __forceinline__ void mw_div_17_8(uint32_t *un, const uint32_t *vn, uint32_t *qd)
+{
+    const uint64_t b = 1ull << 32;
+#pragma unroll
+    for (int j = 8; j >= 0; j--) {
+        const uint64_t top = ((uint64_t)un[j + 8] << 32) | un[j + 7];
+        uint64_t qhat = top / vn[7], rhat = top - qhat * vn[7];
+        while (qhat >= b || qhat * vn[6] > ((rhat << 32) | un[j + 6])) {
+            qhat--;
+            rhat += vn[7];
+            if (rhat >= b) break;
+        }
+        int64_t tt = 0, kk = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint64_t pr = qhat * vn[i];
+            tt = (int64_t)un[i + j] - kk - (int64_t)(pr & 0xffffffffull);
+            un[i + j] = (uint32_t)tt;
+            kk = (int64_t)(pr >> 32) - (tt >> 32);
+        }
+        tt = (int64_t)un[j + 8] - kk;
+        un[j + 8] = (uint32_t)tt;
+        qd[j] = (uint32_t)qhat;
+        if (tt < 0) {
+            qd[j]--;
+            uint64_t c = 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const uint64_t s2 = (uint64_t)un[i + j] + vn[i] + c;
+                un[i + j] = (uint32_t)s2;
+                c = s2 >> 32;
+            }
+            un[j + 8] = (uint32_t)((uint64_t)un[j + 8] + c);
+        }
+    }
+}
+
+// The 256-bit key of an alpha value: q = floor(rho 2^-E) with q in [2^255, 2^256), a function of
+// the value alone (equal values give equal keys) and monotone in it.  With num, den at the unit
+// 2^t (rho = num / (4 den) 2^(2t)) and k = 256 - bitlen(num) + bitlen(den), floor(num 2^k / den)
+// has 256 or 257 bits; E = 2t - 2 - k.  The division runs on U = num 2^(512 - Ln) (16 digits of 32
+// bits) by V = den 2^(256 - Ld) (8 digits, top bit set): U / V = num 2^k / den.  Knuth's algorithm D
+// with all sizes fixed: 9 quotient digits, each estimated from the top two remainder digits,
+// corrected at most twice, multiplied back and added back on an overdraft.  False when the
+// differences need more than 58 bits or the entity is degenerate (then the big integers decide).
+__host__ __device__ __forceinline__ bool rho_key_fixed(int st, const double2 *p, unsigned long long *q4, int *E)
+{
+    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int npt = st ? 3 : 2;
+    for (int i = 0; i < npt; i++) { v[2 * i] = p[i].x; v[2 * i + 1] = p[i].y; }
+    long long X[6];
+    int B, t;
+    if (!res_ints(v, 6, X, &B, 61, &t)) return false;   // (an edge's unused slots are 0)
+    int D = 0;
+    for (int i = 0; i < npt; i++) {
+        const int j = i + 1 == npt ? 0 : i + 1;
+        D = max(D, max(res_bits(X[2 * i] - X[2 * j]), res_bits(X[2 * i + 1] - X[2 * j + 1])));
+    }
+    if (D > 58) return false;
+    unsigned long long num[8], den[4];
+    rho_numden(st, X, num, den);
+    num[6] = num[7] = 0;
+    const int Ln = mw_bitlen<6>(num), Ld = mw_bitlen<4>(den);
+    if (Ln == 0 || Ld == 0) return false;
+    mw_shl<8>(num, 512 - Ln);
+    mw_shl<4>(den, 256 - Ld);
+    uint32_t un[17], vn[8], qd[9];
+#pragma unroll
+    for (int i = 0; i < 8; i++) { un[2 * i] = (uint32_t)num[i]; un[2 * i + 1] = (uint32_t)(num[i] >> 32); }
+    un[16] = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) { vn[2 * i] = (uint32_t)den[i]; vn[2 * i + 1] = (uint32_t)(den[i] >> 32); }
+    mw_div_17_8(un, vn, qd);
+    int k = 256 - Ln + Ld;
+    unsigned long long q[5];
+#pragma unroll
+    for (int w = 0; w < 4; w++) q[w] = (unsigned long long)qd[2 * w] | ((unsigned long long)qd[2 * w + 1] << 32);
+    q[4] = qd[8];
+    if (q[4] & 1ull) {   // 257 bits: one more halving (floor of a floor is the floor)
+#pragma unroll
+        for (int w = 0; w < 4; w++) q[w] = (q[w] >> 1) | (q[w + 1] << 63);
+        k -= 1;
+    }
+#pragma unroll
+    for (int w = 0; w < 4; w++) q4[w] = q[w];
+    *E = 2 * t - 2 - k;
+    return true;
 }
 
 // Attachment in 128-bit integers: |i-j|^2 - |i+j-2k|^2 >= 0 (k in or on the circle with diameter

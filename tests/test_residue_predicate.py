@@ -39,7 +39,15 @@ int main(int argc, char **argv)
         double2 P[6];
         for (int k = 0; k < 6; k++) P[k] = make_double2(v[2 * k], v[2 * k + 1]);
         int r;
-        if (kind == 0) r = incircle_zero_res(P[0], P[1], P[2], P[3]);
+        if (kind == 10) {   // fixed-width key of entity st = v[0] with points P[1..3] (v[2..7])
+            unsigned long long q[4];
+            int E;
+            const bool ok = rho_key_fixed((int)v[0], P + 1, q, &E);
+            printf("%d %d %llu %llu %llu %llu\n", ok ? 1 : 0, E, q[0], q[1], q[2], q[3]);
+            continue;
+        }
+        if (kind >= 11) r = rho_cmp_fixed((kind - 11) & 1, P, ((kind - 11) >> 1) & 1, P + 3);   // kind 11 + sta + 2 stb
+        else if (kind == 0) r = incircle_zero_res(P[0], P[1], P[2], P[3]);
         else if (kind == 1) r = attached_i128(P[0], P[1], P[2]);
         else r = rho_equal_res((kind - 2) & 1, P, ((kind - 2) >> 1) & 1, P + 3);   // kind 2 + sta + 2 stb
         printf("%d\n", r);
@@ -101,6 +109,10 @@ def make_cases(seed=11, n=1200):
     cases.append(([1e10, 0.3, 1e-10, 2.0, 5.0, 7.0, 0.0, 1.0], -1))
     cases.append(([0.0] * 8, 1))
     return [(0, v + [0.0] * 4, z) for v, z in cases]
+
+
+def _sign(x):
+    return (x > 0) - (x < 0)
 
 
 def rho_value(pts):
@@ -185,3 +197,192 @@ def test_incircle_zero_residues():
     for kind in (0, 1):   # both outcomes of every test are exercised
         assert {z for k, _, z in cases if k == kind} >= {0, 1}
     assert {z for k, _, z in cases if 2 <= k <= 5} == {0, 1}
+
+
+def _run_harness(cases):
+    tmp = tempfile.mkdtemp()
+    try:
+        src, exe, data = (os.path.join(tmp, f) for f in ("h.cpp", "h", "cases.txt"))
+        with open(src, "w") as f:
+            f.write(HARNESS)
+        subprocess.run([HIPCC, "-O2", "-std=c++17", "-I", CSRC, src, "-o", exe], check=True, capture_output=True)
+        with open(data, "w") as f:
+            f.write(f"{len(cases)}\n")
+            for kind, v in cases:
+                f.write(f"{kind} " + " ".join(repr(x) for x in v) + "\n")
+        return subprocess.run([exe, data], check=True, capture_output=True, text=True).stdout.splitlines()
+    finally:
+        shutil.rmtree(tmp)
+
+
+def _entity(rng, kind, sc=1.0):
+    """Points of an alpha-value entity as exact doubles: 2 (edge) or 3 (triangle)."""
+    n = 3 if rng.random() < 0.6 else 2
+    while True:
+        if kind == 0:     # lattice cell i/1414 (rounded doubles, like the lattice workload)
+            i, j = rng.randint(0, 1412), rng.randint(0, 1412)
+            cand = [(i, j), (i + 1, j), (i + 1, j + 1), (i, j + 1)]
+            rng.shuffle(cand)
+            pts = [(a / 1414, b / 1414) for a, b in cand[:n]]
+        elif kind == 1:   # uniform doubles in a small window
+            x0, y0 = rng.random(), rng.random()
+            pts = [(x0 + rng.random() * 1e-3, y0 + rng.random() * 1e-3) for _ in range(n)]
+        else:             # integers at a binary scale sc
+            pts = [(rng.randint(-2 ** 40, 2 ** 40) * sc, rng.randint(-2 ** 40, 2 ** 40) * sc) for _ in range(n)]
+        q = [(Fraction(x), Fraction(y)) for x, y in pts]
+        if n == 3 and (q[1][0] - q[0][0]) * (q[2][1] - q[0][1]) == (q[1][1] - q[0][1]) * (q[2][0] - q[0][0]):
+            continue
+        if n == 2 and q[0] == q[1]:
+            continue
+        return pts, rho_value(q)
+
+
+def _defers(entities):
+    """True when the fixed-width path must defer: at the entities' common unit 2^t (the lowest set
+    bit of any coordinate) a coordinate needs more than 61 bits or a difference within one entity
+    more than 58."""
+    vals = [Fraction(c) for pts in entities for pt in pts for c in pt]
+    t = min((v.numerator & -v.numerator).bit_length() - v.denominator.bit_length() for v in vals if v)
+    X = [[(int(Fraction(x) / Fraction(2) ** t), int(Fraction(y) / Fraction(2) ** t)) for x, y in pts]
+         for pts in entities]
+    if max(abs(c).bit_length() for pts in X for pt in pts for c in pt) > 61:
+        return True
+    D = max(abs(p[k] - q[k]).bit_length() for pts in X for p in pts for q in pts for k in (0, 1))
+    return D > 58
+
+
+def test_fixed_width_alpha_values():
+    """rho_cmp_fixed (sign of rho_a - rho_b) and rho_key_fixed (q = floor(rho 2^-E) in [2^255, 2^256))
+    against rationals: neighbouring lattice cells (equal and unequal values a few ulps apart),
+    close uniform pairs, wide binary scales, and spreads beyond 58 bits that defer."""
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not installed")
+    rng = random.Random(21)
+    cmp_cases, key_cases = [], []
+    for it in range(1500):
+        kind = it % 3
+        sc = 2.0 ** rng.randint(-60, 20)   # one scale for both entities of a pair
+        (pa, ra), (pb, rb) = _entity(rng, kind, sc), _entity(rng, kind, sc)
+        if it % 5 == 0:   # the same cell moved by whole lattice steps: often the same value
+            d = rng.randint(1, 50)
+            pb = [(round(x * 1414) / 1414, (round(y * 1414) + d) / 1414) for x, y in pa] if kind == 0 else pa
+            rb = rho_value([(Fraction(x), Fraction(y)) for x, y in pb])
+        va = [c for pt in pa for c in pt] + [0.0] * (6 - 2 * len(pa))
+        vb = [c for pt in pb for c in pt] + [0.0] * (6 - 2 * len(pb))
+        cmp_cases.append((11 + (len(pa) == 3) + 2 * (len(pb) == 3), va + vb,
+                          2 if _defers([pa, pb]) else _sign(ra - rb)))
+        # rho_value is num / den; the alpha value is num / (4 den)
+        key_cases.append((10, [float(len(pa) == 3), 0.0] + va + [0.0] * 4, None if _defers([pa]) else ra / 4))
+    # spreads beyond 58 bits: both defer
+    wide = [1.0, 0.0, 2.0 ** -70, 0.0]
+    cmp_cases.append((11, wide + [0.0] * 2 + [0.5, 0.5, 0.0, 0.0, 0.0, 0.0], 2))
+    key_cases.append((10, [0.0, 0.0] + wide + [0.0] * 6, None))
+    out = _run_harness([(k, v) for k, v, _ in cmp_cases] + [(k, v) for k, v, _ in key_cases])
+    got_cmp = [int(x) for x in out[:len(cmp_cases)]]
+    want_cmp = [w for _, _, w in cmp_cases]
+    bad = [(i, got_cmp[i], want_cmp[i]) for i in range(len(cmp_cases)) if got_cmp[i] != want_cmp[i]]
+    assert not bad, bad[:5]
+    assert set(want_cmp) == {-1, 0, 1, 2}
+    assert sum(w == 2 for w in want_cmp) < len(want_cmp) // 20   # deferral stays rare
+    nkey = 0
+    for line, (_, v, rho) in zip(out[len(cmp_cases):], key_cases):
+        ok, E, *q = (int(x) for x in line.split())
+        if rho is None:
+            assert ok == 0
+            continue
+        assert ok == 1, v
+        Q = sum(w << (64 * i) for i, w in enumerate(q))
+        e = rho.numerator.bit_length() - rho.denominator.bit_length()   # 2^e <= rho < 2^(e + 1), or one less
+        if Fraction(2) ** e > rho:
+            e -= 1
+        assert E == e - 255, (v, E, e)
+        assert Q == math.floor(rho / Fraction(2) ** E)
+        assert 2 ** 255 <= Q < 2 ** 256
+        nkey += 1
+    assert nkey > 0.95 * len(key_cases)
+
+
+DIV_HARNESS = r"""
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include "dev_common.h"
+int main(int argc, char **argv)
+{
+    FILE *f = fopen(argv[1], "r");
+    int n;
+    if (!f || fscanf(f, "%d", &n) != 1) return 2;
+    for (int c = 0; c < n; c++) {
+        uint32_t un[17], vn[8], qd[9];
+        for (int i = 0; i < 16; i++)
+            if (fscanf(f, "%x", &un[i]) != 1) return 2;
+        un[16] = 0;
+        for (int i = 0; i < 8; i++)
+            if (fscanf(f, "%x", &vn[i]) != 1) return 2;
+        mw_div_17_8(un, vn, qd);
+        for (int i = 8; i >= 0; i--) printf("%08x", qd[i]);
+        printf(" ");
+        for (int i = 7; i >= 0; i--) printf("%08x", un[i]);
+        printf("\n");
+    }
+    return 0;
+}
+"""
+
+
+def _add_backs(U, V):
+    """Quotient digits of U / V (32-bit digits, 17 by 8, V's top bit set) whose two-digit estimate
+    still overshoots, so that the division must add back: the rare branch worth pinning."""
+    b = 1 << 32
+    vn = [(V >> (32 * i)) & (b - 1) for i in range(8)]
+    n = 0
+    R = U
+    for j in range(8, -1, -1):
+        un = [(R >> (32 * i)) & (b - 1) for i in range(17)]
+        top = (un[j + 8] << 32) | un[j + 7]
+        qhat, rhat = divmod(top, vn[7])
+        while qhat >= b or qhat * vn[6] > ((rhat << 32) | un[j + 6]):
+            qhat -= 1
+            rhat += vn[7]
+            if rhat >= b:
+                break
+        qtrue = (R >> (32 * j)) // V
+        n += qhat > qtrue
+        R -= qtrue * V << (32 * j)
+    return n
+
+
+def test_fixed_division_add_back():
+    """mw_div_17_8 (the key's long division) against Python's integers, on digits drawn from edge
+    values (0, 1, 2^31 - 1, 2^31, 2^32 - 1) that drive the estimate into its corrections and the
+    add-back branch, which random keys almost never reach."""
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not installed")
+    rng = random.Random(31)
+    edge = [0, 1, 2 ** 31 - 1, 2 ** 31, 2 ** 32 - 1]
+    cases = []
+    while len(cases) < 3000:
+        vd = [rng.choice(edge) if rng.random() < 0.8 else rng.getrandbits(32) for _ in range(8)]
+        vd[7] |= 1 << 31
+        V = sum(d << (32 * i) for i, d in enumerate(vd))
+        ud = [rng.choice(edge) if rng.random() < 0.8 else rng.getrandbits(32) for _ in range(16)]
+        U = sum(d << (32 * i) for i, d in enumerate(ud))
+        cases.append((U, V))
+    hits = sum(_add_backs(U, V) > 0 for U, V in cases)
+    assert hits >= 10   # the add-back branch is exercised
+    tmp = tempfile.mkdtemp()
+    try:
+        src, exe, data = (os.path.join(tmp, f) for f in ("d.cpp", "d", "cases.txt"))
+        with open(src, "w") as f:
+            f.write(DIV_HARNESS)
+        subprocess.run([HIPCC, "-O2", "-std=c++17", "-I", CSRC, src, "-o", exe], check=True, capture_output=True)
+        with open(data, "w") as f:
+            f.write(f"{len(cases)}\n")
+            for U, V in cases:
+                f.write(" ".join("%x" % ((U >> (32 * i)) & 0xffffffff) for i in range(16)) + " ")
+                f.write(" ".join("%x" % ((V >> (32 * i)) & 0xffffffff) for i in range(8)) + "\n")
+        out = subprocess.run([exe, data], check=True, capture_output=True, text=True).stdout.split("\n")
+    finally:
+        shutil.rmtree(tmp)
+    for (U, V), line in zip(cases, out):
+        q, r = (int(x, 16) for x in line.split())
+        assert (q, r) == divmod(U, V), (hex(U), hex(V))
