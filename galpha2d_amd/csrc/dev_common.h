@@ -713,6 +713,54 @@ __host__ __device__ __forceinline__ bool rho_key_fixed(int st, const double2 *p,
     return true;
 }
 
+// Exact incircle sign in fixed width: with the differences to d below 2^D (D <= 58), each term
+// lift_k * minor_k is below 2^(4D + 2) <= 2^234 and the determinant below 2^(4D + 4) <= 2^236, so a
+// 256-bit two's complement sum cannot overflow and its top bit is the sign.  The same determinant
+// as incircle_exact; 2 when the differences need more bits (then the big integers decide).
+__host__ __device__ __forceinline__ int incircle_fixed(double2 a, double2 b, double2 c, double2 d)
+{
+    const double v[8] = {a.x, a.y, b.x, b.y, c.x, c.y, d.x, d.y};
+    long long X[8];
+    int B;
+    if (!res_ints(v, 8, X, &B, 61)) return 2;
+    const long long dx[3] = {X[0] - X[6], X[2] - X[6], X[4] - X[6]};
+    const long long dy[3] = {X[1] - X[7], X[3] - X[7], X[5] - X[7]};
+    int D = 0;
+    for (int k = 0; k < 3; k++) D = max(D, max(res_bits(dx[k]), res_bits(dy[k])));
+    if (D > 58) return 2;
+    unsigned long long acc[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int i = k == 2 ? 0 : k + 1, j = k == 0 ? 2 : k - 1;   // (k + 1) % 3, (k + 2) % 3
+        const unsigned __int128 lf = (unsigned __int128)((__int128)dx[k] * dx[k]) +
+                                     (unsigned __int128)((__int128)dy[k] * dy[k]);
+        const __int128 mn = (__int128)dx[i] * dy[j] - (__int128)dx[j] * dy[i];
+        const unsigned __int128 am = (unsigned __int128)(mn < 0 ? -mn : mn);
+        const unsigned long long L[2] = {(unsigned long long)lf, (unsigned long long)(lf >> 64)};
+        const unsigned long long M[2] = {(unsigned long long)am, (unsigned long long)(am >> 64)};
+        unsigned long long P[4];
+        mw_mul<2, 2>(L, M, P);
+        if (mn < 0) {   // two's complement: invert and add one
+            unsigned long long c = 1;
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+                const unsigned long long x = ~P[w] + c;
+                c = (c && x == 0) ? 1 : 0;
+                P[w] = x;
+            }
+        }
+        unsigned long long c = 0;
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            const unsigned __int128 sum = (unsigned __int128)acc[w] + P[w] + c;
+            acc[w] = (unsigned long long)sum;
+            c = (unsigned long long)(sum >> 64);
+        }
+    }
+    if ((acc[0] | acc[1] | acc[2] | acc[3]) == 0) return 0;
+    return (acc[3] >> 63) ? -1 : 1;
+}
+
 // Attachment in 128-bit integers: |i-j|^2 - |i+j-2k|^2 >= 0 (k in or on the circle with diameter
 // ij) with the coordinates as integers below 2^60 (res_ints): its terms stay below 2^125.  1
 // attached, 0 not, -1 when the coordinates need more bits (then the big integers decide).
@@ -730,8 +778,10 @@ __host__ __device__ __forceinline__ int attached_i128(double2 i, double2 j, doub
 GA_INLINE int incircle_exact_sos(double2 a, int ia, double2 b, int ib, double2 c, int ic,
                                  double2 d, int id, int *ovf)
 {
-    // a determinant the residues show to be zero skips the big integers
-    int s = incircle_zero_res(a, b, c, d) == 1 ? 0 : incircle_exact(a, b, c, d, ovf);
+    // differences within 58 bits: the sign in fixed-width registers; beyond, a determinant the
+    // residues show to be zero skips the big integers
+    int s = incircle_fixed(a, b, c, d);
+    if (s == 2) s = incircle_zero_res(a, b, c, d) == 1 ? 0 : incircle_exact(a, b, c, d, ovf);
     if (s != 0) return s;
     return incircle_sos_tie(a, ia, b, ib, c, ic, d, id, ovf);
 }

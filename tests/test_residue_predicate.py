@@ -46,7 +46,8 @@ int main(int argc, char **argv)
             printf("%d %d %llu %llu %llu %llu\n", ok ? 1 : 0, E, q[0], q[1], q[2], q[3]);
             continue;
         }
-        if (kind >= 11) r = rho_cmp_fixed((kind - 11) & 1, P, ((kind - 11) >> 1) & 1, P + 3);   // kind 11 + sta + 2 stb
+        if (kind == 16) r = incircle_fixed(P[0], P[1], P[2], P[3]);
+        else if (kind >= 11) r = rho_cmp_fixed((kind - 11) & 1, P, ((kind - 11) >> 1) & 1, P + 3);   // kind 11 + sta + 2 stb
         else if (kind == 0) r = incircle_zero_res(P[0], P[1], P[2], P[3]);
         else if (kind == 1) r = attached_i128(P[0], P[1], P[2]);
         else r = rho_equal_res((kind - 2) & 1, P, ((kind - 2) >> 1) & 1, P + 3);   // kind 2 + sta + 2 stb
@@ -237,7 +238,7 @@ def _entity(rng, kind, sc=1.0):
         return pts, rho_value(q)
 
 
-def _defers(entities):
+def _defers(entities, to_last=False):
     """True when the fixed-width path must defer: at the entities' common unit 2^t (the lowest set
     bit of any coordinate) a coordinate needs more than 61 bits or a difference within one entity
     more than 58."""
@@ -247,7 +248,10 @@ def _defers(entities):
          for pts in entities]
     if max(abs(c).bit_length() for pts in X for pt in pts for c in pt) > 61:
         return True
-    D = max(abs(p[k] - q[k]).bit_length() for pts in X for p in pts for q in pts for k in (0, 1))
+    if to_last:   # incircle: differences to the last point only
+        D = max(abs(p[k] - pts[-1][k]).bit_length() for pts in X for p in pts for k in (0, 1))
+    else:
+        D = max(abs(p[k] - q[k]).bit_length() for pts in X for p in pts for q in pts for k in (0, 1))
     return D > 58
 
 
@@ -386,3 +390,31 @@ def test_fixed_division_add_back():
     for (U, V), line in zip(cases, out):
         q, r = (int(x, 16) for x in line.split())
         assert (q, r) == divmod(U, V), (hex(U), hex(V))
+
+
+def test_fixed_width_incircle():
+    """incircle_fixed (the determinant's sign in 256-bit two's complement) against rationals on the
+    co-circular, perturbed and random quadruples of the residue test, plus far translations and
+    spreads beyond 58 bits, which defer (2)."""
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not installed")
+    rng = random.Random(41)
+    cases = []
+    for _, v, _ in make_cases(seed=13, n=1600):
+        pts = [(v[2 * k], v[2 * k + 1]) for k in range(4)]
+        if rng.random() < 0.3:   # shift far away: long coordinates, short differences
+            sh = rng.randint(1, 2 ** 20) * 2.0 ** 20
+            moved = [(x + sh, y - sh) for x, y in pts]
+            if any(Fraction(mx) != Fraction(x) + Fraction(sh) or Fraction(my) != Fraction(y) - Fraction(sh)
+                   for (x, y), (mx, my) in zip(pts, moved)):
+                continue   # the shift was not exact in doubles
+            pts = moved
+        q = [(Fraction(x), Fraction(y)) for x, y in pts]
+        want = 2 if _defers([pts], to_last=True) else _sign(incircle_det(q))
+        cases.append((16, [c for p in pts for c in p] + [0.0] * 4, want))
+    out = _run_harness([(k, v) for k, v, _ in cases])
+    got = [int(x) for x in out]
+    bad = [(i, cases[i][1][:8], got[i], cases[i][2]) for i in range(len(cases)) if got[i] != cases[i][2]]
+    assert not bad, bad[:5]
+    assert {w for _, _, w in cases} == {-1, 0, 1, 2}
+    assert sum(w == 2 for _, _, w in cases) < len(cases) // 4
