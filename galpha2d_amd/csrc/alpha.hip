@@ -179,6 +179,7 @@ struct AlphaK {
     int *counters;       // [ALPHA_COUNTER_INTS]: 64 counters, then the stripes
     int *bigstart;             // large clusters (start position)
     int *xclu;                 // small clusters that need exact comparisons (start position)
+    unsigned long long *xpair; // their pairwise exact order, 2 bits per pair (k_clusters_pairs)
     int *xatt;                 // edges whose attachment needs the exact predicate
     int bigcap;
     int nS0;
@@ -955,7 +956,7 @@ __global__ void k_big_fix(AlphaK a, BigK b, int Mbig, const int *closef, const i
 
 // One thread per cluster start with size >= 2.  A small cluster whose certified intervals are
 // pairwise disjoint is ordered by them (its values are distinct: every entry starts a group);
-// one with overlapping intervals goes to xclu for k_clusters_exact, which holds the expansion
+// one with overlapping intervals goes to xclu for k_clusters_pairs, which holds the expansion
 // arithmetic (so this kernel needs no scratch).  The cluster count and largest size go to the
 // counters once per wave.
 __global__ void k_clusters(AlphaK a, int S0)
@@ -998,7 +999,9 @@ __global__ void k_clusters(AlphaK a, int S0)
             for (int j = i + 1; j < MAX_SMALL_CLUSTER; j++)
                 if (j < m && !(hiv[i] < lov[j] || hiv[j] < lov[i])) open = true;
         if (open) {
-            a.xclu[atomicAdd(&a.counters[A_EXCLU], 1)] = k;
+            const int q = atomicAdd(&a.counters[A_EXCLU], 1);
+            a.xclu[q] = k;
+            a.xpair[q] = 0;   // k_clusters_pairs ORs its results in
             continue;
         }
 #pragma unroll
@@ -1051,25 +1054,73 @@ __device__ int4 fold_stats(const AlphaK &a)
     return make_int4(c, m, x, t);
 }
 
-// the small clusters with overlapping intervals: exact insertion sort, then group flags
-__global__ void k_clusters_exact(AlphaK a, int S0)
+// The small clusters with overlapping intervals, in two launches.  k_clusters_pairs runs one
+// exact comparison per thread: work item w is pair p = w % 28 (i < j, row by row: (0,1) .. (0,7),
+// (1,2) ..) of cluster q = w / 28, with code 1 (equal), 2 (entry i smaller) or 3 (entry j smaller)
+// in bits 2p of xpair[q].  k_clusters_rank then puts each cluster in order: entry i goes to
+// position #{j : v_j < v_i} + #{j < i : v_j = v_i}, the order of a stable sort by value, and starts
+// a group unless an earlier entry has its value.  One thread ran a whole cluster's insertion sort
+// before (up to 35 exact comparisons in sequence): 2.9 ms on the 2M lattice + cluster set.
+static_assert(MAX_SMALL_CLUSTER == 8, "28 pairs of 2 bits in one 64-bit word");
+__device__ __forceinline__ int pair_index(int i, int j) { return i * (15 - i) / 2 + j - i - 1; }
+
+// the entries of the cluster that starts at k (2 .. MAX_SMALL_CLUSTER)
+__device__ __forceinline__ int small_cluster_size(const AlphaK &a, int k, int S0)
+{
+    int end = k + 1;
+    while (end < S0 && end < k + MAX_SMALL_CLUSTER && !a.flag[end]) end++;
+    return end - k;
+}
+
+__global__ void k_clusters_pairs(AlphaK a, int S0)
 {
     const int nx = a.counters[A_EXCLU];
     int ncmp = 0;
-    GS(q, nx) {
+    const long long nw = 28ll * nx;   // (above 2^31 for 77M clusters)
+    for (long long w = blockIdx.x * (long long)blockDim.x + threadIdx.x; w < nw;
+         w += (long long)gridDim.x * blockDim.x) {
+        const int q = (int)(w / 28);
+        int r = (int)(w - 28ll * q), i = 0;
+        while (r >= 7 - i) { r -= 7 - i; i++; }
+        const int j = i + 1 + r;
         const int k = a.xclu[q];
-        int end = k + 1;
-        while (end < S0 && !a.flag[end]) end++;
-        const int m = end - k;
-        int *ids = a.id2 + k;
-        for (int i = 1; i < m; i++) {
-            int x = ids[i], j = i - 1;
-            while (j >= 0 && cmp_ids(a, ids[j], x, &ncmp) > 0) { ids[j + 1] = ids[j]; j--; }
-            ids[j + 1] = x;
-        }
-        for (int i = 1; i < m; i++) a.gflag[k + i] = cmp_ids(a, ids[i - 1], ids[i], &ncmp) != 0;
+        if (j >= small_cluster_size(a, k, S0)) continue;
+        const int c = cmp_ids(a, a.id2[k + i], a.id2[k + j], &ncmp);
+        atomicOr(&a.xpair[q], (unsigned long long)(c == 0 ? 1 : (c < 0 ? 2 : 3)) << (2 * pair_index(i, j)));
     }
     stat_add(a.counters, ST_EXACT_CMP, ncmp);
+}
+
+__global__ void k_clusters_rank(AlphaK a, int S0)
+{
+    const int nx = a.counters[A_EXCLU];
+    GS(q, nx) {
+        const int k = a.xclu[q];
+        const int m = small_cluster_size(a, k, S0);
+        const unsigned long long code = a.xpair[q];
+        int *ids = a.id2 + k;
+        int idv[MAX_SMALL_CLUSTER];
+#pragma unroll
+        for (int i = 0; i < MAX_SMALL_CLUSTER; i++) idv[i] = i < m ? ids[i] : 0;
+#pragma unroll
+        for (int i = 0; i < MAX_SMALL_CLUSTER; i++) {
+            if (i >= m) continue;
+            int pos = 0;
+            bool eq_before = false;
+#pragma unroll
+            for (int j = 0; j < MAX_SMALL_CLUSTER; j++) {
+                if (j >= m || j == i) continue;
+                const int lo = i < j ? i : j, hi = i < j ? j : i;
+                const int c = (int)((code >> (2 * pair_index(lo, hi))) & 3ull);
+                // c: 1 equal, 2 entry lo smaller, 3 entry hi smaller
+                const bool j_smaller = (c == 2 && lo == j) || (c == 3 && hi == j);
+                if (j_smaller || (c == 1 && j < i)) pos++;
+                if (c == 1 && j < i) eq_before = true;
+            }
+            ids[pos] = idv[i];
+            if (pos > 0) a.gflag[k + pos] = eq_before ? 0 : 1;
+        }
+    }
 }
 
 // End (one past the last entry) of every large cluster, cstart[0..nbig) in increasing order: a
@@ -1386,7 +1437,7 @@ enum {
     AB_ID2, AB_LO, AB_HI, AB_SLO, AB_SHI, AB_FLAG, AB_RANK, AB_SPECBITS, AB_VMIN, AB_VMAX, AB_STATUS, AB_CNTRS,
     AB_TMP, AB_SCAN, AB_NEG, AB_PMAX, AB_NSMIN, AB_MKEY, AB_MVAL, AB_MFLAG, AB_MPOS, AB_OKEY, AB_OVAL,
     AB_SKEY, AB_SVAL, AB_SPEC, AB_SP, AB_MISC, AB_BIG0, AB_BIG1, AB_BIG2, AB_BIG3, AB_BIGL,
-    AB_BS0, AB_BS1, AB_BS2, AB_BS3, AB_BS4, AB_BSET, AB_MSET, AB_VHULL, AB_RANKQ, AB_RD, AB_XATT, AB_XCLU, AB_VNUM, AB_CXY, AB_COUNT
+    AB_BS0, AB_BS1, AB_BS2, AB_BS3, AB_BS4, AB_BSET, AB_MSET, AB_VHULL, AB_RANKQ, AB_RD, AB_XATT, AB_XCLU, AB_XPAIR, AB_VNUM, AB_CXY, AB_COUNT
 };
 static_assert(AB_COUNT <= 64, "AlphaDev holds 64 buffers");
 
@@ -1712,9 +1763,12 @@ void alpha_run_batch(AlphaDev &d, const double *d_xy, int n, const int *d_tris9,
     a.bigstart = AP<int>(d, AB_BIGL);
     ab(d, AB_XCLU, 4 * ((size_t)S0 / 2 + 1) + 64);
     a.xclu = AP<int>(d, AB_XCLU);
+    ab(d, AB_XPAIR, 8 * ((size_t)S0 / 2 + 1) + 64);
+    a.xpair = AP<unsigned long long>(d, AB_XPAIR);
     GA_LAUNCH(k_clusters, dim3(AGRID), dim3(256), 0, st, a, S0);
-    // small clusters with overlapping intervals (the kernel reads their count)
-    GA_LAUNCH(k_clusters_exact, dim3(256), dim3(256), 0, st, a, S0);
+    // small clusters with overlapping intervals (the kernels read their count)
+    GA_LAUNCH(k_clusters_pairs, dim3(AGRID), dim3(256), 0, st, a, S0);
+    GA_LAUNCH(k_clusters_rank, dim3(256), dim3(256), 0, st, a, S0);
     // Large clusters: a batch reads their count here.  A single set goes on as if there were none
     // and reads the count with its final counters; when there are some, it runs their exact
     // order and everything after it again (the first pass's kernels below k_clusters return at
