@@ -715,6 +715,17 @@ GA_INLINE int cmp_ids(const AlphaK &a, int ia, int ib, int *ncmp)
     return r;
 }
 
+// equality of two entries whose 256-bit keys are equal (1 equal, 0 different, -1 unknown)
+GA_INLINE int key_equal_values(const AlphaK &a, int ia, int ib, int *ncmp)
+{
+    (*ncmp)++;
+    int sa, sb;
+    double2 pa[3], pb[3];
+    entry_pts(a, ia, &sa, pa);
+    entry_pts(a, ib, &sb, pb);
+    return rho_equal_res(sa, pa, sb, pb, 1);
+}
+
 // Clusters of up to MAX_SMALL_CLUSTER entries are put in order by one thread (exact insertion
 // sort: up to m (m - 1) / 2 exact comparisons in sequence); larger ones by the exact-key path,
 // which works on all their entries in parallel.  On the 2M lattice + cluster set a threshold of
@@ -921,7 +932,10 @@ __global__ void k_big_pairs(AlphaK a, BigK b, int Mbig, const int *sorted_j, int
         int f = 1;
         if (same) {
             closef[gpos] = 1;
-            int r = cmp_ids(a, b.eid[jp], b.eid[jc], &ncmp);
+            // equal keys: equality needs fewer primes (the values differ by less than 2^-255
+            // relative); a different pair goes on to the full exact comparison
+            int r = 0;
+            if (key_equal_values(a, b.eid[jp], b.eid[jc], &ncmp) != 1) r = cmp_ids(a, b.eid[jp], b.eid[jc], &ncmp);
             if (r > 0) inv[gpos] = 1;
             f = r != 0;
         }

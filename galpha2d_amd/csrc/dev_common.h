@@ -479,13 +479,21 @@ __host__ __device__ __forceinline__ bool rho_pair_ints(int sta, const double2 *p
     return true;
 }
 
-__host__ __device__ __forceinline__ int rho_equal_res(int sta, const double2 *pa, int stb, const double2 *pb)
+//
+// key_equal: the two values are known to share their 256-bit key (rho_key_fixed: the same E and
+// q = floor(rho 2^-E) in [2^255, 2^256)), so |rho_a - rho_b| < 2^E <= 2^-255 rho_a and the
+// cross difference is below num_a den_b 2^-255 < 2^(10D - 250): fewer primes, and none when
+// 10D <= 250 (an integer below 1 in magnitude is 0).
+__host__ __device__ __forceinline__ int rho_equal_res(int sta, const double2 *pa, int stb, const double2 *pb,
+                                                      int key_equal = 0)
 {
     long long X[12];
     int D;
     if (!rho_pair_ints(sta, pa, stb, pb, X, &D)) return -1;
     const int stv[2] = {sta, stb};
-    const int np = (10 * D + 5 + 29) / 30;
+    const int bound = key_equal ? 10 * D - 250 : 10 * D + 5;
+    if (bound <= 0) return 1;
+    const int np = (bound + 29) / 30;
     if (np > RES_NP) return -1;
     for (int q = 0; q < np; q++) {
         const ResMod p = res_mod(q);

@@ -46,7 +46,8 @@ int main(int argc, char **argv)
             printf("%d %d %llu %llu %llu %llu\n", ok ? 1 : 0, E, q[0], q[1], q[2], q[3]);
             continue;
         }
-        if (kind == 16) r = incircle_fixed(P[0], P[1], P[2], P[3]);
+        if (kind >= 21) r = rho_equal_res((kind - 21) & 1, P, ((kind - 21) >> 1) & 1, P + 3, 1);   // key-equal
+        else if (kind == 16) r = incircle_fixed(P[0], P[1], P[2], P[3]);
         else if (kind >= 11) r = rho_cmp_fixed((kind - 11) & 1, P, ((kind - 11) >> 1) & 1, P + 3);   // kind 11 + sta + 2 stb
         else if (kind == 0) r = incircle_zero_res(P[0], P[1], P[2], P[3]);
         else if (kind == 1) r = attached_i128(P[0], P[1], P[2]);
@@ -418,3 +419,37 @@ def test_fixed_width_incircle():
     assert not bad, bad[:5]
     assert {w for _, _, w in cases} == {-1, 0, 1, 2}
     assert sum(w == 2 for _, _, w in cases) < len(cases) // 4
+
+
+def test_key_equal_residue_bound():
+    """rho_equal_res with key_equal (the values share their 256-bit key, so fewer primes or none):
+    equal values stay certified equal at every difference length, from lattice-sized cells
+    (no prime needed) to cells whose differences take 40-58 bits."""
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not installed")
+    rng = random.Random(51)
+    cases = []
+    for it in range(600):
+        bits = rng.choice([10, 20, 30, 40, 48, 52])   # coordinates stay exact doubles
+        t = [(rng.randint(0, 2 ** bits), rng.randint(0, 2 ** bits)) for _ in range(3)]
+        if (t[1][0] - t[0][0]) * (t[2][1] - t[0][1]) == (t[1][1] - t[0][1]) * (t[2][0] - t[0][0]):
+            continue
+        ox, oy = rng.randint(0, 2 ** 4), rng.randint(0, 2 ** 4)
+        u = [(ox - y, oy + x) for x, y in t]   # turned by 90 degrees and moved: the same value
+        rng.shuffle(u)
+        if rng.random() < 0.3:   # an edge against the same edge turned
+            t, u = t[:2], [(ox - y, oy + x) for x, y in t[:2]]
+        sc = Fraction(1, 2 ** 60)
+        qa = [(Fraction(x) * sc, Fraction(y) * sc) for x, y in t]
+        qb = [(Fraction(x) * sc, Fraction(y) * sc) for x, y in u]
+        assert rho_value(qa) == rho_value(qb)
+        assert all(Fraction(float(c)) == c for pt in qa + qb for c in pt)
+        va = [float(c) for pt in qa for c in pt] + [0.0] * (6 - 2 * len(qa))
+        vb = [float(c) for pt in qb for c in pt] + [0.0] * (6 - 2 * len(qb))
+        want = -1 if _defers([[(float(x), float(y)) for x, y in qa], [(float(x), float(y)) for x, y in qb]]) else 1
+        cases.append((21 + (len(qa) == 3) + 2 * (len(qb) == 3), va + vb, want))
+    out = _run_harness([(k, v) for k, v, _ in cases])
+    got = [int(x) for x in out]
+    bad = [(i, got[i], cases[i][2]) for i in range(len(cases)) if got[i] != cases[i][2] and cases[i][2] == 1]
+    assert not bad, bad[:5]
+    assert sum(w == 1 for _, _, w in cases) > 0.9 * len(cases)
