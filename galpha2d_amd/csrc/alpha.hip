@@ -1572,11 +1572,16 @@ static void big_clusters(AlphaDev &d, AlphaK &a, int S0, int nbig, hipStream_t s
     chk(rocprim::radix_sort_pairs(nullptr, tb, b.kt, b.kt2, b.ej, b.vj, (unsigned)Mb, 0u, 64u, st), "sort");
     ab(d, AB_TMP, tb + 256);
     int *perm_in = b.ej, *perm_out = b.vj;
+    // the top word (cluster << 32 | exponent + 2^30) has no bits above 32 + bitlen(nbig - 1)
+    int cbits = 0;
+    while ((1 << cbits) < nbig) cbits++;
     for (int w = 0; w < 5; w++) {
         GA_LAUNCH(k_gather_keys, dim3(AGRID), dim3(256), 0, st, Mb, (const int *)perm_in,
                   (const unsigned long long *)b.key[w], b.kt);
         size_t t2 = d.b[AB_TMP].bytes;
-        chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, t2, b.kt, b.kt2, perm_in, perm_out, (unsigned)Mb, 0u, 64u, st),
+        const unsigned end_bit = w == 4 ? 32u + (unsigned)cbits : 64u;
+        chk(rocprim::radix_sort_pairs(d.b[AB_TMP].p, t2, b.kt, b.kt2, perm_in, perm_out, (unsigned)Mb, 0u, end_bit,
+                                      st),
             "sort");
         std::swap(perm_in, perm_out);
         if (w == 0) perm_out = b.vj2;
