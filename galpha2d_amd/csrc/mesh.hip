@@ -2527,29 +2527,35 @@ __global__ void k_canon_newid(int *order, const unsigned *skey, const unsigned *
     }
 }
 
-// out9 holds global vertex ids and global triangle positions.  The live slots are walked in slot
-// order (their reads stay spatially coherent) and each writes its output record at its position.
+// out9 holds global vertex ids and global triangle positions.  One thread per output record r,
+// from the slot order[r] that k_canon_newid left there (the inverse of newid): the records are
+// written whole by neighbouring lanes, and the slot reads follow the canonical order, which is
+// spatially coherent (vertices are numbered in spatial order).
 // Batches: rset[r] = set of output triangle r, for k_canon_set_first.
-__global__ void k_canon_emit(MeshDev m, const int *newid, int *out9, int *rset)
+__global__ void k_canon_emit(MeshDev m, const int *order, const int *newid, const int *dF, int *out9, int *rset)
 {
-    int n = ntri_used(m);
-    GRID_STRIDE(s, n) {
-        int4 v = m.tv[s];
-        if (v.x == GA_DEAD) continue;
-        int nr = newid[s], r = nr >> 2, ro = nr & 3;
-        int4 nb = m.tn[s];
-        int *o = out9 + (size_t)r * 9;
+    const int F = *dF;
+    GRID_STRIDE(r, F) {
+        const int s = order[r];
+        const int4 v = m.tv[s];
+        const int ro = newid[s] & 3;
+        const int4 nb = m.tn[s];
+        int rec[9];
+#pragma unroll
         for (int i = 0; i < 3; i++) {
-            int si = (i + ro) % 3;
-            o[3 + i] = oid(m, ic(v, si));
-            int e = ic(nb, si);
-            if (e < 0) o[i] = -1;
+            const int si = (i + ro) % 3;
+            rec[3 + i] = oid(m, ic(v, si));
+            const int e = ic(nb, si);
+            if (e < 0) rec[i] = -1;
             else {
-                int nu = newid[enc_t(e)];
-                o[i] = enc(nu >> 2, (enc_j(e) - (nu & 3) + 3) % 3);
+                const int nu = newid[enc_t(e)];
+                rec[i] = enc(nu >> 2, (enc_j(e) - (nu & 3) + 3) % 3);
             }
-            o[6 + i] = 0;
+            rec[6 + i] = 0;
         }
+        int *o = out9 + (size_t)r * 9;
+#pragma unroll
+        for (int i = 0; i < 9; i++) o[i] = rec[i];
         if (m.nsets > 1) rset[r] = m.vset[v.x];
     }
 }
@@ -2573,7 +2579,8 @@ void launch_canon_emit(MeshDev &m, int *order, const unsigned *skey, const unsig
 {
     launch_fill_i32(set_first, -1, (size_t)m.nsets, st);
     GA_LAUNCH(k_canon_newid, dim3(FIXED_GRID), dim3(256), 0, st, order, skey, v1s, rot, dF, newid);
-    GA_LAUNCH(k_canon_emit, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)newid, out9, rset);
+    GA_LAUNCH(k_canon_emit, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)order, (const int *)newid, dF, out9,
+              rset);
     GA_LAUNCH(k_canon_set_first, dim3(FIXED_GRID), dim3(256), 0, st, m, (const int *)rset, dF, set_first);
 }
 
