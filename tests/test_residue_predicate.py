@@ -7,7 +7,9 @@ compiled for the host and checked against exact rational arithmetic:
 - `attached_i128` (an exact sign in 128-bit integers): apexes on, inside and outside the circle
   over an edge;
 and coordinate spreads beyond their bit limits defer (-1) instead of answering.  A wrong prime, a
-dropped term or a bound that lets a nonzero value vanish fails here."""
+dropped term or a bound that lets a nonzero value vanish fails here.  The fixed-width forms
+(incircle_fixed, rho_cmp_fixed, rho_key_fixed and its long division mw_div_17_8) and the
+big-integer fallbacks (orient_exact, incircle_exact) are checked the same way below."""
 import math
 import os
 import random
@@ -453,3 +455,90 @@ def test_key_equal_residue_bound():
     bad = [(i, got[i], cases[i][2]) for i in range(len(cases)) if got[i] != cases[i][2] and cases[i][2] == 1]
     assert not bad, bad[:5]
     assert sum(w == 1 for _, _, w in cases) > 0.9 * len(cases)
+
+
+BIGINT_HARNESS = r"""
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include "dev_common.h"
+int main(int argc, char **argv)
+{
+    FILE *f = fopen(argv[1], "r");
+    int n;
+    if (!f || fscanf(f, "%d", &n) != 1) return 2;
+    for (int c = 0; c < n; c++) {
+        int kind;
+        double v[8];
+        if (fscanf(f, "%d", &kind) != 1) return 2;
+        for (int k = 0; k < 8; k++)
+            if (fscanf(f, "%lf", &v[k]) != 1) return 2;
+        double2 P[4];
+        for (int k = 0; k < 4; k++) P[k] = make_double2(v[2 * k], v[2 * k + 1]);
+        int ovf = 0;
+        const int r = kind == 0 ? orient_exact(P[0], P[1], P[2], &ovf) : incircle_exact(P[0], P[1], P[2], P[3], &ovf);
+        printf("%d %d\n", r, ovf);
+    }
+    return 0;
+}
+"""
+
+
+def test_bigint_predicates_host():
+    """The big-integer fallbacks orient_exact and incircle_exact (the device's last resort behind
+    every filter, residue and fixed-width form), built for the host from the same header, against
+    rationals: collinear and co-circular pixel-grid cases at the snapping scales, random doubles,
+    and integers over wide binary scales (beyond the fixed-width limits)."""
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not installed")
+    rng = random.Random(61)
+    cases = []
+    for it in range(1200):
+        T = rng.choice([256, 1024, 4096])
+        sc = 2.0 ** -math.ceil(math.log2(T)) * (1 + rng.randint(0, 7) / 8)   # few significant bits
+        mode = it % 3
+        if mode == 0:     # pixel-grid triples and quadruples, many collinear / co-circular
+            a = (rng.randint(0, T), rng.randint(0, T))
+            d = (rng.randint(-4, 4), rng.randint(-4, 4))
+            pts = [a, (a[0] + d[0], a[1] + d[1]), (a[0] + 2 * d[0], a[1] + 2 * d[1] + rng.choice([0, 0, 1])),
+                   (a[0] + d[1], a[1] - d[0])]
+            pts = [(x * sc, y * sc) for x, y in pts]
+        elif mode == 1:   # random doubles
+            pts = [(rng.random(), rng.random()) for _ in range(4)]
+        else:             # integers at a wide binary scale: long coordinates
+            s2 = 2.0 ** rng.randint(-200, 100)
+            pts = [(rng.randint(-2 ** 52, 2 ** 52) * s2, rng.randint(-2 ** 52, 2 ** 52) * s2) for _ in range(4)]
+        q = [(Fraction(x), Fraction(y)) for x, y in pts]
+        (ax, ay), (bx, by), (cx, cy) = q[:3]
+        o = (ax - cx) * (by - cy) - (ay - cy) * (bx - cx)
+        v = [c for p in pts for c in p]
+        cases.append((0, v, _sign(o)))
+        cases.append((1, v, _sign(incircle_det(q))))
+    tmp = tempfile.mkdtemp()
+    try:
+        # the header's device-only inline functions, declared for the host as well
+        with open(os.path.join(CSRC, "dev_common.h")) as f:
+            hdr = f.read()
+        assert "#define GA_INLINE __device__ __forceinline__" in hdr
+        with open(os.path.join(tmp, "dev_common.h"), "w") as f:
+            f.write(hdr.replace("#define GA_INLINE __device__ __forceinline__",
+                                "#define GA_INLINE __host__ __device__ __forceinline__"))
+        src, exe, data = (os.path.join(tmp, x) for x in ("b.cpp", "b", "cases.txt"))
+        with open(src, "w") as f:
+            f.write(BIGINT_HARNESS)
+        subprocess.run([HIPCC, "-O2", "-std=c++17", "-I", tmp, src, "-o", exe], check=True, capture_output=True)
+        with open(data, "w") as f:
+            f.write(f"{len(cases)}\n")
+            for kind, v, _ in cases:
+                f.write(f"{kind} " + " ".join(repr(x) for x in v) + "\n")
+        out = subprocess.run([exe, data], check=True, capture_output=True, text=True).stdout.splitlines()
+    finally:
+        shutil.rmtree(tmp)
+    bad = []
+    for (kind, v, want), line in zip(cases, out):
+        r, ovf = (int(x) for x in line.split())
+        assert ovf == 0, v
+        if r != want:
+            bad.append((kind, v, r, want))
+    assert not bad, bad[:5]
+    for kind in (0, 1):
+        assert {w for k, _, w in cases if k == kind} == {-1, 0, 1}
