@@ -775,11 +775,11 @@ static bool bulk_frame_removal(galpha_ctx *c, MeshDev &m, int nsets, hipStream_t
         return true;
     }
     // few sets: grid-wide rounds, eight per host read (a round after one without candidates
-    // returns at once; 1M uniform points take about 20 rounds)
+    // returns at once).  Sets of 256K points and more take 20-24 rounds (1M uniform points,
+    // the 2M lattice + cluster set), so their first read comes after sixteen.
     launch_ear_grid(m, B, bnd, hcnt, hoff, nsets, st);   // also zeroes counters[C_EAR]
-    const int per_read = 8;
-    int round = 0;
-    for (;; round += per_read) {
+    int round = 0, per_read = m.n >= (1 << 18) ? 16 : 8;
+    for (;; round += per_read, per_read = 8) {
         if (round > 200000) throw GalphaError("hull convexification did not finish");
         launch_ear_rounds(m, B, bnd, hcnt, hoff, nsets, round, per_read, c->modF.as<int>(), C_FRAME_LIST, c->op++,
                           st);
