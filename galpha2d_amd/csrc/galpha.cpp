@@ -985,9 +985,11 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
         moved_before = moved_total;
     }
     int deleted_before = 0;
-    for (int r = 0; pend > 0; r += 3) {   // three rounds per read (C2 takes six)
+    // three rounds per read; sets of 256K points and more start with six (1M uniform points take
+    // six), saving a read
+    for (int r = 0, per = m.n >= (1 << 18) ? 6 : 3; pend > 0; r += per, per = 3) {
         if (r > 10000) throw GalphaError("vertex deletion did not finish");
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < per; k++) {
             // (k_del_prio zeroes C_SCR and C_MODN)
             launch_delete_round(m, c->mx.as<unsigned int>(), next_prio_round(), c->op++, pgrid, st);
         }
