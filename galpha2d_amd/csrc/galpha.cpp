@@ -1115,8 +1115,10 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
         if (last.status())
             throw GalphaError("missing-site insertion failed (status " + std::to_string(last.status()) + ")");
         if (r > 5000) throw GalphaError("missing-site insertion did not finish");
-        // two rounds per read (a round after the last insertion finds every entry done)
-        for (int k = 0; k < 2; k++) {
+        // two rounds per read (a round after the last insertion finds every entry done); sets of
+        // 256K points and more start with four (1M uniform points take four), saving a read
+        const int per = r == 0 && m.n >= (1 << 18) ? 4 : 2;
+        for (int k = 0; k < per; k++) {
             int opn = c->op++;
             launch_insert_round(m, c->mlist.as<int>(), c->loc.as<int>(), opn, round_grid(nmiss), st);   // (zeroes C_MODN)
             launch_relink(m, m.modA, C_MODN, opn, round_grid(4LL * left), st);
