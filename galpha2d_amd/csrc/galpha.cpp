@@ -978,24 +978,31 @@ static int run_batch(galpha_ctx *c, const double *d_xy, const BatchPlan &plan)
     // on 1M uniform points the shifting stage took 1.68 ms with up to 48 rounds, 1.40 ms with 8.
     const int pgrid = round_grid(pend);   // the pending list keeps its length (done entries are skipped)
     // (all eight before one read: a round after convergence moves nothing)
+    // Sets of 256K points and more always leave vertices to delete after the greedy rounds, so
+    // their moved count comes with the deletion loop's first snapshot instead of a read of its own.
+    const bool large = m.n >= (1 << 18);
     if (pend > 0) {
         for (int k = 0; k < 8; k++) launch_shift_greedy(m, k, pgrid, st);
-        int moved_total = ctx_stat(c, C_MOVED);
-        pend -= moved_total - moved_before;
-        moved_before = moved_total;
+        if (!large) {
+            int moved_total = ctx_stat(c, C_MOVED);
+            pend -= moved_total - moved_before;
+            moved_before = moved_total;
+        }
     }
     int deleted_before = 0;
     // three rounds per read; sets of 256K points and more start with six (1M uniform points take
     // six), saving a read
-    for (int r = 0, per = m.n >= (1 << 18) ? 6 : 3; pend > 0; r += per, per = 3) {
+    for (int r = 0, per = large ? 6 : 3; pend > 0; r += per, per = 3) {
         if (r > 10000) throw GalphaError("vertex deletion did not finish");
         for (int k = 0; k < per; k++) {
-            // (k_del_prio zeroes C_SCR and C_MODN)
+            // (k_del_prio zeroes C_SCR and C_MODN; rounds with nothing pending return at once)
             launch_delete_round(m, c->mx.as<unsigned int>(), next_prio_round(), c->op++, pgrid, st);
         }
         Snap sn = ctx_snapshot(c);
         int after = sn.stat(C_DELETED);
-        pend -= after - deleted_before;
+        const int moved_total = sn.stat(C_MOVED);
+        pend -= (after - deleted_before) + (moved_total - moved_before);
+        moved_before = moved_total;
         deleted_before = after;
         int stt = sn.status();
         if (stt) {
